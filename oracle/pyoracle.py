@@ -1,0 +1,301 @@
+"""ctypes binding of the CPU oracle (oracle/qpad_oracle.c).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and
+bench.py's cpu_baseline leg as the checker / CPU baseline.  The product
+(qpad_amd) never imports it.
+
+Arrays returned by the accessors are numpy views on the oracle's own memory
+(layout documented in qpad_oracle.h): slice fields are [ncomp, nm, ld] with
+ld = nr + 2 (nodes 0..nr+1), histories are [nxi + 2, ncomp, nm, ld].
+"""
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "build", "libqpad_oracle.so")
+
+PSI, E, E_SPE, B, B_SPE, B_BEAM, CU, ACU, AMU, DCU, Q_SPE, Q_BEAM = range(12)
+NCOMP = [1, 3, 3, 3, 3, 3, 3, 2, 3, 2, 1, 1]
+FIELD_NAMES = ["psi", "e", "e_spe", "b", "b_spe", "b_beam", "cu", "acu", "amu",
+               "dcu", "q_spe", "q_beam"]
+FK_PSI, FK_EZ, FK_BZ, FK_BT, FK_BPLUS, FK_BMINUS = range(6)
+BND_ZERO, BND_OPEN = 2, 3
+
+_dp = C.POINTER(C.c_double)
+_ip = C.POINTER(C.c_int32)
+
+
+class Grid(C.Structure):
+    _fields_ = [("nr", C.c_int), ("nxi", C.c_int), ("max_mode", C.c_int),
+                ("dr", C.c_double), ("dxi", C.c_double), ("relax_fac", C.c_double),
+                ("bnd", C.c_int), ("iter_max", C.c_int), ("iter_reltol", C.c_double),
+                ("iter_abstol", C.c_double), ("dt", C.c_double)]
+
+
+class SpeciesCfg(C.Structure):
+    _fields_ = [("ppc1", C.c_int), ("ppc2", C.c_int), ("num_theta", C.c_int),
+                ("qm", C.c_double), ("mass", C.c_double), ("density", C.c_double),
+                ("uth", C.c_double * 3), ("neutralized", C.c_int), ("den_min", C.c_double),
+                ("seed", C.c_uint64)]
+
+
+class BeamCfg(C.Structure):
+    _fields_ = [("total_num", C.c_int64), ("total_charge", C.c_double), ("qm", C.c_double),
+                ("mass", C.c_double), ("gamma", C.c_double), ("center", C.c_double * 3),
+                ("sigma", C.c_double * 3), ("range_lo", C.c_double * 3),
+                ("range_hi", C.c_double * 3), ("uth", C.c_double * 3), ("quiet", C.c_int),
+                ("evol", C.c_int), ("seed", C.c_uint64)]
+
+
+class _Species(C.Structure):
+    _fields_ = [("npp", C.c_int64), ("npmax", C.c_int64),
+                ("x1", _dp), ("x2", _dp), ("p1", _dp), ("p2", _dp), ("p3", _dp),
+                ("q", _dp), ("gamma", _dp), ("psi", _dp), ("id", _ip),
+                ("qbm", C.c_double), ("qm", C.c_double), ("dt", C.c_double),
+                ("cfg", SpeciesCfg),
+                ("q_own", _dp), ("qn", _dp), ("cu_own", _dp), ("acu_own", _dp),
+                ("amu_own", _dp), ("q_hist", _dp)]
+
+
+class _Beam(C.Structure):
+    _fields_ = [("npp", C.c_int64), ("npmax", C.c_int64),
+                ("x1", _dp), ("x2", _dp), ("x3", _dp), ("p1", _dp), ("p2", _dp),
+                ("p3", _dp), ("q", _dp), ("id", _ip),
+                ("qbm", C.c_double), ("dt", C.c_double), ("evol", C.c_int),
+                ("q_own_hist", _dp)]
+
+
+def build():
+    """Compile the oracle with its Makefile (gcc)."""
+    subprocess.run(["make", "-s", "-C", _HERE], check=True)
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            build()
+        L = C.CDLL(LIB_PATH)
+        L.ora_create.restype = C.c_void_p
+        L.ora_create.argtypes = [C.POINTER(Grid)]
+        L.ora_destroy.argtypes = [C.c_void_p]
+        L.ora_add_species.argtypes = [C.c_void_p, C.POINTER(SpeciesCfg)]
+        L.ora_add_beam.argtypes = [C.c_void_p, C.POINTER(BeamCfg), C.c_int64]
+        L.ora_beam_set.argtypes = [C.c_void_p, C.c_int, C.c_int64] + [_dp] * 7
+        L.ora_step3d_begin.argtypes = [C.c_void_p]
+        L.ora_slice.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_int)]
+        L.ora_step3d_end.argtypes = [C.c_void_p, C.c_int]
+        L.ora_run.argtypes = [C.c_void_p, C.c_int, C.c_int]
+        L.ora_field.restype = _dp
+        L.ora_field.argtypes = [C.c_void_p, C.c_int]
+        L.ora_hist.restype = _dp
+        L.ora_hist.argtypes = [C.c_void_p, C.c_int]
+        L.ora_get_species.restype = C.POINTER(_Species)
+        L.ora_get_species.argtypes = [C.c_void_p, C.c_int]
+        L.ora_get_beam.restype = C.POINTER(_Beam)
+        L.ora_get_beam.argtypes = [C.c_void_p, C.c_int]
+        L.ora_matrix.argtypes = [C.c_int, C.c_int, C.c_int, C.c_double, C.c_double, C.c_int,
+                                 _dp, _dp, _dp]
+        L.ora_thomas_factor.argtypes = [C.c_int, _dp, _dp, _dp, _dp, _dp, _dp]
+        L.ora_thomas_solve.argtypes = [C.c_int, _dp, _dp, _dp, _dp]
+        for name in ("ora_solve_psi", "ora_solve_ez", "ora_solve_bz", "ora_solve_bt_beam"):
+            getattr(L, name).argtypes = [C.c_void_p, _dp, _dp]
+        for name in ("ora_solve_et", "ora_solve_bt_iter", "ora_solve_djdxi"):
+            getattr(L, name).argtypes = [C.c_void_p, _dp, _dp, _dp]
+        L.ora_conv_record.argtypes = [C.c_void_p, _dp]
+        L.ora_conv_compare.argtypes = [C.c_void_p, _dp, _dp, _dp]
+        L.ora_qdeposit.argtypes = [C.c_void_p, C.POINTER(_Species), _dp]
+        L.ora_amjdeposit_robust.argtypes = [C.c_void_p, C.POINTER(_Species), _dp, _dp, _dp, _dp,
+                                            _dp, C.c_double]
+        L.ora_push_u_robust.argtypes = [C.c_void_p, C.POINTER(_Species), _dp, _dp, C.c_double]
+        L.ora_push_x.argtypes = [C.POINTER(_Species), C.c_double]
+        L.ora_update_bound.argtypes = [C.c_void_p, C.POINTER(_Species)]
+        L.ora_interp_point.argtypes = [C.c_void_p, _dp, C.c_int, C.c_double, C.c_double, _dp]
+        L.ora_beam_qdeposit.argtypes = [C.c_void_p, C.c_int, _dp]
+        L.ora_beam_push.argtypes = [C.c_void_p, C.c_int]
+        L.ora_species_inject.argtypes = [C.c_void_p, C.c_int]
+        L.ora_species_init_q.argtypes = [C.c_void_p, C.c_int]
+        L.ora_last_iters.argtypes = [C.c_void_p]
+        L.ora_philox4x32.argtypes = [C.POINTER(C.c_uint32), C.POINTER(C.c_uint32),
+                                     C.POINTER(C.c_uint32)]
+        L.ora_normal4.argtypes = [C.c_uint64, C.c_uint64, C.c_uint32, _dp]
+        _lib = L
+    return _lib
+
+
+def ptr(a):
+    assert a.dtype == np.float64 and a.flags["C_CONTIGUOUS"]
+    return a.ctypes.data_as(_dp)
+
+
+def make_grid(nr, nxi, max_mode, dr, dxi, relax_fac=None, bnd=BND_OPEN, iter_max=10,
+              reltol=1e-3, abstol=1e-3, dt=10.0):
+    if relax_fac is None:
+        relax_fac = 1.0e-3 * (dr / 0.02) ** 2   # sim_fields_class.f03:137
+    return Grid(nr, nxi, max_mode, dr, dxi, relax_fac, bnd, iter_max, reltol, abstol, dt)
+
+
+def species_cfg(ppc1, ppc2, num_theta, qm=-1.0, mass=1.0, density=1.0, uth=(0.0, 0.0, 0.0),
+                neutralized=True, den_min=1e-10, seed=0):
+    return SpeciesCfg(ppc1, ppc2, num_theta, qm, mass, density, (C.c_double * 3)(*uth),
+                      int(neutralized), den_min, seed)
+
+
+def beam_cfg(total_num, total_charge, center, sigma, range_lo, range_hi, qm=-1.0, mass=1.0,
+             gamma=20000.0, uth=(0.0, 0.0, 0.0), quiet=False, evol=True, seed=1):
+    d3 = C.c_double * 3
+    return BeamCfg(total_num, total_charge, qm, mass, gamma, d3(*center), d3(*sigma),
+                   d3(*range_lo), d3(*range_hi), d3(*uth), int(quiet), int(evol), seed)
+
+
+def matrix(kind, mode, nr, dr, relax_fac, bnd=BND_OPEN):
+    a, b, c = (np.zeros(nr) for _ in range(3))
+    lib().ora_matrix(kind, mode, nr, dr, relax_fac, bnd, ptr(a), ptr(b), ptr(c))
+    return a, b, c
+
+
+def thomas(a, b, c, d):
+    n = len(b)
+    lo, inv, cp = (np.zeros(n) for _ in range(3))
+    lib().ora_thomas_factor(n, ptr(np.ascontiguousarray(a)), ptr(np.ascontiguousarray(b)),
+                            ptr(np.ascontiguousarray(c)), ptr(lo), ptr(inv), ptr(cp))
+    x = np.array(d, dtype=np.float64)
+    lib().ora_thomas_solve(n, ptr(lo), ptr(inv), ptr(cp), ptr(x))
+    return x
+
+
+def philox4x32(ctr, key):
+    c = (C.c_uint32 * 4)(*ctr)
+    k = (C.c_uint32 * 2)(*key)
+    o = (C.c_uint32 * 4)()
+    lib().ora_philox4x32(c, k, o)
+    return list(o)
+
+
+def normal4(seed, stream, sub):
+    out = np.zeros(4)
+    lib().ora_normal4(seed, stream, sub, ptr(out))
+    return out
+
+
+class Oracle:
+    """One oracle simulation instance (simulation_class restated)."""
+
+    def __init__(self, grid):
+        self.grid = grid
+        self.h = lib().ora_create(C.byref(grid))
+        self.nm = 2 * grid.max_mode + 1
+        self.ld = grid.nr + 2
+        self.nspecies = 0
+        self.nbeams = 0
+
+    def close(self):
+        if self.h:
+            lib().ora_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # -- construction
+    def add_species(self, cfg):
+        r = lib().ora_add_species(self.h, C.byref(cfg))
+        self.nspecies += 1
+        return r
+
+    def add_beam(self, cfg, npmax=0):
+        r = lib().ora_add_beam(self.h, C.byref(cfg), npmax)
+        self.nbeams += 1
+        return r
+
+    def set_beam(self, ib, x1, x2, x3, p1, p2, p3, q):
+        arrs = [np.ascontiguousarray(a, dtype=np.float64) for a in (x1, x2, x3, p1, p2, p3, q)]
+        rc = lib().ora_beam_set(self.h, ib, len(arrs[0]), *[ptr(a) for a in arrs])
+        assert rc == 0
+
+    # -- stepping
+    def step_begin(self):
+        lib().ora_step3d_begin(self.h)
+
+    def slice(self, j):
+        n = C.c_int(0)
+        lib().ora_slice(self.h, j, C.byref(n))
+        return n.value
+
+    def step_end(self, i=1):
+        lib().ora_step3d_end(self.h, i)
+
+    def run(self, nstep3d, nslices=0):
+        lib().ora_run(self.h, nstep3d, nslices)
+
+    # -- state access (views)
+    def field(self, fid):
+        p = lib().ora_field(self.h, fid)
+        n = NCOMP[fid] * self.nm * self.ld
+        return np.ctypeslib.as_array(p, shape=(n,)).reshape(NCOMP[fid], self.nm, self.ld)
+
+    def hist(self, fid):
+        p = lib().ora_hist(self.h, fid)
+        ns = self.grid.nxi + 2
+        n = ns * NCOMP[fid] * self.nm * self.ld
+        return np.ctypeslib.as_array(p, shape=(n,)).reshape(ns, NCOMP[fid], self.nm, self.ld)
+
+    def species_raw(self, isp):
+        return lib().ora_get_species(self.h, isp)
+
+    def species(self, isp):
+        s = self.species_raw(isp).contents
+        n = int(s.npp)
+        out = {}
+        for name in ("x1", "x2", "p1", "p2", "p3", "q", "gamma", "psi"):
+            out[name] = np.ctypeslib.as_array(getattr(s, name), shape=(int(s.npmax),))[:n]
+        out["id"] = np.ctypeslib.as_array(s.id, shape=(int(s.npmax),))[:n]
+        return out
+
+    def set_species(self, isp, arrs):
+        """Overwrite species isp's particles (used to feed identical inputs)."""
+        s = self.species_raw(isp).contents
+        n = len(arrs["x1"])
+        assert n <= s.npmax
+        for name in ("x1", "x2", "p1", "p2", "p3", "q", "gamma", "psi"):
+            dst = np.ctypeslib.as_array(getattr(s, name), shape=(int(s.npmax),))
+            src = arrs.get(name)
+            dst[:n] = src if src is not None else 0.0
+        ids = np.ctypeslib.as_array(s.id, shape=(int(s.npmax),))
+        ids[:n] = arrs.get("id", np.arange(n))
+        self.species_raw(isp).contents.npp = n
+
+    def species_q_own(self, isp):
+        s = self.species_raw(isp).contents
+        n = self.nm * self.ld
+        return (np.ctypeslib.as_array(s.q_own, shape=(n,)).reshape(1, self.nm, self.ld),
+                np.ctypeslib.as_array(s.qn, shape=(n,)).reshape(1, self.nm, self.ld))
+
+    def beam(self, ib):
+        b = lib().ora_get_beam(self.h, ib).contents
+        n = int(b.npp)
+        out = {}
+        for name in ("x1", "x2", "x3", "p1", "p2", "p3", "q"):
+            out[name] = np.ctypeslib.as_array(getattr(b, name), shape=(int(b.npmax),))[:n]
+        out["id"] = np.ctypeslib.as_array(b.id, shape=(int(b.npmax),))[:n]
+        return out
+
+    def last_iters(self):
+        return lib().ora_last_iters(self.h)
+
+    # -- building blocks on this instance's grid
+    def new_field(self, ncomp):
+        return np.zeros((ncomp, self.nm, self.ld))
+
+    def solve(self, name, *arrays):
+        fn = getattr(lib(), "ora_solve_" + name)
+        fn(self.h, *[ptr(a) for a in arrays])

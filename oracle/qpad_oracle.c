@@ -1,0 +1,1435 @@
+/*
+ * qpad_oracle.c -- TEST INFRASTRUCTURE: serial FP64 restatement of QPAD's
+ * slice step (source/simulation_class.f03:342-469), 3D step
+ * (:294-512) and the kernels they call.  See qpad_oracle.h for the layout,
+ * the role of this file and what is pinned by what.
+ *
+ * Arithmetic follows the reference's Fortran evaluation order (left to
+ * right, as gfortran evaluates it without fast-math) so that the restatement
+ * is faithful operation by operation.  Compile with -ffp-contract=off.
+ */
+#include "qpad_oracle.h"
+
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+
+#define PI_DB (4.0 * atan(1.0)) /* source/param.f03:35 */
+
+static inline int kre(int m) { return m == 0 ? 0 : 2 * m - 1; }
+static inline int kim(int m) { return 2 * m; }
+
+/* element (component c, mode array k, node i) of a slice array */
+#define FA(arr, c, k, i) ((arr)[((size_t)(c) * nm + (size_t)(k)) * ld + (size_t)(i)])
+
+/* ------------------------------------------------------------------------ */
+/* Tridiagonal matrices: source/fields/field_solver_class.f03:256-561        */
+/* Row r = 0..nr-1 is QPAD grid point r+1 (noff = 0, one radial partition). */
+/* ------------------------------------------------------------------------ */
+void ora_matrix(int kind, int m, int nr, double dr, double relax, int bnd,
+                double *a, double *b, double *c)
+{
+    double dr2 = dr * dr;
+    double m2 = (double)(m * m);
+    /* interior rows: field_solver_class.f03:295-300 (psi/bt/ez/bz),
+       :336-343 (B+), :371-378 (B-) */
+    for (int r = 1; r < nr; r++) {
+        double j = (double)r;
+        a[r] = 1.0 - 0.5 / j;
+        c[r] = 1.0 + 0.5 / j;
+        if (kind == ORA_FK_BPLUS) {
+            double t = (double)(m + 1) / j;
+            b[r] = -2.0 - t * t - relax;
+        } else if (kind == ORA_FK_BMINUS) {
+            double t = (double)(m - 1) / j;
+            b[r] = -2.0 - t * t - relax;
+        } else {
+            b[r] = -2.0 - m2 / (j * j);
+        }
+    }
+    /* first grid point: :302-330 (psi group), :345-357 (B+), :380-400 (B-) */
+    int identity = 0;
+    if (kind == ORA_FK_BPLUS) {
+        identity = 1;
+    } else if (kind == ORA_FK_BMINUS) {
+        if (m == 1) { a[0] = 0.0; b[0] = -4.0 - relax; c[0] = 4.0; }
+        else identity = 1;
+    } else {
+        if (m == 0) { a[0] = 0.0; b[0] = -4.0; c[0] = 4.0; }
+        else identity = 1;
+    }
+    if (identity) {
+        a[0] = 0.0; b[0] = 1.0; c[0] = 0.0;
+        if (nr > 1) a[1] = 0.0;   /* HYPRE_BUF(4) = 0: decouples node 2 */
+    }
+    /* upper boundary: :491-548 */
+    int rl = nr - 1;
+    if (bnd == ORA_BND_ZERO) {
+        c[rl] = 0.0;
+    } else {
+        double jmax = (double)nr;
+        double coef = 0.0;
+        int fold = 1;
+        switch (kind) {
+        case ORA_FK_PSI:
+        case ORA_FK_EZ:
+        case ORA_FK_BZ:
+            if (m == 0) fold = 0;
+            else coef = 1.0 - (double)m / jmax;
+            break;
+        case ORA_FK_BT:
+            if (m == 0) coef = 1.0 + 1.0 / (jmax * log(jmax * dr));
+            else coef = 1.0 - (double)m / jmax;
+            break;
+        default: /* B+ and B- both use m+1 (field_solver_class.f03:532-540) */
+            coef = 1.0 - (double)(m + 1) / jmax;
+            break;
+        }
+        if (fold) b[rl] = b[rl] + coef * c[rl];
+        c[rl] = 0.0;
+    }
+    /* HYPRE_BUF = HYPRE_BUF / dr2 (:551) */
+    for (int r = 0; r < nr; r++) {
+        a[r] = a[r] / dr2;
+        b[r] = b[r] / dr2;
+        c[r] = c[r] / dr2;
+    }
+}
+
+/* Thomas factorisation (textbook; source/math_module.f03:270-292 restates
+   the same elimination).  lo = sub-diagonal, inv = 1/pivot, cp = c/pivot. */
+void ora_thomas_factor(int n, const double *a, const double *b,
+                       const double *c, double *lo, double *inv, double *cp)
+{
+    inv[0] = 1.0 / b[0];
+    cp[0] = c[0] * inv[0];
+    lo[0] = 0.0;
+    for (int r = 1; r < n; r++) {
+        double den = b[r] - a[r] * cp[r - 1];
+        inv[r] = 1.0 / den;
+        cp[r] = c[r] * inv[r];
+        lo[r] = a[r];
+    }
+}
+
+void ora_thomas_solve(int n, const double *lo, const double *inv,
+                      const double *cp, double *d)
+{
+    d[0] = d[0] * inv[0];
+    for (int r = 1; r < n; r++) d[r] = (d[r] - lo[r] * d[r - 1]) * inv[r];
+    for (int r = n - 2; r >= 0; r--) d[r] = d[r] - cp[r] * d[r + 1];
+}
+
+static void solve_sys(const ora_sim *s, int kind, int m, double *buf)
+{
+    const ora_factor *f = &s->fac[kind][m];
+    ora_thomas_solve(s->g.nr, f->lo, f->inv, f->cp, buf);
+}
+
+/* ------------------------------------------------------------------------ */
+/* psi: source/fields/field_psi_class.f03:131-254                            */
+/* ------------------------------------------------------------------------ */
+void ora_solve_psi(const ora_sim *s, const double *q, double *psi)
+{
+    int nr = s->g.nr, nm = s->nm, ld = s->ld;
+    double *bre = malloc(sizeof(double) * nr), *bim = malloc(sizeof(double) * nr);
+    for (int m = 0; m <= s->g.max_mode; m++) {
+        for (int i = 1; i <= nr; i++) bre[i - 1] = -1.0 * FA(q, 0, kre(m), i);
+        solve_sys(s, ORA_FK_PSI, m, bre);
+        for (int i = 1; i <= nr; i++) FA(psi, 0, kre(m), i) = bre[i - 1];
+        if (m == 0) continue;
+        for (int i = 1; i <= nr; i++) bim[i - 1] = -1.0 * FA(q, 0, kim(m), i);
+        solve_sys(s, ORA_FK_PSI, m, bim);
+        for (int i = 1; i <= nr; i++) FA(psi, 0, kim(m), i) = bim[i - 1];
+        FA(psi, 0, kre(m), 1) = 0.0; /* psi(m>0) vanishes on axis (:207-210) */
+        FA(psi, 0, kim(m), 1) = 0.0;
+    }
+    free(bre); free(bim);
+}
+
+/* ------------------------------------------------------------------------ */
+/* Ez: source/fields/field_e_class.f03:148-334                               */
+/* ------------------------------------------------------------------------ */
+void ora_solve_ez(const ora_sim *s, const double *jay, double *e)
+{
+    int nr = s->g.nr, nm = s->nm, ld = s->ld;
+    const int ldj = ld; (void)ldj;
+    double idr = 1.0 / s->g.dr, idrh = 0.5 * idr;
+    double *bre = calloc(nr + 1, sizeof(double)), *bim = calloc(nr + 1, sizeof(double));
+    /* 1-based buffers: buf[i], i = 1..nr (buf[0] unused) */
+#define JRE(c, i) FA(jay, (c) - 1, kre(m), (i))
+#define JIM(c, i) FA(jay, (c) - 1, kim(m), (i))
+    for (int m = 0; m <= s->g.max_mode; m++) {
+        memset(bre, 0, sizeof(double) * (nr + 1));
+        memset(bim, 0, sizeof(double) * (nr + 1));
+        if (m == 0) {
+            int i1 = 2, i2 = nr - 2;
+            double div = 0.0, ir;
+            for (int i = i1; i <= i2; i++) {
+                ir = idr / (double)(i - 1);
+                bre[i] = idrh * (JRE(1, i + 1) - JRE(1, i - 1)) + ir * JRE(1, i);
+                div = div + bre[i] * (double)(i - 1);
+            }
+            ir = idr / (double)(nr - 2);
+            bre[nr - 1] = idrh * (JRE(1, nr) - JRE(1, nr - 2)) + ir * JRE(1, nr - 1);
+            ir = idr / (double)(nr - 1);
+            bre[nr] = idr * (JRE(1, nr) - JRE(1, nr - 1)) + ir * JRE(1, nr);
+            div = div - idrh * (JRE(1, nr - 2) + JRE(1, nr - 1)) * ((double)nr - 2.5);
+            bre[1] = div;               /* MPI_REDUCE over one rank */
+            bre[1] = -8.0 * bre[1];
+        } else {
+            double dm = (double)m, ir;
+            for (int i = 2; i <= nr; i++) {
+                ir = idr / (double)(i - 1);
+                bre[i] = idrh * (JRE(1, i + 1) - JRE(1, i - 1)) + ir * JRE(1, i) - dm * ir * JIM(2, i);
+                bim[i] = idrh * (JIM(1, i + 1) - JIM(1, i - 1)) + ir * JIM(1, i) + dm * ir * JRE(2, i);
+            }
+            if (m % 2 == 0) {
+                bre[1] = 2.0 * idr * JRE(1, 2) - dm * idr * JIM(2, 2);
+                bim[1] = 2.0 * idr * JIM(1, 2) + dm * idr * JRE(2, 2);
+            } else {
+                bre[1] = 0.0;
+                bim[1] = 0.0;
+                if (m == 1) {
+                    ir = idr;
+                    bre[2] = idr * (JRE(1, 3) - JRE(1, 2)) + ir * JRE(1, 2) - dm * ir * JIM(2, 2);
+                    bim[2] = idr * (JIM(1, 3) - JIM(1, 2)) + ir * JIM(1, 2) + dm * ir * JRE(2, 2);
+                }
+            }
+            ir = idr / (double)(nr - 1);
+            bre[nr] = idrh * (3.0 * JRE(1, nr) - 4.0 * JRE(1, nr - 1) + JRE(1, nr - 2)) + ir * JRE(1, nr)
+                      - dm * ir * JIM(2, nr);
+            bim[nr] = idrh * (3.0 * JIM(1, nr) - 4.0 * JIM(1, nr - 1) + JIM(1, nr - 2)) + ir * JIM(1, nr)
+                      + dm * ir * JRE(2, nr);
+        }
+        solve_sys(s, ORA_FK_EZ, m, bre + 1);
+        for (int i = 1; i <= nr; i++) FA(e, 2, kre(m), i) = bre[i];
+        if (m > 0) {
+            solve_sys(s, ORA_FK_EZ, m, bim + 1);
+            for (int i = 1; i <= nr; i++) FA(e, 2, kim(m), i) = bim[i];
+            FA(e, 2, kre(m), 1) = 0.0;  /* Ez(m>0) vanishes on axis (:288-290) */
+            FA(e, 2, kim(m), 1) = 0.0;
+        }
+    }
+#undef JRE
+#undef JIM
+    free(bre); free(bim);
+}
+
+/* ------------------------------------------------------------------------ */
+/* E_perp from B and psi: source/fields/field_e_class.f03:412-514            */
+/* ------------------------------------------------------------------------ */
+void ora_solve_et(const ora_sim *s, const double *b, const double *psi, double *e)
+{
+    int nr = s->g.nr, nm = s->nm, ld = s->ld;
+    double idr = 1.0 / s->g.dr, idrh = idr * 0.5;
+    /* m = 0 */
+    {
+        int k = 0;
+        for (int i = 1; i <= nr; i++) {
+            FA(e, 0, k, i) = FA(b, 1, k, i) - idrh * (FA(psi, 0, k, i + 1) - FA(psi, 0, k, i - 1));
+            FA(e, 1, k, i) = -FA(b, 0, k, i);
+        }
+        FA(e, 0, k, 1) = 0.0;
+        FA(e, 1, k, 1) = 0.0;
+        FA(e, 0, k, nr) = FA(b, 1, k, nr) + idrh * (4.0 * FA(psi, 0, k, nr - 1) - FA(psi, 0, k, nr - 2)
+                                                    - 3.0 * FA(psi, 0, k, nr));
+    }
+    for (int m = 1; m <= s->g.max_mode; m++) {
+        int r = kre(m), q = kim(m);
+        double dm = (double)m, ir;
+        for (int i = 2; i <= nr; i++) {
+            ir = idr / (double)(i - 1);
+            FA(e, 0, r, i) = FA(b, 1, r, i) - idrh * (FA(psi, 0, r, i + 1) - FA(psi, 0, r, i - 1));
+            FA(e, 0, q, i) = FA(b, 1, q, i) - idrh * (FA(psi, 0, q, i + 1) - FA(psi, 0, q, i - 1));
+            FA(e, 1, r, i) = -FA(b, 0, r, i) + ir * dm * FA(psi, 0, q, i);
+            FA(e, 1, q, i) = -FA(b, 0, q, i) - ir * dm * FA(psi, 0, r, i);
+        }
+        if (m == 1) {
+            FA(e, 0, r, 1) = FA(b, 1, r, 1) - idr * FA(psi, 0, r, 2);
+            FA(e, 0, q, 1) = FA(b, 1, q, 1) - idr * FA(psi, 0, q, 2);
+            FA(e, 1, r, 1) = -FA(b, 0, r, 1) + idr * FA(psi, 0, q, 2);
+            FA(e, 1, q, 1) = -FA(b, 0, q, 1) - idr * FA(psi, 0, r, 2);
+        } else {
+            FA(e, 0, r, 1) = 0.0; FA(e, 0, q, 1) = 0.0;
+            FA(e, 1, r, 1) = 0.0; FA(e, 1, q, 1) = 0.0;
+        }
+        FA(e, 0, r, nr) = FA(b, 1, r, nr) + idrh * (4.0 * FA(psi, 0, r, nr - 1) - FA(psi, 0, r, nr - 2)
+                                                    - 3.0 * FA(psi, 0, r, nr));
+        FA(e, 0, q, nr) = FA(b, 1, q, nr) + idrh * (4.0 * FA(psi, 0, q, nr - 1) - FA(psi, 0, q, nr - 2)
+                                                    - 3.0 * FA(psi, 0, q, nr));
+    }
+}
+
+/* ------------------------------------------------------------------------ */
+/* Bz: source/fields/field_b_class.f03:205-307 (source), :508-543 (solution),*/
+/* :760-796 (driver)                                                         */
+/* ------------------------------------------------------------------------ */
+void ora_solve_bz(const ora_sim *s, const double *jay, double *bf)
+{
+    int nr = s->g.nr, nm = s->nm, ld = s->ld;
+    double idr = 1.0 / s->g.dr, idrh = 0.5 * idr;
+    double *bre = calloc(nr + 1, sizeof(double)), *bim = calloc(nr + 1, sizeof(double));
+#define JRE(c, i) FA(jay, (c) - 1, kre(m), (i))
+#define JIM(c, i) FA(jay, (c) - 1, kim(m), (i))
+    for (int m = 0; m <= s->g.max_mode; m++) {
+        memset(bre, 0, sizeof(double) * (nr + 1));
+        memset(bim, 0, sizeof(double) * (nr + 1));
+        double ir;
+        if (m == 0) {
+            for (int i = 2; i <= nr; i++) {
+                ir = idr / (double)(i - 1);
+                bre[i] = -idrh * (JRE(2, i + 1) - JRE(2, i - 1)) - ir * JRE(2, i);
+            }
+            bre[1] = -2.0 * idr * JRE(2, 2);
+            ir = idr / (double)(nr - 1);
+            bre[nr] = -idrh * (3.0 * JRE(2, nr) - 4.0 * JRE(2, nr - 1) + JRE(2, nr - 2)) - ir * JRE(2, nr);
+        } else {
+            double dm = (double)m;
+            for (int i = 2; i <= nr; i++) {
+                ir = idr / (double)(i - 1);
+                bre[i] = -idrh * (JRE(2, i + 1) - JRE(2, i - 1)) - ir * JRE(2, i) - dm * ir * JIM(1, i);
+                bim[i] = -idrh * (JIM(2, i + 1) - JIM(2, i - 1)) - ir * JIM(2, i) + dm * ir * JRE(1, i);
+            }
+            if (m % 2 == 0) {
+                bre[1] = -2.0 * idr * JRE(2, 2) - dm * idr * JIM(1, 2);
+                bim[1] = -2.0 * idr * JIM(2, 2) + dm * idr * JRE(1, 2);
+            } else {
+                bre[1] = 0.0;
+                bim[1] = 0.0;
+                if (m == 1) {
+                    ir = idr;
+                    bre[2] = -idr * (JRE(2, 3) - JRE(2, 2)) - ir * JRE(2, 2) - dm * ir * JIM(1, 2);
+                    bim[2] = -idr * (JIM(2, 3) - JIM(2, 2)) - ir * JIM(2, 2) + dm * ir * JRE(1, 2);
+                }
+            }
+            ir = idr / (double)(nr - 1);
+            bre[nr] = -idrh * (3.0 * JRE(2, nr) - 4.0 * JRE(2, nr - 1) + JRE(2, nr - 2)) - ir * JRE(2, nr)
+                      - dm * ir * JIM(1, nr);
+            bim[nr] = -idrh * (3.0 * JIM(2, nr) - 4.0 * JIM(2, nr - 1) + JIM(2, nr - 2)) - ir * JIM(2, nr)
+                      + dm * ir * JRE(1, nr);
+        }
+        solve_sys(s, ORA_FK_BZ, m, bre + 1);
+        for (int i = 1; i <= nr; i++) FA(bf, 2, kre(m), i) = bre[i];
+        if (m > 0) {
+            solve_sys(s, ORA_FK_BZ, m, bim + 1);
+            for (int i = 1; i <= nr; i++) FA(bf, 2, kim(m), i) = bim[i];
+            FA(bf, 2, kre(m), 1) = 0.0;   /* Bz(m>0) vanishes on axis */
+            FA(bf, 2, kim(m), 1) = 0.0;
+        }
+    }
+#undef JRE
+#undef JIM
+    free(bre); free(bim);
+}
+
+/* ------------------------------------------------------------------------ */
+/* Beam B_perp: source/fields/field_b_class.f03:309-358 (source),            */
+/* :545-701 (solution), :798-834 (driver)                                    */
+/* ------------------------------------------------------------------------ */
+void ora_solve_bt_beam(const ora_sim *s, const double *q, double *bf)
+{
+    int nr = s->g.nr, nm = s->nm, ld = s->ld;
+    double idr = 1.0 / s->g.dr, idrh = 0.5 * idr;
+    double *bre = calloc(nr + 1, sizeof(double)), *bim = calloc(nr + 1, sizeof(double));
+    for (int m = 0; m <= s->g.max_mode; m++) {
+        for (int i = 1; i <= nr; i++) bre[i] = -1.0 * FA(q, 0, kre(m), i);
+        solve_sys(s, ORA_FK_BT, m, bre + 1);
+        if (m == 0) {
+            for (int i = 2; i <= nr - 1; i++) {
+                FA(bf, 0, 0, i) = 0.0;
+                FA(bf, 1, 0, i) = -idrh * (bre[i + 1] - bre[i - 1]);
+            }
+            FA(bf, 0, 0, 1) = 0.0;
+            FA(bf, 1, 0, 1) = 0.0;
+            FA(bf, 0, 0, nr) = 0.0;
+            FA(bf, 1, 0, nr) = -idrh * (3.0 * bre[nr] - 4.0 * bre[nr - 1] + bre[nr - 2]);
+            continue;
+        }
+        for (int i = 1; i <= nr; i++) bim[i] = -1.0 * FA(q, 0, kim(m), i);
+        solve_sys(s, ORA_FK_BT, m, bim + 1);
+        int r = kre(m), k = kim(m);
+        double dm = (double)m, ir;
+        for (int i = 2; i <= nr - 1; i++) {
+            ir = idr / (double)(i - 1);
+            FA(bf, 0, r, i) = -ir * dm * bim[i];
+            FA(bf, 0, k, i) = ir * dm * bre[i];
+            FA(bf, 1, r, i) = -idrh * (bre[i + 1] - bre[i - 1]);
+            FA(bf, 1, k, i) = -idrh * (bim[i + 1] - bim[i - 1]);
+        }
+        if (m == 1) {
+            FA(bf, 0, r, 1) = -idr * dm * bim[2];
+            FA(bf, 0, k, 1) = idr * dm * bre[2];
+            FA(bf, 1, r, 1) = -idr * bre[2];
+            FA(bf, 1, k, 1) = -idr * bim[2];
+        } else {
+            FA(bf, 0, r, 1) = 0.0; FA(bf, 0, k, 1) = 0.0;
+            FA(bf, 1, r, 1) = 0.0; FA(bf, 1, k, 1) = 0.0;
+        }
+        ir = idr / (double)(nr - 1);
+        FA(bf, 0, r, nr) = -ir * dm * bim[nr];
+        FA(bf, 0, k, nr) = ir * dm * bre[nr];
+        FA(bf, 1, r, nr) = -idrh * (3.0 * bre[nr] - 4.0 * bre[nr - 1] + bre[nr - 2]);
+        FA(bf, 1, k, nr) = -idrh * (3.0 * bim[nr] - 4.0 * bim[nr - 1] + bim[nr - 2]);
+    }
+    free(bre); free(bim);
+}
+
+/* ------------------------------------------------------------------------ */
+/* Plasma B_perp predictor-corrector: source/fields/field_b_class.f03:360-506*/
+/* (source, relaxation uses the current b), :703-758 (B+/B- recombination),  */
+/* :836-879 (driver)                                                         */
+/* ------------------------------------------------------------------------ */
+void ora_solve_bt_iter(const ora_sim *s, const double *djdxi, const double *jay, double *bf)
+{
+    int nr = s->g.nr, nm = s->nm, ld = s->ld;
+    double idr = 1.0 / s->g.dr, idrh = 0.5 * idr;
+    double rx = s->g.relax_fac * (idr * idr);   /* relax_fac * idr**2 */
+    double *b1r = calloc(nr + 1, sizeof(double)), *b1i = calloc(nr + 1, sizeof(double));
+    double *b2r = calloc(nr + 1, sizeof(double)), *b2i = calloc(nr + 1, sizeof(double));
+    for (int m = 0; m <= s->g.max_mode; m++) {
+        int kr = kre(m), ki = kim(m);
+        memset(b1r, 0, sizeof(double) * (nr + 1)); memset(b2r, 0, sizeof(double) * (nr + 1));
+        memset(b1i, 0, sizeof(double) * (nr + 1)); memset(b2i, 0, sizeof(double) * (nr + 1));
+#define F1R(c, i) FA(djdxi, (c) - 1, kr, (i))
+#define F1I(c, i) FA(djdxi, (c) - 1, ki, (i))
+#define F2R(c, i) FA(jay, (c) - 1, kr, (i))
+#define F2I(c, i) FA(jay, (c) - 1, ki, (i))
+#define F3R(c, i) FA(bf, (c) - 1, kr, (i))
+#define F3I(c, i) FA(bf, (c) - 1, ki, (i))
+        if (m == 0) {
+            for (int i = 2; i <= nr; i++) {
+                b1r[i] = -F1R(2, i) - F3R(1, i) * rx;
+                b2r[i] = F1R(1, i) + idrh * (F2R(3, i + 1) - F2R(3, i - 1)) - F3R(2, i) * rx;
+            }
+            b1r[1] = 0.0;
+            b2r[1] = 0.0;
+            b2r[2] = F1R(1, 2) + idr * (F2R(3, 3) - F2R(3, 2)) - F3R(2, 2) * rx;
+            b1r[nr] = -F1R(2, nr) - F3R(1, nr) * rx;
+            b2r[nr] = F1R(1, nr) + idrh * (3.0 * F2R(3, nr) - 4.0 * F2R(3, nr - 1) + F2R(3, nr - 2))
+                      - F3R(2, nr) * rx;
+            solve_sys(s, ORA_FK_BPLUS, 0, b1r + 1);
+            solve_sys(s, ORA_FK_BMINUS, 0, b2r + 1);
+            for (int i = 1; i <= nr; i++) {
+                FA(bf, 0, 0, i) = b1r[i];
+                FA(bf, 1, 0, i) = b2r[i];
+            }
+            FA(bf, 0, 0, 1) = 0.0;
+            FA(bf, 1, 0, 1) = 0.0;
+            continue;
+        }
+        double dm = (double)m, ir, s1r, s1i, s2r, s2i;
+        for (int i = 2; i <= nr; i++) {
+            ir = idr / (double)(i - 1);
+            s1r = -F1R(2, i) + dm * F2I(3, i) * ir;
+            s1i = -F1I(2, i) - dm * F2R(3, i) * ir;
+            s2r = F1R(1, i) + idrh * (F2R(3, i + 1) - F2R(3, i - 1));
+            s2i = F1I(1, i) + idrh * (F2I(3, i + 1) - F2I(3, i - 1));
+            b1r[i] = s1r - s2i - (F3R(1, i) - F3I(2, i)) * rx;
+            b1i[i] = s1i + s2r - (F3I(1, i) + F3R(2, i)) * rx;
+            b2r[i] = s1r + s2i - (F3R(1, i) + F3I(2, i)) * rx;
+            b2i[i] = s1i - s2r - (F3I(1, i) - F3R(2, i)) * rx;
+        }
+        if (m == 1) {
+            s1r = -F1R(2, 1) + idr * dm * F2I(3, 2);
+            s1i = -F1I(2, 1) - idr * dm * F2R(3, 2);
+            s2r = F1R(1, 1) + idr * F2R(3, 2);
+            s2i = F1I(1, 1) + idr * F2I(3, 2);
+        } else if (m % 2 == 0) {
+            s1r = 0.0; s1i = 0.0; s2r = 0.0; s2i = 0.0;
+        } else {
+            s1r = idr * dm * F2I(3, 2);
+            s1i = idr * dm * F2R(3, 2);
+            s2r = idr * F2R(3, 2);
+            s2i = idr * F2I(3, 2);
+        }
+        b1r[1] = s1r - s2i - (F3R(1, 1) - F3I(2, 1)) * rx;
+        b1i[1] = s1i + s2r - (F3I(1, 1) + F3R(2, 1)) * rx;
+        b2r[1] = s1r + s2i - (F3R(1, 1) + F3I(2, 1)) * rx;
+        b2i[1] = s1i - s2r - (F3I(1, 1) - F3R(2, 1)) * rx;
+        ir = idr / (double)(nr - 1);
+        s1r = -F1R(2, nr) + dm * F2I(3, nr) * ir;
+        s1i = -F1I(2, nr) - dm * F2R(3, nr) * ir;
+        s2r = F1R(1, nr) + idrh * (3.0 * F2R(3, nr) - 4.0 * F2R(3, nr - 1) + F2R(3, nr - 2));
+        s2i = F1I(1, nr) + idrh * (3.0 * F2I(3, nr) - 4.0 * F2I(3, nr - 1) + F2I(3, nr - 2));
+        b1r[nr] = s1r - s2i - (F3R(1, nr) - F3I(2, nr)) * rx;
+        b1i[nr] = s1i + s2r - (F3I(1, nr) + F3R(2, nr)) * rx;
+        b2r[nr] = s1r + s2i - (F3R(1, nr) + F3I(2, nr)) * rx;
+        b2i[nr] = s1i - s2r - (F3I(1, nr) - F3R(2, nr)) * rx;
+        solve_sys(s, ORA_FK_BPLUS, m, b1r + 1);
+        solve_sys(s, ORA_FK_BPLUS, m, b1i + 1);
+        solve_sys(s, ORA_FK_BMINUS, m, b2r + 1);
+        solve_sys(s, ORA_FK_BMINUS, m, b2i + 1);
+        for (int i = 1; i <= nr; i++) {
+            FA(bf, 0, kr, i) = 0.5 * (b1r[i] + b2r[i]);
+            FA(bf, 0, ki, i) = 0.5 * (b1i[i] + b2i[i]);
+            FA(bf, 1, kr, i) = 0.5 * (b1i[i] - b2i[i]);
+            FA(bf, 1, ki, i) = 0.5 * (-b1r[i] + b2r[i]);
+        }
+        if (m != 1) {
+            FA(bf, 0, kr, 1) = 0.0; FA(bf, 0, ki, 1) = 0.0;
+            FA(bf, 1, kr, 1) = 0.0; FA(bf, 1, ki, 1) = 0.0;
+        }
+#undef F1R
+#undef F1I
+#undef F2R
+#undef F2I
+#undef F3R
+#undef F3I
+    }
+    free(b1r); free(b1i); free(b2r); free(b2i);
+}
+
+/* ------------------------------------------------------------------------ */
+/* dJ_perp/dxi: source/fields/field_src_class.f03:273-405                    */
+/* ------------------------------------------------------------------------ */
+void ora_solve_djdxi(const ora_sim *s, const double *acu, const double *amu, double *dcu)
+{
+    int nr = s->g.nr, nm = s->nm, ld = s->ld;
+    double idr = 1.0 / s->g.dr, idrh = idr * 0.5;
+    for (int m = 0; m <= s->g.max_mode; m++) {
+        int kr = kre(m), ki = kim(m);
+#define AR(c, i) FA(acu, (c) - 1, kr, (i))
+#define AI(c, i) FA(acu, (c) - 1, ki, (i))
+#define MR(c, i) FA(amu, (c) - 1, kr, (i))
+#define MI(c, i) FA(amu, (c) - 1, ki, (i))
+#define DR(c, i) FA(dcu, (c) - 1, kr, (i))
+#define DI(c, i) FA(dcu, (c) - 1, ki, (i))
+        double ir;
+        if (m == 0) {
+            for (int i = 2; i <= nr; i++) {
+                ir = idr / (double)(i - 1);
+                DR(1, i) = AR(1, i) - idrh * (MR(1, i + 1) - MR(1, i - 1)) - ir * MR(1, i);
+                DR(2, i) = AR(2, i) - idrh * (MR(2, i + 1) - MR(2, i - 1)) - ir * MR(2, i);
+            }
+            DR(1, 1) = 0.0;
+            DR(2, 1) = 0.0;
+            ir = idr / (double)(nr - 1);
+            DR(1, nr) = AR(1, nr) + idrh * (4.0 * MR(1, nr - 1) - MR(1, nr - 2) - 3.0 * MR(1, nr)) - ir * MR(1, nr);
+            DR(2, nr) = AR(2, nr) + idrh * (4.0 * MR(2, nr - 1) - MR(2, nr - 2) - 3.0 * MR(2, nr)) - ir * MR(2, nr);
+            continue;
+        }
+        double dm = (double)m;
+        for (int i = 2; i <= nr; i++) {
+            ir = idr / (double)(i - 1);
+            DR(1, i) = AR(1, i) - idrh * (MR(1, i + 1) - MR(1, i - 1)) - ir * MR(1, i) + dm * ir * MI(2, i);
+            DR(2, i) = AR(2, i) - idrh * (MR(2, i + 1) - MR(2, i - 1)) - ir * MR(2, i) + dm * ir * MI(3, i);
+            DI(1, i) = AI(1, i) - idrh * (MI(1, i + 1) - MI(1, i - 1)) - ir * MI(1, i) - dm * ir * MR(2, i);
+            DI(2, i) = AI(2, i) - idrh * (MI(2, i + 1) - MI(2, i - 1)) - ir * MI(2, i) - dm * ir * MR(3, i);
+        }
+        if (m == 1) {
+            DR(1, 1) = AR(1, 1) - 2.0 * idr * MR(1, 2) + dm * idr * MI(2, 2);
+            DR(2, 1) = AR(2, 1) - 2.0 * idr * MR(2, 2) + dm * idr * MI(3, 2);
+            DI(1, 1) = AI(1, 1) - 2.0 * idr * MI(1, 2) - dm * idr * MR(2, 2);
+            DI(2, 1) = AI(2, 1) - 2.0 * idr * MI(2, 2) - dm * idr * MR(3, 2);
+        } else {
+            DR(1, 1) = 0.0; DR(2, 1) = 0.0; DI(1, 1) = 0.0; DI(2, 1) = 0.0;
+            if (m == 2) {
+                ir = idr;
+                DR(1, 2) = AR(1, 2) - idr * (MR(1, 3) - MR(1, 2)) - ir * MR(1, 2) + dm * ir * MI(2, 2);
+                DR(2, 2) = AR(2, 2) - idr * (MR(2, 3) - MR(2, 2)) - ir * MR(2, 2) + dm * ir * MI(3, 2);
+                DI(1, 2) = AI(1, 2) - idr * (MI(1, 3) - MI(1, 2)) - ir * MI(1, 2) - dm * ir * MR(2, 2);
+                DI(2, 2) = AI(2, 2) - idr * (MI(2, 3) - MI(2, 2)) - ir * MI(2, 2) - dm * ir * MR(3, 2);
+            }
+        }
+        ir = idr / (double)(nr - 1);
+        DR(1, nr) = AR(1, nr) + idrh * (4.0 * MR(1, nr - 1) - MR(1, nr - 2) - 3.0 * MR(1, nr)) - ir * MR(1, nr)
+                    + dm * ir * MI(2, nr);
+        DR(2, nr) = AR(2, nr) + idrh * (4.0 * MR(2, nr - 1) - MR(2, nr - 2) - 3.0 * MR(2, nr)) - ir * MR(2, nr)
+                    + dm * ir * MI(3, nr);
+        DI(1, nr) = AI(1, nr) + idrh * (4.0 * MI(1, nr - 1) - MI(1, nr - 2) - 3.0 * MI(1, nr)) - ir * MI(1, nr)
+                    - dm * ir * MR(2, nr);
+        DI(2, nr) = AI(2, nr) + idrh * (4.0 * MI(2, nr - 1) - MI(2, nr - 2) - 3.0 * MI(2, nr)) - ir * MI(2, nr)
+                    - dm * ir * MR(3, nr);
+#undef AR
+#undef AI
+#undef MR
+#undef MI
+#undef DR
+#undef DI
+    }
+}
+
+/* ------------------------------------------------------------------------ */
+/* convergence_tester: source/simulation_class.f03:522-606 (dim = 2, B_phi)  */
+/* ------------------------------------------------------------------------ */
+void ora_conv_record(ora_sim *s, const double *b)
+{
+    int nr = s->g.nr, nm = s->nm, ld = s->ld;
+    for (int i = 1; i <= nr; i++) { s->conv_re[i] = 0.0; s->conv_im[i] = 0.0; }
+    for (int m = 0; m <= s->g.max_mode; m++) {
+        for (int i = 1; i <= nr; i++) s->conv_re[i] = s->conv_re[i] + fabs(FA(b, 1, kre(m), i));
+        if (m == 0) continue;
+        for (int i = 1; i <= nr; i++) s->conv_im[i] = s->conv_im[i] + fabs(FA(b, 1, kim(m), i));
+    }
+}
+
+void ora_conv_compare(ora_sim *s, const double *b, double *rel, double *abs_)
+{
+    int nr = s->g.nr, nm = s->nm, ld = s->ld;
+    double mx = -INFINITY;
+    for (int i = 1; i <= nr; i++) {
+        double v = s->conv_re[i] * s->conv_re[i] + s->conv_im[i] * s->conv_im[i];
+        if (v > mx) mx = v;
+    }
+    double old_norm = sqrt(mx);
+    for (int m = 0; m <= s->g.max_mode; m++) {
+        for (int i = 1; i <= nr; i++) s->conv_re[i] = s->conv_re[i] - fabs(FA(b, 1, kre(m), i));
+        if (m == 0) continue;
+        for (int i = 1; i <= nr; i++) s->conv_im[i] = s->conv_im[i] - fabs(FA(b, 1, kim(m), i));
+    }
+    mx = -INFINITY;
+    for (int i = 1; i <= nr; i++) {
+        double v = s->conv_re[i] * s->conv_re[i] + s->conv_im[i] * s->conv_im[i];
+        if (v > mx) mx = v;
+    }
+    *abs_ = sqrt(mx);
+    if (old_norm > 2.220446049250313e-16) *rel = *abs_ / old_norm;   /* epsilon(1.0d0) */
+    else *rel = 1.7976931348623157e308;                              /* huge(1.0)     */
+}
+
+/* ------------------------------------------------------------------------ */
+/* Plasma particles                                                          */
+/* ------------------------------------------------------------------------ */
+
+/* gen_interp_info: source/species/interp_part2d.f03:28-64 */
+static inline void interp_info(double x1, double x2, double idr, int *idx,
+                               double *w0, double *w1, double *pc, double *ps)
+{
+    double pos = sqrt(x1 * x1 + x2 * x2);
+    *pc = x1 / pos;
+    *ps = x2 / pos;
+    pos = pos * idr;
+    int ip = (int)pos;
+    *idx = ip + 1;
+    pos = pos - (double)ip;
+    *w0 = 1.0 - pos;   /* spline_linear, source/interpolation.f03:10-20 */
+    *w1 = pos;
+}
+
+/* interp_field (vector form): source/species/interp_part2d.f03:66-110 */
+static inline void interp_vec(const ora_sim *s, const double *f, int ncomp,
+                              int idx, double w0, double w1, double pc, double ps,
+                              double *fp)
+{
+    int nm = s->nm, ld = s->ld;
+    for (int c = 0; c < ncomp; c++) fp[c] = 0.0;
+    for (int c = 0; c < ncomp; c++) {
+        fp[c] = fp[c] + FA(f, c, 0, idx) * w0;
+        fp[c] = fp[c] + FA(f, c, 0, idx + 1) * w1;
+    }
+    double phr = 1.0, phi = 0.0;
+    for (int m = 1; m <= s->g.max_mode; m++) {
+        double nr_ = phr * pc - phi * ps, ni_ = phr * ps + phi * pc;
+        phr = nr_; phi = ni_;
+        double ph_re = 2.0 * phr, ph_im = 2.0 * phi;
+        for (int c = 0; c < ncomp; c++) {
+            fp[c] = fp[c] + (FA(f, c, kre(m), idx) * ph_re - FA(f, c, kim(m), idx) * ph_im) * w0;
+            fp[c] = fp[c] + (FA(f, c, kre(m), idx + 1) * ph_re - FA(f, c, kim(m), idx + 1) * ph_im) * w1;
+        }
+    }
+}
+
+void ora_interp_point(const ora_sim *s, const double *f, int ncomp, double x1,
+                      double x2, double *out)
+{
+    int idx; double w0, w1, pc, ps;
+    interp_info(x1, x2, 1.0 / s->g.dr, &idx, &w0, &w1, &pc, &ps);
+    interp_vec(s, f, ncomp, idx, w0, w1, pc, ps, out);
+}
+
+/* qdeposit_part2d: source/species/part2d_class.f03:231-359 */
+void ora_qdeposit(const ora_sim *s, const ora_species *sp, double *q)
+{
+    int nr = s->g.nr, nm = s->nm, ld = s->ld, M = s->g.max_mode;
+    double idr = 1.0 / s->g.dr;
+    for (int64_t p = 0; p < sp->npp; p++) {
+        double pos = sqrt(sp->x1[p] * sp->x1[p] + sp->x2[p] * sp->x2[p]) * idr;
+        double ph0r = sp->x1[p] / pos * idr, ph0i = -sp->x2[p] / pos * idr;
+        int nn = (int)floor(pos);
+        pos = pos - (double)nn;
+        nn = nn + 1;
+        double wt[2] = {1.0 - pos, pos};
+        double phr = 1.0 * sp->q[p], phi = 0.0 * sp->q[p];
+        for (int j = 0; j < 2; j++) FA(q, 0, 0, nn + j) = FA(q, 0, 0, nn + j) + wt[j] * phr;
+        for (int m = 1; m <= M; m++) {
+            double a = phr * ph0r - phi * ph0i, b = phr * ph0i + phi * ph0r;
+            phr = a; phi = b;
+            for (int j = 0; j < 2; j++) {
+                FA(q, 0, kre(m), nn + j) = FA(q, 0, kre(m), nn + j) + wt[j] * phr;
+                FA(q, 0, kim(m), nn + j) = FA(q, 0, kim(m), nn + j) + wt[j] * phi;
+            }
+        }
+    }
+    /* axis and 1/r (:313-336) */
+    FA(q, 0, 0, 0) = 0.0;
+    FA(q, 0, 0, 1) = 8.0 * FA(q, 0, 0, 1);
+    for (int j = 2; j <= nr + 1; j++) {
+        double ir = 1.0 / (double)(j - 1);
+        FA(q, 0, 0, j) = FA(q, 0, 0, j) * ir;
+    }
+    for (int m = 1; m <= M; m++) {
+        FA(q, 0, kre(m), 0) = 0.0; FA(q, 0, kim(m), 0) = 0.0;
+        FA(q, 0, kre(m), 1) = 0.0; FA(q, 0, kim(m), 1) = 0.0;
+        for (int j = 2; j <= nr + 1; j++) {
+            double ir = 1.0 / (double)(j - 1);
+            FA(q, 0, kre(m), j) = FA(q, 0, kre(m), j) * ir;
+            FA(q, 0, kim(m), j) = FA(q, 0, kim(m), j) * ir;
+        }
+    }
+}
+
+/* amjdeposit_robust_part2d: source/species/part2d_class.f03:746-1010 */
+void ora_amjdeposit_robust(const ora_sim *s, ora_species *sp, const double *ef,
+                           const double *bf, double *cu, double *amu,
+                           double *dcu, double dt)
+{
+    int nr = s->g.nr, nm = s->nm, ld = s->ld, M = s->g.max_mode;
+    double idr = 1.0 / s->g.dr;
+    double idt = 1.0 / dt;
+    double qtmh = 0.5 * sp->qbm * dt;
+    for (int64_t p = 0; p < sp->npp; p++) {
+        int ix; double w0, w1, pc, ps;
+        interp_info(sp->x1[p], sp->x2[p], idr, &ix, &w0, &w1, &pc, &ps);
+        double ep[3], bp[3], wp[3], u0[3], u[3], ut[3], du[3], u2[3];
+        interp_vec(s, ef, 3, ix, w0, w1, pc, ps, ep);
+        interp_vec(s, bf, 3, ix, w0, w1, pc, ps, bp);
+        wp[0] = ep[0] - bp[1];
+        wp[1] = ep[1] + bp[0];
+        wp[2] = ep[2];
+        u0[0] = sp->p1[p] * pc + sp->p2[p] * ps;
+        u0[1] = sp->p2[p] * pc - sp->p1[p] * ps;
+        u0[2] = sp->p3[p];
+        double gam = sqrt(1.0 + u0[0] * u0[0] + u0[1] * u0[1] + u0[2] * u0[2]);
+        double qtmh1 = qtmh * gam / (gam - u0[2]);
+        for (int c = 0; c < 3; c++) { ep[c] = ep[c] * qtmh1; ut[c] = u0[c] + ep[c]; }
+        gam = sqrt(1.0 + ut[0] * ut[0] + ut[1] * ut[1] + ut[2] * ut[2]);
+        double qtmh2 = qtmh / (gam - ut[2]);
+        for (int c = 0; c < 3; c++) bp[c] = bp[c] * qtmh2;
+        u[0] = ut[0] + ut[1] * bp[2] - ut[2] * bp[1];
+        u[1] = ut[1] + ut[2] * bp[0] - ut[0] * bp[2];
+        u[2] = ut[2] + ut[0] * bp[1] - ut[1] * bp[0];
+        double ostq = 2.0 / (1.0 + bp[0] * bp[0] + bp[1] * bp[1] + bp[2] * bp[2]);
+        for (int c = 0; c < 3; c++) bp[c] = bp[c] * ostq;
+        ut[0] = ut[0] + u[1] * bp[2] - u[2] * bp[1];
+        ut[1] = ut[1] + u[2] * bp[0] - u[0] * bp[2];
+        ut[2] = ut[2] + u[0] * bp[1] - u[1] * bp[0];
+        for (int c = 0; c < 3; c++) u[c] = ut[c] + ep[c];
+
+        du[0] = idt * (u[0] - u0[0]);
+        du[1] = idt * (u[1] - u0[1]);
+        for (int c = 0; c < 3; c++) u[c] = 0.5 * (u[c] + u0[c]);
+        sp->gamma[p] = sqrt(1.0 + u[0] * u[0] + u[1] * u[1] + u[2] * u[2]);
+        double ipsi = 1.0 / (sp->gamma[p] - u[2]);
+        sp->psi[p] = (1.0 - 1.0 / ipsi) / sp->qbm;
+        double dpsi = sp->qbm * (wp[2] - (wp[0] * u[0] + wp[1] * u[1]) * ipsi);
+        du[0] = du[0] + u[0] * dpsi * ipsi;
+        du[1] = du[1] + u[1] * dpsi * ipsi;
+        u2[0] = u[0] * u[0] * ipsi;
+        u2[1] = u[0] * u[1] * ipsi;
+        u2[2] = u[1] * u[1] * ipsi;
+        double ph0r = pc, ph0i = -ps;
+        double phr = 1.0 * sp->q[p] * ipsi, phi = 0.0 * sp->q[p] * ipsi;
+        double wt[2] = {w0, w1};
+        for (int j = 0; j < 2; j++) {
+            double w = wt[j] * phr;
+            for (int c = 0; c < 3; c++) FA(cu, c, 0, ix + j) = FA(cu, c, 0, ix + j) + w * u[c];
+            for (int c = 0; c < 2; c++) FA(dcu, c, 0, ix + j) = FA(dcu, c, 0, ix + j) + w * du[c];
+            for (int c = 0; c < 3; c++) FA(amu, c, 0, ix + j) = FA(amu, c, 0, ix + j) + w * u2[c];
+        }
+        for (int m = 1; m <= M; m++) {
+            double a = phr * ph0r - phi * ph0i, b = phr * ph0i + phi * ph0r;
+            phr = a; phi = b;
+            int kr = kre(m), ki = kim(m);
+            for (int j = 0; j < 2; j++) {
+                double w = wt[j] * phr;
+                for (int c = 0; c < 3; c++) FA(cu, c, kr, ix + j) = FA(cu, c, kr, ix + j) + w * u[c];
+                for (int c = 0; c < 2; c++) FA(dcu, c, kr, ix + j) = FA(dcu, c, kr, ix + j) + w * du[c];
+                for (int c = 0; c < 3; c++) FA(amu, c, kr, ix + j) = FA(amu, c, kr, ix + j) + w * u2[c];
+                w = wt[j] * phi;
+                for (int c = 0; c < 3; c++) FA(cu, c, ki, ix + j) = FA(cu, c, ki, ix + j) + w * u[c];
+                for (int c = 0; c < 2; c++) FA(dcu, c, ki, ix + j) = FA(dcu, c, ki, ix + j) + w * du[c];
+                for (int c = 0; c < 3; c++) FA(amu, c, ki, ix + j) = FA(amu, c, ki, ix + j) + w * u2[c];
+            }
+        }
+    }
+    /* axis rules (:914-960) */
+    for (int c = 0; c < 3; c++) { FA(cu, c, 0, 0) = 0.0; FA(amu, c, 0, 0) = 0.0; }
+    for (int c = 0; c < 2; c++) FA(dcu, c, 0, 0) = 0.0;
+    FA(cu, 0, 0, 1) = 0.0; FA(cu, 1, 0, 1) = 0.0; FA(cu, 2, 0, 1) = 8.0 * FA(cu, 2, 0, 1);
+    FA(dcu, 0, 0, 1) = 0.0; FA(dcu, 1, 0, 1) = 0.0;
+    for (int c = 0; c < 3; c++) FA(amu, c, 0, 1) = 0.0;
+    for (int m = 1; m <= M; m++) {
+        int ks[2] = {kre(m), kim(m)};
+        for (int t = 0; t < 2; t++) {
+            int k = ks[t];
+            for (int c = 0; c < 3; c++) { FA(cu, c, k, 0) = 0.0; FA(amu, c, k, 0) = 0.0; }
+            for (int c = 0; c < 2; c++) FA(dcu, c, k, 0) = 0.0;
+            if (m == 1) {
+                FA(cu, 0, k, 1) = 8.0 * FA(cu, 0, k, 1); FA(cu, 1, k, 1) = 8.0 * FA(cu, 1, k, 1);
+                FA(cu, 2, k, 1) = 0.0;
+                FA(dcu, 0, k, 1) = 8.0 * FA(dcu, 0, k, 1); FA(dcu, 1, k, 1) = 8.0 * FA(dcu, 1, k, 1);
+                for (int c = 0; c < 3; c++) FA(amu, c, k, 1) = 0.0;
+            } else if (m == 2) {
+                for (int c = 0; c < 3; c++) FA(cu, c, k, 1) = 0.0;
+                for (int c = 0; c < 2; c++) FA(dcu, c, k, 1) = 0.0;
+                for (int c = 0; c < 3; c++) FA(amu, c, k, 1) = 8.0 * FA(amu, c, k, 1);
+            } else {
+                for (int c = 0; c < 3; c++) FA(cu, c, k, 1) = 0.0;
+                for (int c = 0; c < 2; c++) FA(dcu, c, k, 1) = 0.0;
+                for (int c = 0; c < 3; c++) FA(amu, c, k, 1) = 0.0;
+            }
+        }
+    }
+    /* divide by the node index (:962-1004) */
+    for (int k = 0; k < nm; k++) {
+        for (int j = 2; j <= nr + 1; j++) {
+            double ir = 1.0 / (double)(j - 1);
+            for (int c = 0; c < 3; c++) FA(cu, c, k, j) = FA(cu, c, k, j) * ir;
+            for (int c = 0; c < 2; c++) FA(dcu, c, k, j) = FA(dcu, c, k, j) * ir;
+            for (int c = 0; c < 3; c++) FA(amu, c, k, j) = FA(amu, c, k, j) * ir;
+        }
+    }
+}
+
+/* push_u_robust_part2d: source/species/part2d_class.f03:1879-1965 */
+void ora_push_u_robust(const ora_sim *s, ora_species *sp, const double *ef,
+                       const double *bf, double dt)
+{
+    double idr = 1.0 / s->g.dr;
+    double qtmh = sp->qbm * dt * 0.5;
+    for (int64_t p = 0; p < sp->npp; p++) {
+        int ix; double w0, w1, pc, ps;
+        interp_info(sp->x1[p], sp->x2[p], idr, &ix, &w0, &w1, &pc, &ps);
+        double ep[3], bp[3], ut[3], pp[3];
+        interp_vec(s, ef, 3, ix, w0, w1, pc, ps, ep);
+        interp_vec(s, bf, 3, ix, w0, w1, pc, ps, bp);
+        /* transform_to_cartesian (interp_part2d.f03:155-171) */
+        double t = ep[0] * pc - ep[1] * ps; ep[1] = ep[0] * ps + ep[1] * pc; ep[0] = t;
+        t = bp[0] * pc - bp[1] * ps; bp[1] = bp[0] * ps + bp[1] * pc; bp[0] = t;
+        pp[0] = sp->p1[p]; pp[1] = sp->p2[p]; pp[2] = sp->p3[p];
+        double gam = sqrt(1.0 + pp[0] * pp[0] + pp[1] * pp[1] + pp[2] * pp[2]);
+        double qtmh1 = qtmh / (gam - pp[2]);
+        double qtmh2 = qtmh1 * gam;
+        for (int c = 0; c < 3; c++) { ep[c] = ep[c] * qtmh2; bp[c] = bp[c] * qtmh1; }
+        for (int c = 0; c < 3; c++) ut[c] = pp[c] + ep[c];
+        pp[0] = ut[0] + ut[1] * bp[2] - ut[2] * bp[1];
+        pp[1] = ut[1] + ut[2] * bp[0] - ut[0] * bp[2];
+        pp[2] = ut[2] + ut[0] * bp[1] - ut[1] * bp[0];
+        double ostq = 2.0 / (1.0 + bp[0] * bp[0] + bp[1] * bp[1] + bp[2] * bp[2]);
+        for (int c = 0; c < 3; c++) bp[c] = bp[c] * ostq;
+        ut[0] = ut[0] + pp[1] * bp[2] - pp[2] * bp[1];
+        ut[1] = ut[1] + pp[2] * bp[0] - pp[0] * bp[2];
+        ut[2] = ut[2] + pp[0] * bp[1] - pp[1] * bp[0];
+        for (int c = 0; c < 3; c++) pp[c] = ut[c] + ep[c];
+        sp->p1[p] = pp[0]; sp->p2[p] = pp[1]; sp->p3[p] = pp[2];
+        sp->gamma[p] = sqrt(1.0 + pp[0] * pp[0] + pp[1] * pp[1] + pp[2] * pp[2]);
+    }
+}
+
+/* push_x_part2d: source/species/part2d_class.f03:2221-2262 */
+void ora_push_x(ora_species *sp, double dt)
+{
+    for (int64_t p = 0; p < sp->npp; p++) {
+        double dtc = dt / (sp->gamma[p] - sp->p3[p]);
+        sp->x1[p] = sp->x1[p] + sp->p1[p] * dtc;
+        sp->x2[p] = sp->x2[p] + sp->p2[p] * dtc;
+    }
+}
+
+/* update_bound_part2d: source/species/part2d_class.f03:2307-2353
+   (swap-with-last removal of r >= edge) */
+void ora_update_bound(const ora_sim *s, ora_species *sp)
+{
+    if (sp->npp == 0) return;
+    double edge = (double)s->g.nr * s->g.dr;
+    int64_t i = 0;
+#define MOVE(dst, src) do { sp->x1[dst] = sp->x1[src]; sp->x2[dst] = sp->x2[src]; \
+        sp->p1[dst] = sp->p1[src]; sp->p2[dst] = sp->p2[src]; sp->p3[dst] = sp->p3[src]; \
+        sp->gamma[dst] = sp->gamma[src]; sp->psi[dst] = sp->psi[src]; sp->q[dst] = sp->q[src]; \
+        sp->id[dst] = sp->id[src]; } while (0)
+    while (i < sp->npp - 1) {
+        double pos = sqrt(sp->x1[i] * sp->x1[i] + sp->x2[i] * sp->x2[i]);
+        if (pos >= edge) {
+            MOVE(i, sp->npp - 1);
+            sp->npp--;
+            continue;
+        }
+        i++;
+    }
+    int64_t l = sp->npp - 1;
+    double pos = sqrt(sp->x1[l] * sp->x1[l] + sp->x2[l] * sp->x2[l]);
+    if (pos >= edge) sp->npp--;
+#undef MOVE
+}
+
+/* ------------------------------------------------------------------------ */
+/* Philox4x32-10 counter-based generator (Salmon, Moraes, Dror, Shaw,        */
+/* "Parallel random numbers: as easy as 1, 2, 3", SC'11)                     */
+/* ------------------------------------------------------------------------ */
+void ora_philox4x32(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4])
+{
+    uint32_t c0 = ctr[0], c1 = ctr[1], c2 = ctr[2], c3 = ctr[3];
+    uint32_t k0 = key[0], k1 = key[1];
+    for (int r = 0; r < 10; r++) {
+        if (r > 0) { k0 += 0x9E3779B9u; k1 += 0xBB67AE85u; }
+        uint64_t p0 = (uint64_t)0xD2511F53u * c0;
+        uint64_t p1 = (uint64_t)0xCD9E8D57u * c2;
+        uint32_t hi0 = (uint32_t)(p0 >> 32), lo0 = (uint32_t)p0;
+        uint32_t hi1 = (uint32_t)(p1 >> 32), lo1 = (uint32_t)p1;
+        uint32_t n0 = hi1 ^ c1 ^ k0, n2 = hi0 ^ c3 ^ k1;
+        c0 = n0; c1 = lo1; c2 = n2; c3 = lo0;
+    }
+    out[0] = c0; out[1] = c1; out[2] = c2; out[3] = c3;
+}
+
+/* Two uniforms (53-bit) -> Box-Muller pair; the stand-in for rand_norm
+   (source/math_module.f03:77-102).  out[0..1] = pair from sub, out[2..3] =
+   pair from sub+1. */
+void ora_normal4(uint64_t seed, uint64_t stream, uint32_t sub, double out[4])
+{
+    uint32_t key[2] = {(uint32_t)seed, (uint32_t)(seed >> 32)};
+    for (int h = 0; h < 2; h++) {
+        uint32_t ctr[4] = {(uint32_t)stream, (uint32_t)(stream >> 32), sub + (uint32_t)h, 0x51ED2D0Bu};
+        uint32_t o[4];
+        ora_philox4x32(ctr, key, o);
+        double u1 = ((double)(o[0] >> 5) * 67108864.0 + (double)(o[1] >> 6)) * (1.0 / 9007199254740992.0);
+        double u2 = ((double)(o[2] >> 5) * 67108864.0 + (double)(o[3] >> 6)) * (1.0 / 9007199254740992.0);
+        double theta = 2.0 * PI_DB * u1;
+        double amp = sqrt(-2.0 * log(1.0 - u2));
+        out[2 * h] = amp * cos(theta);
+        out[2 * h + 1] = amp * sin(theta);
+    }
+}
+
+/* inject_fdist2d: source/species/fdist2d_class.f03:289-357 (uniform
+   perpendicular and longitudinal profiles, deterministic theta) */
+void ora_species_inject(ora_sim *s, int isp)
+{
+    ora_species *sp = &s->sp[isp];
+    const ora_species_cfg *c = &sp->cfg;
+    int nr = s->g.nr;
+    double dr = s->g.dr;
+    double dtheta = 2.0 * PI_DB / (double)c->num_theta;
+    int ppc_tot = c->ppc1 * c->ppc2;
+    double den_lon = 1.0, den_perp = 1.0;
+    double coef = copysign(1.0, c->qm) / ((double)ppc_tot * (double)c->num_theta);
+    int64_t ip = 0;
+    for (int j = 1; j <= c->num_theta; j++) {
+        for (int i = 1; i <= nr; i++) {
+            for (int i1 = 1; i1 <= c->ppc1; i1++) {
+                double rn = ((double)i1 - 0.5) / (double)c->ppc1 + (double)(i - 1);
+                for (int i2 = 1; i2 <= c->ppc2; i2++) {
+                    double theta = (((double)i2 - 0.5) / (double)c->ppc2 + (double)j - 1.0) * dtheta;
+                    double xt1 = rn * dr * cos(theta), xt2 = rn * dr * sin(theta);
+                    if (den_lon * den_perp * c->density < c->den_min) continue;
+                    sp->x1[ip] = xt1;
+                    sp->x2[ip] = xt2;
+                    sp->q[ip] = rn * den_perp * den_lon * c->density * coef;
+                    double th[4] = {0.0, 0.0, 0.0, 0.0};
+                    if (c->uth[0] != 0.0 || c->uth[1] != 0.0 || c->uth[2] != 0.0)
+                        ora_normal4(c->seed, (uint64_t)ip, 0u, th);
+                    sp->p1[ip] = c->uth[0] * th[0];
+                    sp->p2[ip] = c->uth[1] * th[1];
+                    sp->p3[ip] = c->uth[2] * th[2];
+                    sp->gamma[ip] = sqrt(1.0 + sp->p1[ip] * sp->p1[ip] + sp->p2[ip] * sp->p2[ip]
+                                         + sp->p3[ip] * sp->p3[ip]);
+                    sp->psi[ip] = (1.0 - sp->gamma[ip] + sp->p3[ip]) / c->qm;
+                    sp->id[ip] = (int32_t)ip;
+                    ip++;
+                }
+            }
+        }
+    }
+    sp->npp = ip;
+}
+
+/* species2d%new / renew charge set-up: source/species/species2d_class.f03:
+   108-130 and :154-184 (q deposit, slice-1 copy, neutralising background) */
+void ora_species_init_q(ora_sim *s, int isp)
+{
+    ora_species *sp = &s->sp[isp];
+    int nm = s->nm, ld = s->ld;
+    size_t n1 = (size_t)nm * ld;
+    memset(sp->q_own, 0, sizeof(double) * n1);
+    ora_qdeposit(s, sp, sp->q_own);
+    memcpy(sp->q_hist + 1 * n1, sp->q_own, sizeof(double) * n1);
+    if (sp->cfg.neutralized) {
+        for (size_t t = 0; t < n1; t++) sp->qn[t] = -1.0 * sp->q_own[t];
+    }
+}
+
+/* ------------------------------------------------------------------------ */
+/* Beam                                                                      */
+/* ------------------------------------------------------------------------ */
+
+/* inject_fdist3d_rnd: source/beam/fdist3d_rnd_class.f03:360-513 (Cartesian
+   geometry, Gaussian profiles, no Twiss tilt, no xi-dependent offset).  The
+   draws come from Philox (counter = particle index, attempt) instead of
+   gfortran's random_number. */
+void ora_beam_inject(ora_sim *s, int ib, const ora_beam_cfg *c)
+{
+    ora_beam *bm = &s->bm[ib];
+    double dr = s->g.dr, dz = s->g.dxi;
+    double edge_r = (double)s->g.nr * dr, edge_z = (double)s->g.nxi * dz;
+    double q_per_part = copysign(1.0, c->qm) * c->total_charge
+                        / (2.0 * PI_DB * (double)c->total_num * dr * dr * dz);
+    int64_t n = 0;
+    for (int64_t i = 0; i < c->total_num; i++) {
+        double xt[3];
+        for (uint32_t a = 0;; a++) {
+            double g[4];
+            ora_normal4(c->seed, (uint64_t)i, 2u * a, g);
+            int ok = 1;
+            for (int d = 0; d < 3 && ok; d++) {
+                xt[d] = g[d] * c->sigma[d] + c->center[d];
+                if (xt[d] < c->range_lo[d] || xt[d] >= c->range_hi[d]) ok = 0;
+            }
+            if (ok) break;
+        }
+        double r_tmp = sqrt(xt[0] * xt[0] + xt[1] * xt[1]);
+        if (r_tmp < 0.0 || r_tmp >= edge_r) continue;
+        if (xt[2] < 0.0 || xt[2] >= edge_z) continue;
+        bm->x1[n] = xt[0]; bm->x2[n] = xt[1]; bm->x3[n] = xt[2];
+        bm->q[n] = q_per_part;
+        double g[4];
+        ora_normal4(c->seed, (uint64_t)i, 0x80000000u, g);
+        bm->p1[n] = c->uth[0] * g[0];
+        bm->p2[n] = c->uth[1] * g[1];
+        bm->p3[n] = c->uth[2] * g[2] + c->gamma;
+        bm->p3[n] = sqrt(bm->p3[n] * bm->p3[n] - bm->p1[n] * bm->p1[n] - bm->p2[n] * bm->p2[n] - 1.0);
+        bm->id[n] = (int32_t)i;
+        n++;
+    }
+    if (c->quiet) {
+        for (int64_t i = 0; i < n; i++) {
+            bm->x1[n + i] = -bm->x1[i]; bm->x2[n + i] = -bm->x2[i]; bm->x3[n + i] = bm->x3[i];
+            bm->p1[n + i] = -bm->p1[i]; bm->p2[n + i] = -bm->p2[i]; bm->p3[n + i] = bm->p3[i];
+            bm->q[i] = 0.5 * bm->q[i];
+            bm->q[n + i] = bm->q[i];
+            bm->id[n + i] = bm->id[i] + (int32_t)c->total_num;
+        }
+        n *= 2;
+    }
+    bm->npp = n;
+}
+
+/* qdeposit_part3d: source/beam/part3d_class.f03:221-356 into the beam's f2 */
+void ora_beam_qdeposit(ora_sim *s, int ib, double *h)
+{
+    ora_beam *bm = &s->bm[ib];
+    int nr = s->g.nr, nm = s->nm, ld = s->ld, M = s->g.max_mode, nzp = s->g.nxi;
+    double idr = 1.0 / s->g.dr, idz = 1.0 / s->g.dxi;
+    size_t sl = (size_t)nm * ld;   /* one slice of a 1-component field */
+#define HQ(k, i, z) h[(size_t)(z) * sl + (size_t)(k) * ld + (size_t)(i)]
+    for (int64_t p = 0; p < bm->npp; p++) {
+        double pr = sqrt(bm->x1[p] * bm->x1[p] + bm->x2[p] * bm->x2[p]) * idr;
+        double pz = bm->x3[p] * idz;
+        double ph0r = bm->x1[p] / pr * idr, ph0i = -bm->x2[p] / pr * idr;
+        int nn = (int)floor(pr), mm = (int)floor(pz);
+        pr = pr - (double)nn;
+        pz = pz - (double)mm;
+        nn = nn + 1; mm = mm + 1;
+        double wr[2] = {1.0 - pr, pr}, wz[2] = {1.0 - pz, pz};
+        double phr = 1.0 * bm->q[p], phi = 0.0 * bm->q[p];
+        for (int k = 0; k < 2; k++)
+            for (int j = 0; j < 2; j++)
+                HQ(0, nn + j, mm + k) = HQ(0, nn + j, mm + k) + wr[j] * wz[k] * phr;
+        for (int m = 1; m <= M; m++) {
+            double a = phr * ph0r - phi * ph0i, b = phr * ph0i + phi * ph0r;
+            phr = a; phi = b;
+            for (int k = 0; k < 2; k++)
+                for (int j = 0; j < 2; j++) {
+                    HQ(kre(m), nn + j, mm + k) = HQ(kre(m), nn + j, mm + k) + wr[j] * wz[k] * phr;
+                    HQ(kim(m), nn + j, mm + k) = HQ(kim(m), nn + j, mm + k) + wr[j] * wz[k] * phi;
+                }
+        }
+    }
+    for (int z = 1; z <= nzp; z++) {
+        HQ(0, 0, z) = 0.0;
+        HQ(0, 1, z) = 8.0 * HQ(0, 1, z);
+        for (int m = 1; m <= M; m++) {
+            HQ(kre(m), 0, z) = 0.0; HQ(kim(m), 0, z) = 0.0;
+            HQ(kre(m), 1, z) = 0.0; HQ(kim(m), 1, z) = 0.0;
+        }
+    }
+    for (int j = 2; j <= nr + 1; j++) {
+        double ir = 1.0 / (double)(j - 1);
+        for (int z = 1; z <= nzp; z++) HQ(0, j, z) = HQ(0, j, z) * ir;
+    }
+    for (int m = 1; m <= M; m++)
+        for (int j = 2; j <= nr + 1; j++) {
+            double ir = 1.0 / (double)(j - 1);
+            for (int z = 1; z <= nzp; z++) {
+                HQ(kre(m), j, z) = HQ(kre(m), j, z) * ir;
+                HQ(kim(m), j, z) = HQ(kim(m), j, z) * ir;
+            }
+        }
+#undef HQ
+}
+
+/* push_reduced_part3d + interp_emf_part3d + update_bound_part3d:
+   source/beam/part3d_class.f03:477-576, :691-790, :640-689 */
+void ora_beam_push(ora_sim *s, int ib)
+{
+    ora_beam *bm = &s->bm[ib];
+    if (!bm->evol) return;
+    int nm = s->nm, ld = s->ld, M = s->g.max_mode;
+    double idr = 1.0 / s->g.dr, idz = 1.0 / s->g.dxi;
+    double qtmh = bm->qbm * bm->dt * 0.5;
+    size_t sl = (size_t)3 * nm * ld;
+    const double *he = s->h_e, *hb = s->h_b;
+#define H3(h, c, k, i, z) (h)[(size_t)(z) * sl + ((size_t)(c) * nm + (size_t)(k)) * ld + (size_t)(i)]
+    for (int64_t p = 0; p < bm->npp; p++) {
+        double pr = sqrt(bm->x1[p] * bm->x1[p] + bm->x2[p] * bm->x2[p]) * idr;
+        double pz = bm->x3[p] * idz;
+        double cc = bm->x1[p] / pr * idr, ss = bm->x2[p] / pr * idr;
+        int nn = (int)pr, mm = (int)pz;
+        pr = pr - (double)nn;
+        pz = pz - (double)mm;
+        nn = nn + 1; mm = mm + 1;
+        double wr[2] = {1.0 - pr, pr}, wz[2] = {1.0 - pz, pz};
+        double ep[3] = {0, 0, 0}, bp[3] = {0, 0, 0};
+        for (int k = 0; k < 2; k++)
+            for (int j = 0; j < 2; j++) {
+                double wt = wr[j] * wz[k];
+                for (int c = 0; c < 3; c++) {
+                    ep[c] = ep[c] + H3(he, c, 0, nn + j, mm + k) * wt;
+                    bp[c] = bp[c] + H3(hb, c, 0, nn + j, mm + k) * wt;
+                }
+            }
+        double phr = 1.0, phi = 0.0;
+        for (int m = 1; m <= M; m++) {
+            double a = phr * cc - phi * ss, b = phr * ss + phi * cc;
+            phr = a; phi = b;
+            double ph_r = 2.0 * phr, ph_i = 2.0 * phi;
+            for (int k = 0; k < 2; k++)
+                for (int j = 0; j < 2; j++) {
+                    double wt = wr[j] * wz[k];
+                    for (int c = 0; c < 3; c++) {
+                        ep[c] = ep[c] + (H3(he, c, kre(m), nn + j, mm + k) * ph_r
+                                         - H3(he, c, kim(m), nn + j, mm + k) * ph_i) * wt;
+                        bp[c] = bp[c] + (H3(hb, c, kre(m), nn + j, mm + k) * ph_r
+                                         - H3(hb, c, kim(m), nn + j, mm + k) * ph_i) * wt;
+                    }
+                }
+        }
+        double t = ep[0] * cc - ep[1] * ss; double u = ep[0] * ss + ep[1] * cc; ep[0] = t; ep[1] = u;
+        t = bp[0] * cc - bp[1] * ss; u = bp[0] * ss + bp[1] * cc; bp[0] = t; bp[1] = u;
+        double wp[3];
+        for (int c = 0; c < 3; c++) { ep[c] = ep[c] * qtmh; bp[c] = bp[c] * qtmh; }
+        wp[0] = ep[0] - bp[1];
+        wp[1] = ep[1] + bp[0];
+        wp[2] = ep[2];
+        bm->p1[p] = bm->p1[p] + wp[0]; bm->p2[p] = bm->p2[p] + wp[1]; bm->p3[p] = bm->p3[p] + wp[2];
+        bm->p1[p] = bm->p1[p] + wp[0]; bm->p2[p] = bm->p2[p] + wp[1]; bm->p3[p] = bm->p3[p] + wp[2];
+        double dt_gam = bm->dt / sqrt(1.0 + bm->p1[p] * bm->p1[p] + bm->p2[p] * bm->p2[p]
+                                      + bm->p3[p] * bm->p3[p]);
+        bm->x1[p] = bm->x1[p] + bm->p1[p] * dt_gam;
+        bm->x2[p] = bm->x2[p] + bm->p2[p] * dt_gam;
+        bm->x3[p] = bm->x3[p] - bm->p3[p] * dt_gam + bm->dt;
+    }
+#undef H3
+    /* update_bound_part3d (swap with last) */
+    if (bm->npp == 0) return;
+    double e1 = (double)s->g.nr * s->g.dr, e2 = (double)s->g.nxi * s->g.dxi;
+#define BMOVE(dst, src) do { bm->x1[dst] = bm->x1[src]; bm->x2[dst] = bm->x2[src]; \
+        bm->x3[dst] = bm->x3[src]; bm->p1[dst] = bm->p1[src]; bm->p2[dst] = bm->p2[src]; \
+        bm->p3[dst] = bm->p3[src]; bm->q[dst] = bm->q[src]; bm->id[dst] = bm->id[src]; } while (0)
+    int64_t i = 0;
+    while (i < bm->npp - 1) {
+        double pr = sqrt(bm->x1[i] * bm->x1[i] + bm->x2[i] * bm->x2[i]);
+        if (pr >= e1 || bm->x3[i] >= e2) { BMOVE(i, bm->npp - 1); bm->npp--; continue; }
+        i++;
+    }
+    int64_t l = bm->npp - 1;
+    double pr = sqrt(bm->x1[l] * bm->x1[l] + bm->x2[l] * bm->x2[l]);
+    if (pr >= e1 || bm->x3[l] >= e2) bm->npp--;
+#undef BMOVE
+}
+
+/* ------------------------------------------------------------------------ */
+/* Simulation object                                                         */
+/* ------------------------------------------------------------------------ */
+static const int field_ncomp[ORA_NFIELD] = {1, 3, 3, 3, 3, 3, 3, 2, 3, 2, 1, 1};
+
+ora_sim *ora_create(const ora_grid *g)
+{
+    ora_sim *s = calloc(1, sizeof(ora_sim));
+    s->g = *g;
+    s->nm = 2 * g->max_mode + 1;
+    s->ld = g->nr + 2;
+    size_t n1 = (size_t)s->nm * s->ld;
+    for (int f = 0; f < ORA_NFIELD; f++) {
+        s->ncomp[f] = field_ncomp[f];
+        s->f[f] = calloc(n1 * field_ncomp[f], sizeof(double));
+    }
+    size_t ns = (size_t)g->nxi + 2;
+    s->h_psi = calloc(ns * n1, sizeof(double));
+    s->h_e = calloc(ns * n1 * 3, sizeof(double));
+    s->h_b = calloc(ns * n1 * 3, sizeof(double));
+    s->h_b_spe = calloc(ns * n1 * 3, sizeof(double));
+    s->h_e_spe = calloc(ns * n1 * 3, sizeof(double));
+    s->h_cu = calloc(ns * n1 * 3, sizeof(double));
+    s->h_q_spe = calloc(ns * n1, sizeof(double));
+    s->h_q_beam = calloc(ns * n1, sizeof(double));
+    s->conv_re = calloc(g->nr + 2, sizeof(double));
+    s->conv_im = calloc(g->nr + 2, sizeof(double));
+    int kinds[6] = {ORA_FK_PSI, ORA_FK_EZ, ORA_FK_BZ, ORA_FK_BT, ORA_FK_BPLUS, ORA_FK_BMINUS};
+    double *a = malloc(sizeof(double) * g->nr), *b = malloc(sizeof(double) * g->nr),
+           *c = malloc(sizeof(double) * g->nr);
+    for (int t = 0; t < 6; t++)
+        for (int m = 0; m <= g->max_mode; m++) {
+            ora_factor *f = &s->fac[kinds[t]][m];
+            f->lo = malloc(sizeof(double) * g->nr);
+            f->inv = malloc(sizeof(double) * g->nr);
+            f->cp = malloc(sizeof(double) * g->nr);
+            ora_matrix(kinds[t], m, g->nr, g->dr, g->relax_fac, g->bnd, a, b, c);
+            ora_thomas_factor(g->nr, a, b, c, f->lo, f->inv, f->cp);
+        }
+    free(a); free(b); free(c);
+    return s;
+}
+
+void ora_destroy(ora_sim *s)
+{
+    if (!s) return;
+    for (int f = 0; f < ORA_NFIELD; f++) free(s->f[f]);
+    free(s->h_psi); free(s->h_e); free(s->h_b); free(s->h_b_spe); free(s->h_e_spe);
+    free(s->h_cu); free(s->h_q_spe); free(s->h_q_beam);
+    free(s->conv_re); free(s->conv_im);
+    for (int t = 0; t < 6; t++)
+        for (int m = 0; m <= s->g.max_mode; m++) {
+            free(s->fac[t][m].lo); free(s->fac[t][m].inv); free(s->fac[t][m].cp);
+        }
+    for (int i = 0; i < s->nspecies; i++) {
+        ora_species *sp = &s->sp[i];
+        free(sp->x1); free(sp->x2); free(sp->p1); free(sp->p2); free(sp->p3);
+        free(sp->q); free(sp->gamma); free(sp->psi); free(sp->id);
+        free(sp->q_own); free(sp->qn); free(sp->cu_own); free(sp->acu_own); free(sp->amu_own);
+        free(sp->q_hist);
+    }
+    for (int i = 0; i < s->nbeams; i++) {
+        ora_beam *bm = &s->bm[i];
+        free(bm->x1); free(bm->x2); free(bm->x3); free(bm->p1); free(bm->p2); free(bm->p3);
+        free(bm->q); free(bm->id); free(bm->q_own_hist);
+    }
+    free(s);
+}
+
+int ora_add_species(ora_sim *s, const ora_species_cfg *cfg)
+{
+    if (s->nspecies >= 8) return -1;
+    int isp = s->nspecies++;
+    ora_species *sp = &s->sp[isp];
+    memset(sp, 0, sizeof(*sp));
+    sp->cfg = *cfg;
+    /* npmax: fdist2d_class.f03:259-262 (xtra = 2) */
+    sp->npmax = (int64_t)s->g.nr * cfg->ppc1 * cfg->ppc2 * cfg->num_theta * 2;
+    int64_t n = sp->npmax;
+    sp->x1 = calloc(n, sizeof(double)); sp->x2 = calloc(n, sizeof(double));
+    sp->p1 = calloc(n, sizeof(double)); sp->p2 = calloc(n, sizeof(double));
+    sp->p3 = calloc(n, sizeof(double)); sp->q = calloc(n, sizeof(double));
+    sp->gamma = calloc(n, sizeof(double)); sp->psi = calloc(n, sizeof(double));
+    sp->id = calloc(n, sizeof(int32_t));
+    sp->qm = cfg->qm;
+    sp->qbm = cfg->qm / cfg->mass;           /* sim_plasma_class.f03:120-122 */
+    sp->dt = s->g.dxi;
+    size_t n1 = (size_t)s->nm * s->ld;
+    sp->q_own = calloc(n1, sizeof(double));
+    sp->qn = calloc(n1, sizeof(double));
+    sp->cu_own = calloc(3 * n1, sizeof(double));
+    sp->acu_own = calloc(2 * n1, sizeof(double));
+    sp->amu_own = calloc(3 * n1, sizeof(double));
+    sp->q_hist = calloc(((size_t)s->g.nxi + 2) * n1, sizeof(double));
+    ora_species_inject(s, isp);
+    ora_species_init_q(s, isp);
+    return isp;
+}
+
+int ora_add_beam(ora_sim *s, const ora_beam_cfg *cfg, int64_t npmax)
+{
+    if (s->nbeams >= 8) return -1;
+    int ib = s->nbeams++;
+    ora_beam *bm = &s->bm[ib];
+    memset(bm, 0, sizeof(*bm));
+    if (npmax <= 0) npmax = cfg->total_num * (cfg->quiet ? 2 : 1);
+    bm->npmax = npmax;
+    int64_t n = npmax;
+    bm->x1 = calloc(n, sizeof(double)); bm->x2 = calloc(n, sizeof(double));
+    bm->x3 = calloc(n, sizeof(double)); bm->p1 = calloc(n, sizeof(double));
+    bm->p2 = calloc(n, sizeof(double)); bm->p3 = calloc(n, sizeof(double));
+    bm->q = calloc(n, sizeof(double)); bm->id = calloc(n, sizeof(int32_t));
+    bm->qbm = cfg->qm / cfg->mass;
+    bm->dt = s->g.dt;
+    bm->evol = cfg->evol;
+    bm->q_own_hist = calloc(((size_t)s->g.nxi + 2) * s->nm * s->ld, sizeof(double));
+    if (cfg->total_num > 0) ora_beam_inject(s, ib, cfg);
+    return ib;
+}
+
+int ora_beam_set(ora_sim *s, int ib, int64_t n, const double *x1, const double *x2,
+                 const double *x3, const double *p1, const double *p2,
+                 const double *p3, const double *q)
+{
+    ora_beam *bm = &s->bm[ib];
+    if (n > bm->npmax) return -1;
+    memcpy(bm->x1, x1, n * sizeof(double)); memcpy(bm->x2, x2, n * sizeof(double));
+    memcpy(bm->x3, x3, n * sizeof(double)); memcpy(bm->p1, p1, n * sizeof(double));
+    memcpy(bm->p2, p2, n * sizeof(double)); memcpy(bm->p3, p3, n * sizeof(double));
+    memcpy(bm->q, q, n * sizeof(double));
+    for (int64_t i = 0; i < n; i++) bm->id[i] = (int32_t)i;
+    bm->npp = n;
+    return 0;
+}
+
+/* 3D step prologue: source/simulation_class.f03:297-340 */
+void ora_step3d_begin(ora_sim *s)
+{
+    size_t n1 = (size_t)s->nm * s->ld, ns = (size_t)s->g.nxi + 2;
+    memset(s->h_q_beam, 0, ns * n1 * sizeof(double));   /* q_beam%as(0.0) */
+    memset(s->h_q_spe, 0, ns * n1 * sizeof(double));    /* q_spe%as(0.0)  */
+    for (int ib = 0; ib < s->nbeams; ib++) {            /* beam3d%qdp     */
+        ora_beam *bm = &s->bm[ib];
+        memset(bm->q_own_hist, 0, ns * n1 * sizeof(double));
+        ora_beam_qdeposit(s, ib, bm->q_own_hist);
+        for (size_t t = 0; t < ns * n1; t++) s->h_q_beam[t] = s->h_q_beam[t] + bm->q_own_hist[t];
+    }
+    int zero[] = {ORA_B, ORA_E, ORA_B_SPE, ORA_E_SPE, ORA_PSI, ORA_CU, ORA_ACU, ORA_AMU};
+    for (int t = 0; t < 8; t++) memset(s->f[zero[t]], 0, n1 * s->ncomp[zero[t]] * sizeof(double));
+}
+
+static void add_into(double *dst, const double *src, size_t n)
+{
+    for (size_t t = 0; t < n; t++) dst[t] = dst[t] + src[t];
+}
+
+/* One xi-slice: source/simulation_class.f03:342-469 */
+void ora_slice(ora_sim *s, int j, int *n_iter)
+{
+    int nm = s->nm, ld = s->ld;
+    size_t n1 = (size_t)nm * ld;
+    double *psi = s->f[ORA_PSI], *e = s->f[ORA_E], *e_spe = s->f[ORA_E_SPE];
+    double *b = s->f[ORA_B], *b_spe = s->f[ORA_B_SPE], *b_beam = s->f[ORA_B_BEAM];
+    double *cu = s->f[ORA_CU], *acu = s->f[ORA_ACU], *amu = s->f[ORA_AMU];
+    double *dcu = s->f[ORA_DCU], *q_spe = s->f[ORA_Q_SPE], *q_beam = s->f[ORA_Q_BEAM];
+
+    memcpy(q_beam, s->h_q_beam + (size_t)j * n1, n1 * sizeof(double));  /* copy_slice 2to1 */
+    ora_solve_bt_beam(s, q_beam, b_beam);
+    memset(q_spe, 0, n1 * sizeof(double));
+    for (int k = 0; k < s->nspecies; k++) {                              /* species2d%qdp */
+        ora_species *sp = &s->sp[k];
+        memset(sp->q_own, 0, n1 * sizeof(double));
+        ora_qdeposit(s, sp, sp->q_own);
+        add_into(q_spe, sp->q_own, n1);
+        if (sp->cfg.neutralized) add_into(q_spe, sp->qn, n1);
+    }
+    ora_solve_psi(s, q_spe, psi);
+    ora_solve_bz(s, cu, b_spe);
+
+    int l, nit = 0;
+    double rel = 0.0, ab = 0.0;
+    for (l = 1; l <= s->g.iter_max; l++) {
+        nit = l;
+        ora_conv_record(s, b_spe);
+        for (size_t t = 0; t < 3 * n1; t++) b[t] = b_spe[t] + b_beam[t];
+        ora_solve_ez(s, cu, e);
+        ora_solve_et(s, b, psi, e);
+        memset(cu, 0, 3 * n1 * sizeof(double));
+        memset(acu, 0, 2 * n1 * sizeof(double));
+        memset(amu, 0, 3 * n1 * sizeof(double));
+        for (int k = 0; k < s->nspecies; k++) {                          /* species2d%amjdp */
+            ora_species *sp = &s->sp[k];
+            memset(sp->cu_own, 0, 3 * n1 * sizeof(double));
+            memset(sp->acu_own, 0, 2 * n1 * sizeof(double));
+            memset(sp->amu_own, 0, 3 * n1 * sizeof(double));
+            ora_amjdeposit_robust(s, sp, e, b, sp->cu_own, sp->amu_own, sp->acu_own, s->g.dxi);
+            add_into(cu, sp->cu_own, 3 * n1);
+            add_into(acu, sp->acu_own, 2 * n1);
+            add_into(amu, sp->amu_own, 3 * n1);
+        }
+        ora_solve_djdxi(s, acu, amu, dcu);
+        ora_solve_bt_iter(s, dcu, cu, b_spe);
+        ora_solve_bz(s, cu, b_spe);
+        ora_conv_compare(s, b_spe, &rel, &ab);
+        if (rel < s->g.iter_reltol || ab < s->g.iter_abstol) break;
+    }
+    s->last_iters = nit;
+    s->last_rel = rel;
+    s->last_abs = ab;
+    if (n_iter) *n_iter = nit;
+
+    for (int k = 0; k < s->nspecies; k++) {                              /* species2d%cbq */
+        ora_species *sp = &s->sp[k];
+        for (size_t t = 0; t < n1; t++) sp->q_own[t] = sp->q_own[t] + sp->cu_own[2 * n1 + t];
+        memcpy(sp->q_hist + (size_t)j * n1, sp->q_own, n1 * sizeof(double));
+    }
+    memcpy(s->h_cu + (size_t)j * 3 * n1, cu, 3 * n1 * sizeof(double));
+    for (size_t t = 0; t < n1; t++) q_spe[t] = q_spe[t] + cu[2 * n1 + t];
+    memcpy(s->h_q_spe + (size_t)j * n1, q_spe, n1 * sizeof(double));
+
+    for (size_t t = 0; t < 3 * n1; t++) b[t] = b_spe[t] + b_beam[t];
+    ora_solve_et(s, b_spe, psi, e_spe);
+    ora_solve_ez(s, cu, e);
+    ora_solve_et(s, b, psi, e);
+
+    for (size_t t = 0; t < 2 * n1; t++) dcu[t] = s->g.dxi * dcu[t];      /* dot_f1 */
+    for (size_t t = 0; t < 2 * n1; t++) cu[t] = cu[t] + dcu[t];          /* J_perp predictor */
+
+    for (int k = 0; k < s->nspecies; k++) {
+        ora_species *sp = &s->sp[k];
+        ora_push_u_robust(s, sp, e, b, s->g.dxi);
+        ora_push_x(sp, s->g.dxi);
+        ora_update_bound(s, sp);
+    }
+    memcpy(s->h_e + (size_t)j * 3 * n1, e, 3 * n1 * sizeof(double));
+    memcpy(s->h_b + (size_t)j * 3 * n1, b, 3 * n1 * sizeof(double));
+    memcpy(s->h_psi + (size_t)j * n1, psi, n1 * sizeof(double));
+    memcpy(s->h_b_spe + (size_t)j * 3 * n1, b_spe, 3 * n1 * sizeof(double));
+    memcpy(s->h_e_spe + (size_t)j * 3 * n1, e_spe, 3 * n1 * sizeof(double));
+}
+
+/* 3D step epilogue: source/simulation_class.f03:485-501 (beam push, species
+   renew) */
+void ora_step3d_end(ora_sim *s, int istep)
+{
+    (void)istep;
+    for (int ib = 0; ib < s->nbeams; ib++) ora_beam_push(s, ib);
+    for (int k = 0; k < s->nspecies; k++) {
+        ora_species_inject(s, k);
+        ora_species_init_q(s, k);
+    }
+}
+
+void ora_run(ora_sim *s, int nstep3d, int nslices)
+{
+    if (nslices <= 0 || nslices > s->g.nxi) nslices = s->g.nxi;
+    for (int i = 1; i <= nstep3d; i++) {
+        ora_step3d_begin(s);
+        for (int j = 1; j <= nslices; j++) ora_slice(s, j, NULL);
+        ora_step3d_end(s, i);
+    }
+}
+
+double *ora_field(ora_sim *s, int fid) { return (fid >= 0 && fid < ORA_NFIELD) ? s->f[fid] : NULL; }
+
+double *ora_hist(ora_sim *s, int fid)
+{
+    switch (fid) {
+    case ORA_PSI: return s->h_psi;
+    case ORA_E: return s->h_e;
+    case ORA_B: return s->h_b;
+    case ORA_B_SPE: return s->h_b_spe;
+    case ORA_E_SPE: return s->h_e_spe;
+    case ORA_CU: return s->h_cu;
+    case ORA_Q_SPE: return s->h_q_spe;
+    case ORA_Q_BEAM: return s->h_q_beam;
+    default: return NULL;
+    }
+}
+
+int ora_nm(const ora_sim *s) { return s->nm; }
+int ora_ld(const ora_sim *s) { return s->ld; }
+ora_species *ora_get_species(ora_sim *s, int isp) { return &s->sp[isp]; }
+ora_beam *ora_get_beam(ora_sim *s, int ib) { return &s->bm[ib]; }
+int ora_last_iters(const ora_sim *s) { return s->last_iters; }

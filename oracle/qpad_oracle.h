@@ -1,0 +1,191 @@
+/*
+ * qpad_oracle.h -- serial FP64 CPU restatement of QPAD's per-xi-slice plasma
+ * response and once-per-3D-step beam push.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This code is the correctness oracle (and the
+ * timed CPU baseline) for the HIP library in qpad_amd/csrc.  Only tests/,
+ * __graft_entry__.smoke() and bench.py's cpu_baseline leg may link or call
+ * it, and only as the checker.  The product never routes through it.
+ *
+ * Every routine cites the reference file:line it restates (paths relative to
+ * the QPAD source tree).  Data layout (shared with the HIP library so the two
+ * can be compared array-for-array):
+ *   slice field  f[(c*nm + k)*ld + i]   c = component, k = mode array,
+ *                                        i = radial node 0..nr+1 (0 and nr+1
+ *                                        are the guard nodes of ufield%f1)
+ *   mode arrays  k = 0 : Re m=0 ;  k = 2m-1 : Re m ;  k = 2m : Im m
+ *   history      h[(s*ncomp + c)*nm + k)*ld + i], slice s = 0..nxi+1
+ *                (s = 1..nxi are QPAD's f2 slices, s = nxi+1 is the upper
+ *                 guard slice, s = 0 is unused)
+ *
+ * Parity pins (see the tests): closed forms, manufactured solutions,
+ * invariants and dense solves.  HYPRE StructCycRed (the reference's
+ * tridiagonal solver, third-party, unversioned) is replaced by Thomas'
+ * algorithm on the same matrices: parity unpinned against HYPRE's rounding.
+ * gfortran's random_number stream is replaced by Philox4x32-10: thermal
+ * plasma (uth != 0) and random beams are parity unpinned against QPAD's own
+ * draws; beams are shared with the GPU side as particle arrays instead.
+ */
+#ifndef QPAD_ORACLE_H
+#define QPAD_ORACLE_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* field kinds of field_solver (source/param.f03:72-80) */
+enum { ORA_FK_PSI = 0, ORA_FK_EZ = 1, ORA_FK_BZ = 2, ORA_FK_BT = 3,
+       ORA_FK_BPLUS = 4, ORA_FK_BMINUS = 5 };
+/* boundary conditions (source/param.f03:69-71) */
+enum { ORA_BND_ZERO = 2, ORA_BND_OPEN = 3 };
+
+/* field ids of the slice state (source/sim_fields_class.f03:147-160) */
+enum { ORA_PSI = 0, ORA_E, ORA_E_SPE, ORA_B, ORA_B_SPE, ORA_B_BEAM,
+       ORA_CU, ORA_ACU, ORA_AMU, ORA_DCU, ORA_Q_SPE, ORA_Q_BEAM, ORA_NFIELD };
+
+typedef struct {
+    int nr, nxi, max_mode;     /* grid(1), grid(2), simulation.max_mode     */
+    double dr, dxi;            /* options_class.f03:89,93                   */
+    double relax_fac;          /* sim_fields_class.f03:137-140              */
+    int bnd;                   /* ORA_BND_OPEN / ORA_BND_ZERO               */
+    int iter_max;              /* simulation.iter_max                       */
+    double iter_reltol, iter_abstol;
+    double dt;                 /* 3D time step (beam push)                  */
+} ora_grid;
+
+typedef struct {
+    int ppc1, ppc2, num_theta; /* species(i).ppc, num_theta                 */
+    double qm, mass;           /* species(i).q, species(i).m                */
+    double density;
+    double uth[3];
+    int neutralized;           /* species(i).neutralized (default true)     */
+    double den_min;            /* species(i).den_min (default 1e-10)        */
+    uint64_t seed;             /* Philox key for thermal draws (uth != 0)   */
+} ora_species_cfg;
+
+typedef struct {
+    int64_t total_num;         /* beam(i).total_num                         */
+    double total_charge;       /* beam(i).total_charge                      */
+    double qm, mass, gamma;
+    double center[3], sigma[3];/* gauss_center, gauss_sigma (xi relative to
+                                  the box start, i.e. already minus z0)     */
+    double range_lo[3], range_hi[3];
+    double uth[3];
+    int quiet;                 /* quiet_start                               */
+    int evol;                  /* evolution                                 */
+    uint64_t seed;
+} ora_beam_cfg;
+
+typedef struct {
+    int64_t npp, npmax;
+    double *x1, *x2, *p1, *p2, *p3, *q, *gamma, *psi;
+    int32_t *id;
+    double qbm, qm, dt;
+    ora_species_cfg cfg;
+    double *q_own, *qn;       /* species%q, species%qn slice arrays        */
+    double *cu_own, *acu_own, *amu_own;
+    double *q_hist;           /* species q history (cbq)                   */
+} ora_species;
+
+typedef struct {
+    int64_t npp, npmax;
+    double *x1, *x2, *x3, *p1, *p2, *p3, *q;
+    int32_t *id;
+    double qbm, dt;
+    int evol;
+    double *q_own_hist;       /* beam3d%q f2                               */
+} ora_beam;
+
+typedef struct {
+    double *lo, *inv, *cp;    /* Thomas factors, n = nr rows               */
+} ora_factor;
+
+typedef struct {
+    ora_grid g;
+    int nm, ld;
+    double *f[ORA_NFIELD];    /* slice arrays                              */
+    int ncomp[ORA_NFIELD];
+    double *h_psi, *h_e, *h_b, *h_b_spe, *h_e_spe, *h_cu, *h_q_spe, *h_q_beam;
+    int nspecies, nbeams;
+    ora_species sp[8];
+    ora_beam bm[8];
+    /* factors[kind][mode] */
+    ora_factor fac[6][16];
+    /* convergence_tester saved vectors (simulation_class.f03:534) */
+    double *conv_re, *conv_im;
+    /* diagnostics of the last slice */
+    int last_iters;
+    double last_rel, last_abs;
+} ora_sim;
+
+/* ---- low-level building blocks (exported for unit tests) ---- */
+void ora_matrix(int kind, int mode, int nr, double dr, double relax_fac,
+                int bnd, double *a, double *b, double *c);
+void ora_thomas_factor(int n, const double *a, const double *b,
+                       const double *c, double *lo, double *inv, double *cp);
+void ora_thomas_solve(int n, const double *lo, const double *inv,
+                      const double *cp, double *d);
+
+void ora_solve_psi(const ora_sim *s, const double *q, double *psi);
+void ora_solve_ez(const ora_sim *s, const double *jay, double *e);
+void ora_solve_et(const ora_sim *s, const double *b, const double *psi,
+                  double *e);
+void ora_solve_bz(const ora_sim *s, const double *jay, double *b);
+void ora_solve_bt_beam(const ora_sim *s, const double *q, double *b);
+void ora_solve_bt_iter(const ora_sim *s, const double *djdxi,
+                       const double *jay, double *b);
+void ora_solve_djdxi(const ora_sim *s, const double *acu, const double *amu,
+                     double *dcu);
+void ora_conv_record(ora_sim *s, const double *b);
+void ora_conv_compare(ora_sim *s, const double *b, double *rel, double *abs_);
+
+void ora_qdeposit(const ora_sim *s, const ora_species *sp, double *q);
+void ora_amjdeposit_robust(const ora_sim *s, ora_species *sp,
+                           const double *e, const double *b, double *cu,
+                           double *amu, double *dcu, double dt);
+void ora_push_u_robust(const ora_sim *s, ora_species *sp, const double *e,
+                       const double *b, double dt);
+void ora_push_x(ora_species *sp, double dt);
+void ora_update_bound(const ora_sim *s, ora_species *sp);
+void ora_interp_point(const ora_sim *s, const double *f, int ncomp,
+                      double x1, double x2, double *out);
+
+/* ---- simulation object ---- */
+ora_sim *ora_create(const ora_grid *g);
+void ora_destroy(ora_sim *s);
+int ora_add_species(ora_sim *s, const ora_species_cfg *cfg);
+int ora_add_beam(ora_sim *s, const ora_beam_cfg *cfg, int64_t npmax);
+int ora_beam_set(ora_sim *s, int ib, int64_t n, const double *x1,
+                 const double *x2, const double *x3, const double *p1,
+                 const double *p2, const double *p3, const double *q);
+void ora_species_inject(ora_sim *s, int isp);
+void ora_species_init_q(ora_sim *s, int isp);
+void ora_beam_inject(ora_sim *s, int ib, const ora_beam_cfg *cfg);
+void ora_beam_qdeposit(ora_sim *s, int ib, double *q_hist);
+void ora_beam_push(ora_sim *s, int ib);
+
+void ora_step3d_begin(ora_sim *s);
+void ora_slice(ora_sim *s, int j, int *n_iter);
+void ora_step3d_end(ora_sim *s, int istep);
+void ora_run(ora_sim *s, int nstep3d, int nslices);
+
+double *ora_field(ora_sim *s, int fid);
+double *ora_hist(ora_sim *s, int fid);
+int ora_nm(const ora_sim *s);
+int ora_ld(const ora_sim *s);
+ora_species *ora_get_species(ora_sim *s, int isp);
+ora_beam *ora_get_beam(ora_sim *s, int ib);
+int ora_last_iters(const ora_sim *s);
+
+/* Philox4x32-10 (Salmon et al., SC'11) and the Gaussian draw built on it */
+void ora_philox4x32(const uint32_t ctr[4], const uint32_t key[2],
+                    uint32_t out[4]);
+void ora_normal4(uint64_t seed, uint64_t stream, uint32_t sub,
+                 double out[4]);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
