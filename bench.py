@@ -1,0 +1,222 @@
+"""Benchmark: QPAD per-xi-slice plasma response + beam push on one MI355X.
+
+python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload c2|c1|c3]
+                [--cpu-slices S] [--no-cpu]
+
+A step is one 3D step of the workload: every xi-slice of the plasma response
+(with every B_perp predictor-corrector iteration), the beam deposit, the beam
+B of all slices and the beam push (include/qpad_hip.h: qpad_step3d).  Inputs
+are synthetic (lattice plasma, Philox-drawn Gaussian beam) and resident in
+HBM before the timed region.  Prints one JSON line.
+"""
+import argparse
+import json
+import multiprocessing as mp
+import os
+import sys
+import tempfile
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0   # MI355X published HBM3E bandwidth
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--workload", default="c2", choices=["c1", "c2", "c3"])
+    ap.add_argument("--cpu-slices", type=int, default=24)
+    ap.add_argument("--cpu-start", type=int, default=100)
+    ap.add_argument("--no-cpu", action="store_true")
+    return ap.parse_args()
+
+
+def deck_for(name):
+    from qpad_amd import configs
+    return {"c1": configs.c1, "c2": configs.c2, "c3": configs.c3}[name]()
+
+
+def algorithmic_bytes_per_step(grid, np_per_slice, iters, nbeam):
+    """Algorithmic HBM bytes of one 3D step (FP64, robust pusher, fused
+    minimum): per particle-slice 88 + 48 N_it, per slice of field history
+    8 Nm (Nr+2) 18, beam Nb 136 + 8 Nm (Nr+2)(Nxi+1) 7."""
+    nm = 2 * grid.max_mode + 1
+    b = 0.0
+    for it in iters:
+        b += np_per_slice * (88 + 48 * it) + 8 * nm * (grid.nr + 2) * 18
+    b += nbeam * 136 + 8 * nm * (grid.nr + 2) * (grid.nxi + 1) * 7
+    return b
+
+
+# ---------------------------------------------------------------------------
+# CPU baseline: the oracle (serial FP64 C restatement) on a window of slices
+# replayed from a GPU snapshot, one process per core
+# ---------------------------------------------------------------------------
+def _cpu_worker(args):
+    snap_path, nslices, out_q = args
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import pyoracle as po
+    z = np.load(snap_path, allow_pickle=False)
+    g = po.Grid(*[z["grid_" + f][()] for f, _ in po.Grid._fields_])
+    o = po.Oracle(g)
+    nsp = int(z["nspecies"])
+    for k in range(nsp):
+        c = z[f"sp{k}_cfg"]
+        o.add_species(po.species_cfg(int(c[0]), int(c[1]), int(c[2]), c[3], c[4], c[5],
+                                     (c[6], c[7], c[8]), bool(c[9]), c[10], 0))
+        o.set_species(k, {n: z[f"sp{k}_{n}"] for n in ("x1", "x2", "p1", "p2", "p3", "q",
+                                                        "gamma", "psi", "id")})
+        o.species_q_own(k)[1][...] = z[f"sp{k}_qn"]
+    o.field(po.CU)[...] = z["cu"]
+    o.field(po.B_SPE)[...] = z["b_spe"]
+    j0 = int(z["j0"])
+    o.hist(po.Q_BEAM)[j0:j0 + nslices] = z["q_beam"][:nslices]
+    t0 = time.perf_counter()
+    its = [o.slice(j0 + s) for s in range(nslices)]
+    dt = time.perf_counter() - t0
+    out_q.put((dt, its))
+
+
+def cpu_baseline(sim, per_slice_iters, j0, nslices, cores):
+    from qpad_amd import _lib as L
+    g = sim.grid
+    snap = {"j0": j0, "nspecies": len(sim.spe)}
+    for f, _ in L.Grid._fields_:
+        snap["grid_" + f] = getattr(g, f)
+    for k, sp in enumerate(sim.spe):
+        c = sp.cfg
+        snap[f"sp{k}_cfg"] = np.array([c.ppc1, c.ppc2, c.num_theta, c.qm, c.mass, c.density,
+                                       c.uth[0], c.uth[1], c.uth[2], c.neutralized, c.den_min])
+        for n, v in sp.get().items():
+            snap[f"sp{k}_{n}"] = v
+        snap[f"sp{k}_qn"] = sp.qn()
+    snap["cu"] = sim.fields["cu"].get()
+    snap["b_spe"] = sim.fields["b_spe"].get()
+    snap["q_beam"] = np.stack([sim.fields["q_beam"].hist(j0 + s) for s in range(nslices)])
+    fd, path = tempfile.mkstemp(suffix=".npz", dir=os.environ.get("TMPDIR", "/tmp"))
+    os.close(fd)
+    np.savez(path, **snap)
+    ctx = mp.get_context("spawn")
+    results = {}
+    try:
+        for ncore in sorted(set([1, cores])):
+            q = ctx.Queue()
+            procs = [ctx.Process(target=_cpu_worker, args=((path, nslices, q),)) for _ in range(ncore)]
+            for p in procs:
+                p.start()
+            out = [q.get() for _ in procs]
+            for p in procs:
+                p.join()
+            results[ncore] = out
+    finally:
+        os.unlink(path)
+    # cost model: particle passes per slice = 2 + N_it (qdeposit, N_it
+    # deposits, push); scale the sample to the full 3D step's iteration profile
+    dt1, its1 = results[1][0]
+    passes_sample = sum(2 + i for i in its1)
+    passes_step = sum(2 + i for i in per_slice_iters)
+    t_step_1 = dt1 / passes_sample * passes_step
+    per_core = [dt / sum(2 + i for i in its) * passes_step for dt, its in results[cores]]
+    agg = sum(1.0 / t for t in per_core)   # 3D steps/s of `cores` concurrent instances
+    np_slice = sum(sp.count() for sp in sim.spe)
+    return {
+        "value": agg, "unit": "3D steps/s", "cores": cores, "kind": "port",
+        "sample": (f"oracle (serial FP64 C, gcc -O2) replaying slices {j0}..{j0 + nslices - 1} of "
+                   f"the workload from a GPU snapshot, {cores} concurrent instances; scaled to a "
+                   f"3D step by particle passes (2 + N_it per slice)"),
+        "one_core_3d_steps_per_s": 1.0 / t_step_1,
+        "one_core_particle_slice_pushes_per_s": np_slice * nslices / dt1,
+        "sample_iters": its1,
+    }
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build targets one MI355X: multi-GPU runs are not "
+                                   "part of it (use --gpus 1)", "n_gpus": args.gpus}))
+        sys.exit(2)
+    from qpad_amd import Simulation
+    deck = deck_for(args.workload)
+    sim = Simulation(deck)
+    g = sim.grid
+    np_slice = sum(sp.count() for sp in sim.spe)
+    nbeam = sum(b.count() for b in sim.beam)
+    for _ in range(args.warmup):
+        sim.step3d()
+    # timed region: K whole 3D steps, HIP events around the dominant kernel
+    sim.kernel_time_reset(True)
+    t0 = time.perf_counter()
+    total_iters = 0
+    for _ in range(args.steps):
+        total_iters += sim.step3d()
+    t1 = time.perf_counter()
+    dep_ms, dep_n = sim.kernel_time(0)
+    push_ms, push_n = sim.kernel_time(1)
+    sol_ms, sol_n = sim.kernel_time(2)
+    sim.kernel_time_reset(False)
+    secs = t1 - t0
+    steps_per_s = args.steps / secs
+    iters_per_slice = total_iters / (args.steps * g.nxi)
+    # per-slice iteration profile (one instrumented, untimed step)
+    sim.step3d_begin()
+    per_slice = [sim.slice(j) for j in range(1, g.nxi + 1)]
+    sim.step3d_end()
+    j0 = max(1, min(args.cpu_start, g.nxi - args.cpu_slices + 1))
+    # deposit kernel roofline: 48 B read (x1, x2, p1, p2, p3, q) + 16 B written
+    # (gamma, psi) per plasma particle per launch
+    dep_avg_s = dep_ms / max(dep_n, 1) / 1e3
+    dep_bytes = 64.0 * np_slice
+    achieved = dep_bytes / dep_avg_s / 1e9 if dep_avg_s > 0 else 0.0
+    step_bytes = algorithmic_bytes_per_step(g, np_slice, per_slice, nbeam)
+    result = {
+        "metric": "3D steps/s & plasma particle-slice pushes/s; learned-seed vs plain, same tol",
+        "value": steps_per_s,
+        "unit": "3D steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": 1e3 * secs / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (lattice plasma, Philox Gaussian beam)",
+        "config": {"workload": f"{args.workload.upper()}: Nr={g.nr} Nxi={g.nxi} m=0..{g.max_mode} "
+                               f"plasma {np_slice}/slice beam {nbeam}, plain seed, tol "
+                               f"{g.iter_reltol:g}",
+                   "nr": g.nr, "nxi": g.nxi, "max_mode": g.max_mode,
+                   "plasma_particles_per_slice": np_slice, "beam_particles": nbeam,
+                   "iter_tol": g.iter_reltol, "seed": "plain"},
+        "particle_slice_pushes_per_s": np_slice * g.nxi * steps_per_s,
+        "iters_per_slice": iters_per_slice,
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "kernel": "k_amjdeposit (gather + robust predictor + J/dJ/momentum-flux deposit)",
+                     "avg_launch_us": dep_avg_s * 1e6, "launches": dep_n,
+                     "bytes_per_launch": dep_bytes},
+        "step_roofline": {"bytes_per_step": step_bytes,
+                          "achieved_GBs": step_bytes * steps_per_s / 1e9,
+                          "frac": step_bytes * steps_per_s / 1e9 / HBM_PEAK_GBS},
+        "kernel_ms_per_step": {"amjdeposit": dep_ms / args.steps, "push": push_ms / args.steps,
+                               "solves": sol_ms / args.steps},
+    }
+    if not args.no_cpu:
+        # replay window: re-run the step up to j0 - 1 and snapshot there
+        sim.step3d_begin()
+        for j in range(1, j0):
+            sim.slice(j)
+        cores = min(os.cpu_count() or 1, 16)
+        result["cpu_baseline"] = cpu_baseline(sim, per_slice, j0, args.cpu_slices, cores)
+    sim.close()
+    print(json.dumps(result))
+
+
+if __name__ == "__main__":
+    main()

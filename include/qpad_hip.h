@@ -1,0 +1,216 @@
+/*
+ * qpad_hip.h -- C ABI of libqpad_hip.so, the MI355X (gfx950) implementation of
+ * QPAD's per-xi-slice plasma response and per-3D-step beam push.
+ *
+ * The entry points are the type-bound procedures QPAD's simulation loop
+ * calls (source/simulation_class.f03:226-520), flattened to plain C so that a
+ * Fortran ISO_C_BINDING module (qpad_amd/fortran/qpad_hip_mod.f03) can bind
+ * them one-for-one.  Each declaration cites the reference procedure it
+ * replaces (paths relative to the QPAD source tree).
+ *
+ * Conventions
+ *  - every function returns int: 0 = success, < 0 = error (see QPAD_E_*);
+ *    qpad_last_error() returns a message for the calling thread.  The
+ *    library never aborts and never prints.
+ *  - all particle and field state lives in device memory inside the context;
+ *    host arrays cross only in the qpad_get_* / qpad_set_* calls.
+ *  - slice field arrays cross as [ncomp][2*max_mode+1][nr+2] doubles: mode
+ *    array k = 0 is Re m=0, k = 2m-1 is Re m, k = 2m is Im m; node i = 0..nr+1
+ *    is QPAD's ufield%f1(:, i) (0 and nr+1 are guard nodes).
+ *  - slice indices j are 1-based like QPAD's (1..nxi); history slice nxi+1 is
+ *    the upper guard slice of ufield%f2.
+ *  - one context = one simulation instance on the current HIP device; calls on
+ *    one context are serialised on its stream and return when the work is
+ *    done (synchronous at the boundary, like the Fortran procedures).
+ */
+#ifndef QPAD_HIP_H
+#define QPAD_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define QPAD_HIP_ABI_VERSION 1
+
+/* error codes */
+#define QPAD_OK 0
+#define QPAD_E_INVALID (-1)   /* bad argument / unsupported configuration   */
+#define QPAD_E_HIP (-2)       /* a HIP runtime call failed                  */
+#define QPAD_E_NOMEM (-3)     /* device allocation failed                   */
+#define QPAD_E_STATE (-4)     /* call out of sequence                       */
+
+/* field ids (source/sim_fields_class.f03:147-160) */
+#define QPAD_F_PSI 0
+#define QPAD_F_E 1
+#define QPAD_F_E_SPE 2
+#define QPAD_F_B 3
+#define QPAD_F_B_SPE 4
+#define QPAD_F_B_BEAM 5
+#define QPAD_F_CU 6
+#define QPAD_F_ACU 7
+#define QPAD_F_AMU 8
+#define QPAD_F_DCU 9
+#define QPAD_F_Q_SPE 10
+#define QPAD_F_Q_BEAM 11
+#define QPAD_NFIELD 12
+
+/* boundary conditions (source/param.f03:69-71) */
+#define QPAD_BND_ZERO 2
+#define QPAD_BND_OPEN 3
+
+/* plasma pusher (source/param.f03:63-64); only "robust" is on this path */
+#define QPAD_PUSH2_ROBUST 1
+
+/* grid and iteration settings: options_class.f03:44-117 (nr, nxi, dr, dxi),
+   simulation_class.f03:141-172 (iter_max/reltol/abstol, dt),
+   sim_fields_class.f03:112-140 (field_boundary, relax_fac) */
+typedef struct {
+    int32_t nr, nxi, max_mode;
+    double dr, dxi;
+    double relax_fac;
+    int32_t bnd;
+    int32_t iter_max;
+    double iter_reltol, iter_abstol;
+    double dt;
+} qpad_grid;
+
+/* species(i) section: fdist2d_class.f03:117-272 (uniform profiles only) and
+   sim_plasma_class.f03:116-157 (q, m, push_type) */
+typedef struct {
+    int32_t ppc1, ppc2, num_theta;
+    double qm, mass, density;
+    double uth[3];
+    int32_t neutralized;
+    double den_min;
+    uint64_t seed;
+    int32_t push_type;
+} qpad_species_cfg;
+
+/* beam(i) section for profile_type "random", Cartesian Gaussian profile:
+   fdist3d_rnd_class.f03:104-356; xi positions are relative to the box start */
+typedef struct {
+    int64_t total_num;
+    double total_charge;
+    double qm, mass, gamma;
+    double center[3], sigma[3];
+    double range_lo[3], range_hi[3];
+    double uth[3];
+    int32_t quiet;
+    int32_t evol;
+    uint64_t seed;
+} qpad_beam_cfg;
+
+typedef struct qpad_ctx qpad_ctx;
+
+const char *qpad_last_error(void);
+int qpad_abi_version(void);
+
+/* simulation%alloc + %new without species/beams (simulation_class.f03:77-199,
+   sim_fields_class.f03:91-160: fields, solver matrices) */
+int qpad_create(const qpad_grid *grid, qpad_ctx **out);
+/* simulation%del (simulation_class.f03:200-224) */
+int qpad_destroy(qpad_ctx *ctx);
+
+/* species2d%new: inject, initial charge, neutralising background
+   (species2d_class.f03:75-134, part2d_class.f03:108-142,
+   fdist2d_class.f03:289-357) */
+int qpad_species_new(qpad_ctx *ctx, const qpad_species_cfg *cfg, int32_t *sp_id);
+/* species2d%renew (species2d_class.f03:154-184) */
+int qpad_species_renew(qpad_ctx *ctx, int32_t sp);
+/* species2d%qdp: deposit rho - Jz of species sp and add it (plus the
+   background) into q_spe (species2d_class.f03:186-208,
+   part2d_class.f03:231-359).  The caller zeroes q_spe first, as QPAD does. */
+int qpad_species_qdp(qpad_ctx *ctx, int32_t sp);
+/* species2d%amjdp: deposit cu, acu (dJ/dxi part) and amu of species sp into
+   the totals using fields e and b (species2d_class.f03:233-280,
+   part2d_class.f03:746-1010, robust pusher) */
+int qpad_species_amjdp(qpad_ctx *ctx, int32_t sp, double dxi);
+/* species2d%push_u + %push_x (+ update_bound, move_part2d_comm):
+   part2d_class.f03:1879-1965, :2221-2262, :2307-2353 */
+int qpad_species_push(qpad_ctx *ctx, int32_t sp, double dxi);
+
+/* field_psi%solve(q_spe) (field_psi_class.f03:218-254) */
+int qpad_field_solve_psi(qpad_ctx *ctx);
+/* field_e%solve(jay): Ez of field `dst` from current `jay`
+   (field_e_class.f03:148-334) */
+int qpad_field_solve_ez(qpad_ctx *ctx, int32_t dst, int32_t jay);
+/* field_e%solve(b, psi): E_perp of `dst` from `b` and psi
+   (field_e_class.f03:412-514) */
+int qpad_field_solve_et(qpad_ctx *ctx, int32_t dst, int32_t b);
+/* field_b%solve(jay): Bz of `dst` from `jay` (field_b_class.f03:760-796) */
+int qpad_field_solve_bz(qpad_ctx *ctx, int32_t dst, int32_t jay);
+/* field_b%solve(rho): beam B_perp of slice j from the q_beam history
+   (field_b_class.f03:798-834, with q_beam%copy_slice(j, p_copy_2to1)) */
+int qpad_field_solve_bt_beam(qpad_ctx *ctx, int32_t j);
+/* field_djdxi%solve(acu, amu) (field_src_class.f03:273-405) */
+int qpad_field_solve_djdxi(qpad_ctx *ctx);
+/* field_b%solve(djdxi, jay): plasma B_perp predictor-corrector solve into
+   b_spe (field_b_class.f03:836-879) */
+int qpad_field_solve_bt_iter(qpad_ctx *ctx);
+/* add_f1(b_spe, b_beam, b) (ufield_class.f03:817-881) */
+int qpad_field_add_b(qpad_ctx *ctx);
+/* convergence_tester(b_spe, 2, op) (simulation_class.f03:522-606);
+   op 0 = 'record', 1 = 'compare' (rel/abs returned for 'compare') */
+int qpad_convergence_tester(qpad_ctx *ctx, int32_t op, double *rel_res, double *abs_res);
+
+/* beam3d%qdp for all beams into the q_beam history (beam3d_class.f03:193-221,
+   part3d_class.f03:221-356) */
+int qpad_beam_qdp(qpad_ctx *ctx);
+/* beam3d%push (reduced pusher) for all beams, with update_bound
+   (beam3d_class.f03:223-254, part3d_class.f03:477-576, :640-790) */
+int qpad_beam_push(qpad_ctx *ctx);
+/* beam3d%new with a random Gaussian profile (beam3d_class.f03:121-177,
+   fdist3d_rnd_class.f03:360-513); draws from Philox4x32-10 */
+int qpad_beam_new(qpad_ctx *ctx, const qpad_beam_cfg *cfg, int32_t *beam_id);
+
+/* The fused forms of simulation%run (simulation_class.f03:294-512):
+   qpad_step3d_begin = beam deposit, beam B for every slice, field zeroing
+   (:297-340); qpad_slice = one xi-slice (:342-469) with every predictor-
+   corrector iteration (iters_out = iterations executed); qpad_step3d_end =
+   beam push and species renew (:485-501); qpad_step3d = all three. */
+int qpad_step3d_begin(qpad_ctx *ctx);
+int qpad_slice(qpad_ctx *ctx, int32_t j, int32_t *iters_out);
+int qpad_step3d_end(qpad_ctx *ctx);
+int qpad_step3d(qpad_ctx *ctx, int32_t *total_iters_out);
+
+/* state exchange with the host (diagnostics, restart, parity checks) */
+int qpad_get_field(qpad_ctx *ctx, int32_t fid, double *out);
+int qpad_set_field(qpad_ctx *ctx, int32_t fid, const double *in);
+/* history slice s (0..nxi+1) of psi, e, b, b_spe, e_spe, cu, q_spe, q_beam
+   (and b_beam, whose third component is always zero) */
+int qpad_get_hist(qpad_ctx *ctx, int32_t fid, int32_t s, double *out);
+int qpad_set_hist(qpad_ctx *ctx, int32_t fid, int32_t s, const double *in);
+int qpad_species_count(qpad_ctx *ctx, int32_t sp, int64_t *npp);
+/* any pointer may be NULL to skip that array; arrays hold npp entries in the
+   library's (radial-cell sorted) order */
+int qpad_get_species(qpad_ctx *ctx, int32_t sp, double *x1, double *x2, double *p1,
+                     double *p2, double *p3, double *q, double *gamma, double *psi,
+                     int32_t *id);
+/* replace species sp's particles (n entries; gamma/psi/id may be NULL) */
+int qpad_set_species(qpad_ctx *ctx, int32_t sp, int64_t n, const double *x1,
+                     const double *x2, const double *p1, const double *p2,
+                     const double *p3, const double *q, const double *gamma,
+                     const double *psi, const int32_t *id);
+/* the species' neutralising background qn (1 x nm x (nr+2)) */
+int qpad_get_species_qn(qpad_ctx *ctx, int32_t sp, double *out);
+int qpad_set_species_qn(qpad_ctx *ctx, int32_t sp, const double *in);
+int qpad_beam_count(qpad_ctx *ctx, int32_t ib, int64_t *npp);
+int qpad_get_beam(qpad_ctx *ctx, int32_t ib, double *x1, double *x2, double *x3,
+                  double *p1, double *p2, double *p3, double *q);
+int qpad_set_beam(qpad_ctx *ctx, int32_t ib, int64_t n, const double *x1,
+                  const double *x2, const double *x3, const double *p1,
+                  const double *p2, const double *p3, const double *q);
+
+/* timing of the dominant kernels (HIP events on the context's stream):
+   which = 0: plasma current deposit (amjdeposit), 1: plasma push,
+   2: field solves.  Returns the summed milliseconds and launch count since
+   the last reset. */
+int qpad_kernel_time(qpad_ctx *ctx, int32_t which, double *ms, int64_t *launches);
+int qpad_kernel_time_reset(qpad_ctx *ctx, int32_t enable);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

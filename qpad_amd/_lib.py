@@ -1,0 +1,135 @@
+"""ctypes binding of libqpad_hip.so (C ABI in include/qpad_hip.h).
+
+The library is the only compute path: importing this module without the
+built .so raises, there is no CPU fallback.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libqpad_hip.so")
+
+F_PSI, F_E, F_E_SPE, F_B, F_B_SPE, F_B_BEAM, F_CU, F_ACU, F_AMU, F_DCU, F_Q_SPE, F_Q_BEAM = range(12)
+NFIELD = 12
+NCOMP = [1, 3, 3, 3, 3, 3, 3, 2, 3, 2, 1, 1]
+FIELD_IDS = {"psi": F_PSI, "e": F_E, "e_spe": F_E_SPE, "b": F_B, "b_spe": F_B_SPE,
+             "b_beam": F_B_BEAM, "cu": F_CU, "acu": F_ACU, "amu": F_AMU, "dcu": F_DCU,
+             "q_spe": F_Q_SPE, "q_beam": F_Q_BEAM}
+BND_ZERO, BND_OPEN = 2, 3
+PUSH2_ROBUST = 1
+
+_dp = C.POINTER(C.c_double)
+_ip = C.POINTER(C.c_int32)
+
+
+class QpadError(RuntimeError):
+    """A call into libqpad_hip failed (QPAD's write_err equivalent)."""
+
+
+class Grid(C.Structure):
+    _fields_ = [("nr", C.c_int32), ("nxi", C.c_int32), ("max_mode", C.c_int32),
+                ("dr", C.c_double), ("dxi", C.c_double), ("relax_fac", C.c_double),
+                ("bnd", C.c_int32), ("iter_max", C.c_int32), ("iter_reltol", C.c_double),
+                ("iter_abstol", C.c_double), ("dt", C.c_double)]
+
+
+class SpeciesCfg(C.Structure):
+    _fields_ = [("ppc1", C.c_int32), ("ppc2", C.c_int32), ("num_theta", C.c_int32),
+                ("qm", C.c_double), ("mass", C.c_double), ("density", C.c_double),
+                ("uth", C.c_double * 3), ("neutralized", C.c_int32), ("den_min", C.c_double),
+                ("seed", C.c_uint64), ("push_type", C.c_int32)]
+
+
+class BeamCfg(C.Structure):
+    _fields_ = [("total_num", C.c_int64), ("total_charge", C.c_double), ("qm", C.c_double),
+                ("mass", C.c_double), ("gamma", C.c_double), ("center", C.c_double * 3),
+                ("sigma", C.c_double * 3), ("range_lo", C.c_double * 3),
+                ("range_hi", C.c_double * 3), ("uth", C.c_double * 3), ("quiet", C.c_int32),
+                ("evol", C.c_int32), ("seed", C.c_uint64)]
+
+
+# name -> (restype, argtypes); every entry point returns int status
+_CTX = C.c_void_p
+SIGNATURES = {
+    "qpad_last_error": (C.c_char_p, []),
+    "qpad_abi_version": (C.c_int, []),
+    "qpad_create": (C.c_int, [C.POINTER(Grid), C.POINTER(C.c_void_p)]),
+    "qpad_destroy": (C.c_int, [_CTX]),
+    "qpad_species_new": (C.c_int, [_CTX, C.POINTER(SpeciesCfg), C.POINTER(C.c_int32)]),
+    "qpad_species_renew": (C.c_int, [_CTX, C.c_int32]),
+    "qpad_species_qdp": (C.c_int, [_CTX, C.c_int32]),
+    "qpad_species_amjdp": (C.c_int, [_CTX, C.c_int32, C.c_double]),
+    "qpad_species_push": (C.c_int, [_CTX, C.c_int32, C.c_double]),
+    "qpad_field_solve_psi": (C.c_int, [_CTX]),
+    "qpad_field_solve_ez": (C.c_int, [_CTX, C.c_int32, C.c_int32]),
+    "qpad_field_solve_et": (C.c_int, [_CTX, C.c_int32, C.c_int32]),
+    "qpad_field_solve_bz": (C.c_int, [_CTX, C.c_int32, C.c_int32]),
+    "qpad_field_solve_bt_beam": (C.c_int, [_CTX, C.c_int32]),
+    "qpad_field_solve_djdxi": (C.c_int, [_CTX]),
+    "qpad_field_solve_bt_iter": (C.c_int, [_CTX]),
+    "qpad_field_add_b": (C.c_int, [_CTX]),
+    "qpad_convergence_tester": (C.c_int, [_CTX, C.c_int32, _dp, _dp]),
+    "qpad_beam_qdp": (C.c_int, [_CTX]),
+    "qpad_beam_push": (C.c_int, [_CTX]),
+    "qpad_beam_new": (C.c_int, [_CTX, C.POINTER(BeamCfg), C.POINTER(C.c_int32)]),
+    "qpad_step3d_begin": (C.c_int, [_CTX]),
+    "qpad_slice": (C.c_int, [_CTX, C.c_int32, C.POINTER(C.c_int32)]),
+    "qpad_step3d_end": (C.c_int, [_CTX]),
+    "qpad_step3d": (C.c_int, [_CTX, C.POINTER(C.c_int32)]),
+    "qpad_get_field": (C.c_int, [_CTX, C.c_int32, _dp]),
+    "qpad_set_field": (C.c_int, [_CTX, C.c_int32, _dp]),
+    "qpad_get_hist": (C.c_int, [_CTX, C.c_int32, C.c_int32, _dp]),
+    "qpad_set_hist": (C.c_int, [_CTX, C.c_int32, C.c_int32, _dp]),
+    "qpad_species_count": (C.c_int, [_CTX, C.c_int32, C.POINTER(C.c_int64)]),
+    "qpad_get_species": (C.c_int, [_CTX, C.c_int32] + [_dp] * 8 + [_ip]),
+    "qpad_set_species": (C.c_int, [_CTX, C.c_int32, C.c_int64] + [_dp] * 8 + [_ip]),
+    "qpad_get_species_qn": (C.c_int, [_CTX, C.c_int32, _dp]),
+    "qpad_set_species_qn": (C.c_int, [_CTX, C.c_int32, _dp]),
+    "qpad_beam_count": (C.c_int, [_CTX, C.c_int32, C.POINTER(C.c_int64)]),
+    "qpad_get_beam": (C.c_int, [_CTX, C.c_int32] + [_dp] * 7),
+    "qpad_set_beam": (C.c_int, [_CTX, C.c_int32, C.c_int64] + [_dp] * 7),
+    "qpad_kernel_time": (C.c_int, [_CTX, C.c_int32, _dp, C.POINTER(C.c_int64)]),
+    "qpad_kernel_time_reset": (C.c_int, [_CTX, C.c_int32]),
+}
+
+_lib = None
+
+
+def load(path=None):
+    """Load libqpad_hip.so and attach the C signatures (raises if absent)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    path = path or LIB_PATH
+    if not os.path.exists(path):
+        raise QpadError(f"{path} is missing: build it with `python -m qpad_amd.build` "
+                        "(there is no CPU fallback)")
+    lib = C.CDLL(path)
+    for name, (res, args) in SIGNATURES.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = lib
+    return lib
+
+
+def check(rc):
+    if rc != 0:
+        msg = load().qpad_last_error()
+        raise QpadError(f"libqpad_hip error {rc}: {msg.decode() if msg else ''}")
+
+
+def dptr(a):
+    if a is None:
+        return None
+    assert a.dtype == np.float64 and a.flags["C_CONTIGUOUS"], "need contiguous float64"
+    return a.ctypes.data_as(_dp)
+
+
+def iptr(a):
+    if a is None:
+        return None
+    assert a.dtype == np.int32 and a.flags["C_CONTIGUOUS"], "need contiguous int32"
+    return a.ctypes.data_as(_ip)

@@ -1,0 +1,92 @@
+"""The benchmark workloads as qpinput.json-style decks.
+
+C1: blowout wake, Nr=256, r_max=6, m=0..1, Nxi=256, dxi=0.04, ppc 2x8,
+    tri-Gaussian beam sigma_r=0.25, sigma_xi=0.5, n_b/n_p=4, gamma=2e4,
+    2^18 macro-particles, 1 3D step.
+C2: Nr=4096, r_max=12, m=0..2, Nxi=1024, dxi=0.04, ppc 4x32 (524,288 plasma
+    particles per slice), sigma_r=0.5, sigma_xi=1, n_b/n_p=8, 2^22 beam
+    particles, 10 3D steps with dt=10.
+C3: as C2 with Nr=2048, m=0..4, electrons and mobile ions (m=1836) at ppc
+    4x16, both not neutralised, beam offset by 0.1 sigma_r, 20 3D steps.
+"""
+import copy
+
+from .input import tri_gaussian_charge
+
+
+def _deck(nr, rmax, max_mode, nxi, dxi, species, beam, dt=10.0, nstep=1, iter_max=10, tol=1e-3):
+    zlen = nxi * dxi
+    return {
+        "simulation": {
+            "nodes": [1, 1], "grid": [nr, nxi],
+            "box": {"r": [0.0, rmax], "z": [-zlen / 2, zlen / 2]},
+            "field_boundary": "open", "max_mode": max_mode, "interpolation": "linear",
+            "n0": 1.0e16, "time": dt * nstep, "dt": dt,
+            "nbeams": 1, "nspecies": len(species), "nneutrals": 0, "nlasers": 0,
+            "iter_max": iter_max, "iter_reltol": tol, "iter_abstol": tol,
+            "random_seed": 10, "algorithm": "standard",
+        },
+        "beam": [beam],
+        "species": species,
+    }
+
+
+def _beam(zlen, sigma_r, sigma_xi, n_peak, total_num, offset_x=0.0, gamma=20000.0):
+    z_front = -zlen / 2
+    zc = z_front + 5.0 * sigma_xi
+    sig = [sigma_r, sigma_r, sigma_xi]
+    return {
+        "profile_type": "random", "geometry": "cartesian",
+        "profile": ["gaussian", "gaussian", "gaussian"], "evolution": True,
+        "push_type": "reduced", "has_spin": False, "q": -1.0, "m": 1.0, "gamma": gamma,
+        "total_num": int(total_num), "total_charge": tri_gaussian_charge(n_peak, sig),
+        "quiet_start": False, "gauss_center": [offset_x, 0.0, zc], "gauss_sigma": sig,
+        "range1": [offset_x - 5 * sigma_r, offset_x + 5 * sigma_r],
+        "range2": [-5 * sigma_r, 5 * sigma_r],
+        "range3": [z_front, zc + 5.0 * sigma_xi],
+        "uth": [5.0, 5.0, 0.0],
+    }
+
+
+def _electrons(ppc_r, ppc_phi, neutralized=True):
+    return {"profile": ["uniform", "uniform"], "ppc": [ppc_r, 1], "num_theta": ppc_phi,
+            "q": -1.0, "m": 1.0, "density": 1.0, "push_type": "robust",
+            "neutralized": neutralized}
+
+
+def c1(total_num=1 << 18):
+    nxi, dxi = 256, 0.04
+    return _deck(256, 6.0, 1, nxi, dxi, [_electrons(2, 8)],
+                 _beam(nxi * dxi, 0.25, 0.5, 4.0, total_num), nstep=1)
+
+
+def c2(total_num=1 << 22, nstep=10):
+    nxi, dxi = 1024, 0.04
+    return _deck(4096, 12.0, 2, nxi, dxi, [_electrons(4, 32)],
+                 _beam(nxi * dxi, 0.5, 1.0, 8.0, total_num), nstep=nstep)
+
+
+def c3(total_num=1 << 22, nstep=20):
+    nxi, dxi = 1024, 0.04
+    ions = _electrons(4, 16, neutralized=False)
+    ions.update({"q": 1.0, "m": 1836.0})
+    return _deck(2048, 12.0, 4, nxi, dxi, [_electrons(4, 16, neutralized=False), ions],
+                 _beam(nxi * dxi, 0.5, 1.0, 8.0, total_num, offset_x=0.05), nstep=nstep)
+
+
+def small(nr=64, nxi=48, max_mode=1, ppc=(2, 4), total_num=1 << 14, rmax=4.0, dxi=0.05,
+          n_peak=4.0, sigma_r=0.3, sigma_xi=0.4, offset_x=0.0, ions=False):
+    """A reduced deck with the same physics for fast parity checks."""
+    species = [_electrons(ppc[0], ppc[1], neutralized=not ions)]
+    if ions:
+        ion = _electrons(ppc[0], ppc[1], neutralized=False)
+        ion.update({"q": 1.0, "m": 1836.0})
+        species.append(ion)
+    return _deck(nr, rmax, max_mode, nxi, dxi, species,
+                 _beam(nxi * dxi, sigma_r, sigma_xi, n_peak, total_num, offset_x=offset_x), nstep=1)
+
+
+def with_overrides(deck, **sim):
+    d = copy.deepcopy(deck)
+    d["simulation"].update(sim)
+    return d

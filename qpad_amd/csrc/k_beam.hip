@@ -1,0 +1,463 @@
+// k_beam.hip -- 3D beam on the GPU (FP64): random Gaussian injection,
+// (r, xi) charge deposit into the q_beam history and the reduced pusher.
+//
+// Beam particles are sorted by their (xi, r) cell before the deposit; every
+// history node then sums the particles of its four neighbouring cells in a
+// fixed order (node-centric gather, deterministic, no atomics).
+#include <cstring>
+
+#include <rocprim/rocprim.hpp>
+
+#include "kernels.h"
+#include "rng.h"
+
+namespace qpad {
+
+__device__ inline double bradius(double x1, double x2)
+{
+    return sqrt(__dadd_rn(__dmul_rn(x1, x1), __dmul_rn(x2, x2)));
+}
+
+// ---------------------------------------------------------------------------
+// injection: fdist3d_rnd_class.f03:360-513 (Cartesian Gaussian profile)
+// ---------------------------------------------------------------------------
+struct BeamGenArgs {
+    int64_t n;
+    uint64_t seed;
+    double center[3], sigma[3], lo[3], hi[3], uth[3];
+    double gamma, edge_r, edge_z, qpp;
+    double *x1, *x2, *x3, *p1, *p2, *p3, *q;
+    uint32_t *flag;
+};
+
+__global__ void k_beam_gen(BeamGenArgs a)
+{
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= a.n) return;
+    double xt[3] = {0.0, 0.0, 0.0};
+    // redraw until inside range1..range3 (bounded: the probability of 4096
+    // consecutive rejections is nil for ranges of a few sigma)
+    for (uint32_t att = 0; att < 4096u; att++) {
+        double g[4];
+        normal4(a.seed, (uint64_t)i, 2u * att, g);
+        bool ok = true;
+        for (int d = 0; d < 3; d++) {
+            xt[d] = g[d] * a.sigma[d] + a.center[d];
+            if (xt[d] < a.lo[d] || xt[d] >= a.hi[d]) ok = false;
+        }
+        if (ok) break;
+    }
+    double r = sqrt(xt[0] * xt[0] + xt[1] * xt[1]);
+    bool keep = (r < a.edge_r) && (xt[2] >= 0.0) && (xt[2] < a.edge_z);
+    double g[4];
+    normal4(a.seed, (uint64_t)i, 0x80000000u, g);
+    double p1 = a.uth[0] * g[0], p2 = a.uth[1] * g[1];
+    double p3 = a.uth[2] * g[2] + a.gamma;
+    p3 = sqrt(p3 * p3 - p1 * p1 - p2 * p2 - 1.0);
+    a.x1[i] = xt[0];
+    a.x2[i] = xt[1];
+    a.x3[i] = xt[2];
+    a.p1[i] = p1;
+    a.p2[i] = p2;
+    a.p3[i] = p3;
+    a.q[i] = a.qpp;
+    a.flag[i] = keep ? 1u : 0u;
+}
+
+struct CompactArgs {
+    int64_t n;
+    const uint32_t *flag, *pos;
+    const double *s[7];
+    double *d[7];
+};
+
+__global__ void k_beam_compact(CompactArgs a)
+{
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= a.n || !a.flag[i]) return;
+    uint32_t o = a.pos[i];
+    for (int f = 0; f < 7; f++) a.d[f][o] = a.s[f][i];
+}
+
+// quiet start: mirrored copies (-x, -y, -p_perp), charge halved
+__global__ void k_beam_quiet(double *x1, double *x2, double *x3, double *p1, double *p2, double *p3,
+                             double *q, int64_t n)
+{
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    x1[n + i] = -x1[i];
+    x2[n + i] = -x2[i];
+    x3[n + i] = x3[i];
+    p1[n + i] = -p1[i];
+    p2[n + i] = -p2[i];
+    p3[n + i] = p3[i];
+    double qq = 0.5 * q[i];
+    q[i] = qq;
+    q[n + i] = qq;
+}
+
+int launch_beam_inject(qpad_ctx *c, int ib)
+{
+    Beam &bm = c->bm[ib];
+    const qpad_beam_cfg &cf = bm.cfg;
+    const double pi = 4.0 * atan(1.0);
+    int64_t n = cf.total_num;
+    if (n <= 0) {
+        bm.npp = 0;
+        return 0;
+    }
+    BeamGenArgs a;
+    a.n = n;
+    a.seed = cf.seed;
+    for (int d = 0; d < 3; d++) {
+        a.center[d] = cf.center[d];
+        a.sigma[d] = cf.sigma[d];
+        a.lo[d] = cf.range_lo[d];
+        a.hi[d] = cf.range_hi[d];
+        a.uth[d] = cf.uth[d];
+    }
+    a.gamma = cf.gamma;
+    a.edge_r = (double)c->g.nr * c->g.dr;
+    a.edge_z = (double)c->g.nxi * c->g.dxi;
+    a.qpp = copysign(1.0, cf.qm) * cf.total_charge /
+            (2.0 * pi * (double)n * c->g.dr * c->g.dr * c->g.dxi);
+    a.x1 = bm.a_x1; a.x2 = bm.a_x2; a.x3 = bm.a_x3;
+    a.p1 = bm.a_p1; a.p2 = bm.a_p2; a.p3 = bm.a_p3; a.q = bm.a_q;
+    a.flag = bm.key_alt;
+    unsigned nb = (unsigned)((n + 255) / 256);
+    hipLaunchKernelGGL(k_beam_gen, dim3(nb), dim3(256), 0, c->stream, a);
+    QPAD_HIP_CHECK(hipGetLastError());
+    size_t bytes = bm.sort_tmp_bytes;
+    QPAD_HIP_CHECK(rocprim::exclusive_scan(bm.sort_tmp, bytes, bm.key_alt, bm.idx_alt, 0u, (size_t)n,
+                                           rocprim::plus<uint32_t>(), c->stream));
+    CompactArgs ca;
+    ca.n = n;
+    ca.flag = bm.key_alt;
+    ca.pos = bm.idx_alt;
+    const double *src[7] = {bm.a_x1, bm.a_x2, bm.a_x3, bm.a_p1, bm.a_p2, bm.a_p3, bm.a_q};
+    double *dst[7] = {bm.x1, bm.x2, bm.x3, bm.p1, bm.p2, bm.p3, bm.q};
+    for (int f = 0; f < 7; f++) {
+        ca.s[f] = src[f];
+        ca.d[f] = dst[f];
+    }
+    hipLaunchKernelGGL(k_beam_compact, dim3(nb), dim3(256), 0, c->stream, ca);
+    QPAD_HIP_CHECK(hipGetLastError());
+    uint32_t last_pos = 0, last_flag = 0;
+    QPAD_HIP_CHECK(hipMemcpyAsync(&last_pos, bm.idx_alt + (n - 1), sizeof(uint32_t), hipMemcpyDeviceToHost,
+                                  c->stream));
+    QPAD_HIP_CHECK(hipMemcpyAsync(&last_flag, bm.key_alt + (n - 1), sizeof(uint32_t), hipMemcpyDeviceToHost,
+                                  c->stream));
+    QPAD_HIP_CHECK(hipStreamSynchronize(c->stream));
+    int64_t nk = (int64_t)last_pos + last_flag;
+    if (cf.quiet) {
+        if (nk > 0) {
+            hipLaunchKernelGGL(k_beam_quiet, dim3((unsigned)((nk + 255) / 256)), dim3(256), 0, c->stream,
+                               bm.x1, bm.x2, bm.x3, bm.p1, bm.p2, bm.p3, bm.q, nk);
+            QPAD_HIP_CHECK(hipGetLastError());
+        }
+        nk *= 2;
+    }
+    bm.npp = nk;
+    return 0;
+}
+
+// ---------------------------------------------------------------------------
+// sort by (xi, r) cell: key = mm * nr + nn (0-based cell indices); particles
+// outside the box get key = nr * nxi and fall past cell_start[ncell]
+// ---------------------------------------------------------------------------
+__global__ void k_beam_keys(const double *x1, const double *x2, const double *x3, uint32_t *key, uint32_t *idx,
+                            int64_t n, double idr, double idz, double edge_r, double edge_z, int nr, int nxi)
+{
+    int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= n) return;
+    double r = bradius(x1[p], x2[p]);
+    double z = x3[p];
+    uint32_t k;
+    if (!(r < edge_r) || !(z < edge_z) || !(z >= 0.0)) {
+        k = (uint32_t)nr * (uint32_t)nxi;
+    } else {
+        int nn = (int)__dmul_rn(r, idr);
+        int mm = (int)(z * idz);
+        if (nn > nr - 1) nn = nr - 1;
+        if (mm > nxi - 1) mm = nxi - 1;
+        k = (uint32_t)mm * (uint32_t)nr + (uint32_t)nn;
+    }
+    key[p] = k;
+    idx[p] = (uint32_t)p;
+}
+
+int launch_beam_keys(qpad_ctx *c, int ib)
+{
+    Beam &bm = c->bm[ib];
+    if (bm.npp == 0) return 0;
+    hipLaunchKernelGGL(k_beam_keys, dim3((unsigned)((bm.npp + 255) / 256)), dim3(256), 0, c->stream, bm.x1,
+                       bm.x2, bm.x3, bm.key, bm.idx, bm.npp, 1.0 / c->g.dr, 1.0 / c->g.dxi,
+                       (double)c->g.nr * c->g.dr, (double)c->g.nxi * c->g.dxi, c->g.nr, c->g.nxi);
+    QPAD_HIP_CHECK(hipGetLastError());
+    return 0;
+}
+
+struct BPermArgs {
+    int64_t n;
+    const uint32_t *idx;
+    const double *s[7];
+    double *d[7];
+};
+
+__global__ void k_beam_permute(BPermArgs a)
+{
+    int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= a.n) return;
+    uint32_t src = a.idx[p];
+    for (int f = 0; f < 7; f++) a.d[f][p] = a.s[f][src];
+}
+
+__global__ void k_cellstart_b(const uint32_t *key, int64_t n, int ncell, int32_t *cs)
+{
+    int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p > n) return;
+    int64_t kp = (p == 0) ? -1 : (int64_t)key[p - 1];
+    int64_t kc = (p == n) ? (int64_t)ncell : (int64_t)key[p];
+    for (int64_t c = kp + 1; c <= kc; c++) cs[c] = (int32_t)p;
+}
+
+int launch_beam_sort(qpad_ctx *c, int ib)
+{
+    Beam &bm = c->bm[ib];
+    int64_t n = bm.npp;
+    int ncell = c->g.nr * c->g.nxi;
+    int bits = 1;
+    while ((1u << bits) <= (uint32_t)ncell) bits++;
+    if (n > 0) {
+        QPAD_TRY(launch_beam_keys(c, ib));
+        size_t bytes = bm.sort_tmp_bytes;
+        QPAD_HIP_CHECK(rocprim::radix_sort_pairs(bm.sort_tmp, bytes, bm.key, bm.key_alt, bm.idx, bm.idx_alt,
+                                                 (size_t)n, 0, bits, c->stream));
+        BPermArgs a;
+        a.n = n;
+        a.idx = bm.idx_alt;
+        const double *src[7] = {bm.x1, bm.x2, bm.x3, bm.p1, bm.p2, bm.p3, bm.q};
+        double *dst[7] = {bm.a_x1, bm.a_x2, bm.a_x3, bm.a_p1, bm.a_p2, bm.a_p3, bm.a_q};
+        for (int f = 0; f < 7; f++) {
+            a.s[f] = src[f];
+            a.d[f] = dst[f];
+        }
+        hipLaunchKernelGGL(k_beam_permute, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, c->stream, a);
+        QPAD_HIP_CHECK(hipGetLastError());
+        std::swap(bm.x1, bm.a_x1); std::swap(bm.x2, bm.a_x2); std::swap(bm.x3, bm.a_x3);
+        std::swap(bm.p1, bm.a_p1); std::swap(bm.p2, bm.a_p2); std::swap(bm.p3, bm.a_p3);
+        std::swap(bm.q, bm.a_q);
+        std::swap(bm.key, bm.key_alt);
+    }
+    // cs[ncell] = number of particles inside the box, cs[ncell + 1] = n
+    hipLaunchKernelGGL(k_cellstart_b, dim3((unsigned)((n + 1 + 255) / 256)), dim3(256), 0, c->stream, bm.key, n,
+                       ncell, bm.cell_start);
+    QPAD_HIP_CHECK(hipGetLastError());
+    return 0;
+}
+
+// ---------------------------------------------------------------------------
+// charge deposit: qdeposit_part3d (part3d_class.f03:221-356), node-centric
+// ---------------------------------------------------------------------------
+struct BDepArgs {
+    int nr, nxi, nm, ld, M, nbeams;
+    double idr, idz;
+    const double *x1[kMaxBeams], *x2[kMaxBeams], *x3[kMaxBeams], *q[kMaxBeams];
+    const int32_t *cs[kMaxBeams];
+    double *h;   // q_beam history [s][nm][ld]
+};
+
+__global__ __launch_bounds__(256) void k_beam_qdeposit(BDepArgs a)
+{
+    const int64_t tid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int nr = a.nr, nxi = a.nxi, nm = a.nm, ld = a.ld;
+    const int64_t nnode = (int64_t)(nr + 2) * (nxi + 1);
+    if (tid >= nnode) return;
+    const int i = (int)(tid % (nr + 2));
+    const int s = (int)(tid / (nr + 2)) + 1;   // history slice 1..nxi+1
+    double tot[kMaxNm];
+    for (int k = 0; k < nm; k++) tot[k] = 0.0;
+    for (int b = 0; b < a.nbeams; b++) {
+        double acc[kMaxNm];
+        for (int k = 0; k < nm; k++) acc[k] = 0.0;
+        for (int dk = 0; dk < 2; dk++) {          // xi cell mm = s-1-dk
+            const int mm = s - 1 - dk;
+            if (mm < 0 || mm >= nxi) continue;
+            for (int dj = 0; dj < 2; dj++) {      // r cell nn = i-1-dj
+                const int nn = i - 1 - dj;
+                if (nn < 0 || nn >= nr) continue;
+                const int cell = mm * nr + nn;
+                const int q0 = a.cs[b][cell], q1 = a.cs[b][cell + 1];
+                for (int p = q0; p < q1; p++) {
+                    const double x1 = a.x1[b][p], x2 = a.x2[b][p];
+                    double pr = __dmul_rn(bradius(x1, x2), a.idr);
+                    double pz = a.x3[b][p] * a.idz;
+                    double ph0r = x1 / pr * a.idr, ph0i = -x2 / pr * a.idr;
+                    double wr = pr - (double)nn, wz = pz - (double)mm;
+                    double wrj = dj ? wr : 1.0 - wr;
+                    double wzk = dk ? wz : 1.0 - wz;
+                    double w = wrj * wzk;
+                    double phr = 1.0 * a.q[b][p], phi = 0.0;
+                    acc[0] += w * phr;
+                    for (int m = 1; m <= a.M; m++) {
+                        double tr = phr * ph0r - phi * ph0i, ti = phr * ph0i + phi * ph0r;
+                        phr = tr;
+                        phi = ti;
+                        acc[2 * m - 1] += w * phr;
+                        acc[2 * m] += w * phi;
+                    }
+                }
+            }
+        }
+        // axis and 1/r for slices 1..nzp (part3d_class.f03:315-351)
+        for (int k = 0; k < nm; k++) {
+            double v = acc[k];
+            if (s <= nxi) {
+                if (i == 0) v = 0.0;
+                else if (i == 1) v = (k == 0) ? 8.0 * v : 0.0;
+                else v = v * (1.0 / (double)(i - 1));
+            }
+            tot[k] = tot[k] + v;
+        }
+    }
+    for (int k = 0; k < nm; k++) a.h[(size_t)s * nm * ld + (size_t)k * ld + i] = tot[k];
+}
+
+int launch_beam_qdeposit(qpad_ctx *c)
+{
+    BDepArgs a;
+    a.nr = c->g.nr;
+    a.nxi = c->g.nxi;
+    a.nm = c->g.nm;
+    a.ld = c->g.ld;
+    a.M = c->g.M;
+    a.nbeams = c->nbeams;
+    a.idr = 1.0 / c->g.dr;
+    a.idz = 1.0 / c->g.dxi;
+    for (int b = 0; b < kMaxBeams; b++) {
+        bool on = b < c->nbeams;
+        a.x1[b] = on ? c->bm[b].x1 : nullptr;
+        a.x2[b] = on ? c->bm[b].x2 : nullptr;
+        a.x3[b] = on ? c->bm[b].x3 : nullptr;
+        a.q[b] = on ? c->bm[b].q : nullptr;
+        a.cs[b] = on ? c->bm[b].cell_start : nullptr;
+    }
+    a.h = c->h_q_beam;
+    int64_t nnode = (int64_t)(c->g.nr + 2) * (c->g.nxi + 1);
+    hipLaunchKernelGGL(k_beam_qdeposit, dim3((unsigned)((nnode + 255) / 256)), dim3(256), 0, c->stream, a);
+    QPAD_HIP_CHECK(hipGetLastError());
+    return 0;
+}
+
+// ---------------------------------------------------------------------------
+// reduced pusher: push_reduced_part3d (part3d_class.f03:477-576) with the
+// bilinear mode gather of interp_emf_part3d (:691-790)
+// ---------------------------------------------------------------------------
+struct BPushArgs {
+    int nr, nxi, nm, ld, M;
+    int64_t n;
+    double idr, idz, qtmh, dt;
+    const double *he, *hb;
+    double *x1, *x2, *x3, *p1, *p2, *p3;
+    const int32_t *cs;
+    int ncell;
+};
+
+__global__ __launch_bounds__(256) void k_beam_push(BPushArgs a)
+{
+    int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= a.n) return;
+    if (p >= a.cs[a.ncell]) return;   // already outside the box
+    const int nm = a.nm, ld = a.ld;
+    const size_t sl = (size_t)3 * nm * ld;
+    double x1 = a.x1[p], x2 = a.x2[p], x3 = a.x3[p];
+    double pr = __dmul_rn(bradius(x1, x2), a.idr);
+    double pz = x3 * a.idz;
+    double cc = x1 / pr * a.idr, ss = x2 / pr * a.idr;
+    int nn = (int)pr, mm = (int)pz;
+    pr = pr - (double)nn;
+    pz = pz - (double)mm;
+    nn = nn + 1;
+    mm = mm + 1;
+    if (nn > a.nr) nn = a.nr;
+    if (mm > a.nxi) mm = a.nxi;
+    double wr[2] = {1.0 - pr, pr}, wz[2] = {1.0 - pz, pz};
+    double ep[3] = {0.0, 0.0, 0.0}, bp[3] = {0.0, 0.0, 0.0};
+    for (int k = 0; k < 2; k++)
+        for (int j = 0; j < 2; j++) {
+            double wt = wr[j] * wz[k];
+            const size_t o = (size_t)(mm + k) * sl + (size_t)(nn + j);
+            for (int c3 = 0; c3 < 3; c3++) {
+                ep[c3] = ep[c3] + a.he[o + (size_t)c3 * nm * ld] * wt;
+                bp[c3] = bp[c3] + a.hb[o + (size_t)c3 * nm * ld] * wt;
+            }
+        }
+    double phr = 1.0, phi = 0.0;
+    for (int m = 1; m <= a.M; m++) {
+        double tr = phr * cc - phi * ss, ti = phr * ss + phi * cc;
+        phr = tr;
+        phi = ti;
+        const double ph_r = 2.0 * phr, ph_i = 2.0 * phi;
+        const int kr = 2 * m - 1, ki = 2 * m;
+        for (int k = 0; k < 2; k++)
+            for (int j = 0; j < 2; j++) {
+                double wt = wr[j] * wz[k];
+                const size_t o = (size_t)(mm + k) * sl + (size_t)(nn + j);
+                for (int c3 = 0; c3 < 3; c3++) {
+                    const size_t ocr = o + ((size_t)c3 * nm + kr) * ld, oci = o + ((size_t)c3 * nm + ki) * ld;
+                    ep[c3] = ep[c3] + (a.he[ocr] * ph_r - a.he[oci] * ph_i) * wt;
+                    bp[c3] = bp[c3] + (a.hb[ocr] * ph_r - a.hb[oci] * ph_i) * wt;
+                }
+            }
+    }
+    double t = ep[0] * cc - ep[1] * ss, u = ep[0] * ss + ep[1] * cc;
+    ep[0] = t;
+    ep[1] = u;
+    t = bp[0] * cc - bp[1] * ss;
+    u = bp[0] * ss + bp[1] * cc;
+    bp[0] = t;
+    bp[1] = u;
+    for (int c3 = 0; c3 < 3; c3++) {
+        ep[c3] = ep[c3] * a.qtmh;
+        bp[c3] = bp[c3] * a.qtmh;
+    }
+    double wp0 = ep[0] - bp[1], wp1 = ep[1] + bp[0], wp2 = ep[2];
+    double q1 = a.p1[p], q2 = a.p2[p], q3 = a.p3[p];
+    q1 = q1 + wp0; q2 = q2 + wp1; q3 = q3 + wp2;
+    q1 = q1 + wp0; q2 = q2 + wp1; q3 = q3 + wp2;
+    double dt_gam = a.dt / sqrt(1.0 + q1 * q1 + q2 * q2 + q3 * q3);
+    a.p1[p] = q1;
+    a.p2[p] = q2;
+    a.p3[p] = q3;
+    a.x1[p] = x1 + q1 * dt_gam;
+    a.x2[p] = x2 + q2 * dt_gam;
+    a.x3[p] = x3 - q3 * dt_gam + a.dt;
+}
+
+int launch_beam_push(qpad_ctx *c, int ib)
+{
+    Beam &bm = c->bm[ib];
+    if (!bm.cfg.evol || bm.npp == 0) return 0;
+    BPushArgs a;
+    a.nr = c->g.nr;
+    a.nxi = c->g.nxi;
+    a.nm = c->g.nm;
+    a.ld = c->g.ld;
+    a.M = c->g.M;
+    a.n = bm.npp;
+    a.idr = 1.0 / c->g.dr;
+    a.idz = 1.0 / c->g.dxi;
+    a.qtmh = bm.qbm * c->g.dt * 0.5;
+    a.dt = c->g.dt;
+    a.he = c->h_e;
+    a.hb = c->h_b;
+    a.x1 = bm.x1; a.x2 = bm.x2; a.x3 = bm.x3;
+    a.p1 = bm.p1; a.p2 = bm.p2; a.p3 = bm.p3;
+    a.cs = bm.cell_start;
+    a.ncell = c->g.nr * c->g.nxi;
+    hipLaunchKernelGGL(k_beam_push, dim3((unsigned)((bm.npp + 255) / 256)), dim3(256), 0, c->stream, a);
+    QPAD_HIP_CHECK(hipGetLastError());
+    return 0;
+}
+
+}  // namespace qpad

@@ -1,0 +1,637 @@
+// k_plasma.hip -- 2D plasma species on the GPU (FP64): lattice injection,
+// radial-cell sort, rho - Jz deposit, the fused gather + robust predictor +
+// current/acceleration/momentum-flux deposit, and the fused push_u + push_x.
+//
+// Particles are kept sorted by radial cell (cell c = int(r/dr), nodes c+1 and
+// c+2).  Deposits run one wavefront per cell: each lane accumulates its
+// particles in registers, the wavefront reduces with a fixed butterfly and
+// writes per-cell partials P0 (node c+1) and P1 (node c+2) once; the field
+// kernels add P0[i-1] + P1[i-2] per node.  No atomics, deterministic order.
+#include <algorithm>
+#include <cstring>
+
+#include <rocprim/rocprim.hpp>
+
+#include "kernels.h"
+#include "rng.h"
+
+namespace qpad {
+
+#define FX(p, c, k, i) (p)[((size_t)(c) * nm + (size_t)(k)) * ld + (size_t)(i)]
+
+// Radius and normalised position without contraction, so that every kernel
+// computes the same cell for the same particle.
+__device__ inline double radius(double x1, double x2)
+{
+    return sqrt(__dadd_rn(__dmul_rn(x1, x1), __dmul_rn(x2, x2)));
+}
+
+__device__ inline uint32_t cell_key(double x1, double x2, double idr, double edge, int nr)
+{
+    double r = radius(x1, x2);
+    if (!(r < edge)) return (uint32_t)nr;   // leaves the box (update_bound)
+    int c = (int)__dmul_rn(r, idr);
+    return (uint32_t)(c < nr ? c : nr - 1);
+}
+
+__device__ inline double wave_sum(double v)
+{
+#pragma unroll
+    for (int d = 32; d > 0; d >>= 1) v += __shfl_xor(v, d, 64);
+    return v;
+}
+
+// ---------------------------------------------------------------------------
+// injection: fdist2d_class.f03:289-357 (uniform profiles, lattice theta)
+// ---------------------------------------------------------------------------
+struct InjArgs {
+    int nr, ppc1, ppc2, nth;
+    int64_t n;
+    double dr, idr, edge, density, coef, qm, dtheta;
+    double uth0, uth1, uth2;
+    uint64_t seed;
+    double *x1, *x2, *p1, *p2, *p3, *q, *gamma, *psi;
+    int32_t *id;
+    uint32_t *key, *idx;
+};
+
+__global__ void k_inject(InjArgs a)
+{
+    int64_t ip = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (ip >= a.n) return;
+    int64_t t = ip;
+    const int i2 = (int)(t % a.ppc2); t /= a.ppc2;
+    const int i1 = (int)(t % a.ppc1); t /= a.ppc1;
+    const int i = (int)(t % a.nr);
+    const int j = (int)(t / a.nr);
+    // 1-based loop indices of the reference: j+1, i+1, i1+1, i2+1
+    double rn = ((double)(i1 + 1) - 0.5) / (double)a.ppc1 + (double)i;
+    double theta = (((double)(i2 + 1) - 0.5) / (double)a.ppc2 + (double)(j + 1) - 1.0) * a.dtheta;
+    double x1 = rn * a.dr * cos(theta), x2 = rn * a.dr * sin(theta);
+    double th[4] = {0.0, 0.0, 0.0, 0.0};
+    if (a.uth0 != 0.0 || a.uth1 != 0.0 || a.uth2 != 0.0) normal4(a.seed, (uint64_t)ip, 0u, th);
+    double p1 = a.uth0 * th[0], p2 = a.uth1 * th[1], p3 = a.uth2 * th[2];
+    double gam = sqrt(1.0 + p1 * p1 + p2 * p2 + p3 * p3);
+    a.x1[ip] = x1;
+    a.x2[ip] = x2;
+    a.q[ip] = rn * 1.0 * 1.0 * a.density * a.coef;
+    a.p1[ip] = p1;
+    a.p2[ip] = p2;
+    a.p3[ip] = p3;
+    a.gamma[ip] = gam;
+    a.psi[ip] = (1.0 - gam + p3) / a.qm;
+    a.id[ip] = (int32_t)ip;
+    a.key[ip] = cell_key(x1, x2, a.idr, a.edge, a.nr);
+    a.idx[ip] = (uint32_t)ip;
+}
+
+int launch_inject(qpad_ctx *c, int s)
+{
+    Species &sp = c->sp[s];
+    const qpad_species_cfg &cf = sp.cfg;
+    InjArgs a;
+    a.nr = c->g.nr;
+    a.ppc1 = cf.ppc1;
+    a.ppc2 = cf.ppc2;
+    a.nth = cf.num_theta;
+    a.n = (int64_t)cf.num_theta * c->g.nr * cf.ppc1 * cf.ppc2;
+    // den_lon * den_perp * density < den_min drops every particle
+    if (1.0 * 1.0 * cf.density < cf.den_min) a.n = 0;
+    a.dr = c->g.dr;
+    a.idr = 1.0 / c->g.dr;
+    a.edge = (double)c->g.nr * c->g.dr;
+    a.density = cf.density;
+    a.coef = copysign(1.0, cf.qm) / ((double)(cf.ppc1 * cf.ppc2) * (double)cf.num_theta);
+    a.qm = cf.qm;
+    a.dtheta = 2.0 * (4.0 * atan(1.0)) / (double)cf.num_theta;
+    a.uth0 = cf.uth[0];
+    a.uth1 = cf.uth[1];
+    a.uth2 = cf.uth[2];
+    a.seed = cf.seed;
+    a.x1 = sp.x1; a.x2 = sp.x2; a.p1 = sp.p1; a.p2 = sp.p2; a.p3 = sp.p3;
+    a.q = sp.q; a.gamma = sp.gamma; a.psi = sp.psi; a.id = sp.id;
+    a.key = sp.key;
+    a.idx = sp.idx;
+    sp.npp = a.n;
+    if (a.n > 0) {
+        hipLaunchKernelGGL(k_inject, dim3((unsigned)((a.n + 255) / 256)), dim3(256), 0, c->stream, a);
+        QPAD_HIP_CHECK(hipGetLastError());
+    }
+    return 0;
+}
+
+// ---------------------------------------------------------------------------
+// radial-cell sort (stable): rocPRIM radix sort of (key, index), permutation
+// of the SoA arrays, cell start offsets.  Particles with key = nr have left
+// the box and end up past cell_start[nr] (update_bound).
+// ---------------------------------------------------------------------------
+struct PermArgs {
+    int64_t n;
+    const uint32_t *idx;
+    const double *s[8];
+    double *d[8];
+    const int32_t *sid;
+    int32_t *did;
+};
+
+__global__ void k_permute(PermArgs a)
+{
+    int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= a.n) return;
+    uint32_t src = a.idx[p];
+#pragma unroll
+    for (int f = 0; f < 8; f++) a.d[f][p] = a.s[f][src];
+    a.did[p] = a.sid[src];
+}
+
+__global__ void k_cellstart(const uint32_t *key, int64_t n, int ncell, int32_t *cs)
+{
+    int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p > n) return;
+    int kp = (p == 0) ? -1 : (int)key[p - 1];
+    int kc = (p == n) ? ncell + 1 : (int)key[p];
+    for (int c = kp + 1; c <= kc; c++) cs[c] = (int32_t)p;
+}
+
+// scratch of rocPRIM's radix sort (uint32 keys and values) and exclusive scan
+int sort_scratch(size_t n, int bits, size_t *bytes)
+{
+    size_t b1 = 0, b2 = 0;
+    uint32_t *z = nullptr;
+    QPAD_HIP_CHECK(rocprim::radix_sort_pairs(nullptr, b1, z, z, z, z, n, 0, bits));
+    QPAD_HIP_CHECK(rocprim::exclusive_scan(nullptr, b2, z, z, 0u, n, rocprim::plus<uint32_t>()));
+    *bytes = std::max(b1, b2) + 256;
+    return 0;
+}
+
+static int bits_for(uint32_t maxkey)
+{
+    int b = 1;
+    while ((1u << b) <= maxkey) b++;
+    return b;
+}
+
+int launch_sort(qpad_ctx *c, int s)
+{
+    Species &sp = c->sp[s];
+    int64_t n = sp.npp;
+    int bits = bits_for((uint32_t)c->g.nr);
+    if (n > 0) {
+        size_t bytes = sp.sort_tmp_bytes;
+        QPAD_HIP_CHECK(rocprim::radix_sort_pairs(sp.sort_tmp, bytes, sp.key, sp.key_alt, sp.idx, sp.idx_alt,
+                                                 (size_t)n, 0, bits, c->stream));
+        PermArgs a;
+        a.n = n;
+        a.idx = sp.idx_alt;
+        const double *src[8] = {sp.x1, sp.x2, sp.p1, sp.p2, sp.p3, sp.q, sp.gamma, sp.psi};
+        double *dst[8] = {sp.a_x1, sp.a_x2, sp.a_p1, sp.a_p2, sp.a_p3, sp.a_q, sp.a_gamma, sp.a_psi};
+        for (int f = 0; f < 8; f++) {
+            a.s[f] = src[f];
+            a.d[f] = dst[f];
+        }
+        a.sid = sp.id;
+        a.did = sp.a_id;
+        hipLaunchKernelGGL(k_permute, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, c->stream, a);
+        QPAD_HIP_CHECK(hipGetLastError());
+        std::swap(sp.x1, sp.a_x1); std::swap(sp.x2, sp.a_x2);
+        std::swap(sp.p1, sp.a_p1); std::swap(sp.p2, sp.a_p2); std::swap(sp.p3, sp.a_p3);
+        std::swap(sp.q, sp.a_q); std::swap(sp.gamma, sp.a_gamma); std::swap(sp.psi, sp.a_psi);
+        std::swap(sp.id, sp.a_id);
+        std::swap(sp.key, sp.key_alt);
+    }
+    hipLaunchKernelGGL(k_cellstart, dim3((unsigned)((n + 1 + 255) / 256)), dim3(256), 0, c->stream,
+                       sp.key, n, c->g.nr, sp.cell_start);
+    QPAD_HIP_CHECK(hipGetLastError());
+    return 0;
+}
+
+__global__ void k_keys(const double *x1, const double *x2, uint32_t *key, uint32_t *idx, int64_t n,
+                       double idr, double edge, int nr)
+{
+    int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= n) return;
+    key[p] = cell_key(x1[p], x2[p], idr, edge, nr);
+    idx[p] = (uint32_t)p;
+}
+
+int launch_keys(qpad_ctx *c, int s)
+{
+    Species &sp = c->sp[s];
+    if (sp.npp == 0) return 0;
+    hipLaunchKernelGGL(k_keys, dim3((unsigned)((sp.npp + 255) / 256)), dim3(256), 0, c->stream, sp.x1, sp.x2,
+                       sp.key, sp.idx, sp.npp, 1.0 / c->g.dr, (double)c->g.nr * c->g.dr, c->g.nr);
+    QPAD_HIP_CHECK(hipGetLastError());
+    return 0;
+}
+
+// ---------------------------------------------------------------------------
+// rho - Jz deposit: qdeposit_part2d (part2d_class.f03:231-359), per-cell
+// partials; axis rules and 1/r are applied by the consumer
+// ---------------------------------------------------------------------------
+struct QDepArgs {
+    int nr;
+    double idr;
+    const double *x1, *x2, *q;
+    const int32_t *cs;
+    double *P0, *P1;
+};
+
+template <int M>
+__global__ __launch_bounds__(64 * kDepWaves) void k_qdeposit(QDepArgs a)
+{
+    constexpr int NM = 2 * M + 1;
+    const int lane = threadIdx.x & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int c = blockIdx.x * kDepWaves + wave;
+    if (c >= a.nr) return;
+    const int p0 = a.cs[c], p1 = a.cs[c + 1];
+    double acc0[NM], acc1[NM];
+#pragma unroll
+    for (int k = 0; k < NM; k++) { acc0[k] = 0.0; acc1[k] = 0.0; }
+    for (int p = p0 + lane; p < p1; p += 64) {
+        const double x1 = a.x1[p], x2 = a.x2[p], q = a.q[p];
+        double pos = __dmul_rn(radius(x1, x2), a.idr);
+        double ph0r = x1 / pos * a.idr, ph0i = -x2 / pos * a.idr;
+        double w1 = pos - (double)c, w0 = 1.0 - w1;
+        double phr = 1.0 * q, phi = 0.0 * q;
+        acc0[0] += w0 * phr;
+        acc1[0] += w1 * phr;
+#pragma unroll
+        for (int m = 1; m <= M; m++) {
+            double tr = phr * ph0r - phi * ph0i, ti = phr * ph0i + phi * ph0r;
+            phr = tr;
+            phi = ti;
+            acc0[2 * m - 1] += w0 * phr;
+            acc1[2 * m - 1] += w1 * phr;
+            acc0[2 * m] += w0 * phi;
+            acc1[2 * m] += w1 * phi;
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < NM; k++) {
+        double s0 = wave_sum(acc0[k]), s1 = wave_sum(acc1[k]);
+        if (lane == k) {
+            a.P0[(size_t)c * NM + k] = s0;
+            a.P1[(size_t)c * NM + k] = s1;
+        }
+    }
+}
+
+int launch_qdeposit(qpad_ctx *c, int s)
+{
+    Species &sp = c->sp[s];
+    QDepArgs a;
+    a.nr = c->g.nr;
+    a.idr = 1.0 / c->g.dr;
+    a.x1 = sp.x1; a.x2 = sp.x2; a.q = sp.q;
+    a.cs = sp.cell_start;
+    a.P0 = sp.P0;
+    a.P1 = sp.P1;
+    dim3 grid((c->g.nr + kDepWaves - 1) / kDepWaves), blk(64 * kDepWaves);
+    switch (c->g.M) {
+    case 0: hipLaunchKernelGGL(k_qdeposit<0>, grid, blk, 0, c->stream, a); break;
+    case 1: hipLaunchKernelGGL(k_qdeposit<1>, grid, blk, 0, c->stream, a); break;
+    case 2: hipLaunchKernelGGL(k_qdeposit<2>, grid, blk, 0, c->stream, a); break;
+    case 3: hipLaunchKernelGGL(k_qdeposit<3>, grid, blk, 0, c->stream, a); break;
+    default: hipLaunchKernelGGL(k_qdeposit<4>, grid, blk, 0, c->stream, a); break;
+    }
+    QPAD_HIP_CHECK(hipGetLastError());
+    return 0;
+}
+
+// one species' q (axis rules, 1/r) times `scale` into out (1 x nm x ld)
+__global__ void k_combine_q(const double *P0, const double *P1, double *out, int nr, int nm, int ld,
+                            double scale, int accumulate)
+{
+    int64_t tid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (tid >= (int64_t)nm * (nr + 2)) return;
+    int i = (int)(tid % (nr + 2)), k = (int)(tid / (nr + 2));
+    double v = 0.0;
+    if (i >= 1 && i <= nr) v += P0[(size_t)(i - 1) * nm + k];
+    if (i >= 2) v += P1[(size_t)(i - 2) * nm + k];
+    if (i == 0) v = 0.0;
+    else if (i == 1) v = (k == 0) ? 8.0 * v : 0.0;
+    else v = v * (1.0 / (double)(i - 1));
+    double *o = out + (size_t)k * ld + i;
+    *o = accumulate ? *o + scale * v : scale * v;
+}
+
+int launch_combine_q(qpad_ctx *c, int s, double *out, double scale, bool accumulate)
+{
+    int64_t n = (int64_t)c->g.nm * (c->g.nr + 2);
+    hipLaunchKernelGGL(k_combine_q, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, c->stream, c->sp[s].P0,
+                       c->sp[s].P1, out, c->g.nr, c->g.nm, c->g.ld, scale, accumulate ? 1 : 0);
+    QPAD_HIP_CHECK(hipGetLastError());
+    return 0;
+}
+
+// ---------------------------------------------------------------------------
+// gather of a 3-component field at nodes ix, ix+1 (interp_part2d.f03:66-110)
+// ---------------------------------------------------------------------------
+template <int M>
+__device__ inline void gather3(const double *f, int nm, int ld, int ix, double w0, double w1, double pc,
+                               double ps, double out[3])
+{
+#pragma unroll
+    for (int cc = 0; cc < 3; cc++) {
+        double v = 0.0;
+        v = v + FX(f, cc, 0, ix) * w0;
+        v = v + FX(f, cc, 0, ix + 1) * w1;
+        out[cc] = v;
+    }
+    double phr = 1.0, phi = 0.0;
+#pragma unroll
+    for (int m = 1; m <= M; m++) {
+        double tr = phr * pc - phi * ps, ti = phr * ps + phi * pc;
+        phr = tr;
+        phi = ti;
+        const double ph_re = 2.0 * phr, ph_im = 2.0 * phi;
+#pragma unroll
+        for (int cc = 0; cc < 3; cc++) {
+            out[cc] = out[cc] + (FX(f, cc, 2 * m - 1, ix) * ph_re - FX(f, cc, 2 * m, ix) * ph_im) * w0;
+            out[cc] = out[cc] + (FX(f, cc, 2 * m - 1, ix + 1) * ph_re - FX(f, cc, 2 * m, ix + 1) * ph_im) * w1;
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// amjdeposit_robust_part2d (part2d_class.f03:746-1010): gather E, B; robust
+// predictor on a copy of the momentum; deposit cu (3), acu (2), amu (3) per
+// mode.  Value index v = comp * NM + k, comp 0..2 cu, 3..4 acu, 5..7 amu.
+// ---------------------------------------------------------------------------
+struct JDepArgs {
+    int nr, ld, nm;
+    double idr, qbm, qtmh, idt;
+    const double *x1, *x2, *p1, *p2, *p3, *q;
+    double *gamma, *psi;
+    int store_diag;
+    const int32_t *cs;
+    const double *e, *b;
+    double *P0, *P1;
+    const IterState *it;
+    int check_done;
+};
+
+template <int M>
+__global__ __launch_bounds__(64 * kDepWaves) void k_amjdeposit(JDepArgs a)
+{
+    constexpr int NM = 2 * M + 1;
+    constexpr int NV = 8 * NM;
+    if (a.check_done && a.it->done) return;
+    const int lane = threadIdx.x & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int c = blockIdx.x * kDepWaves + wave;
+    if (c >= a.nr) return;
+    const int nm = a.nm, ld = a.ld;
+    const int p0 = a.cs[c], p1 = a.cs[c + 1];
+    const int ix = c + 1;
+    double acc0[NV], acc1[NV];
+#pragma unroll
+    for (int v = 0; v < NV; v++) { acc0[v] = 0.0; acc1[v] = 0.0; }
+    for (int p = p0 + lane; p < p1; p += 64) {
+        const double x1 = a.x1[p], x2 = a.x2[p];
+        double r = radius(x1, x2);
+        const double pc = x1 / r, ps = x2 / r;
+        double pos = __dmul_rn(r, a.idr);
+        const double w1 = pos - (double)c, w0 = 1.0 - w1;
+        double ep[3], bp[3];
+        gather3<M>(a.e, nm, ld, ix, w0, w1, pc, ps, ep);
+        gather3<M>(a.b, nm, ld, ix, w0, w1, pc, ps, bp);
+        double wp0 = ep[0] - bp[1], wp1 = ep[1] + bp[0], wp2 = ep[2];
+        const double pp1 = a.p1[p], pp2 = a.p2[p], pp3 = a.p3[p];
+        double u0[3] = {pp1 * pc + pp2 * ps, pp2 * pc - pp1 * ps, pp3};
+        double gam = sqrt(1.0 + u0[0] * u0[0] + u0[1] * u0[1] + u0[2] * u0[2]);
+        double qtmh1 = a.qtmh * gam / (gam - u0[2]);
+        double ut[3], u[3];
+#pragma unroll
+        for (int cc = 0; cc < 3; cc++) { ep[cc] = ep[cc] * qtmh1; ut[cc] = u0[cc] + ep[cc]; }
+        gam = sqrt(1.0 + ut[0] * ut[0] + ut[1] * ut[1] + ut[2] * ut[2]);
+        double qtmh2 = a.qtmh / (gam - ut[2]);
+#pragma unroll
+        for (int cc = 0; cc < 3; cc++) bp[cc] = bp[cc] * qtmh2;
+        u[0] = ut[0] + ut[1] * bp[2] - ut[2] * bp[1];
+        u[1] = ut[1] + ut[2] * bp[0] - ut[0] * bp[2];
+        u[2] = ut[2] + ut[0] * bp[1] - ut[1] * bp[0];
+        double ostq = 2.0 / (1.0 + bp[0] * bp[0] + bp[1] * bp[1] + bp[2] * bp[2]);
+#pragma unroll
+        for (int cc = 0; cc < 3; cc++) bp[cc] = bp[cc] * ostq;
+        ut[0] = ut[0] + u[1] * bp[2] - u[2] * bp[1];
+        ut[1] = ut[1] + u[2] * bp[0] - u[0] * bp[2];
+        ut[2] = ut[2] + u[0] * bp[1] - u[1] * bp[0];
+#pragma unroll
+        for (int cc = 0; cc < 3; cc++) u[cc] = ut[cc] + ep[cc];
+        double du0 = a.idt * (u[0] - u0[0]), du1 = a.idt * (u[1] - u0[1]);
+#pragma unroll
+        for (int cc = 0; cc < 3; cc++) u[cc] = 0.5 * (u[cc] + u0[cc]);
+        double g2 = sqrt(1.0 + u[0] * u[0] + u[1] * u[1] + u[2] * u[2]);
+        double ipsi = 1.0 / (g2 - u[2]);
+        if (a.store_diag) {
+            a.gamma[p] = g2;
+            a.psi[p] = (1.0 - 1.0 / ipsi) / a.qbm;
+        }
+        double dpsi = a.qbm * (wp2 - (wp0 * u[0] + wp1 * u[1]) * ipsi);
+        du0 = du0 + u[0] * dpsi * ipsi;
+        du1 = du1 + u[1] * dpsi * ipsi;
+        const double val[8] = {u[0], u[1], u[2], du0, du1, u[0] * u[0] * ipsi, u[0] * u[1] * ipsi,
+                               u[1] * u[1] * ipsi};
+        double phr = 1.0 * a.q[p] * ipsi, phi = 0.0;
+        {
+            const double wa = w0 * phr, wb = w1 * phr;
+#pragma unroll
+            for (int cc = 0; cc < 8; cc++) {
+                acc0[cc * NM] += wa * val[cc];
+                acc1[cc * NM] += wb * val[cc];
+            }
+        }
+        const double ph0r = pc, ph0i = -ps;
+#pragma unroll
+        for (int m = 1; m <= M; m++) {
+            double tr = phr * ph0r - phi * ph0i, ti = phr * ph0i + phi * ph0r;
+            phr = tr;
+            phi = ti;
+            const double war = w0 * phr, wbr = w1 * phr, wai = w0 * phi, wbi = w1 * phi;
+#pragma unroll
+            for (int cc = 0; cc < 8; cc++) {
+                acc0[cc * NM + 2 * m - 1] += war * val[cc];
+                acc1[cc * NM + 2 * m - 1] += wbr * val[cc];
+                acc0[cc * NM + 2 * m] += wai * val[cc];
+                acc1[cc * NM + 2 * m] += wbi * val[cc];
+            }
+        }
+    }
+#pragma unroll
+    for (int v = 0; v < NV; v++) {
+        double s0 = wave_sum(acc0[v]), s1 = wave_sum(acc1[v]);
+        if (lane == (v & 63)) {
+            a.P0[(size_t)c * NV + v] = s0;
+            a.P1[(size_t)c * NV + v] = s1;
+        }
+    }
+}
+
+void time_begin(qpad_ctx *c, int which, hipEvent_t *stop);
+void time_end(qpad_ctx *c, int which, hipEvent_t stop);
+
+int launch_amjdeposit(qpad_ctx *c, int s, bool check_done, bool store_diag)
+{
+    Species &sp = c->sp[s];
+    JDepArgs a;
+    a.nr = c->g.nr;
+    a.ld = c->g.ld;
+    a.nm = c->g.nm;
+    a.idr = 1.0 / c->g.dr;
+    a.qbm = sp.qbm;
+    a.qtmh = 0.5 * sp.qbm * c->g.dxi;
+    a.idt = 1.0 / c->g.dxi;
+    a.x1 = sp.x1; a.x2 = sp.x2; a.p1 = sp.p1; a.p2 = sp.p2; a.p3 = sp.p3; a.q = sp.q;
+    a.gamma = sp.gamma;
+    a.psi = sp.psi;
+    a.store_diag = store_diag;
+    a.cs = sp.cell_start;
+    a.e = c->f[QPAD_F_E];
+    a.b = c->f[QPAD_F_B];
+    a.P0 = sp.P0;
+    a.P1 = sp.P1;
+    a.it = c->it;
+    a.check_done = check_done;
+    dim3 grid((c->g.nr + kDepWaves - 1) / kDepWaves), blk(64 * kDepWaves);
+    hipEvent_t stop = nullptr;
+    time_begin(c, 0, &stop);
+    switch (c->g.M) {
+    case 0: hipLaunchKernelGGL(k_amjdeposit<0>, grid, blk, 0, c->stream, a); break;
+    case 1: hipLaunchKernelGGL(k_amjdeposit<1>, grid, blk, 0, c->stream, a); break;
+    case 2: hipLaunchKernelGGL(k_amjdeposit<2>, grid, blk, 0, c->stream, a); break;
+    case 3: hipLaunchKernelGGL(k_amjdeposit<3>, grid, blk, 0, c->stream, a); break;
+    default: hipLaunchKernelGGL(k_amjdeposit<4>, grid, blk, 0, c->stream, a); break;
+    }
+    QPAD_HIP_CHECK(hipGetLastError());
+    time_end(c, 0, stop);
+    return 0;
+}
+
+// ---------------------------------------------------------------------------
+// push_u_robust (part2d_class.f03:1879-1965) + push_x (:2221-2262) + the new
+// sort key (update_bound :2307-2353 is the key nr).  Thread per particle;
+// particles past npp only refresh their sort index.
+// ---------------------------------------------------------------------------
+struct PushArgs {
+    int nr, ld, nm;
+    int64_t nslots;
+    double idr, edge, qtmh, dt;
+    double *x1, *x2, *p1, *p2, *p3, *gamma;
+    const int32_t *cs;
+    uint32_t *key, *idx;
+    const double *e, *b;
+};
+
+template <int M>
+__global__ __launch_bounds__(256) void k_push(PushArgs a)
+{
+    const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= a.nslots) return;
+    a.idx[p] = (uint32_t)p;
+    const int npp = a.cs[a.nr];
+    if (p >= npp) return;
+    const int nm = a.nm, ld = a.ld;
+    double x1 = a.x1[p], x2 = a.x2[p];
+    double r = radius(x1, x2);
+    const double pc = x1 / r, ps = x2 / r;
+    double pos = __dmul_rn(r, a.idr);
+    if (!(pos >= 0.0)) pos = 0.0;   // r = 0 or NaN: keep the gather in bounds
+    int ic = (int)fmin(pos, (double)(a.nr - 1));
+    const int ix = ic + 1;
+    const double w1 = pos - (double)ic, w0 = 1.0 - w1;
+    double ep[3], bp[3];
+    gather3<M>(a.e, nm, ld, ix, w0, w1, pc, ps, ep);
+    gather3<M>(a.b, nm, ld, ix, w0, w1, pc, ps, bp);
+    double t = ep[0] * pc - ep[1] * ps;
+    ep[1] = ep[0] * ps + ep[1] * pc;
+    ep[0] = t;
+    t = bp[0] * pc - bp[1] * ps;
+    bp[1] = bp[0] * ps + bp[1] * pc;
+    bp[0] = t;
+    double pp[3] = {a.p1[p], a.p2[p], a.p3[p]}, ut[3];
+    double gam = sqrt(1.0 + pp[0] * pp[0] + pp[1] * pp[1] + pp[2] * pp[2]);
+    double qtmh1 = a.qtmh / (gam - pp[2]);
+    double qtmh2 = qtmh1 * gam;
+#pragma unroll
+    for (int cc = 0; cc < 3; cc++) {
+        ep[cc] = ep[cc] * qtmh2;
+        bp[cc] = bp[cc] * qtmh1;
+        ut[cc] = pp[cc] + ep[cc];
+    }
+    pp[0] = ut[0] + ut[1] * bp[2] - ut[2] * bp[1];
+    pp[1] = ut[1] + ut[2] * bp[0] - ut[0] * bp[2];
+    pp[2] = ut[2] + ut[0] * bp[1] - ut[1] * bp[0];
+    double ostq = 2.0 / (1.0 + bp[0] * bp[0] + bp[1] * bp[1] + bp[2] * bp[2]);
+#pragma unroll
+    for (int cc = 0; cc < 3; cc++) bp[cc] = bp[cc] * ostq;
+    ut[0] = ut[0] + pp[1] * bp[2] - pp[2] * bp[1];
+    ut[1] = ut[1] + pp[2] * bp[0] - pp[0] * bp[2];
+    ut[2] = ut[2] + pp[0] * bp[1] - pp[1] * bp[0];
+#pragma unroll
+    for (int cc = 0; cc < 3; cc++) pp[cc] = ut[cc] + ep[cc];
+    double g2 = sqrt(1.0 + pp[0] * pp[0] + pp[1] * pp[1] + pp[2] * pp[2]);
+    double dtc = a.dt / (g2 - pp[2]);
+    x1 = x1 + pp[0] * dtc;
+    x2 = x2 + pp[1] * dtc;
+    a.p1[p] = pp[0];
+    a.p2[p] = pp[1];
+    a.p3[p] = pp[2];
+    a.gamma[p] = g2;
+    a.x1[p] = x1;
+    a.x2[p] = x2;
+    a.key[p] = cell_key(x1, x2, a.idr, a.edge, a.nr);
+}
+
+int launch_push(qpad_ctx *c, int s)
+{
+    Species &sp = c->sp[s];
+    if (sp.npp == 0) return 0;
+    PushArgs a;
+    a.nr = c->g.nr;
+    a.ld = c->g.ld;
+    a.nm = c->g.nm;
+    a.nslots = sp.npp;
+    a.idr = 1.0 / c->g.dr;
+    a.edge = (double)c->g.nr * c->g.dr;
+    a.qtmh = sp.qbm * c->g.dxi * 0.5;
+    a.dt = c->g.dxi;
+    a.x1 = sp.x1; a.x2 = sp.x2; a.p1 = sp.p1; a.p2 = sp.p2; a.p3 = sp.p3; a.gamma = sp.gamma;
+    a.cs = sp.cell_start;
+    a.key = sp.key;
+    a.idx = sp.idx;
+    a.e = c->f[QPAD_F_E];
+    a.b = c->f[QPAD_F_B];
+    dim3 grid((unsigned)((sp.npp + 255) / 256)), blk(256);
+    hipEvent_t stop = nullptr;
+    time_begin(c, 1, &stop);
+    switch (c->g.M) {
+    case 0: hipLaunchKernelGGL(k_push<0>, grid, blk, 0, c->stream, a); break;
+    case 1: hipLaunchKernelGGL(k_push<1>, grid, blk, 0, c->stream, a); break;
+    case 2: hipLaunchKernelGGL(k_push<2>, grid, blk, 0, c->stream, a); break;
+    case 3: hipLaunchKernelGGL(k_push<3>, grid, blk, 0, c->stream, a); break;
+    default: hipLaunchKernelGGL(k_push<4>, grid, blk, 0, c->stream, a); break;
+    }
+    QPAD_HIP_CHECK(hipGetLastError());
+    time_end(c, 1, stop);
+    return 0;
+}
+
+// q_spe += qn (the neutralising background of species s)
+__global__ void k_axpy(double *y, const double *x, int64_t n)
+{
+    int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < n) y[t] = y[t] + x[t];
+}
+
+int launch_add_qn_into_qspe(qpad_ctx *c, int s)
+{
+    int64_t n = (int64_t)c->g.nm * c->g.ld;
+    hipLaunchKernelGGL(k_axpy, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, c->stream, c->f[QPAD_F_Q_SPE],
+                       c->sp[s].qn, n);
+    QPAD_HIP_CHECK(hipGetLastError());
+    return 0;
+}
+
+}  // namespace qpad

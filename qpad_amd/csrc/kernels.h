@@ -1,0 +1,66 @@
+// kernels.h -- host-side launch wrappers of the HIP kernels (one stream per
+// context; every wrapper only enqueues work).
+#pragma once
+
+#include "qpad_internal.h"
+
+namespace qpad {
+
+// ---- fields (k_fields.hip) ----
+// psi from the q deposit partials (+ background) and Bz of b_spe from cu;
+// also 'record' of the convergence tester and reset of the iteration state.
+int launch_psi_bz(qpad_ctx *c, bool do_psi, bool do_bz, bool do_record);
+// psi from the already assembled q_spe (field_psi%solve)
+int launch_psi_from_qspe(qpad_ctx *c);
+// b = b_spe + b_beam(j), Ez of e from cu, E_perp of e from b and psi.
+// final = slice epilogue: q_spe += cu_z, e_spe, history copies of slice j.
+int launch_ez_et(qpad_ctx *c, int j, bool final, bool check_done);
+// fine-grained field pieces of the drop-in API
+int launch_ez_only(qpad_ctx *c, int dst, int jay);
+int launch_et_only(qpad_ctx *c, int dst, int bsrc);
+int launch_bz_only(qpad_ctx *c, int dst, int jay);
+int launch_add_b(qpad_ctx *c, int j);
+// cu, acu, amu from the amjdeposit partials of every species
+int launch_combine_j(qpad_ctx *c, bool check_done);
+// add one species' cu, acu, amu into the totals (species2d%amjdp)
+int launch_combine_j_species(qpad_ctx *c, int sp);
+// dcu, B_perp (B+/B-), Bz of b_spe, convergence 'compare' (last block)
+int launch_bperp_bz(qpad_ctx *c, bool do_bperp, bool do_bz, bool do_compare, bool check_done);
+int launch_djdxi_only(qpad_ctx *c);
+// beam B_perp of every slice 1..nxi from the q_beam history
+int launch_bt_beam_all(qpad_ctx *c);
+// beam B_perp of slice j into the b_beam slice field
+int launch_bt_beam_one(qpad_ctx *c, int j);
+// convergence_tester 'compare' alone
+int launch_compare_only(qpad_ctx *c);
+// J_perp predictor: dcu *= dxi ; cu(1:2) += dcu(1:2)
+int launch_predictor(qpad_ctx *c);
+
+// ---- plasma (k_plasma.hip) ----
+int launch_inject(qpad_ctx *c, int sp);
+int launch_sort(qpad_ctx *c, int sp);            // sort by key, permute, cell starts
+int launch_keys(qpad_ctx *c, int sp);            // keys of the current positions
+int launch_qdeposit(qpad_ctx *c, int sp);        // per-cell partials of rho - Jz
+int launch_combine_q(qpad_ctx *c, int sp, double *out, double scale, bool accumulate);
+int launch_amjdeposit(qpad_ctx *c, int sp, bool check_done, bool store_diag);
+int launch_push(qpad_ctx *c, int sp);            // push_u + push_x + keys
+int launch_add_qn_into_qspe(qpad_ctx *c, int sp);
+
+// ---- beam (k_beam.hip) ----
+int launch_beam_inject(qpad_ctx *c, int ib);
+int launch_beam_sort(qpad_ctx *c, int ib);
+int launch_beam_keys(qpad_ctx *c, int ib);
+int launch_beam_qdeposit(qpad_ctx *c);           // all beams into h_q_beam
+int launch_beam_push(qpad_ctx *c, int ib);
+
+// ---- generic ----
+int field_kernels_init(size_t dyn_lds);
+int sort_scratch(size_t n, int bits, size_t *bytes);
+int launch_fill(qpad_ctx *c, double *p, size_t n, double v);
+int launch_copy(qpad_ctx *c, double *dst, const double *src, size_t n);
+
+// event timing of dominant kernels
+void time_begin(qpad_ctx *c, int which, hipEvent_t *ev_stop_out);
+void time_end(qpad_ctx *c, int which, hipEvent_t ev_stop);
+
+}  // namespace qpad

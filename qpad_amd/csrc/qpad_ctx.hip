@@ -1,0 +1,876 @@
+// qpad_ctx.cpp -- host side of libqpad_hip.so: context lifetime, solver
+// matrices, the per-slice / per-3D-step orchestration and the C ABI
+// declared in include/qpad_hip.h.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "kernels.h"
+
+namespace qpad {
+
+static thread_local std::string g_err;
+
+void set_error(const std::string &msg) { g_err = msg; }
+
+static int fail(int code, const std::string &msg)
+{
+    g_err = msg;
+    return code;
+}
+
+#define TRY(expr)                 \
+    do {                          \
+        int rc_ = (expr);         \
+        if (rc_ != 0) return rc_; \
+    } while (0)
+
+// ---------------------------------------------------------------------------
+// Tridiagonal matrices of field_solver (source/fields/field_solver_class.f03:
+// 256-561): rows 0..nr-1 = grid points 1..nr of one radial partition
+// (noff = 0).  k = m for psi/Ez/Bz/beam bt, m+1 for B+, m-1 for B-; B+/B-
+// carry -relax_fac on the diagonal; open outer boundary folds
+// x_{N+1} = coef * x_N into the diagonal.
+// ---------------------------------------------------------------------------
+static void build_matrix(int kind, int m, int nr, double dr, double relax, int bnd, double *a, double *b,
+                         double *c)
+{
+    const double dr2 = dr * dr;
+    const double m2 = (double)(m * m);
+    for (int r = 1; r < nr; r++) {
+        const double j = (double)r;
+        a[r] = 1.0 - 0.5 / j;
+        c[r] = 1.0 + 0.5 / j;
+        if (kind == FK_BPLUS) {
+            const double t = (double)(m + 1) / j;
+            b[r] = -2.0 - t * t - relax;
+        } else if (kind == FK_BMINUS) {
+            const double t = (double)(m - 1) / j;
+            b[r] = -2.0 - t * t - relax;
+        } else {
+            b[r] = -2.0 - m2 / (j * j);
+        }
+    }
+    bool identity;
+    if (kind == FK_BPLUS) {
+        identity = true;
+    } else if (kind == FK_BMINUS) {
+        identity = (m != 1);
+        if (!identity) { a[0] = 0.0; b[0] = -4.0 - relax; c[0] = 4.0; }
+    } else {
+        identity = (m != 0);
+        if (!identity) { a[0] = 0.0; b[0] = -4.0; c[0] = 4.0; }
+    }
+    if (identity) {
+        // arbitrary non-singular first row; a(2) = 0 decouples the axis
+        a[0] = 0.0; b[0] = 1.0; c[0] = 0.0;
+        if (nr > 1) a[1] = 0.0;
+    }
+    const int rl = nr - 1;
+    if (bnd == QPAD_BND_ZERO) {
+        c[rl] = 0.0;
+    } else {
+        const double jmax = (double)nr;
+        double coef = 0.0;
+        bool fold = true;
+        switch (kind) {
+        case FK_PSI:
+        case FK_EZ:
+        case FK_BZ:
+            if (m == 0) fold = false;
+            else coef = 1.0 - (double)m / jmax;
+            break;
+        case FK_BT:
+            coef = (m == 0) ? 1.0 + 1.0 / (jmax * std::log(jmax * dr)) : 1.0 - (double)m / jmax;
+            break;
+        default:  // B+ and B- both fold with m+1
+            coef = 1.0 - (double)(m + 1) / jmax;
+            break;
+        }
+        if (fold) b[rl] = b[rl] + coef * c[rl];
+        c[rl] = 0.0;
+    }
+    for (int r = 0; r < nr; r++) {
+        a[r] = a[r] / dr2;
+        b[r] = b[r] / dr2;
+        c[r] = c[r] / dr2;
+    }
+}
+
+static void thomas_factor(int n, const double *a, const double *b, const double *c, double *lo, double *inv,
+                          double *cp)
+{
+    inv[0] = 1.0 / b[0];
+    cp[0] = c[0] * inv[0];
+    lo[0] = 0.0;
+    for (int r = 1; r < n; r++) {
+        const double den = b[r] - a[r] * cp[r - 1];
+        inv[r] = 1.0 / den;
+        cp[r] = c[r] * inv[r];
+        lo[r] = a[r];
+    }
+}
+
+// ---------------------------------------------------------------------------
+// event timing of the dominant kernels
+// ---------------------------------------------------------------------------
+static void pool_drain(qpad_ctx *c, EventPool &p)
+{
+    if (p.used == 0) return;
+    (void)hipEventSynchronize(p.ev[2 * p.used - 1]);
+    for (size_t i = 0; i < p.used; i++) {
+        float ms = 0.0f;
+        if (hipEventElapsedTime(&ms, p.ev[2 * i], p.ev[2 * i + 1]) == hipSuccess) p.acc_ms += ms;
+    }
+    p.used = 0;
+}
+
+void time_begin(qpad_ctx *c, int which, hipEvent_t *stop)
+{
+    *stop = nullptr;
+    if (!c->timing) return;
+    EventPool &p = c->pools[which];
+    if (p.used * 2 + 2 > p.ev.size()) {
+        if (p.ev.size() < 8192) {
+            size_t old = p.ev.size();
+            p.ev.resize(old + 512);
+            for (size_t i = old; i < p.ev.size(); i++) (void)hipEventCreate(&p.ev[i]);
+        } else {
+            pool_drain(c, p);
+        }
+    }
+    (void)hipEventRecord(p.ev[2 * p.used], c->stream);
+    *stop = p.ev[2 * p.used + 1];
+}
+
+void time_end(qpad_ctx *c, int which, hipEvent_t stop)
+{
+    if (!stop) return;
+    EventPool &p = c->pools[which];
+    (void)hipEventRecord(stop, c->stream);
+    p.used++;
+    p.launches++;
+}
+
+// ---------------------------------------------------------------------------
+// allocation helpers
+// ---------------------------------------------------------------------------
+template <class T>
+static int dalloc(T **p, size_t n)
+{
+    *p = nullptr;
+    if (n == 0) n = 1;
+    hipError_t e = hipMalloc((void **)p, n * sizeof(T));
+    if (e != hipSuccess) {
+        *p = nullptr;
+        return fail(QPAD_E_NOMEM, std::string("hipMalloc failed: ") + hipGetErrorString(e));
+    }
+    return 0;
+}
+
+template <class T>
+static void dfree(T *&p)
+{
+    if (p) (void)hipFree((void *)p);
+    p = nullptr;
+}
+
+static int zero_dev(qpad_ctx *c, double *p, size_t n) { return launch_fill(c, p, n, 0.0); }
+
+static size_t f1_size(const qpad_ctx *c, int ncomp) { return (size_t)ncomp * c->g.nm * c->g.ld; }
+
+static int sync(qpad_ctx *c)
+{
+    QPAD_HIP_CHECK(hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+static int refresh_npp(qpad_ctx *c, int s)
+{
+    Species &sp = c->sp[s];
+    int32_t v = 0;
+    QPAD_HIP_CHECK(hipMemcpyAsync(&v, sp.cell_start + c->g.nr, sizeof(int32_t), hipMemcpyDeviceToHost, c->stream));
+    TRY(sync(c));
+    sp.npp = v;
+    return 0;
+}
+
+static int refresh_beam_npp(qpad_ctx *c, int b)
+{
+    Beam &bm = c->bm[b];
+    int32_t v = 0;
+    size_t ncell = (size_t)c->g.nr * c->g.nxi;
+    QPAD_HIP_CHECK(hipMemcpyAsync(&v, bm.cell_start + ncell, sizeof(int32_t), hipMemcpyDeviceToHost, c->stream));
+    TRY(sync(c));
+    bm.npp = v;
+    return 0;
+}
+
+// species charge setup after (re)injection: sort, background qn = -q
+static int species_charge_setup(qpad_ctx *c, int s)
+{
+    TRY(launch_sort(c, s));
+    TRY(refresh_npp(c, s));
+    TRY(launch_qdeposit(c, s));
+    if (c->sp[s].cfg.neutralized) TRY(launch_combine_q(c, s, c->sp[s].qn, -1.0, false));
+    return 0;
+}
+
+}  // namespace qpad
+
+using namespace qpad;
+
+extern "C" {
+
+const char *qpad_last_error(void) { return g_err.c_str(); }
+
+int qpad_abi_version(void) { return QPAD_HIP_ABI_VERSION; }
+
+int qpad_create(const qpad_grid *gr, qpad_ctx **out)
+{
+    if (!gr || !out) return fail(QPAD_E_INVALID, "null argument");
+    *out = nullptr;
+    if (gr->nr < 8) return fail(QPAD_E_INVALID, "nr must be >= 8");
+    if (gr->nxi < 1) return fail(QPAD_E_INVALID, "nxi must be >= 1");
+    if (gr->max_mode < 0 || gr->max_mode > kMaxMode)
+        return fail(QPAD_E_INVALID, "max_mode must be in 0..4");
+    if (!(gr->dr > 0.0) || !(gr->dxi > 0.0)) return fail(QPAD_E_INVALID, "dr and dxi must be > 0");
+    if (gr->bnd != QPAD_BND_OPEN && gr->bnd != QPAD_BND_ZERO)
+        return fail(QPAD_E_INVALID, "field_boundary must be open or zero");
+    if (gr->iter_max < 1) return fail(QPAD_E_INVALID, "iter_max must be >= 1");
+    int dev = 0;
+    QPAD_HIP_CHECK(hipGetDevice(&dev));
+    hipDeviceProp_t prop;
+    QPAD_HIP_CHECK(hipGetDeviceProperties(&prop, dev));
+    size_t lds_need = sizeof(double) * 4 * (size_t)gr->nr + 1024;
+    if (lds_need > prop.sharedMemPerBlock)
+        return fail(QPAD_E_INVALID, "nr too large for the LDS-resident B_perp solve");
+
+    if (field_kernels_init(lds_need) != 0) return QPAD_E_HIP;
+    qpad_ctx *c = new qpad_ctx{};
+    c->device = dev;
+    Grid &g = c->g;
+    g.nr = gr->nr;
+    g.nxi = gr->nxi;
+    g.M = gr->max_mode;
+    g.nm = 2 * gr->max_mode + 1;
+    g.ld = ((gr->nr + 2) + 7) & ~7;
+    g.dr = gr->dr;
+    g.dxi = gr->dxi;
+    g.idr = 1.0 / gr->dr;
+    g.relax_fac = gr->relax_fac;
+    g.dt = gr->dt;
+    g.bnd = gr->bnd;
+    g.iter_max = gr->iter_max;
+    g.reltol = gr->iter_reltol;
+    g.abstol = gr->iter_abstol;
+    int rc = 0;
+    hipError_t he = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+    if (he != hipSuccess) {
+        delete c;
+        return fail(QPAD_E_HIP, std::string("hipStreamCreate: ") + hipGetErrorString(he));
+    }
+    static const int ncomp[QPAD_NFIELD] = {1, 3, 3, 3, 3, 3, 3, 2, 3, 2, 1, 1};
+    for (int f = 0; f < QPAD_NFIELD && !rc; f++) {
+        c->ncomp[f] = ncomp[f];
+        rc = dalloc(&c->f[f], f1_size(c, ncomp[f]));
+    }
+    const size_t ns = (size_t)g.nxi + 2;
+    if (!rc) rc = dalloc(&c->h_psi, ns * f1_size(c, 1));
+    if (!rc) rc = dalloc(&c->h_e, ns * f1_size(c, 3));
+    if (!rc) rc = dalloc(&c->h_b, ns * f1_size(c, 3));
+    if (!rc) rc = dalloc(&c->h_b_spe, ns * f1_size(c, 3));
+    if (!rc) rc = dalloc(&c->h_e_spe, ns * f1_size(c, 3));
+    if (!rc) rc = dalloc(&c->h_cu, ns * f1_size(c, 3));
+    if (!rc) rc = dalloc(&c->h_q_spe, ns * f1_size(c, 1));
+    if (!rc) rc = dalloc(&c->h_q_beam, ns * f1_size(c, 1));
+    if (!rc) rc = dalloc(&c->h_b_beam, ns * f1_size(c, 3));
+    if (!rc) rc = dalloc(&c->conv_re, (size_t)g.nr + 2);
+    if (!rc) rc = dalloc(&c->conv_im, (size_t)g.nr + 2);
+    if (!rc) rc = dalloc(&c->it, 1);
+    size_t nfac = (size_t)FK_N * (kMaxMode + 1) * 3 * g.nr;
+    if (!rc) rc = dalloc(&c->fac, nfac);
+    if (!rc) {
+        he = hipHostMalloc((void **)&c->it_host, sizeof(IterState), hipHostMallocDefault);
+        if (he != hipSuccess) rc = fail(QPAD_E_NOMEM, "hipHostMalloc failed");
+    }
+    if (rc) {
+        qpad_destroy(c);
+        return rc;
+    }
+    // solver factors
+    std::vector<double> hf(nfac, 0.0), a(g.nr), b(g.nr), cc(g.nr);
+    for (int kind = 0; kind < FK_N; kind++)
+        for (int m = 0; m <= g.M; m++) {
+            build_matrix(kind, m, g.nr, g.dr, g.relax_fac, g.bnd, a.data(), b.data(), cc.data());
+            double *base = hf.data() + (((size_t)kind * (kMaxMode + 1) + m) * 3) * g.nr;
+            thomas_factor(g.nr, a.data(), b.data(), cc.data(), base, base + g.nr, base + 2 * g.nr);
+        }
+    rc = 0;
+    he = hipMemcpyAsync(c->fac, hf.data(), nfac * sizeof(double), hipMemcpyHostToDevice, c->stream);
+    if (he != hipSuccess) rc = fail(QPAD_E_HIP, "upload of solver factors failed");
+    // zero every array once (fill kernels, not memset)
+    for (int f = 0; f < QPAD_NFIELD && !rc; f++) rc = zero_dev(c, c->f[f], f1_size(c, c->ncomp[f]));
+    double *hs[] = {c->h_psi, c->h_q_spe, c->h_q_beam};
+    for (double *h : hs)
+        if (!rc) rc = zero_dev(c, h, ns * f1_size(c, 1));
+    double *hs3[] = {c->h_e, c->h_b, c->h_b_spe, c->h_e_spe, c->h_cu, c->h_b_beam};
+    for (double *h : hs3)
+        if (!rc) rc = zero_dev(c, h, ns * f1_size(c, 3));
+    if (!rc) rc = zero_dev(c, c->conv_re, (size_t)g.nr + 2);
+    if (!rc) rc = zero_dev(c, c->conv_im, (size_t)g.nr + 2);
+    if (!rc) {
+        memset(c->it_host, 0, sizeof(IterState));
+        he = hipMemcpyAsync(c->it, c->it_host, sizeof(IterState), hipMemcpyHostToDevice, c->stream);
+        if (he != hipSuccess) rc = fail(QPAD_E_HIP, "iteration state upload failed");
+    }
+    if (!rc) rc = sync(c);
+    if (rc) {
+        qpad_destroy(c);
+        return rc;
+    }
+    *out = c;
+    return 0;
+}
+
+int qpad_destroy(qpad_ctx *c)
+{
+    if (!c) return 0;
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    for (int f = 0; f < QPAD_NFIELD; f++) dfree(c->f[f]);
+    dfree(c->h_psi); dfree(c->h_e); dfree(c->h_b); dfree(c->h_b_spe); dfree(c->h_e_spe);
+    dfree(c->h_cu); dfree(c->h_q_spe); dfree(c->h_q_beam); dfree(c->h_b_beam);
+    dfree(c->conv_re); dfree(c->conv_im); dfree(c->it); dfree(c->fac);
+    if (c->it_host) (void)hipHostFree(c->it_host);
+    for (int s = 0; s < c->nspecies; s++) {
+        Species &sp = c->sp[s];
+        dfree(sp.x1); dfree(sp.x2); dfree(sp.p1); dfree(sp.p2); dfree(sp.p3); dfree(sp.q);
+        dfree(sp.gamma); dfree(sp.psi); dfree(sp.id);
+        dfree(sp.a_x1); dfree(sp.a_x2); dfree(sp.a_p1); dfree(sp.a_p2); dfree(sp.a_p3); dfree(sp.a_q);
+        dfree(sp.a_gamma); dfree(sp.a_psi); dfree(sp.a_id);
+        dfree(sp.key); dfree(sp.key_alt); dfree(sp.idx); dfree(sp.idx_alt);
+        dfree(sp.cell_start); dfree(sp.qn); dfree(sp.P0); dfree(sp.P1);
+        if (sp.sort_tmp) (void)hipFree(sp.sort_tmp);
+    }
+    for (int b = 0; b < c->nbeams; b++) {
+        Beam &bm = c->bm[b];
+        dfree(bm.x1); dfree(bm.x2); dfree(bm.x3); dfree(bm.p1); dfree(bm.p2); dfree(bm.p3); dfree(bm.q);
+        dfree(bm.a_x1); dfree(bm.a_x2); dfree(bm.a_x3); dfree(bm.a_p1); dfree(bm.a_p2); dfree(bm.a_p3);
+        dfree(bm.a_q);
+        dfree(bm.key); dfree(bm.key_alt); dfree(bm.idx); dfree(bm.idx_alt); dfree(bm.cell_start);
+        if (bm.sort_tmp) (void)hipFree(bm.sort_tmp);
+    }
+    for (auto &p : c->pools)
+        for (auto e : p.ev) (void)hipEventDestroy(e);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
+    delete c;
+    return 0;
+}
+
+
+int qpad_species_new(qpad_ctx *c, const qpad_species_cfg *cfg, int32_t *sp_id)
+{
+    if (!c || !cfg) return fail(QPAD_E_INVALID, "null argument");
+    if (c->nspecies >= kMaxSpecies) return fail(QPAD_E_INVALID, "too many species");
+    if (cfg->ppc1 < 1 || cfg->ppc2 < 1 || cfg->num_theta < 1)
+        return fail(QPAD_E_INVALID, "ppc and num_theta must be >= 1");
+    if (cfg->push_type != QPAD_PUSH2_ROBUST)
+        return fail(QPAD_E_INVALID, "only push_type 'robust' is implemented on the GPU path");
+    if (cfg->mass == 0.0 || cfg->qm == 0.0) return fail(QPAD_E_INVALID, "q and m must be non-zero");
+    const int s = c->nspecies;
+    Species &sp = c->sp[s];
+    memset((void *)&sp, 0, sizeof(Species));
+    sp.cfg = *cfg;
+    sp.qbm = cfg->qm / cfg->mass;
+    // npmax: fdist2d_class.f03:258-262 (xtra = 2)
+    const int64_t nlat = (int64_t)c->g.nr * cfg->ppc1 * cfg->ppc2 * cfg->num_theta;
+    if (nlat > 0x7fffffff / 2) return fail(QPAD_E_INVALID, "too many particles per species");
+    sp.npmax = 2 * nlat;
+    const size_t n = (size_t)sp.npmax;
+    c->nspecies++;
+    int rc = 0;
+    double **dp[] = {&sp.x1, &sp.x2, &sp.p1, &sp.p2, &sp.p3, &sp.q, &sp.gamma, &sp.psi,
+                     &sp.a_x1, &sp.a_x2, &sp.a_p1, &sp.a_p2, &sp.a_p3, &sp.a_q, &sp.a_gamma, &sp.a_psi};
+    for (double **p : dp)
+        if (!rc) rc = dalloc(p, n);
+    if (!rc) rc = dalloc(&sp.id, n);
+    if (!rc) rc = dalloc(&sp.a_id, n);
+    if (!rc) rc = dalloc(&sp.key, n);
+    if (!rc) rc = dalloc(&sp.key_alt, n);
+    if (!rc) rc = dalloc(&sp.idx, n);
+    if (!rc) rc = dalloc(&sp.idx_alt, n);
+    if (!rc) rc = dalloc(&sp.cell_start, (size_t)c->g.nr + 2);
+    if (!rc) rc = dalloc(&sp.qn, f1_size(c, 1));
+    if (!rc) rc = dalloc(&sp.P0, (size_t)c->g.nr * 8 * c->g.nm);
+    if (!rc) rc = dalloc(&sp.P1, (size_t)c->g.nr * 8 * c->g.nm);
+    if (!rc) {
+        int bits = 1;
+        while ((1u << bits) <= (uint32_t)c->g.nr) bits++;
+        rc = sort_scratch(n, bits, &sp.sort_tmp_bytes);
+        if (!rc) {
+            hipError_t e = hipMalloc(&sp.sort_tmp, sp.sort_tmp_bytes);
+            if (e != hipSuccess) rc = fail(QPAD_E_NOMEM, "sort scratch allocation failed");
+        }
+    }
+    if (!rc) rc = zero_dev(c, sp.qn, f1_size(c, 1));
+    if (!rc) rc = launch_inject(c, s);
+    if (!rc) rc = species_charge_setup(c, s);
+    if (!rc) rc = sync(c);
+    if (rc) return rc;
+    if (sp_id) *sp_id = s;
+    return 0;
+}
+
+int qpad_species_renew(qpad_ctx *c, int32_t s)
+{
+    if (!c || s < 0 || s >= c->nspecies) return fail(QPAD_E_INVALID, "bad species");
+    TRY(launch_inject(c, s));
+    TRY(species_charge_setup(c, s));
+    return sync(c);
+}
+
+int qpad_beam_new(qpad_ctx *c, const qpad_beam_cfg *cfg, int32_t *beam_id)
+{
+    if (!c || !cfg) return fail(QPAD_E_INVALID, "null argument");
+    if (c->nbeams >= kMaxBeams) return fail(QPAD_E_INVALID, "too many beams");
+    if (cfg->mass == 0.0) return fail(QPAD_E_INVALID, "beam mass must be non-zero");
+    if (cfg->total_num < 0 || cfg->total_num > 0x3fffffff) return fail(QPAD_E_INVALID, "bad total_num");
+    for (int d = 0; d < 3; d++)
+        if (!(cfg->sigma[d] > 0.0) || !(cfg->range_hi[d] > cfg->range_lo[d]))
+            return fail(QPAD_E_INVALID, "beam sigma and ranges must be positive");
+    if ((int64_t)c->g.nr * c->g.nxi >= 0x7fffffff) return fail(QPAD_E_INVALID, "grid too large for beam cells");
+    const int b = c->nbeams;
+    Beam &bm = c->bm[b];
+    memset((void *)&bm, 0, sizeof(Beam));
+    bm.cfg = *cfg;
+    bm.qbm = cfg->qm / cfg->mass;
+    bm.npmax = std::max<int64_t>(1, cfg->total_num * (cfg->quiet ? 2 : 1));
+    c->nbeams++;
+    const size_t n = (size_t)bm.npmax;
+    int rc = 0;
+    double **dp[] = {&bm.x1, &bm.x2, &bm.x3, &bm.p1, &bm.p2, &bm.p3, &bm.q,
+                     &bm.a_x1, &bm.a_x2, &bm.a_x3, &bm.a_p1, &bm.a_p2, &bm.a_p3, &bm.a_q};
+    for (double **p : dp)
+        if (!rc) rc = dalloc(p, n);
+    if (!rc) rc = dalloc(&bm.key, n);
+    if (!rc) rc = dalloc(&bm.key_alt, n);
+    if (!rc) rc = dalloc(&bm.idx, n);
+    if (!rc) rc = dalloc(&bm.idx_alt, n);
+    if (!rc) rc = dalloc(&bm.cell_start, (size_t)c->g.nr * c->g.nxi + 2);
+    if (!rc) {
+        int bits = 1;
+        while ((1u << bits) <= (uint32_t)(c->g.nr * c->g.nxi)) bits++;
+        rc = sort_scratch(n, bits, &bm.sort_tmp_bytes);
+        if (!rc) {
+            hipError_t e = hipMalloc(&bm.sort_tmp, bm.sort_tmp_bytes);
+            if (e != hipSuccess) rc = fail(QPAD_E_NOMEM, "beam sort scratch allocation failed");
+        }
+    }
+    if (!rc) rc = launch_beam_inject(c, b);
+    if (!rc) rc = launch_beam_sort(c, b);
+    if (!rc) rc = refresh_beam_npp(c, b);
+    if (rc) return rc;
+    if (beam_id) *beam_id = b;
+    return 0;
+}
+
+// ---- fused step ----
+
+int qpad_step3d_begin(qpad_ctx *c)
+{
+    if (!c) return fail(QPAD_E_INVALID, "null context");
+    // q_beam%as(0); beam3d%qdp (simulation_class.f03:297-306)
+    for (int b = 0; b < c->nbeams; b++) TRY(launch_beam_sort(c, b));
+    if (c->nbeams > 0) TRY(launch_beam_qdeposit(c));
+    else TRY(zero_dev(c, c->h_q_beam, ((size_t)c->g.nxi + 2) * f1_size(c, 1)));
+    // beam B_perp for every slice (hoisted out of the slice loop: it depends
+    // on the q_beam history only)
+    TRY(launch_bt_beam_all(c));
+    TRY(zero_dev(c, c->h_q_spe, ((size_t)c->g.nxi + 2) * f1_size(c, 1)));
+    const int zf[] = {QPAD_F_B, QPAD_F_E, QPAD_F_B_SPE, QPAD_F_E_SPE, QPAD_F_PSI, QPAD_F_CU, QPAD_F_ACU,
+                      QPAD_F_AMU};
+    for (int f : zf) TRY(zero_dev(c, c->f[f], f1_size(c, c->ncomp[f])));
+    c->step_open = true;
+    return sync(c);
+}
+
+int qpad_slice(qpad_ctx *c, int32_t j, int32_t *iters_out)
+{
+    if (!c) return fail(QPAD_E_INVALID, "null context");
+    if (j < 1 || j > c->g.nxi) return fail(QPAD_E_INVALID, "slice index out of range");
+    // rho - Jz of every species, psi, Bz from the J predictor, 'record'
+    for (int s = 0; s < c->nspecies; s++) TRY(launch_qdeposit(c, s));
+    hipEvent_t stop = nullptr;
+    time_begin(c, 2, &stop);
+    TRY(launch_psi_bz(c, true, true, true));
+    time_end(c, 2, stop);
+    int iters = 0;
+    for (int l = 1; l <= c->g.iter_max; l++) {
+        TRY(launch_ez_et(c, j, false, false));
+        for (int s = 0; s < c->nspecies; s++) TRY(launch_amjdeposit(c, s, false, true));
+        TRY(launch_combine_j(c, false));
+        time_begin(c, 2, &stop);
+        TRY(launch_bperp_bz(c, true, true, true, false));
+        time_end(c, 2, stop);
+        QPAD_HIP_CHECK(hipMemcpyAsync(c->it_host, c->it, sizeof(IterState), hipMemcpyDeviceToHost, c->stream));
+        TRY(sync(c));
+        iters = c->it_host->iters;
+        if (c->it_host->done) break;
+    }
+    // slice epilogue: cbq, history, final E (simulation_class.f03:401-456)
+    TRY(launch_ez_et(c, j, true, false));
+    TRY(launch_predictor(c));
+    for (int s = 0; s < c->nspecies; s++) {
+        TRY(launch_push(c, s));
+        TRY(launch_sort(c, s));
+    }
+    TRY(sync(c));
+    if (iters_out) *iters_out = iters;
+    return 0;
+}
+
+int qpad_step3d_end(qpad_ctx *c)
+{
+    if (!c) return fail(QPAD_E_INVALID, "null context");
+    for (int b = 0; b < c->nbeams; b++) TRY(launch_beam_push(c, b));
+    for (int s = 0; s < c->nspecies; s++) {
+        TRY(launch_inject(c, s));
+        TRY(species_charge_setup(c, s));
+    }
+    c->step_open = false;
+    return sync(c);
+}
+
+int qpad_step3d(qpad_ctx *c, int32_t *total_iters_out)
+{
+    TRY(qpad_step3d_begin(c));
+    int64_t tot = 0;
+    for (int j = 1; j <= c->g.nxi; j++) {
+        int32_t it = 0;
+        TRY(qpad_slice(c, j, &it));
+        tot += it;
+    }
+    TRY(qpad_step3d_end(c));
+    if (total_iters_out) *total_iters_out = (int32_t)tot;
+    return 0;
+}
+
+// ---- fine-grained drop-in entry points ----
+
+int qpad_species_qdp(qpad_ctx *c, int32_t s)
+{
+    if (!c || s < 0 || s >= c->nspecies) return fail(QPAD_E_INVALID, "bad species");
+    TRY(launch_qdeposit(c, s));
+    TRY(launch_combine_q(c, s, c->f[QPAD_F_Q_SPE], 1.0, true));
+    if (c->sp[s].cfg.neutralized) TRY(launch_add_qn_into_qspe(c, s));
+    return sync(c);
+}
+
+int qpad_species_amjdp(qpad_ctx *c, int32_t s, double dxi)
+{
+    if (!c || s < 0 || s >= c->nspecies) return fail(QPAD_E_INVALID, "bad species");
+    if (dxi != c->g.dxi) return fail(QPAD_E_INVALID, "dxi differs from the grid's dxi");
+    TRY(launch_amjdeposit(c, s, false, true));
+    // add this species' cu/acu/amu into the totals
+    TRY(launch_combine_j_species(c, s));
+    return sync(c);
+}
+
+int qpad_species_push(qpad_ctx *c, int32_t s, double dxi)
+{
+    if (!c || s < 0 || s >= c->nspecies) return fail(QPAD_E_INVALID, "bad species");
+    if (dxi != c->g.dxi) return fail(QPAD_E_INVALID, "dxi differs from the grid's dxi");
+    TRY(launch_push(c, s));
+    TRY(launch_sort(c, s));
+    return sync(c);
+}
+
+int qpad_field_solve_psi(qpad_ctx *c)
+{
+    if (!c) return fail(QPAD_E_INVALID, "null context");
+    TRY(launch_psi_from_qspe(c));
+    return sync(c);
+}
+
+static bool valid_fid(int f) { return f >= 0 && f < QPAD_NFIELD; }
+
+int qpad_field_solve_ez(qpad_ctx *c, int32_t dst, int32_t jay)
+{
+    if (!c || !valid_fid(dst) || !valid_fid(jay) || c->ncomp[dst] != 3 || c->ncomp[jay] != 3)
+        return fail(QPAD_E_INVALID, "bad field ids");
+    TRY(launch_ez_only(c, dst, jay));
+    return sync(c);
+}
+
+int qpad_field_solve_et(qpad_ctx *c, int32_t dst, int32_t b)
+{
+    if (!c || !valid_fid(dst) || !valid_fid(b) || c->ncomp[dst] != 3 || c->ncomp[b] != 3)
+        return fail(QPAD_E_INVALID, "bad field ids");
+    TRY(launch_et_only(c, dst, b));
+    return sync(c);
+}
+
+int qpad_field_solve_bz(qpad_ctx *c, int32_t dst, int32_t jay)
+{
+    if (!c || !valid_fid(dst) || !valid_fid(jay) || c->ncomp[dst] != 3 || c->ncomp[jay] != 3)
+        return fail(QPAD_E_INVALID, "bad field ids");
+    TRY(launch_bz_only(c, dst, jay));
+    return sync(c);
+}
+
+int qpad_field_solve_bt_beam(qpad_ctx *c, int32_t j)
+{
+    if (!c || j < 1 || j > c->g.nxi) return fail(QPAD_E_INVALID, "bad slice");
+    TRY(launch_bt_beam_one(c, j));
+    return sync(c);
+}
+
+int qpad_field_solve_djdxi(qpad_ctx *c)
+{
+    if (!c) return fail(QPAD_E_INVALID, "null context");
+    TRY(launch_djdxi_only(c));
+    return sync(c);
+}
+
+int qpad_field_solve_bt_iter(qpad_ctx *c)
+{
+    if (!c) return fail(QPAD_E_INVALID, "null context");
+    TRY(launch_bperp_bz(c, true, false, false, false));
+    return sync(c);
+}
+
+int qpad_field_add_b(qpad_ctx *c)
+{
+    if (!c) return fail(QPAD_E_INVALID, "null context");
+    TRY(launch_add_b(c, 0));
+    return sync(c);
+}
+
+int qpad_convergence_tester(qpad_ctx *c, int32_t op, double *rel, double *ab)
+{
+    if (!c || (op != 0 && op != 1)) return fail(QPAD_E_INVALID, "bad convergence_tester operation");
+    if (op == 0) {
+        TRY(launch_psi_bz(c, false, false, true));
+        return sync(c);
+    }
+    TRY(launch_compare_only(c));
+    QPAD_HIP_CHECK(hipMemcpyAsync(c->it_host, c->it, sizeof(IterState), hipMemcpyDeviceToHost, c->stream));
+    TRY(sync(c));
+    if (rel) *rel = c->it_host->rel;
+    if (ab) *ab = c->it_host->abs_;
+    return 0;
+}
+
+int qpad_beam_qdp(qpad_ctx *c)
+{
+    if (!c) return fail(QPAD_E_INVALID, "null context");
+    for (int b = 0; b < c->nbeams; b++) TRY(launch_beam_sort(c, b));
+    if (c->nbeams > 0) TRY(launch_beam_qdeposit(c));
+    return sync(c);
+}
+
+int qpad_beam_push(qpad_ctx *c)
+{
+    if (!c) return fail(QPAD_E_INVALID, "null context");
+    for (int b = 0; b < c->nbeams; b++) TRY(launch_beam_push(c, b));
+    return sync(c);
+}
+
+// ---- state exchange ----
+
+static double *hist_ptr(qpad_ctx *c, int fid, int *ncomp)
+{
+    switch (fid) {
+    case QPAD_F_PSI: *ncomp = 1; return c->h_psi;
+    case QPAD_F_E: *ncomp = 3; return c->h_e;
+    case QPAD_F_B: *ncomp = 3; return c->h_b;
+    case QPAD_F_B_SPE: *ncomp = 3; return c->h_b_spe;
+    case QPAD_F_E_SPE: *ncomp = 3; return c->h_e_spe;
+    case QPAD_F_CU: *ncomp = 3; return c->h_cu;
+    case QPAD_F_Q_SPE: *ncomp = 1; return c->h_q_spe;
+    case QPAD_F_Q_BEAM: *ncomp = 1; return c->h_q_beam;
+    case QPAD_F_B_BEAM: *ncomp = 3; return c->h_b_beam;
+    default: return nullptr;
+    }
+}
+
+// packed host layout [ncomp][nm][nr+2] <-> padded device rows of length ld
+static int copy_rows(qpad_ctx *c, double *dev, int ncomp, double *host_out, const double *host_in)
+{
+    const size_t rows = (size_t)ncomp * c->g.nm, w = (size_t)c->g.nr + 2;
+    if (host_out) {
+        QPAD_HIP_CHECK(hipMemcpy2DAsync(host_out, w * sizeof(double), dev, c->g.ld * sizeof(double),
+                                        w * sizeof(double), rows, hipMemcpyDeviceToHost, c->stream));
+    } else {
+        QPAD_HIP_CHECK(hipMemcpy2DAsync(dev, c->g.ld * sizeof(double), host_in, w * sizeof(double),
+                                        w * sizeof(double), rows, hipMemcpyHostToDevice, c->stream));
+    }
+    return sync(c);
+}
+
+int qpad_get_field(qpad_ctx *c, int32_t fid, double *out)
+{
+    if (!c || !valid_fid(fid) || !out) return fail(QPAD_E_INVALID, "bad field id");
+    return copy_rows(c, c->f[fid], c->ncomp[fid], out, nullptr);
+}
+
+int qpad_set_field(qpad_ctx *c, int32_t fid, const double *in)
+{
+    if (!c || !valid_fid(fid) || !in) return fail(QPAD_E_INVALID, "bad field id");
+    return copy_rows(c, c->f[fid], c->ncomp[fid], nullptr, in);
+}
+
+int qpad_get_hist(qpad_ctx *c, int32_t fid, int32_t s, double *out)
+{
+    int nc = 0;
+    double *h = c ? hist_ptr(c, fid, &nc) : nullptr;
+    if (!h || !out || s < 0 || s > c->g.nxi + 1) return fail(QPAD_E_INVALID, "bad history request");
+    return copy_rows(c, h + (size_t)s * f1_size(c, nc), nc, out, nullptr);
+}
+
+int qpad_set_hist(qpad_ctx *c, int32_t fid, int32_t s, const double *in)
+{
+    int nc = 0;
+    double *h = c ? hist_ptr(c, fid, &nc) : nullptr;
+    if (!h || !in || s < 0 || s > c->g.nxi + 1) return fail(QPAD_E_INVALID, "bad history request");
+    return copy_rows(c, h + (size_t)s * f1_size(c, nc), nc, nullptr, in);
+}
+
+int qpad_species_count(qpad_ctx *c, int32_t s, int64_t *npp)
+{
+    if (!c || s < 0 || s >= c->nspecies || !npp) return fail(QPAD_E_INVALID, "bad species");
+    TRY(refresh_npp(c, s));
+    *npp = c->sp[s].npp;
+    return 0;
+}
+
+int qpad_get_species(qpad_ctx *c, int32_t s, double *x1, double *x2, double *p1, double *p2, double *p3,
+                     double *q, double *gamma, double *psi, int32_t *id)
+{
+    if (!c || s < 0 || s >= c->nspecies) return fail(QPAD_E_INVALID, "bad species");
+    TRY(refresh_npp(c, s));
+    Species &sp = c->sp[s];
+    const size_t n = (size_t)sp.npp;
+    double *outs[8] = {x1, x2, p1, p2, p3, q, gamma, psi};
+    const double *srcs[8] = {sp.x1, sp.x2, sp.p1, sp.p2, sp.p3, sp.q, sp.gamma, sp.psi};
+    for (int f = 0; f < 8; f++)
+        if (outs[f] && n) QPAD_HIP_CHECK(hipMemcpyAsync(outs[f], srcs[f], n * sizeof(double),
+                                                        hipMemcpyDeviceToHost, c->stream));
+    if (id && n) QPAD_HIP_CHECK(hipMemcpyAsync(id, sp.id, n * sizeof(int32_t), hipMemcpyDeviceToHost, c->stream));
+    return sync(c);
+}
+
+int qpad_set_species(qpad_ctx *c, int32_t s, int64_t n, const double *x1, const double *x2, const double *p1,
+                     const double *p2, const double *p3, const double *q, const double *gamma, const double *psi,
+                     const int32_t *id)
+{
+    if (!c || s < 0 || s >= c->nspecies) return fail(QPAD_E_INVALID, "bad species");
+    Species &sp = c->sp[s];
+    if (n < 0 || n > sp.npmax) return fail(QPAD_E_INVALID, "too many particles for this species");
+    if (!x1 || !x2 || !p1 || !p2 || !p3 || !q) return fail(QPAD_E_INVALID, "x, p and q are required");
+    const double *srcs[8] = {x1, x2, p1, p2, p3, q, gamma, psi};
+    double *dsts[8] = {sp.x1, sp.x2, sp.p1, sp.p2, sp.p3, sp.q, sp.gamma, sp.psi};
+    for (int f = 0; f < 8; f++) {
+        if (!n) break;
+        if (srcs[f]) QPAD_HIP_CHECK(hipMemcpyAsync(dsts[f], srcs[f], n * sizeof(double),
+                                                   hipMemcpyHostToDevice, c->stream));
+        else TRY(zero_dev(c, dsts[f], (size_t)n));
+    }
+    std::vector<int32_t> ids;
+    const int32_t *idp = id;
+    if (!idp) {
+        ids.resize((size_t)n);
+        for (int64_t i = 0; i < n; i++) ids[(size_t)i] = (int32_t)i;
+        idp = ids.data();
+    }
+    if (n) QPAD_HIP_CHECK(hipMemcpyAsync(sp.id, idp, n * sizeof(int32_t), hipMemcpyHostToDevice, c->stream));
+    TRY(sync(c));
+    sp.npp = n;
+    TRY(launch_keys(c, s));
+    TRY(launch_sort(c, s));
+    TRY(refresh_npp(c, s));
+    return 0;
+}
+
+int qpad_get_species_qn(qpad_ctx *c, int32_t s, double *out)
+{
+    if (!c || s < 0 || s >= c->nspecies || !out) return fail(QPAD_E_INVALID, "bad species");
+    return copy_rows(c, c->sp[s].qn, 1, out, nullptr);
+}
+
+int qpad_set_species_qn(qpad_ctx *c, int32_t s, const double *in)
+{
+    if (!c || s < 0 || s >= c->nspecies || !in) return fail(QPAD_E_INVALID, "bad species");
+    return copy_rows(c, c->sp[s].qn, 1, nullptr, in);
+}
+
+int qpad_beam_count(qpad_ctx *c, int32_t b, int64_t *npp)
+{
+    if (!c || b < 0 || b >= c->nbeams || !npp) return fail(QPAD_E_INVALID, "bad beam");
+    TRY(launch_beam_sort(c, b));
+    TRY(refresh_beam_npp(c, b));
+    *npp = c->bm[b].npp;
+    return 0;
+}
+
+int qpad_get_beam(qpad_ctx *c, int32_t b, double *x1, double *x2, double *x3, double *p1, double *p2, double *p3,
+                  double *q)
+{
+    if (!c || b < 0 || b >= c->nbeams) return fail(QPAD_E_INVALID, "bad beam");
+    TRY(launch_beam_sort(c, b));
+    TRY(refresh_beam_npp(c, b));
+    Beam &bm = c->bm[b];
+    const size_t n = (size_t)bm.npp;
+    double *outs[7] = {x1, x2, x3, p1, p2, p3, q};
+    const double *srcs[7] = {bm.x1, bm.x2, bm.x3, bm.p1, bm.p2, bm.p3, bm.q};
+    for (int f = 0; f < 7; f++)
+        if (outs[f] && n) QPAD_HIP_CHECK(hipMemcpyAsync(outs[f], srcs[f], n * sizeof(double),
+                                                        hipMemcpyDeviceToHost, c->stream));
+    return sync(c);
+}
+
+int qpad_set_beam(qpad_ctx *c, int32_t b, int64_t n, const double *x1, const double *x2, const double *x3,
+                  const double *p1, const double *p2, const double *p3, const double *q)
+{
+    if (!c || b < 0 || b >= c->nbeams) return fail(QPAD_E_INVALID, "bad beam");
+    Beam &bm = c->bm[b];
+    if (n < 0 || n > bm.npmax) return fail(QPAD_E_INVALID, "too many beam particles");
+    if (!x1 || !x2 || !x3 || !p1 || !p2 || !p3 || !q) return fail(QPAD_E_INVALID, "null beam array");
+    const double *srcs[7] = {x1, x2, x3, p1, p2, p3, q};
+    double *dsts[7] = {bm.x1, bm.x2, bm.x3, bm.p1, bm.p2, bm.p3, bm.q};
+    for (int f = 0; f < 7 && n; f++)
+        QPAD_HIP_CHECK(hipMemcpyAsync(dsts[f], srcs[f], n * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    TRY(sync(c));
+    bm.npp = n;
+    TRY(launch_beam_sort(c, b));
+    TRY(refresh_beam_npp(c, b));
+    return 0;
+}
+
+int qpad_kernel_time(qpad_ctx *c, int32_t which, double *ms, int64_t *launches)
+{
+    if (!c || which < 0 || which > 2) return fail(QPAD_E_INVALID, "bad timer id");
+    pool_drain(c, c->pools[which]);
+    if (ms) *ms = c->pools[which].acc_ms;
+    if (launches) *launches = c->pools[which].launches;
+    return 0;
+}
+
+int qpad_kernel_time_reset(qpad_ctx *c, int32_t enable)
+{
+    if (!c) return fail(QPAD_E_INVALID, "null context");
+    TRY(sync(c));
+    for (auto &p : c->pools) {
+        pool_drain(c, p);
+        p.acc_ms = 0.0;
+        p.launches = 0;
+    }
+    c->timing = enable != 0;
+    return 0;
+}
+
+}  // extern "C"

@@ -1,0 +1,267 @@
+"""Host-side mirror of QPAD's simulation objects over libqpad_hip.
+
+The classes keep the reference's names and call sequence
+(source/simulation_class.f03, species/species2d_class.f03,
+fields/field_{psi,e,b,src}_class.f03, beam/beam3d_class.f03); every method
+is one C ABI call, and all state stays in device memory.  Errors raise
+QpadError (the reference's write_err stops the run).
+"""
+import ctypes as C
+
+import numpy as np
+
+from . import _lib as L
+from .input import beam_from_deck, grid_from_deck, species_from_deck
+
+
+class Field:
+    """Handle on one slice field of the context (field_class's rf_re/rf_im)."""
+
+    def __init__(self, sim, name):
+        self.sim = sim
+        self.name = name
+        self.fid = L.FIELD_IDS[name]
+        self.ncomp = L.NCOMP[self.fid]
+
+    def shape(self):
+        return (self.ncomp, self.sim.nm, self.sim.grid.nr + 2)
+
+    def get(self):
+        """f1 as [ncomp, 2*max_mode+1, nr+2] (k=0 Re m0, 2m-1 Re m, 2m Im m)."""
+        out = np.zeros(self.shape())
+        L.check(self.sim.lib.qpad_get_field(self.sim.ctx, self.fid, L.dptr(out)))
+        return out
+
+    def set(self, a):
+        a = np.ascontiguousarray(a, dtype=np.float64).reshape(self.shape())
+        L.check(self.sim.lib.qpad_set_field(self.sim.ctx, self.fid, L.dptr(a)))
+
+    def hist(self, s):
+        """f2 slice s (1..nxi, nxi+1 = upper guard slice)."""
+        out = np.zeros(self.shape())
+        L.check(self.sim.lib.qpad_get_hist(self.sim.ctx, self.fid, s, L.dptr(out)))
+        return out
+
+    def set_hist(self, s, a):
+        a = np.ascontiguousarray(a, dtype=np.float64).reshape(self.shape())
+        L.check(self.sim.lib.qpad_set_hist(self.sim.ctx, self.fid, s, L.dptr(a)))
+
+
+class FieldPsi(Field):
+    def solve(self, q):
+        """field_psi%solve(q_spe) (field_psi_class.f03:218-254)."""
+        if q.name != "q_spe":
+            raise L.QpadError("field_psi%solve takes q_spe")
+        L.check(self.sim.lib.qpad_field_solve_psi(self.sim.ctx))
+
+
+class FieldE(Field):
+    def solve(self, *args):
+        """solve(jay): Ez (field_e_class.f03:298-334); solve(b, psi): E_perp
+        (field_e_class.f03:412-514)."""
+        if len(args) == 1:
+            L.check(self.sim.lib.qpad_field_solve_ez(self.sim.ctx, self.fid, args[0].fid))
+        elif len(args) == 2:
+            if args[1].name != "psi":
+                raise L.QpadError("field_e%solve(b, psi) takes psi")
+            L.check(self.sim.lib.qpad_field_solve_et(self.sim.ctx, self.fid, args[0].fid))
+        else:
+            raise L.QpadError("field_e%solve takes (jay) or (b, psi)")
+
+
+class FieldB(Field):
+    def solve(self, *args, slice_index=None):
+        """solve(jay): Bz; solve(q_beam): beam B_perp of slice `slice_index`;
+        solve(djdxi, jay): plasma B_perp (field_b_class.f03:760-879)."""
+        if len(args) == 1 and args[0].name == "q_beam":
+            if self.name != "b_beam" or slice_index is None:
+                raise L.QpadError("beam B_perp is solved into b_beam for a given slice")
+            L.check(self.sim.lib.qpad_field_solve_bt_beam(self.sim.ctx, int(slice_index)))
+        elif len(args) == 1:
+            L.check(self.sim.lib.qpad_field_solve_bz(self.sim.ctx, self.fid, args[0].fid))
+        elif len(args) == 2:
+            if self.name != "b_spe" or args[0].name != "dcu" or args[1].name != "cu":
+                raise L.QpadError("the B_perp iteration solves b_spe from (dcu, cu)")
+            L.check(self.sim.lib.qpad_field_solve_bt_iter(self.sim.ctx))
+        else:
+            raise L.QpadError("bad field_b%solve arguments")
+
+
+class FieldDjdxi(Field):
+    def solve(self, acu, amu):
+        """field_djdxi%solve(acu, amu) (field_src_class.f03:273-405)."""
+        if self.name != "dcu" or acu.name != "acu" or amu.name != "amu":
+            raise L.QpadError("dcu%solve takes (acu, amu)")
+        L.check(self.sim.lib.qpad_field_solve_djdxi(self.sim.ctx))
+
+
+class Species2D:
+    """species2d (species2d_class.f03:24-54), robust pusher."""
+
+    def __init__(self, sim, cfg):
+        self.sim = sim
+        sid = C.c_int32(0)
+        L.check(sim.lib.qpad_species_new(sim.ctx, C.byref(cfg), C.byref(sid)))
+        self.id = sid.value
+        self.cfg = cfg
+
+    def renew(self):
+        L.check(self.sim.lib.qpad_species_renew(self.sim.ctx, self.id))
+
+    def qdp(self, q):
+        if q.name != "q_spe":
+            raise L.QpadError("species2d%qdp deposits into q_spe")
+        L.check(self.sim.lib.qpad_species_qdp(self.sim.ctx, self.id))
+
+    def amjdp(self, ef, bf, cu, amu, acu, dxi):
+        if (ef.name, bf.name, cu.name, amu.name, acu.name) != ("e", "b", "cu", "amu", "acu"):
+            raise L.QpadError("species2d%amjdp(e, b, cu, amu, acu, dxi)")
+        L.check(self.sim.lib.qpad_species_amjdp(self.sim.ctx, self.id, float(dxi)))
+
+    def push(self, dxi):
+        """push_u + push_x (+ update_bound)."""
+        L.check(self.sim.lib.qpad_species_push(self.sim.ctx, self.id, float(dxi)))
+
+    def count(self):
+        n = C.c_int64(0)
+        L.check(self.sim.lib.qpad_species_count(self.sim.ctx, self.id, C.byref(n)))
+        return n.value
+
+    def get(self):
+        n = self.count()
+        out = {k: np.zeros(n) for k in ("x1", "x2", "p1", "p2", "p3", "q", "gamma", "psi")}
+        out["id"] = np.zeros(n, dtype=np.int32)
+        L.check(self.sim.lib.qpad_get_species(
+            self.sim.ctx, self.id, *[L.dptr(out[k]) for k in ("x1", "x2", "p1", "p2", "p3", "q",
+                                                               "gamma", "psi")], L.iptr(out["id"])))
+        return out
+
+    def set(self, arrs):
+        n = len(arrs["x1"])
+        a = {k: (np.ascontiguousarray(arrs[k], dtype=np.float64) if k in arrs else None)
+             for k in ("x1", "x2", "p1", "p2", "p3", "q", "gamma", "psi")}
+        ids = np.ascontiguousarray(arrs["id"], dtype=np.int32) if "id" in arrs else None
+        L.check(self.sim.lib.qpad_set_species(
+            self.sim.ctx, self.id, n, *[L.dptr(a[k]) for k in ("x1", "x2", "p1", "p2", "p3", "q",
+                                                              "gamma", "psi")], L.iptr(ids)))
+
+    def qn(self):
+        out = np.zeros((1, self.sim.nm, self.sim.grid.nr + 2))
+        L.check(self.sim.lib.qpad_get_species_qn(self.sim.ctx, self.id, L.dptr(out)))
+        return out
+
+    def set_qn(self, a):
+        a = np.ascontiguousarray(a, dtype=np.float64)
+        L.check(self.sim.lib.qpad_set_species_qn(self.sim.ctx, self.id, L.dptr(a)))
+
+
+class Beam3D:
+    """beam3d (beam3d_class.f03:51-52): random Gaussian beam, reduced pusher."""
+
+    def __init__(self, sim, cfg):
+        self.sim = sim
+        bid = C.c_int32(0)
+        L.check(sim.lib.qpad_beam_new(sim.ctx, C.byref(cfg), C.byref(bid)))
+        self.id = bid.value
+        self.cfg = cfg
+
+    def count(self):
+        n = C.c_int64(0)
+        L.check(self.sim.lib.qpad_beam_count(self.sim.ctx, self.id, C.byref(n)))
+        return n.value
+
+    def get(self):
+        n = self.count()
+        out = {k: np.zeros(n) for k in ("x1", "x2", "x3", "p1", "p2", "p3", "q")}
+        L.check(self.sim.lib.qpad_get_beam(self.sim.ctx, self.id,
+                                           *[L.dptr(out[k]) for k in ("x1", "x2", "x3", "p1", "p2",
+                                                                       "p3", "q")]))
+        return out
+
+    def set(self, arrs):
+        a = [np.ascontiguousarray(arrs[k], dtype=np.float64) for k in ("x1", "x2", "x3", "p1",
+                                                                       "p2", "p3", "q")]
+        L.check(self.sim.lib.qpad_set_beam(self.sim.ctx, self.id, len(a[0]),
+                                           *[L.dptr(x) for x in a]))
+
+
+class Simulation:
+    """simulation (simulation_class.f03:34-69) driven from a qpinput deck."""
+
+    def __init__(self, deck, lib_path=None):
+        self.lib = L.load(lib_path)
+        self.deck = deck
+        self.grid, self.nstep3d, z0 = grid_from_deck(deck)
+        self.nm = 2 * self.grid.max_mode + 1
+        ctx = C.c_void_p()
+        L.check(self.lib.qpad_create(C.byref(self.grid), C.byref(ctx)))
+        self.ctx = ctx
+        seed = int(deck["simulation"].get("random_seed", 10))
+        self.fields = {name: cls(self, name) for name, cls in (
+            ("psi", FieldPsi), ("e", FieldE), ("e_spe", FieldE), ("b", FieldB),
+            ("b_spe", FieldB), ("b_beam", FieldB), ("cu", Field), ("acu", Field), ("amu", Field),
+            ("dcu", FieldDjdxi), ("q_spe", Field), ("q_beam", Field))}
+        nsp = deck["simulation"].get("nspecies", len(deck.get("species", [])))
+        nbm = deck["simulation"].get("nbeams", len(deck.get("beam", [])))
+        self.spe = [Species2D(self, species_from_deck(sp, seed=seed + 1000 * k))
+                    for k, sp in enumerate(deck.get("species", [])[:nsp])]
+        self.beam = [Beam3D(self, beam_from_deck(bm, z0, seed=seed + 7 * (k + 1)))
+                     for k, bm in enumerate(deck.get("beam", [])[:nbm])]
+        self.tstep = 0
+
+    # -- fused run loop (simulation_class.f03:226-520)
+    def step3d_begin(self):
+        L.check(self.lib.qpad_step3d_begin(self.ctx))
+
+    def slice(self, j):
+        it = C.c_int32(0)
+        L.check(self.lib.qpad_slice(self.ctx, int(j), C.byref(it)))
+        return it.value
+
+    def step3d_end(self):
+        L.check(self.lib.qpad_step3d_end(self.ctx))
+        self.tstep += 1
+
+    def step3d(self):
+        it = C.c_int32(0)
+        L.check(self.lib.qpad_step3d(self.ctx, C.byref(it)))
+        self.tstep += 1
+        return it.value
+
+    def run(self, nstep3d=None):
+        total = 0
+        for _ in range(self.nstep3d if nstep3d is None else nstep3d):
+            total += self.step3d()
+        return total
+
+    def convergence_tester(self, op):
+        """'record' or 'compare' on b_spe component 2 (simulation_class.f03:522-606)."""
+        rel, ab = C.c_double(0.0), C.c_double(0.0)
+        L.check(self.lib.qpad_convergence_tester(self.ctx, 0 if op == "record" else 1,
+                                                 C.byref(rel), C.byref(ab)))
+        return rel.value, ab.value
+
+    def kernel_time(self, which):
+        ms, n = C.c_double(0.0), C.c_int64(0)
+        L.check(self.lib.qpad_kernel_time(self.ctx, int(which), C.byref(ms), C.byref(n)))
+        return ms.value, n.value
+
+    def kernel_time_reset(self, enable=True):
+        L.check(self.lib.qpad_kernel_time_reset(self.ctx, int(enable)))
+
+    def close(self):
+        if getattr(self, "ctx", None):
+            self.lib.qpad_destroy(self.ctx)
+            self.ctx = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
