@@ -1,0 +1,409 @@
+! qpad_hip_mod.f03 -- ISO_C_BINDING interface to libqpad_hip.so for QPAD's
+! Fortran side.  One interface per entry point of include/qpad_hip.h; the
+! derived types mirror the C structs field for field (bind(C)).
+!
+! QPAD never checks return codes: its procedures stop through write_err
+! (source/sysutil_module.f03:165-183).  qpad_hip_check turns a nonzero
+! return into that same path: the caller passes its own error routine.
+module qpad_hip_mod
+
+  use, intrinsic :: iso_c_binding
+  implicit none
+  private
+
+  integer(c_int), parameter, public :: QPAD_HIP_ABI_VERSION = 1
+
+  integer(c_int), parameter, public :: QPAD_OK = 0, QPAD_E_INVALID = -1, &
+    QPAD_E_HIP = -2, QPAD_E_NOMEM = -3, QPAD_E_STATE = -4
+
+  integer(c_int), parameter, public :: QPAD_F_PSI = 0, QPAD_F_E = 1, &
+    QPAD_F_E_SPE = 2, QPAD_F_B = 3, QPAD_F_B_SPE = 4, QPAD_F_B_BEAM = 5, &
+    QPAD_F_CU = 6, QPAD_F_ACU = 7, QPAD_F_AMU = 8, QPAD_F_DCU = 9, &
+    QPAD_F_Q_SPE = 10, QPAD_F_Q_BEAM = 11
+
+  integer(c_int), parameter, public :: QPAD_BND_ZERO = 2, QPAD_BND_OPEN = 3
+  integer(c_int), parameter, public :: QPAD_PUSH2_ROBUST = 1
+
+  type, bind(C), public :: qpad_grid
+    integer(c_int32_t) :: nr, nxi, max_mode
+    real(c_double) :: dr, dxi
+    real(c_double) :: relax_fac
+    integer(c_int32_t) :: bnd
+    integer(c_int32_t) :: iter_max
+    real(c_double) :: iter_reltol, iter_abstol
+    real(c_double) :: dt
+  end type qpad_grid
+
+  type, bind(C), public :: qpad_species_cfg
+    integer(c_int32_t) :: ppc1, ppc2, num_theta
+    real(c_double) :: qm, mass, density
+    real(c_double) :: uth(3)
+    integer(c_int32_t) :: neutralized
+    real(c_double) :: den_min
+    integer(c_int64_t) :: seed
+    integer(c_int32_t) :: push_type
+  end type qpad_species_cfg
+
+  type, bind(C), public :: qpad_beam_cfg
+    integer(c_int64_t) :: total_num
+    real(c_double) :: total_charge
+    real(c_double) :: qm, mass, gamma
+    real(c_double) :: center(3), sigma(3)
+    real(c_double) :: range_lo(3), range_hi(3)
+    real(c_double) :: uth(3)
+    integer(c_int32_t) :: quiet
+    integer(c_int32_t) :: evol
+    integer(c_int64_t) :: seed
+  end type qpad_beam_cfg
+
+  public :: qpad_hip_check, qpad_error_message
+
+  interface
+
+    function qpad_last_error() bind(C, name="qpad_last_error") result(msg)
+      import :: c_ptr
+      type(c_ptr) :: msg
+    end function
+
+    function qpad_abi_version() bind(C, name="qpad_abi_version") result(v)
+      import :: c_int
+      integer(c_int) :: v
+    end function
+
+    ! simulation%alloc/%new without species and beams
+    function qpad_create(grid, ctx) bind(C, name="qpad_create") result(rc)
+      import :: c_int, c_ptr, qpad_grid
+      type(qpad_grid), intent(in) :: grid
+      type(c_ptr), intent(out) :: ctx
+      integer(c_int) :: rc
+    end function
+
+    function qpad_destroy(ctx) bind(C, name="qpad_destroy") result(rc)
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx
+      integer(c_int) :: rc
+    end function
+
+    ! species2d%new / %renew / %qdp / %amjdp / %push_u + %push_x
+    function qpad_species_new(ctx, cfg, sp_id) bind(C, name="qpad_species_new") result(rc)
+      import :: c_int, c_int32_t, c_ptr, qpad_species_cfg
+      type(c_ptr), value :: ctx
+      type(qpad_species_cfg), intent(in) :: cfg
+      integer(c_int32_t), intent(out) :: sp_id
+      integer(c_int) :: rc
+    end function
+
+    function qpad_species_renew(ctx, sp) bind(C, name="qpad_species_renew") result(rc)
+      import :: c_int, c_int32_t, c_ptr
+      type(c_ptr), value :: ctx
+      integer(c_int32_t), value :: sp
+      integer(c_int) :: rc
+    end function
+
+    function qpad_species_qdp(ctx, sp) bind(C, name="qpad_species_qdp") result(rc)
+      import :: c_int, c_int32_t, c_ptr
+      type(c_ptr), value :: ctx
+      integer(c_int32_t), value :: sp
+      integer(c_int) :: rc
+    end function
+
+    function qpad_species_amjdp(ctx, sp, dxi) bind(C, name="qpad_species_amjdp") result(rc)
+      import :: c_int, c_int32_t, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      integer(c_int32_t), value :: sp
+      real(c_double), value :: dxi
+      integer(c_int) :: rc
+    end function
+
+    function qpad_species_push(ctx, sp, dxi) bind(C, name="qpad_species_push") result(rc)
+      import :: c_int, c_int32_t, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      integer(c_int32_t), value :: sp
+      real(c_double), value :: dxi
+      integer(c_int) :: rc
+    end function
+
+    ! field_psi%solve, field_e%solve(jay) / (b, psi), field_b%solve(jay) /
+    ! (rho) / (djdxi, jay), field_djdxi%solve, add_f1, convergence_tester
+    function qpad_field_solve_psi(ctx) bind(C, name="qpad_field_solve_psi") result(rc)
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx
+      integer(c_int) :: rc
+    end function
+
+    function qpad_field_solve_ez(ctx, dst, jay) bind(C, name="qpad_field_solve_ez") result(rc)
+      import :: c_int, c_int32_t, c_ptr
+      type(c_ptr), value :: ctx
+      integer(c_int32_t), value :: dst, jay
+      integer(c_int) :: rc
+    end function
+
+    function qpad_field_solve_et(ctx, dst, b) bind(C, name="qpad_field_solve_et") result(rc)
+      import :: c_int, c_int32_t, c_ptr
+      type(c_ptr), value :: ctx
+      integer(c_int32_t), value :: dst, b
+      integer(c_int) :: rc
+    end function
+
+    function qpad_field_solve_bz(ctx, dst, jay) bind(C, name="qpad_field_solve_bz") result(rc)
+      import :: c_int, c_int32_t, c_ptr
+      type(c_ptr), value :: ctx
+      integer(c_int32_t), value :: dst, jay
+      integer(c_int) :: rc
+    end function
+
+    function qpad_field_solve_bt_beam(ctx, j) bind(C, name="qpad_field_solve_bt_beam") result(rc)
+      import :: c_int, c_int32_t, c_ptr
+      type(c_ptr), value :: ctx
+      integer(c_int32_t), value :: j
+      integer(c_int) :: rc
+    end function
+
+    function qpad_field_solve_djdxi(ctx) bind(C, name="qpad_field_solve_djdxi") result(rc)
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx
+      integer(c_int) :: rc
+    end function
+
+    function qpad_field_solve_bt_iter(ctx) bind(C, name="qpad_field_solve_bt_iter") result(rc)
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx
+      integer(c_int) :: rc
+    end function
+
+    function qpad_field_add_b(ctx) bind(C, name="qpad_field_add_b") result(rc)
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx
+      integer(c_int) :: rc
+    end function
+
+    function qpad_convergence_tester(ctx, op, rel_res, abs_res) &
+        bind(C, name="qpad_convergence_tester") result(rc)
+      import :: c_int, c_int32_t, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      integer(c_int32_t), value :: op
+      real(c_double), intent(out) :: rel_res, abs_res
+      integer(c_int) :: rc
+    end function
+
+    ! beam3d%qdp / %push / %new
+    function qpad_beam_qdp(ctx) bind(C, name="qpad_beam_qdp") result(rc)
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx
+      integer(c_int) :: rc
+    end function
+
+    function qpad_beam_push(ctx) bind(C, name="qpad_beam_push") result(rc)
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx
+      integer(c_int) :: rc
+    end function
+
+    function qpad_beam_new(ctx, cfg, beam_id) bind(C, name="qpad_beam_new") result(rc)
+      import :: c_int, c_int32_t, c_ptr, qpad_beam_cfg
+      type(c_ptr), value :: ctx
+      type(qpad_beam_cfg), intent(in) :: cfg
+      integer(c_int32_t), intent(out) :: beam_id
+      integer(c_int) :: rc
+    end function
+
+    ! fused simulation%run pieces
+    function qpad_step3d_begin(ctx) bind(C, name="qpad_step3d_begin") result(rc)
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx
+      integer(c_int) :: rc
+    end function
+
+    function qpad_slice(ctx, j, iters) bind(C, name="qpad_slice") result(rc)
+      import :: c_int, c_int32_t, c_ptr
+      type(c_ptr), value :: ctx
+      integer(c_int32_t), value :: j
+      integer(c_int32_t), intent(out) :: iters
+      integer(c_int) :: rc
+    end function
+
+    function qpad_step3d_end(ctx) bind(C, name="qpad_step3d_end") result(rc)
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx
+      integer(c_int) :: rc
+    end function
+
+    function qpad_step3d(ctx, total_iters) bind(C, name="qpad_step3d") result(rc)
+      import :: c_int, c_int32_t, c_ptr
+      type(c_ptr), value :: ctx
+      integer(c_int32_t), intent(out) :: total_iters
+      integer(c_int) :: rc
+    end function
+
+    ! host <-> device state: fields as (0:nr+1, 2*max_mode+1, ncomp)
+    function qpad_get_field(ctx, fid, out) bind(C, name="qpad_get_field") result(rc)
+      import :: c_int, c_int32_t, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      integer(c_int32_t), value :: fid
+      real(c_double), intent(out) :: out(*)
+      integer(c_int) :: rc
+    end function
+
+    function qpad_set_field(ctx, fid, in) bind(C, name="qpad_set_field") result(rc)
+      import :: c_int, c_int32_t, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      integer(c_int32_t), value :: fid
+      real(c_double), intent(in) :: in(*)
+      integer(c_int) :: rc
+    end function
+
+    function qpad_get_hist(ctx, fid, s, out) bind(C, name="qpad_get_hist") result(rc)
+      import :: c_int, c_int32_t, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      integer(c_int32_t), value :: fid, s
+      real(c_double), intent(out) :: out(*)
+      integer(c_int) :: rc
+    end function
+
+    function qpad_set_hist(ctx, fid, s, in) bind(C, name="qpad_set_hist") result(rc)
+      import :: c_int, c_int32_t, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      integer(c_int32_t), value :: fid, s
+      real(c_double), intent(in) :: in(*)
+      integer(c_int) :: rc
+    end function
+
+    function qpad_species_count(ctx, sp, npp) bind(C, name="qpad_species_count") result(rc)
+      import :: c_int, c_int32_t, c_int64_t, c_ptr
+      type(c_ptr), value :: ctx
+      integer(c_int32_t), value :: sp
+      integer(c_int64_t), intent(out) :: npp
+      integer(c_int) :: rc
+    end function
+
+    ! pass c_null_ptr (via c_loc of nothing) for arrays to skip
+    function qpad_get_species(ctx, sp, x1, x2, p1, p2, p3, q, gamma, psi, id) &
+        bind(C, name="qpad_get_species") result(rc)
+      import :: c_int, c_int32_t, c_ptr
+      type(c_ptr), value :: ctx
+      integer(c_int32_t), value :: sp
+      type(c_ptr), value :: x1, x2, p1, p2, p3, q, gamma, psi, id
+      integer(c_int) :: rc
+    end function
+
+    function qpad_set_species(ctx, sp, n, x1, x2, p1, p2, p3, q, gamma, psi, id) &
+        bind(C, name="qpad_set_species") result(rc)
+      import :: c_int, c_int32_t, c_int64_t, c_ptr
+      type(c_ptr), value :: ctx
+      integer(c_int32_t), value :: sp
+      integer(c_int64_t), value :: n
+      type(c_ptr), value :: x1, x2, p1, p2, p3, q, gamma, psi, id
+      integer(c_int) :: rc
+    end function
+
+    function qpad_get_species_qn(ctx, sp, out) bind(C, name="qpad_get_species_qn") result(rc)
+      import :: c_int, c_int32_t, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      integer(c_int32_t), value :: sp
+      real(c_double), intent(out) :: out(*)
+      integer(c_int) :: rc
+    end function
+
+    function qpad_set_species_qn(ctx, sp, in) bind(C, name="qpad_set_species_qn") result(rc)
+      import :: c_int, c_int32_t, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      integer(c_int32_t), value :: sp
+      real(c_double), intent(in) :: in(*)
+      integer(c_int) :: rc
+    end function
+
+    function qpad_beam_count(ctx, ib, npp) bind(C, name="qpad_beam_count") result(rc)
+      import :: c_int, c_int32_t, c_int64_t, c_ptr
+      type(c_ptr), value :: ctx
+      integer(c_int32_t), value :: ib
+      integer(c_int64_t), intent(out) :: npp
+      integer(c_int) :: rc
+    end function
+
+    function qpad_get_beam(ctx, ib, x1, x2, x3, p1, p2, p3, q) bind(C, name="qpad_get_beam") result(rc)
+      import :: c_int, c_int32_t, c_ptr
+      type(c_ptr), value :: ctx
+      integer(c_int32_t), value :: ib
+      type(c_ptr), value :: x1, x2, x3, p1, p2, p3, q
+      integer(c_int) :: rc
+    end function
+
+    function qpad_set_beam(ctx, ib, n, x1, x2, x3, p1, p2, p3, q) bind(C, name="qpad_set_beam") result(rc)
+      import :: c_int, c_int32_t, c_int64_t, c_ptr
+      type(c_ptr), value :: ctx
+      integer(c_int32_t), value :: ib
+      integer(c_int64_t), value :: n
+      type(c_ptr), value :: x1, x2, x3, p1, p2, p3, q
+      integer(c_int) :: rc
+    end function
+
+    function qpad_kernel_time(ctx, which, ms, launches) bind(C, name="qpad_kernel_time") result(rc)
+      import :: c_int, c_int32_t, c_int64_t, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      integer(c_int32_t), value :: which
+      real(c_double), intent(out) :: ms
+      integer(c_int64_t), intent(out) :: launches
+      integer(c_int) :: rc
+    end function
+
+    function qpad_kernel_time_reset(ctx, enable) bind(C, name="qpad_kernel_time_reset") result(rc)
+      import :: c_int, c_int32_t, c_ptr
+      type(c_ptr), value :: ctx
+      integer(c_int32_t), value :: enable
+      integer(c_int) :: rc
+    end function
+
+  end interface
+
+  public :: qpad_last_error, qpad_abi_version, qpad_create, qpad_destroy, &
+    qpad_species_new, qpad_species_renew, qpad_species_qdp, qpad_species_amjdp, &
+    qpad_species_push, qpad_field_solve_psi, qpad_field_solve_ez, &
+    qpad_field_solve_et, qpad_field_solve_bz, qpad_field_solve_bt_beam, &
+    qpad_field_solve_djdxi, qpad_field_solve_bt_iter, qpad_field_add_b, &
+    qpad_convergence_tester, qpad_beam_qdp, qpad_beam_push, qpad_beam_new, &
+    qpad_step3d_begin, qpad_slice, qpad_step3d_end, qpad_step3d, &
+    qpad_get_field, qpad_set_field, qpad_get_hist, qpad_set_hist, &
+    qpad_species_count, qpad_get_species, qpad_set_species, &
+    qpad_get_species_qn, qpad_set_species_qn, qpad_beam_count, qpad_get_beam, &
+    qpad_set_beam, qpad_kernel_time, qpad_kernel_time_reset
+
+contains
+
+  ! the library's message for the last failed call of this thread
+  function qpad_error_message() result(msg)
+    character(len=:), allocatable :: msg
+    type(c_ptr) :: p
+    character(kind=c_char), pointer :: s(:)
+    integer :: n
+    interface
+      function c_strlen(str) bind(C, name="strlen") result(len)
+        import :: c_ptr, c_size_t
+        type(c_ptr), value :: str
+        integer(c_size_t) :: len
+      end function
+    end interface
+    p = qpad_last_error()
+    if (.not. c_associated(p)) then
+      msg = ''
+      return
+    end if
+    n = int(c_strlen(p))
+    call c_f_pointer(p, s, [n])
+    allocate (character(len=n) :: msg)
+    msg = transfer(s(1:n), msg)
+  end function qpad_error_message
+
+  ! QPAD-style error handling: a nonzero code goes to the caller's error
+  ! routine (write_err in QPAD) with the library's message
+  subroutine qpad_hip_check(rc, where, on_error)
+    integer(c_int), intent(in) :: rc
+    character(len=*), intent(in) :: where
+    interface
+      subroutine on_error(msg)
+        character(len=*), intent(in) :: msg
+      end subroutine
+    end interface
+    if (rc /= QPAD_OK) call on_error('HIP: '//where//': '//qpad_error_message())
+  end subroutine qpad_hip_check
+
+end module qpad_hip_mod
