@@ -9,6 +9,7 @@
 #include <rocprim/rocprim.hpp>
 
 #include "kernels.h"
+#include "sort_config.h"
 #include "rng.h"
 
 namespace qpad {
@@ -212,15 +213,6 @@ __global__ void k_beam_permute(BPermArgs a)
     for (int f = 0; f < 7; f++) a.d[f][p] = a.s[f][src];
 }
 
-__global__ void k_cellstart_b(const uint32_t *key, int64_t n, int ncell, int32_t *cs)
-{
-    int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (p > n) return;
-    int64_t kp = (p == 0) ? -1 : (int64_t)key[p - 1];
-    int64_t kc = (p == n) ? (int64_t)ncell : (int64_t)key[p];
-    for (int64_t c = kp + 1; c <= kc; c++) cs[c] = (int32_t)p;
-}
-
 int launch_beam_sort(qpad_ctx *c, int ib)
 {
     Beam &bm = c->bm[ib];
@@ -231,7 +223,7 @@ int launch_beam_sort(qpad_ctx *c, int ib)
     if (n > 0) {
         QPAD_TRY(launch_beam_keys(c, ib));
         size_t bytes = bm.sort_tmp_bytes;
-        QPAD_HIP_CHECK(rocprim::radix_sort_pairs(bm.sort_tmp, bytes, bm.key, bm.key_alt, bm.idx, bm.idx_alt,
+        QPAD_HIP_CHECK(rocprim::radix_sort_pairs<SortConfig>(bm.sort_tmp, bytes, bm.key, bm.key_alt, bm.idx, bm.idx_alt,
                                                  (size_t)n, 0, bits, c->stream));
         BPermArgs a;
         a.n = n;
@@ -250,10 +242,7 @@ int launch_beam_sort(qpad_ctx *c, int ib)
         std::swap(bm.key, bm.key_alt);
     }
     // cs[ncell] = number of particles inside the box, cs[ncell + 1] = n
-    hipLaunchKernelGGL(k_cellstart_b, dim3((unsigned)((n + 1 + 255) / 256)), dim3(256), 0, c->stream, bm.key, n,
-                       ncell, bm.cell_start);
-    QPAD_HIP_CHECK(hipGetLastError());
-    return 0;
+    return launch_cell_start(c, bm.key, n, ncell + 2, bm.cell_start);
 }
 
 // ---------------------------------------------------------------------------

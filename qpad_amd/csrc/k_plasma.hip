@@ -13,7 +13,9 @@
 #include <rocprim/rocprim.hpp>
 
 #include "kernels.h"
+#include "sort_config.h"
 #include "rng.h"
+#include "wave_reduce.h"
 
 namespace qpad {
 
@@ -32,13 +34,6 @@ __device__ inline uint32_t cell_key(double x1, double x2, double idr, double edg
     if (!(r < edge)) return (uint32_t)nr;   // leaves the box (update_bound)
     int c = (int)__dmul_rn(r, idr);
     return (uint32_t)(c < nr ? c : nr - 1);
-}
-
-__device__ inline double wave_sum(double v)
-{
-#pragma unroll
-    for (int d = 32; d > 0; d >>= 1) v += __shfl_xor(v, d, 64);
-    return v;
 }
 
 // ---------------------------------------------------------------------------
@@ -144,13 +139,28 @@ __global__ void k_permute(PermArgs a)
     a.did[p] = a.sid[src];
 }
 
-__global__ void k_cellstart(const uint32_t *key, int64_t n, int ncell, int32_t *cs)
+// cs[c] = first position p with key[p] >= c (lower bound in the sorted
+// keys), for c = 0 .. ncs-1: one thread per cell, so empty runs of cells
+// cost nothing extra
+__global__ __launch_bounds__(256) void k_cellstart(const uint32_t *key, int64_t n, int ncs, int32_t *cs)
 {
-    int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (p > n) return;
-    int kp = (p == 0) ? -1 : (int)key[p - 1];
-    int kc = (p == n) ? ncell + 1 : (int)key[p];
-    for (int c = kp + 1; c <= kc; c++) cs[c] = (int32_t)p;
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= ncs) return;
+    int64_t lo = 0, hi = n;
+    while (lo < hi) {
+        const int64_t mid = lo + ((hi - lo) >> 1);
+        if (key[mid] < (uint32_t)c) lo = mid + 1;
+        else hi = mid;
+    }
+    cs[c] = (int32_t)lo;
+}
+
+int launch_cell_start(qpad_ctx *c, const uint32_t *key, int64_t n, int ncs, int32_t *cs)
+{
+    hipLaunchKernelGGL(k_cellstart, dim3((unsigned)((ncs + 255) / 256)), dim3(256), 0, c->stream, key, n, ncs,
+                       cs);
+    QPAD_HIP_CHECK(hipGetLastError());
+    return 0;
 }
 
 // scratch of rocPRIM's radix sort (uint32 keys and values) and exclusive scan
@@ -158,7 +168,7 @@ int sort_scratch(size_t n, int bits, size_t *bytes)
 {
     size_t b1 = 0, b2 = 0;
     uint32_t *z = nullptr;
-    QPAD_HIP_CHECK(rocprim::radix_sort_pairs(nullptr, b1, z, z, z, z, n, 0, bits));
+    QPAD_HIP_CHECK(rocprim::radix_sort_pairs<SortConfig>(nullptr, b1, z, z, z, z, n, 0, bits));
     QPAD_HIP_CHECK(rocprim::exclusive_scan(nullptr, b2, z, z, 0u, n, rocprim::plus<uint32_t>()));
     *bytes = std::max(b1, b2) + 256;
     return 0;
@@ -178,7 +188,7 @@ int launch_sort(qpad_ctx *c, int s)
     int bits = bits_for((uint32_t)c->g.nr);
     if (n > 0) {
         size_t bytes = sp.sort_tmp_bytes;
-        QPAD_HIP_CHECK(rocprim::radix_sort_pairs(sp.sort_tmp, bytes, sp.key, sp.key_alt, sp.idx, sp.idx_alt,
+        QPAD_HIP_CHECK(rocprim::radix_sort_pairs<SortConfig>(sp.sort_tmp, bytes, sp.key, sp.key_alt, sp.idx, sp.idx_alt,
                                                  (size_t)n, 0, bits, c->stream));
         PermArgs a;
         a.n = n;
@@ -199,10 +209,7 @@ int launch_sort(qpad_ctx *c, int s)
         std::swap(sp.id, sp.a_id);
         std::swap(sp.key, sp.key_alt);
     }
-    hipLaunchKernelGGL(k_cellstart, dim3((unsigned)((n + 1 + 255) / 256)), dim3(256), 0, c->stream,
-                       sp.key, n, c->g.nr, sp.cell_start);
-    QPAD_HIP_CHECK(hipGetLastError());
-    return 0;
+    return launch_cell_start(c, sp.key, n, c->g.nr + 2, sp.cell_start);
 }
 
 __global__ void k_keys(const double *x1, const double *x2, uint32_t *key, uint32_t *idx, int64_t n,
@@ -245,34 +252,38 @@ __global__ __launch_bounds__(64 * kDepWaves) void k_qdeposit(QDepArgs a)
     const int c = blockIdx.x * kDepWaves + wave;
     if (c >= a.nr) return;
     const int p0 = a.cs[c], p1 = a.cs[c + 1];
-    double acc0[NM], acc1[NM];
+    // acc[k]: node ix, acc[NM + k]: node ix + 1; padded to a multiple of 16
+    constexpr int NB = (2 * NM + 15) / 16;
+    double acc[16 * NB];
 #pragma unroll
-    for (int k = 0; k < NM; k++) { acc0[k] = 0.0; acc1[k] = 0.0; }
+    for (int v = 0; v < 16 * NB; v++) acc[v] = 0.0;
     for (int p = p0 + lane; p < p1; p += 64) {
         const double x1 = a.x1[p], x2 = a.x2[p], q = a.q[p];
         double pos = __dmul_rn(radius(x1, x2), a.idr);
         double ph0r = x1 / pos * a.idr, ph0i = -x2 / pos * a.idr;
         double w1 = pos - (double)c, w0 = 1.0 - w1;
         double phr = 1.0 * q, phi = 0.0 * q;
-        acc0[0] += w0 * phr;
-        acc1[0] += w1 * phr;
+        acc[0] += w0 * phr;
+        acc[NM] += w1 * phr;
 #pragma unroll
         for (int m = 1; m <= M; m++) {
             double tr = phr * ph0r - phi * ph0i, ti = phr * ph0i + phi * ph0r;
             phr = tr;
             phi = ti;
-            acc0[2 * m - 1] += w0 * phr;
-            acc1[2 * m - 1] += w1 * phr;
-            acc0[2 * m] += w0 * phi;
-            acc1[2 * m] += w1 * phi;
+            acc[2 * m - 1] += w0 * phr;
+            acc[NM + 2 * m - 1] += w1 * phr;
+            acc[2 * m] += w0 * phi;
+            acc[NM + 2 * m] += w1 * phi;
         }
     }
+    wave_reduce_scatter<NB>(acc);
+    const int g = lane >> 2;
 #pragma unroll
-    for (int k = 0; k < NM; k++) {
-        double s0 = wave_sum(acc0[k]), s1 = wave_sum(acc1[k]);
-        if (lane == k) {
-            a.P0[(size_t)c * NM + k] = s0;
-            a.P1[(size_t)c * NM + k] = s1;
+    for (int i = 0; i < NB; i++) {
+        const int v = g * NB + i;
+        if ((lane & 3) == (i & 3) && v < 2 * NM) {
+            if (v < NM) a.P0[(size_t)c * NM + v] = acc[i];
+            else a.P1[(size_t)c * NM + v - NM] = acc[i];
         }
     }
 }
@@ -328,15 +339,15 @@ int launch_combine_q(qpad_ctx *c, int s, double *out, double scale, bool accumul
 // ---------------------------------------------------------------------------
 // gather of a 3-component field at nodes ix, ix+1 (interp_part2d.f03:66-110)
 // ---------------------------------------------------------------------------
-template <int M>
-__device__ inline void gather3(const double *f, int nm, int ld, int ix, double w0, double w1, double pc,
-                               double ps, double out[3])
+// f(cc, k, n): component cc of mode array k at node ix + n (n = 0, 1)
+template <int M, class F>
+__device__ __forceinline__ void gather3(const F &f, double w0, double w1, double pc, double ps, double out[3])
 {
 #pragma unroll
     for (int cc = 0; cc < 3; cc++) {
         double v = 0.0;
-        v = v + FX(f, cc, 0, ix) * w0;
-        v = v + FX(f, cc, 0, ix + 1) * w1;
+        v = v + f(cc, 0, 0) * w0;
+        v = v + f(cc, 0, 1) * w1;
         out[cc] = v;
     }
     double phr = 1.0, phi = 0.0;
@@ -348,11 +359,28 @@ __device__ inline void gather3(const double *f, int nm, int ld, int ix, double w
         const double ph_re = 2.0 * phr, ph_im = 2.0 * phi;
 #pragma unroll
         for (int cc = 0; cc < 3; cc++) {
-            out[cc] = out[cc] + (FX(f, cc, 2 * m - 1, ix) * ph_re - FX(f, cc, 2 * m, ix) * ph_im) * w0;
-            out[cc] = out[cc] + (FX(f, cc, 2 * m - 1, ix + 1) * ph_re - FX(f, cc, 2 * m, ix + 1) * ph_im) * w1;
+            out[cc] = out[cc] + (f(cc, 2 * m - 1, 0) * ph_re - f(cc, 2 * m, 0) * ph_im) * w0;
+            out[cc] = out[cc] + (f(cc, 2 * m - 1, 1) * ph_re - f(cc, 2 * m, 1) * ph_im) * w1;
         }
     }
 }
+
+// nodes ix, ix + 1 of a slice field in HBM
+struct NodePair {
+    const double *f;
+    int nm, ld, ix;
+    __device__ __forceinline__ double operator()(int cc, int k, int n) const
+    {
+        return f[((size_t)cc * nm + k) * ld + ix + n];
+    }
+};
+
+// nodes ix, ix + 1 of a slice field staged in LDS as [n][cc][k]
+template <int NM>
+struct NodePairLds {
+    const double *s;
+    __device__ __forceinline__ double operator()(int cc, int k, int n) const { return s[(n * 3 + cc) * NM + k]; }
+};
 
 // ---------------------------------------------------------------------------
 // amjdeposit_robust_part2d (part2d_class.f03:746-1010): gather E, B; robust
@@ -370,6 +398,7 @@ struct JDepArgs {
     double *P0, *P1;
     const IterState *it;
     int check_done;
+    int *counter;   // counts launches that did work
 };
 
 template <int M>
@@ -377,28 +406,53 @@ __global__ __launch_bounds__(64 * kDepWaves) void k_amjdeposit(JDepArgs a)
 {
     constexpr int NM = 2 * M + 1;
     constexpr int NV = 8 * NM;
+    // E and B at the two nodes of each wave's cell: [wave][E|B][n][cc][k]
+    __shared__ double sf[kDepWaves][2][2 * 3 * NM];
     if (a.check_done && a.it->done) return;
     const int lane = threadIdx.x & 63;
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int c = blockIdx.x * kDepWaves + wave;
-    if (c >= a.nr) return;
     const int nm = a.nm, ld = a.ld;
-    const int p0 = a.cs[c], p1 = a.cs[c + 1];
     const int ix = c + 1;
-    double acc0[NV], acc1[NV];
+    if (c < a.nr) {
+        for (int l = lane; l < 12 * NM; l += 64) {
+            const int fb = l / (6 * NM), r = l % (6 * NM);
+            const int n = r / (3 * NM), cc = (r / NM) % 3, k = r % NM;
+            const double *f = fb ? a.b : a.e;
+            sf[wave][fb][r] = f[((size_t)cc * nm + k) * ld + ix + n];
+        }
+    }
+    __syncthreads();
+    if (c >= a.nr) return;
+    if (blockIdx.x == 0 && threadIdx.x == 0) a.counter[0] += 1;
+    const int p0 = a.cs[c], p1 = a.cs[c + 1];
+    const NodePairLds<NM> fe{sf[wave][0]}, fbf{sf[wave][1]};
+    const double *__restrict__ X1 = a.x1;
+    const double *__restrict__ X2 = a.x2;
+    const double *__restrict__ PP1 = a.p1;
+    const double *__restrict__ PP2 = a.p2;
+    const double *__restrict__ PP3 = a.p3;
+    const double *__restrict__ Q = a.q;
+    double *__restrict__ GAM = a.gamma;
+    double *__restrict__ PSI = a.psi;
+    // acc[v]: node ix, acc[NV + v]: node ix + 1 (2 NV = 16 NM values)
+    double acc[2 * NV];
 #pragma unroll
-    for (int v = 0; v < NV; v++) { acc0[v] = 0.0; acc1[v] = 0.0; }
+    for (int v = 0; v < 2 * NV; v++) acc[v] = 0.0;
     for (int p = p0 + lane; p < p1; p += 64) {
-        const double x1 = a.x1[p], x2 = a.x2[p];
+        // the staged fields are re-read from LDS (broadcast) every particle
+        // rather than held in registers: the accumulators need the VGPRs
+        __asm__ volatile("" ::: "memory");
+        const double x1 = X1[p], x2 = X2[p];
+        const double pp1 = PP1[p], pp2 = PP2[p], pp3 = PP3[p], qp = Q[p];
         double r = radius(x1, x2);
         const double pc = x1 / r, ps = x2 / r;
         double pos = __dmul_rn(r, a.idr);
         const double w1 = pos - (double)c, w0 = 1.0 - w1;
         double ep[3], bp[3];
-        gather3<M>(a.e, nm, ld, ix, w0, w1, pc, ps, ep);
-        gather3<M>(a.b, nm, ld, ix, w0, w1, pc, ps, bp);
+        gather3<M>(fe, w0, w1, pc, ps, ep);
+        gather3<M>(fbf, w0, w1, pc, ps, bp);
         double wp0 = ep[0] - bp[1], wp1 = ep[1] + bp[0], wp2 = ep[2];
-        const double pp1 = a.p1[p], pp2 = a.p2[p], pp3 = a.p3[p];
         double u0[3] = {pp1 * pc + pp2 * ps, pp2 * pc - pp1 * ps, pp3};
         double gam = sqrt(1.0 + u0[0] * u0[0] + u0[1] * u0[1] + u0[2] * u0[2]);
         double qtmh1 = a.qtmh * gam / (gam - u0[2]);
@@ -426,21 +480,21 @@ __global__ __launch_bounds__(64 * kDepWaves) void k_amjdeposit(JDepArgs a)
         double g2 = sqrt(1.0 + u[0] * u[0] + u[1] * u[1] + u[2] * u[2]);
         double ipsi = 1.0 / (g2 - u[2]);
         if (a.store_diag) {
-            a.gamma[p] = g2;
-            a.psi[p] = (1.0 - 1.0 / ipsi) / a.qbm;
+            GAM[p] = g2;
+            PSI[p] = (1.0 - 1.0 / ipsi) / a.qbm;
         }
         double dpsi = a.qbm * (wp2 - (wp0 * u[0] + wp1 * u[1]) * ipsi);
         du0 = du0 + u[0] * dpsi * ipsi;
         du1 = du1 + u[1] * dpsi * ipsi;
         const double val[8] = {u[0], u[1], u[2], du0, du1, u[0] * u[0] * ipsi, u[0] * u[1] * ipsi,
                                u[1] * u[1] * ipsi};
-        double phr = 1.0 * a.q[p] * ipsi, phi = 0.0;
+        double phr = 1.0 * qp * ipsi, phi = 0.0;
         {
             const double wa = w0 * phr, wb = w1 * phr;
 #pragma unroll
             for (int cc = 0; cc < 8; cc++) {
-                acc0[cc * NM] += wa * val[cc];
-                acc1[cc * NM] += wb * val[cc];
+                acc[cc * NM] += wa * val[cc];
+                acc[NV + cc * NM] += wb * val[cc];
             }
         }
         const double ph0r = pc, ph0i = -ps;
@@ -452,19 +506,21 @@ __global__ __launch_bounds__(64 * kDepWaves) void k_amjdeposit(JDepArgs a)
             const double war = w0 * phr, wbr = w1 * phr, wai = w0 * phi, wbi = w1 * phi;
 #pragma unroll
             for (int cc = 0; cc < 8; cc++) {
-                acc0[cc * NM + 2 * m - 1] += war * val[cc];
-                acc1[cc * NM + 2 * m - 1] += wbr * val[cc];
-                acc0[cc * NM + 2 * m] += wai * val[cc];
-                acc1[cc * NM + 2 * m] += wbi * val[cc];
+                acc[cc * NM + 2 * m - 1] += war * val[cc];
+                acc[NV + cc * NM + 2 * m - 1] += wbr * val[cc];
+                acc[cc * NM + 2 * m] += wai * val[cc];
+                acc[NV + cc * NM + 2 * m] += wbi * val[cc];
             }
         }
     }
+    wave_reduce_scatter<NM>(acc);
+    const int g = lane >> 2;
 #pragma unroll
-    for (int v = 0; v < NV; v++) {
-        double s0 = wave_sum(acc0[v]), s1 = wave_sum(acc1[v]);
-        if (lane == (v & 63)) {
-            a.P0[(size_t)c * NV + v] = s0;
-            a.P1[(size_t)c * NV + v] = s1;
+    for (int i = 0; i < NM; i++) {
+        const int v = g * NM + i;
+        if ((lane & 3) == (i & 3)) {
+            if (v < NV) a.P0[(size_t)c * NV + v] = acc[i];
+            else a.P1[(size_t)c * NV + v - NV] = acc[i];
         }
     }
 }
@@ -494,6 +550,7 @@ int launch_amjdeposit(qpad_ctx *c, int s, bool check_done, bool store_diag)
     a.P1 = sp.P1;
     a.it = c->it;
     a.check_done = check_done;
+    a.counter = &c->it->real_deposits;
     dim3 grid((c->g.nr + kDepWaves - 1) / kDepWaves), blk(64 * kDepWaves);
     hipEvent_t stop = nullptr;
     time_begin(c, 0, &stop);
@@ -542,8 +599,8 @@ __global__ __launch_bounds__(256) void k_push(PushArgs a)
     const int ix = ic + 1;
     const double w1 = pos - (double)ic, w0 = 1.0 - w1;
     double ep[3], bp[3];
-    gather3<M>(a.e, nm, ld, ix, w0, w1, pc, ps, ep);
-    gather3<M>(a.b, nm, ld, ix, w0, w1, pc, ps, bp);
+    gather3<M>(NodePair{a.e, nm, ld, ix}, w0, w1, pc, ps, ep);
+    gather3<M>(NodePair{a.b, nm, ld, ix}, w0, w1, pc, ps, bp);
     double t = ep[0] * pc - ep[1] * ps;
     ep[1] = ep[0] * ps + ep[1] * pc;
     ep[0] = t;

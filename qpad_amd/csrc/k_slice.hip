@@ -1,0 +1,511 @@
+// k_slice.hip -- the fused B_perp predictor-corrector iteration of a slice
+// (simulation_class.f03:370-398) after the current deposit:
+//
+//   k_src    every node and mode in parallel: cu, acu, amu from the per-cell
+//            deposit partials (written out), dJ/dxi (written out), and the
+//            right-hand sides of every solve of the iteration (B+/B-, Bz,
+//            Ez) into the `src` buffer, rows interleaved for coalescing;
+//   k_solve  one workgroup per system: B_perp (4 systems per mode, then the
+//            recombination, E_perp and b = b_spe + b_beam of that mode), Bz,
+//            Ez; the last B_perp workgroup runs the convergence test.
+//
+// E and b written by k_solve are the fields of the next iteration's deposit
+// (the reference recomputes them at the top of that iteration from the same
+// cu, b_spe and psi).  k_epilogue is the slice's closing copy-out
+// (simulation_class.f03:401-456).  Every kernel returns at once when the
+// device-side `done` flag says the iteration has converged, so iterations
+// are enqueued speculatively with no host round trip.
+#include <algorithm>
+#include <cstring>
+
+#include "field_src.h"
+#include "kernels.h"
+
+namespace qpad {
+
+// src buffer rows: B+/B- of mode m at 4m + s (s = Re B+, Im B+, Re B-,
+// Im B-), Bz of mode array k at 4(M+1) + k, Ez at 4(M+1) + nm + k; row
+// r = i - 1 of each system at [sys * nr + r]
+__host__ __device__ inline int src_bz(int M, int k) { return 4 * (M + 1) + k; }
+__host__ __device__ inline int src_ez(int M, int nm, int k) { return 4 * (M + 1) + nm + k; }
+
+struct SrcArgs {
+    FieldCommon g;
+    const double *P0[kMaxSpecies], *P1[kMaxSpecies];
+    int ns;
+    double *cu_out, *acu_out, *amu_out, *dcu;
+    const double *b_spe;
+    double *src;
+    const IterState *it;
+};
+
+// nodes per k_src workgroup; its LDS tile also holds a halo of 2 nodes on
+// each side (the sources use nodes i-2 .. i+1)
+constexpr int kSrcTile = 32;
+constexpr int kSrcHalo = 2;
+constexpr int kSrcTL = kSrcTile + 2 * kSrcHalo;
+
+// cu (base 0), acu (3) or amu (5) of the tile in LDS as [comp8][k][j],
+// j = i - (i0 - 2)
+struct TileArr {
+    const double *s;
+    int nm, base, i0;
+    __device__ __forceinline__ double operator()(int c, int k, int i) const
+    {
+        return s[((size_t)(base + c) * nm + k) * kSrcTL + (i - i0 + kSrcHalo)];
+    }
+};
+
+// cu, acu, amu at node i (0 <= i <= nr+1) from the per-cell partials of
+// every species, with the axis rules and 1/r of amjdeposit_robust_part2d
+// (part2d_class.f03:914-1004).  P0[cell][v] is the deposit of cell `cell`
+// on its inner node, P1 on its outer node, v = comp8 * nm + k.  Split in an
+// address step (clamped, so the loads are always valid) and a combine step.
+__device__ __forceinline__ void partial_addr(const FieldCommon &g, int comp, int k, int i, size_t &c0,
+                                             size_t &c1)
+{
+    const int nm = g.nm, nr = g.nr, NV = 8 * nm, v = comp * nm + k;
+    c0 = (size_t)min(max(i - 1, 0), nr - 1) * NV + v;
+    c1 = (size_t)min(max(i - 2, 0), nr - 1) * NV + v;
+}
+
+template <int NS>
+__device__ __forceinline__ double combine_node(int nr, int comp, int k, int i, const double (&d0)[NS],
+                                               const double (&d1)[NS])
+{
+    const int m = (k + 1) >> 1;
+    const bool in0 = i >= 1 && i <= nr, in1 = i >= 2;
+    double val = 0.0;
+#pragma unroll
+    for (int s = 0; s < NS; s++) {
+        if (in0) val += d0[s];
+        if (in1) val += d1[s];
+    }
+    double fct;
+    if (m == 0) fct = (comp == 2) ? 8.0 : 0.0;
+    else if (m == 1) fct = (comp <= 1 || comp == 3 || comp == 4) ? 8.0 : 0.0;
+    else if (m == 2) fct = (comp >= 5) ? 8.0 : 0.0;
+    else fct = 0.0;
+    const double axis = (fct == 0.0) ? 0.0 : fct * val;
+    const double inner = val * (1.0 / (double)max(i - 1, 1));
+    return i == 0 ? 0.0 : (i == 1 ? axis : inner);
+}
+
+// one workgroup per tile of kSrcTile nodes: (1) cu, acu, amu of the tile
+// and its halo into LDS; (2) B+/B- sources of every mode and dJ/dxi at the
+// tile's nodes 1..nr; (3) cu, acu, amu out (nodes 0..nr+1) and the Bz and Ez
+// sources of every mode array
+template <int NS>
+__global__ __launch_bounds__(256) void k_src(SrcArgs a)
+{
+    extern __shared__ double tile[];
+    if (a.it->done) return;
+    const FieldCommon &g = a.g;
+    const int nm = g.nm, ld = g.ld, nr = g.nr, M = g.M;
+    const int t = threadIdx.x, T = blockDim.x;
+    const int i0 = blockIdx.x * kSrcTile;
+    const int n_tile = 8 * nm * kSrcTL;
+    // 8 entries per thread per pass so that their loads are in flight together
+    for (int base = t; base < n_tile; base += 8 * T) {
+        double d0[8][NS], d1[8][NS];
+#pragma unroll
+        for (int u = 0; u < 8; u++) {
+            const int idx = min(base + u * T, n_tile - 1);
+            const int comp = idx / (nm * kSrcTL), k = (idx / kSrcTL) % nm, j = idx % kSrcTL;
+            const int i = min(max(i0 - kSrcHalo + j, 0), nr + 1);
+            size_t c0, c1;
+            partial_addr(g, comp, k, i, c0, c1);
+#pragma unroll
+            for (int q = 0; q < NS; q++) {
+                d0[u][q] = a.P0[q][c0];
+                d1[u][q] = a.P1[q][c1];
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < 8; u++) {
+            const int idx = base + u * T;
+            if (idx >= n_tile) break;
+            const int comp = idx / (nm * kSrcTL), k = (idx / kSrcTL) % nm, j = idx % kSrcTL;
+            const int i = i0 - kSrcHalo + j;
+            const double v = combine_node<NS>(nr, comp, k, min(max(i, 0), nr + 1), d0[u], d1[u]);
+            tile[idx] = (i >= 0 && i <= nr + 1) ? v : 0.0;
+        }
+    }
+    __syncthreads();
+    const TileArr cu{tile, nm, 0, i0}, acu{tile, nm, 3, i0}, amu{tile, nm, 5, i0};
+    // B+/B- of mode m and dJ/dxi
+    for (int it = t; it < kSrcTile * (M + 1); it += T) {
+        const int m = it / kSrcTile, i = i0 + it % kSrcTile;
+        if (i < 1 || i > nr) continue;
+        const int kr = m == 0 ? 0 : 2 * m - 1, ki = 2 * m;
+        double dr_[2], di_[2] = {0.0, 0.0}, bpm[4];
+        djdxi_node(g, acu, amu, kr, i, dr_);
+        if (m > 0) djdxi_node(g, acu, amu, ki, i, di_);
+        bpm_source(g, cu, FieldArr{a.b_spe, nm, ld}, m, i, dr_, di_, bpm);
+        FX(a.dcu, 0, kr, i) = dr_[0];
+        FX(a.dcu, 1, kr, i) = dr_[1];
+        if (m > 0) {
+            FX(a.dcu, 0, ki, i) = di_[0];
+            FX(a.dcu, 1, ki, i) = di_[1];
+        }
+#pragma unroll
+        for (int q = 0; q < 4; q++) a.src[(size_t)(4 * m + q) * nr + i - 1] = bpm[q];
+    }
+    // cu, acu, amu out; Bz and Ez sources
+    for (int it = t; it < kSrcTile * nm; it += T) {
+        const int k = it / kSrcTile, i = i0 + it % kSrcTile, m = mode_of(k);
+        if (i > nr + 1) continue;
+#pragma unroll
+        for (int c = 0; c < 3; c++) FX(a.cu_out, c, k, i) = cu(c, k, i);
+#pragma unroll
+        for (int c = 0; c < 2; c++) FX(a.acu_out, c, k, i) = acu(c, k, i);
+#pragma unroll
+        for (int c = 0; c < 3; c++) FX(a.amu_out, c, k, i) = amu(c, k, i);
+        if (i < 1 || i > nr) continue;
+        a.src[(size_t)src_bz(M, k) * nr + i - 1] = bz_source(g, cu, k, i);
+        a.src[(size_t)src_ez(M, nm, k) * nr + i - 1] =
+            (m == 0) ? ez_source_0(g, cu, i) : ez_source_m(g, cu, k, i);
+    }
+}
+
+struct SolveArgs {
+    FieldCommon g;
+    const double *src;
+    double *b_spe, *e, *b_out;
+    const double *b_beam;   // slice j of the beam B history
+    const double *psi, *cu;
+    double *conv_re, *conv_im;
+    IterState *it;
+    double reltol, abstol;
+    int iter_max;
+};
+
+// rows r = t + kk * kSolveThreads of a system (clamped loads, so that all K
+// are in flight together; rows past nr read as 0)
+template <int K>
+__device__ inline void load_rows(const double *p, int nr, double (&v)[K])
+{
+    double x[K];
+#pragma unroll
+    for (int kk = 0; kk < K; kk++) x[kk] = p[min((int)threadIdx.x + kk * kSolveThreads, nr - 1)];
+#pragma unroll
+    for (int kk = 0; kk < K; kk++) v[kk] = ((int)threadIdx.x + kk * kSolveThreads < nr) ? x[kk] : 0.0;
+}
+
+// one component row of a slice field at nodes i = r + 1 (clamped loads)
+template <int K>
+__device__ inline void load_nodes(const double *row, int nr, double (&v)[K])
+{
+#pragma unroll
+    for (int kk = 0; kk < K; kk++) v[kk] = row[min((int)threadIdx.x + kk * kSolveThreads, nr - 1) + 1];
+}
+
+// psi row (nodes 0..nr+1) of mode array k into LDS, all loads in flight
+template <int K>
+__device__ inline void stage_psi(double *st, const double *psi_row, int nr)
+{
+    double x[K + 1];
+#pragma unroll
+    for (int kk = 0; kk <= K; kk++) x[kk] = psi_row[min((int)threadIdx.x + kk * kSolveThreads, nr + 1)];
+#pragma unroll
+    for (int kk = 0; kk <= K; kk++) {
+        const int i = (int)threadIdx.x + kk * kSolveThreads;
+        if (i <= nr + 1) st[i] = x[kk];
+    }
+}
+
+struct LdsRow {
+    const double *s;
+    __device__ __forceinline__ double operator()(int, int, int i) const { return s[i]; }
+};
+
+template <int K>
+__global__ __launch_bounds__(kSolveThreads) void k_solve(SolveArgs a)
+{
+    extern __shared__ double stage[];
+    __shared__ double red[kRed];
+    __shared__ int s_last;
+    if (a.it->done) return;
+    const FieldCommon &g = a.g;
+    const int nm = g.nm, ld = g.ld, nr = g.nr, M = g.M;
+    const int t = threadIdx.x;
+    const int b = blockIdx.x;
+    if (b <= M) {
+        // ---- B_perp of mode m: (nabla^2_{m+-1} - alpha) B+- = S+- ----
+        const int m = b, kr = m == 0 ? 0 : 2 * m - 1, ki = 2 * m;
+        double v[4][K], d[4][K];
+#pragma unroll
+        for (int s = 0; s < 4; s++) load_rows<K>(a.src + (size_t)(4 * m + s) * nr, nr, v[s]);
+#pragma unroll
+        for (int s = 0; s < 4; s++) stage_in<K>(stage, v[s], nr, d[s]);
+        const FactorT fp = dfactor(g, FK_BPLUS, m), fm = dfactor(g, FK_BMINUS, m);
+        const FactorT f[4] = {fp, fp, fm, fm};
+        seg_solve<K, 4>(f, d, red);
+#pragma unroll
+        for (int s = 0; s < 4; s++) stage_out<K>(stage, d[s], v[s]);
+        // recombination into b_spe (field_b_class.f03:715-752)
+        double bs[4][K];
+#pragma unroll
+        for (int kk = 0; kk < K; kk++) {
+            const int i = t + kk * kSolveThreads + 1;
+            const double sv[4] = {v[0][kk], v[1][kk], v[2][kk], v[3][kk]};
+            double o[4];
+            bpm_recombine(m, i, sv, o);
+#pragma unroll
+            for (int q = 0; q < 4; q++) bs[q][kk] = o[q];
+        }
+        // b = b_spe + b_beam and E_perp (field_e_class.f03:443-504) of both
+        // mode arrays; psi comes through LDS because E_perp reads neighbours
+#pragma unroll
+        for (int q = 0; q < 2; q++) {
+            if (q == 1 && m == 0) break;
+            const int k = q == 0 ? kr : ki;
+            double bbr[K], bbp[K];
+            load_nodes<K>(&FX(a.b_beam, 0, k, 0), nr, bbr);
+            load_nodes<K>(&FX(a.b_beam, 1, k, 0), nr, bbp);
+            const int p = partner(k);
+            double pp[K];
+            load_nodes<K>(&FX(a.psi, 0, p, 0), nr, pp);
+            const double pp2 = FX(a.psi, 0, p, 2);
+            stage_psi<K>(stage, &FX(a.psi, 0, k, 0), nr);
+            __syncthreads();
+            const LdsRow psi{stage};
+#pragma unroll
+            for (int kk = 0; kk < K; kk++) {
+                const int r = t + kk * kSolveThreads, i = r + 1;
+                if (r >= nr) continue;
+                const double bsr = bs[q][kk], bsp = bs[2 + q][kk];
+                FX(a.b_spe, 0, k, i) = bsr;
+                FX(a.b_spe, 1, k, i) = bsp;
+                const double br = bsr + bbr[kk], bphi = bsp + bbp[kk];
+                FX(a.b_out, 0, k, i) = br;
+                FX(a.b_out, 1, k, i) = bphi;
+                double ev[2];
+                et_core(g, br, bphi, psi, pp[kk], pp2, k, i, ev);
+                FX(a.e, 0, k, i) = ev[0];
+                FX(a.e, 1, k, i) = ev[1];
+            }
+            __syncthreads();
+        }
+        if (t < 4) {
+            // guard nodes of b (b_spe and b_beam guards are never solved for)
+            const int q = t >> 1, i = (t & 1) ? nr + 1 : 0;
+            if (!(q == 1 && m == 0)) {
+                const int k = q == 0 ? kr : ki;
+                for (int c = 0; c < 2; c++) FX(a.b_out, c, k, i) = FX(a.b_spe, c, k, i) + FX(a.b_beam, c, k, i);
+            }
+        }
+        conv_compare_last<K>(g, a.b_spe, a.conv_re, a.conv_im, a.it, a.reltol, a.abstol, a.iter_max, red,
+                             &s_last);
+        return;
+    }
+    if (b < M + 1 + nm) {
+        // ---- Bz of mode array k (field_b_class.f03:760-796) ----
+        const int k = b - (M + 1), m = mode_of(k);
+        double v[K], d[1][K];
+        load_rows<K>(a.src + (size_t)src_bz(M, k) * nr, nr, v);
+        stage_in<K>(stage, v, nr, d[0]);
+        const FactorT f[1] = {dfactor(g, FK_BZ, m)};
+        seg_solve<K, 1>(f, d, red);
+        stage_out<K>(stage, d[0], v);
+        double bb[K];
+        load_nodes<K>(&FX(a.b_beam, 2, k, 0), nr, bb);
+#pragma unroll
+        for (int kk = 0; kk < K; kk++) {
+            const int r = t + kk * kSolveThreads, i = r + 1;
+            if (r >= nr) continue;
+            const double bz = (m > 0 && i == 1) ? 0.0 : v[kk];
+            FX(a.b_spe, 2, k, i) = bz;
+            FX(a.b_out, 2, k, i) = bz + bb[kk];
+        }
+        if (t < 2) {
+            const int i = t ? nr + 1 : 0;
+            FX(a.b_out, 2, k, i) = FX(a.b_spe, 2, k, i) + FX(a.b_beam, 2, k, i);
+        }
+        return;
+    }
+    // ---- Ez of mode array k (field_e_class.f03:148-334) ----
+    const int k = b - (M + 1 + nm), m = mode_of(k);
+    double v[K], d[1][K];
+    load_rows<K>(a.src + (size_t)src_ez(M, nm, k) * nr, nr, v);
+    if (m == 0) {
+        // discrete Gauss law on the axis row
+        double part = 0.0;
+#pragma unroll
+        for (int kk = 0; kk < K; kk++) {
+            const int i = t + kk * kSolveThreads + 1;
+            if (i >= 2 && i <= nr - 2) part += v[kk] * (double)(i - 1);
+        }
+        double div = block_sum(part, red);
+        if (t == 0) {
+            div = div - ez_axis_edge(g, FieldArr{a.cu, nm, ld});
+            v[0] = -8.0 * div;
+        }
+    }
+    stage_in<K>(stage, v, nr, d[0]);
+    const FactorT f[1] = {dfactor(g, FK_EZ, m)};
+    seg_solve<K, 1>(f, d, red);
+    stage_out<K>(stage, d[0], v);
+#pragma unroll
+    for (int kk = 0; kk < K; kk++) {
+        const int r = t + kk * kSolveThreads, i = r + 1;
+        if (r < nr) FX(a.e, 2, k, i) = (m > 0 && i == 1) ? 0.0 : v[kk];
+    }
+}
+
+int launch_iteration_fields(qpad_ctx *c, int j)
+{
+    const FieldCommon g = field_common(c);
+    SrcArgs s;
+    memset(&s, 0, sizeof(s));
+    s.g = g;
+    s.ns = c->nspecies;
+    for (int q = 0; q < c->nspecies; q++) {
+        s.P0[q] = c->sp[q].P0;
+        s.P1[q] = c->sp[q].P1;
+    }
+    s.cu_out = c->f[QPAD_F_CU];
+    s.acu_out = c->f[QPAD_F_ACU];
+    s.amu_out = c->f[QPAD_F_AMU];
+    s.dcu = c->f[QPAD_F_DCU];
+    s.b_spe = c->f[QPAD_F_B_SPE];
+    s.src = c->src;
+    s.it = c->it;
+    const size_t src_lds = sizeof(double) * 8 * c->g.nm * kSrcTL;
+    const dim3 sgrid((unsigned)((c->g.nr + 2 + kSrcTile - 1) / kSrcTile));
+    switch (c->nspecies) {
+    case 1: hipLaunchKernelGGL(k_src<1>, sgrid, dim3(256), src_lds, c->stream, s); break;
+    case 2: hipLaunchKernelGGL(k_src<2>, sgrid, dim3(256), src_lds, c->stream, s); break;
+    case 3: hipLaunchKernelGGL(k_src<3>, sgrid, dim3(256), src_lds, c->stream, s); break;
+    default: hipLaunchKernelGGL(k_src<4>, sgrid, dim3(256), src_lds, c->stream, s); break;
+    }
+    QPAD_HIP_CHECK(hipGetLastError());
+
+    SolveArgs a;
+    a.g = g;
+    a.src = c->src;
+    a.b_spe = c->f[QPAD_F_B_SPE];
+    a.e = c->f[QPAD_F_E];
+    a.b_out = c->f[QPAD_F_B];
+    a.b_beam = c->h_b_beam + (size_t)j * 3 * c->g.nm * c->g.ld;
+    a.psi = c->f[QPAD_F_PSI];
+    a.cu = c->f[QPAD_F_CU];
+    a.conv_re = c->conv_re;
+    a.conv_im = c->conv_im;
+    a.it = c->it;
+    a.reltol = c->g.reltol;
+    a.abstol = c->g.abstol;
+    a.iter_max = c->g.iter_max;
+    const int nblk = (c->g.M + 1) + 2 * c->g.nm;
+    QPAD_DISPATCH_K(c->g.nr, k_solve, dim3(nblk), 0, c->stream, a);
+    QPAD_HIP_CHECK(hipGetLastError());
+    return 0;
+}
+
+// ---------------------------------------------------------------------------
+// slice epilogue (simulation_class.f03:401-456): cbq, E of the plasma alone,
+// history copies, beam fields of slice j into the slice arrays
+// ---------------------------------------------------------------------------
+struct EpiArgs {
+    FieldCommon g;
+    const double *e, *b, *psi, *cu, *b_spe, *q_beam_j, *b_beam_j;
+    double *e_spe, *q_spe;
+    double *h_e, *h_b, *h_psi, *h_b_spe, *h_e_spe, *h_cu, *h_q_spe;
+    double *q_beam_f1, *b_beam_f1;
+    IterState *it;
+};
+
+// thread per (node i = 0..nr+1, mode array k)
+__global__ __launch_bounds__(256) void k_epilogue(EpiArgs a)
+{
+    const FieldCommon &g = a.g;
+    const int nm = g.nm, ld = g.ld, nr = g.nr;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int k = blockIdx.y;
+    if (i > nr + 1) return;
+    if (i == 0 && k == 0) a.it->total_iters += a.it->iters;
+    for (int c = 0; c < 3; c++) {
+        FX(a.h_e, c, k, i) = FX(a.e, c, k, i);
+        FX(a.h_b, c, k, i) = FX(a.b, c, k, i);
+        FX(a.h_cu, c, k, i) = FX(a.cu, c, k, i);
+        FX(a.h_b_spe, c, k, i) = FX(a.b_spe, c, k, i);
+        FX(a.b_beam_f1, c, k, i) = FX(a.b_beam_j, c, k, i);
+    }
+    // E_perp of the plasma fields alone (e_spe, for the next slice's pipeline
+    // and diagnostics)
+    double ev[2];
+    if (i >= 1 && i <= nr) {
+        et_node(g, FX(a.b_spe, 0, k, i), FX(a.b_spe, 1, k, i), FieldArr{a.psi, nm, ld}, k, i, ev);
+        FX(a.e_spe, 0, k, i) = ev[0];
+        FX(a.e_spe, 1, k, i) = ev[1];
+    } else {
+        ev[0] = FX(a.e_spe, 0, k, i);
+        ev[1] = FX(a.e_spe, 1, k, i);
+    }
+    FX(a.h_e_spe, 0, k, i) = ev[0];
+    FX(a.h_e_spe, 1, k, i) = ev[1];
+    FX(a.h_e_spe, 2, k, i) = FX(a.e_spe, 2, k, i);
+    // q_spe = (rho - Jz) + Jz (species2d%cbq)
+    const size_t o = (size_t)k * ld + i;
+    const double q = a.q_spe[o] + FX(a.cu, 2, k, i);
+    a.q_spe[o] = q;
+    a.h_q_spe[o] = q;
+    a.h_psi[o] = a.psi[o];
+    a.q_beam_f1[o] = a.q_beam_j[o];
+}
+
+int launch_epilogue(qpad_ctx *c, int j)
+{
+    const size_t s1 = (size_t)j * c->g.nm * c->g.ld, s3 = 3 * s1;
+    EpiArgs a;
+    a.g = field_common(c);
+    a.e = c->f[QPAD_F_E];
+    a.b = c->f[QPAD_F_B];
+    a.psi = c->f[QPAD_F_PSI];
+    a.cu = c->f[QPAD_F_CU];
+    a.b_spe = c->f[QPAD_F_B_SPE];
+    a.q_beam_j = c->h_q_beam + s1;
+    a.b_beam_j = c->h_b_beam + s3;
+    a.e_spe = c->f[QPAD_F_E_SPE];
+    a.q_spe = c->f[QPAD_F_Q_SPE];
+    a.h_e = c->h_e + s3;
+    a.h_b = c->h_b + s3;
+    a.h_psi = c->h_psi + s1;
+    a.h_b_spe = c->h_b_spe + s3;
+    a.h_e_spe = c->h_e_spe + s3;
+    a.h_cu = c->h_cu + s3;
+    a.h_q_spe = c->h_q_spe + s1;
+    a.q_beam_f1 = c->f[QPAD_F_Q_BEAM];
+    a.b_beam_f1 = c->f[QPAD_F_B_BEAM];
+    a.it = c->it;
+    hipLaunchKernelGGL(k_epilogue, dim3((unsigned)((c->g.nr + 2 + 255) / 256), c->g.nm), dim3(256), 0, c->stream,
+                       a);
+    QPAD_HIP_CHECK(hipGetLastError());
+    return 0;
+}
+
+template <int K>
+static int set_lds_attr_slice()
+{
+    // the solve stage; it later holds one psi row (nr + 2 <= stage)
+    const int bytes = 8 * stage_doubles<K>();
+    QPAD_HIP_CHECK(hipFuncSetAttribute((const void *)k_solve<K>, hipFuncAttributeMaxDynamicSharedMemorySize, bytes));
+    return 0;
+}
+
+int slice_kernels_init()
+{
+    const void *srcs[] = {(const void *)k_src<1>, (const void *)k_src<2>, (const void *)k_src<3>,
+                          (const void *)k_src<4>};
+    for (const void *k : srcs)
+        QPAD_HIP_CHECK(hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                           (int)(sizeof(double) * 8 * kMaxNm * kSrcTL)));
+    QPAD_TRY(set_lds_attr_slice<1>());
+    QPAD_TRY(set_lds_attr_slice<2>());
+    QPAD_TRY(set_lds_attr_slice<4>());
+    QPAD_TRY(set_lds_attr_slice<8>());
+    QPAD_TRY(set_lds_attr_slice<16>());
+    return 0;
+}
+
+}  // namespace qpad

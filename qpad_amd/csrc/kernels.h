@@ -34,10 +34,19 @@ int launch_bt_beam_one(qpad_ctx *c, int j);
 // convergence_tester 'compare' alone
 int launch_compare_only(qpad_ctx *c);
 // J_perp predictor: dcu *= dxi ; cu(1:2) += dcu(1:2)
+// fused predictor-corrector iteration after the current deposit: combine +
+// all sources (k_src), then B_perp, Bz, Ez, E_perp, b and the convergence
+// test (k_solve); both return at once once the iteration has converged
+int launch_iteration_fields(qpad_ctx *c, int j);
+// slice epilogue: cbq, e_spe, history copies, beam fields of slice j
+int launch_epilogue(qpad_ctx *c, int j);
+int slice_kernels_init();
 int launch_predictor(qpad_ctx *c);
 
 // ---- plasma (k_plasma.hip) ----
 int launch_inject(qpad_ctx *c, int sp);
+// cell_start[c] = lower bound of c in the sorted keys, c = 0 .. ncs-1
+int launch_cell_start(qpad_ctx *c, const uint32_t *key, int64_t n, int ncs, int32_t *cs);
 int launch_sort(qpad_ctx *c, int sp);            // sort by key, permute, cell starts
 int launch_keys(qpad_ctx *c, int sp);            // keys of the current positions
 int launch_qdeposit(qpad_ctx *c, int sp);        // per-cell partials of rho - Jz

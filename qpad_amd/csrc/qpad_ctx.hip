@@ -246,11 +246,14 @@ int qpad_create(const qpad_grid *gr, qpad_ctx **out)
     QPAD_HIP_CHECK(hipGetDevice(&dev));
     hipDeviceProp_t prop;
     QPAD_HIP_CHECK(hipGetDeviceProperties(&prop, dev));
-    size_t lds_need = sizeof(double) * 4 * (size_t)gr->nr + 1024;
-    if (lds_need > prop.sharedMemPerBlock)
-        return fail(QPAD_E_INVALID, "nr too large for the LDS-resident B_perp solve");
+    if (gr->nr > 16 * kSolveThreads) return fail(QPAD_E_INVALID, "nr must be <= 8192");
+    // the field solves stage (K + 1) * kSolveThreads doubles in LDS
+    const int kpt = rows_per_thread(gr->nr);
+    size_t lds_need = sizeof(double) * (size_t)(kpt + 1) * kSolveThreads;
+    if (lds_need + 1024 > prop.sharedMemPerBlock)
+        return fail(QPAD_E_INVALID, "nr too large for this device's LDS");
 
-    if (field_kernels_init(lds_need) != 0) return QPAD_E_HIP;
+    if (field_kernels_init(lds_need) != 0 || slice_kernels_init() != 0) return QPAD_E_HIP;
     qpad_ctx *c = new qpad_ctx{};
     c->device = dev;
     Grid &g = c->g;
@@ -292,8 +295,13 @@ int qpad_create(const qpad_grid *gr, qpad_ctx **out)
     if (!rc) rc = dalloc(&c->conv_re, (size_t)g.nr + 2);
     if (!rc) rc = dalloc(&c->conv_im, (size_t)g.nr + 2);
     if (!rc) rc = dalloc(&c->it, 1);
-    size_t nfac = (size_t)FK_N * (kMaxMode + 1) * 3 * g.nr;
+    // factors stored transposed for the solve kernels: [kind][m][lo|inv|cp]
+    // then row r = t * K + kk at kk * kSolveThreads + t; rows >= nr are
+    // identity rows (lo = 0, inv = 1, cp = 0)
+    const size_t kt = (size_t)kpt * kSolveThreads;
+    size_t nfac = (size_t)FK_N * (kMaxMode + 1) * 3 * kt;
     if (!rc) rc = dalloc(&c->fac, nfac);
+    if (!rc) rc = dalloc(&c->src, (size_t)(4 * (g.M + 1) + 2 * g.nm) * g.nr);
     if (!rc) {
         he = hipHostMalloc((void **)&c->it_host, sizeof(IterState), hipHostMallocDefault);
         if (he != hipSuccess) rc = fail(QPAD_E_NOMEM, "hipHostMalloc failed");
@@ -303,15 +311,22 @@ int qpad_create(const qpad_grid *gr, qpad_ctx **out)
         return rc;
     }
     // solver factors
-    std::vector<double> hf(nfac, 0.0), a(g.nr), b(g.nr), cc(g.nr);
+    std::vector<double> hf(nfac, 0.0), a(g.nr), b(g.nr), cc(g.nr), lo(g.nr), inv(g.nr), cp(g.nr);
     for (int kind = 0; kind < FK_N; kind++)
         for (int m = 0; m <= g.M; m++) {
             build_matrix(kind, m, g.nr, g.dr, g.relax_fac, g.bnd, a.data(), b.data(), cc.data());
-            double *base = hf.data() + (((size_t)kind * (kMaxMode + 1) + m) * 3) * g.nr;
-            thomas_factor(g.nr, a.data(), b.data(), cc.data(), base, base + g.nr, base + 2 * g.nr);
+            thomas_factor(g.nr, a.data(), b.data(), cc.data(), lo.data(), inv.data(), cp.data());
+            double *base = hf.data() + (((size_t)kind * (kMaxMode + 1) + m) * 3) * kt;
+            for (size_t r = 0; r < kt; r++) {
+                const size_t t = r / kpt, kk = r % kpt, at = kk * kSolveThreads + t;
+                const bool in = r < (size_t)g.nr;
+                base[at] = in ? lo[r] : 0.0;
+                base[kt + at] = in ? inv[r] : 1.0;
+                base[2 * kt + at] = in ? cp[r] : 0.0;
+            }
         }
     rc = 0;
-    he = hipMemcpyAsync(c->fac, hf.data(), nfac * sizeof(double), hipMemcpyHostToDevice, c->stream);
+    he = hipMemcpy(c->fac, hf.data(), nfac * sizeof(double), hipMemcpyHostToDevice);
     if (he != hipSuccess) rc = fail(QPAD_E_HIP, "upload of solver factors failed");
     // zero every array once (fill kernels, not memset)
     for (int f = 0; f < QPAD_NFIELD && !rc; f++) rc = zero_dev(c, c->f[f], f1_size(c, c->ncomp[f]));
@@ -344,7 +359,7 @@ int qpad_destroy(qpad_ctx *c)
     for (int f = 0; f < QPAD_NFIELD; f++) dfree(c->f[f]);
     dfree(c->h_psi); dfree(c->h_e); dfree(c->h_b); dfree(c->h_b_spe); dfree(c->h_e_spe);
     dfree(c->h_cu); dfree(c->h_q_spe); dfree(c->h_q_beam); dfree(c->h_b_beam);
-    dfree(c->conv_re); dfree(c->conv_im); dfree(c->it); dfree(c->fac);
+    dfree(c->conv_re); dfree(c->conv_im); dfree(c->it); dfree(c->fac); dfree(c->src);
     if (c->it_host) (void)hipHostFree(c->it_host);
     for (int s = 0; s < c->nspecies; s++) {
         Species &sp = c->sp[s];
@@ -498,38 +513,58 @@ int qpad_step3d_begin(qpad_ctx *c)
     return sync(c);
 }
 
-int qpad_slice(qpad_ctx *c, int32_t j, int32_t *iters_out)
+}  // extern "C"
+
+namespace qpad {
+
+// One xi-slice (simulation_class.f03:342-469), enqueued without host
+// synchronisation: all iter_max predictor-corrector iterations are queued and
+// each of their kernels returns at once after the device-side convergence
+// test has set IterState::done.
+static int slice_enqueue(qpad_ctx *c, int j)
 {
-    if (!c) return fail(QPAD_E_INVALID, "null context");
-    if (j < 1 || j > c->g.nxi) return fail(QPAD_E_INVALID, "slice index out of range");
     // rho - Jz of every species, psi, Bz from the J predictor, 'record'
     for (int s = 0; s < c->nspecies; s++) TRY(launch_qdeposit(c, s));
     hipEvent_t stop = nullptr;
     time_begin(c, 2, &stop);
     TRY(launch_psi_bz(c, true, true, true));
+    // E and b for the first deposit (simulation_class.f03:372-377)
+    TRY(launch_ez_et(c, j, false, false));
     time_end(c, 2, stop);
-    int iters = 0;
+    // speculative iterations: each kernel returns at once after convergence
     for (int l = 1; l <= c->g.iter_max; l++) {
-        TRY(launch_ez_et(c, j, false, false));
-        for (int s = 0; s < c->nspecies; s++) TRY(launch_amjdeposit(c, s, false, true));
-        TRY(launch_combine_j(c, false));
+        for (int s = 0; s < c->nspecies; s++) TRY(launch_amjdeposit(c, s, true, true));
         time_begin(c, 2, &stop);
-        TRY(launch_bperp_bz(c, true, true, true, false));
+        TRY(launch_iteration_fields(c, j));
         time_end(c, 2, stop);
-        QPAD_HIP_CHECK(hipMemcpyAsync(c->it_host, c->it, sizeof(IterState), hipMemcpyDeviceToHost, c->stream));
-        TRY(sync(c));
-        iters = c->it_host->iters;
-        if (c->it_host->done) break;
     }
-    // slice epilogue: cbq, history, final E (simulation_class.f03:401-456)
-    TRY(launch_ez_et(c, j, true, false));
+    // slice epilogue: cbq, e_spe, history (simulation_class.f03:401-456)
+    TRY(launch_epilogue(c, j));
     TRY(launch_predictor(c));
     for (int s = 0; s < c->nspecies; s++) {
         TRY(launch_push(c, s));
         TRY(launch_sort(c, s));
     }
-    TRY(sync(c));
-    if (iters_out) *iters_out = iters;
+    return 0;
+}
+
+static int read_iter_state(qpad_ctx *c)
+{
+    QPAD_HIP_CHECK(hipMemcpyAsync(c->it_host, c->it, sizeof(IterState), hipMemcpyDeviceToHost, c->stream));
+    return sync(c);
+}
+
+}  // namespace qpad
+
+extern "C" {
+
+int qpad_slice(qpad_ctx *c, int32_t j, int32_t *iters_out)
+{
+    if (!c) return fail(QPAD_E_INVALID, "null context");
+    if (j < 1 || j > c->g.nxi) return fail(QPAD_E_INVALID, "slice index out of range");
+    TRY(slice_enqueue(c, j));
+    TRY(read_iter_state(c));
+    if (iters_out) *iters_out = c->it_host->iters;
     return 0;
 }
 
@@ -548,14 +583,14 @@ int qpad_step3d_end(qpad_ctx *c)
 int qpad_step3d(qpad_ctx *c, int32_t *total_iters_out)
 {
     TRY(qpad_step3d_begin(c));
-    int64_t tot = 0;
-    for (int j = 1; j <= c->g.nxi; j++) {
-        int32_t it = 0;
-        TRY(qpad_slice(c, j, &it));
-        tot += it;
-    }
+    TRY(read_iter_state(c));
+    c->it_host->total_iters = 0;
+    QPAD_HIP_CHECK(hipMemcpyAsync(c->it, c->it_host, sizeof(IterState), hipMemcpyHostToDevice, c->stream));
+    for (int j = 1; j <= c->g.nxi; j++) TRY(slice_enqueue(c, j));
+    TRY(read_iter_state(c));
+    const int32_t tot = c->it_host->total_iters;
     TRY(qpad_step3d_end(c));
-    if (total_iters_out) *total_iters_out = (int32_t)tot;
+    if (total_iters_out) *total_iters_out = tot;
     return 0;
 }
 
@@ -856,7 +891,15 @@ int qpad_kernel_time(qpad_ctx *c, int32_t which, double *ms, int64_t *launches)
     if (!c || which < 0 || which > 2) return fail(QPAD_E_INVALID, "bad timer id");
     pool_drain(c, c->pools[which]);
     if (ms) *ms = c->pools[which].acc_ms;
-    if (launches) *launches = c->pools[which].launches;
+    if (launches) {
+        *launches = c->pools[which].launches;
+        if (which == 0) {
+            // deposit launches that did work (the rest returned at once after
+            // convergence); their count since the last reset
+            TRY(read_iter_state(c));
+            *launches = c->it_host->real_deposits - c->dep_base;
+        }
+    }
     return 0;
 }
 
@@ -869,6 +912,8 @@ int qpad_kernel_time_reset(qpad_ctx *c, int32_t enable)
         p.acc_ms = 0.0;
         p.launches = 0;
     }
+    TRY(read_iter_state(c));
+    c->dep_base = c->it_host->real_deposits;
     c->timing = enable != 0;
     return 0;
 }

@@ -20,7 +20,7 @@ constexpr int kMaxMode = 4;          // modes m = 0..4
 constexpr int kMaxNm = 2 * kMaxMode + 1;
 constexpr int kMaxSpecies = 4;
 constexpr int kMaxBeams = 4;
-constexpr int kSolveThreads = 256;   // workgroup of the tridiagonal solves
+constexpr int kSolveThreads = 512;   // workgroup of the tridiagonal solves
 constexpr int kDepWaves = 4;         // waves (cells) per deposit workgroup
 
 // field kinds (source/param.f03:72-80)
@@ -34,12 +34,6 @@ struct Grid {
     double dr, dxi, idr, relax_fac, dt;
     int bnd, iter_max;
     double reltol, abstol;
-};
-
-// Thomas factors of one matrix: lo = sub-diagonal, inv = 1/pivot,
-// cp = super-diagonal/pivot, rows 0..nr-1 = QPAD grid points 1..nr.
-struct Factor {
-    const double *lo, *inv, *cp;
 };
 
 struct Species {
@@ -67,7 +61,7 @@ struct Beam {
     double *x1, *x2, *x3, *p1, *p2, *p3, *q;
     double *a_x1, *a_x2, *a_x3, *a_p1, *a_p2, *a_p3, *a_q;
     uint32_t *key, *key_alt, *idx, *idx_alt;
-    int32_t *cell_start;     // ncell + 1 entries (cell = (mm-1) * nr + (nn-1))
+    int32_t *cell_start;     // ncell + 2 entries (cell = (mm-1) * nr + (nn-1))
     void *sort_tmp;
     size_t sort_tmp_bytes;
 };
@@ -84,7 +78,9 @@ struct IterState {        // device-resident iteration bookkeeping
     int done;             // converged in the current slice
     int counter;          // last-block-done counter of the B-perp solve
     int iters;            // iterations executed in the current slice
-    int pad[3];
+    int total_iters;      // iterations summed over the slices of a 3D step
+    int real_deposits;    // amjdeposit launches that did work (not early exits)
+    int pad;
 };
 
 }  // namespace qpad
@@ -98,9 +94,10 @@ struct qpad_ctx {
     int ncomp[QPAD_NFIELD];
     // histories (slices 0..nxi+1)
     double *h_psi, *h_e, *h_b, *h_b_spe, *h_e_spe, *h_cu, *h_q_spe, *h_q_beam, *h_b_beam;
-    // beam q scratch for the per-beam deposit
-    // solver factors [kind][mode][3][nr]
+    // solver factors [kind][mode][lo|inv|cp][K * kSolveThreads], transposed
     double *fac;
+    // right-hand sides of the iteration's solves [4 (M+1) + 2 nm][nr]
+    double *src;
     // convergence_tester vectors (nr + 2)
     double *conv_re, *conv_im;
     qpad::IterState *it;      // device
@@ -109,6 +106,7 @@ struct qpad_ctx {
     qpad::Species sp[qpad::kMaxSpecies];
     qpad::Beam bm[qpad::kMaxBeams];
     bool timing;
+    int dep_base;             // real_deposits at the last timer reset
     qpad::EventPool pools[3];
     bool step_open;
 };
@@ -132,100 +130,13 @@ void set_error(const std::string &msg);
         if (rc_ != 0) return rc_; \
     } while (0)
 
-inline const double *fac_ptr(const qpad_ctx *c, int kind, int m, int which)
+// rows per thread of the field solves: the smallest power of two K with
+// K * kSolveThreads >= nr (K <= 16)
+inline int rows_per_thread(int nr)
 {
-    return c->fac + (((size_t)kind * (kMaxMode + 1) + m) * 3 + which) * c->g.nr;
-}
-inline Factor factor(const qpad_ctx *c, int kind, int m)
-{
-    return Factor{fac_ptr(c, kind, m, 0), fac_ptr(c, kind, m, 1), fac_ptr(c, kind, m, 2)};
-}
-
-// ---- device helpers ----
-
-// Composition of affine maps y -> A*y + B over the threads of a workgroup.
-// Returns, for thread t, the B-component of (map_{t-1} o ... o map_0), i.e.
-// the value entering thread t's segment when the first segment starts from 0.
-// `red` must hold 2 * (blockDim.x / 64) doubles of LDS.
-__device__ inline double block_affine_exclusive(double A, double B, double *red)
-{
-    const int lane = threadIdx.x & 63;
-    const int wave = threadIdx.x >> 6;
-    const int nwave = blockDim.x >> 6;
-#pragma unroll
-    for (int d = 1; d < 64; d <<= 1) {
-        double Ap = __shfl_up(A, d, 64);
-        double Bp = __shfl_up(B, d, 64);
-        if (lane >= d) {
-            B = A * Bp + B;
-            A = A * Ap;
-        }
-    }
-    if (lane == 63) {
-        red[2 * wave] = A;
-        red[2 * wave + 1] = B;
-    }
-    __syncthreads();
-    // prefix of the earlier waves (applied before this wave's maps)
-    double PA = 1.0, PB = 0.0;
-    for (int w = 0; w < wave; w++) {
-        double wa = red[2 * w], wb = red[2 * w + 1];
-        PB = wa * PB + wb;
-        PA = wa * PA;
-    }
-    (void)nwave;
-    // inclusive result for this thread: (A,B) o (PA,PB)
-    double incB = A * PB + B;
-    // exclusive: value from the previous lane, or the wave prefix for lane 0
-    double exB = __shfl_up(incB, 1, 64);
-    if (lane == 0) exB = PB;
-    __syncthreads();
-    return exB;
-}
-
-// Solve the tridiagonal system with precomputed Thomas factors in place on
-// `d` (n rows, LDS or global), with the whole workgroup.  Each thread owns a
-// contiguous segment; segment maps are combined with a block scan and each
-// segment is then recomputed serially with Thomas' recurrences.
-__device__ inline void block_thomas(const Factor f, double *d, int n, double *red)
-{
-    const int T = blockDim.x;
-    const int t = threadIdx.x;
-    const int K = (n + T - 1) / T;
-    const int s0 = t * K;
-    const int s1 = min(s0 + K, n);
-    // forward: y_i = (d_i - lo_i y_{i-1}) inv_i  =>  y_i = al_i y_{i-1} + be_i
-    double A = 1.0, B = 0.0;
-    for (int i = s0; i < s1; i++) {
-        double inv = f.inv[i];
-        double al = -f.lo[i] * inv;
-        B = al * B + d[i] * inv;
-        A = al * A;
-    }
-    double y = block_affine_exclusive(A, B, red);
-    for (int i = s0; i < s1; i++) {
-        y = (d[i] - f.lo[i] * y) * f.inv[i];
-        d[i] = y;
-    }
-    __syncthreads();
-    // backward: x_i = y_i - cp_i x_{i+1}; thread order reversed
-    const int tr = T - 1 - t;
-    const int r0 = tr * K;
-    const int r1 = min(r0 + K, n);
-    A = 1.0;
-    B = 0.0;
-    for (int i = r1 - 1; i >= r0; i--) {
-        double al = -f.cp[i];
-        B = al * B + d[i];
-        A = al * A;
-    }
-    double x = block_affine_exclusive(A, B, red);
-    // x is the value of x_{r1} entering from above (0 beyond the last row)
-    for (int i = r1 - 1; i >= r0; i--) {
-        x = d[i] - f.cp[i] * x;
-        d[i] = x;
-    }
-    __syncthreads();
+    int k = 1;
+    while (k * kSolveThreads < nr) k <<= 1;
+    return k;
 }
 
 }  // namespace qpad
