@@ -34,7 +34,28 @@ def parse():
     ap.add_argument("--cpu-slices", type=int, default=24)
     ap.add_argument("--cpu-start", type=int, default=100)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--profile-slices", type=int, default=0,
+                    help="run only a 64 MiB calibration copy and this many slices of the workload "
+                         "(for rocprofv3 counter runs), print nothing")
     return ap.parse_args()
+
+
+def profile_slices(args):
+    """Short run for hardware-counter passes: one device copy of known size
+    (read 64 MiB, write 64 MiB) to calibrate FETCH_SIZE / WRITE_SIZE, then
+    the first N slices of one 3D step of the workload."""
+    import torch
+    from qpad_amd import Simulation
+    src = torch.ones(8 << 20, dtype=torch.float64, device="cuda")
+    dst = torch.empty_like(src)
+    dst.copy_(src)
+    torch.cuda.synchronize()
+    sim = Simulation(deck_for(args.workload))
+    sim.step3d_begin()
+    for j in range(1, args.profile_slices + 1):
+        sim.slice(j)
+    sim.close()
+    print(f"profiled {args.profile_slices} slices", flush=True)
 
 
 def deck_for(name):
@@ -138,6 +159,9 @@ def cpu_baseline(sim, per_slice_iters, j0, nslices, cores):
 
 def main():
     args = parse()
+    if args.profile_slices:
+        profile_slices(args)
+        return
     if args.gpus != 1:
         print(json.dumps({"error": "this build targets one MI355X: multi-GPU runs are not "
                                    "part of it (use --gpus 1)", "n_gpus": args.gpus}))
@@ -170,10 +194,15 @@ def main():
     sim.step3d_end()
     j0 = max(1, min(args.cpu_start, g.nxi - args.cpu_slices + 1))
     # deposit kernel roofline: 48 B read (x1, x2, p1, p2, p3, q) + 16 B written
-    # (gamma, psi) per plasma particle per launch
+    # (gamma, psi) per plasma particle per launch that does work.  Iterations
+    # are enqueued speculatively; launches after convergence return at once,
+    # so bytes per launch are averaged over every launch, like the duration
+    # (rocprof's AverageNs of the same kernel).
     dep_avg_s = dep_ms / max(dep_n, 1) / 1e3
+    real_n = total_iters * len(sim.spe)
     dep_bytes = 64.0 * np_slice
-    achieved = dep_bytes / dep_avg_s / 1e9 if dep_avg_s > 0 else 0.0
+    bytes_per_launch = dep_bytes * real_n / max(dep_n, 1)
+    achieved = bytes_per_launch / dep_avg_s / 1e9 if dep_avg_s > 0 else 0.0
     step_bytes = algorithmic_bytes_per_step(g, np_slice, per_slice, nbeam)
     result = {
         "metric": "3D steps/s & plasma particle-slice pushes/s; learned-seed vs plain, same tol",
@@ -199,8 +228,9 @@ def main():
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None,
                      "kernel": "k_amjdeposit (gather + robust predictor + J/dJ/momentum-flux deposit)",
-                     "avg_launch_us": dep_avg_s * 1e6, "launches": dep_n,
-                     "bytes_per_launch": dep_bytes},
+                     "avg_launch_us": dep_avg_s * 1e6, "launches": dep_n, "working_launches": real_n,
+                     "bytes_per_launch": bytes_per_launch, "bytes_per_working_launch": dep_bytes,
+                     "working_launch_us": dep_ms * 1e3 / max(real_n, 1)},
         "step_roofline": {"bytes_per_step": step_bytes,
                           "achieved_GBs": step_bytes * steps_per_s / 1e9,
                           "frac": step_bytes * steps_per_s / 1e9 / HBM_PEAK_GBS},

@@ -205,8 +205,11 @@ int qpad_set_beam(qpad_ctx *ctx, int32_t ib, int64_t n, const double *x1,
 
 /* timing of the dominant kernels (HIP events on the context's stream):
    which = 0: plasma current deposit (amjdeposit), 1: plasma push,
-   2: field solves.  Returns the summed milliseconds and launch count since
-   the last reset. */
+   2: field sources and solves.  Returns the summed milliseconds and the
+   launch count since the last reset.  Iterations are enqueued
+   speculatively: launches after convergence return at once and are
+   counted too (the iterations that did work are what qpad_step3d and
+   qpad_slice report). */
 int qpad_kernel_time(qpad_ctx *ctx, int32_t which, double *ms, int64_t *launches);
 int qpad_kernel_time_reset(qpad_ctx *ctx, int32_t enable);
 

@@ -193,6 +193,40 @@ __device__ __forceinline__ void et_core(const FieldCommon &g, double br, double 
     out[1] = im ? -br - ir * dm * psip_i : -br + ir * dm * psip_i;
 }
 
+// et_core with psi of mode array k given as the window psi(i-2 .. i+1)
+// (clamped at the edges) so that a caller can load it branch-free
+__device__ __forceinline__ void et_window(const FieldCommon &g, double br, double bphi, double psim2,
+                                          double psim1, double psi0, double psip1, double psip_i,
+                                          double psip_2, int k, int i, double out[2])
+{
+    const int nr = g.nr;
+    const int m = mode_of(k);
+    const double idr = g.idr, idrh = g.idrh;
+    if (m == 0) {
+        if (i == 1) { out[0] = 0.0; out[1] = 0.0; return; }
+        if (i == nr) out[0] = bphi + idrh * (4.0 * psim1 - psim2 - 3.0 * psi0);
+        else out[0] = bphi - idrh * (psip1 - psim1);
+        out[1] = -br;
+        return;
+    }
+    const bool im = is_im(k);
+    const double dm = (double)m;
+    if (i == 1) {
+        if (m == 1) {
+            out[0] = bphi - idr * psip1;
+            out[1] = im ? -br - idr * psip_2 : -br + idr * psip_2;
+        } else {
+            out[0] = 0.0;
+            out[1] = 0.0;
+        }
+        return;
+    }
+    double ir = idr / (double)(i - 1);
+    if (i == nr) out[0] = bphi + idrh * (4.0 * psim1 - psim2 - 3.0 * psi0);
+    else out[0] = bphi - idrh * (psip1 - psim1);
+    out[1] = im ? -br - ir * dm * psip_i : -br + ir * dm * psip_i;
+}
+
 template <class A>
 __device__ __forceinline__ void et_node(const FieldCommon &g, double br, double bphi, const A &psi, int k,
                                         int i, double out[2])

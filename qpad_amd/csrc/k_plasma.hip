@@ -398,7 +398,6 @@ struct JDepArgs {
     double *P0, *P1;
     const IterState *it;
     int check_done;
-    int *counter;   // counts launches that did work
 };
 
 template <int M>
@@ -424,7 +423,6 @@ __global__ __launch_bounds__(64 * kDepWaves) void k_amjdeposit(JDepArgs a)
     }
     __syncthreads();
     if (c >= a.nr) return;
-    if (blockIdx.x == 0 && threadIdx.x == 0) a.counter[0] += 1;
     const int p0 = a.cs[c], p1 = a.cs[c + 1];
     const NodePairLds<NM> fe{sf[wave][0]}, fbf{sf[wave][1]};
     const double *__restrict__ X1 = a.x1;
@@ -550,7 +548,6 @@ int launch_amjdeposit(qpad_ctx *c, int s, bool check_done, bool store_diag)
     a.P1 = sp.P1;
     a.it = c->it;
     a.check_done = check_done;
-    a.counter = &c->it->real_deposits;
     dim3 grid((c->g.nr + kDepWaves - 1) / kDepWaves), blk(64 * kDepWaves);
     hipEvent_t stop = nullptr;
     time_begin(c, 0, &stop);

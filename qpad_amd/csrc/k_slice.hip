@@ -24,10 +24,12 @@
 namespace qpad {
 
 // src buffer rows: B+/B- of mode m at 4m + s (s = Re B+, Im B+, Re B-,
-// Im B-), Bz of mode array k at 4(M+1) + k, Ez at 4(M+1) + nm + k; row
-// r = i - 1 of each system at [sys * nr + r]
+// Im B-), Bz of mode array k at 4(M+1) + k, Ez at 4(M+1) + nm + k, psi at
+// 4(M+1) + 2nm + k; row r = i - 1 of each system at [sys * nr + r]
 __host__ __device__ inline int src_bz(int M, int k) { return 4 * (M + 1) + k; }
 __host__ __device__ inline int src_ez(int M, int nm, int k) { return 4 * (M + 1) + nm + k; }
+__host__ __device__ inline int src_psi(int M, int nm, int k) { return 4 * (M + 1) + 2 * nm + k; }
+__host__ __device__ inline int src_rows(int M, int nm) { return 4 * (M + 1) + 3 * nm; }
 
 struct SrcArgs {
     FieldCommon g;
@@ -170,14 +172,10 @@ __global__ __launch_bounds__(256) void k_src(SrcArgs a)
 
 struct SolveArgs {
     FieldCommon g;
-    const double *src;
-    double *b_spe, *e, *b_out;
-    const double *b_beam;   // slice j of the beam B history
-    const double *psi, *cu;
-    double *conv_re, *conv_im;
-    IterState *it;
-    double reltol, abstol;
-    int iter_max;
+    double *src;             // B+/B- solutions are written back into their rows
+    double *b_spe, *e;
+    const double *cu;
+    const IterState *it;
 };
 
 // rows r = t + kk * kSolveThreads of a system (clamped loads, so that all K
@@ -192,144 +190,43 @@ __device__ inline void load_rows(const double *p, int nr, double (&v)[K])
     for (int kk = 0; kk < K; kk++) v[kk] = ((int)threadIdx.x + kk * kSolveThreads < nr) ? x[kk] : 0.0;
 }
 
-// one component row of a slice field at nodes i = r + 1 (clamped loads)
-template <int K>
-__device__ inline void load_nodes(const double *row, int nr, double (&v)[K])
-{
-#pragma unroll
-    for (int kk = 0; kk < K; kk++) v[kk] = row[min((int)threadIdx.x + kk * kSolveThreads, nr - 1) + 1];
-}
-
-// psi row (nodes 0..nr+1) of mode array k into LDS, all loads in flight
-template <int K>
-__device__ inline void stage_psi(double *st, const double *psi_row, int nr)
-{
-    double x[K + 1];
-#pragma unroll
-    for (int kk = 0; kk <= K; kk++) x[kk] = psi_row[min((int)threadIdx.x + kk * kSolveThreads, nr + 1)];
-#pragma unroll
-    for (int kk = 0; kk <= K; kk++) {
-        const int i = (int)threadIdx.x + kk * kSolveThreads;
-        if (i <= nr + 1) st[i] = x[kk];
-    }
-}
-
-struct LdsRow {
-    const double *s;
-    __device__ __forceinline__ double operator()(int, int, int i) const { return s[i]; }
-};
-
+// one workgroup per system: 4 (M+1) B+/B- systems (field_b_class.f03:
+// 388-497; the m = 0 imaginary ones are empty and skipped), then Bz and Ez
+// of every mode array
 template <int K>
 __global__ __launch_bounds__(kSolveThreads) void k_solve(SolveArgs a)
 {
     extern __shared__ double stage[];
     __shared__ double red[kRed];
-    __shared__ int s_last;
     if (a.it->done) return;
     const FieldCommon &g = a.g;
     const int nm = g.nm, ld = g.ld, nr = g.nr, M = g.M;
     const int t = threadIdx.x;
     const int b = blockIdx.x;
-    if (b <= M) {
-        // ---- B_perp of mode m: (nabla^2_{m+-1} - alpha) B+- = S+- ----
-        const int m = b, kr = m == 0 ? 0 : 2 * m - 1, ki = 2 * m;
-        double v[4][K], d[4][K];
-#pragma unroll
-        for (int s = 0; s < 4; s++) load_rows<K>(a.src + (size_t)(4 * m + s) * nr, nr, v[s]);
-#pragma unroll
-        for (int s = 0; s < 4; s++) stage_in<K>(stage, v[s], nr, d[s]);
-        const FactorT fp = dfactor(g, FK_BPLUS, m), fm = dfactor(g, FK_BMINUS, m);
-        const FactorT f[4] = {fp, fp, fm, fm};
-        seg_solve<K, 4>(f, d, red);
-#pragma unroll
-        for (int s = 0; s < 4; s++) stage_out<K>(stage, d[s], v[s]);
-        // recombination into b_spe (field_b_class.f03:715-752)
-        double bs[4][K];
-#pragma unroll
-        for (int kk = 0; kk < K; kk++) {
-            const int i = t + kk * kSolveThreads + 1;
-            const double sv[4] = {v[0][kk], v[1][kk], v[2][kk], v[3][kk]};
-            double o[4];
-            bpm_recombine(m, i, sv, o);
-#pragma unroll
-            for (int q = 0; q < 4; q++) bs[q][kk] = o[q];
-        }
-        // b = b_spe + b_beam and E_perp (field_e_class.f03:443-504) of both
-        // mode arrays; psi comes through LDS because E_perp reads neighbours
-#pragma unroll
-        for (int q = 0; q < 2; q++) {
-            if (q == 1 && m == 0) break;
-            const int k = q == 0 ? kr : ki;
-            double bbr[K], bbp[K];
-            load_nodes<K>(&FX(a.b_beam, 0, k, 0), nr, bbr);
-            load_nodes<K>(&FX(a.b_beam, 1, k, 0), nr, bbp);
-            const int p = partner(k);
-            double pp[K];
-            load_nodes<K>(&FX(a.psi, 0, p, 0), nr, pp);
-            const double pp2 = FX(a.psi, 0, p, 2);
-            stage_psi<K>(stage, &FX(a.psi, 0, k, 0), nr);
-            __syncthreads();
-            const LdsRow psi{stage};
-#pragma unroll
-            for (int kk = 0; kk < K; kk++) {
-                const int r = t + kk * kSolveThreads, i = r + 1;
-                if (r >= nr) continue;
-                const double bsr = bs[q][kk], bsp = bs[2 + q][kk];
-                FX(a.b_spe, 0, k, i) = bsr;
-                FX(a.b_spe, 1, k, i) = bsp;
-                const double br = bsr + bbr[kk], bphi = bsp + bbp[kk];
-                FX(a.b_out, 0, k, i) = br;
-                FX(a.b_out, 1, k, i) = bphi;
-                double ev[2];
-                et_core(g, br, bphi, psi, pp[kk], pp2, k, i, ev);
-                FX(a.e, 0, k, i) = ev[0];
-                FX(a.e, 1, k, i) = ev[1];
-            }
-            __syncthreads();
-        }
-        if (t < 4) {
-            // guard nodes of b (b_spe and b_beam guards are never solved for)
-            const int q = t >> 1, i = (t & 1) ? nr + 1 : 0;
-            if (!(q == 1 && m == 0)) {
-                const int k = q == 0 ? kr : ki;
-                for (int c = 0; c < 2; c++) FX(a.b_out, c, k, i) = FX(a.b_spe, c, k, i) + FX(a.b_beam, c, k, i);
-            }
-        }
-        conv_compare_last<K>(g, a.b_spe, a.conv_re, a.conv_im, a.it, a.reltol, a.abstol, a.iter_max, red,
-                             &s_last);
-        return;
+    int row, kind, m, role;
+    if (b < 4 * (M + 1)) {
+        m = b >> 2;
+        if (m == 0 && (b & 1)) return;
+        row = b;
+        kind = (b & 2) ? FK_BMINUS : FK_BPLUS;
+        role = 0;
+    } else if (b < 4 * (M + 1) + nm) {
+        const int k = b - 4 * (M + 1);
+        m = mode_of(k);
+        row = src_bz(M, k);
+        kind = FK_BZ;
+        role = 1;
+    } else {
+        const int k = b - 4 * (M + 1) - nm;
+        m = mode_of(k);
+        row = src_ez(M, nm, k);
+        kind = FK_EZ;
+        role = 2;
     }
-    if (b < M + 1 + nm) {
-        // ---- Bz of mode array k (field_b_class.f03:760-796) ----
-        const int k = b - (M + 1), m = mode_of(k);
-        double v[K], d[1][K];
-        load_rows<K>(a.src + (size_t)src_bz(M, k) * nr, nr, v);
-        stage_in<K>(stage, v, nr, d[0]);
-        const FactorT f[1] = {dfactor(g, FK_BZ, m)};
-        seg_solve<K, 1>(f, d, red);
-        stage_out<K>(stage, d[0], v);
-        double bb[K];
-        load_nodes<K>(&FX(a.b_beam, 2, k, 0), nr, bb);
-#pragma unroll
-        for (int kk = 0; kk < K; kk++) {
-            const int r = t + kk * kSolveThreads, i = r + 1;
-            if (r >= nr) continue;
-            const double bz = (m > 0 && i == 1) ? 0.0 : v[kk];
-            FX(a.b_spe, 2, k, i) = bz;
-            FX(a.b_out, 2, k, i) = bz + bb[kk];
-        }
-        if (t < 2) {
-            const int i = t ? nr + 1 : 0;
-            FX(a.b_out, 2, k, i) = FX(a.b_spe, 2, k, i) + FX(a.b_beam, 2, k, i);
-        }
-        return;
-    }
-    // ---- Ez of mode array k (field_e_class.f03:148-334) ----
-    const int k = b - (M + 1 + nm), m = mode_of(k);
     double v[K], d[1][K];
-    load_rows<K>(a.src + (size_t)src_ez(M, nm, k) * nr, nr, v);
-    if (m == 0) {
-        // discrete Gauss law on the axis row
+    load_rows<K>(a.src + (size_t)row * nr, nr, v);
+    if (role == 2 && m == 0) {
+        // discrete Gauss law on the axis row of Ez
         double part = 0.0;
 #pragma unroll
         for (int kk = 0; kk < K; kk++) {
@@ -343,14 +240,151 @@ __global__ __launch_bounds__(kSolveThreads) void k_solve(SolveArgs a)
         }
     }
     stage_in<K>(stage, v, nr, d[0]);
-    const FactorT f[1] = {dfactor(g, FK_EZ, m)};
+    const FactorT f[1] = {dfactor(g, kind, m)};
     seg_solve<K, 1>(f, d, red);
     stage_out<K>(stage, d[0], v);
 #pragma unroll
     for (int kk = 0; kk < K; kk++) {
         const int r = t + kk * kSolveThreads, i = r + 1;
-        if (r < nr) FX(a.e, 2, k, i) = (m > 0 && i == 1) ? 0.0 : v[kk];
+        if (r >= nr) continue;
+        if (role == 0) {
+            a.src[(size_t)row * nr + r] = v[kk];
+        } else {
+            const int k = role == 1 ? b - 4 * (M + 1) : b - 4 * (M + 1) - nm;
+            const double x = (m > 0 && i == 1) ? 0.0 : v[kk];
+            if (role == 1) FX(a.b_spe, 2, k, i) = x;
+            else FX(a.e, 2, k, i) = x;
+        }
     }
+}
+
+struct BfieldArgs {
+    FieldCommon g;
+    const double *src, *b_beam, *psi;
+    double *b_spe, *b_out, *e;
+    double *conv_re, *conv_im, *conv_blk;
+    IterState *it;
+    double reltol, abstol;
+    int iter_max;
+};
+
+// thread per node i = 0..nr+1, all modes: B_r, B_phi of b_spe from the
+// B+/B- solutions (field_b_class.f03:715-752), b = b_spe + b_beam, E_perp
+// (field_e_class.f03:443-504), and convergence_tester 'compare'
+// (simulation_class.f03:560-599): the node's sum_m |B_phi,m| against the
+// recorded one, max over nodes by the last workgroup to finish.  Loads are
+// issued for all modes before any arithmetic.
+template <int M>
+__global__ __launch_bounds__(256) void k_bfields(BfieldArgs a)
+{
+    constexpr int NM = 2 * M + 1;
+    __shared__ double red[8];
+    __shared__ int s_last;
+    if (a.it->done) return;
+    const FieldCommon &g = a.g;
+    const int nm = NM, ld = g.ld, nr = g.nr;
+    const int t = threadIdx.x;
+    const int i = blockIdx.x * blockDim.x + t;
+    const bool valid = i <= nr + 1;
+    const int ic = min(i, nr + 1);
+    const bool inner = valid && ic >= 1 && ic <= nr;
+    const int r = min(max(ic - 1, 0), nr - 1);
+    // ---- loads ----
+    double raw[M + 1][4], bb[NM][3], bs[NM][3], pw[NM][4], pp_i[NM], pp_2[NM];
+#pragma unroll
+    for (int m = 0; m <= M; m++)
+#pragma unroll
+        for (int q = 0; q < 4; q++) raw[m][q] = a.src[(size_t)(4 * m + q) * nr + r];
+#pragma unroll
+    for (int k = 0; k < NM; k++) {
+        const int p = k == 0 ? 0 : ((k & 1) ? k + 1 : k - 1);
+#pragma unroll
+        for (int c = 0; c < 3; c++) {
+            bb[k][c] = FX(a.b_beam, c, k, ic);
+            bs[k][c] = FX(a.b_spe, c, k, ic);
+        }
+#pragma unroll
+        for (int w = 0; w < 4; w++) pw[k][w] = a.psi[(size_t)k * ld + min(max(ic - 2 + w, 0), nr + 1)];
+        pp_i[k] = a.psi[(size_t)p * ld + ic];
+        pp_2[k] = a.psi[(size_t)p * ld + 2];
+    }
+    const double ore = a.conv_re[ic], oim = a.conv_im[ic];
+    // ---- b_spe (inner nodes), b, E_perp ----
+#pragma unroll
+    for (int m = 0; m <= M; m++) {
+        if (!inner) break;
+        const int kr = m == 0 ? 0 : 2 * m - 1, ki = 2 * m;
+        const double sv[4] = {raw[m][0], m ? raw[m][1] : 0.0, raw[m][2], m ? raw[m][3] : 0.0};
+        double o[4];
+        bpm_recombine(m, ic, sv, o);
+        bs[kr][0] = o[0];
+        bs[kr][1] = o[2];
+        if (m > 0) {
+            bs[ki][0] = o[1];
+            bs[ki][1] = o[3];
+        }
+    }
+    double sre = 0.0, sim = 0.0, dre = ore, dim = oim;
+#pragma unroll
+    for (int m = 0; m <= M; m++) {
+        const double vr = fabs(bs[m == 0 ? 0 : 2 * m - 1][1]);
+        dre = dre - vr;
+        sre = sre + vr;
+        if (m > 0) {
+            const double vi = fabs(bs[2 * m][1]);
+            dim = dim - vi;
+            sim = sim + vi;
+        }
+    }
+    if (valid) {
+#pragma unroll
+        for (int k = 0; k < NM; k++) {
+            const double br = bs[k][0] + bb[k][0], bphi = bs[k][1] + bb[k][1];
+            FX(a.b_out, 0, k, ic) = br;
+            FX(a.b_out, 1, k, ic) = bphi;
+            FX(a.b_out, 2, k, ic) = bs[k][2] + bb[k][2];
+            if (!inner) continue;
+            FX(a.b_spe, 0, k, ic) = bs[k][0];
+            FX(a.b_spe, 1, k, ic) = bs[k][1];
+            double ev[2];
+            et_window(g, br, bphi, pw[k][0], pw[k][1], pw[k][2], pw[k][3], pp_i[k], pp_2[k], k, ic, ev);
+            FX(a.e, 0, k, ic) = ev[0];
+            FX(a.e, 1, k, ic) = ev[1];
+        }
+    }
+    double mx_old = -INFINITY, mx_d = -INFINITY;
+    if (inner) {
+        mx_old = ore * ore + oim * oim;
+        mx_d = dre * dre + dim * dim;
+        a.conv_re[ic] = sre;
+        a.conv_im[ic] = sim;
+    }
+    mx_old = block_max(mx_old, red);
+    mx_d = block_max(mx_d, red);
+    if (t == 0) {
+        a.conv_blk[2 * blockIdx.x] = mx_old;
+        a.conv_blk[2 * blockIdx.x + 1] = mx_d;
+        __threadfence();
+        const int prev = __hip_atomic_fetch_add(&a.it->counter, 1, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+        s_last = (prev == (int)gridDim.x - 1);
+    }
+    __syncthreads();
+    if (!s_last || t != 0) return;
+    __threadfence();
+    double o2 = -INFINITY, d2 = -INFINITY;
+    for (int q = 0; q < (int)gridDim.x; q++) {
+        o2 = fmax(o2, __builtin_nontemporal_load(&a.conv_blk[2 * q]));
+        d2 = fmax(d2, __builtin_nontemporal_load(&a.conv_blk[2 * q + 1]));
+    }
+    const double old_norm = sqrt(o2);
+    const double ab = sqrt(d2);
+    const double rel = (old_norm > 2.220446049250313e-16) ? ab / old_norm : 1.7976931348623157e308;
+    IterState *it = a.it;
+    it->rel = rel;
+    it->abs_ = ab;
+    it->iters = it->iters + 1;
+    it->counter = 0;
+    it->done = (rel < a.reltol || ab < a.abstol || it->iters >= a.iter_max) ? 1 : 0;
 }
 
 int launch_iteration_fields(qpad_ctx *c, int j)
@@ -386,18 +420,35 @@ int launch_iteration_fields(qpad_ctx *c, int j)
     a.src = c->src;
     a.b_spe = c->f[QPAD_F_B_SPE];
     a.e = c->f[QPAD_F_E];
-    a.b_out = c->f[QPAD_F_B];
-    a.b_beam = c->h_b_beam + (size_t)j * 3 * c->g.nm * c->g.ld;
-    a.psi = c->f[QPAD_F_PSI];
     a.cu = c->f[QPAD_F_CU];
-    a.conv_re = c->conv_re;
-    a.conv_im = c->conv_im;
     a.it = c->it;
-    a.reltol = c->g.reltol;
-    a.abstol = c->g.abstol;
-    a.iter_max = c->g.iter_max;
-    const int nblk = (c->g.M + 1) + 2 * c->g.nm;
+    const int nblk = 4 * (c->g.M + 1) + 2 * c->g.nm;
     QPAD_DISPATCH_K(c->g.nr, k_solve, dim3(nblk), 0, c->stream, a);
+    QPAD_HIP_CHECK(hipGetLastError());
+
+    BfieldArgs f;
+    f.g = g;
+    f.src = c->src;
+    f.b_beam = c->h_b_beam + (size_t)j * 3 * c->g.nm * c->g.ld;
+    f.psi = c->f[QPAD_F_PSI];
+    f.b_spe = c->f[QPAD_F_B_SPE];
+    f.b_out = c->f[QPAD_F_B];
+    f.e = c->f[QPAD_F_E];
+    f.conv_re = c->conv_re;
+    f.conv_im = c->conv_im;
+    f.conv_blk = c->conv_blk;
+    f.it = c->it;
+    f.reltol = c->g.reltol;
+    f.abstol = c->g.abstol;
+    f.iter_max = c->g.iter_max;
+    const dim3 fgrid((unsigned)((c->g.nr + 2 + 255) / 256));
+    switch (c->g.M) {
+    case 0: hipLaunchKernelGGL(k_bfields<0>, fgrid, dim3(256), 0, c->stream, f); break;
+    case 1: hipLaunchKernelGGL(k_bfields<1>, fgrid, dim3(256), 0, c->stream, f); break;
+    case 2: hipLaunchKernelGGL(k_bfields<2>, fgrid, dim3(256), 0, c->stream, f); break;
+    case 3: hipLaunchKernelGGL(k_bfields<3>, fgrid, dim3(256), 0, c->stream, f); break;
+    default: hipLaunchKernelGGL(k_bfields<4>, fgrid, dim3(256), 0, c->stream, f); break;
+    }
     QPAD_HIP_CHECK(hipGetLastError());
     return 0;
 }
@@ -484,12 +535,215 @@ int launch_epilogue(qpad_ctx *c, int j)
     return 0;
 }
 
+// ---------------------------------------------------------------------------
+// start of a slice (simulation_class.f03:342-377): rho - Jz of the plasma,
+// psi, Bz and Ez from the J predictor, 'record', then E_perp and b
+// ---------------------------------------------------------------------------
+struct StartSrcArgs {
+    FieldCommon g;
+    const double *P0[kMaxSpecies], *P1[kMaxSpecies], *qn[kMaxSpecies];
+    const double *cu;
+    double *q_spe, *src;
+};
+
+// thread per (node i = 0..nr+1, mode array k): q_spe = rho - Jz of every
+// species from the charge-deposit partials (axis rules and 1/r of
+// qdeposit_part2d, part2d_class.f03:313-336) plus the neutralising
+// backgrounds; the psi, Bz and Ez right-hand sides at node i
+template <int NS>
+__global__ __launch_bounds__(256) void k_start_src(StartSrcArgs a)
+{
+    const FieldCommon &g = a.g;
+    const int nm = g.nm, ld = g.ld, nr = g.nr, M = g.M;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int k = blockIdx.y, m = mode_of(k);
+    if (i > nr + 1) return;
+    const size_t c0 = (size_t)min(max(i - 1, 0), nr - 1) * nm + k;
+    const size_t c1 = (size_t)min(max(i - 2, 0), nr - 1) * nm + k;
+    double d0[NS], d1[NS], qb[NS];
+#pragma unroll
+    for (int s = 0; s < NS; s++) {
+        d0[s] = a.P0[s][c0];
+        d1[s] = a.P1[s][c1];
+        qb[s] = a.qn[s] ? a.qn[s][(size_t)k * ld + i] : 0.0;
+    }
+    double v = 0.0;
+#pragma unroll
+    for (int s = 0; s < NS; s++) {
+        if (i >= 1 && i <= nr) v += d0[s];
+        if (i >= 2) v += d1[s];
+    }
+    if (i == 0) v = 0.0;
+    else if (i == 1) v = (k == 0) ? 8.0 * v : 0.0;
+    else v = v * (1.0 / (double)(i - 1));
+#pragma unroll
+    for (int s = 0; s < NS; s++)
+        if (a.qn[s]) v = v + qb[s];
+    a.q_spe[(size_t)k * ld + i] = v;
+    if (i < 1 || i > nr) return;
+    const FieldArr cu{a.cu, nm, ld};
+    a.src[(size_t)src_psi(M, nm, k) * nr + i - 1] = -1.0 * v;
+    a.src[(size_t)src_bz(M, k) * nr + i - 1] = bz_source(g, cu, k, i);
+    a.src[(size_t)src_ez(M, nm, k) * nr + i - 1] = (m == 0) ? ez_source_0(g, cu, i) : ez_source_m(g, cu, k, i);
+}
+
+struct StartSolveArgs {
+    FieldCommon g;
+    const double *src, *cu;
+    double *psi, *b_spe, *e;
+    double *conv_re, *conv_im;
+    IterState *it;
+};
+
+// workgroups: psi of each mode array, Bz of each, Ez of each, then one
+// 'record' workgroup (simulation_class.f03:549-558)
+template <int K>
+__global__ __launch_bounds__(kSolveThreads) void k_start_solve(StartSolveArgs a)
+{
+    extern __shared__ double stage[];
+    __shared__ double red[kRed];
+    const FieldCommon &g = a.g;
+    const int nm = g.nm, ld = g.ld, nr = g.nr, M = g.M;
+    const int t = threadIdx.x;
+    const int b = blockIdx.x;
+    if (b == 3 * nm) {
+        // 'record' of sum_m |B_phi,m| (b_spe component 2 is not read)
+        if (t == 0) {
+            a.it->done = 0;
+            a.it->iters = 0;
+            a.it->counter = 0;
+        }
+        for (int i = t + 1; i <= nr; i += kSolveThreads) {
+            double re = 0.0, im = 0.0;
+            for (int mm = 0; mm <= M; mm++) {
+                re = re + fabs(FX(a.b_spe, 1, kre(mm), i));
+                if (mm > 0) im = im + fabs(FX(a.b_spe, 1, kim(mm), i));
+            }
+            a.conv_re[i] = re;
+            a.conv_im[i] = im;
+        }
+        return;
+    }
+    const int role = b / nm, k = b % nm, m = mode_of(k);
+    const int row = role == 0 ? src_psi(M, nm, k) : (role == 1 ? src_bz(M, k) : src_ez(M, nm, k));
+    double v[K], d[1][K];
+    load_rows<K>(a.src + (size_t)row * nr, nr, v);
+    if (role == 2 && m == 0) {
+        // discrete Gauss law on the axis row of Ez
+        double part = 0.0;
+#pragma unroll
+        for (int kk = 0; kk < K; kk++) {
+            const int i = t + kk * kSolveThreads + 1;
+            if (i >= 2 && i <= nr - 2) part += v[kk] * (double)(i - 1);
+        }
+        double div = block_sum(part, red);
+        if (t == 0) {
+            div = div - ez_axis_edge(g, FieldArr{a.cu, nm, ld});
+            v[0] = -8.0 * div;
+        }
+    }
+    stage_in<K>(stage, v, nr, d[0]);
+    const FactorT f[1] = {dfactor(g, role == 0 ? FK_PSI : (role == 1 ? FK_BZ : FK_EZ), m)};
+    seg_solve<K, 1>(f, d, red);
+    stage_out<K>(stage, d[0], v);
+#pragma unroll
+    for (int kk = 0; kk < K; kk++) {
+        const int r = t + kk * kSolveThreads, i = r + 1;
+        if (r >= nr) continue;
+        const double x = (m > 0 && i == 1) ? 0.0 : v[kk];
+        if (role == 0) a.psi[(size_t)k * ld + i] = x;
+        else if (role == 1) FX(a.b_spe, 2, k, i) = x;
+        else FX(a.e, 2, k, i) = x;
+    }
+}
+
+struct EperpArgs {
+    FieldCommon g;
+    const double *b_spe, *b_beam, *psi;
+    double *b_out, *e;
+};
+
+// thread per (node i = 0..nr+1, mode array k): b = b_spe + b_beam and
+// E_perp from b and psi (field_e_class.f03:443-504)
+__global__ __launch_bounds__(256) void k_eperp(EperpArgs a)
+{
+    const FieldCommon &g = a.g;
+    const int nm = g.nm, ld = g.ld, nr = g.nr;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int k = blockIdx.y;
+    if (i > nr + 1) return;
+    double bv[3];
+#pragma unroll
+    for (int c = 0; c < 3; c++) {
+        bv[c] = FX(a.b_spe, c, k, i) + FX(a.b_beam, c, k, i);
+        FX(a.b_out, c, k, i) = bv[c];
+    }
+    if (i < 1 || i > nr) return;
+    double ev[2];
+    et_node(g, bv[0], bv[1], FieldArr{a.psi, nm, ld}, k, i, ev);
+    FX(a.e, 0, k, i) = ev[0];
+    FX(a.e, 1, k, i) = ev[1];
+}
+
+int src_rows_host(int M, int nm) { return src_rows(M, nm); }
+
+int launch_slice_start(qpad_ctx *c, int j)
+{
+    const FieldCommon g = field_common(c);
+    const int nm = c->g.nm;
+    const dim3 ngrid((unsigned)((c->g.nr + 2 + 255) / 256), nm);
+    StartSrcArgs s;
+    memset(&s, 0, sizeof(s));
+    s.g = g;
+    for (int q = 0; q < c->nspecies; q++) {
+        s.P0[q] = c->sp[q].P0;
+        s.P1[q] = c->sp[q].P1;
+        s.qn[q] = c->sp[q].cfg.neutralized ? c->sp[q].qn : nullptr;
+    }
+    s.cu = c->f[QPAD_F_CU];
+    s.q_spe = c->f[QPAD_F_Q_SPE];
+    s.src = c->src;
+    switch (c->nspecies) {
+    case 1: hipLaunchKernelGGL(k_start_src<1>, ngrid, dim3(256), 0, c->stream, s); break;
+    case 2: hipLaunchKernelGGL(k_start_src<2>, ngrid, dim3(256), 0, c->stream, s); break;
+    case 3: hipLaunchKernelGGL(k_start_src<3>, ngrid, dim3(256), 0, c->stream, s); break;
+    default: hipLaunchKernelGGL(k_start_src<4>, ngrid, dim3(256), 0, c->stream, s); break;
+    }
+    QPAD_HIP_CHECK(hipGetLastError());
+
+    StartSolveArgs a;
+    a.g = g;
+    a.src = c->src;
+    a.cu = c->f[QPAD_F_CU];
+    a.psi = c->f[QPAD_F_PSI];
+    a.b_spe = c->f[QPAD_F_B_SPE];
+    a.e = c->f[QPAD_F_E];
+    a.conv_re = c->conv_re;
+    a.conv_im = c->conv_im;
+    a.it = c->it;
+    QPAD_DISPATCH_K(c->g.nr, k_start_solve, dim3(3 * nm + 1), 0, c->stream, a);
+    QPAD_HIP_CHECK(hipGetLastError());
+
+    EperpArgs e;
+    e.g = g;
+    e.b_spe = c->f[QPAD_F_B_SPE];
+    e.b_beam = c->h_b_beam + (size_t)j * 3 * nm * c->g.ld;
+    e.psi = c->f[QPAD_F_PSI];
+    e.b_out = c->f[QPAD_F_B];
+    e.e = c->f[QPAD_F_E];
+    hipLaunchKernelGGL(k_eperp, ngrid, dim3(256), 0, c->stream, e);
+    QPAD_HIP_CHECK(hipGetLastError());
+    return 0;
+}
+
 template <int K>
 static int set_lds_attr_slice()
 {
     // the solve stage; it later holds one psi row (nr + 2 <= stage)
     const int bytes = 8 * stage_doubles<K>();
     QPAD_HIP_CHECK(hipFuncSetAttribute((const void *)k_solve<K>, hipFuncAttributeMaxDynamicSharedMemorySize, bytes));
+    QPAD_HIP_CHECK(hipFuncSetAttribute((const void *)k_start_solve<K>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                       bytes));
     return 0;
 }
 

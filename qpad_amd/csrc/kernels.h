@@ -41,6 +41,11 @@ int launch_iteration_fields(qpad_ctx *c, int j);
 // slice epilogue: cbq, e_spe, history copies, beam fields of slice j
 int launch_epilogue(qpad_ctx *c, int j);
 int slice_kernels_init();
+// start of a slice after the charge deposit: q_spe, psi, Bz, Ez, 'record',
+// then E_perp and b for the first current deposit
+int launch_slice_start(qpad_ctx *c, int j);
+// rows of the solve right-hand-side buffer
+int src_rows_host(int M, int nm);
 int launch_predictor(qpad_ctx *c);
 
 // ---- plasma (k_plasma.hip) ----

@@ -294,6 +294,7 @@ int qpad_create(const qpad_grid *gr, qpad_ctx **out)
     if (!rc) rc = dalloc(&c->h_b_beam, ns * f1_size(c, 3));
     if (!rc) rc = dalloc(&c->conv_re, (size_t)g.nr + 2);
     if (!rc) rc = dalloc(&c->conv_im, (size_t)g.nr + 2);
+    if (!rc) rc = dalloc(&c->conv_blk, 2 * (size_t)((g.nr + 2 + 255) / 256));
     if (!rc) rc = dalloc(&c->it, 1);
     // factors stored transposed for the solve kernels: [kind][m][lo|inv|cp]
     // then row r = t * K + kk at kk * kSolveThreads + t; rows >= nr are
@@ -301,7 +302,7 @@ int qpad_create(const qpad_grid *gr, qpad_ctx **out)
     const size_t kt = (size_t)kpt * kSolveThreads;
     size_t nfac = (size_t)FK_N * (kMaxMode + 1) * 3 * kt;
     if (!rc) rc = dalloc(&c->fac, nfac);
-    if (!rc) rc = dalloc(&c->src, (size_t)(4 * (g.M + 1) + 2 * g.nm) * g.nr);
+    if (!rc) rc = dalloc(&c->src, (size_t)src_rows_host(g.M, g.nm) * g.nr);
     if (!rc) {
         he = hipHostMalloc((void **)&c->it_host, sizeof(IterState), hipHostMallocDefault);
         if (he != hipSuccess) rc = fail(QPAD_E_NOMEM, "hipHostMalloc failed");
@@ -359,7 +360,7 @@ int qpad_destroy(qpad_ctx *c)
     for (int f = 0; f < QPAD_NFIELD; f++) dfree(c->f[f]);
     dfree(c->h_psi); dfree(c->h_e); dfree(c->h_b); dfree(c->h_b_spe); dfree(c->h_e_spe);
     dfree(c->h_cu); dfree(c->h_q_spe); dfree(c->h_q_beam); dfree(c->h_b_beam);
-    dfree(c->conv_re); dfree(c->conv_im); dfree(c->it); dfree(c->fac); dfree(c->src);
+    dfree(c->conv_re); dfree(c->conv_im); dfree(c->conv_blk); dfree(c->it); dfree(c->fac); dfree(c->src);
     if (c->it_host) (void)hipHostFree(c->it_host);
     for (int s = 0; s < c->nspecies; s++) {
         Species &sp = c->sp[s];
@@ -527,9 +528,7 @@ static int slice_enqueue(qpad_ctx *c, int j)
     for (int s = 0; s < c->nspecies; s++) TRY(launch_qdeposit(c, s));
     hipEvent_t stop = nullptr;
     time_begin(c, 2, &stop);
-    TRY(launch_psi_bz(c, true, true, true));
-    // E and b for the first deposit (simulation_class.f03:372-377)
-    TRY(launch_ez_et(c, j, false, false));
+    TRY(launch_slice_start(c, j));
     time_end(c, 2, stop);
     // speculative iterations: each kernel returns at once after convergence
     for (int l = 1; l <= c->g.iter_max; l++) {
@@ -891,15 +890,7 @@ int qpad_kernel_time(qpad_ctx *c, int32_t which, double *ms, int64_t *launches)
     if (!c || which < 0 || which > 2) return fail(QPAD_E_INVALID, "bad timer id");
     pool_drain(c, c->pools[which]);
     if (ms) *ms = c->pools[which].acc_ms;
-    if (launches) {
-        *launches = c->pools[which].launches;
-        if (which == 0) {
-            // deposit launches that did work (the rest returned at once after
-            // convergence); their count since the last reset
-            TRY(read_iter_state(c));
-            *launches = c->it_host->real_deposits - c->dep_base;
-        }
-    }
+    if (launches) *launches = c->pools[which].launches;
     return 0;
 }
 
@@ -912,8 +903,6 @@ int qpad_kernel_time_reset(qpad_ctx *c, int32_t enable)
         p.acc_ms = 0.0;
         p.launches = 0;
     }
-    TRY(read_iter_state(c));
-    c->dep_base = c->it_host->real_deposits;
     c->timing = enable != 0;
     return 0;
 }

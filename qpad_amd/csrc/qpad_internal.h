@@ -79,8 +79,7 @@ struct IterState {        // device-resident iteration bookkeeping
     int counter;          // last-block-done counter of the B-perp solve
     int iters;            // iterations executed in the current slice
     int total_iters;      // iterations summed over the slices of a 3D step
-    int real_deposits;    // amjdeposit launches that did work (not early exits)
-    int pad;
+    int pad[2];
 };
 
 }  // namespace qpad
@@ -96,17 +95,16 @@ struct qpad_ctx {
     double *h_psi, *h_e, *h_b, *h_b_spe, *h_e_spe, *h_cu, *h_q_spe, *h_q_beam, *h_b_beam;
     // solver factors [kind][mode][lo|inv|cp][K * kSolveThreads], transposed
     double *fac;
-    // right-hand sides of the iteration's solves [4 (M+1) + 2 nm][nr]
+    // right-hand sides of a slice's solves [4 (M+1) + 3 nm][nr]
     double *src;
-    // convergence_tester vectors (nr + 2)
-    double *conv_re, *conv_im;
+    // convergence_tester vectors (nr + 2) and per-workgroup maxima
+    double *conv_re, *conv_im, *conv_blk;
     qpad::IterState *it;      // device
     qpad::IterState *it_host; // pinned host mirror
     int nspecies, nbeams;
     qpad::Species sp[qpad::kMaxSpecies];
     qpad::Beam bm[qpad::kMaxBeams];
     bool timing;
-    int dep_base;             // real_deposits at the last timer reset
     qpad::EventPool pools[3];
     bool step_open;
 };
