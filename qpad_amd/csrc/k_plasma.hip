@@ -47,7 +47,7 @@ struct InjArgs {
     uint64_t seed;
     double *x1, *x2, *p1, *p2, *p3, *q, *gamma, *psi;
     int32_t *id;
-    uint32_t *key, *idx;
+    uint32_t *key;
 };
 
 __global__ void k_inject(InjArgs a)
@@ -77,7 +77,6 @@ __global__ void k_inject(InjArgs a)
     a.psi[ip] = (1.0 - gam + p3) / a.qm;
     a.id[ip] = (int32_t)ip;
     a.key[ip] = cell_key(x1, x2, a.idr, a.edge, a.nr);
-    a.idx[ip] = (uint32_t)ip;
 }
 
 int launch_inject(qpad_ctx *c, int s)
@@ -106,7 +105,6 @@ int launch_inject(qpad_ctx *c, int s)
     a.x1 = sp.x1; a.x2 = sp.x2; a.p1 = sp.p1; a.p2 = sp.p2; a.p3 = sp.p3;
     a.q = sp.q; a.gamma = sp.gamma; a.psi = sp.psi; a.id = sp.id;
     a.key = sp.key;
-    a.idx = sp.idx;
     sp.npp = a.n;
     if (a.n > 0) {
         hipLaunchKernelGGL(k_inject, dim3((unsigned)((a.n + 255) / 256)), dim3(256), 0, c->stream, a);
@@ -116,29 +114,8 @@ int launch_inject(qpad_ctx *c, int s)
 }
 
 // ---------------------------------------------------------------------------
-// radial-cell sort (stable): rocPRIM radix sort of (key, index), permutation
-// of the SoA arrays, cell start offsets.  Particles with key = nr have left
-// the box and end up past cell_start[nr] (update_bound).
+// cell start offsets of sorted keys (the beam's (xi, r)-cell sort)
 // ---------------------------------------------------------------------------
-struct PermArgs {
-    int64_t n;
-    const uint32_t *idx;
-    const double *s[8];
-    double *d[8];
-    const int32_t *sid;
-    int32_t *did;
-};
-
-__global__ void k_permute(PermArgs a)
-{
-    int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= a.n) return;
-    uint32_t src = a.idx[p];
-#pragma unroll
-    for (int f = 0; f < 8; f++) a.d[f][p] = a.s[f][src];
-    a.did[p] = a.sid[src];
-}
-
 // cs[c] = first position p with key[p] >= c (lower bound in the sorted
 // keys), for c = 0 .. ncs-1: one thread per cell, so empty runs of cells
 // cost nothing extra
@@ -174,51 +151,12 @@ int sort_scratch(size_t n, int bits, size_t *bytes)
     return 0;
 }
 
-static int bits_for(uint32_t maxkey)
-{
-    int b = 1;
-    while ((1u << b) <= maxkey) b++;
-    return b;
-}
-
-int launch_sort(qpad_ctx *c, int s)
-{
-    Species &sp = c->sp[s];
-    int64_t n = sp.npp;
-    int bits = bits_for((uint32_t)c->g.nr);
-    if (n > 0) {
-        size_t bytes = sp.sort_tmp_bytes;
-        QPAD_HIP_CHECK(rocprim::radix_sort_pairs<SortConfig>(sp.sort_tmp, bytes, sp.key, sp.key_alt, sp.idx, sp.idx_alt,
-                                                 (size_t)n, 0, bits, c->stream));
-        PermArgs a;
-        a.n = n;
-        a.idx = sp.idx_alt;
-        const double *src[8] = {sp.x1, sp.x2, sp.p1, sp.p2, sp.p3, sp.q, sp.gamma, sp.psi};
-        double *dst[8] = {sp.a_x1, sp.a_x2, sp.a_p1, sp.a_p2, sp.a_p3, sp.a_q, sp.a_gamma, sp.a_psi};
-        for (int f = 0; f < 8; f++) {
-            a.s[f] = src[f];
-            a.d[f] = dst[f];
-        }
-        a.sid = sp.id;
-        a.did = sp.a_id;
-        hipLaunchKernelGGL(k_permute, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, c->stream, a);
-        QPAD_HIP_CHECK(hipGetLastError());
-        std::swap(sp.x1, sp.a_x1); std::swap(sp.x2, sp.a_x2);
-        std::swap(sp.p1, sp.a_p1); std::swap(sp.p2, sp.a_p2); std::swap(sp.p3, sp.a_p3);
-        std::swap(sp.q, sp.a_q); std::swap(sp.gamma, sp.a_gamma); std::swap(sp.psi, sp.a_psi);
-        std::swap(sp.id, sp.a_id);
-        std::swap(sp.key, sp.key_alt);
-    }
-    return launch_cell_start(c, sp.key, n, c->g.nr + 2, sp.cell_start);
-}
-
-__global__ void k_keys(const double *x1, const double *x2, uint32_t *key, uint32_t *idx, int64_t n,
-                       double idr, double edge, int nr)
+__global__ void k_keys(const double *x1, const double *x2, uint32_t *key, int64_t n, double idr, double edge,
+                       int nr)
 {
     int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= n) return;
     key[p] = cell_key(x1[p], x2[p], idr, edge, nr);
-    idx[p] = (uint32_t)p;
 }
 
 int launch_keys(qpad_ctx *c, int s)
@@ -226,7 +164,7 @@ int launch_keys(qpad_ctx *c, int s)
     Species &sp = c->sp[s];
     if (sp.npp == 0) return 0;
     hipLaunchKernelGGL(k_keys, dim3((unsigned)((sp.npp + 255) / 256)), dim3(256), 0, c->stream, sp.x1, sp.x2,
-                       sp.key, sp.idx, sp.npp, 1.0 / c->g.dr, (double)c->g.nr * c->g.dr, c->g.nr);
+                       sp.key, sp.npp, 1.0 / c->g.dr, (double)c->g.nr * c->g.dr, c->g.nr);
     QPAD_HIP_CHECK(hipGetLastError());
     return 0;
 }
@@ -566,7 +504,7 @@ int launch_amjdeposit(qpad_ctx *c, int s, bool check_done, bool store_diag)
 // ---------------------------------------------------------------------------
 // push_u_robust (part2d_class.f03:1879-1965) + push_x (:2221-2262) + the new
 // sort key (update_bound :2307-2353 is the key nr).  Thread per particle;
-// particles past npp only refresh their sort index.
+// particles that already left the box (past cell_start[nr]) keep key nr.
 // ---------------------------------------------------------------------------
 struct PushArgs {
     int nr, ld, nm;
@@ -574,7 +512,7 @@ struct PushArgs {
     double idr, edge, qtmh, dt;
     double *x1, *x2, *p1, *p2, *p3, *gamma;
     const int32_t *cs;
-    uint32_t *key, *idx;
+    uint32_t *key;
     const double *e, *b;
 };
 
@@ -583,7 +521,6 @@ __global__ __launch_bounds__(256) void k_push(PushArgs a)
 {
     const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= a.nslots) return;
-    a.idx[p] = (uint32_t)p;
     const int npp = a.cs[a.nr];
     if (p >= npp) return;
     const int nm = a.nm, ld = a.ld;
@@ -654,7 +591,6 @@ int launch_push(qpad_ctx *c, int s)
     a.x1 = sp.x1; a.x2 = sp.x2; a.p1 = sp.p1; a.p2 = sp.p2; a.p3 = sp.p3; a.gamma = sp.gamma;
     a.cs = sp.cell_start;
     a.key = sp.key;
-    a.idx = sp.idx;
     a.e = c->f[QPAD_F_E];
     a.b = c->f[QPAD_F_B];
     dim3 grid((unsigned)((sp.npp + 255) / 256)), blk(256);

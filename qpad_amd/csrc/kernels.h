@@ -70,6 +70,8 @@ int launch_beam_push(qpad_ctx *c, int ib);
 // ---- generic ----
 int field_kernels_init(size_t dyn_lds);
 int sort_scratch(size_t n, int bits, size_t *bytes);
+// histogram entries of the plasma counting sort for up to npmax particles
+int sort_workspace(int64_t npmax, int nr, size_t *hist_ints);
 int launch_fill(qpad_ctx *c, double *p, size_t n, double v);
 int launch_copy(qpad_ctx *c, double *dst, const double *src, size_t n);
 

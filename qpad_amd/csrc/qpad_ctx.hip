@@ -368,9 +368,9 @@ int qpad_destroy(qpad_ctx *c)
         dfree(sp.gamma); dfree(sp.psi); dfree(sp.id);
         dfree(sp.a_x1); dfree(sp.a_x2); dfree(sp.a_p1); dfree(sp.a_p2); dfree(sp.a_p3); dfree(sp.a_q);
         dfree(sp.a_gamma); dfree(sp.a_psi); dfree(sp.a_id);
-        dfree(sp.key); dfree(sp.key_alt); dfree(sp.idx); dfree(sp.idx_alt);
+        dfree(sp.key); dfree(sp.key_alt);
         dfree(sp.cell_start); dfree(sp.qn); dfree(sp.P0); dfree(sp.P1);
-        if (sp.sort_tmp) (void)hipFree(sp.sort_tmp);
+        dfree(sp.hist); dfree(sp.tot);
     }
     for (int b = 0; b < c->nbeams; b++) {
         Beam &bm = c->bm[b];
@@ -417,21 +417,14 @@ int qpad_species_new(qpad_ctx *c, const qpad_species_cfg *cfg, int32_t *sp_id)
     if (!rc) rc = dalloc(&sp.a_id, n);
     if (!rc) rc = dalloc(&sp.key, n);
     if (!rc) rc = dalloc(&sp.key_alt, n);
-    if (!rc) rc = dalloc(&sp.idx, n);
-    if (!rc) rc = dalloc(&sp.idx_alt, n);
     if (!rc) rc = dalloc(&sp.cell_start, (size_t)c->g.nr + 2);
     if (!rc) rc = dalloc(&sp.qn, f1_size(c, 1));
     if (!rc) rc = dalloc(&sp.P0, (size_t)c->g.nr * 8 * c->g.nm);
     if (!rc) rc = dalloc(&sp.P1, (size_t)c->g.nr * 8 * c->g.nm);
-    if (!rc) {
-        int bits = 1;
-        while ((1u << bits) <= (uint32_t)c->g.nr) bits++;
-        rc = sort_scratch(n, bits, &sp.sort_tmp_bytes);
-        if (!rc) {
-            hipError_t e = hipMalloc(&sp.sort_tmp, sp.sort_tmp_bytes);
-            if (e != hipSuccess) rc = fail(QPAD_E_NOMEM, "sort scratch allocation failed");
-        }
-    }
+    size_t hist_ints = 0;
+    if (!rc) rc = sort_workspace(sp.npmax, c->g.nr, &hist_ints);
+    if (!rc) rc = dalloc(&sp.hist, hist_ints);
+    if (!rc) rc = dalloc(&sp.tot, (size_t)c->g.nr + 1);
     if (!rc) rc = zero_dev(c, sp.qn, f1_size(c, 1));
     if (!rc) rc = launch_inject(c, s);
     if (!rc) rc = species_charge_setup(c, s);

@@ -46,12 +46,11 @@ struct Species {
     int32_t *id;
     double *a_x1, *a_x2, *a_p1, *a_p2, *a_p3, *a_q, *a_gamma, *a_psi;
     int32_t *a_id;
-    uint32_t *key, *key_alt, *idx, *idx_alt;
-    int32_t *cell_start;     // nr + 2 entries, cell_start[nr] = npp
+    uint32_t *key, *key_alt;  // radial cell of each particle (nr = left the box)
+    int32_t *cell_start;     // nr + 2 entries, cell_start[nr] = live count
     double *qn;              // neutralising background, 1 x nm x ld
     double *P0, *P1;         // per-cell deposit partials [nr][nval]
-    void *sort_tmp;
-    size_t sort_tmp_bytes;
+    int *hist, *tot;         // counting-sort workspace: [chunks][nr+1], [nr+1]
 };
 
 struct Beam {
