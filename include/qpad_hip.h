@@ -203,13 +203,13 @@ int qpad_set_beam(qpad_ctx *ctx, int32_t ib, int64_t n, const double *x1,
                   const double *x2, const double *x3, const double *p1,
                   const double *p2, const double *p3, const double *q);
 
-/* timing of the dominant kernels (HIP events on the context's stream):
-   which = 0: plasma current deposit (amjdeposit), 1: plasma push,
-   2: field sources and solves.  Returns the summed milliseconds and the
-   launch count since the last reset.  Iterations are enqueued
-   speculatively: launches after convergence return at once and are
-   counted too (the iterations that did work are what qpad_step3d and
-   qpad_slice report). */
+/* timing of the dominant kernels (HIP events on the context's stream),
+   sampled on the first predictor-corrector iteration of every slice (which
+   always does work; events around every launch would slow the step):
+   which = 0: plasma current deposit (amjdeposit), 1: plasma push (every
+   slice), 2: the field sources and solves of the first iteration.  Returns
+   the summed milliseconds and the number of timed launches since the last
+   reset. */
 int qpad_kernel_time(qpad_ctx *ctx, int32_t which, double *ms, int64_t *launches);
 int qpad_kernel_time_reset(qpad_ctx *ctx, int32_t enable);
 

@@ -335,17 +335,19 @@ struct JDepArgs {
     const double *e, *b;
     double *P0, *P1;
     const IterState *it;
-    int check_done;
 };
 
-template <int M>
+// SPEC: a speculative launch of iteration >= 2 of a slice, which returns at
+// once when the iteration has already converged (a separate instance, so
+// that profiles show the first-iteration launches, which always work, apart)
+template <int M, bool SPEC>
 __global__ __launch_bounds__(64 * kDepWaves) void k_amjdeposit(JDepArgs a)
 {
     constexpr int NM = 2 * M + 1;
     constexpr int NV = 8 * NM;
     // E and B at the two nodes of each wave's cell: [wave][E|B][n][cc][k]
     __shared__ double sf[kDepWaves][2][2 * 3 * NM];
-    if (a.check_done && a.it->done) return;
+    if (SPEC && a.it->done) return;
     const int lane = threadIdx.x & 63;
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int c = blockIdx.x * kDepWaves + wave;
@@ -464,7 +466,7 @@ __global__ __launch_bounds__(64 * kDepWaves) void k_amjdeposit(JDepArgs a)
 void time_begin(qpad_ctx *c, int which, hipEvent_t *stop);
 void time_end(qpad_ctx *c, int which, hipEvent_t stop);
 
-int launch_amjdeposit(qpad_ctx *c, int s, bool check_done, bool store_diag)
+int launch_amjdeposit(qpad_ctx *c, int s, bool speculative, bool store_diag, bool timed)
 {
     Species &sp = c->sp[s];
     JDepArgs a;
@@ -485,19 +487,24 @@ int launch_amjdeposit(qpad_ctx *c, int s, bool check_done, bool store_diag)
     a.P0 = sp.P0;
     a.P1 = sp.P1;
     a.it = c->it;
-    a.check_done = check_done;
     dim3 grid((c->g.nr + kDepWaves - 1) / kDepWaves), blk(64 * kDepWaves);
     hipEvent_t stop = nullptr;
-    time_begin(c, 0, &stop);
+    if (timed) time_begin(c, 0, &stop);
+#define QPAD_DEP(MM)                                                                               \
+    do {                                                                                           \
+        if (speculative) hipLaunchKernelGGL((k_amjdeposit<MM, true>), grid, blk, 0, c->stream, a); \
+        else hipLaunchKernelGGL((k_amjdeposit<MM, false>), grid, blk, 0, c->stream, a);            \
+    } while (0)
     switch (c->g.M) {
-    case 0: hipLaunchKernelGGL(k_amjdeposit<0>, grid, blk, 0, c->stream, a); break;
-    case 1: hipLaunchKernelGGL(k_amjdeposit<1>, grid, blk, 0, c->stream, a); break;
-    case 2: hipLaunchKernelGGL(k_amjdeposit<2>, grid, blk, 0, c->stream, a); break;
-    case 3: hipLaunchKernelGGL(k_amjdeposit<3>, grid, blk, 0, c->stream, a); break;
-    default: hipLaunchKernelGGL(k_amjdeposit<4>, grid, blk, 0, c->stream, a); break;
+    case 0: QPAD_DEP(0); break;
+    case 1: QPAD_DEP(1); break;
+    case 2: QPAD_DEP(2); break;
+    case 3: QPAD_DEP(3); break;
+    default: QPAD_DEP(4); break;
     }
+#undef QPAD_DEP
     QPAD_HIP_CHECK(hipGetLastError());
-    time_end(c, 0, stop);
+    if (timed) time_end(c, 0, stop);
     return 0;
 }
 

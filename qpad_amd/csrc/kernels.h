@@ -56,7 +56,9 @@ int launch_sort(qpad_ctx *c, int sp);            // sort by key, permute, cell s
 int launch_keys(qpad_ctx *c, int sp);            // keys of the current positions
 int launch_qdeposit(qpad_ctx *c, int sp);        // per-cell partials of rho - Jz
 int launch_combine_q(qpad_ctx *c, int sp, double *out, double scale, bool accumulate);
-int launch_amjdeposit(qpad_ctx *c, int sp, bool check_done, bool store_diag);
+// speculative: iteration >= 2 of a slice (returns at once after convergence);
+// timed: bracket the launch with the timing events of pool 0
+int launch_amjdeposit(qpad_ctx *c, int sp, bool speculative, bool store_diag, bool timed);
 int launch_push(qpad_ctx *c, int sp);            // push_u + push_x + keys
 int launch_add_qn_into_qspe(qpad_ctx *c, int sp);
 

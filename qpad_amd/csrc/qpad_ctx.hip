@@ -519,16 +519,17 @@ static int slice_enqueue(qpad_ctx *c, int j)
 {
     // rho - Jz of every species, psi, Bz from the J predictor, 'record'
     for (int s = 0; s < c->nspecies; s++) TRY(launch_qdeposit(c, s));
-    hipEvent_t stop = nullptr;
-    time_begin(c, 2, &stop);
     TRY(launch_slice_start(c, j));
-    time_end(c, 2, stop);
-    // speculative iterations: each kernel returns at once after convergence
+    // speculative iterations: each kernel returns at once after convergence.
+    // Timing events (when enabled) bracket only the first iteration, which
+    // always does work: an event per launch would cost a quarter of the step.
     for (int l = 1; l <= c->g.iter_max; l++) {
-        for (int s = 0; s < c->nspecies; s++) TRY(launch_amjdeposit(c, s, true, true));
-        time_begin(c, 2, &stop);
+        const bool first = l == 1;
+        for (int s = 0; s < c->nspecies; s++) TRY(launch_amjdeposit(c, s, !first, true, first));
+        hipEvent_t stop = nullptr;
+        if (first) time_begin(c, 2, &stop);
         TRY(launch_iteration_fields(c, j));
-        time_end(c, 2, stop);
+        if (first) time_end(c, 2, stop);
     }
     // slice epilogue: cbq, e_spe, history (simulation_class.f03:401-456)
     TRY(launch_epilogue(c, j));
@@ -601,7 +602,7 @@ int qpad_species_amjdp(qpad_ctx *c, int32_t s, double dxi)
 {
     if (!c || s < 0 || s >= c->nspecies) return fail(QPAD_E_INVALID, "bad species");
     if (dxi != c->g.dxi) return fail(QPAD_E_INVALID, "dxi differs from the grid's dxi");
-    TRY(launch_amjdeposit(c, s, false, true));
+    TRY(launch_amjdeposit(c, s, false, true, false));
     // add this species' cu/acu/amu into the totals
     TRY(launch_combine_j_species(c, s));
     return sync(c);
