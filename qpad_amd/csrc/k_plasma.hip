@@ -16,6 +16,7 @@
 #include "sort_config.h"
 #include "rng.h"
 #include "wave_reduce.h"
+#include "conv.h"
 
 namespace qpad {
 
@@ -334,7 +335,12 @@ struct JDepArgs {
     const int32_t *cs;
     const double *e, *b;
     double *P0, *P1;
-    const IterState *it;
+    IterState *it;
+    // SPEC launches: convergence of the previous iteration from the field
+    // kernel's per-workgroup maxima
+    const double *conv_prev;
+    int nblk;
+    double reltol, abstol;
 };
 
 // SPEC: a speculative launch of iteration >= 2 of a slice, which returns at
@@ -347,7 +353,17 @@ __global__ __launch_bounds__(64 * kDepWaves) void k_amjdeposit(JDepArgs a)
     constexpr int NV = 8 * NM;
     // E and B at the two nodes of each wave's cell: [wave][E|B][n][cc][k]
     __shared__ double sf[kDepWaves][2][2 * 3 * NM];
-    if (SPEC && a.it->done) return;
+    if (SPEC) {
+        if (a.it->done) return;
+        // 'compare' of the previous iteration (simulation_class.f03:396):
+        // every wave evaluates it; workgroup 0 records the decision
+        double rel, ab;
+        conv_residuals(a.conv_prev, a.nblk, rel, ab);
+        if (rel < a.reltol || ab < a.abstol) {
+            if (blockIdx.x == 0 && threadIdx.x == 0) a.it->done = 1;
+            return;
+        }
+    }
     const int lane = threadIdx.x & 63;
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int c = blockIdx.x * kDepWaves + wave;
@@ -466,8 +482,9 @@ __global__ __launch_bounds__(64 * kDepWaves) void k_amjdeposit(JDepArgs a)
 void time_begin(qpad_ctx *c, int which, hipEvent_t *stop);
 void time_end(qpad_ctx *c, int which, hipEvent_t stop);
 
-int launch_amjdeposit(qpad_ctx *c, int s, bool speculative, bool store_diag, bool timed)
+int launch_amjdeposit(qpad_ctx *c, int s, int l, bool store_diag, bool timed)
 {
+    const bool speculative = l >= 2;
     Species &sp = c->sp[s];
     JDepArgs a;
     a.nr = c->g.nr;
@@ -487,6 +504,10 @@ int launch_amjdeposit(qpad_ctx *c, int s, bool speculative, bool store_diag, boo
     a.P0 = sp.P0;
     a.P1 = sp.P1;
     a.it = c->it;
+    a.nblk = conv_blocks(c->g.nr);
+    a.conv_prev = c->conv_blk + (size_t)((l - 1) & 1) * 2 * a.nblk;
+    a.reltol = c->g.reltol;
+    a.abstol = c->g.abstol;
     dim3 grid((c->g.nr + kDepWaves - 1) / kDepWaves), blk(64 * kDepWaves);
     hipEvent_t stop = nullptr;
     if (timed) time_begin(c, 0, &stop);

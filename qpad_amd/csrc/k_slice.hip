@@ -20,6 +20,7 @@
 
 #include "field_src.h"
 #include "kernels.h"
+#include "conv.h"
 
 namespace qpad {
 
@@ -262,24 +263,24 @@ struct BfieldArgs {
     FieldCommon g;
     const double *src, *b_beam, *psi;
     double *b_spe, *b_out, *e;
-    double *conv_re, *conv_im, *conv_blk;
+    double *conv_re, *conv_im;
+    double *conv_blk;   // this iteration's half: [workgroup][max old^2, max diff^2]
     IterState *it;
-    double reltol, abstol;
-    int iter_max;
+    int l;              // iteration index (1-based)
 };
 
 // thread per node i = 0..nr+1, all modes: B_r, B_phi of b_spe from the
 // B+/B- solutions (field_b_class.f03:715-752), b = b_spe + b_beam, E_perp
 // (field_e_class.f03:443-504), and convergence_tester 'compare'
 // (simulation_class.f03:560-599): the node's sum_m |B_phi,m| against the
-// recorded one, max over nodes by the last workgroup to finish.  Loads are
-// issued for all modes before any arithmetic.
+// recorded one, max over the workgroup's nodes (the next iteration's deposit
+// reduces the workgroup maxima).  Loads are issued for all modes before any
+// arithmetic.
 template <int M>
 __global__ __launch_bounds__(256) void k_bfields(BfieldArgs a)
 {
     constexpr int NM = 2 * M + 1;
     __shared__ double red[8];
-    __shared__ int s_last;
     if (a.it->done) return;
     const FieldCommon &g = a.g;
     const int nm = NM, ld = g.ld, nr = g.nr;
@@ -364,30 +365,11 @@ __global__ __launch_bounds__(256) void k_bfields(BfieldArgs a)
     if (t == 0) {
         a.conv_blk[2 * blockIdx.x] = mx_old;
         a.conv_blk[2 * blockIdx.x + 1] = mx_d;
-        __threadfence();
-        const int prev = __hip_atomic_fetch_add(&a.it->counter, 1, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
-        s_last = (prev == (int)gridDim.x - 1);
+        if (blockIdx.x == 0) a.it->iters = a.l;
     }
-    __syncthreads();
-    if (!s_last || t != 0) return;
-    __threadfence();
-    double o2 = -INFINITY, d2 = -INFINITY;
-    for (int q = 0; q < (int)gridDim.x; q++) {
-        o2 = fmax(o2, __builtin_nontemporal_load(&a.conv_blk[2 * q]));
-        d2 = fmax(d2, __builtin_nontemporal_load(&a.conv_blk[2 * q + 1]));
-    }
-    const double old_norm = sqrt(o2);
-    const double ab = sqrt(d2);
-    const double rel = (old_norm > 2.220446049250313e-16) ? ab / old_norm : 1.7976931348623157e308;
-    IterState *it = a.it;
-    it->rel = rel;
-    it->abs_ = ab;
-    it->iters = it->iters + 1;
-    it->counter = 0;
-    it->done = (rel < a.reltol || ab < a.abstol || it->iters >= a.iter_max) ? 1 : 0;
 }
 
-int launch_iteration_fields(qpad_ctx *c, int j)
+int launch_iteration_fields(qpad_ctx *c, int j, int l)
 {
     const FieldCommon g = field_common(c);
     SrcArgs s;
@@ -436,11 +418,9 @@ int launch_iteration_fields(qpad_ctx *c, int j)
     f.e = c->f[QPAD_F_E];
     f.conv_re = c->conv_re;
     f.conv_im = c->conv_im;
-    f.conv_blk = c->conv_blk;
+    f.conv_blk = c->conv_blk + (size_t)(l & 1) * 2 * conv_blocks(c->g.nr);
     f.it = c->it;
-    f.reltol = c->g.reltol;
-    f.abstol = c->g.abstol;
-    f.iter_max = c->g.iter_max;
+    f.l = l;
     const dim3 fgrid((unsigned)((c->g.nr + 2 + 255) / 256));
     switch (c->g.M) {
     case 0: hipLaunchKernelGGL(k_bfields<0>, fgrid, dim3(256), 0, c->stream, f); break;
@@ -464,6 +444,8 @@ struct EpiArgs {
     double *h_e, *h_b, *h_psi, *h_b_spe, *h_e_spe, *h_cu, *h_q_spe;
     double *q_beam_f1, *b_beam_f1;
     IterState *it;
+    const double *conv_blk;   // both parities
+    int nblk;
 };
 
 // thread per (node i = 0..nr+1, mode array k)
@@ -473,8 +455,18 @@ __global__ __launch_bounds__(256) void k_epilogue(EpiArgs a)
     const int nm = g.nm, ld = g.ld, nr = g.nr;
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     const int k = blockIdx.y;
+    if (blockIdx.x == 0 && k == 0 && threadIdx.x < 64) {
+        // residuals of the last iteration executed (rel/abs of 'compare')
+        const int iters = a.it->iters;
+        double rel, ab;
+        conv_residuals(a.conv_blk + (size_t)(iters & 1) * 2 * a.nblk, a.nblk, rel, ab);
+        if (threadIdx.x == 0) {
+            a.it->rel = rel;
+            a.it->abs_ = ab;
+            a.it->total_iters += iters;
+        }
+    }
     if (i > nr + 1) return;
-    if (i == 0 && k == 0) a.it->total_iters += a.it->iters;
     for (int c = 0; c < 3; c++) {
         FX(a.h_e, c, k, i) = FX(a.e, c, k, i);
         FX(a.h_b, c, k, i) = FX(a.b, c, k, i);
@@ -529,6 +521,8 @@ int launch_epilogue(qpad_ctx *c, int j)
     a.q_beam_f1 = c->f[QPAD_F_Q_BEAM];
     a.b_beam_f1 = c->f[QPAD_F_B_BEAM];
     a.it = c->it;
+    a.conv_blk = c->conv_blk;
+    a.nblk = conv_blocks(c->g.nr);
     hipLaunchKernelGGL(k_epilogue, dim3((unsigned)((c->g.nr + 2 + 255) / 256), c->g.nm), dim3(256), 0, c->stream,
                        a);
     QPAD_HIP_CHECK(hipGetLastError());

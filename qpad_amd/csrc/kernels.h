@@ -37,7 +37,7 @@ int launch_compare_only(qpad_ctx *c);
 // fused predictor-corrector iteration after the current deposit: combine +
 // all sources (k_src), then B_perp, Bz, Ez, E_perp, b and the convergence
 // test (k_solve); both return at once once the iteration has converged
-int launch_iteration_fields(qpad_ctx *c, int j);
+int launch_iteration_fields(qpad_ctx *c, int j, int l);
 // slice epilogue: cbq, e_spe, history copies, beam fields of slice j
 int launch_epilogue(qpad_ctx *c, int j);
 int slice_kernels_init();
@@ -56,9 +56,13 @@ int launch_sort(qpad_ctx *c, int sp);            // sort by key, permute, cell s
 int launch_keys(qpad_ctx *c, int sp);            // keys of the current positions
 int launch_qdeposit(qpad_ctx *c, int sp);        // per-cell partials of rho - Jz
 int launch_combine_q(qpad_ctx *c, int sp, double *out, double scale, bool accumulate);
-// speculative: iteration >= 2 of a slice (returns at once after convergence);
+// l: predictor-corrector iteration of the slice (1-based; l >= 2 is a
+// speculative launch that first evaluates the convergence of iteration l-1
+// and returns at once when it has converged; l = 0 for a plain deposit);
 // timed: bracket the launch with the timing events of pool 0
-int launch_amjdeposit(qpad_ctx *c, int sp, bool speculative, bool store_diag, bool timed);
+int launch_amjdeposit(qpad_ctx *c, int sp, int l, bool store_diag, bool timed);
+// workgroups of the B_perp field kernel (one convergence maximum pair each)
+inline int conv_blocks(int nr) { return (nr + 2 + 255) / 256; }
 int launch_push(qpad_ctx *c, int sp);            // push_u + push_x + keys
 int launch_add_qn_into_qspe(qpad_ctx *c, int sp);
 

@@ -294,7 +294,7 @@ int qpad_create(const qpad_grid *gr, qpad_ctx **out)
     if (!rc) rc = dalloc(&c->h_b_beam, ns * f1_size(c, 3));
     if (!rc) rc = dalloc(&c->conv_re, (size_t)g.nr + 2);
     if (!rc) rc = dalloc(&c->conv_im, (size_t)g.nr + 2);
-    if (!rc) rc = dalloc(&c->conv_blk, 2 * (size_t)((g.nr + 2 + 255) / 256));
+    if (!rc) rc = dalloc(&c->conv_blk, 2 * 2 * (size_t)conv_blocks(g.nr));
     if (!rc) rc = dalloc(&c->it, 1);
     // factors stored transposed for the solve kernels: [kind][m][lo|inv|cp]
     // then row r = t * K + kk at kk * kSolveThreads + t; rows >= nr are
@@ -525,10 +525,10 @@ static int slice_enqueue(qpad_ctx *c, int j)
     // always does work: an event per launch would cost a quarter of the step.
     for (int l = 1; l <= c->g.iter_max; l++) {
         const bool first = l == 1;
-        for (int s = 0; s < c->nspecies; s++) TRY(launch_amjdeposit(c, s, !first, true, first));
+        for (int s = 0; s < c->nspecies; s++) TRY(launch_amjdeposit(c, s, l, true, first));
         hipEvent_t stop = nullptr;
         if (first) time_begin(c, 2, &stop);
-        TRY(launch_iteration_fields(c, j));
+        TRY(launch_iteration_fields(c, j, l));
         if (first) time_end(c, 2, stop);
     }
     // slice epilogue: cbq, e_spe, history (simulation_class.f03:401-456)
@@ -602,7 +602,7 @@ int qpad_species_amjdp(qpad_ctx *c, int32_t s, double dxi)
 {
     if (!c || s < 0 || s >= c->nspecies) return fail(QPAD_E_INVALID, "bad species");
     if (dxi != c->g.dxi) return fail(QPAD_E_INVALID, "dxi differs from the grid's dxi");
-    TRY(launch_amjdeposit(c, s, false, true, false));
+    TRY(launch_amjdeposit(c, s, 0, true, false));
     // add this species' cu/acu/amu into the totals
     TRY(launch_combine_j_species(c, s));
     return sync(c);
