@@ -35,21 +35,25 @@ def parse():
     ap.add_argument("--cpu-start", type=int, default=100)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--profile-slices", type=int, default=0,
-                    help="run only a 64 MiB calibration copy and this many slices of the workload "
+                    help="run only a 512 MiB calibration copy and this many slices of the workload "
                          "(for rocprofv3 counter runs), print nothing")
     return ap.parse_args()
 
 
 def profile_slices(args):
     """Short run for hardware-counter passes: one device copy of known size
-    (read 64 MiB, write 64 MiB) to calibrate FETCH_SIZE / WRITE_SIZE, then
-    the first N slices of one 3D step of the workload."""
+    (read 512 MiB, write 512 MiB; its source evicted from the caches by a
+    second 512 MiB fill) to calibrate FETCH_SIZE / WRITE_SIZE, then the first
+    N slices of one 3D step of the workload."""
     import torch
     from qpad_amd import Simulation
-    src = torch.ones(8 << 20, dtype=torch.float64, device="cuda")
+    src = torch.ones(64 << 20, dtype=torch.float64, device="cuda")
+    evict = torch.full((64 << 20,), 2.0, dtype=torch.float64, device="cuda")
+    torch.cuda.synchronize()
     dst = torch.empty_like(src)
     dst.copy_(src)
     torch.cuda.synchronize()
+    del src, evict, dst
     sim = Simulation(deck_for(args.workload))
     sim.step3d_begin()
     for j in range(1, args.profile_slices + 1):
@@ -61,6 +65,22 @@ def profile_slices(args):
 def deck_for(name):
     from qpad_amd import configs
     return {"c1": configs.c1, "c2": configs.c2, "c3": configs.c3}[name]()
+
+
+def measured_traffic(workload, max_mode):
+    """HBM bytes per launch of the first-iteration deposit from the committed
+    rocprofv3 FETCH_SIZE / WRITE_SIZE summary (calibrated against a copy of
+    known size), or None when no summary covers this workload."""
+    path = os.path.join(ROOT, "profiles", f"k_amjdeposit_{workload}_hbm_traffic.txt")
+    if not os.path.exists(path):
+        return None
+    text = open(path).read()
+    if f"k_amjdeposit<{max_mode}, false>" not in text:
+        return None
+    for line in text.splitlines():
+        if line.startswith("traffic_bytes_per_launch:"):
+            return float(line.split(":", 1)[1])
+    return None
 
 
 def algorithmic_bytes_per_step(grid, np_per_slice, iters, nbeam):
@@ -225,7 +245,8 @@ def main():
         "particle_slice_pushes_per_s": np_slice * g.nxi * steps_per_s,
         "iters_per_slice": iters_per_slice,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": measured_traffic(args.workload, g.max_mode),
                      "kernel": f"k_amjdeposit<{g.max_mode}, false> (first iteration of each slice: gather + "
                                "robust predictor + J/dJ/momentum-flux deposit)",
                      "avg_launch_us": dep_avg_s * 1e6, "timed_launches": dep_n,
