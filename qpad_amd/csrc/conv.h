@@ -12,16 +12,20 @@
 
 namespace qpad {
 
-// rel and abs residuals: ||old - new||_inf and that over ||old||_inf (huge
-// when ||old|| <= epsilon).  Called by every lane of a wave; nblk <= 64.
-__device__ inline void conv_residuals(const double *blk, int nblk, double &rel, double &ab)
+// this lane's pair of workgroup maxima (-inf past nblk); nblk <= 64
+__device__ inline void conv_load(const double *blk, int nblk, double &o, double &d)
 {
     const int lane = threadIdx.x & 63;
-    double o = -INFINITY, d = -INFINITY;
-    if (lane < nblk) {
-        o = blk[2 * lane];
-        d = blk[2 * lane + 1];
-    }
+    const int q = min(lane, nblk - 1);
+    const double o1 = blk[2 * q], d1 = blk[2 * q + 1];
+    o = lane < nblk ? o1 : -INFINITY;
+    d = lane < nblk ? d1 : -INFINITY;
+}
+
+// rel and abs residuals: ||old - new||_inf and that over ||old||_inf (huge
+// when ||old|| <= epsilon), from every lane's conv_load pair (whole wave)
+__device__ inline void conv_finish(double o, double d, double &rel, double &ab)
+{
     for (int s = 32; s > 0; s >>= 1) {
         o = fmax(o, __shfl_xor(o, s, 64));
         d = fmax(d, __shfl_xor(d, s, 64));
@@ -29,6 +33,13 @@ __device__ inline void conv_residuals(const double *blk, int nblk, double &rel, 
     const double old_norm = sqrt(o);
     ab = sqrt(d);
     rel = (old_norm > 2.220446049250313e-16) ? ab / old_norm : 1.7976931348623157e308;
+}
+
+__device__ inline void conv_residuals(const double *blk, int nblk, double &rel, double &ab)
+{
+    double o, d;
+    conv_load(blk, nblk, o, d);
+    conv_finish(o, d, rel, ab);
 }
 
 }  // namespace qpad

@@ -343,6 +343,85 @@ struct JDepArgs {
     double reltol, abstol;
 };
 
+// robust predictor and current deposit of one particle of cell c into this
+// lane's accumulators: acc[v] node ix, acc[NV + v] node ix + 1
+template <int M>
+__device__ __forceinline__ void deposit_particle(const JDepArgs &a, const NodePairLds<2 * M + 1> &fe,
+                                                 const NodePairLds<2 * M + 1> &fb, int c, int p, double x1,
+                                                 double x2, double pp1, double pp2, double pp3, double qp,
+                                                 double (&acc)[16 * (2 * M + 1)])
+{
+    constexpr int NM = 2 * M + 1;
+    constexpr int NV = 8 * NM;
+    double r = radius(x1, x2);
+    const double pc = x1 / r, ps = x2 / r;
+    double pos = __dmul_rn(r, a.idr);
+    const double w1 = pos - (double)c, w0 = 1.0 - w1;
+    double ep[3], bp[3];
+    gather3<M>(fe, w0, w1, pc, ps, ep);
+    gather3<M>(fb, w0, w1, pc, ps, bp);
+    double wp0 = ep[0] - bp[1], wp1 = ep[1] + bp[0], wp2 = ep[2];
+    double u0[3] = {pp1 * pc + pp2 * ps, pp2 * pc - pp1 * ps, pp3};
+    double gam = sqrt(1.0 + u0[0] * u0[0] + u0[1] * u0[1] + u0[2] * u0[2]);
+    double qtmh1 = a.qtmh * gam / (gam - u0[2]);
+    double ut[3], u[3];
+#pragma unroll
+    for (int cc = 0; cc < 3; cc++) { ep[cc] = ep[cc] * qtmh1; ut[cc] = u0[cc] + ep[cc]; }
+    gam = sqrt(1.0 + ut[0] * ut[0] + ut[1] * ut[1] + ut[2] * ut[2]);
+    double qtmh2 = a.qtmh / (gam - ut[2]);
+#pragma unroll
+    for (int cc = 0; cc < 3; cc++) bp[cc] = bp[cc] * qtmh2;
+    u[0] = ut[0] + ut[1] * bp[2] - ut[2] * bp[1];
+    u[1] = ut[1] + ut[2] * bp[0] - ut[0] * bp[2];
+    u[2] = ut[2] + ut[0] * bp[1] - ut[1] * bp[0];
+    double ostq = 2.0 / (1.0 + bp[0] * bp[0] + bp[1] * bp[1] + bp[2] * bp[2]);
+#pragma unroll
+    for (int cc = 0; cc < 3; cc++) bp[cc] = bp[cc] * ostq;
+    ut[0] = ut[0] + u[1] * bp[2] - u[2] * bp[1];
+    ut[1] = ut[1] + u[2] * bp[0] - u[0] * bp[2];
+    ut[2] = ut[2] + u[0] * bp[1] - u[1] * bp[0];
+#pragma unroll
+    for (int cc = 0; cc < 3; cc++) u[cc] = ut[cc] + ep[cc];
+    double du0 = a.idt * (u[0] - u0[0]), du1 = a.idt * (u[1] - u0[1]);
+#pragma unroll
+    for (int cc = 0; cc < 3; cc++) u[cc] = 0.5 * (u[cc] + u0[cc]);
+    double g2 = sqrt(1.0 + u[0] * u[0] + u[1] * u[1] + u[2] * u[2]);
+    double ipsi = 1.0 / (g2 - u[2]);
+    if (a.store_diag) {
+        a.gamma[p] = g2;
+        a.psi[p] = (1.0 - 1.0 / ipsi) / a.qbm;
+    }
+    double dpsi = a.qbm * (wp2 - (wp0 * u[0] + wp1 * u[1]) * ipsi);
+    du0 = du0 + u[0] * dpsi * ipsi;
+    du1 = du1 + u[1] * dpsi * ipsi;
+    const double val[8] = {u[0], u[1], u[2], du0, du1, u[0] * u[0] * ipsi, u[0] * u[1] * ipsi,
+                           u[1] * u[1] * ipsi};
+    double phr = 1.0 * qp * ipsi, phi = 0.0;
+    {
+        const double wa = w0 * phr, wb = w1 * phr;
+#pragma unroll
+        for (int cc = 0; cc < 8; cc++) {
+            acc[cc * NM] += wa * val[cc];
+            acc[NV + cc * NM] += wb * val[cc];
+        }
+    }
+    const double ph0r = pc, ph0i = -ps;
+#pragma unroll
+    for (int m = 1; m <= M; m++) {
+        double tr = phr * ph0r - phi * ph0i, ti = phr * ph0i + phi * ph0r;
+        phr = tr;
+        phi = ti;
+        const double war = w0 * phr, wbr = w1 * phr, wai = w0 * phi, wbi = w1 * phi;
+#pragma unroll
+        for (int cc = 0; cc < 8; cc++) {
+            acc[cc * NM + 2 * m - 1] += war * val[cc];
+            acc[NV + cc * NM + 2 * m - 1] += wbr * val[cc];
+            acc[cc * NM + 2 * m] += wai * val[cc];
+            acc[NV + cc * NM + 2 * m] += wbi * val[cc];
+        }
+    }
+}
+
 // SPEC: a speculative launch of iteration >= 2 of a slice, which returns at
 // once when the iteration has already converged (a separate instance, so
 // that profiles show the first-iteration launches, which always work, apart)
@@ -351,35 +430,52 @@ __global__ __launch_bounds__(64 * kDepWaves) void k_amjdeposit(JDepArgs a)
 {
     constexpr int NM = 2 * M + 1;
     constexpr int NV = 8 * NM;
+    constexpr int NF = 12 * NM;   // staged field values per cell
     // E and B at the two nodes of each wave's cell: [wave][E|B][n][cc][k]
     __shared__ double sf[kDepWaves][2][2 * 3 * NM];
+    const int lane = threadIdx.x & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int c = blockIdx.x * kDepWaves + wave;
+    const int cc = min(c, a.nr - 1);
+    const int nm = a.nm, ld = a.ld;
+    const int ix = cc + 1;
+    // every independent load of the prologue first (one round trip): the
+    // convergence state, the cell's particle range and its field values
+    int done = 0;
+    double co = 0.0, cd = 0.0;
     if (SPEC) {
-        if (a.it->done) return;
+        done = a.it->done;
+        conv_load(a.conv_prev, a.nblk, co, cd);
+    }
+    const int p0 = a.cs[cc], p1 = a.cs[cc + 1];
+    double fv[(NF + 63) / 64];
+#pragma unroll
+    for (int u = 0; u < (NF + 63) / 64; u++) {
+        const int l = min(lane + 64 * u, NF - 1);
+        const int fb = l / (6 * NM), r = l % (6 * NM);
+        const int n = r / (3 * NM), comp = (r / NM) % 3, k = r % NM;
+        const double *f = fb ? a.b : a.e;
+        fv[u] = f[((size_t)comp * nm + k) * ld + ix + n];
+    }
+    if (SPEC) {
+        if (done) return;
         // 'compare' of the previous iteration (simulation_class.f03:396):
-        // every wave evaluates it; workgroup 0 records the decision
+        // every wave evaluates it (the same decision everywhere);
+        // workgroup 0 records it
         double rel, ab;
-        conv_residuals(a.conv_prev, a.nblk, rel, ab);
+        conv_finish(co, cd, rel, ab);
         if (rel < a.reltol || ab < a.abstol) {
             if (blockIdx.x == 0 && threadIdx.x == 0) a.it->done = 1;
             return;
         }
     }
-    const int lane = threadIdx.x & 63;
-    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-    const int c = blockIdx.x * kDepWaves + wave;
-    const int nm = a.nm, ld = a.ld;
-    const int ix = c + 1;
-    if (c < a.nr) {
-        for (int l = lane; l < 12 * NM; l += 64) {
-            const int fb = l / (6 * NM), r = l % (6 * NM);
-            const int n = r / (3 * NM), cc = (r / NM) % 3, k = r % NM;
-            const double *f = fb ? a.b : a.e;
-            sf[wave][fb][r] = f[((size_t)cc * nm + k) * ld + ix + n];
-        }
+#pragma unroll
+    for (int u = 0; u < (NF + 63) / 64; u++) {
+        const int l = lane + 64 * u;
+        if (l < NF) sf[wave][l / (6 * NM)][l % (6 * NM)] = fv[u];
     }
     __syncthreads();
     if (c >= a.nr) return;
-    const int p0 = a.cs[c], p1 = a.cs[c + 1];
     const NodePairLds<NM> fe{sf[wave][0]}, fbf{sf[wave][1]};
     const double *__restrict__ X1 = a.x1;
     const double *__restrict__ X2 = a.x2;
@@ -387,85 +483,23 @@ __global__ __launch_bounds__(64 * kDepWaves) void k_amjdeposit(JDepArgs a)
     const double *__restrict__ PP2 = a.p2;
     const double *__restrict__ PP3 = a.p3;
     const double *__restrict__ Q = a.q;
-    double *__restrict__ GAM = a.gamma;
-    double *__restrict__ PSI = a.psi;
-    // acc[v]: node ix, acc[NV + v]: node ix + 1 (2 NV = 16 NM values)
     double acc[2 * NV];
 #pragma unroll
     for (int v = 0; v < 2 * NV; v++) acc[v] = 0.0;
-    for (int p = p0 + lane; p < p1; p += 64) {
+    // software-pipelined: the next pass's particle loads are in flight while
+    // this pass computes
+    int p = p0 + lane;
+    const int pl = max(p1 - 1, p0);
+    int pc = min(p, pl);
+    double x1 = X1[pc], x2 = X2[pc], pp1 = PP1[pc], pp2 = PP2[pc], pp3 = PP3[pc], qp = Q[pc];
+    for (; p < p1; p += 64) {
         // the staged fields are re-read from LDS (broadcast) every particle
         // rather than held in registers: the accumulators need the VGPRs
         __asm__ volatile("" ::: "memory");
-        const double x1 = X1[p], x2 = X2[p];
-        const double pp1 = PP1[p], pp2 = PP2[p], pp3 = PP3[p], qp = Q[p];
-        double r = radius(x1, x2);
-        const double pc = x1 / r, ps = x2 / r;
-        double pos = __dmul_rn(r, a.idr);
-        const double w1 = pos - (double)c, w0 = 1.0 - w1;
-        double ep[3], bp[3];
-        gather3<M>(fe, w0, w1, pc, ps, ep);
-        gather3<M>(fbf, w0, w1, pc, ps, bp);
-        double wp0 = ep[0] - bp[1], wp1 = ep[1] + bp[0], wp2 = ep[2];
-        double u0[3] = {pp1 * pc + pp2 * ps, pp2 * pc - pp1 * ps, pp3};
-        double gam = sqrt(1.0 + u0[0] * u0[0] + u0[1] * u0[1] + u0[2] * u0[2]);
-        double qtmh1 = a.qtmh * gam / (gam - u0[2]);
-        double ut[3], u[3];
-#pragma unroll
-        for (int cc = 0; cc < 3; cc++) { ep[cc] = ep[cc] * qtmh1; ut[cc] = u0[cc] + ep[cc]; }
-        gam = sqrt(1.0 + ut[0] * ut[0] + ut[1] * ut[1] + ut[2] * ut[2]);
-        double qtmh2 = a.qtmh / (gam - ut[2]);
-#pragma unroll
-        for (int cc = 0; cc < 3; cc++) bp[cc] = bp[cc] * qtmh2;
-        u[0] = ut[0] + ut[1] * bp[2] - ut[2] * bp[1];
-        u[1] = ut[1] + ut[2] * bp[0] - ut[0] * bp[2];
-        u[2] = ut[2] + ut[0] * bp[1] - ut[1] * bp[0];
-        double ostq = 2.0 / (1.0 + bp[0] * bp[0] + bp[1] * bp[1] + bp[2] * bp[2]);
-#pragma unroll
-        for (int cc = 0; cc < 3; cc++) bp[cc] = bp[cc] * ostq;
-        ut[0] = ut[0] + u[1] * bp[2] - u[2] * bp[1];
-        ut[1] = ut[1] + u[2] * bp[0] - u[0] * bp[2];
-        ut[2] = ut[2] + u[0] * bp[1] - u[1] * bp[0];
-#pragma unroll
-        for (int cc = 0; cc < 3; cc++) u[cc] = ut[cc] + ep[cc];
-        double du0 = a.idt * (u[0] - u0[0]), du1 = a.idt * (u[1] - u0[1]);
-#pragma unroll
-        for (int cc = 0; cc < 3; cc++) u[cc] = 0.5 * (u[cc] + u0[cc]);
-        double g2 = sqrt(1.0 + u[0] * u[0] + u[1] * u[1] + u[2] * u[2]);
-        double ipsi = 1.0 / (g2 - u[2]);
-        if (a.store_diag) {
-            GAM[p] = g2;
-            PSI[p] = (1.0 - 1.0 / ipsi) / a.qbm;
-        }
-        double dpsi = a.qbm * (wp2 - (wp0 * u[0] + wp1 * u[1]) * ipsi);
-        du0 = du0 + u[0] * dpsi * ipsi;
-        du1 = du1 + u[1] * dpsi * ipsi;
-        const double val[8] = {u[0], u[1], u[2], du0, du1, u[0] * u[0] * ipsi, u[0] * u[1] * ipsi,
-                               u[1] * u[1] * ipsi};
-        double phr = 1.0 * qp * ipsi, phi = 0.0;
-        {
-            const double wa = w0 * phr, wb = w1 * phr;
-#pragma unroll
-            for (int cc = 0; cc < 8; cc++) {
-                acc[cc * NM] += wa * val[cc];
-                acc[NV + cc * NM] += wb * val[cc];
-            }
-        }
-        const double ph0r = pc, ph0i = -ps;
-#pragma unroll
-        for (int m = 1; m <= M; m++) {
-            double tr = phr * ph0r - phi * ph0i, ti = phr * ph0i + phi * ph0r;
-            phr = tr;
-            phi = ti;
-            const double war = w0 * phr, wbr = w1 * phr, wai = w0 * phi, wbi = w1 * phi;
-#pragma unroll
-            for (int cc = 0; cc < 8; cc++) {
-                acc[cc * NM + 2 * m - 1] += war * val[cc];
-                acc[NV + cc * NM + 2 * m - 1] += wbr * val[cc];
-                acc[cc * NM + 2 * m] += wai * val[cc];
-                acc[NV + cc * NM + 2 * m] += wbi * val[cc];
-            }
-        }
+        const int pn = min(p + 64, pl);
+        const double nx1 = X1[pn], nx2 = X2[pn], np1 = PP1[pn], np2 = PP2[pn], np3 = PP3[pn], nq = Q[pn];
+        deposit_particle<M>(a, fe, fbf, c, p, x1, x2, pp1, pp2, pp3, qp, acc);
+        x1 = nx1; x2 = nx2; pp1 = np1; pp2 = np2; pp3 = np3; qp = nq;
     }
     wave_reduce_scatter<NM>(acc);
     const int g = lane >> 2;
