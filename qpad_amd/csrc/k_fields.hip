@@ -782,27 +782,6 @@ int field_kernels_init(size_t)
 }
 
 // ---------------------------------------------------------------------------
-// J_perp predictor for the next slice (simulation_class.f03:425-426)
-// ---------------------------------------------------------------------------
-__global__ void k_predictor(double *cu, double *dcu, double dxi, int64_t n)
-{
-    int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= n) return;
-    double d = dxi * dcu[t];
-    dcu[t] = d;
-    cu[t] = cu[t] + d;
-}
-
-int launch_predictor(qpad_ctx *c)
-{
-    int64_t n = (int64_t)2 * c->g.nm * c->g.ld;
-    hipLaunchKernelGGL(k_predictor, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, c->stream,
-                       c->f[QPAD_F_CU], c->f[QPAD_F_DCU], c->g.dxi, n);
-    QPAD_HIP_CHECK(hipGetLastError());
-    return 0;
-}
-
-// ---------------------------------------------------------------------------
 // generic fill / copy (used instead of hipMemsetAsync so that captured
 // graphs never contain memset nodes)
 // ---------------------------------------------------------------------------

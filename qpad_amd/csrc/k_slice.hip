@@ -435,11 +435,14 @@ int launch_iteration_fields(qpad_ctx *c, int j, int l)
 
 // ---------------------------------------------------------------------------
 // slice epilogue (simulation_class.f03:401-456): cbq, E of the plasma alone,
-// history copies, beam fields of slice j into the slice arrays
+// history copies, beam fields of slice j into the slice arrays, J_perp
+// predictor
 // ---------------------------------------------------------------------------
 struct EpiArgs {
     FieldCommon g;
-    const double *e, *b, *psi, *cu, *b_spe, *q_beam_j, *b_beam_j;
+    const double *e, *b, *psi, *b_spe, *q_beam_j, *b_beam_j;
+    double *cu, *dcu;   // J_perp predictor in place: dcu *= dxi, cu(1:2) += dcu
+    double dxi;
     double *e_spe, *q_spe;
     double *h_e, *h_b, *h_psi, *h_b_spe, *h_e_spe, *h_cu, *h_q_spe;
     double *q_beam_f1, *b_beam_f1;
@@ -495,6 +498,13 @@ __global__ __launch_bounds__(256) void k_epilogue(EpiArgs a)
     a.h_q_spe[o] = q;
     a.h_psi[o] = a.psi[o];
     a.q_beam_f1[o] = a.q_beam_j[o];
+    // J_perp predictor for the next slice, after the history copy of cu
+    // (simulation_class.f03:425-426)
+    for (int c = 0; c < 2; c++) {
+        const double d = a.dxi * FX(a.dcu, c, k, i);
+        FX(a.dcu, c, k, i) = d;
+        FX(a.cu, c, k, i) = FX(a.cu, c, k, i) + d;
+    }
 }
 
 int launch_epilogue(qpad_ctx *c, int j)
@@ -506,6 +516,8 @@ int launch_epilogue(qpad_ctx *c, int j)
     a.b = c->f[QPAD_F_B];
     a.psi = c->f[QPAD_F_PSI];
     a.cu = c->f[QPAD_F_CU];
+    a.dcu = c->f[QPAD_F_DCU];
+    a.dxi = c->g.dxi;
     a.b_spe = c->f[QPAD_F_B_SPE];
     a.q_beam_j = c->h_q_beam + s1;
     a.b_beam_j = c->h_b_beam + s3;

@@ -33,12 +33,13 @@ int launch_bt_beam_all(qpad_ctx *c);
 int launch_bt_beam_one(qpad_ctx *c, int j);
 // convergence_tester 'compare' alone
 int launch_compare_only(qpad_ctx *c);
-// J_perp predictor: dcu *= dxi ; cu(1:2) += dcu(1:2)
-// fused predictor-corrector iteration after the current deposit: combine +
-// all sources (k_src), then B_perp, Bz, Ez, E_perp, b and the convergence
-// test (k_solve); both return at once once the iteration has converged
+// field kernels of predictor-corrector iteration l after its current
+// deposit: cu/acu/amu and every right-hand side (k_src), the solves
+// (k_solve), then b_spe, b, E_perp and the convergence maxima (k_bfields);
+// each returns at once once the iteration has converged
 int launch_iteration_fields(qpad_ctx *c, int j, int l);
-// slice epilogue: cbq, e_spe, history copies, beam fields of slice j
+// slice epilogue: cbq, e_spe, history copies, beam fields of slice j, the
+// J_perp predictor for the next slice
 int launch_epilogue(qpad_ctx *c, int j);
 int slice_kernels_init();
 // start of a slice after the charge deposit: q_spe, psi, Bz, Ez, 'record',
@@ -46,7 +47,7 @@ int slice_kernels_init();
 int launch_slice_start(qpad_ctx *c, int j);
 // rows of the solve right-hand-side buffer
 int src_rows_host(int M, int nm);
-int launch_predictor(qpad_ctx *c);
+// rows of the solve right-hand-side buffer
 
 // ---- plasma (k_plasma.hip) ----
 int launch_inject(qpad_ctx *c, int sp);

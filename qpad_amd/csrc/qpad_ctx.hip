@@ -533,7 +533,6 @@ static int slice_enqueue(qpad_ctx *c, int j)
     }
     // slice epilogue: cbq, e_spe, history (simulation_class.f03:401-456)
     TRY(launch_epilogue(c, j));
-    TRY(launch_predictor(c));
     for (int s = 0; s < c->nspecies; s++) {
         TRY(launch_push(c, s));
         TRY(launch_sort(c, s));
