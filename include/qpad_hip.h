@@ -213,6 +213,83 @@ int qpad_set_beam(qpad_ctx *ctx, int32_t ib, int64_t n, const double *x1,
 int qpad_kernel_time(qpad_ctx *ctx, int32_t which, double *ms, int64_t *launches);
 int qpad_kernel_time_reset(qpad_ctx *ctx, int32_t enable);
 
+/* ---- learned B_perp seed of the predictor-corrector iteration ----
+   No counterpart in QPAD.  QPAD seeds the iteration of slice xi with the
+   converged B_perp of slice xi-1 (b_spe is zeroed only at the 3D-step start,
+   simulation_class.f03:326, and read by the first 'record', :370-375).  The
+   seed network replaces that seed by 2 B_perp(xi-1) - B_perp(xi-2) + dB: the
+   first-order extrapolation in xi plus a correction dB predicted by a 1D
+   convolution network over r from the fields of slice xi-1 and the sources
+   of slice xi (the output layer starts at zero: untrained, the seed is the
+   extrapolation).  The iteration still runs to the same tolerance and
+   iter_max, so the plain method remains the judge.
+
+   Network: `layers` conv1d layers over the radial nodes 1..nr, kernel 3,
+   zero padding, SiLU between layers, `width` hidden channels.  Inputs per
+   node (8 (2M+1) + 1 channels, each group scaled by its maximum over the
+   slice):
+     B_r, B_phi of b_spe (slice xi-1);  B_r, B_phi of b_spe(xi-1) -
+     b_spe(xi-2);  J_r, J_phi of cu (the predictor of slice xi);  q_spe and
+     q_beam of slice xi;  r / r_max.
+   Output: dB_r, dB_phi (2 (2M+1) channels) in units of out_scale times the
+   B_perp scale; training target: (converged - extrapolated) in the same
+   units, clipped to [-1, 1] (so the correction never moves the seed by more
+   than out_scale max|B_perp| from the extrapolation).
+   Weights and activations are BF16, products accumulate in FP32 (matrix
+   cores); master weights and Adam moments are FP32.
+
+   Parameter layout (FP32, qpad_seednet_get_params): layer after layer,
+   W[3][cout_p][cin_p] then bias[cout_p], with the padded channel counts of
+   qpad_seednet_info; padding entries are zero and stay zero. */
+typedef struct {
+    int32_t width;   /* hidden channels (64) */
+    int32_t layers;  /* conv layers, 2..8 (4) */
+    float lr;        /* Adam step size (1e-3) */
+    float beta1;     /* (0.9) */
+    float beta2;     /* (0.999) */
+    float eps;       /* (1e-8) */
+    float out_scale; /* correction units, fraction of max|B_perp| (0.02) */
+    uint64_t seed;   /* weight initialisation */
+} qpad_seednet_cfg;
+
+typedef struct {
+    int32_t layers;
+    int32_t cin;          /* input channels used: 8 (2M+1) + 1 */
+    int32_t cout;         /* output channels used: 2 (2M+1) */
+    int32_t npos;         /* radial nodes (nr) */
+    int32_t cin_p[8];     /* padded input channels of each layer */
+    int32_t cout_p[8];    /* padded output channels of each layer */
+    int64_t nparams;      /* FP32 parameters, padding included */
+    int64_t steps;        /* Adam steps taken */
+} qpad_seednet_info;
+
+/* create (or re-create) the context's seed network; both modes start off */
+int qpad_seednet_create(qpad_ctx *ctx, const qpad_seednet_cfg *cfg);
+int qpad_seednet_get_info(qpad_ctx *ctx, qpad_seednet_info *out);
+/* seed != 0: qpad_slice / qpad_step3d seed each slice's iteration with the
+   prediction; train != 0: they take one Adam step per slice on (inputs,
+   converged B_perp - extrapolation) once the slice has converged */
+int qpad_seednet_mode(qpad_ctx *ctx, int32_t seed, int32_t train);
+int qpad_seednet_get_params(qpad_ctx *ctx, float *theta, int64_t n);
+int qpad_seednet_set_params(qpad_ctx *ctx, const float *theta, int64_t n);
+/* training losses of Adam steps first .. first + n - 1 (the last 65536 are
+   kept; mean squared error of the normalised output) */
+int qpad_seednet_losses(qpad_ctx *ctx, int64_t first, int64_t n, float *out);
+/* Fine-grained form for a driver that calls QPAD's procedures one by one:
+   qpad_seednet_seed after q_spe and cu of slice j are final and before the
+   first 'record' (scales and packs the inputs; in seed mode it also sets
+   b_spe to the seed), qpad_seednet_learn once slice j has converged
+   (one Adam step; needs qpad_seednet_seed of the same slice first). */
+int qpad_seednet_seed(qpad_ctx *ctx, int32_t j);
+int qpad_seednet_learn(qpad_ctx *ctx, int32_t j);
+/* The network on host inputs (tests, offline use): x [npos][cin] FP32
+   (rounded to BF16 on the device), t [npos][cout] FP32 target or NULL.
+   Returns y [npos][cout] and, with t, the loss and the gradient of every
+   parameter (grad: nparams entries or NULL); update != 0 also takes the
+   Adam step.  npos <= nr.  Slice state is not touched. */
+int qpad_seednet_eval(qpad_ctx *ctx, const float *x, const float *t, int32_t npos,
+                      float *y, float *loss, float *grad, int32_t update);
+
 #ifdef __cplusplus
 }
 #endif

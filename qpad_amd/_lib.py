@@ -22,6 +22,7 @@ PUSH2_ROBUST = 1
 
 _dp = C.POINTER(C.c_double)
 _ip = C.POINTER(C.c_int32)
+_fp = C.POINTER(C.c_float)
 
 
 class QpadError(RuntimeError):
@@ -40,6 +41,18 @@ class SpeciesCfg(C.Structure):
                 ("qm", C.c_double), ("mass", C.c_double), ("density", C.c_double),
                 ("uth", C.c_double * 3), ("neutralized", C.c_int32), ("den_min", C.c_double),
                 ("seed", C.c_uint64), ("push_type", C.c_int32)]
+
+
+class SeedNetCfg(C.Structure):
+    _fields_ = [("width", C.c_int32), ("layers", C.c_int32), ("lr", C.c_float),
+                ("beta1", C.c_float), ("beta2", C.c_float), ("eps", C.c_float),
+                ("out_scale", C.c_float), ("seed", C.c_uint64)]
+
+
+class SeedNetInfo(C.Structure):
+    _fields_ = [("layers", C.c_int32), ("cin", C.c_int32), ("cout", C.c_int32),
+                ("npos", C.c_int32), ("cin_p", C.c_int32 * 8), ("cout_p", C.c_int32 * 8),
+                ("nparams", C.c_int64), ("steps", C.c_int64)]
 
 
 class BeamCfg(C.Structure):
@@ -92,6 +105,15 @@ SIGNATURES = {
     "qpad_set_beam": (C.c_int, [_CTX, C.c_int32, C.c_int64] + [_dp] * 7),
     "qpad_kernel_time": (C.c_int, [_CTX, C.c_int32, _dp, C.POINTER(C.c_int64)]),
     "qpad_kernel_time_reset": (C.c_int, [_CTX, C.c_int32]),
+    "qpad_seednet_create": (C.c_int, [_CTX, C.POINTER(SeedNetCfg)]),
+    "qpad_seednet_get_info": (C.c_int, [_CTX, C.POINTER(SeedNetInfo)]),
+    "qpad_seednet_mode": (C.c_int, [_CTX, C.c_int32, C.c_int32]),
+    "qpad_seednet_get_params": (C.c_int, [_CTX, _fp, C.c_int64]),
+    "qpad_seednet_set_params": (C.c_int, [_CTX, _fp, C.c_int64]),
+    "qpad_seednet_losses": (C.c_int, [_CTX, C.c_int64, C.c_int64, _fp]),
+    "qpad_seednet_seed": (C.c_int, [_CTX, C.c_int32]),
+    "qpad_seednet_learn": (C.c_int, [_CTX, C.c_int32]),
+    "qpad_seednet_eval": (C.c_int, [_CTX, _fp, _fp, C.c_int32, _fp, _fp, _fp, C.c_int32]),
 }
 
 _lib = None
@@ -126,6 +148,13 @@ def dptr(a):
         return None
     assert a.dtype == np.float64 and a.flags["C_CONTIGUOUS"], "need contiguous float64"
     return a.ctypes.data_as(_dp)
+
+
+def fptr(a):
+    if a is None:
+        return None
+    assert a.dtype == np.float32 and a.flags["C_CONTIGUOUS"], "need contiguous float32"
+    return a.ctypes.data_as(_fp)
 
 
 def iptr(a):

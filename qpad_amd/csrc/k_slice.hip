@@ -693,7 +693,8 @@ __global__ __launch_bounds__(256) void k_eperp(EperpArgs a)
 
 int src_rows_host(int M, int nm) { return src_rows(M, nm); }
 
-int launch_slice_start(qpad_ctx *c, int j)
+// sources of the slice start (q_spe, the psi / Bz / Ez right-hand sides)
+int launch_slice_sources(qpad_ctx *c)
 {
     const FieldCommon g = field_common(c);
     const int nm = c->g.nm;
@@ -716,7 +717,16 @@ int launch_slice_start(qpad_ctx *c, int j)
     default: hipLaunchKernelGGL(k_start_src<4>, ngrid, dim3(256), 0, c->stream, s); break;
     }
     QPAD_HIP_CHECK(hipGetLastError());
+    return 0;
+}
 
+// psi, Bz from the J predictor, Ez, the first 'record' and E_perp: everything
+// of the slice start that reads b_spe (the iteration's seed)
+int launch_slice_solves(qpad_ctx *c, int j)
+{
+    const FieldCommon g = field_common(c);
+    const int nm = c->g.nm;
+    const dim3 ngrid((unsigned)((c->g.nr + 2 + 255) / 256), nm);
     StartSolveArgs a;
     a.g = g;
     a.src = c->src;

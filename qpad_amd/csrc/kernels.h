@@ -44,7 +44,16 @@ int launch_epilogue(qpad_ctx *c, int j);
 int slice_kernels_init();
 // start of a slice after the charge deposit: q_spe, psi, Bz, Ez, 'record',
 // then E_perp and b for the first current deposit
-int launch_slice_start(qpad_ctx *c, int j);
+int launch_slice_sources(qpad_ctx *c);
+int launch_slice_solves(qpad_ctx *c, int j);
+
+// learned B_perp seed (k_seednet.hip): inputs and, in seed mode, the seed at
+// the slice start (force: also when both modes are off); one Adam step after
+// the slice has converged (train mode); the 3D-step start
+int seednet_slice_start(qpad_ctx *c, int j, bool force);
+int seednet_slice_end(qpad_ctx *c, int j);
+int seednet_step_begin(qpad_ctx *c);
+void seednet_destroy(qpad_ctx *c);
 // rows of the solve right-hand-side buffer
 int src_rows_host(int M, int nm);
 // rows of the solve right-hand-side buffer

@@ -357,6 +357,7 @@ int qpad_destroy(qpad_ctx *c)
 {
     if (!c) return 0;
     if (c->stream) (void)hipStreamSynchronize(c->stream);
+    seednet_destroy(c);
     for (int f = 0; f < QPAD_NFIELD; f++) dfree(c->f[f]);
     dfree(c->h_psi); dfree(c->h_e); dfree(c->h_b); dfree(c->h_b_spe); dfree(c->h_e_spe);
     dfree(c->h_cu); dfree(c->h_q_spe); dfree(c->h_q_beam); dfree(c->h_b_beam);
@@ -503,6 +504,7 @@ int qpad_step3d_begin(qpad_ctx *c)
     const int zf[] = {QPAD_F_B, QPAD_F_E, QPAD_F_B_SPE, QPAD_F_E_SPE, QPAD_F_PSI, QPAD_F_CU, QPAD_F_ACU,
                       QPAD_F_AMU};
     for (int f : zf) TRY(zero_dev(c, c->f[f], f1_size(c, c->ncomp[f])));
+    TRY(seednet_step_begin(c));
     c->step_open = true;
     return sync(c);
 }
@@ -519,7 +521,9 @@ static int slice_enqueue(qpad_ctx *c, int j)
 {
     // rho - Jz of every species, psi, Bz from the J predictor, 'record'
     for (int s = 0; s < c->nspecies; s++) TRY(launch_qdeposit(c, s));
-    TRY(launch_slice_start(c, j));
+    TRY(launch_slice_sources(c));
+    TRY(seednet_slice_start(c, j, false));
+    TRY(launch_slice_solves(c, j));
     // speculative iterations: each kernel returns at once after convergence.
     // Timing events (when enabled) bracket only the first iteration, which
     // always does work: an event per launch would cost a quarter of the step.
@@ -533,6 +537,7 @@ static int slice_enqueue(qpad_ctx *c, int j)
     }
     // slice epilogue: cbq, e_spe, history (simulation_class.f03:401-456)
     TRY(launch_epilogue(c, j));
+    TRY(seednet_slice_end(c, j));
     for (int s = 0; s < c->nspecies; s++) {
         TRY(launch_push(c, s));
         TRY(launch_sort(c, s));

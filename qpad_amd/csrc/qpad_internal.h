@@ -83,6 +83,10 @@ struct IterState {        // device-resident iteration bookkeeping
 
 }  // namespace qpad
 
+namespace qpad {
+struct SeedNet;   // the learned B_perp seed (k_seednet.hip)
+}
+
 struct qpad_ctx {
     qpad::Grid g;
     hipStream_t stream;
@@ -106,6 +110,7 @@ struct qpad_ctx {
     bool timing;
     qpad::EventPool pools[3];
     bool step_open;
+    qpad::SeedNet *net;       // null unless qpad_seednet_create
 };
 
 namespace qpad {

@@ -56,6 +56,19 @@ module qpad_hip_mod
     integer(c_int64_t) :: seed
   end type qpad_beam_cfg
 
+  ! learned B_perp seed of the iteration (no counterpart in QPAD)
+  type, bind(C), public :: qpad_seednet_cfg
+    integer(c_int32_t) :: width, layers
+    real(c_float) :: lr, beta1, beta2, eps, out_scale
+    integer(c_int64_t) :: seed
+  end type qpad_seednet_cfg
+
+  type, bind(C), public :: qpad_seednet_info
+    integer(c_int32_t) :: layers, cin, cout, npos
+    integer(c_int32_t) :: cin_p(8), cout_p(8)
+    integer(c_int64_t) :: nparams, steps
+  end type qpad_seednet_info
+
   public :: qpad_hip_check, qpad_error_message
 
   interface
@@ -353,6 +366,74 @@ module qpad_hip_mod
       integer(c_int) :: rc
     end function
 
+    function qpad_seednet_create(ctx, cfg) bind(C, name="qpad_seednet_create") result(rc)
+      import :: c_int, c_ptr, qpad_seednet_cfg
+      type(c_ptr), value :: ctx
+      type(qpad_seednet_cfg), intent(in) :: cfg
+      integer(c_int) :: rc
+    end function
+
+    function qpad_seednet_get_info(ctx, info) bind(C, name="qpad_seednet_get_info") result(rc)
+      import :: c_int, c_ptr, qpad_seednet_info
+      type(c_ptr), value :: ctx
+      type(qpad_seednet_info), intent(out) :: info
+      integer(c_int) :: rc
+    end function
+
+    function qpad_seednet_mode(ctx, seed, train) bind(C, name="qpad_seednet_mode") result(rc)
+      import :: c_int, c_int32_t, c_ptr
+      type(c_ptr), value :: ctx
+      integer(c_int32_t), value :: seed, train
+      integer(c_int) :: rc
+    end function
+
+    function qpad_seednet_get_params(ctx, theta, n) bind(C, name="qpad_seednet_get_params") result(rc)
+      import :: c_int, c_int64_t, c_ptr, c_float
+      type(c_ptr), value :: ctx
+      real(c_float), intent(out) :: theta(*)
+      integer(c_int64_t), value :: n
+      integer(c_int) :: rc
+    end function
+
+    function qpad_seednet_set_params(ctx, theta, n) bind(C, name="qpad_seednet_set_params") result(rc)
+      import :: c_int, c_int64_t, c_ptr, c_float
+      type(c_ptr), value :: ctx
+      real(c_float), intent(in) :: theta(*)
+      integer(c_int64_t), value :: n
+      integer(c_int) :: rc
+    end function
+
+    function qpad_seednet_losses(ctx, first, n, out) bind(C, name="qpad_seednet_losses") result(rc)
+      import :: c_int, c_int64_t, c_ptr, c_float
+      type(c_ptr), value :: ctx
+      integer(c_int64_t), value :: first, n
+      real(c_float), intent(out) :: out(*)
+      integer(c_int) :: rc
+    end function
+
+    function qpad_seednet_seed(ctx, j) bind(C, name="qpad_seednet_seed") result(rc)
+      import :: c_int, c_int32_t, c_ptr
+      type(c_ptr), value :: ctx
+      integer(c_int32_t), value :: j
+      integer(c_int) :: rc
+    end function
+
+    function qpad_seednet_learn(ctx, j) bind(C, name="qpad_seednet_learn") result(rc)
+      import :: c_int, c_int32_t, c_ptr
+      type(c_ptr), value :: ctx
+      integer(c_int32_t), value :: j
+      integer(c_int) :: rc
+    end function
+
+    ! x, t, y, grad: pass c_loc of the arrays, or c_null_ptr to omit
+    function qpad_seednet_eval(ctx, x, t, npos, y, loss, grad, update) bind(C, name="qpad_seednet_eval") &
+        result(rc)
+      import :: c_int, c_int32_t, c_ptr
+      type(c_ptr), value :: ctx, x, t, y, loss, grad
+      integer(c_int32_t), value :: npos, update
+      integer(c_int) :: rc
+    end function
+
   end interface
 
   public :: qpad_last_error, qpad_abi_version, qpad_create, qpad_destroy, &
@@ -365,7 +446,10 @@ module qpad_hip_mod
     qpad_get_field, qpad_set_field, qpad_get_hist, qpad_set_hist, &
     qpad_species_count, qpad_get_species, qpad_set_species, &
     qpad_get_species_qn, qpad_set_species_qn, qpad_beam_count, qpad_get_beam, &
-    qpad_set_beam, qpad_kernel_time, qpad_kernel_time_reset
+    qpad_set_beam, qpad_kernel_time, qpad_kernel_time_reset, &
+    qpad_seednet_create, qpad_seednet_get_info, qpad_seednet_mode, qpad_seednet_get_params, &
+    qpad_seednet_set_params, qpad_seednet_losses, qpad_seednet_seed, qpad_seednet_learn, &
+    qpad_seednet_eval
 
 contains
 

@@ -185,6 +185,70 @@ class Beam3D:
                                            *[L.dptr(x) for x in a]))
 
 
+class SeedNet:
+    """The learned B_perp seed of the predictor-corrector iteration (no
+    counterpart in QPAD: it replaces the b_spe(xi-1) seed that QPAD's
+    iteration starts from, simulation_class.f03:326,370-375).  A 1D conv
+    network over r; BF16 weights and activations, FP32 accumulation, Adam on
+    FP32 master weights, all on the GPU (include/qpad_hip.h)."""
+
+    def __init__(self, sim, width=64, layers=4, lr=1e-4, beta1=0.9, beta2=0.999, eps=1e-8, out_scale=0.02,
+                 seed=1):
+        self.sim = sim
+        self.cfg = L.SeedNetCfg(width, layers, lr, beta1, beta2, eps, out_scale, seed)
+        L.check(sim.lib.qpad_seednet_create(sim.ctx, C.byref(self.cfg)))
+
+    def info(self):
+        i = L.SeedNetInfo()
+        L.check(self.sim.lib.qpad_seednet_get_info(self.sim.ctx, C.byref(i)))
+        return i
+
+    @property
+    def nparams(self):
+        return self.info().nparams
+
+    def mode(self, seed, train):
+        L.check(self.sim.lib.qpad_seednet_mode(self.sim.ctx, int(bool(seed)), int(bool(train))))
+
+    def get_params(self):
+        th = np.zeros(self.nparams, np.float32)
+        L.check(self.sim.lib.qpad_seednet_get_params(self.sim.ctx, L.fptr(th), th.size))
+        return th
+
+    def set_params(self, th):
+        th = np.ascontiguousarray(th, np.float32)
+        L.check(self.sim.lib.qpad_seednet_set_params(self.sim.ctx, L.fptr(th), th.size))
+
+    def losses(self, first=0, n=None):
+        steps = self.info().steps
+        n = steps - first if n is None else n
+        out = np.zeros(max(n, 0), np.float32)
+        if n > 0:
+            L.check(self.sim.lib.qpad_seednet_losses(self.sim.ctx, int(first), int(n), L.fptr(out)))
+        return out
+
+    def seed(self, j):
+        L.check(self.sim.lib.qpad_seednet_seed(self.sim.ctx, int(j)))
+
+    def learn(self, j):
+        L.check(self.sim.lib.qpad_seednet_learn(self.sim.ctx, int(j)))
+
+    def eval(self, x, t=None, grad=False, update=False):
+        """y (and with t: loss, gradient) of the network on host inputs x [npos][cin]."""
+        inf = self.info()
+        x = np.ascontiguousarray(x, np.float32)
+        npos = x.shape[0]
+        assert x.shape == (npos, inf.cin)
+        y = np.zeros((npos, inf.cout), np.float32)
+        loss = C.c_float(0.0)
+        g = np.zeros(inf.nparams, np.float32) if grad else None
+        tt = None if t is None else np.ascontiguousarray(t, np.float32)
+        L.check(self.sim.lib.qpad_seednet_eval(self.sim.ctx, L.fptr(x), L.fptr(tt), npos, L.fptr(y),
+                                               C.byref(loss) if tt is not None else None, L.fptr(g),
+                                               int(bool(update))))
+        return y, (loss.value if tt is not None else None), g
+
+
 class Simulation:
     """simulation (simulation_class.f03:34-69) driven from a qpinput deck."""
 
