@@ -1,13 +1,21 @@
 """Benchmark: QPAD per-xi-slice plasma response + beam push on one MI355X.
 
 python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload c2|c1|c3]
-                [--cpu-slices S] [--no-cpu]
+                [--seed learned|plain] [--train-steps T] [--cpu-slices S] [--no-cpu]
 
 A step is one 3D step of the workload: every xi-slice of the plasma response
 (with every B_perp predictor-corrector iteration), the beam deposit, the beam
 B of all slices and the beam push (include/qpad_hip.h: qpad_step3d).  Inputs
 are synthetic (lattice plasma, Philox-drawn Gaussian beam) and resident in
 HBM before the timed region.  Prints one JSON line.
+
+Two legs run from identical initial states, to the same iteration
+tolerance: the plain method (QPAD's seed b_spe(xi-1)) and the learned seed
+(first-order extrapolation plus the conv network's correction, trained
+online during the W + T untimed warm-up steps, frozen in the timed steps).
+Both time the same K steps (3D steps W+T+1 .. W+T+K of the same physics).
+`value` is the leg named by --seed (learned by default); the other leg is
+reported beside it.
 """
 import argparse
 import json
@@ -34,6 +42,10 @@ def parse():
     ap.add_argument("--cpu-slices", type=int, default=24)
     ap.add_argument("--cpu-start", type=int, default=100)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--seed", default="learned", choices=["learned", "plain"],
+                    help="the leg reported as value (both legs always run)")
+    ap.add_argument("--train-steps", type=int, default=2,
+                    help="extra untimed 3D steps of online training of the seed network")
     ap.add_argument("--profile-slices", type=int, default=0,
                     help="run only a 512 MiB calibration copy and this many slices of the workload "
                          "(for rocprofv3 counter runs), print nothing")
@@ -186,32 +198,64 @@ def main():
         print(json.dumps({"error": "this build targets one MI355X: multi-GPU runs are not "
                                    "part of it (use --gpus 1)", "n_gpus": args.gpus}))
         sys.exit(2)
-    from qpad_amd import Simulation
+    from qpad_amd import SeedNet, Simulation
     deck = deck_for(args.workload)
-    sim = Simulation(deck)
+    legs = {"plain": Simulation(deck), "learned": Simulation(deck)}
+    net = SeedNet(legs["learned"])
+    net.mode(seed=True, train=True)
+    sim = legs[args.seed]
     g = sim.grid
     np_slice = sum(sp.count() for sp in sim.spe)
     nbeam = sum(b.count() for b in sim.beam)
-    for _ in range(args.warmup):
-        sim.step3d()
-    # timed region: K whole 3D steps, HIP events around the dominant kernel
-    sim.kernel_time_reset(True)
-    t0 = time.perf_counter()
-    total_iters = 0
-    for _ in range(args.steps):
-        total_iters += sim.step3d()
-    t1 = time.perf_counter()
+    # first warm-up step of both legs from the same state: the learned seed's
+    # converged fields against the plain method's, every 8th slice
+    warm_iters = {k: s.step3d() / g.nxi for k, s in legs.items()}
+    field_diff = {}
+    for name in ("psi", "b_spe", "e"):
+        fp, fl = legs["plain"].fields[name], legs["learned"].fields[name]
+        num = den = 0.0
+        for j in range(1, g.nxi + 1, 8):
+            a, b = fp.hist(j), fl.hist(j)
+            num = max(num, float(np.max(np.abs(a - b))))
+            den = max(den, float(np.max(np.abs(a))))
+        field_diff[name] = num / den if den > 0 else num
+    for _ in range(args.warmup + args.train_steps - 1):
+        for s_ in legs.values():
+            s_.step3d()
+    net.mode(seed=True, train=False)
+    train_losses = net.losses()
+    # timed region of each leg: K whole 3D steps; HIP events around the
+    # dominant kernel in the reported leg
+    timing = {}
+    for name, s_ in legs.items():
+        if s_ is sim:
+            s_.kernel_time_reset(True)
+        t0 = time.perf_counter()
+        it = 0
+        for _ in range(args.steps):
+            it += s_.step3d()
+        timing[name] = (time.perf_counter() - t0, it / (args.steps * g.nxi))
     dep_ms, dep_n = sim.kernel_time(0)
     push_ms, push_n = sim.kernel_time(1)
     sol_ms, sol_n = sim.kernel_time(2)
     sim.kernel_time_reset(False)
-    secs = t1 - t0
+    secs, iters_per_slice = timing[args.seed]
     steps_per_s = args.steps / secs
-    iters_per_slice = total_iters / (args.steps * g.nxi)
-    # per-slice iteration profile (one instrumented, untimed step)
-    sim.step3d_begin()
-    per_slice = [sim.slice(j) for j in range(1, g.nxi + 1)]
-    sim.step3d_end()
+    legs_out = {name: {"value": args.steps / t, "ms_per_step": 1e3 * t / args.steps, "iters_per_slice": its}
+                for name, (t, its) in timing.items()}
+    legs_out["learned"].update({
+        "train_steps": int(net.info().steps), "nparams": int(net.nparams),
+        "loss_median_last_step": float(np.median(train_losses[-g.nxi:])) if len(train_losses) else None,
+        "first_step_iters_per_slice": warm_iters["learned"],
+        "fields_max_rel_diff_vs_plain_first_step": field_diff})
+    legs_out["plain"]["first_step_iters_per_slice"] = warm_iters["plain"]
+    # per-slice iteration profiles (one instrumented, untimed step per leg)
+    prof = {}
+    for name, s_ in legs.items():
+        s_.step3d_begin()
+        prof[name] = [s_.slice(j) for j in range(1, g.nxi + 1)]
+        s_.step3d_end()
+    per_slice = prof[args.seed]
     j0 = max(1, min(args.cpu_start, g.nxi - args.cpu_slices + 1))
     # deposit kernel roofline: 48 B read (x1, x2, p1, p2, p3, q) + 16 B written
     # (gamma, psi) per plasma particle per launch.  The HIP events bracket the
@@ -237,13 +281,15 @@ def main():
         "dtype": "f64",
         "data": "synthetic (lattice plasma, Philox Gaussian beam)",
         "config": {"workload": f"{args.workload.upper()}: Nr={g.nr} Nxi={g.nxi} m=0..{g.max_mode} "
-                               f"plasma {np_slice}/slice beam {nbeam}, plain seed, tol "
+                               f"plasma {np_slice}/slice beam {nbeam}, {args.seed} seed, tol "
                                f"{g.iter_reltol:g}",
                    "nr": g.nr, "nxi": g.nxi, "max_mode": g.max_mode,
                    "plasma_particles_per_slice": np_slice, "beam_particles": nbeam,
-                   "iter_tol": g.iter_reltol, "seed": "plain"},
+                   "iter_tol": g.iter_reltol, "seed": args.seed},
         "particle_slice_pushes_per_s": np_slice * g.nxi * steps_per_s,
         "iters_per_slice": iters_per_slice,
+        "seed_legs": legs_out,
+        "learned_vs_plain_speedup": legs_out["learned"]["value"] / legs_out["plain"]["value"],
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS,
                      "traffic": measured_traffic(args.workload, g.max_mode),
@@ -258,14 +304,17 @@ def main():
                                                "sources_and_solves": sol_ms / args.steps},
         "push_ms_per_step": push_ms / args.steps,
     }
+    legs["learned"].close()
     if not args.no_cpu:
-        # replay window: re-run the step up to j0 - 1 and snapshot there
-        sim.step3d_begin()
+        # the oracle is the plain method: replay window of the plain leg (re-run
+        # its step up to j0 - 1 and snapshot there), scaled by its iterations
+        sp = legs["plain"]
+        sp.step3d_begin()
         for j in range(1, j0):
-            sim.slice(j)
+            sp.slice(j)
         cores = min(os.cpu_count() or 1, 16)
-        result["cpu_baseline"] = cpu_baseline(sim, per_slice, j0, args.cpu_slices, cores)
-    sim.close()
+        result["cpu_baseline"] = cpu_baseline(sp, prof["plain"], j0, args.cpu_slices, cores)
+    legs["plain"].close()
     print(json.dumps(result))
 
 

@@ -174,6 +174,15 @@ int qpad_step3d_begin(qpad_ctx *ctx);
 int qpad_slice(qpad_ctx *ctx, int32_t j, int32_t *iters_out);
 int qpad_step3d_end(qpad_ctx *ctx);
 int qpad_step3d(qpad_ctx *ctx, int32_t *total_iters_out);
+/* How qpad_slice / qpad_step3d queue the predictor-corrector iterations (no
+   counterpart in QPAD, whose loop is serial on the host; results do not
+   depend on it).  adaptive = 0: all iter_max iterations are queued at once
+   and the ones after convergence return immediately on the device.
+   adaptive != 0 (default, margin 0): the slice queues as many iterations as
+   the previous slice needed plus `margin`, then one further iteration at a
+   time while the device reports no convergence (one host wait per further
+   iteration, overlapped with that iteration's field solves). */
+int qpad_iteration_policy(qpad_ctx *ctx, int32_t adaptive, int32_t margin);
 
 /* state exchange with the host (diagnostics, restart, parity checks) */
 int qpad_get_field(qpad_ctx *ctx, int32_t fid, double *out);

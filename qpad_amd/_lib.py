@@ -91,6 +91,7 @@ SIGNATURES = {
     "qpad_slice": (C.c_int, [_CTX, C.c_int32, C.POINTER(C.c_int32)]),
     "qpad_step3d_end": (C.c_int, [_CTX]),
     "qpad_step3d": (C.c_int, [_CTX, C.POINTER(C.c_int32)]),
+    "qpad_iteration_policy": (C.c_int, [_CTX, C.c_int32, C.c_int32]),
     "qpad_get_field": (C.c_int, [_CTX, C.c_int32, _dp]),
     "qpad_set_field": (C.c_int, [_CTX, C.c_int32, _dp]),
     "qpad_get_hist": (C.c_int, [_CTX, C.c_int32, C.c_int32, _dp]),

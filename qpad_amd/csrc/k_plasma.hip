@@ -341,6 +341,9 @@ struct JDepArgs {
     const double *conv_prev;
     int nblk;
     double reltol, abstol;
+    // host-visible copy of the decision (done, iterations so far) written by
+    // workgroup 0 of a SPEC launch, or null
+    IterState *report;
 };
 
 // robust predictor and current deposit of one particle of cell c into this
@@ -443,8 +446,10 @@ __global__ __launch_bounds__(64 * kDepWaves) void k_amjdeposit(JDepArgs a)
     // convergence state, the cell's particle range and its field values
     int done = 0;
     double co = 0.0, cd = 0.0;
+    int iters_now = 0;
     if (SPEC) {
         done = a.it->done;
+        iters_now = a.it->iters;
         conv_load(a.conv_prev, a.nblk, co, cd);
     }
     const int p0 = a.cs[cc], p1 = a.cs[cc + 1];
@@ -458,16 +463,21 @@ __global__ __launch_bounds__(64 * kDepWaves) void k_amjdeposit(JDepArgs a)
         fv[u] = f[((size_t)comp * nm + k) * ld + ix + n];
     }
     if (SPEC) {
-        if (done) return;
-        // 'compare' of the previous iteration (simulation_class.f03:396):
-        // every wave evaluates it (the same decision everywhere);
-        // workgroup 0 records it
-        double rel, ab;
-        conv_finish(co, cd, rel, ab);
-        if (rel < a.reltol || ab < a.abstol) {
-            if (blockIdx.x == 0 && threadIdx.x == 0) a.it->done = 1;
-            return;
+        int stop = done;
+        if (!stop) {
+            // 'compare' of the previous iteration (simulation_class.f03:396):
+            // every wave evaluates it (the same decision everywhere);
+            // workgroup 0 records it
+            double rel, ab;
+            conv_finish(co, cd, rel, ab);
+            stop = rel < a.reltol || ab < a.abstol;
+            if (stop && blockIdx.x == 0 && threadIdx.x == 0) a.it->done = 1;
         }
+        if (a.report && blockIdx.x == 0 && threadIdx.x == 0) {
+            a.report->done = stop;
+            a.report->iters = iters_now;
+        }
+        if (stop) return;
     }
 #pragma unroll
     for (int u = 0; u < (NF + 63) / 64; u++) {
@@ -516,7 +526,7 @@ __global__ __launch_bounds__(64 * kDepWaves) void k_amjdeposit(JDepArgs a)
 void time_begin(qpad_ctx *c, int which, hipEvent_t *stop);
 void time_end(qpad_ctx *c, int which, hipEvent_t stop);
 
-int launch_amjdeposit(qpad_ctx *c, int s, int l, bool store_diag, bool timed)
+int launch_amjdeposit(qpad_ctx *c, int s, int l, bool store_diag, bool timed, bool report)
 {
     const bool speculative = l >= 2;
     Species &sp = c->sp[s];
@@ -542,6 +552,7 @@ int launch_amjdeposit(qpad_ctx *c, int s, int l, bool store_diag, bool timed)
     a.conv_prev = c->conv_blk + (size_t)((l - 1) & 1) * 2 * a.nblk;
     a.reltol = c->g.reltol;
     a.abstol = c->g.abstol;
+    a.report = (report && speculative) ? c->it_report : nullptr;
     dim3 grid((c->g.nr + kDepWaves - 1) / kDepWaves), blk(64 * kDepWaves);
     hipEvent_t stop = nullptr;
     if (timed) time_begin(c, 0, &stop);

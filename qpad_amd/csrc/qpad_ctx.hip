@@ -307,6 +307,18 @@ int qpad_create(const qpad_grid *gr, qpad_ctx **out)
         he = hipHostMalloc((void **)&c->it_host, sizeof(IterState), hipHostMallocDefault);
         if (he != hipSuccess) rc = fail(QPAD_E_NOMEM, "hipHostMalloc failed");
     }
+    if (!rc) {
+        he = hipHostMalloc((void **)&c->it_report, sizeof(IterState), hipHostMallocCoherent);
+        if (he != hipSuccess) rc = fail(QPAD_E_NOMEM, "hipHostMalloc (coherent) failed");
+        else memset(c->it_report, 0, sizeof(IterState));
+    }
+    if (!rc) {
+        he = hipEventCreateWithFlags(&c->dec_ev, hipEventDisableTiming);
+        if (he != hipSuccess) rc = fail(QPAD_E_HIP, "hipEventCreate failed");
+    }
+    c->adaptive = 1;
+    c->spec_margin = 0;
+    c->spec_guess = 2;
     if (rc) {
         qpad_destroy(c);
         return rc;
@@ -363,6 +375,8 @@ int qpad_destroy(qpad_ctx *c)
     dfree(c->h_cu); dfree(c->h_q_spe); dfree(c->h_q_beam); dfree(c->h_b_beam);
     dfree(c->conv_re); dfree(c->conv_im); dfree(c->conv_blk); dfree(c->it); dfree(c->fac); dfree(c->src);
     if (c->it_host) (void)hipHostFree(c->it_host);
+    if (c->it_report) (void)hipHostFree(c->it_report);
+    if (c->dec_ev) (void)hipEventDestroy(c->dec_ev);
     for (int s = 0; s < c->nspecies; s++) {
         Species &sp = c->sp[s];
         dfree(sp.x1); dfree(sp.x2); dfree(sp.p1); dfree(sp.p2); dfree(sp.p3); dfree(sp.q);
@@ -505,6 +519,7 @@ int qpad_step3d_begin(qpad_ctx *c)
                       QPAD_F_AMU};
     for (int f : zf) TRY(zero_dev(c, c->f[f], f1_size(c, c->ncomp[f])));
     TRY(seednet_step_begin(c));
+    c->spec_guess = 2;
     c->step_open = true;
     return sync(c);
 }
@@ -524,17 +539,48 @@ static int slice_enqueue(qpad_ctx *c, int j)
     TRY(launch_slice_sources(c));
     TRY(seednet_slice_start(c, j, false));
     TRY(launch_slice_solves(c, j));
-    // speculative iterations: each kernel returns at once after convergence.
+    // predictor-corrector iterations, queued without waiting for the device.
+    // Each iteration's kernels return at once once the device-side test has
+    // set IterState::done, so queuing more iterations than needed changes no
+    // result, only costs launches.  Adaptive mode queues as many iterations
+    // as the previous slice needed (plus the margin); after that, the deposit
+    // of each further iteration (which decides whether the one before it
+    // converged) reports the decision to the host, with that iteration's
+    // field kernels already queued behind it, and the host stops queuing at
+    // the first converged report.  Fixed mode queues all iter_max iterations.
     // Timing events (when enabled) bracket only the first iteration, which
     // always does work: an event per launch would cost a quarter of the step.
-    for (int l = 1; l <= c->g.iter_max; l++) {
-        const bool first = l == 1;
-        for (int s = 0; s < c->nspecies; s++) TRY(launch_amjdeposit(c, s, l, true, first));
-        hipEvent_t stop = nullptr;
-        if (first) time_begin(c, 2, &stop);
+    const int imax = c->g.iter_max;
+    // (at most imax - 1 up front, so that the deposit of iteration imax still
+    // reports and the next slice's guess keeps following the iterations)
+    int upto = c->adaptive ? std::max(1, std::min(imax - 1, c->spec_guess + c->spec_margin)) : imax;
+    int queued = 0, known_iters = imax;
+    for (;;) {
+        for (int l = queued + 1; l <= upto; l++) {
+            const bool first = l == 1;
+            for (int s = 0; s < c->nspecies; s++) TRY(launch_amjdeposit(c, s, l, true, first));
+            hipEvent_t stop = nullptr;
+            if (first) time_begin(c, 2, &stop);
+            TRY(launch_iteration_fields(c, j, l));
+            if (first) time_end(c, 2, stop);
+        }
+        queued = upto;
+        if (queued >= imax) break;
+        // iteration queued + 1: its deposit decides on iteration `queued`
+        const int l = queued + 1;
+        TRY(launch_amjdeposit(c, 0, l, true, false, true));
+        QPAD_HIP_CHECK(hipEventRecord(c->dec_ev, c->stream));
+        for (int s = 1; s < c->nspecies; s++) TRY(launch_amjdeposit(c, s, l, true, false));
         TRY(launch_iteration_fields(c, j, l));
-        if (first) time_end(c, 2, stop);
+        queued = l;
+        QPAD_HIP_CHECK(hipEventSynchronize(c->dec_ev));
+        if (c->it_report->done) {
+            known_iters = c->it_report->iters;
+            break;
+        }
+        upto = queued;
     }
+    if (c->adaptive) c->spec_guess = known_iters;
     // slice epilogue: cbq, e_spe, history (simulation_class.f03:401-456)
     TRY(launch_epilogue(c, j));
     TRY(seednet_slice_end(c, j));
@@ -588,6 +634,15 @@ int qpad_step3d(qpad_ctx *c, int32_t *total_iters_out)
     const int32_t tot = c->it_host->total_iters;
     TRY(qpad_step3d_end(c));
     if (total_iters_out) *total_iters_out = tot;
+    return 0;
+}
+
+int qpad_iteration_policy(qpad_ctx *c, int32_t adaptive, int32_t margin)
+{
+    if (!c) return fail(QPAD_E_INVALID, "null context");
+    if (margin < -c->g.iter_max || margin > c->g.iter_max) return fail(QPAD_E_INVALID, "margin out of range");
+    c->adaptive = adaptive != 0;
+    c->spec_margin = margin;
     return 0;
 }
 

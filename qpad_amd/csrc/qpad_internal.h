@@ -104,6 +104,12 @@ struct qpad_ctx {
     double *conv_re, *conv_im, *conv_blk;
     qpad::IterState *it;      // device
     qpad::IterState *it_host; // pinned host mirror
+    // adaptive speculation: the deposit that decides convergence writes
+    // (done, iterations) into it_report (coherent pinned host memory); the
+    // host waits on dec_ev recorded after it
+    qpad::IterState *it_report;
+    hipEvent_t dec_ev;
+    int adaptive, spec_margin, spec_guess;
     int nspecies, nbeams;
     qpad::Species sp[qpad::kMaxSpecies];
     qpad::Beam bm[qpad::kMaxBeams];

@@ -366,6 +366,13 @@ module qpad_hip_mod
       integer(c_int) :: rc
     end function
 
+    function qpad_iteration_policy(ctx, adaptive, margin) bind(C, name="qpad_iteration_policy") result(rc)
+      import :: c_int, c_int32_t, c_ptr
+      type(c_ptr), value :: ctx
+      integer(c_int32_t), value :: adaptive, margin
+      integer(c_int) :: rc
+    end function
+
     function qpad_seednet_create(ctx, cfg) bind(C, name="qpad_seednet_create") result(rc)
       import :: c_int, c_ptr, qpad_seednet_cfg
       type(c_ptr), value :: ctx
@@ -446,7 +453,7 @@ module qpad_hip_mod
     qpad_get_field, qpad_set_field, qpad_get_hist, qpad_set_hist, &
     qpad_species_count, qpad_get_species, qpad_set_species, &
     qpad_get_species_qn, qpad_set_species_qn, qpad_beam_count, qpad_get_beam, &
-    qpad_set_beam, qpad_kernel_time, qpad_kernel_time_reset, &
+    qpad_set_beam, qpad_kernel_time, qpad_kernel_time_reset, qpad_iteration_policy, &
     qpad_seednet_create, qpad_seednet_get_info, qpad_seednet_mode, qpad_seednet_get_params, &
     qpad_seednet_set_params, qpad_seednet_losses, qpad_seednet_seed, qpad_seednet_learn, &
     qpad_seednet_eval

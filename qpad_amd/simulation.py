@@ -298,6 +298,11 @@ class Simulation:
             total += self.step3d()
         return total
 
+    def iteration_policy(self, adaptive=True, margin=0):
+        """How the fused slice queues its predictor-corrector iterations
+        (results do not depend on it; include/qpad_hip.h)."""
+        L.check(self.lib.qpad_iteration_policy(self.ctx, int(bool(adaptive)), int(margin)))
+
     def convergence_tester(self, op):
         """'record' or 'compare' on b_spe component 2 (simulation_class.f03:522-606)."""
         rel, ab = C.c_double(0.0), C.c_double(0.0)
