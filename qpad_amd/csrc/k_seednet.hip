@@ -20,6 +20,7 @@
 // positions in a fixed order, so training is deterministic.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstring>
 #include <string>
@@ -57,6 +58,9 @@ struct SeedNet {
     float *z[kNetMaxLayers];    // pre-activation of layer l: [npos_pad][cout_p]
     __bf16 *dz[kNetMaxLayers];  // d loss / d z of layer l: [npos_pad + 2][cout_p]
     double *scales;         // B_perp, J_perp, q_spe, q_beam maxima of the slice
+    double *scale_part;     // per-workgroup maxima [nscale_blk][4]
+    int nscale_blk;
+    int infer_stride;       // BF16 per LDS row of the fused inference
     double *bprev;          // b_spe components 0, 1 at the last two slice starts
     float *loss_part, *loss_log;
     float *xin, *tin;       // host inputs of qpad_seednet_eval
@@ -66,7 +70,9 @@ struct SeedNet {
 // ---------------------------------------------------------------------------
 // device helpers
 // ---------------------------------------------------------------------------
-__device__ __forceinline__ float net_sigmoid(float z) { return 1.0f / (1.0f + expf(-z)); }
+// logistic function for SiLU: fast exp and reciprocal (relative error near
+// 1e-7, far below the BF16 rounding of the activations it feeds)
+__device__ __forceinline__ float net_sigmoid(float z) { return __builtin_amdgcn_rcpf(1.0f + __expf(-z)); }
 
 __device__ __forceinline__ f32x4 mfma_bf16(bf16x8 a, bf16x8 b, f32x4 c)
 {
@@ -80,18 +86,25 @@ __device__ __forceinline__ f32x4 mfma_bf16(bf16x8 a, bf16x8 b, f32x4 c)
 // ---------------------------------------------------------------------------
 struct NetScaleArgs {
     const double *b_spe, *cu, *q_spe, *q_beam;
-    double *scales;
+    double *bsave;         // copy of b_spe components 0, 1 (the seed is
+                           // written into b_spe while neighbours still read)
+    double *part;          // per-workgroup maxima [gridDim.x][4]
     int nr, nm, ld;
 };
 
-__global__ __launch_bounds__(1024) void k_net_scale(NetScaleArgs a)
+// maxima over nodes 1..nr and every mode array of |B_r|,|B_phi|; |J_r|,
+// |J_phi|; |q_spe|; |q_beam| (per workgroup) and the saved copy of B_perp
+__global__ __launch_bounds__(256) void k_net_scale(NetScaleArgs a)
 {
-    __shared__ double red[4][32];
+    __shared__ double red[4][4];
     const int nr = a.nr, nm = a.nm, ld = a.ld;
     double s[4] = {0.0, 0.0, 0.0, 0.0};
-    for (int idx = threadIdx.x; idx < nm * nr; idx += blockDim.x) {
+    for (int idx = blockIdx.x * blockDim.x + threadIdx.x; idx < nm * nr; idx += gridDim.x * blockDim.x) {
         const int k = idx / nr, i = idx % nr + 1;
-        s[0] = fmax(s[0], fmax(fabs(NX(a.b_spe, 0, k, i)), fabs(NX(a.b_spe, 1, k, i))));
+        const double b0 = NX(a.b_spe, 0, k, i), b1 = NX(a.b_spe, 1, k, i);
+        NX(a.bsave, 0, k, i) = b0;
+        NX(a.bsave, 1, k, i) = b1;
+        s[0] = fmax(s[0], fmax(fabs(b0), fabs(b1)));
         s[1] = fmax(s[1], fmax(fabs(NX(a.cu, 0, k, i)), fabs(NX(a.cu, 1, k, i))));
         s[2] = fmax(s[2], fabs(NX(a.q_spe, 0, k, i)));
         s[3] = fmax(s[3], fabs(NX(a.q_beam, 0, k, i)));
@@ -99,66 +112,307 @@ __global__ __launch_bounds__(1024) void k_net_scale(NetScaleArgs a)
 #pragma unroll
     for (int q = 0; q < 4; q++)
         for (int o = 32; o > 0; o >>= 1) s[q] = fmax(s[q], __shfl_xor(s[q], o, 64));
-    const int w = threadIdx.x >> 6, nw = blockDim.x >> 6;
+    const int w = threadIdx.x >> 6;
     if ((threadIdx.x & 63) == 0)
         for (int q = 0; q < 4; q++) red[q][w] = s[q];
     __syncthreads();
     if (threadIdx.x < 4) {
         double m = 0.0;
-        for (int v = 0; v < nw; v++) m = fmax(m, red[threadIdx.x][v]);
-        // an all-zero quantity (ahead of the beam) keeps the inputs at zero
-        a.scales[threadIdx.x] = (m > 0.0 && m < INFINITY) ? m : 1.0;
+        for (int v = 0; v < 4; v++) m = fmax(m, red[threadIdx.x][v]);
+        a.part[blockIdx.x * 4 + threadIdx.x] = m;
     }
 }
 
+// the slice's scales from the per-workgroup maxima, into LDS (threads 0..3;
+// the caller synchronises); workgroup 0 also stores them for later kernels.
+// An all-zero quantity (ahead of the beam) keeps its inputs at zero.
+__device__ __forceinline__ void net_final_scales(const double *part, int nsp, double *out, double *ssc)
+{
+    // wave 0: one workgroup's maxima per lane (nsp <= 64), then a wave max
+    if (threadIdx.x < 64) {
+        const int l = threadIdx.x;
+        const int g = min(l, nsp - 1);
+        double v[4];
+#pragma unroll
+        for (int q = 0; q < 4; q++) v[q] = l < nsp ? part[g * 4 + q] : 0.0;
+#pragma unroll
+        for (int q = 0; q < 4; q++)
+            for (int o = 32; o > 0; o >>= 1) v[q] = fmax(v[q], __shfl_xor(v[q], o, 64));
+        if (l < 4) {
+            const double m = l == 0 ? v[0] : (l == 1 ? v[1] : (l == 2 ? v[2] : v[3]));
+            const double x = (m > 0.0 && m < INFINITY) ? m : 1.0;
+            ssc[l] = x;
+            if (blockIdx.x == 0) out[l] = x;
+        }
+    }
+}
+
+// input channel ch of node i: qty * nm + k over the 8 quantities, then
+// r / r_max.  Branch-free (selects, one or two unconditional loads) so that
+// a thread's channels issue their loads together.
+__device__ __forceinline__ float net_input(int ch, int qc, int kc, int i, int nm, int ld, const double *b1,
+                                           const double *b2, const double *cu, const double *q_spe,
+                                           const double *q_beam, const double *inv, float rinv)
+{
+    // qc, kc: ch / nm and ch % nm
+    const bool field = ch < kNetQty * nm;
+    const int qty = field ? qc : 0, k = field ? kc : 0;
+    // qty: 0, 1 B_r, B_phi; 2, 3 their change; 4, 5 J_r, J_phi; 6 q_spe; 7 q_beam
+    const int comp = qty & 1 & (qty < 6);
+    const double *arr = qty < 4 ? b1 : (qty < 6 ? cu : (qty == 6 ? q_spe : q_beam));
+    const int sidx = qty < 4 ? 0 : (qty < 6 ? 1 : qty - 4);
+    const size_t off = ((size_t)comp * nm + (size_t)k) * ld + (size_t)i;
+    const double xa = arr[off];
+    const double xb = b2[off];
+    const double x = (xa - ((qty == 2 || qty == 3) ? xb : 0.0)) * inv[sidx];
+    return field ? (float)x : (ch == kNetQty * nm ? (float)(i - 1) * rinv : 0.0f);
+}
+
 struct NetPackArgs {
-    const double *b_spe, *cu, *q_spe, *q_beam;
-    const double *bold;    // b_spe components 0, 1 at the previous slice start
-    double *bsave;         // this slice's copy
-    const double *scales;
+    const double *bsave;   // B_perp of slice xi-1 (saved by k_net_scale)
+    const double *bold;    // B_perp of slice xi-2 (saved at the previous slice start)
+    const double *cu, *q_spe, *q_beam;
+    const double *part;
+    int nsp;
+    double *scales;
     __bf16 *h0;
     int nr, nm, ld, cin, cin_p;
     float rinv;            // 1 / nr: the radius feature (i - 1) / nr = r / r_max
 };
 
-// one thread per (group of 8 channels, node): channel c = qty * nm + k
+// one thread per (group of 8 channels, node)
 __global__ __launch_bounds__(256) void k_net_pack(NetPackArgs a)
 {
+    __shared__ double ssc[4];
+    net_final_scales(a.part, a.nsp, a.scales, ssc);
+    __syncthreads();
     const int nr = a.nr, nm = a.nm, ld = a.ld;
     const int ng = a.cin_p >> 3;
     const int idx = blockIdx.x * blockDim.x + threadIdx.x;
     if (idx >= ng * nr) return;
     const int g = idx / nr, pos = idx % nr, i = pos + 1;
-    const double iB = 1.0 / a.scales[0], iJ = 1.0 / a.scales[1], iq = 1.0 / a.scales[2],
-                 iqb = 1.0 / a.scales[3];
+    const double inv[4] = {1.0 / ssc[0], 1.0 / ssc[1], 1.0 / ssc[2], 1.0 / ssc[3]};
     bf16x8 out;
+    int qc = (8 * g) / nm, kc = (8 * g) % nm;
 #pragma unroll
     for (int e = 0; e < 8; e++) {
-        const int ch = 8 * g + e;
-        float v = 0.0f;
-        if (ch < kNetQty * nm) {
-            const int qty = ch / nm, k = ch % nm;
-            double x;
-            switch (qty) {
-            case 0: x = NX(a.b_spe, 0, k, i) * iB; break;
-            case 1: x = NX(a.b_spe, 1, k, i) * iB; break;
-            case 2: x = (NX(a.b_spe, 0, k, i) - NX(a.bold, 0, k, i)) * iB; break;
-            case 3: x = (NX(a.b_spe, 1, k, i) - NX(a.bold, 1, k, i)) * iB; break;
-            case 4: x = NX(a.cu, 0, k, i) * iJ; break;
-            case 5: x = NX(a.cu, 1, k, i) * iJ; break;
-            case 6: x = NX(a.q_spe, 0, k, i) * iq; break;
-            default: x = NX(a.q_beam, 0, k, i) * iqb; break;
-            }
-            v = (float)x;
-        } else if (ch == kNetQty * nm) {
-            v = (float)(i - 1) * a.rinv;
+        out[e] = (__bf16)net_input(8 * g + e, qc, kc, i, nm, ld, a.bsave, a.bold, a.cu, a.q_spe, a.q_beam, inv,
+                                   a.rinv);
+        if (++kc == nm) {
+            kc = 0;
+            qc++;
         }
-        out[e] = (__bf16)v;
     }
     *(bf16x8 *)(a.h0 + (size_t)i * a.cin_p + 8 * g) = out;
-    if (g == 0) {
-        for (int c = 0; c < 2; c++)
-            for (int k = 0; k < nm; k++) NX(a.bsave, c, k, i) = NX(a.b_spe, c, k, i);
+}
+
+// ---------------------------------------------------------------------------
+// fused inference of one slice (seed mode): inputs, every layer and the seed
+// in one launch.  A workgroup owns kInferT nodes; it recomputes a halo of
+// one node per remaining layer on each side, so the layers run from LDS
+// without a grid-wide step between them.  LDS row r holds node position
+// p0 - L + r; rows outside 0 <= p < nr stay zero (the convolution's
+// padding) at every layer.  The same products in the same order as the
+// layer-by-layer kernels: the outputs are identical.  Everything the
+// workgroup reads from memory (all layers' weights, the input fields, the
+// FP64 B_perp values of its own nodes for the seed) is requested up front,
+// so the launch costs about two memory round trips.  With `store`, the
+// owned rows' layer inputs and pre-activations go to global memory for the
+// training step.
+//
+// Dynamic LDS: [2 (B(xi-1), B(xi-2))][2 comp][nm][kInferT] doubles, then two
+// activation buffers [kInferRows][stride] and every layer's weights
+// (BF16, the global layout).
+// ---------------------------------------------------------------------------
+constexpr int kInferT = 32;
+constexpr int kInferThreads = 256;
+constexpr int kInferRows = 64;          // >= kInferT + 2 L + 16 (tile overhang), L <= 8
+constexpr int kInferMaxW = 49152;       // BF16 weights staged in LDS (96 KB)
+constexpr int kInferWVec = kInferMaxW / 8 / kInferThreads;   // 16-byte loads per thread
+constexpr int kInferItems = 2;          // input (row, channel group) items per thread
+constexpr int kInferMaxLds = 160 * 1024;
+
+struct NetInferArgs {
+    const double *bsave, *bold, *cu, *q_spe, *q_beam;
+    const double *part;
+    int nsp;
+    double *scales;
+    double *b_spe;         // seed: components 0, 1 of nodes 1..nr
+    double oscale;
+    int nr, nm, ld, cin, layers, cout, store, stride, nwb;
+    float rinv;
+    const __bf16 *wb;      // every layer's BF16 weights
+    long long wb_off[kNetMaxLayers];
+    const float *bias[kNetMaxLayers];
+    int cin_p[kNetMaxLayers], cout_p[kNetMaxLayers];
+    __bf16 *h[kNetMaxLayers];
+    float *z[kNetMaxLayers];
+};
+
+inline size_t infer_lds_bytes(int nm, int stride, int nwb)
+{
+    return sizeof(double) * 4 * nm * kInferT + sizeof(__bf16) * (2 * kInferRows * stride + (size_t)nwb);
+}
+
+__global__ __launch_bounds__(kInferThreads) void k_net_infer(NetInferArgs a)
+{
+    extern __shared__ double lds_d[];
+    __shared__ double ssc[4];
+    const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63, j = lane & 15, q = lane >> 4;
+    const int L = a.layers, T = kInferT, nr = a.nr, nm = a.nm, ld = a.ld, S = a.stride;
+    const int p0 = blockIdx.x * T, pbase = p0 - L;
+    double *bcache = lds_d;                                   // [2][2][nm][T]
+    __bf16 *buf0 = (__bf16 *)(lds_d + 4 * nm * T);
+    __bf16 *bufs[2] = {buf0, buf0 + kInferRows * S};
+    __bf16 *wl = buf0 + 2 * kInferRows * S;
+    // 1. requests: weights, input fields of every (row, channel group) item,
+    //    the FP64 B_perp of the owned rows, the scales' partial maxima
+    bf16x8 wv[kInferWVec];
+    const int nwv = a.nwb >> 3;
+#pragma unroll
+    for (int u = 0; u < kInferWVec; u++) {
+        const int v = tid + u * kInferThreads;
+        if (v < nwv) wv[u] = ((const bf16x8 *)a.wb)[v];
+    }
+    const int ng = a.cin_p[0] >> 3, nrow = T + 2 * L;
+    double xa[kInferItems][8], xb[kInferItems][8];
+    int sidx[kInferItems][8];
+    unsigned kind[kInferItems];   // per channel 2 bits: 0 zero, 1 field, 2 radius
+#pragma unroll
+    for (int it = 0; it < kInferItems; it++) {
+        const int idx = tid + it * kInferThreads;
+        const int g = idx / nrow, r = idx % nrow, p = pbase + r;
+        const bool use = idx < nrow * ng && p >= 0 && p < nr;
+        const int i = use ? p + 1 : 1;
+        kind[it] = 0;
+        // (quantity, mode array) of channel 8 g, then stepped per channel
+        int qc = (8 * g) / nm, kc = (8 * g) % nm;
+#pragma unroll
+        for (int e = 0; e < 8; e++) {
+            const int ch = 8 * g + e;
+            const bool field = use && ch < kNetQty * nm;
+            const int qty = field ? qc : 0, k = field ? kc : 0;
+            if (++kc == nm) {
+                kc = 0;
+                qc++;
+            }
+            const int comp = qty & 1 & (qty < 6);
+            const double *arr = qty < 4 ? a.bsave : (qty < 6 ? a.cu : (qty == 6 ? a.q_spe : a.q_beam));
+            const int off = (comp * nm + k) * ld + i;
+            xa[it][e] = arr[off];
+            xb[it][e] = (qty == 2 || qty == 3) ? a.bold[off] : 0.0;
+            sidx[it][e] = qty < 4 ? 0 : (qty < 6 ? 1 : qty - 4);
+            kind[it] |= (field ? 1u : (use && ch == kNetQty * nm ? 2u : 0u)) << (2 * e);
+        }
+    }
+    static_assert((kInferT & (kInferT - 1)) == 0, "kInferT must be a power of two");
+    for (int v = tid; v < 4 * nm * T; v += kInferThreads) {
+        // v = (which 2 nm + c nm + k) T + r; c nm + k is the row of component c, mode array k
+        const int r = v & (T - 1), row = v / T, which = row >= 2 * nm, crow = which ? row - 2 * nm : row;
+        const int p = p0 + r;
+        const double *src = which ? a.bold : a.bsave;
+        bcache[v] = p < nr ? src[crow * ld + p + 1] : 0.0;
+    }
+    net_final_scales(a.part, a.nsp, a.scales, ssc);
+    // 2. LDS: weights, zeroed activations
+#pragma unroll
+    for (int u = 0; u < kInferWVec; u++) {
+        const int v = tid + u * kInferThreads;
+        if (v < nwv) ((bf16x8 *)wl)[v] = wv[u];
+    }
+    {
+        bf16x8 zero;
+#pragma unroll
+        for (int e = 0; e < 8; e++) zero[e] = (__bf16)0.0f;
+        for (int v = tid; v < 2 * kInferRows * S / 8; v += kInferThreads) ((bf16x8 *)buf0)[v] = zero;
+    }
+    __syncthreads();
+    // 3. the inputs, scaled
+    {
+        const double inv[4] = {1.0 / ssc[0], 1.0 / ssc[1], 1.0 / ssc[2], 1.0 / ssc[3]};
+#pragma unroll
+        for (int it = 0; it < kInferItems; it++) {
+            const int idx = tid + it * kInferThreads;
+            const int g = idx / nrow, r = idx % nrow, p = pbase + r;
+            if (idx >= nrow * ng || p < 0 || p >= nr) continue;
+            bf16x8 v;
+#pragma unroll
+            for (int e = 0; e < 8; e++) {
+                const unsigned kd = (kind[it] >> (2 * e)) & 3u;
+                const float x = kd == 1u ? (float)((xa[it][e] - xb[it][e]) * inv[sidx[it][e]])
+                                         : (kd == 2u ? (float)p * a.rinv : 0.0f);
+                v[e] = (__bf16)x;
+            }
+            *(bf16x8 *)(bufs[0] + r * S + 8 * g) = v;
+            if (a.store && r >= L && r < L + T) *(bf16x8 *)(a.h[0] + (size_t)(p + 1) * a.cin_p[0] + 8 * g) = v;
+        }
+    }
+    __syncthreads();
+    // 4. the layers
+    for (int l = 0; l < L; l++) {
+        const __bf16 *in = bufs[l & 1];
+        __bf16 *out = bufs[(l + 1) & 1];
+        const int cin_p = a.cin_p[l], cout_p = a.cout_p[l], nt = cout_p >> 4, nkb = cin_p >> 5;
+        const bool last = l == L - 1;
+        // rows l + 1 .. T + 2L - l - 2 are exact at this layer
+        const int r_lo = l + 1, nrt = (T + 2 * L - 2 * l - 2 + 15) >> 4;
+        // each wave owns output tile t (nt = 2 or 4 tiles: 4 or 2 waves per
+        // tile) with its weights, every tap and k-block, in registers
+        const int t = wave % nt, wstep = (kInferThreads / 64) / nt;
+        const __bf16 *W = wl + a.wb_off[l];
+        bf16x8 av[3][kNetMaxCinP / 32];
+#pragma unroll
+        for (int tap = 0; tap < 3; tap++) {
+            const __bf16 *wrow = W + (tap * cout_p + 16 * t + j) * cin_p + 8 * q;
+#pragma unroll
+            for (int kb = 0; kb < kNetMaxCinP / 32; kb++)
+                if (kb < nkb) av[tap][kb] = *(const bf16x8 *)(wrow + 32 * kb);
+        }
+        f32x4 bv4;
+#pragma unroll
+        for (int rr = 0; rr < 4; rr++) bv4[rr] = a.bias[l][16 * t + 4 * q + rr];
+        for (int rt = wave / nt; rt < nrt; rt += wstep) {
+            const int r0 = r_lo + 16 * rt;
+            f32x4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+            for (int tap = 0; tap < 3; tap++) {
+                const __bf16 *hrow = in + (r0 + j + tap - 1) * S + 8 * q;
+#pragma unroll
+                for (int kb = 0; kb < kNetMaxCinP / 32; kb++)
+                    if (kb < nkb) acc = mfma_bf16(av[tap][kb], *(const bf16x8 *)(hrow + 32 * kb), acc);
+            }
+            const int r = r0 + j, p = pbase + r, co0 = 16 * t + 4 * q;
+            const bool inside = p >= 0 && p < nr;
+            const bool own = inside && r >= L && r < L + T;
+            f32x4 zz;
+#pragma unroll
+            for (int rr = 0; rr < 4; rr++) zz[rr] = acc[rr] + bv4[rr];
+            if (last && own) {
+                if (a.store) *(f32x4 *)(a.z[l] + (size_t)p * cout_p + co0) = zz;
+                const double sc = ssc[0] * a.oscale;
+                const int rl = r - L;
+#pragma unroll
+                for (int rr = 0; rr < 4; rr++) {
+                    // output channel co = cc nm + k: B_perp component cc, mode array k
+                    const int co = co0 + rr;
+                    if (co < a.cout) {
+                        const double b = bcache[co * T + rl];
+                        const double bo = bcache[(2 * nm + co) * T + rl];
+                        a.b_spe[(size_t)co * ld + p + 1] = b + (b - bo) + (double)zz[rr] * sc;
+                    }
+                }
+            } else if (!last) {
+                bf16x4 hv;
+#pragma unroll
+                for (int rr = 0; rr < 4; rr++) hv[rr] = inside ? (__bf16)(zz[rr] * net_sigmoid(zz[rr])) : (__bf16)0.0f;
+                *(bf16x4 *)(out + r * S + co0) = hv;
+                if (a.store && own) {
+                    *(f32x4 *)(a.z[l] + (size_t)p * cout_p + co0) = zz;
+                    *(bf16x4 *)(a.h[l + 1] + (size_t)(p + 1) * cout_p + co0) = hv;
+                }
+            }
+        }
+        __syncthreads();
     }
 }
 
@@ -181,13 +435,6 @@ struct NetFwdArgs {
     float *z;             // [npos_pad][cout_p]
     __bf16 *h_out;        // [npos_pad + 2][cout_p] (hidden layers)
     int cin_p, cout_p, npos;
-    // last layer in seed mode: b_spe components 0, 1 become the first-order
-    // extrapolation 2 b_spe - bold plus y * scale
-    double *b_spe;
-    const double *bold;
-    const double *scales;
-    double oscale;         // correction units: oscale * B_perp scale
-    int nm, ld, cout;
 };
 
 template <int NT, bool LAST>
@@ -220,21 +467,7 @@ __global__ __launch_bounds__(64) void k_net_fwd(NetFwdArgs a)
 #pragma unroll
         for (int r = 0; r < 4; r++) zz[r] = acc[t][r] + a.bias[co0 + r];
         *(f32x4 *)(a.z + (size_t)pos * cout_p + co0) = zz;
-        if (LAST) {
-            if (a.b_spe) {
-                const int nm = a.nm, ld = a.ld;
-                const double s = a.scales[0] * a.oscale;
-#pragma unroll
-                for (int r = 0; r < 4; r++) {
-                    const int co = co0 + r;
-                    if (co < a.cout) {
-                        const int cc = co / nm, k = co % nm;
-                        const double b = NX(a.b_spe, cc, k, pos + 1);
-                        NX(a.b_spe, cc, k, pos + 1) = b + (b - NX(a.bold, cc, k, pos + 1)) + (double)zz[r] * s;
-                    }
-                }
-            }
-        } else {
+        if (!LAST) {
             bf16x4 hv;
 #pragma unroll
             for (int r = 0; r < 4; r++) hv[r] = (__bf16)(zz[r] * net_sigmoid(zz[r]));
@@ -427,17 +660,32 @@ struct NetAdamArgs {
 
 __global__ __launch_bounds__(256) void k_net_adam(NetAdamArgs a)
 {
+    __shared__ float red[256];
     const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i == 0 && a.loss_slot) {
+    if (blockIdx.x == 0 && a.loss_slot) {
+        // sum of the loss partials: per-thread strided sums, then a tree
         float s = 0.0f;
-        for (int b = 0; b < a.nloss; b++) s += a.loss_part[b];
-        *a.loss_slot = s * a.loss_norm;
+        for (int b = threadIdx.x; b < a.nloss; b += 256) s += a.loss_part[b];
+        red[threadIdx.x] = s;
+        __syncthreads();
+        for (int w = 128; w > 0; w >>= 1) {
+            if (threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+            __syncthreads();
+        }
+        if (threadIdx.x == 0) *a.loss_slot = red[0] * a.loss_norm;
     }
     if (i >= a.nparams) return;
     float th = a.theta[i];
     if (a.flags & (kAdamGrad | kAdamUpdate)) {
+        // chunks in order, loads issued eight at a time
         float g = 0.0f;
-        for (int c = 0; c < a.nchunk; c++) g += a.part[(size_t)c * a.nparams + i];
+        for (int c0 = 0; c0 < a.nchunk; c0 += 8) {
+            float v[8];
+#pragma unroll
+            for (int u = 0; u < 8; u++) v[u] = c0 + u < a.nchunk ? a.part[(size_t)(c0 + u) * a.nparams + i] : 0.0f;
+#pragma unroll
+            for (int u = 0; u < 8; u++) g += v[u];
+        }
         if (a.flags & kAdamGrad) a.grad_out[i] = g;
         if (a.flags & kAdamUpdate) {
             const float mm = a.b1 * a.m[i] + (1.0f - a.b1) * g;
@@ -501,7 +749,7 @@ void seednet_destroy(qpad_ctx *c)
     net_free(n->theta); net_free(n->am); net_free(n->av); net_free(n->part); net_free(n->grad);
     net_free(n->wb); net_free(n->wtb);
     for (int l = 0; l < kNetMaxLayers; l++) { net_free(n->h[l]); net_free(n->z[l]); net_free(n->dz[l]); }
-    net_free(n->scales); net_free(n->bprev); net_free(n->loss_part); net_free(n->loss_log);
+    net_free(n->scales); net_free(n->scale_part); net_free(n->bprev); net_free(n->loss_part); net_free(n->loss_log);
     net_free(n->xin); net_free(n->tin);
     delete n;
     c->net = nullptr;
@@ -582,8 +830,9 @@ static int net_adam_launch(qpad_ctx *c, int flags, float *grad_out, float *loss_
 
 #define NET_FWD(NT, LAST) hipLaunchKernelGGL((k_net_fwd<NT, LAST>), grid, dim3(64), 0, c->stream, a)
 
-// forward pass over npos positions; apply: seed b_spe with the prediction
-static int net_forward(qpad_ctx *c, int npos, bool apply)
+// layer-by-layer forward pass over npos positions (training without the
+// seed, qpad_seednet_eval)
+static int net_forward(qpad_ctx *c, int npos)
 {
     SeedNet &n = *c->net;
     const dim3 grid((unsigned)(round_up(npos, 16) / 16));
@@ -599,15 +848,6 @@ static int net_forward(qpad_ctx *c, int npos, bool apply)
         a.cin_p = n.cin_p[l];
         a.cout_p = n.cout_p[l];
         a.npos = npos;
-        if (last && apply) {
-            a.b_spe = c->f[QPAD_F_B_SPE];
-            a.bold = n.bprev + (size_t)((n.packed_slice - 1) & 1) * 2 * c->g.nm * c->g.ld;
-            a.scales = n.scales;
-            a.oscale = n.cfg.out_scale;
-        }
-        a.nm = c->g.nm;
-        a.ld = c->g.ld;
-        a.cout = n.cout;
         const int nt = n.cout_p[l] / 16;
         if (last) {
             if (nt == 2) NET_FWD(2, true);
@@ -689,7 +929,9 @@ static int net_backward(qpad_ctx *c, int npos, const float *tin, int flags, floa
     return net_adam_launch(c, flags, grad_out, slot, n.nloss_blk, 1.0f / (float)((long long)npos * n.cout));
 }
 
-// scale and pack the inputs of slice j; in seed mode add the prediction to b_spe
+// scales of slice j and the saved B_perp; in seed mode the fused inference
+// seeds b_spe (and keeps the activations when training), otherwise the
+// packed inputs wait for the training step's forward pass
 int seednet_slice_start(qpad_ctx *c, int j, bool force)
 {
     SeedNet *n = c->net;
@@ -697,27 +939,70 @@ int seednet_slice_start(qpad_ctx *c, int j, bool force)
     const int nr = c->g.nr, nm = c->g.nm, ld = c->g.ld;
     const size_t half = (size_t)2 * nm * ld;
     const double *q_beam = c->h_q_beam + (size_t)j * nm * ld;
+    double *bsave = n->bprev + (size_t)(j & 1) * half;
+    const double *bold = n->bprev + (size_t)((j - 1) & 1) * half;
     {
         NetScaleArgs a;
         a.b_spe = c->f[QPAD_F_B_SPE];
         a.cu = c->f[QPAD_F_CU];
         a.q_spe = c->f[QPAD_F_Q_SPE];
         a.q_beam = q_beam;
-        a.scales = n->scales;
+        a.bsave = bsave;
+        a.part = n->scale_part;
         a.nr = nr;
         a.nm = nm;
         a.ld = ld;
-        hipLaunchKernelGGL(k_net_scale, dim3(1), dim3(1024), 0, c->stream, a);
+        hipLaunchKernelGGL(k_net_scale, dim3((unsigned)n->nscale_blk), dim3(256), 0, c->stream, a);
         QPAD_HIP_CHECK(hipGetLastError());
     }
-    {
-        NetPackArgs a;
-        a.b_spe = c->f[QPAD_F_B_SPE];
+    n->packed_slice = j;
+    n->fwd_done = false;
+    if (n->seed_on) {
+        NetInferArgs a;
+        memset(&a, 0, sizeof(a));
+        a.bsave = bsave;
+        a.bold = bold;
         a.cu = c->f[QPAD_F_CU];
         a.q_spe = c->f[QPAD_F_Q_SPE];
         a.q_beam = q_beam;
-        a.bold = n->bprev + (size_t)((j - 1) & 1) * half;
-        a.bsave = n->bprev + (size_t)(j & 1) * half;
+        a.part = n->scale_part;
+        a.nsp = n->nscale_blk;
+        a.scales = n->scales;
+        a.b_spe = c->f[QPAD_F_B_SPE];
+        a.oscale = n->cfg.out_scale;
+        a.nr = nr;
+        a.nm = nm;
+        a.ld = ld;
+        a.cin = n->cin;
+        a.layers = n->layers;
+        a.cout = n->cout;
+        a.store = n->train_on;
+        a.stride = n->infer_stride;
+        a.nwb = (int)n->nwb;
+        a.rinv = 1.0f / (float)nr;
+        a.wb = n->wb;
+        for (int l = 0; l < n->layers; l++) {
+            a.wb_off[l] = n->wb_off[l];
+            a.bias[l] = n->theta + n->b_off[l];
+            a.cin_p[l] = n->cin_p[l];
+            a.cout_p[l] = n->cout_p[l];
+            a.h[l] = n->h[l];
+            a.z[l] = n->z[l];
+        }
+        const size_t lds = infer_lds_bytes(nm, n->infer_stride, (int)n->nwb);
+        hipLaunchKernelGGL(k_net_infer, dim3((unsigned)((nr + kInferT - 1) / kInferT)), dim3(kInferThreads), lds,
+                           c->stream, a);
+        QPAD_HIP_CHECK(hipGetLastError());
+        n->fwd_done = true;
+    } else {
+        NetPackArgs a;
+        a.bsave = bsave;
+        a.bold = bold;
+        a.cu = c->f[QPAD_F_CU];
+        a.q_spe = c->f[QPAD_F_Q_SPE];
+        a.q_beam = q_beam;
+        a.part = n->scale_part;
+        a.nsp = n->nscale_blk;
         a.scales = n->scales;
         a.h0 = n->h[0];
         a.nr = nr;
@@ -730,12 +1015,6 @@ int seednet_slice_start(qpad_ctx *c, int j, bool force)
         hipLaunchKernelGGL(k_net_pack, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, c->stream, a);
         QPAD_HIP_CHECK(hipGetLastError());
     }
-    n->packed_slice = j;
-    n->fwd_done = false;
-    if (n->seed_on) {
-        QPAD_TRY(net_forward(c, nr, true));
-        n->fwd_done = true;
-    }
     return 0;
 }
 
@@ -745,7 +1024,7 @@ int seednet_slice_end(qpad_ctx *c, int j)
     SeedNet *n = c->net;
     if (!n || !n->train_on) return 0;
     if (n->packed_slice != j) return net_fail(QPAD_E_STATE, "seed network: inputs of this slice were not packed");
-    if (!n->fwd_done) QPAD_TRY(net_forward(c, c->g.nr, false));
+    if (!n->fwd_done) QPAD_TRY(net_forward(c, c->g.nr));
     QPAD_TRY(net_backward(c, c->g.nr, nullptr, kAdamUpdate, nullptr));
     n->packed_slice = 0;
     return 0;
@@ -805,6 +1084,24 @@ int qpad_seednet_create(qpad_ctx *c, const qpad_seednet_cfg *cfg)
     }
     n->nparams = off;
     n->nwb = woff;
+    {
+        int cmax = cfg->width;
+        for (int l = 0; l < n->layers; l++) cmax = std::max(cmax, n->cin_p[l]);
+        n->infer_stride = cmax + 8;
+        const size_t lds = infer_lds_bytes(c->g.nm, n->infer_stride, (int)n->nwb);
+        if (n->nwb > kInferMaxW || lds > (size_t)kInferMaxLds ||
+            kInferT + 2 * n->layers + 16 > kInferRows ||
+            (size_t)(kInferT + 2 * n->layers) * (n->cin_p[0] / 8) > (size_t)kInferItems * kInferThreads) {
+            seednet_destroy(c);
+            return net_fail(QPAD_E_INVALID, "seed network too large for the fused inference kernel");
+        }
+        hipError_t e = hipFuncSetAttribute((const void *)k_net_infer, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                           (int)lds);
+        if (e != hipSuccess) {
+            seednet_destroy(c);
+            return net_fail(QPAD_E_HIP, std::string("hipFuncSetAttribute: ") + hipGetErrorString(e));
+        }
+    }
     int rc = 0;
     const size_t rows = (size_t)n->npos_pad + 2;
     if (!rc) rc = net_alloc(&n->theta, (size_t)n->nparams);
@@ -820,6 +1117,8 @@ int qpad_seednet_create(qpad_ctx *c, const qpad_seednet_cfg *cfg)
         if (!rc) rc = net_alloc(&n->dz[l], rows * n->cout_p[l]);
     }
     if (!rc) rc = net_alloc(&n->scales, 4);
+    n->nscale_blk = std::max(1, std::min(64, (c->g.nm * c->g.nr + 255) / 256));
+    if (!rc) rc = net_alloc(&n->scale_part, (size_t)4 * n->nscale_blk);
     if (!rc) rc = net_alloc(&n->bprev, (size_t)4 * c->g.nm * c->g.ld);
     if (!rc) rc = net_alloc(&n->loss_part, (size_t)(rows * 32 / kNetLossThreads + 1));
     if (!rc) rc = net_alloc(&n->loss_log, (size_t)kNetLossLog);
@@ -951,7 +1250,7 @@ int qpad_seednet_eval(qpad_ctx *c, const float *x, const float *t, int32_t npos,
     hipLaunchKernelGGL(k_net_pack_host, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, c->stream, n.xin, n.h[0],
                        npos, n.cin, n.cin_p[0]);
     QPAD_HIP_CHECK(hipGetLastError());
-    QPAD_TRY(net_forward(c, npos, false));
+    QPAD_TRY(net_forward(c, npos));
     if (t) {
         QPAD_HIP_CHECK(
             hipMemcpyAsync(n.tin, t, sizeof(float) * (size_t)npos * n.cout, hipMemcpyHostToDevice, c->stream));

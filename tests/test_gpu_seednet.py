@@ -174,3 +174,56 @@ def test_seeded_iteration_converges_like_plain():
     dpsi = np.max(np.abs(ps - pp)) / np.max(np.abs(pp))
     print(f"iterations plain {sum(its_p)} seeded {sum(its_s)}; B_perp diff {db:.2e}, psi diff {dpsi:.2e}")
     assert db <= 5e-2 and dpsi <= 5e-2
+
+
+def test_slice_training_step_matches_oracle():
+    """Seed and one Adam step at slice j (fused forward keeping the
+    activations, then the layer-by-layer backward): the logged loss and the
+    direction of the update against the oracle on the same fields."""
+    sim = small_sim(max_mode=2, nr=96, nxi=40)
+    lr = 1e-4
+    net = SeedNet(sim, lr=lr, seed=8)
+    shape = shape_of(net)
+    rng = np.random.default_rng(13)
+    theta = shape.random_params(rng, scale=0.5)
+    net.set_params(theta)
+    g = sim.grid
+    f = sim.fields
+    net.mode(seed=False, train=False)
+    sim.step3d_begin()
+    j0 = 28
+    for j in range(1, j0):
+        if j == j0 - 1:
+            b_old = f["b_spe"].get()
+        sim.slice(j)
+    f["q_spe"].set(np.zeros(f["q_spe"].shape()))
+    for sp in sim.spe:
+        sp.qdp(f["q_spe"])
+    b1, cu, q = f["b_spe"].get(), f["cu"].get(), f["q_spe"].get()
+    qb = f["q_beam"].hist(j0)
+    f["b_spe"].set(b_old)
+    net.seed(j0 - 1)
+    f["b_spe"].set(b1)
+    net.mode(seed=True, train=True)
+    net.seed(j0)
+    x, sB = SN.slice_inputs(b1, b_old, cu, q, qb, g.nr)
+    osc = net.cfg.out_scale
+    # a "converged" field: the extrapolation plus a smooth correction
+    b_conv = b1.copy()
+    r = np.arange(g.nr + 2)
+    for c in range(2):
+        b_conv[c, :, :] = 2 * b1[c] - b_old[c] + 0.3 * osc * sB * np.sin(0.1 * r + c)[None, :]
+    f["b_spe"].set(b_conv)
+    net.learn(j0)
+    loss = net.losses()[-1]
+    th1 = net.get_params()
+    sim.close()
+    y, cache = SN.forward(shape, theta, x)
+    t = SN.slice_target(b_conv, b1, b_old, sB, osc, g.nr)
+    lo, go = SN.backward(shape, cache, t)
+    assert abs(loss - lo) <= 2e-3 * lo, (loss, lo)
+    step = th1 - theta
+    big = np.abs(go) > 1e-2 * np.max(np.abs(go))
+    agree = np.mean(np.sign(step[big]) == -np.sign(go[big]))
+    assert agree >= 0.98, agree
+    assert np.allclose(np.abs(step[big]), lr, rtol=0.05)
