@@ -228,18 +228,21 @@ int qpad_kernel_time_reset(qpad_ctx *ctx, int32_t enable);
    simulation_class.f03:326, and read by the first 'record', :370-375).  The
    seed network replaces that seed by 2 B_perp(xi-1) - B_perp(xi-2) + dB: the
    first-order extrapolation in xi plus a correction dB predicted by a 1D
-   convolution network over r from the fields of slice xi-1 and the sources
-   of slice xi (the output layer starts at zero: untrained, the seed is the
+   convolution network over r from the fields slice xi-1 left (its B_perp
+   and charge, the J predictor of slice xi) and the beam charge of slice xi
+   (the output layer starts at zero: untrained, the seed is the
    extrapolation).  The iteration still runs to the same tolerance and
-   iter_max, so the plain method remains the judge.
+   iter_max, so the plain method remains the judge.  Since the inputs are
+   ready when slice xi-1 ends, qpad_step3d computes each seed on a second
+   stream beside the previous slice's push and sort.
 
    Network: `layers` conv1d layers over the radial nodes 1..nr, kernel 3,
    zero padding, SiLU between layers, `width` hidden channels.  Inputs per
    node (8 (2M+1) + 1 channels, each group scaled by its maximum over the
    slice):
      B_r, B_phi of b_spe (slice xi-1);  B_r, B_phi of b_spe(xi-1) -
-     b_spe(xi-2);  J_r, J_phi of cu (the predictor of slice xi);  q_spe and
-     q_beam of slice xi;  r / r_max.
+     b_spe(xi-2);  J_r, J_phi of cu (the predictor of slice xi);  q_spe of
+     slice xi-1;  q_beam of slice xi;  r / r_max.
    Output: dB_r, dB_phi (2 (2M+1) channels) in units of out_scale times the
    B_perp scale; training target: (converged - extrapolated) in the same
    units, clipped to [-1, 1] (so the correction never moves the seed by more
@@ -285,10 +288,11 @@ int qpad_seednet_set_params(qpad_ctx *ctx, const float *theta, int64_t n);
    kept; mean squared error of the normalised output) */
 int qpad_seednet_losses(qpad_ctx *ctx, int64_t first, int64_t n, float *out);
 /* Fine-grained form for a driver that calls QPAD's procedures one by one:
-   qpad_seednet_seed after q_spe and cu of slice j are final and before the
-   first 'record' (scales and packs the inputs; in seed mode it also sets
-   b_spe to the seed), qpad_seednet_learn once slice j has converged
-   (one Adam step; needs qpad_seednet_seed of the same slice first). */
+   qpad_seednet_seed(j) first in slice j, with the fields slice j - 1 left
+   (before q_spe is recomputed; scales and packs the inputs, and in seed
+   mode sets b_spe's B_perp to the seed), qpad_seednet_learn(j) once slice
+   j has converged (one Adam step; needs qpad_seednet_seed of the same slice
+   first). */
 int qpad_seednet_seed(qpad_ctx *ctx, int32_t j);
 int qpad_seednet_learn(qpad_ctx *ctx, int32_t j);
 /* The network on host inputs (tests, offline use): x [npos][cin] FP32

@@ -200,7 +200,9 @@ def slice_scales(b_spe, cu, q_spe, q_beam, nr):
 
 
 def slice_inputs(b_spe, b_old, cu, q_spe, q_beam, nr):
-    """x [nr][8 nm + 1]: channel qty * nm + k, then r / r_max"""
+    """x [nr][8 nm + 1]: channel qty * nm + k, then r / r_max.  The fields
+    as slice xi-1 left them (b_spe, cu = the J predictor, q_spe), b_old =
+    b_spe at the previous slice start, q_beam of slice xi."""
     nm = b_spe.shape[1]
     sB, sJ, sq, sqb = slice_scales(b_spe, cu, q_spe, q_beam, nr)
     sl = slice(1, nr + 1)

@@ -61,6 +61,9 @@ struct SeedNet {
     double *scale_part;     // per-workgroup maxima [nscale_blk][4]
     int nscale_blk;
     int infer_stride;       // BF16 per LDS row of the fused inference
+    double *seed_buf;       // the seed of the next slice [2][nm][ld]
+    hipStream_t stream;     // the network's stream (default priority)
+    hipEvent_t ev_main, ev_net;
     double *bprev;          // b_spe components 0, 1 at the last two slice starts
     float *loss_part, *loss_log;
     float *xin, *tin;       // host inputs of qpad_seednet_eval
@@ -237,7 +240,7 @@ struct NetInferArgs {
     const double *part;
     int nsp;
     double *scales;
-    double *b_spe;         // seed: components 0, 1 of nodes 1..nr
+    double *seed;          // seed: components 0, 1 of nodes 1..nr ([2][nm][ld])
     double oscale;
     int nr, nm, ld, cin, layers, cout, store, stride, nwb;
     float rinv;
@@ -398,7 +401,7 @@ __global__ __launch_bounds__(kInferThreads) void k_net_infer(NetInferArgs a)
                     if (co < a.cout) {
                         const double b = bcache[co * T + rl];
                         const double bo = bcache[(2 * nm + co) * T + rl];
-                        a.b_spe[(size_t)co * ld + p + 1] = b + (b - bo) + (double)zz[rr] * sc;
+                        a.seed[(size_t)co * ld + p + 1] = b + (b - bo) + (double)zz[rr] * sc;
                     }
                 }
             } else if (!last) {
@@ -746,6 +749,12 @@ void seednet_destroy(qpad_ctx *c)
     SeedNet *n = c->net;
     if (!n) return;
     if (c->stream) (void)hipStreamSynchronize(c->stream);
+    if (n->stream) (void)hipStreamSynchronize(n->stream);
+    net_free(n->seed_buf);
+    if (n->ev_main) (void)hipEventDestroy(n->ev_main);
+    if (n->ev_net) (void)hipEventDestroy(n->ev_net);
+    if (n->stream) (void)hipStreamDestroy(n->stream);
+    c->net_pre = 0;
     net_free(n->theta); net_free(n->am); net_free(n->av); net_free(n->part); net_free(n->grad);
     net_free(n->wb); net_free(n->wtb);
     for (int l = 0; l < kNetMaxLayers; l++) { net_free(n->h[l]); net_free(n->z[l]); net_free(n->dz[l]); }
@@ -786,7 +795,8 @@ static void net_init_params(const SeedNet &n, std::vector<float> &th)
     }
 }
 
-static int net_adam_launch(qpad_ctx *c, int flags, float *grad_out, float *loss_slot, int nloss, float loss_norm)
+static int net_adam_launch(qpad_ctx *c, hipStream_t st, int flags, float *grad_out, float *loss_slot, int nloss,
+                           float loss_norm)
 {
     SeedNet &n = *c->net;
     NetAdamArgs a;
@@ -823,16 +833,16 @@ static int net_adam_launch(qpad_ctx *c, int flags, float *grad_out, float *loss_
         a.cout_p[l] = n.cout_p[l];
     }
     const unsigned nb = (unsigned)((n.nparams + 255) / 256);
-    hipLaunchKernelGGL(k_net_adam, dim3(nb), dim3(256), 0, c->stream, a);
+    hipLaunchKernelGGL(k_net_adam, dim3(nb), dim3(256), 0, st, a);
     QPAD_HIP_CHECK(hipGetLastError());
     return 0;
 }
 
-#define NET_FWD(NT, LAST) hipLaunchKernelGGL((k_net_fwd<NT, LAST>), grid, dim3(64), 0, c->stream, a)
+#define NET_FWD(NT, LAST) hipLaunchKernelGGL((k_net_fwd<NT, LAST>), grid, dim3(64), 0, st, a)
 
 // layer-by-layer forward pass over npos positions (training without the
 // seed, qpad_seednet_eval)
-static int net_forward(qpad_ctx *c, int npos)
+static int net_forward(qpad_ctx *c, hipStream_t st, int npos)
 {
     SeedNet &n = *c->net;
     const dim3 grid((unsigned)(round_up(npos, 16) / 16));
@@ -865,7 +875,7 @@ static int net_forward(qpad_ctx *c, int npos)
 
 // loss (host target tin or the converged slice), backward, weight gradients
 // and the Adam step (flags: kAdamGrad / kAdamUpdate)
-static int net_backward(qpad_ctx *c, int npos, const float *tin, int flags, float *grad_out)
+static int net_backward(qpad_ctx *c, hipStream_t st, int npos, const float *tin, int flags, float *grad_out)
 {
     SeedNet &n = *c->net;
     const int L = n.layers;
@@ -887,7 +897,7 @@ static int net_backward(qpad_ctx *c, int npos, const float *tin, int flags, floa
         a.nm = c->g.nm;
         a.ld = c->g.ld;
         const int nb = (npos * a.cout_p + kNetLossThreads - 1) / kNetLossThreads;
-        hipLaunchKernelGGL(k_net_loss, dim3(nb), dim3(kNetLossThreads), 0, c->stream, a);
+        hipLaunchKernelGGL(k_net_loss, dim3(nb), dim3(kNetLossThreads), 0, st, a);
         QPAD_HIP_CHECK(hipGetLastError());
         n.nloss_blk = nb;
     }
@@ -901,8 +911,8 @@ static int net_backward(qpad_ctx *c, int npos, const float *tin, int flags, floa
         a.cin_p = n.cin_p[l];
         a.cout_p = n.cout_p[l];
         a.npos = npos;
-        if (n.cin_p[l] / 16 == 2) hipLaunchKernelGGL(k_net_bwd<2>, grid, dim3(64), 0, c->stream, a);
-        else hipLaunchKernelGGL(k_net_bwd<4>, grid, dim3(64), 0, c->stream, a);
+        if (n.cin_p[l] / 16 == 2) hipLaunchKernelGGL(k_net_bwd<2>, grid, dim3(64), 0, st, a);
+        else hipLaunchKernelGGL(k_net_bwd<4>, grid, dim3(64), 0, st, a);
         QPAD_HIP_CHECK(hipGetLastError());
     }
     {
@@ -918,7 +928,7 @@ static int net_backward(qpad_ctx *c, int npos, const float *tin, int flags, floa
         }
         a.part = n.part;
         a.nparams = n.nparams;
-        hipLaunchKernelGGL(k_net_wgrad, dim3((unsigned)n.nchunk, (unsigned)L), dim3(256), 0, c->stream, a);
+        hipLaunchKernelGGL(k_net_wgrad, dim3((unsigned)n.nchunk, (unsigned)L), dim3(256), 0, st, a);
         QPAD_HIP_CHECK(hipGetLastError());
     }
     float *slot = nullptr;
@@ -926,16 +936,26 @@ static int net_backward(qpad_ctx *c, int npos, const float *tin, int flags, floa
         n.steps++;
         slot = n.loss_log + (size_t)((n.steps - 1) % kNetLossLog);
     }
-    return net_adam_launch(c, flags, grad_out, slot, n.nloss_blk, 1.0f / (float)((long long)npos * n.cout));
+    return net_adam_launch(c, st, flags, grad_out, slot, n.nloss_blk, 1.0f / (float)((long long)npos * n.cout));
 }
 
-// scales of slice j and the saved B_perp; in seed mode the fused inference
-// seeds b_spe (and keeps the activations when training), otherwise the
-// packed inputs wait for the training step's forward pass
-int seednet_slice_start(qpad_ctx *c, int j, bool force)
+// k_net_apply: the seed into b_spe (fine-grained form)
+__global__ __launch_bounds__(256) void k_net_apply(const double *seed, double *b_spe, int nr, int nm, int ld)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x + 1, k = blockIdx.y;
+    if (i > nr) return;
+    b_spe[(size_t)k * ld + i] = seed[(size_t)k * ld + i];
+    b_spe[((size_t)nm + k) * ld + i] = seed[((size_t)nm + k) * ld + i];
+}
+
+// the inputs of slice j from the fields as slice j - 1 left them (b_spe =
+// B_perp(xi-1), cu = the J predictor of slice j, q_spe of slice j - 1,
+// q_beam of slice j): scales and saved B_perp, then in seed mode the fused
+// inference into seed_buf (keeping the activations when training),
+// otherwise the packed inputs for the training step's forward pass
+static int queue_slice_inputs(qpad_ctx *c, hipStream_t st, int j)
 {
     SeedNet *n = c->net;
-    if (!n || (!force && !n->seed_on && !n->train_on)) return 0;
     const int nr = c->g.nr, nm = c->g.nm, ld = c->g.ld;
     const size_t half = (size_t)2 * nm * ld;
     const double *q_beam = c->h_q_beam + (size_t)j * nm * ld;
@@ -952,7 +972,7 @@ int seednet_slice_start(qpad_ctx *c, int j, bool force)
         a.nr = nr;
         a.nm = nm;
         a.ld = ld;
-        hipLaunchKernelGGL(k_net_scale, dim3((unsigned)n->nscale_blk), dim3(256), 0, c->stream, a);
+        hipLaunchKernelGGL(k_net_scale, dim3((unsigned)n->nscale_blk), dim3(256), 0, st, a);
         QPAD_HIP_CHECK(hipGetLastError());
     }
     n->packed_slice = j;
@@ -968,7 +988,7 @@ int seednet_slice_start(qpad_ctx *c, int j, bool force)
         a.part = n->scale_part;
         a.nsp = n->nscale_blk;
         a.scales = n->scales;
-        a.b_spe = c->f[QPAD_F_B_SPE];
+        a.seed = n->seed_buf;
         a.oscale = n->cfg.out_scale;
         a.nr = nr;
         a.nm = nm;
@@ -991,7 +1011,7 @@ int seednet_slice_start(qpad_ctx *c, int j, bool force)
         }
         const size_t lds = infer_lds_bytes(nm, n->infer_stride, (int)n->nwb);
         hipLaunchKernelGGL(k_net_infer, dim3((unsigned)((nr + kInferT - 1) / kInferT)), dim3(kInferThreads), lds,
-                           c->stream, a);
+                           st, a);
         QPAD_HIP_CHECK(hipGetLastError());
         n->fwd_done = true;
     } else {
@@ -1012,21 +1032,55 @@ int seednet_slice_start(qpad_ctx *c, int j, bool force)
         a.cin_p = n->cin_p[0];
         a.rinv = 1.0f / (float)nr;
         const int tot = (n->cin_p[0] / 8) * nr;
-        hipLaunchKernelGGL(k_net_pack, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, c->stream, a);
+        hipLaunchKernelGGL(k_net_pack, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, st, a);
         QPAD_HIP_CHECK(hipGetLastError());
     }
     return 0;
 }
 
-// one Adam step on the converged slice j (train mode)
-int seednet_slice_end(qpad_ctx *c, int j)
+// one Adam step on the converged slice j (its inputs were queued)
+static int queue_slice_training(qpad_ctx *c, hipStream_t st, int j)
 {
     SeedNet *n = c->net;
-    if (!n || !n->train_on) return 0;
     if (n->packed_slice != j) return net_fail(QPAD_E_STATE, "seed network: inputs of this slice were not packed");
-    if (!n->fwd_done) QPAD_TRY(net_forward(c, c->g.nr));
-    QPAD_TRY(net_backward(c, c->g.nr, nullptr, kAdamUpdate, nullptr));
+    if (!n->fwd_done) QPAD_TRY(net_forward(c, st, c->g.nr));
+    QPAD_TRY(net_backward(c, st, c->g.nr, nullptr, kAdamUpdate, nullptr));
     n->packed_slice = 0;
+    return 0;
+}
+
+int seednet_active(const qpad_ctx *c) { return c->net && (c->net->seed_on || c->net->train_on); }
+
+int seednet_after_epilogue(qpad_ctx *c, int j, int next)
+{
+    SeedNet *n = c->net;
+    QPAD_HIP_CHECK(hipEventRecord(n->ev_main, c->stream));
+    QPAD_HIP_CHECK(hipStreamWaitEvent(n->stream, n->ev_main, 0));
+    if (j > 0 && n->train_on) QPAD_TRY(queue_slice_training(c, n->stream, j));
+    c->net_pre = 0;
+    if (next > 0) {
+        QPAD_TRY(queue_slice_inputs(c, n->stream, next));
+        c->net_pre = next;
+    }
+    QPAD_HIP_CHECK(hipEventRecord(n->ev_net, n->stream));
+    return 0;
+}
+
+int seednet_before_sources(qpad_ctx *c, int j, const double **seed)
+{
+    SeedNet *n = c->net;
+    if (c->net_pre != j) return net_fail(QPAD_E_STATE, "seed network: slice inputs not queued");
+    QPAD_HIP_CHECK(hipStreamWaitEvent(c->stream, n->ev_net, 0));
+    *seed = n->seed_on ? n->seed_buf : nullptr;
+    c->net_pre = 0;
+    return 0;
+}
+
+// the host waits for the network's stream (its last work may still read
+// b_spe or write its own buffers)
+int seednet_quiesce(qpad_ctx *c)
+{
+    if (c->net && c->net->stream) QPAD_HIP_CHECK(hipStreamSynchronize(c->net->stream));
     return 0;
 }
 
@@ -1035,8 +1089,10 @@ int seednet_step_begin(qpad_ctx *c)
 {
     SeedNet *n = c->net;
     if (!n) return 0;
+    QPAD_TRY(seednet_quiesce(c));
     QPAD_HIP_CHECK(hipMemsetAsync(n->bprev, 0, sizeof(double) * 4 * c->g.nm * c->g.ld, c->stream));
     n->packed_slice = 0;
+    c->net_pre = 0;
     return 0;
 }
 
@@ -1122,6 +1178,12 @@ int qpad_seednet_create(qpad_ctx *c, const qpad_seednet_cfg *cfg)
     if (!rc) rc = net_alloc(&n->bprev, (size_t)4 * c->g.nm * c->g.ld);
     if (!rc) rc = net_alloc(&n->loss_part, (size_t)(rows * 32 / kNetLossThreads + 1));
     if (!rc) rc = net_alloc(&n->loss_log, (size_t)kNetLossLog);
+    if (!rc) rc = net_alloc(&n->seed_buf, (size_t)2 * c->g.nm * c->g.ld);
+    if (!rc && hipStreamCreateWithFlags(&n->stream, hipStreamNonBlocking) != hipSuccess)
+        rc = net_fail(QPAD_E_HIP, "hipStreamCreate failed");
+    if (!rc && (hipEventCreateWithFlags(&n->ev_main, hipEventDisableTiming) != hipSuccess ||
+                hipEventCreateWithFlags(&n->ev_net, hipEventDisableTiming) != hipSuccess))
+        rc = net_fail(QPAD_E_HIP, "hipEventCreate failed");
     if (!rc) rc = net_alloc(&n->xin, (size_t)n->npos * n->cin);
     if (!rc) rc = net_alloc(&n->tin, (size_t)n->npos * n->cout);
     if (rc) {
@@ -1134,6 +1196,7 @@ int qpad_seednet_create(qpad_ctx *c, const qpad_seednet_cfg *cfg)
         QPAD_HIP_CHECK(hipMemsetAsync(n->dz[l], 0, rows * n->cout_p[l] * sizeof(__bf16), c->stream));
     }
     QPAD_HIP_CHECK(hipMemsetAsync(n->bprev, 0, sizeof(double) * 4 * c->g.nm * c->g.ld, c->stream));
+    QPAD_HIP_CHECK(hipMemsetAsync(n->seed_buf, 0, sizeof(double) * 2 * c->g.nm * c->g.ld, c->stream));
     QPAD_HIP_CHECK(hipMemsetAsync(n->loss_log, 0, sizeof(float) * kNetLossLog, c->stream));
     std::vector<float> th;
     net_init_params(*n, th);
@@ -1144,6 +1207,7 @@ int qpad_seednet_get_info(qpad_ctx *c, qpad_seednet_info *out)
 {
     if (!c || !out) return net_fail(QPAD_E_INVALID, "null argument");
     if (!c->net) return net_fail(QPAD_E_STATE, "no seed network (qpad_seednet_create)");
+    QPAD_TRY(seednet_quiesce(c));
     const SeedNet &n = *c->net;
     memset(out, 0, sizeof(*out));
     out->layers = n.layers;
@@ -1163,9 +1227,11 @@ int qpad_seednet_mode(qpad_ctx *c, int32_t seed, int32_t train)
 {
     if (!c) return net_fail(QPAD_E_INVALID, "null context");
     if (!c->net) return net_fail(QPAD_E_STATE, "no seed network (qpad_seednet_create)");
+    QPAD_TRY(seednet_quiesce(c));
     c->net->seed_on = seed != 0;
     c->net->train_on = train != 0;
     c->net->packed_slice = 0;
+    c->net_pre = 0;
     return 0;
 }
 
@@ -1173,6 +1239,7 @@ int qpad_seednet_get_params(qpad_ctx *c, float *theta, int64_t n)
 {
     if (!c || !theta) return net_fail(QPAD_E_INVALID, "null argument");
     if (!c->net) return net_fail(QPAD_E_STATE, "no seed network (qpad_seednet_create)");
+    QPAD_TRY(seednet_quiesce(c));
     if (n != c->net->nparams) return net_fail(QPAD_E_INVALID, "parameter count mismatch");
     QPAD_HIP_CHECK(hipMemcpyAsync(theta, c->net->theta, sizeof(float) * (size_t)n, hipMemcpyDeviceToHost, c->stream));
     QPAD_HIP_CHECK(hipStreamSynchronize(c->stream));
@@ -1184,13 +1251,14 @@ int qpad_seednet_set_params(qpad_ctx *c, const float *theta, int64_t n)
 {
     if (!c || !theta) return net_fail(QPAD_E_INVALID, "null argument");
     if (!c->net) return net_fail(QPAD_E_STATE, "no seed network (qpad_seednet_create)");
+    QPAD_TRY(seednet_quiesce(c));
     SeedNet &s = *c->net;
     if (n != s.nparams) return net_fail(QPAD_E_INVALID, "parameter count mismatch");
     QPAD_HIP_CHECK(hipMemcpyAsync(s.theta, theta, sizeof(float) * (size_t)n, hipMemcpyHostToDevice, c->stream));
     QPAD_HIP_CHECK(hipMemsetAsync(s.am, 0, sizeof(float) * (size_t)n, c->stream));
     QPAD_HIP_CHECK(hipMemsetAsync(s.av, 0, sizeof(float) * (size_t)n, c->stream));
     s.steps = 0;
-    QPAD_TRY(net_adam_launch(c, kAdamRefresh, nullptr, nullptr, 0, 0.0f));
+    QPAD_TRY(net_adam_launch(c, c->stream, kAdamRefresh, nullptr, nullptr, 0, 0.0f));
     QPAD_HIP_CHECK(hipStreamSynchronize(c->stream));
     return 0;
 }
@@ -1199,6 +1267,7 @@ int qpad_seednet_losses(qpad_ctx *c, int64_t first, int64_t n, float *out)
 {
     if (!c || !out) return net_fail(QPAD_E_INVALID, "null argument");
     if (!c->net) return net_fail(QPAD_E_STATE, "no seed network (qpad_seednet_create)");
+    QPAD_TRY(seednet_quiesce(c));
     const SeedNet &s = *c->net;
     if (first < 0 || n < 0 || first + n > s.steps || s.steps - first > kNetLossLog)
         return net_fail(QPAD_E_INVALID, "loss range not held");
@@ -1215,7 +1284,15 @@ int qpad_seednet_seed(qpad_ctx *c, int32_t j)
     if (!c) return net_fail(QPAD_E_INVALID, "null context");
     if (!c->net) return net_fail(QPAD_E_STATE, "no seed network (qpad_seednet_create)");
     if (j < 1 || j > c->g.nxi) return net_fail(QPAD_E_INVALID, "slice index out of range");
-    QPAD_TRY(seednet_slice_start(c, j, true));
+    SeedNet &n = *c->net;
+    QPAD_TRY(seednet_quiesce(c));
+    QPAD_TRY(queue_slice_inputs(c, c->stream, j));
+    if (n.seed_on) {
+        const dim3 grid((unsigned)((c->g.nr + 255) / 256), (unsigned)c->g.nm);
+        hipLaunchKernelGGL(k_net_apply, grid, dim3(256), 0, c->stream, n.seed_buf, c->f[QPAD_F_B_SPE], c->g.nr,
+                           c->g.nm, c->g.ld);
+        QPAD_HIP_CHECK(hipGetLastError());
+    }
     QPAD_HIP_CHECK(hipStreamSynchronize(c->stream));
     return 0;
 }
@@ -1225,7 +1302,8 @@ int qpad_seednet_learn(qpad_ctx *c, int32_t j)
     if (!c) return net_fail(QPAD_E_INVALID, "null context");
     if (!c->net) return net_fail(QPAD_E_STATE, "no seed network (qpad_seednet_create)");
     if (!c->net->train_on) return net_fail(QPAD_E_STATE, "seed network is not in train mode");
-    QPAD_TRY(seednet_slice_end(c, j));
+    QPAD_TRY(seednet_quiesce(c));
+    QPAD_TRY(queue_slice_training(c, c->stream, j));
     QPAD_HIP_CHECK(hipStreamSynchronize(c->stream));
     return 0;
 }
@@ -1238,6 +1316,7 @@ int qpad_seednet_eval(qpad_ctx *c, const float *x, const float *t, int32_t npos,
     SeedNet &n = *c->net;
     if (npos < 1 || npos > n.npos) return net_fail(QPAD_E_INVALID, "npos must be 1..nr");
     if ((loss || grad || update) && !t) return net_fail(QPAD_E_INVALID, "loss, gradient and update need a target");
+    QPAD_TRY(seednet_quiesce(c));
     const int L = n.layers;
     const size_t rows = (size_t)n.npos_pad + 2;
     // rows past npos must read as zero padding
@@ -1250,12 +1329,12 @@ int qpad_seednet_eval(qpad_ctx *c, const float *x, const float *t, int32_t npos,
     hipLaunchKernelGGL(k_net_pack_host, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, c->stream, n.xin, n.h[0],
                        npos, n.cin, n.cin_p[0]);
     QPAD_HIP_CHECK(hipGetLastError());
-    QPAD_TRY(net_forward(c, npos));
+    QPAD_TRY(net_forward(c, c->stream, npos));
     if (t) {
         QPAD_HIP_CHECK(
             hipMemcpyAsync(n.tin, t, sizeof(float) * (size_t)npos * n.cout, hipMemcpyHostToDevice, c->stream));
         const int flags = (grad ? kAdamGrad : 0) | (update ? kAdamUpdate : 0);
-        QPAD_TRY(net_backward(c, npos, n.tin, flags, grad ? n.grad : nullptr));
+        QPAD_TRY(net_backward(c, c->stream, npos, n.tin, flags, grad ? n.grad : nullptr));
     }
     if (y) {
         const int cp = n.cout_p[L - 1];

@@ -550,6 +550,10 @@ struct StartSrcArgs {
     const double *P0[kMaxSpecies], *P1[kMaxSpecies], *qn[kMaxSpecies];
     const double *cu;
     double *q_spe, *src;
+    // learned seed of the iteration (B_perp components 0, 1 of nodes
+    // 1..nr), copied into b_spe here, or null
+    const double *seed;
+    double *b_spe;
 };
 
 // thread per (node i = 0..nr+1, mode array k): q_spe = rho - Jz of every
@@ -587,6 +591,10 @@ __global__ __launch_bounds__(256) void k_start_src(StartSrcArgs a)
         if (a.qn[s]) v = v + qb[s];
     a.q_spe[(size_t)k * ld + i] = v;
     if (i < 1 || i > nr) return;
+    if (a.seed) {
+        FX(a.b_spe, 0, k, i) = FX(a.seed, 0, k, i);
+        FX(a.b_spe, 1, k, i) = FX(a.seed, 1, k, i);
+    }
     const FieldArr cu{a.cu, nm, ld};
     a.src[(size_t)src_psi(M, nm, k) * nr + i - 1] = -1.0 * v;
     a.src[(size_t)src_bz(M, k) * nr + i - 1] = bz_source(g, cu, k, i);
@@ -693,8 +701,9 @@ __global__ __launch_bounds__(256) void k_eperp(EperpArgs a)
 
 int src_rows_host(int M, int nm) { return src_rows(M, nm); }
 
-// sources of the slice start (q_spe, the psi / Bz / Ez right-hand sides)
-int launch_slice_sources(qpad_ctx *c)
+// sources of the slice start (q_spe, the psi / Bz / Ez right-hand sides);
+// seed: the learned seed to copy into b_spe first, or null
+int launch_slice_sources(qpad_ctx *c, const double *seed)
 {
     const FieldCommon g = field_common(c);
     const int nm = c->g.nm;
@@ -710,6 +719,8 @@ int launch_slice_sources(qpad_ctx *c)
     s.cu = c->f[QPAD_F_CU];
     s.q_spe = c->f[QPAD_F_Q_SPE];
     s.src = c->src;
+    s.seed = seed;
+    s.b_spe = c->f[QPAD_F_B_SPE];
     switch (c->nspecies) {
     case 1: hipLaunchKernelGGL(k_start_src<1>, ngrid, dim3(256), 0, c->stream, s); break;
     case 2: hipLaunchKernelGGL(k_start_src<2>, ngrid, dim3(256), 0, c->stream, s); break;

@@ -44,14 +44,20 @@ int launch_epilogue(qpad_ctx *c, int j);
 int slice_kernels_init();
 // start of a slice after the charge deposit: q_spe, psi, Bz, Ez, 'record',
 // then E_perp and b for the first current deposit
-int launch_slice_sources(qpad_ctx *c);
+int launch_slice_sources(qpad_ctx *c, const double *seed);
 int launch_slice_solves(qpad_ctx *c, int j);
 
-// learned B_perp seed (k_seednet.hip): inputs and, in seed mode, the seed at
-// the slice start (force: also when both modes are off); one Adam step after
-// the slice has converged (train mode); the 3D-step start
-int seednet_slice_start(qpad_ctx *c, int j, bool force);
-int seednet_slice_end(qpad_ctx *c, int j);
+// learned B_perp seed (k_seednet.hip), on its own stream beside the slice:
+// seednet_before_sources makes the main stream wait for slice j's seed (and
+// queues its inputs and inference first if they were not queued at the end
+// of slice j - 1) and returns the seed to copy into b_spe (null without one);
+// seednet_after_epilogue queues slice j's Adam step (train mode) and slice
+// j + 1's inputs and inference (next > 0), behind the epilogue;
+// seednet_quiesce makes the main stream wait for all of the network's work
+int seednet_active(const qpad_ctx *c);
+int seednet_before_sources(qpad_ctx *c, int j, const double **seed);
+int seednet_after_epilogue(qpad_ctx *c, int j, int next);
+int seednet_quiesce(qpad_ctx *c);
 int seednet_step_begin(qpad_ctx *c);
 void seednet_destroy(qpad_ctx *c);
 // rows of the solve right-hand-side buffer

@@ -507,6 +507,7 @@ int qpad_beam_new(qpad_ctx *c, const qpad_beam_cfg *cfg, int32_t *beam_id)
 int qpad_step3d_begin(qpad_ctx *c)
 {
     if (!c) return fail(QPAD_E_INVALID, "null context");
+    TRY(seednet_quiesce(c));
     // q_beam%as(0); beam3d%qdp (simulation_class.f03:297-306)
     for (int b = 0; b < c->nbeams; b++) TRY(launch_beam_sort(c, b));
     if (c->nbeams > 0) TRY(launch_beam_qdeposit(c));
@@ -516,7 +517,7 @@ int qpad_step3d_begin(qpad_ctx *c)
     TRY(launch_bt_beam_all(c));
     TRY(zero_dev(c, c->h_q_spe, ((size_t)c->g.nxi + 2) * f1_size(c, 1)));
     const int zf[] = {QPAD_F_B, QPAD_F_E, QPAD_F_B_SPE, QPAD_F_E_SPE, QPAD_F_PSI, QPAD_F_CU, QPAD_F_ACU,
-                      QPAD_F_AMU};
+                      QPAD_F_AMU, QPAD_F_Q_SPE};
     for (int f : zf) TRY(zero_dev(c, c->f[f], f1_size(c, c->ncomp[f])));
     TRY(seednet_step_begin(c));
     c->spec_guess = 2;
@@ -532,12 +533,20 @@ namespace qpad {
 // synchronisation: all iter_max predictor-corrector iterations are queued and
 // each of their kernels returns at once after the device-side convergence
 // test has set IterState::done.
-static int slice_enqueue(qpad_ctx *c, int j)
+// overlap: also queue the next slice's seed behind this slice's epilogue
+// (qpad_step3d, where nothing can change the fields between slices)
+static int slice_enqueue(qpad_ctx *c, int j, bool overlap)
 {
     // rho - Jz of every species, psi, Bz from the J predictor, 'record'
+    // the learned seed (when on) is computed on the network's stream from
+    // the fields left by slice j - 1, beside the previous slice's push and
+    // sort and this slice's charge deposit
+    const bool net = seednet_active(c);
+    const double *seed = nullptr;
+    if (net && c->net_pre != j) TRY(seednet_after_epilogue(c, 0, j));
     for (int s = 0; s < c->nspecies; s++) TRY(launch_qdeposit(c, s));
-    TRY(launch_slice_sources(c));
-    TRY(seednet_slice_start(c, j, false));
+    if (net) TRY(seednet_before_sources(c, j, &seed));
+    TRY(launch_slice_sources(c, seed));
     TRY(launch_slice_solves(c, j));
     // predictor-corrector iterations, queued without waiting for the device.
     // Each iteration's kernels return at once once the device-side test has
@@ -583,7 +592,7 @@ static int slice_enqueue(qpad_ctx *c, int j)
     if (c->adaptive) c->spec_guess = known_iters;
     // slice epilogue: cbq, e_spe, history (simulation_class.f03:401-456)
     TRY(launch_epilogue(c, j));
-    TRY(seednet_slice_end(c, j));
+    if (net) TRY(seednet_after_epilogue(c, j, (overlap && j < c->g.nxi) ? j + 1 : 0));
     for (int s = 0; s < c->nspecies; s++) {
         TRY(launch_push(c, s));
         TRY(launch_sort(c, s));
@@ -605,7 +614,7 @@ int qpad_slice(qpad_ctx *c, int32_t j, int32_t *iters_out)
 {
     if (!c) return fail(QPAD_E_INVALID, "null context");
     if (j < 1 || j > c->g.nxi) return fail(QPAD_E_INVALID, "slice index out of range");
-    TRY(slice_enqueue(c, j));
+    TRY(slice_enqueue(c, j, false));
     TRY(read_iter_state(c));
     if (iters_out) *iters_out = c->it_host->iters;
     return 0;
@@ -629,7 +638,7 @@ int qpad_step3d(qpad_ctx *c, int32_t *total_iters_out)
     TRY(read_iter_state(c));
     c->it_host->total_iters = 0;
     QPAD_HIP_CHECK(hipMemcpyAsync(c->it, c->it_host, sizeof(IterState), hipMemcpyHostToDevice, c->stream));
-    for (int j = 1; j <= c->g.nxi; j++) TRY(slice_enqueue(c, j));
+    for (int j = 1; j <= c->g.nxi; j++) TRY(slice_enqueue(c, j, true));
     TRY(read_iter_state(c));
     const int32_t tot = c->it_host->total_iters;
     TRY(qpad_step3d_end(c));

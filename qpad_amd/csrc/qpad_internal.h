@@ -117,6 +117,7 @@ struct qpad_ctx {
     qpad::EventPool pools[3];
     bool step_open;
     qpad::SeedNet *net;       // null unless qpad_seednet_create
+    int net_pre;              // slice whose seed is already queued (0: none)
 };
 
 namespace qpad {

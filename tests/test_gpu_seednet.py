@@ -124,10 +124,8 @@ def test_slice_seed_matches_oracle():
         if j == j0 - 1:
             b_old = f["b_spe"].get()        # b_spe at the start of slice j0 - 1
         sim.slice(j)
-    # slice j0 up to the seed: q_spe of the slice, cu is the J predictor
-    f["q_spe"].set(np.zeros(f["q_spe"].shape()))
-    for sp in sim.spe:
-        sp.qdp(f["q_spe"])
+    # the seed of slice j0 comes first in the slice, from the fields slice
+    # j0 - 1 left: b_spe, the J predictor cu, q_spe of slice j0 - 1
     b_spe, cu, q = f["b_spe"].get(), f["cu"].get(), f["q_spe"].get()
     qb = f["q_beam"].hist(j0)
     # the library keeps b_spe of the previous slice start (for dB(xi-1));
@@ -196,9 +194,6 @@ def test_slice_training_step_matches_oracle():
         if j == j0 - 1:
             b_old = f["b_spe"].get()
         sim.slice(j)
-    f["q_spe"].set(np.zeros(f["q_spe"].shape()))
-    for sp in sim.spe:
-        sp.qdp(f["q_spe"])
     b1, cu, q = f["b_spe"].get(), f["cu"].get(), f["q_spe"].get()
     qb = f["q_beam"].hist(j0)
     f["b_spe"].set(b_old)
