@@ -13,8 +13,9 @@ Two legs run from identical initial states, to the same iteration
 tolerance: the plain method (QPAD's seed b_spe(xi-1)) and the learned seed
 (first-order extrapolation plus the conv network's correction, trained
 online during the W + T untimed warm-up steps, frozen in the timed steps).
-Both time the same K steps (3D steps W+T+1 .. W+T+K of the same physics).
-`value` is the leg named by --seed (learned by default); the other leg is
+Both time the same K steps (3D steps W+T+1 .. W+T+K of the same physics),
+interleaved step by step.
+`value` is the leg named by --seed (plain by default); the other leg is
 reported beside it.
 """
 import argparse
@@ -239,17 +240,20 @@ def main():
             s_.step3d()
     net.mode(seed=True, train=False)
     train_losses = net.losses()
-    # timed region of each leg: K whole 3D steps; HIP events around the
-    # dominant kernel in the reported leg
-    timing = {}
-    for name, s_ in legs.items():
-        if s_ is sim:
-            s_.kernel_time_reset(True)
-        t0 = time.perf_counter()
-        it = 0
-        for _ in range(args.steps):
-            it += s_.step3d()
-        timing[name] = (time.perf_counter() - t0, it / (args.steps * g.nxi))
+    # timed region: K whole 3D steps of each leg, interleaved (plain, learned,
+    # then learned, plain, ...) so that both legs see the same device
+    # conditions; each step3d call returns when its step is done.  HIP events
+    # around the dominant kernel in the reported leg.
+    names = list(legs)
+    tsum = {k: 0.0 for k in names}
+    isum = {k: 0 for k in names}
+    sim.kernel_time_reset(True)
+    for step in range(args.steps):
+        for name in (names if step % 2 == 0 else names[::-1]):
+            t0 = time.perf_counter()
+            isum[name] += legs[name].step3d()
+            tsum[name] += time.perf_counter() - t0
+    timing = {k: (tsum[k], isum[k] / (args.steps * g.nxi)) for k in names}
     dep_ms, dep_n = sim.kernel_time(0)
     push_ms, push_n = sim.kernel_time(1)
     sol_ms, sol_n = sim.kernel_time(2)
