@@ -7,6 +7,6 @@ the C ABI.  Simulation() raises QpadError if the library is not built.
 """
 from . import configs  # noqa: F401
 from ._lib import LIB_PATH, QpadError, load  # noqa: F401
-from .simulation import Beam3D, Field, SeedNet, Simulation, Species2D  # noqa: F401
+from .simulation import Batch, Beam3D, Field, SeedNet, Simulation, Species2D  # noqa: F401
 
-__all__ = ["Simulation", "SeedNet", "Species2D", "Beam3D", "Field", "QpadError", "load", "configs", "LIB_PATH"]
+__all__ = ["Simulation", "SeedNet", "Batch", "Species2D", "Beam3D", "Field", "QpadError", "load", "configs", "LIB_PATH"]

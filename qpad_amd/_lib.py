@@ -115,6 +115,10 @@ SIGNATURES = {
     "qpad_seednet_seed": (C.c_int, [_CTX, C.c_int32]),
     "qpad_seednet_learn": (C.c_int, [_CTX, C.c_int32]),
     "qpad_seednet_eval": (C.c_int, [_CTX, _fp, _fp, C.c_int32, _fp, _fp, _fp, C.c_int32]),
+    "qpad_batch_create": (C.c_int, [C.POINTER(C.c_void_p), C.c_int32, C.POINTER(C.c_void_p)]),
+    "qpad_batch_destroy": (C.c_int, [C.c_void_p]),
+    "qpad_batch_mode": (C.c_int, [C.c_void_p, C.c_int32]),
+    "qpad_batch_step3d": (C.c_int, [C.c_void_p, _ip]),
 }
 
 _lib = None

@@ -8,8 +8,15 @@ C2: Nr=4096, r_max=12, m=0..2, Nxi=1024, dxi=0.04, ppc 4x32 (524,288 plasma
     particles, 10 3D steps with dt=10.
 C3: as C2 with Nr=2048, m=0..4, electrons and mobile ions (m=1836) at ppc
     4x16, both not neutralised, beam offset by 0.1 sigma_r, 20 3D steps.
+C4: a parameter scan of 16 independent instances stepped as one batch:
+    Nr uniform in {512, 1024, 2048, 4096} (r_max 12), max mode uniform in
+    1..4, beam n_b/n_p uniform in [1, 16] and transverse offset uniform in
+    [0, 0.2] sigma_r, Nxi=512, dxi=0.04, electrons at ppc 4x16, drawn from a
+    recorded seed.
 """
 import copy
+
+import numpy as np
 
 from .input import tri_gaussian_charge
 
@@ -72,6 +79,23 @@ def c3(total_num=1 << 22, nstep=20):
     ions.update({"q": 1.0, "m": 1836.0})
     return _deck(2048, 12.0, 4, nxi, dxi, [_electrons(4, 16, neutralized=False), ions],
                  _beam(nxi * dxi, 0.5, 1.0, 8.0, total_num, offset_x=0.05), nstep=nstep)
+
+
+def c4(n=16, seed=2024, total_num=1 << 18, nxi=512):
+    """The C4 scan: a list of n decks (sigma_r 0.5, sigma_xi 1, beam of
+    total_num macro-particles each) and the drawn parameters."""
+    rng = np.random.default_rng(seed)
+    dxi, sigma_r = 0.04, 0.5
+    decks, params = [], []
+    for _ in range(n):
+        nr = int(rng.choice([512, 1024, 2048, 4096]))
+        m = int(rng.integers(1, 5))
+        nb = float(rng.uniform(1.0, 16.0))
+        off = float(rng.uniform(0.0, 0.2)) * sigma_r
+        decks.append(_deck(nr, 12.0, m, nxi, dxi, [_electrons(4, 16)],
+                           _beam(nxi * dxi, sigma_r, 1.0, nb, total_num, offset_x=off), nstep=1))
+        params.append({"nr": nr, "max_mode": m, "n_b": nb, "offset_x": off})
+    return decks, params
 
 
 def small(nr=64, nxi=48, max_mode=1, ppc=(2, 4), total_num=1 << 14, rmax=4.0, dxi=0.05,

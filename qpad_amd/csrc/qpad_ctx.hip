@@ -504,9 +504,12 @@ int qpad_beam_new(qpad_ctx *c, const qpad_beam_cfg *cfg, int32_t *beam_id)
 
 // ---- fused step ----
 
-int qpad_step3d_begin(qpad_ctx *c)
+}  // extern "C"
+
+namespace qpad {
+// the 3D-step head without the final host synchronisation
+static int step_begin_enqueue(qpad_ctx *c)
 {
-    if (!c) return fail(QPAD_E_INVALID, "null context");
     TRY(seednet_quiesce(c));
     // q_beam%as(0); beam3d%qdp (simulation_class.f03:297-306)
     for (int b = 0; b < c->nbeams; b++) TRY(launch_beam_sort(c, b));
@@ -522,6 +525,16 @@ int qpad_step3d_begin(qpad_ctx *c)
     TRY(seednet_step_begin(c));
     c->spec_guess = 2;
     c->step_open = true;
+    return 0;
+}
+}  // namespace qpad
+
+extern "C" {
+
+int qpad_step3d_begin(qpad_ctx *c)
+{
+    if (!c) return fail(QPAD_E_INVALID, "null context");
+    TRY(step_begin_enqueue(c));
     return sync(c);
 }
 
@@ -970,3 +983,158 @@ int qpad_kernel_time_reset(qpad_ctx *c, int32_t enable)
 }
 
 }  // extern "C"
+
+// ---------------------------------------------------------------------------
+// batched instances: several independent contexts stepped in one call
+// ---------------------------------------------------------------------------
+struct qpad_batch {
+    std::vector<qpad_ctx *> ctx;
+    int graph = 0;
+    // per context, its captured slice loops.  The kernels capture the
+    // particle buffers by address, and the buffers swap once per sort: an
+    // even number of times in a slice loop, once more in the step's tail
+    // (the renewed plasma is sorted).  So the step alternates between two
+    // buffer assignments, and each context keeps one graph per assignment,
+    // keyed by the first species' x1 buffer at capture.
+    std::vector<hipGraphExec_t> exec[2];
+    std::vector<const void *> key[2];
+};
+
+namespace qpad {
+
+// the slice loop of one 3D step of c, captured into a graph: all iter_max
+// iterations queued (no host feedback), no timing events, no seed network.
+// The particle buffers swap once per slice, so an even nxi returns them to
+// the captured state after every replay.
+static int capture_slices(qpad_ctx *c, hipGraphExec_t *exec)
+{
+    if (c->g.nxi % 2) return fail(QPAD_E_INVALID, "graph mode needs an even nxi");
+    const int adaptive = c->adaptive;
+    const bool timing = c->timing;
+    c->adaptive = 0;
+    c->timing = false;
+    hipGraph_t g = nullptr;
+    QPAD_HIP_CHECK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeRelaxed));
+    int rc = 0;
+    for (int j = 1; j <= c->g.nxi && !rc; j++) rc = slice_enqueue(c, j, false);
+    const hipError_t e = hipStreamEndCapture(c->stream, &g);
+    c->adaptive = adaptive;
+    c->timing = timing;
+    if (rc) {
+        if (e == hipSuccess && g) (void)hipGraphDestroy(g);
+        return rc;
+    }
+    QPAD_HIP_CHECK(e);
+    const hipError_t ei = hipGraphInstantiate(exec, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    QPAD_HIP_CHECK(ei);
+    return 0;
+}
+
+static void batch_drop_graphs(qpad_batch *b)
+{
+    for (int k = 0; k < 2; k++) {
+        for (auto &x : b->exec[k])
+            if (x) (void)hipGraphExecDestroy(x);
+        b->exec[k].assign(b->ctx.size(), nullptr);
+        b->key[k].assign(b->ctx.size(), nullptr);
+    }
+}
+
+}  // namespace qpad
+
+extern "C" {
+
+int qpad_batch_create(qpad_ctx *const *ctxs, int32_t n, qpad_batch **out)
+{
+    if (!ctxs || !out || n < 1) return fail(QPAD_E_INVALID, "bad batch arguments");
+    *out = nullptr;
+    for (int i = 0; i < n; i++) {
+        if (!ctxs[i]) return fail(QPAD_E_INVALID, "null context in batch");
+        for (int k = 0; k < i; k++)
+            if (ctxs[k] == ctxs[i]) return fail(QPAD_E_INVALID, "a context appears twice in the batch");
+        if (ctxs[i]->device != ctxs[0]->device) return fail(QPAD_E_INVALID, "batch contexts on different devices");
+    }
+    qpad_batch *b = new qpad_batch();
+    b->ctx.assign(ctxs, ctxs + n);
+    batch_drop_graphs(b);
+    *out = b;
+    return 0;
+}
+
+int qpad_batch_destroy(qpad_batch *b)
+{
+    if (!b) return 0;
+    for (qpad_ctx *c : b->ctx) (void)hipStreamSynchronize(c->stream);
+    batch_drop_graphs(b);
+    delete b;
+    return 0;
+}
+
+int qpad_batch_mode(qpad_batch *b, int32_t graph)
+{
+    if (!b) return fail(QPAD_E_INVALID, "null batch");
+    if ((graph != 0) != (b->graph != 0)) batch_drop_graphs(b);
+    b->graph = graph != 0;
+    return 0;
+}
+
+// One 3D step of every context: the heads, then the slices of all contexts
+// (round-robin by slice index, each context on its own stream, so their
+// kernels run side by side; or each context's captured slice graph), then
+// the tails.  Every context's results equal its own qpad_step3d.
+int qpad_batch_step3d(qpad_batch *b, int32_t *total_iters_out)
+{
+    if (!b) return fail(QPAD_E_INVALID, "null batch");
+    const int n = (int)b->ctx.size();
+    int nxi_max = 0;
+    for (qpad_ctx *c : b->ctx) {
+        if (seednet_active(c)) return fail(QPAD_E_STATE, "batched contexts step without the seed network");
+        nxi_max = std::max(nxi_max, c->g.nxi);
+    }
+    for (qpad_ctx *c : b->ctx) {
+        TRY(step_begin_enqueue(c));
+        memset(c->it_host, 0, sizeof(IterState));
+        QPAD_HIP_CHECK(hipMemcpyAsync(c->it, c->it_host, sizeof(IterState), hipMemcpyHostToDevice, c->stream));
+    }
+    if (b->graph) {
+        for (int i = 0; i < n; i++) {
+            qpad_ctx *c = b->ctx[i];
+            const void *key = c->nspecies > 0 ? (const void *)c->sp[0].x1 : nullptr;
+            int k = -1;
+            for (int q = 0; q < 2 && k < 0; q++)
+                if (b->exec[q][i] && b->key[q][i] == key) k = q;
+            if (k < 0) {
+                k = b->exec[0][i] ? 1 : 0;
+                if (b->exec[k][i]) (void)hipGraphExecDestroy(b->exec[k][i]);
+                b->exec[k][i] = nullptr;
+                TRY(capture_slices(c, &b->exec[k][i]));
+                b->key[k][i] = key;
+            }
+            QPAD_HIP_CHECK(hipGraphLaunch(b->exec[k][i], c->stream));
+        }
+    } else {
+        // fixed queuing: a host wait for one context would stall the others
+        std::vector<int> adaptive(n);
+        for (int i = 0; i < n; i++) {
+            adaptive[i] = b->ctx[i]->adaptive;
+            b->ctx[i]->adaptive = 0;
+        }
+        int rc = 0;
+        for (int j = 1; j <= nxi_max && !rc; j++)
+            for (int i = 0; i < n && !rc; i++)
+                if (j <= b->ctx[i]->g.nxi) rc = slice_enqueue(b->ctx[i], j, false);
+        for (int i = 0; i < n; i++) b->ctx[i]->adaptive = adaptive[i];
+        TRY(rc);
+    }
+    for (int i = 0; i < n; i++) {
+        qpad_ctx *c = b->ctx[i];
+        TRY(read_iter_state(c));
+        if (total_iters_out) total_iters_out[i] = c->it_host->total_iters;
+        TRY(qpad_step3d_end(c));
+    }
+    return 0;
+}
+
+}  // extern "C"
+

@@ -373,6 +373,35 @@ module qpad_hip_mod
       integer(c_int) :: rc
     end function
 
+    ! ctxs: c_loc of a type(c_ptr) array of n contexts; batch: the new handle
+    function qpad_batch_create(ctxs, n, batch) bind(C, name="qpad_batch_create") result(rc)
+      import :: c_int, c_int32_t, c_ptr
+      type(c_ptr), value :: ctxs
+      integer(c_int32_t), value :: n
+      type(c_ptr), intent(out) :: batch
+      integer(c_int) :: rc
+    end function
+
+    function qpad_batch_destroy(batch) bind(C, name="qpad_batch_destroy") result(rc)
+      import :: c_int, c_ptr
+      type(c_ptr), value :: batch
+      integer(c_int) :: rc
+    end function
+
+    function qpad_batch_mode(batch, graph) bind(C, name="qpad_batch_mode") result(rc)
+      import :: c_int, c_int32_t, c_ptr
+      type(c_ptr), value :: batch
+      integer(c_int32_t), value :: graph
+      integer(c_int) :: rc
+    end function
+
+    function qpad_batch_step3d(batch, total_iters) bind(C, name="qpad_batch_step3d") result(rc)
+      import :: c_int, c_int32_t, c_ptr
+      type(c_ptr), value :: batch
+      integer(c_int32_t), intent(out) :: total_iters(*)
+      integer(c_int) :: rc
+    end function
+
     function qpad_seednet_create(ctx, cfg) bind(C, name="qpad_seednet_create") result(rc)
       import :: c_int, c_ptr, qpad_seednet_cfg
       type(c_ptr), value :: ctx
@@ -456,7 +485,7 @@ module qpad_hip_mod
     qpad_set_beam, qpad_kernel_time, qpad_kernel_time_reset, qpad_iteration_policy, &
     qpad_seednet_create, qpad_seednet_get_info, qpad_seednet_mode, qpad_seednet_get_params, &
     qpad_seednet_set_params, qpad_seednet_losses, qpad_seednet_seed, qpad_seednet_learn, &
-    qpad_seednet_eval
+    qpad_seednet_eval, qpad_batch_create, qpad_batch_destroy, qpad_batch_mode, qpad_batch_step3d
 
 contains
 

@@ -334,3 +334,41 @@ class Simulation:
             self.close()
         except Exception:
             pass
+
+
+class Batch:
+    """Several independent Simulations stepped as one batch (parameter
+    scans): their slices run side by side on the device, each on its own
+    stream (include/qpad_hip.h, qpad_batch_*).  Results equal each
+    simulation's own step3d."""
+
+    def __init__(self, sims, graph=False):
+        self.sims = list(sims)
+        self.lib = self.sims[0].lib
+        arr = (C.c_void_p * len(self.sims))(*[s_.ctx.value for s_ in self.sims])
+        h = C.c_void_p()
+        L.check(self.lib.qpad_batch_create(arr, len(self.sims), C.byref(h)))
+        self.h = h
+        self.mode(graph)
+
+    def mode(self, graph):
+        L.check(self.lib.qpad_batch_mode(self.h, int(bool(graph))))
+
+    def step3d(self):
+        its = np.zeros(len(self.sims), np.int32)
+        L.check(self.lib.qpad_batch_step3d(self.h, L.iptr(its)))
+        for s_ in self.sims:
+            s_.tstep += 1
+        return its
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.qpad_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
