@@ -2,6 +2,7 @@
 
 python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload c2|c1|c3]
                 [--seed learned|plain] [--train-steps T] [--cpu-slices S] [--no-cpu]
+                [--no-c4]
 
 A step is one 3D step of the workload: every xi-slice of the plasma response
 (with every B_perp predictor-corrector iteration), the beam deposit, the beam
@@ -16,7 +17,8 @@ online during the W + T untimed warm-up steps, frozen in the timed steps).
 Both time the same K steps (3D steps W+T+1 .. W+T+K of the same physics),
 interleaved step by step.
 `value` is the leg named by --seed (plain by default); the other leg is
-reported beside it.
+reported beside it.  `batch_c4` times the C4 scan (16 independent
+instances) stepped one after another and as one batch (eager and graph).
 """
 import argparse
 import json
@@ -47,6 +49,8 @@ def parse():
                     help="the leg reported as value (both legs always run)")
     ap.add_argument("--train-steps", type=int, default=2,
                     help="extra untimed 3D steps of online training of the seed network")
+    ap.add_argument("--no-c4", action="store_true",
+                    help="skip the batched-instances scan (C4) section")
     ap.add_argument("--profile-slices", type=int, default=0,
                     help="run only a 512 MiB calibration copy and this many slices of the workload "
                          "(for rocprofv3 counter runs), print nothing")
@@ -106,6 +110,49 @@ def algorithmic_bytes_per_step(grid, np_per_slice, iters, nbeam):
         b += np_per_slice * (88 + 48 * it) + 8 * nm * (grid.nr + 2) * 18
     b += nbeam * 136 + 8 * nm * (grid.nr + 2) * (grid.nxi + 1) * 7
     return b
+
+
+def batch_c4(reps=2):
+    """The C4 scan (16 independent instances of random Nr, modes, beam charge
+    and offset, Nxi 512, from a recorded seed): one untimed 3D step each,
+    then `reps` timed 3D steps of all 16 stepped one after another, as one
+    eager batch, as one graph batch (after the graphs are captured), and as
+    one graph batch after every instance's deposit launch shape is autotuned.
+    Instance 3D steps per second; every form gives the same results (the
+    batch tests)."""
+    from qpad_amd import Batch, Simulation, configs
+    decks, params = configs.c4()
+    sims = [Simulation(d) for d in decks]
+    for s_ in sims:
+        s_.step3d()
+
+    def timed(fn):
+        t0 = time.perf_counter()
+        out = [fn() for _ in range(reps)]
+        return (time.perf_counter() - t0) / reps, out
+
+    t_seq, its = timed(lambda: [s_.step3d() for s_ in sims])
+    batch = Batch(sims)
+    t_eager, _ = timed(batch.step3d)
+    batch.mode(True)
+    batch.step3d()
+    batch.step3d()          # both particle-buffer assignments captured
+    t_graph, _ = timed(batch.step3d)
+    shapes = batch.tune(reps=3)
+    batch.step3d()
+    batch.step3d()          # recaptured with the tuned shapes
+    t_tuned, _ = timed(batch.step3d)
+    batch.close()
+    nsl = sum(s_.grid.nxi for s_ in sims)
+    for s_ in sims:
+        s_.close()
+    n = len(sims)
+    return {"instances": n, "params": params, "iters_per_slice": sum(its[-1]) / nsl,
+            "sequential_instance_steps_per_s": n / t_seq, "batch_eager_instance_steps_per_s": n / t_eager,
+            "batch_graph_instance_steps_per_s": n / t_graph, "batch_graph_tuned_instance_steps_per_s": n / t_tuned,
+            "tuned_dep_cells": shapes, "batch_graph_tuned_speedup_vs_sequential": t_seq / t_tuned,
+            "ms_per_batch_step": {"sequential": 1e3 * t_seq, "eager": 1e3 * t_eager, "graph": 1e3 * t_graph,
+                                  "graph_tuned": 1e3 * t_tuned}}
 
 
 # ---------------------------------------------------------------------------
@@ -324,6 +371,8 @@ def main():
         "push_ms_per_step": push_ms / args.steps,
     }
     legs["learned"].close()
+    if not args.no_c4:
+        result["batch_c4"] = batch_c4()
     if not args.no_cpu:
         # the oracle is the plain method: replay window of the plain leg (re-run
         # its step up to j0 - 1 and snapshot there), scaled by its iterations

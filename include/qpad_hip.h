@@ -183,6 +183,19 @@ int qpad_step3d(qpad_ctx *ctx, int32_t *total_iters_out);
    time while the device reports no convergence (one host wait per further
    iteration, overlapped with that iteration's field solves). */
 int qpad_iteration_policy(qpad_ctx *ctx, int32_t adaptive, int32_t margin);
+/* Launch shapes of the dominant kernels (no counterpart in QPAD; results do
+   not depend on them): dep_cells = radial cells, one wave each, per
+   workgroup of the current and charge deposits (2, 4 or 8; default 4). */
+typedef struct {
+    int32_t dep_cells;
+} qpad_launch_shape;
+int qpad_get_launch_shape(qpad_ctx *ctx, qpad_launch_shape *out);
+int qpad_set_launch_shape(qpad_ctx *ctx, const qpad_launch_shape *shape);
+/* Autotuner: times every candidate shape on the context's current particles
+   and fields (reps launches each; the deposits only rewrite scratch partials,
+   so no result changes) and keeps the fastest.  ms_out: mean ms per
+   candidate dep_cells = 2, 4, 8 (3 entries) or NULL. */
+int qpad_tune_launch(qpad_ctx *ctx, int32_t reps, double *ms_out);
 
 /* state exchange with the host (diagnostics, restart, parity checks) */
 int qpad_get_field(qpad_ctx *ctx, int32_t fid, double *out);
@@ -320,6 +333,8 @@ int qpad_batch_destroy(qpad_batch *batch);
 int qpad_batch_mode(qpad_batch *batch, int32_t graph);
 /* one 3D step of every context; total_iters_out: n entries or NULL */
 int qpad_batch_step3d(qpad_batch *batch, int32_t *total_iters_out);
+/* qpad_tune_launch on every context (captured graphs are dropped) */
+int qpad_batch_tune(qpad_batch *batch, int32_t reps);
 
 #ifdef __cplusplus
 }

@@ -55,6 +55,10 @@ class SeedNetInfo(C.Structure):
                 ("nparams", C.c_int64), ("steps", C.c_int64)]
 
 
+class LaunchShape(C.Structure):
+    _fields_ = [("dep_cells", C.c_int32)]
+
+
 class BeamCfg(C.Structure):
     _fields_ = [("total_num", C.c_int64), ("total_charge", C.c_double), ("qm", C.c_double),
                 ("mass", C.c_double), ("gamma", C.c_double), ("center", C.c_double * 3),
@@ -92,6 +96,9 @@ SIGNATURES = {
     "qpad_step3d_end": (C.c_int, [_CTX]),
     "qpad_step3d": (C.c_int, [_CTX, C.POINTER(C.c_int32)]),
     "qpad_iteration_policy": (C.c_int, [_CTX, C.c_int32, C.c_int32]),
+    "qpad_get_launch_shape": (C.c_int, [_CTX, C.POINTER(LaunchShape)]),
+    "qpad_set_launch_shape": (C.c_int, [_CTX, C.POINTER(LaunchShape)]),
+    "qpad_tune_launch": (C.c_int, [_CTX, C.c_int32, _dp]),
     "qpad_get_field": (C.c_int, [_CTX, C.c_int32, _dp]),
     "qpad_set_field": (C.c_int, [_CTX, C.c_int32, _dp]),
     "qpad_get_hist": (C.c_int, [_CTX, C.c_int32, C.c_int32, _dp]),
@@ -119,6 +126,7 @@ SIGNATURES = {
     "qpad_batch_destroy": (C.c_int, [C.c_void_p]),
     "qpad_batch_mode": (C.c_int, [C.c_void_p, C.c_int32]),
     "qpad_batch_step3d": (C.c_int, [C.c_void_p, _ip]),
+    "qpad_batch_tune": (C.c_int, [C.c_void_p, C.c_int32]),
 }
 
 _lib = None

@@ -182,13 +182,16 @@ struct QDepArgs {
     double *P0, *P1;
 };
 
-template <int M>
-__global__ __launch_bounds__(64 * kDepWaves) void k_qdeposit(QDepArgs a)
+// CPW: radial cells (one wave each) per workgroup, a launch shape the
+// tuner chooses (qpad_tune_launch); every wave's work and result are the
+// same for every shape
+template <int M, int CPW>
+__global__ __launch_bounds__(64 * CPW) void k_qdeposit(QDepArgs a)
 {
     constexpr int NM = 2 * M + 1;
     const int lane = threadIdx.x & 63;
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-    const int c = blockIdx.x * kDepWaves + wave;
+    const int c = blockIdx.x * CPW + wave;
     if (c >= a.nr) return;
     const int p0 = a.cs[c], p1 = a.cs[c + 1];
     // acc[k]: node ix, acc[NM + k]: node ix + 1; padded to a multiple of 16
@@ -237,14 +240,24 @@ int launch_qdeposit(qpad_ctx *c, int s)
     a.cs = sp.cell_start;
     a.P0 = sp.P0;
     a.P1 = sp.P1;
-    dim3 grid((c->g.nr + kDepWaves - 1) / kDepWaves), blk(64 * kDepWaves);
+    const int cpw = c->dep_cells;
+    dim3 grid((c->g.nr + cpw - 1) / cpw), blk(64 * cpw);
+#define QPAD_QDEP(MM, CC) hipLaunchKernelGGL((k_qdeposit<MM, CC>), grid, blk, 0, c->stream, a)
+#define QPAD_QDEP_M(MM)                              \
+    do {                                             \
+        if (cpw == 2) QPAD_QDEP(MM, 2);              \
+        else if (cpw == 8) QPAD_QDEP(MM, 8);         \
+        else QPAD_QDEP(MM, 4);                       \
+    } while (0)
     switch (c->g.M) {
-    case 0: hipLaunchKernelGGL(k_qdeposit<0>, grid, blk, 0, c->stream, a); break;
-    case 1: hipLaunchKernelGGL(k_qdeposit<1>, grid, blk, 0, c->stream, a); break;
-    case 2: hipLaunchKernelGGL(k_qdeposit<2>, grid, blk, 0, c->stream, a); break;
-    case 3: hipLaunchKernelGGL(k_qdeposit<3>, grid, blk, 0, c->stream, a); break;
-    default: hipLaunchKernelGGL(k_qdeposit<4>, grid, blk, 0, c->stream, a); break;
+    case 0: QPAD_QDEP_M(0); break;
+    case 1: QPAD_QDEP_M(1); break;
+    case 2: QPAD_QDEP_M(2); break;
+    case 3: QPAD_QDEP_M(3); break;
+    default: QPAD_QDEP_M(4); break;
     }
+#undef QPAD_QDEP_M
+#undef QPAD_QDEP
     QPAD_HIP_CHECK(hipGetLastError());
     return 0;
 }
@@ -428,17 +441,17 @@ __device__ __forceinline__ void deposit_particle(const JDepArgs &a, const NodePa
 // SPEC: a speculative launch of iteration >= 2 of a slice, which returns at
 // once when the iteration has already converged (a separate instance, so
 // that profiles show the first-iteration launches, which always work, apart)
-template <int M, bool SPEC>
-__global__ __launch_bounds__(64 * kDepWaves) void k_amjdeposit(JDepArgs a)
+template <int M, bool SPEC, int CPW>
+__global__ __launch_bounds__(64 * CPW) void k_amjdeposit(JDepArgs a)
 {
     constexpr int NM = 2 * M + 1;
     constexpr int NV = 8 * NM;
     constexpr int NF = 12 * NM;   // staged field values per cell
     // E and B at the two nodes of each wave's cell: [wave][E|B][n][cc][k]
-    __shared__ double sf[kDepWaves][2][2 * 3 * NM];
+    __shared__ double sf[CPW][2][2 * 3 * NM];
     const int lane = threadIdx.x & 63;
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-    const int c = blockIdx.x * kDepWaves + wave;
+    const int c = blockIdx.x * CPW + wave;
     const int cc = min(c, a.nr - 1);
     const int nm = a.nm, ld = a.ld;
     const int ix = cc + 1;
@@ -553,13 +566,20 @@ int launch_amjdeposit(qpad_ctx *c, int s, int l, bool store_diag, bool timed, bo
     a.reltol = c->g.reltol;
     a.abstol = c->g.abstol;
     a.report = (report && speculative) ? c->it_report : nullptr;
-    dim3 grid((c->g.nr + kDepWaves - 1) / kDepWaves), blk(64 * kDepWaves);
+    const int cpw = c->dep_cells;
+    dim3 grid((c->g.nr + cpw - 1) / cpw), blk(64 * cpw);
     hipEvent_t stop = nullptr;
     if (timed) time_begin(c, 0, &stop);
-#define QPAD_DEP(MM)                                                                               \
-    do {                                                                                           \
-        if (speculative) hipLaunchKernelGGL((k_amjdeposit<MM, true>), grid, blk, 0, c->stream, a); \
-        else hipLaunchKernelGGL((k_amjdeposit<MM, false>), grid, blk, 0, c->stream, a);            \
+#define QPAD_DEP2(MM, CC)                                                                              \
+    do {                                                                                               \
+        if (speculative) hipLaunchKernelGGL((k_amjdeposit<MM, true, CC>), grid, blk, 0, c->stream, a); \
+        else hipLaunchKernelGGL((k_amjdeposit<MM, false, CC>), grid, blk, 0, c->stream, a);            \
+    } while (0)
+#define QPAD_DEP(MM)                                 \
+    do {                                             \
+        if (cpw == 2) QPAD_DEP2(MM, 2);              \
+        else if (cpw == 8) QPAD_DEP2(MM, 8);         \
+        else QPAD_DEP2(MM, 4);                       \
     } while (0)
     switch (c->g.M) {
     case 0: QPAD_DEP(0); break;
@@ -569,6 +589,7 @@ int launch_amjdeposit(qpad_ctx *c, int s, int l, bool store_diag, bool timed, bo
     default: QPAD_DEP(4); break;
     }
 #undef QPAD_DEP
+#undef QPAD_DEP2
     QPAD_HIP_CHECK(hipGetLastError());
     if (timed) time_end(c, 0, stop);
     return 0;

@@ -319,6 +319,7 @@ int qpad_create(const qpad_grid *gr, qpad_ctx **out)
     c->adaptive = 1;
     c->spec_margin = 0;
     c->spec_guess = 2;
+    c->dep_cells = kDepWaves;
     if (rc) {
         qpad_destroy(c);
         return rc;
@@ -657,6 +658,67 @@ int qpad_step3d(qpad_ctx *c, int32_t *total_iters_out)
     TRY(qpad_step3d_end(c));
     if (total_iters_out) *total_iters_out = tot;
     return 0;
+}
+
+int qpad_get_launch_shape(qpad_ctx *c, qpad_launch_shape *out)
+{
+    if (!c || !out) return fail(QPAD_E_INVALID, "null argument");
+    out->dep_cells = c->dep_cells;
+    return 0;
+}
+
+int qpad_set_launch_shape(qpad_ctx *c, const qpad_launch_shape *shape)
+{
+    if (!c || !shape) return fail(QPAD_E_INVALID, "null argument");
+    if (shape->dep_cells != 2 && shape->dep_cells != 4 && shape->dep_cells != 8)
+        return fail(QPAD_E_INVALID, "dep_cells must be 2, 4 or 8");
+    c->dep_cells = shape->dep_cells;
+    return 0;
+}
+
+// Every candidate shape runs the species' current-deposit (first-iteration
+// instance, no gamma/psi stores) and charge-deposit kernels on the current
+// particles and fields, `reps` timed launches each after one untimed.  Both
+// kernels only write the per-cell partials, which every slice recomputes
+// before use, so tuning changes no result.
+int qpad_tune_launch(qpad_ctx *c, int32_t reps, double *ms_out)
+{
+    if (!c) return fail(QPAD_E_INVALID, "null context");
+    if (reps < 1 || reps > 1000) return fail(QPAD_E_INVALID, "reps must be 1..1000");
+    TRY(seednet_quiesce(c));
+    TRY(sync(c));
+    const int cand[3] = {2, 4, 8};
+    const int saved = c->dep_cells;
+    double best = 0.0;
+    int best_c = saved;
+    hipEvent_t e0, e1;
+    QPAD_HIP_CHECK(hipEventCreate(&e0));
+    QPAD_HIP_CHECK(hipEventCreate(&e1));
+    int rc = 0;
+    for (int k = 0; k < 3 && !rc; k++) {
+        c->dep_cells = cand[k];
+        for (int r = 0; r <= reps && !rc; r++) {
+            if (r == 1 && hipEventRecord(e0, c->stream) != hipSuccess) rc = fail(QPAD_E_HIP, "hipEventRecord");
+            for (int s = 0; s < c->nspecies && !rc; s++) {
+                rc = launch_amjdeposit(c, s, 1, false, false);
+                if (!rc) rc = launch_qdeposit(c, s);
+            }
+        }
+        float ms = 0.0f;
+        if (!rc && (hipEventRecord(e1, c->stream) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
+                    hipEventElapsedTime(&ms, e0, e1) != hipSuccess))
+            rc = fail(QPAD_E_HIP, "tuning events failed");
+        const double per = (double)ms / reps;
+        if (ms_out) ms_out[k] = per;
+        if (!rc && (k == 0 || per < best)) {
+            best = per;
+            best_c = cand[k];
+        }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    c->dep_cells = rc ? saved : best_c;
+    return rc;
 }
 
 int qpad_iteration_policy(qpad_ctx *c, int32_t adaptive, int32_t margin)
@@ -1068,6 +1130,16 @@ int qpad_batch_destroy(qpad_batch *b)
     for (qpad_ctx *c : b->ctx) (void)hipStreamSynchronize(c->stream);
     batch_drop_graphs(b);
     delete b;
+    return 0;
+}
+
+// tune every context's launch shape (qpad_tune_launch); captured graphs are
+// dropped (they hold the old shapes)
+int qpad_batch_tune(qpad_batch *b, int32_t reps)
+{
+    if (!b) return fail(QPAD_E_INVALID, "null batch");
+    for (qpad_ctx *c : b->ctx) TRY(qpad_tune_launch(c, reps, nullptr));
+    batch_drop_graphs(b);
     return 0;
 }
 

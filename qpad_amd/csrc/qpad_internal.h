@@ -21,7 +21,7 @@ constexpr int kMaxNm = 2 * kMaxMode + 1;
 constexpr int kMaxSpecies = 4;
 constexpr int kMaxBeams = 4;
 constexpr int kSolveThreads = 512;   // workgroup of the tridiagonal solves
-constexpr int kDepWaves = 4;         // waves (cells) per deposit workgroup
+constexpr int kDepWaves = 4;         // default radial cells (waves) per deposit workgroup
 
 // field kinds (source/param.f03:72-80)
 enum { FK_PSI = 0, FK_EZ = 1, FK_BZ = 2, FK_BT = 3, FK_BPLUS = 4, FK_BMINUS = 5, FK_N = 6 };
@@ -110,6 +110,7 @@ struct qpad_ctx {
     qpad::IterState *it_report;
     hipEvent_t dec_ev;
     int adaptive, spec_margin, spec_guess;
+    int dep_cells;            // radial cells per deposit workgroup (2, 4, 8)
     int nspecies, nbeams;
     qpad::Species sp[qpad::kMaxSpecies];
     qpad::Beam bm[qpad::kMaxBeams];

@@ -69,6 +69,10 @@ module qpad_hip_mod
     integer(c_int64_t) :: nparams, steps
   end type qpad_seednet_info
 
+  type, bind(C), public :: qpad_launch_shape
+    integer(c_int32_t) :: dep_cells
+  end type qpad_launch_shape
+
   public :: qpad_hip_check, qpad_error_message
 
   interface
@@ -374,6 +378,35 @@ module qpad_hip_mod
     end function
 
     ! ctxs: c_loc of a type(c_ptr) array of n contexts; batch: the new handle
+    function qpad_get_launch_shape(ctx, shape) bind(C, name="qpad_get_launch_shape") result(rc)
+      import :: c_int, c_ptr, qpad_launch_shape
+      type(c_ptr), value :: ctx
+      type(qpad_launch_shape), intent(out) :: shape
+      integer(c_int) :: rc
+    end function
+
+    function qpad_set_launch_shape(ctx, shape) bind(C, name="qpad_set_launch_shape") result(rc)
+      import :: c_int, c_ptr, qpad_launch_shape
+      type(c_ptr), value :: ctx
+      type(qpad_launch_shape), intent(in) :: shape
+      integer(c_int) :: rc
+    end function
+
+    ! ms_out: c_loc of real(c_double) :: ms(3), or c_null_ptr
+    function qpad_tune_launch(ctx, reps, ms_out) bind(C, name="qpad_tune_launch") result(rc)
+      import :: c_int, c_int32_t, c_ptr
+      type(c_ptr), value :: ctx, ms_out
+      integer(c_int32_t), value :: reps
+      integer(c_int) :: rc
+    end function
+
+    function qpad_batch_tune(batch, reps) bind(C, name="qpad_batch_tune") result(rc)
+      import :: c_int, c_int32_t, c_ptr
+      type(c_ptr), value :: batch
+      integer(c_int32_t), value :: reps
+      integer(c_int) :: rc
+    end function
+
     function qpad_batch_create(ctxs, n, batch) bind(C, name="qpad_batch_create") result(rc)
       import :: c_int, c_int32_t, c_ptr
       type(c_ptr), value :: ctxs
@@ -485,7 +518,8 @@ module qpad_hip_mod
     qpad_set_beam, qpad_kernel_time, qpad_kernel_time_reset, qpad_iteration_policy, &
     qpad_seednet_create, qpad_seednet_get_info, qpad_seednet_mode, qpad_seednet_get_params, &
     qpad_seednet_set_params, qpad_seednet_losses, qpad_seednet_seed, qpad_seednet_learn, &
-    qpad_seednet_eval, qpad_batch_create, qpad_batch_destroy, qpad_batch_mode, qpad_batch_step3d
+    qpad_seednet_eval, qpad_batch_create, qpad_batch_destroy, qpad_batch_mode, qpad_batch_step3d, &
+    qpad_get_launch_shape, qpad_set_launch_shape, qpad_tune_launch, qpad_batch_tune
 
 contains
 

@@ -303,6 +303,22 @@ class Simulation:
         (results do not depend on it; include/qpad_hip.h)."""
         L.check(self.lib.qpad_iteration_policy(self.ctx, int(bool(adaptive)), int(margin)))
 
+    def launch_shape(self, dep_cells=None):
+        """Get (or set) the deposit workgroup shape (include/qpad_hip.h)."""
+        sh = L.LaunchShape()
+        if dep_cells is not None:
+            sh.dep_cells = int(dep_cells)
+            L.check(self.lib.qpad_set_launch_shape(self.ctx, C.byref(sh)))
+        L.check(self.lib.qpad_get_launch_shape(self.ctx, C.byref(sh)))
+        return sh.dep_cells
+
+    def tune_launch(self, reps=5):
+        """Autotune the launch shape on the current state; returns the mean
+        ms per candidate dep_cells (2, 4, 8) and the choice."""
+        ms = np.zeros(3)
+        L.check(self.lib.qpad_tune_launch(self.ctx, int(reps), L.dptr(ms)))
+        return dict(zip((2, 4, 8), ms.tolist())), self.launch_shape()
+
     def convergence_tester(self, op):
         """'record' or 'compare' on b_spe component 2 (simulation_class.f03:522-606)."""
         rel, ab = C.c_double(0.0), C.c_double(0.0)
@@ -353,6 +369,11 @@ class Batch:
 
     def mode(self, graph):
         L.check(self.lib.qpad_batch_mode(self.h, int(bool(graph))))
+
+    def tune(self, reps=5):
+        """Autotune every instance's launch shape; returns the choices."""
+        L.check(self.lib.qpad_batch_tune(self.h, int(reps)))
+        return [s_.launch_shape() for s_ in self.sims]
 
     def step3d(self):
         its = np.zeros(len(self.sims), np.int32)

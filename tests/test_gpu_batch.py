@@ -63,6 +63,53 @@ def test_batch_matches_instances_stepped_alone(graph):
         sim.close()
 
 
+def test_launch_shapes_and_tuner_change_no_result():
+    """Every deposit workgroup shape gives the same run bit for bit, and the
+    tuner (timed deposits on the live state between steps) changes nothing."""
+    d = scan_decks()[1]
+    ref = Simulation(d)
+    its_ref = [ref.step3d() for _ in range(2)]
+    st_ref = state(ref)
+    ref.close()
+    for cells in (2, 8):
+        sim = Simulation(d)
+        assert sim.launch_shape(cells) == cells
+        its = [sim.step3d() for _ in range(2)]
+        assert its == its_ref, (cells, its, its_ref)
+        assert_same(state(sim), st_ref, f"dep_cells={cells}")
+        sim.close()
+    sim = Simulation(d)
+    its = [sim.step3d()]
+    ms, choice = sim.tune_launch(reps=2)
+    assert choice in (2, 4, 8) and all(v > 0 for v in ms.values()), (ms, choice)
+    its.append(sim.step3d())
+    assert its == its_ref
+    assert_same(state(sim), st_ref, "tuned between steps")
+    with pytest.raises(Exception, match="dep_cells"):
+        sim.launch_shape(3)
+    sim.close()
+
+
+def test_tuned_graph_batch_matches_alone():
+    ref = []
+    for d in scan_decks():
+        sim = Simulation(d)
+        ref.append([(sim.step3d(), state(sim)) for _ in range(2)])
+        sim.close()
+    sims = [Simulation(d) for d in scan_decks()]
+    batch = Batch(sims, graph=True)
+    for step in range(2):
+        if step == 1:
+            assert all(c in (2, 4, 8) for c in batch.tune(reps=2))
+        its = batch.step3d()
+        for i, sim in enumerate(sims):
+            assert its[i] == ref[i][step][0]
+            assert_same(state(sim), ref[i][step][1], f"tuned batch instance {i} step {step}")
+    batch.close()
+    for sim in sims:
+        sim.close()
+
+
 def test_graph_mode_needs_even_nxi():
     deck = configs.small(nr=48, nxi=33, max_mode=1, ppc=(2, 4), total_num=1 << 10)
     sim = Simulation(deck)
