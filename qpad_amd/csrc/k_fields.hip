@@ -26,6 +26,8 @@ struct PsiBzArgs {
     IterState *it;
     int nspecies;
     const double *P0[kMaxSpecies], *P1[kMaxSpecies], *qn[kMaxSpecies];
+    const double *O0[kMaxSpecies], *O1[kMaxSpecies];
+    const int32_t *os[kMaxSpecies];
     int do_psi, do_bz, do_record;
     int from_partials;   // 1: assemble q_spe from the deposit partials first
 };
@@ -37,8 +39,8 @@ __device__ inline double q_node(const PsiBzArgs &a, int k, int i)
     const int nm = a.g.nm, ld = a.g.ld, nr = a.g.nr;
     double v = 0.0;
     for (int s = 0; s < a.nspecies; s++) {
-        if (i >= 1 && i <= nr) v += a.P0[s][(size_t)(i - 1) * nm + k];
-        if (i >= 2) v += a.P1[s][(size_t)(i - 2) * nm + k];
+        if (i >= 1 && i <= nr) v += cell_partial(a.P0[s], a.O0[s], a.os[s], i - 1, nm, k);
+        if (i >= 2) v += cell_partial(a.P1[s], a.O1[s], a.os[s], i - 2, nm, k);
     }
     if (i == 0) v = 0.0;
     else if (i == 1) v = (k == 0) ? 8.0 * v : 0.0;
@@ -154,6 +156,9 @@ static PsiBzArgs psibz_base(qpad_ctx *c)
         a.P0[s] = s < c->nspecies ? c->sp[s].P0 : nullptr;
         a.P1[s] = s < c->nspecies ? c->sp[s].P1 : nullptr;
         a.qn[s] = (s < c->nspecies && c->sp[s].cfg.neutralized) ? c->sp[s].qn : nullptr;
+        a.O0[s] = s < c->nspecies ? c->sp[s].O0 : nullptr;
+        a.O1[s] = s < c->nspecies ? c->sp[s].O1 : nullptr;
+        a.os[s] = s < c->nspecies ? c->sp[s].ov_start : nullptr;
     }
     return a;
 }
@@ -424,6 +429,8 @@ struct CombineJArgs {
     double *cu, *acu, *amu;
     int nspecies;
     const double *P0[kMaxSpecies], *P1[kMaxSpecies];
+    const double *O0[kMaxSpecies], *O1[kMaxSpecies];
+    const int32_t *os[kMaxSpecies];
     const IterState *it;
     int check_done;
     int accumulate;
@@ -442,8 +449,8 @@ __global__ __launch_bounds__(256) void k_combine_j(CombineJArgs a)
     const int comp = v / nm, k = v % nm, m = mode_of(k);
     double val = 0.0;
     for (int s = 0; s < a.nspecies; s++) {
-        if (i >= 1 && i <= nr) val += a.P0[s][(size_t)(i - 1) * NV + v];
-        if (i >= 2) val += a.P1[s][(size_t)(i - 2) * NV + v];
+        if (i >= 1 && i <= nr) val += cell_partial(a.P0[s], a.O0[s], a.os[s], i - 1, NV, v);
+        if (i >= 2) val += cell_partial(a.P1[s], a.O1[s], a.os[s], i - 2, NV, v);
     }
     if (i == 0) {
         val = 0.0;
@@ -492,6 +499,9 @@ int launch_combine_j(qpad_ctx *c, bool check_done)
     for (int s = 0; s < kMaxSpecies; s++) {
         a.P0[s] = s < c->nspecies ? c->sp[s].P0 : nullptr;
         a.P1[s] = s < c->nspecies ? c->sp[s].P1 : nullptr;
+        a.O0[s] = s < c->nspecies ? c->sp[s].O0 : nullptr;
+        a.O1[s] = s < c->nspecies ? c->sp[s].O1 : nullptr;
+        a.os[s] = s < c->nspecies ? c->sp[s].ov_start : nullptr;
     }
     a.check_done = check_done;
     return run_combine(c, a);
@@ -503,6 +513,9 @@ int launch_combine_j_species(qpad_ctx *c, int sp)
     a.nspecies = 1;
     a.P0[0] = c->sp[sp].P0;
     a.P1[0] = c->sp[sp].P1;
+    a.O0[0] = c->sp[sp].O0;
+    a.O1[0] = c->sp[sp].O1;
+    a.os[0] = c->sp[sp].ov_start;
     a.accumulate = 1;
     return run_combine(c, a);
 }

@@ -3,10 +3,13 @@
 // current/acceleration/momentum-flux deposit, and the fused push_u + push_x.
 //
 // Particles are kept sorted by radial cell (cell c = int(r/dr), nodes c+1 and
-// c+2).  Deposits run one wavefront per cell: each lane accumulates its
-// particles in registers, the wavefront reduces with a fixed butterfly and
-// writes per-cell partials P0 (node c+1) and P1 (node c+2) once; the field
-// kernels add P0[i-1] + P1[i-2] per node.  No atomics, deterministic order.
+// c+2).  Deposits run one wavefront per chunk of at most kDepChunk particles
+// of a cell: each lane accumulates its particles in registers, the wavefront
+// reduces with a fixed butterfly and writes the chunk's partials for node c+1
+// (P0) and node c+2 (P1) once.  A cell's first chunk goes to P0/P1[c], its
+// further chunks (crowded cells only) to the overflow rows O0/O1 the sort
+// lists; the field kernels add P0[i-1] + P1[i-2] per node, each with its
+// cell's overflow rows in row order.  No atomics, deterministic order.
 #include <algorithm>
 #include <cstring>
 
@@ -180,20 +183,78 @@ struct QDepArgs {
     const double *x1, *x2, *q;
     const int32_t *cs;
     double *P0, *P1;
+    // overflow chunks (listed by the sort) and their partial rows
+    const int32_t *os;
+    const int4 *desc;
+    double *O0, *O1;
+    int n_ov_max;
 };
 
-// CPW: radial cells (one wave each) per workgroup, a launch shape the
-// tuner chooses (qpad_tune_launch); every wave's work and result are the
-// same for every shape
+// One wave per chunk of at most kDepChunk particles of one cell.  The first
+// ceil(nr / CPW) workgroups take the first chunk of cell (workgroup * CPW +
+// wave) into P0/P1; the workgroups after them take the overflow chunks
+// (row = (workgroup - ceil(nr / CPW)) * CPW + wave) into O0/O1.
+// CPW: waves per workgroup, a launch shape the tuner chooses
+// (qpad_tune_launch); every wave's work and result are the same for every
+// shape.
+struct DepChunk {
+    int cell, pb, pe, row;
+    bool ovf, live;
+};
+
+template <int CPW>
+__device__ __forceinline__ DepChunk dep_chunk(int nr, const int32_t *cs, const int32_t *os, const int4 *desc,
+                                              int n_ov_max, int wave)
+{
+    DepChunk k;
+    const int ncg = (nr + CPW - 1) / CPW;
+    k.ovf = (int)blockIdx.x >= ncg;
+    if (!k.ovf) {
+        const int c = blockIdx.x * CPW + wave;
+        k.live = c < nr;
+        k.cell = min(c, nr - 1);
+        k.row = k.cell;
+        const int p0 = cs[k.cell], p1 = cs[k.cell + 1];
+        k.pb = p0;
+        k.pe = min(p1, p0 + kDepChunk);
+    } else {
+        const int o = ((int)blockIdx.x - ncg) * CPW + wave;
+        const int nov = os[nr];
+        const int4 d = desc[min(o, n_ov_max - 1)];
+        k.live = o < nov;
+        k.cell = min(max(d.x, 0), nr - 1);
+        k.row = min(o, n_ov_max - 1);
+        k.pb = d.y;
+        k.pe = k.live ? d.z : d.y;
+    }
+    return k;
+}
+
+// overflow workgroups none of whose waves has a chunk (the grid is sized
+// for the most chunks the particle count allows); uniform over the workgroup
+template <int CPW>
+__device__ __forceinline__ bool dep_group_idle(int nr, const int32_t *os)
+{
+    const int ncg = (nr + CPW - 1) / CPW;
+    return (int)blockIdx.x >= ncg && ((int)blockIdx.x - ncg) * CPW >= os[nr];
+}
+
+// overflow rows the grid provides for (at most) npp particles in the box
+inline int dep_grid(int nr, int64_t npp, int n_ov_max, int cpw)
+{
+    const int nov = (int)std::min<int64_t>(npp / kDepChunk + 1, n_ov_max);
+    return (nr + cpw - 1) / cpw + (nov + cpw - 1) / cpw;
+}
+
 template <int M, int CPW>
 __global__ __launch_bounds__(64 * CPW) void k_qdeposit(QDepArgs a)
 {
     constexpr int NM = 2 * M + 1;
     const int lane = threadIdx.x & 63;
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-    const int c = blockIdx.x * CPW + wave;
-    if (c >= a.nr) return;
-    const int p0 = a.cs[c], p1 = a.cs[c + 1];
+    const DepChunk ch = dep_chunk<CPW>(a.nr, a.cs, a.os, a.desc, a.n_ov_max, wave);
+    if (!ch.live) return;
+    const int c = ch.cell, p0 = ch.pb, p1 = ch.pe;
     // acc[k]: node ix, acc[NM + k]: node ix + 1; padded to a multiple of 16
     constexpr int NB = (2 * NM + 15) / 16;
     double acc[16 * NB];
@@ -220,12 +281,13 @@ __global__ __launch_bounds__(64 * CPW) void k_qdeposit(QDepArgs a)
     }
     wave_reduce_scatter<NB>(acc);
     const int g = lane >> 2;
+    double *D0 = ch.ovf ? a.O0 : a.P0, *D1 = ch.ovf ? a.O1 : a.P1;
 #pragma unroll
     for (int i = 0; i < NB; i++) {
         const int v = g * NB + i;
         if ((lane & 3) == (i & 3) && v < 2 * NM) {
-            if (v < NM) a.P0[(size_t)c * NM + v] = acc[i];
-            else a.P1[(size_t)c * NM + v - NM] = acc[i];
+            if (v < NM) D0[(size_t)ch.row * NM + v] = acc[i];
+            else D1[(size_t)ch.row * NM + v - NM] = acc[i];
         }
     }
 }
@@ -240,8 +302,13 @@ int launch_qdeposit(qpad_ctx *c, int s)
     a.cs = sp.cell_start;
     a.P0 = sp.P0;
     a.P1 = sp.P1;
+    a.os = sp.ov_start;
+    a.desc = sp.ov_desc;
+    a.O0 = sp.O0;
+    a.O1 = sp.O1;
+    a.n_ov_max = sp.n_ov_max;
     const int cpw = c->dep_cells;
-    dim3 grid((c->g.nr + cpw - 1) / cpw), blk(64 * cpw);
+    dim3 grid(dep_grid(c->g.nr, sp.npp, sp.n_ov_max, cpw)), blk(64 * cpw);
 #define QPAD_QDEP(MM, CC) hipLaunchKernelGGL((k_qdeposit<MM, CC>), grid, blk, 0, c->stream, a)
 #define QPAD_QDEP_M(MM)                              \
     do {                                             \
@@ -263,15 +330,15 @@ int launch_qdeposit(qpad_ctx *c, int s)
 }
 
 // one species' q (axis rules, 1/r) times `scale` into out (1 x nm x ld)
-__global__ void k_combine_q(const double *P0, const double *P1, double *out, int nr, int nm, int ld,
-                            double scale, int accumulate)
+__global__ void k_combine_q(const double *P0, const double *P1, const double *O0, const double *O1,
+                            const int32_t *os, double *out, int nr, int nm, int ld, double scale, int accumulate)
 {
     int64_t tid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (tid >= (int64_t)nm * (nr + 2)) return;
     int i = (int)(tid % (nr + 2)), k = (int)(tid / (nr + 2));
     double v = 0.0;
-    if (i >= 1 && i <= nr) v += P0[(size_t)(i - 1) * nm + k];
-    if (i >= 2) v += P1[(size_t)(i - 2) * nm + k];
+    if (i >= 1 && i <= nr) v += cell_partial(P0, O0, os, i - 1, nm, k);
+    if (i >= 2) v += cell_partial(P1, O1, os, i - 2, nm, k);
     if (i == 0) v = 0.0;
     else if (i == 1) v = (k == 0) ? 8.0 * v : 0.0;
     else v = v * (1.0 / (double)(i - 1));
@@ -282,8 +349,9 @@ __global__ void k_combine_q(const double *P0, const double *P1, double *out, int
 int launch_combine_q(qpad_ctx *c, int s, double *out, double scale, bool accumulate)
 {
     int64_t n = (int64_t)c->g.nm * (c->g.nr + 2);
-    hipLaunchKernelGGL(k_combine_q, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, c->stream, c->sp[s].P0,
-                       c->sp[s].P1, out, c->g.nr, c->g.nm, c->g.ld, scale, accumulate ? 1 : 0);
+    const Species &sp = c->sp[s];
+    hipLaunchKernelGGL(k_combine_q, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, c->stream, sp.P0, sp.P1,
+                       sp.O0, sp.O1, sp.ov_start, out, c->g.nr, c->g.nm, c->g.ld, scale, accumulate ? 1 : 0);
     QPAD_HIP_CHECK(hipGetLastError());
     return 0;
 }
@@ -357,6 +425,11 @@ struct JDepArgs {
     // host-visible copy of the decision (done, iterations so far) written by
     // workgroup 0 of a SPEC launch, or null
     IterState *report;
+    // overflow chunks (listed by the sort) and their partial rows
+    const int32_t *os;
+    const int4 *desc;
+    double *O0, *O1;
+    int n_ov_max;
 };
 
 // robust predictor and current deposit of one particle of cell c into this
@@ -451,10 +524,11 @@ __global__ __launch_bounds__(64 * CPW) void k_amjdeposit(JDepArgs a)
     __shared__ double sf[CPW][2][2 * 3 * NM];
     const int lane = threadIdx.x & 63;
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-    const int c = blockIdx.x * CPW + wave;
-    const int cc = min(c, a.nr - 1);
+    if (dep_group_idle<CPW>(a.nr, a.os)) return;
+    const DepChunk ch = dep_chunk<CPW>(a.nr, a.cs, a.os, a.desc, a.n_ov_max, wave);
+    const int c = ch.cell;
     const int nm = a.nm, ld = a.ld;
-    const int ix = cc + 1;
+    const int ix = c + 1;
     // every independent load of the prologue first (one round trip): the
     // convergence state, the cell's particle range and its field values
     int done = 0;
@@ -465,7 +539,7 @@ __global__ __launch_bounds__(64 * CPW) void k_amjdeposit(JDepArgs a)
         iters_now = a.it->iters;
         conv_load(a.conv_prev, a.nblk, co, cd);
     }
-    const int p0 = a.cs[cc], p1 = a.cs[cc + 1];
+    const int p0 = ch.pb, p1 = ch.pe;
     double fv[(NF + 63) / 64];
 #pragma unroll
     for (int u = 0; u < (NF + 63) / 64; u++) {
@@ -498,7 +572,7 @@ __global__ __launch_bounds__(64 * CPW) void k_amjdeposit(JDepArgs a)
         if (l < NF) sf[wave][l / (6 * NM)][l % (6 * NM)] = fv[u];
     }
     __syncthreads();
-    if (c >= a.nr) return;
+    if (!ch.live) return;
     const NodePairLds<NM> fe{sf[wave][0]}, fbf{sf[wave][1]};
     const double *__restrict__ X1 = a.x1;
     const double *__restrict__ X2 = a.x2;
@@ -526,12 +600,13 @@ __global__ __launch_bounds__(64 * CPW) void k_amjdeposit(JDepArgs a)
     }
     wave_reduce_scatter<NM>(acc);
     const int g = lane >> 2;
+    double *D0 = ch.ovf ? a.O0 : a.P0, *D1 = ch.ovf ? a.O1 : a.P1;
 #pragma unroll
     for (int i = 0; i < NM; i++) {
         const int v = g * NM + i;
         if ((lane & 3) == (i & 3)) {
-            if (v < NV) a.P0[(size_t)c * NV + v] = acc[i];
-            else a.P1[(size_t)c * NV + v - NV] = acc[i];
+            if (v < NV) D0[(size_t)ch.row * NV + v] = acc[i];
+            else D1[(size_t)ch.row * NV + v - NV] = acc[i];
         }
     }
 }
@@ -566,8 +641,13 @@ int launch_amjdeposit(qpad_ctx *c, int s, int l, bool store_diag, bool timed, bo
     a.reltol = c->g.reltol;
     a.abstol = c->g.abstol;
     a.report = (report && speculative) ? c->it_report : nullptr;
+    a.os = sp.ov_start;
+    a.desc = sp.ov_desc;
+    a.O0 = sp.O0;
+    a.O1 = sp.O1;
+    a.n_ov_max = sp.n_ov_max;
     const int cpw = c->dep_cells;
-    dim3 grid((c->g.nr + cpw - 1) / cpw), blk(64 * cpw);
+    dim3 grid(dep_grid(c->g.nr, sp.npp, sp.n_ov_max, cpw)), blk(64 * cpw);
     hipEvent_t stop = nullptr;
     if (timed) time_begin(c, 0, &stop);
 #define QPAD_DEP2(MM, CC)                                                                              \

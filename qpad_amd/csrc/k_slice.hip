@@ -35,6 +35,9 @@ __host__ __device__ inline int src_rows(int M, int nm) { return 4 * (M + 1) + 3 
 struct SrcArgs {
     FieldCommon g;
     const double *P0[kMaxSpecies], *P1[kMaxSpecies];
+    // overflow rows of crowded cells (cell_partial)
+    const double *O0[kMaxSpecies], *O1[kMaxSpecies];
+    const int32_t *os[kMaxSpecies];
     int ns;
     double *cu_out, *acu_out, *amu_out, *dcu;
     const double *b_spe;
@@ -64,12 +67,14 @@ struct TileArr {
 // (part2d_class.f03:914-1004).  P0[cell][v] is the deposit of cell `cell`
 // on its inner node, P1 on its outer node, v = comp8 * nm + k.  Split in an
 // address step (clamped, so the loads are always valid) and a combine step.
-__device__ __forceinline__ void partial_addr(const FieldCommon &g, int comp, int k, int i, size_t &c0,
-                                             size_t &c1)
+__device__ __forceinline__ void partial_addr(const FieldCommon &g, int comp, int k, int i, int &cell0,
+                                             int &cell1, size_t &c0, size_t &c1)
 {
     const int nm = g.nm, nr = g.nr, NV = 8 * nm, v = comp * nm + k;
-    c0 = (size_t)min(max(i - 1, 0), nr - 1) * NV + v;
-    c1 = (size_t)min(max(i - 2, 0), nr - 1) * NV + v;
+    cell0 = min(max(i - 1, 0), nr - 1);
+    cell1 = min(max(i - 2, 0), nr - 1);
+    c0 = (size_t)cell0 * NV + v;
+    c1 = (size_t)cell1 * NV + v;
 }
 
 template <int NS>
@@ -108,6 +113,19 @@ __global__ __launch_bounds__(256) void k_src(SrcArgs a)
     const int t = threadIdx.x, T = blockDim.x;
     const int i0 = blockIdx.x * kSrcTile;
     const int n_tile = 8 * nm * kSrcTL;
+    // does any cell the tile reads have overflow rows (crowded cells)?  The
+    // test is uniform over the workgroup and is only needed after the
+    // pass's loads are issued.  Such tiles take a lean path (one entry at a
+    // time, each cell's overflow rows added in row order), so that the
+    // common path keeps its registers and occupancy.
+    const int lo = max(i0 - kSrcHalo - 2, 0), hi = min(i0 + kSrcTile + kSrcHalo - 2, nr - 1);
+    int os_lo[NS], os_hi[NS];
+#pragma unroll
+    for (int q = 0; q < NS; q++) {
+        os_lo[q] = a.os[q][lo];
+        os_hi[q] = a.os[q][hi + 1];
+    }
+    const int NV = 8 * nm;
     // 8 entries per thread per pass so that their loads are in flight together
     for (int base = t; base < n_tile; base += 8 * T) {
         double d0[8][NS], d1[8][NS];
@@ -116,22 +134,55 @@ __global__ __launch_bounds__(256) void k_src(SrcArgs a)
             const int idx = min(base + u * T, n_tile - 1);
             const int comp = idx / (nm * kSrcTL), k = (idx / kSrcTL) % nm, j = idx % kSrcTL;
             const int i = min(max(i0 - kSrcHalo + j, 0), nr + 1);
+            int cl0, cl1;
             size_t c0, c1;
-            partial_addr(g, comp, k, i, c0, c1);
+            partial_addr(g, comp, k, i, cl0, cl1, c0, c1);
 #pragma unroll
             for (int q = 0; q < NS; q++) {
                 d0[u][q] = a.P0[q][c0];
                 d1[u][q] = a.P1[q][c1];
             }
         }
+        // (the empty asm keeps the compiler from hoisting the test, and the
+        // wait for its loads, above this pass's loads)
+        bool ovf = false;
 #pragma unroll
-        for (int u = 0; u < 8; u++) {
-            const int idx = base + u * T;
-            if (idx >= n_tile) break;
-            const int comp = idx / (nm * kSrcTL), k = (idx / kSrcTL) % nm, j = idx % kSrcTL;
-            const int i = i0 - kSrcHalo + j;
-            const double v = combine_node<NS>(nr, comp, k, min(max(i, 0), nr + 1), d0[u], d1[u]);
-            tile[idx] = (i >= 0 && i <= nr + 1) ? v : 0.0;
+        for (int q = 0; q < NS; q++) {
+            int l_ = os_lo[q], h_ = os_hi[q];
+            __asm__ volatile("" : "+s"(l_), "+s"(h_));
+            ovf = ovf || h_ != l_;
+        }
+        if (!ovf) {
+#pragma unroll
+            for (int u = 0; u < 8; u++) {
+                const int idx = base + u * T;
+                if (idx >= n_tile) break;
+                const int comp = idx / (nm * kSrcTL), k = (idx / kSrcTL) % nm, j = idx % kSrcTL;
+                const int i = i0 - kSrcHalo + j;
+                const double v = combine_node<NS>(nr, comp, k, min(max(i, 0), nr + 1), d0[u], d1[u]);
+                tile[idx] = (i >= 0 && i <= nr + 1) ? v : 0.0;
+            }
+        } else {
+#pragma unroll 1
+            for (int u = 0; u < 8; u++) {
+                const int idx = base + u * T;
+                if (idx >= n_tile) break;
+                const int comp = idx / (nm * kSrcTL), k = (idx / kSrcTL) % nm, j = idx % kSrcTL;
+                const int i = i0 - kSrcHalo + j;
+                const int ic = min(max(i, 0), nr + 1);
+                int cl0, cl1;
+                size_t c0, c1;
+                partial_addr(g, comp, k, ic, cl0, cl1, c0, c1);
+                const int v = comp * nm + k;
+                double e0[NS], e1[NS];
+#pragma unroll
+                for (int q = 0; q < NS; q++) {
+                    e0[q] = add_rows(a.P0[q][c0], a.O0[q], a.os[q][cl0], a.os[q][cl0 + 1], NV, v);
+                    e1[q] = add_rows(a.P1[q][c1], a.O1[q], a.os[q][cl1], a.os[q][cl1 + 1], NV, v);
+                }
+                const double val = combine_node<NS>(nr, comp, k, ic, e0, e1);
+                tile[idx] = (i >= 0 && i <= nr + 1) ? val : 0.0;
+            }
         }
     }
     __syncthreads();
@@ -379,6 +430,9 @@ int launch_iteration_fields(qpad_ctx *c, int j, int l)
     for (int q = 0; q < c->nspecies; q++) {
         s.P0[q] = c->sp[q].P0;
         s.P1[q] = c->sp[q].P1;
+        s.O0[q] = c->sp[q].O0;
+        s.O1[q] = c->sp[q].O1;
+        s.os[q] = c->sp[q].ov_start;
     }
     s.cu_out = c->f[QPAD_F_CU];
     s.acu_out = c->f[QPAD_F_ACU];
@@ -548,6 +602,8 @@ int launch_epilogue(qpad_ctx *c, int j)
 struct StartSrcArgs {
     FieldCommon g;
     const double *P0[kMaxSpecies], *P1[kMaxSpecies], *qn[kMaxSpecies];
+    const double *O0[kMaxSpecies], *O1[kMaxSpecies];
+    const int32_t *os[kMaxSpecies];
     const double *cu;
     double *q_spe, *src;
     // learned seed of the iteration (B_perp components 0, 1 of nodes
@@ -568,14 +624,32 @@ __global__ __launch_bounds__(256) void k_start_src(StartSrcArgs a)
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     const int k = blockIdx.y, m = mode_of(k);
     if (i > nr + 1) return;
-    const size_t c0 = (size_t)min(max(i - 1, 0), nr - 1) * nm + k;
-    const size_t c1 = (size_t)min(max(i - 2, 0), nr - 1) * nm + k;
+    const int cl0 = min(max(i - 1, 0), nr - 1), cl1 = min(max(i - 2, 0), nr - 1);
+    const size_t c0 = (size_t)cl0 * nm + k;
+    const size_t c1 = (size_t)cl1 * nm + k;
+    // any overflow rows (crowded cells) among the workgroup's cells?
+    // (uniform over the workgroup)
+    bool ovf = false;
+    {
+        const int ib = blockIdx.x * blockDim.x;
+        const int lo = max(ib - 2, 0), hi = min(ib + (int)blockDim.x - 2, nr - 1);
+#pragma unroll
+        for (int s = 0; s < NS; s++) ovf = ovf || a.os[s][hi + 1] != a.os[s][lo];
+    }
     double d0[NS], d1[NS], qb[NS];
 #pragma unroll
     for (int s = 0; s < NS; s++) {
         d0[s] = a.P0[s][c0];
         d1[s] = a.P1[s][c1];
         qb[s] = a.qn[s] ? a.qn[s][(size_t)k * ld + i] : 0.0;
+    }
+    if (ovf) {
+        // overflow rows of crowded cells, in row order (cell_partial)
+#pragma unroll
+        for (int s = 0; s < NS; s++) {
+            d0[s] = add_rows(d0[s], a.O0[s], a.os[s][cl0], a.os[s][cl0 + 1], nm, k);
+            d1[s] = add_rows(d1[s], a.O1[s], a.os[s][cl1], a.os[s][cl1 + 1], nm, k);
+        }
     }
     double v = 0.0;
 #pragma unroll
@@ -715,6 +789,9 @@ int launch_slice_sources(qpad_ctx *c, const double *seed)
         s.P0[q] = c->sp[q].P0;
         s.P1[q] = c->sp[q].P1;
         s.qn[q] = c->sp[q].cfg.neutralized ? c->sp[q].qn : nullptr;
+        s.O0[q] = c->sp[q].O0;
+        s.O1[q] = c->sp[q].O1;
+        s.os[q] = c->sp[q].ov_start;
     }
     s.cu = c->f[QPAD_F_CU];
     s.q_spe = c->f[QPAD_F_Q_SPE];

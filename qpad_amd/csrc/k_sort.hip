@@ -13,6 +13,8 @@
 //                   particles with the same key, in the fixed order (round,
 //                   wave, lane), from wave ballots; position = cell_start +
 //                   chunk offset + rank; moves every particle array.
+//                   Workgroup 0 also lists the deposit's overflow chunks
+//                   (cells holding more than kDepChunk particles).
 // Integer counting only, in a fixed order: the permutation is the same on
 // every run.
 #include <algorithm>
@@ -43,6 +45,9 @@ struct SortArgs {
     double *d[8];
     const int32_t *sid;
     int32_t *did;
+    int32_t *os;            // [nb] overflow rows of cell c: os[c] .. os[c+1]-1
+    int4 *desc;             // [n_ov_max] (cell, first particle, end, 0)
+    int n_ov_max;
 };
 
 __global__ __launch_bounds__(kSortThreads) void k_sort_hist(SortArgs a)
@@ -158,6 +163,52 @@ __device__ inline void scan_totals(const SortArgs &a, int *base)
     __syncthreads();
 }
 
+// workgroup 0, after scan_totals: cell c < nr (nr = nb - 1) with n_c
+// particles is deposited in chunks of kDepChunk; chunk 0 by the cell's own
+// wave, chunks 1 .. ceil(n_c / kDepChunk) - 1 by overflow waves.  Their rows
+// are numbered cell by cell, chunk by chunk: os = exclusive prefix of the
+// chunk counts, desc[row] = (cell, first particle, end)
+__device__ inline void overflow_chunks(const SortArgs &a, const int *base)
+{
+    __shared__ int wsum[kSortThreads / 64];
+    const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+    const int nr = a.nb - 1;
+    const int per = (nr + kSortThreads - 1) / kSortThreads;
+    const int c0 = t * per;
+    int local = 0;
+    for (int q = 0; q < per; q++) {
+        const int c = c0 + q;
+        if (c < nr) {
+            const int n = base[c + 1] - base[c];
+            local += n > 0 ? (n - 1) / kDepChunk : 0;
+        }
+    }
+    int incl = local;
+    for (int dd = 1; dd < 64; dd <<= 1) {
+        const int y = __shfl_up(incl, dd, 64);
+        if (lane >= dd) incl += y;
+    }
+    if (lane == 63) wsum[wave] = incl;
+    __syncthreads();
+    int run = incl - local;
+    for (int w = 0; w < wave; w++) run += wsum[w];
+    for (int q = 0; q < per; q++) {
+        const int c = c0 + q;
+        if (c >= nr) break;
+        const int p0 = base[c], p1 = base[c + 1];
+        const int k = p1 > p0 ? (p1 - p0 - 1) / kDepChunk : 0;
+        a.os[c] = min(run, a.n_ov_max);
+        for (int u = 1; u <= k; u++) {
+            const int row = run + u - 1;
+            if (row < a.n_ov_max) a.desc[row] = make_int4(c, p0 + u * kDepChunk, min(p0 + (u + 1) * kDepChunk, p1), 0);
+        }
+        run += k;
+    }
+    // the last thread's running count is the total (its range may be empty);
+    // rows never exceed n_ov_max (the chunks of n particles number < n / kDepChunk)
+    if (t == kSortThreads - 1) a.os[nr] = min(run, a.n_ov_max);
+}
+
 __global__ __launch_bounds__(kSortThreads) void k_sort_scatter(SortArgs a)
 {
     extern __shared__ int base[];                                  // [nb] bin starts
@@ -169,6 +220,7 @@ __global__ __launch_bounds__(kSortThreads) void k_sort_scatter(SortArgs a)
     if (blockIdx.x == 0) {
         for (int b = t; b < a.nb; b += kSortThreads) a.cs[b] = base[b];
         if (t == 0) a.cs[a.nb] = (int32_t)a.n;
+        overflow_chunks(a, base);
     }
     if (a.n == 0) return;
     uint32_t kv[kSortPer];
@@ -271,6 +323,9 @@ int launch_sort(qpad_ctx *c, int s)
     }
     a.sid = sp.id;
     a.did = sp.a_id;
+    a.os = sp.ov_start;
+    a.desc = sp.ov_desc;
+    a.n_ov_max = sp.n_ov_max;
     const size_t lds = sizeof(int) * (size_t)a.nb;
     if (a.nchunks > 0)
         hipLaunchKernelGGL(k_sort_hist, dim3(a.nchunks), dim3(kSortThreads), lds, c->stream, a);

@@ -386,6 +386,7 @@ int qpad_destroy(qpad_ctx *c)
         dfree(sp.a_gamma); dfree(sp.a_psi); dfree(sp.a_id);
         dfree(sp.key); dfree(sp.key_alt);
         dfree(sp.cell_start); dfree(sp.qn); dfree(sp.P0); dfree(sp.P1);
+        dfree(sp.ov_start); dfree(sp.ov_desc); dfree(sp.O0); dfree(sp.O1);
         dfree(sp.hist); dfree(sp.tot);
     }
     for (int b = 0; b < c->nbeams; b++) {
@@ -437,6 +438,14 @@ int qpad_species_new(qpad_ctx *c, const qpad_species_cfg *cfg, int32_t *sp_id)
     if (!rc) rc = dalloc(&sp.qn, f1_size(c, 1));
     if (!rc) rc = dalloc(&sp.P0, (size_t)c->g.nr * 8 * c->g.nm);
     if (!rc) rc = dalloc(&sp.P1, (size_t)c->g.nr * 8 * c->g.nm);
+    sp.n_ov_max = (int)(sp.npmax / kDepChunk + 1);
+    if (!rc) rc = dalloc(&sp.ov_start, (size_t)c->g.nr + 1);
+    if (!rc) rc = dalloc(&sp.ov_desc, (size_t)sp.n_ov_max);
+    if (!rc) rc = dalloc(&sp.O0, (size_t)sp.n_ov_max * 8 * c->g.nm);
+    if (!rc) rc = dalloc(&sp.O1, (size_t)sp.n_ov_max * 8 * c->g.nm);
+    if (!rc && (hipMemsetAsync(sp.ov_start, 0, sizeof(int32_t) * ((size_t)c->g.nr + 1), c->stream) != hipSuccess ||
+                hipMemsetAsync(sp.ov_desc, 0, sizeof(int4) * (size_t)sp.n_ov_max, c->stream) != hipSuccess))
+        rc = fail(QPAD_E_HIP, "hipMemsetAsync of the overflow rows failed");
     size_t hist_ints = 0;
     if (!rc) rc = sort_workspace(sp.npmax, c->g.nr, &hist_ints);
     if (!rc) rc = dalloc(&sp.hist, hist_ints);

@@ -22,6 +22,11 @@ constexpr int kMaxSpecies = 4;
 constexpr int kMaxBeams = 4;
 constexpr int kSolveThreads = 512;   // workgroup of the tridiagonal solves
 constexpr int kDepWaves = 4;         // default radial cells (waves) per deposit workgroup
+// particles per deposit wave: a cell holding more is deposited in chunks of
+// kDepChunk, the first by the cell's own wave, the others by overflow waves
+// with partials of their own (rows of Species::O0/O1), so that a crowded
+// cell no longer holds up a whole deposit launch
+constexpr int kDepChunk = 1024;
 
 // field kinds (source/param.f03:72-80)
 enum { FK_PSI = 0, FK_EZ = 1, FK_BZ = 2, FK_BT = 3, FK_BPLUS = 4, FK_BMINUS = 5, FK_N = 6 };
@@ -49,7 +54,14 @@ struct Species {
     uint32_t *key, *key_alt;  // radial cell of each particle (nr = left the box)
     int32_t *cell_start;     // nr + 2 entries, cell_start[nr] = live count
     double *qn;              // neutralising background, 1 x nm x ld
-    double *P0, *P1;         // per-cell deposit partials [nr][nval]
+    double *P0, *P1;         // per-cell deposit partials [nr][nval] (first chunk)
+    // overflow chunks of crowded cells (written by the sort): rows
+    // ov_start[c] .. ov_start[c+1]-1 belong to cell c, in particle order;
+    // ov_desc[row] = (cell, first particle, end, 0); O0/O1 [row][nval]
+    int32_t *ov_start;       // nr + 1 entries, ov_start[nr] = rows in use
+    int4 *ov_desc;
+    double *O0, *O1;
+    int n_ov_max;            // rows allocated: npmax / kDepChunk + 1
     int *hist, *tot;         // counting-sort workspace: [chunks][nr+1], [nr+1]
 };
 
@@ -139,6 +151,31 @@ void set_error(const std::string &msg);
         int rc_ = (expr);         \
         if (rc_ != 0) return rc_; \
     } while (0)
+
+// p plus value v (rows of nv values) of overflow rows r0 .. r1-1, added one
+// row after another (four loads in flight)
+__device__ __forceinline__ double add_rows(double p, const double *O, int r0, int r1, int nv, int v)
+{
+    int o = r0;
+    for (; o + 4 <= r1; o += 4) {
+        const double x0 = O[(size_t)o * nv + v], x1 = O[(size_t)(o + 1) * nv + v];
+        const double x2 = O[(size_t)(o + 2) * nv + v], x3 = O[(size_t)(o + 3) * nv + v];
+        p += x0;
+        p += x1;
+        p += x2;
+        p += x3;
+    }
+    for (; o < r1; o++) p += O[(size_t)o * nv + v];
+    return p;
+}
+
+// deposit partial v (rows of nv values) of cell c: the row of the cell's
+// own wave plus the cell's overflow rows os[c] .. os[c+1]-1, added in order
+__device__ __forceinline__ double cell_partial(const double *P, const double *O, const int32_t *os, int c, int nv,
+                                               int v)
+{
+    return add_rows(P[(size_t)c * nv + v], O, os[c], os[c + 1], nv, v);
+}
 
 // rows per thread of the field solves: the smallest power of two K with
 // K * kSolveThreads >= nr (K <= 16)
