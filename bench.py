@@ -92,7 +92,7 @@ def measured_traffic(workload, max_mode):
     if not os.path.exists(path):
         return None
     text = open(path).read()
-    if f"k_amjdeposit<{max_mode}, false>" not in text:
+    if f"k_amjdeposit<{max_mode}, false" not in text:
         return None
     for line in text.splitlines():
         if line.startswith("traffic_bytes_per_launch:"):

@@ -43,6 +43,7 @@ struct SrcArgs {
     const double *b_spe;
     double *src;
     const IterState *it;
+    int late_done;   // 1: test the converged flag after the first loads
 };
 
 // nodes per k_src workgroup; its LDS tile also holds a halo of 2 nodes on
@@ -50,6 +51,7 @@ struct SrcArgs {
 constexpr int kSrcTile = 32;
 constexpr int kSrcHalo = 2;
 constexpr int kSrcTL = kSrcTile + 2 * kSrcHalo;
+static_assert(kSrcTile * (kMaxMode + 1) <= 256, "one B+/B- item per k_src thread");
 
 // cu (base 0), acu (3) or amu (5) of the tile in LDS as [comp8][k][j],
 // j = i - (i0 - 2)
@@ -99,6 +101,26 @@ __device__ __forceinline__ double combine_node(int nr, int comp, int k, int i, c
     return i == 0 ? 0.0 : (i == 1 ? axis : inner);
 }
 
+// a uniform int read with a vector load (the zero offset comes from an asm,
+// so the compiler cannot turn it into a scalar load)
+__device__ __forceinline__ int flag_vload(const int *p)
+{
+    int z;
+    __asm__ volatile("v_mov_b32 %0, 0" : "=v"(z));
+    return p[z];
+}
+
+// b_spe components 0, 1 of mode arrays kr and ki (= kr for m = 0) at one
+// node, as bpm_source reads them (f3(c, k, i) at that node only)
+struct BspeNode {
+    int kr;
+    double r0, r1, i0, i1;
+    __device__ __forceinline__ double operator()(int c, int k, int) const
+    {
+        return k == kr ? (c == 0 ? r0 : r1) : (c == 0 ? i0 : i1);
+    }
+};
+
 // one workgroup per tile of kSrcTile nodes: (1) cu, acu, amu of the tile
 // and its halo into LDS; (2) B+/B- sources of every mode and dJ/dxi at the
 // tile's nodes 1..nr; (3) cu, acu, amu out (nodes 0..nr+1) and the Bz and Ez
@@ -107,7 +129,18 @@ template <int NS>
 __global__ __launch_bounds__(256) void k_src(SrcArgs a)
 {
     extern __shared__ double tile[];
-    if (a.it->done) return;
+    // the converged flag: with adaptive queuing (late_done) almost every
+    // launch works, and the flag is tested after the first pass's loads are
+    // issued (one round trip less); with fixed queuing about half the
+    // launches come after convergence and return before loading anything
+    // (a vector load, through an offset the compiler cannot see is zero: its
+    // wait does not also wait for the kernel-argument scalar loads, and it
+    // retires before the data loads issued after it)
+    int done = flag_vload(&a.it->done);   // requested first in both modes
+    if (!a.late_done) {
+        __asm__ volatile("");   // keeps this test a branch of its own
+        if (__builtin_amdgcn_readfirstlane(done)) return;
+    }
     const FieldCommon &g = a.g;
     const int nm = g.nm, ld = g.ld, nr = g.nr, M = g.M;
     const int t = threadIdx.x, T = blockDim.x;
@@ -125,14 +158,26 @@ __global__ __launch_bounds__(256) void k_src(SrcArgs a)
         os_lo[q] = a.os[q][lo];
         os_hi[q] = a.os[q][hi + 1];
     }
+    // b_spe at the node of this thread's B+/B- item (the only b_spe values
+    // the sources read), requested first; one item per thread
+    // (kSrcTile * (kMaxMode + 1) <= 256 threads)
+    const int m2 = t / kSrcTile, i2 = i0 + t % kSrcTile;
+    const bool item2 = t < kSrcTile * (M + 1) && i2 >= 1 && i2 <= nr;
+    const int kr2 = m2 == 0 ? 0 : 2 * m2 - 1, ki2 = m2 == 0 ? 0 : 2 * m2;
+    const int i2c = min(max(i2, 1), nr);
+    const BspeNode bn{kr2, FX(a.b_spe, 0, min(kr2, nm - 1), i2c), FX(a.b_spe, 1, min(kr2, nm - 1), i2c),
+                      FX(a.b_spe, 0, min(ki2, nm - 1), i2c), FX(a.b_spe, 1, min(ki2, nm - 1), i2c)};
     const int NV = 8 * nm;
-    // 8 entries per thread per pass so that their loads are in flight together
-    for (int base = t; base < n_tile; base += 8 * T) {
-        double d0[8][NS], d1[8][NS];
+    // 8 entries per thread per pass so that their loads are in flight
+    // together; entry idx = j * NV + v (node j of the tile, value v), so that
+    // neighbouring threads read neighbouring values of one cell's partial row
+    // (coalesced) and the tile in LDS is [v][j]
+    double d0[8][NS], d1[8][NS];
+    auto pass_loads = [&](int base) {
 #pragma unroll
         for (int u = 0; u < 8; u++) {
             const int idx = min(base + u * T, n_tile - 1);
-            const int comp = idx / (nm * kSrcTL), k = (idx / kSrcTL) % nm, j = idx % kSrcTL;
+            const int j = idx / NV, vv = idx - j * NV, comp = vv / nm, k = vv - comp * nm;
             const int i = min(max(i0 - kSrcHalo + j, 0), nr + 1);
             int cl0, cl1;
             size_t c0, c1;
@@ -143,59 +188,71 @@ __global__ __launch_bounds__(256) void k_src(SrcArgs a)
                 d1[u][q] = a.P1[q][c1];
             }
         }
-        // (the empty asm keeps the compiler from hoisting the test, and the
-        // wait for its loads, above this pass's loads)
-        bool ovf = false;
-#pragma unroll
-        for (int q = 0; q < NS; q++) {
-            int l_ = os_lo[q], h_ = os_hi[q];
-            __asm__ volatile("" : "+s"(l_), "+s"(h_));
-            ovf = ovf || h_ != l_;
-        }
+    };
+    auto pass_combine = [&](int base, bool ovf) {
         if (!ovf) {
 #pragma unroll
             for (int u = 0; u < 8; u++) {
                 const int idx = base + u * T;
                 if (idx >= n_tile) break;
-                const int comp = idx / (nm * kSrcTL), k = (idx / kSrcTL) % nm, j = idx % kSrcTL;
+                const int j = idx / NV, vv = idx - j * NV, comp = vv / nm, k = vv - comp * nm;
                 const int i = i0 - kSrcHalo + j;
                 const double v = combine_node<NS>(nr, comp, k, min(max(i, 0), nr + 1), d0[u], d1[u]);
-                tile[idx] = (i >= 0 && i <= nr + 1) ? v : 0.0;
+                tile[vv * kSrcTL + j] = (i >= 0 && i <= nr + 1) ? v : 0.0;
             }
         } else {
 #pragma unroll 1
             for (int u = 0; u < 8; u++) {
                 const int idx = base + u * T;
                 if (idx >= n_tile) break;
-                const int comp = idx / (nm * kSrcTL), k = (idx / kSrcTL) % nm, j = idx % kSrcTL;
+                const int j = idx / NV, vv = idx - j * NV, comp = vv / nm, k = vv - comp * nm;
                 const int i = i0 - kSrcHalo + j;
                 const int ic = min(max(i, 0), nr + 1);
                 int cl0, cl1;
                 size_t c0, c1;
                 partial_addr(g, comp, k, ic, cl0, cl1, c0, c1);
-                const int v = comp * nm + k;
                 double e0[NS], e1[NS];
 #pragma unroll
                 for (int q = 0; q < NS; q++) {
-                    e0[q] = add_rows(a.P0[q][c0], a.O0[q], a.os[q][cl0], a.os[q][cl0 + 1], NV, v);
-                    e1[q] = add_rows(a.P1[q][c1], a.O1[q], a.os[q][cl1], a.os[q][cl1 + 1], NV, v);
+                    e0[q] = add_rows(a.P0[q][c0], a.O0[q], a.os[q][cl0], a.os[q][cl0 + 1], NV, vv);
+                    e1[q] = add_rows(a.P1[q][c1], a.O1[q], a.os[q][cl1], a.os[q][cl1 + 1], NV, vv);
                 }
                 const double val = combine_node<NS>(nr, comp, k, ic, e0, e1);
-                tile[idx] = (i >= 0 && i <= nr + 1) ? val : 0.0;
+                tile[vv * kSrcTL + j] = (i >= 0 && i <= nr + 1) ? val : 0.0;
             }
         }
+    };
+    // first pass peeled (a loop header would wait for every load in flight,
+    // the b_spe requests above included, before this pass's loads)
+    pass_loads(t);
+    // (the empty asms keep the compiler from hoisting the tests, and the
+    // wait for their loads, above this pass's loads)
+    if (a.late_done) {
+        __asm__ volatile("" : "+v"(done));
+        if (__builtin_amdgcn_readfirstlane(done)) return;
+    }
+    bool ovf = false;
+#pragma unroll
+    for (int q = 0; q < NS; q++) {
+        int l_ = os_lo[q], h_ = os_hi[q];
+        __asm__ volatile("" : "+v"(l_), "+v"(h_));
+        ovf = ovf || __builtin_amdgcn_readfirstlane(h_) != __builtin_amdgcn_readfirstlane(l_);
+    }
+    pass_combine(t, ovf);
+    for (int base = t + 8 * T; base < n_tile; base += 8 * T) {
+        pass_loads(base);
+        pass_combine(base, ovf);
     }
     __syncthreads();
     const TileArr cu{tile, nm, 0, i0}, acu{tile, nm, 3, i0}, amu{tile, nm, 5, i0};
     // B+/B- of mode m and dJ/dxi
-    for (int it = t; it < kSrcTile * (M + 1); it += T) {
-        const int m = it / kSrcTile, i = i0 + it % kSrcTile;
-        if (i < 1 || i > nr) continue;
-        const int kr = m == 0 ? 0 : 2 * m - 1, ki = 2 * m;
+    if (item2) {
+        const int m = m2, i = i2;
+        const int kr = kr2, ki = 2 * m;
         double dr_[2], di_[2] = {0.0, 0.0}, bpm[4];
         djdxi_node(g, acu, amu, kr, i, dr_);
         if (m > 0) djdxi_node(g, acu, amu, ki, i, di_);
-        bpm_source(g, cu, FieldArr{a.b_spe, nm, ld}, m, i, dr_, di_, bpm);
+        bpm_source(g, cu, bn, m, i, dr_, di_, bpm);
         FX(a.dcu, 0, kr, i) = dr_[0];
         FX(a.dcu, 1, kr, i) = dr_[1];
         if (m > 0) {
@@ -228,6 +285,7 @@ struct SolveArgs {
     double *b_spe, *e;
     const double *cu;
     const IterState *it;
+    int late_done;   // as in SrcArgs
 };
 
 // rows r = t + kk * kSolveThreads of a system (clamped loads, so that all K
@@ -250,7 +308,12 @@ __global__ __launch_bounds__(kSolveThreads) void k_solve(SolveArgs a)
 {
     extern __shared__ double stage[];
     __shared__ double red[kRed];
-    if (a.it->done) return;
+    // tested once the rows are requested (late_done); a vector load
+    int done = flag_vload(&a.it->done);
+    if (!a.late_done) {
+        __asm__ volatile("");
+        if (__builtin_amdgcn_readfirstlane(done)) return;
+    }
     const FieldCommon &g = a.g;
     const int nm = g.nm, ld = g.ld, nr = g.nr, M = g.M;
     const int t = threadIdx.x;
@@ -277,6 +340,14 @@ __global__ __launch_bounds__(kSolveThreads) void k_solve(SolveArgs a)
     }
     double v[K], d[1][K];
     load_rows<K>(a.src + (size_t)row * nr, nr, v);
+    // the factor rows are requested together with the right-hand side
+    const FactorT f[1] = {dfactor(g, kind, m)};
+    FactorRows<K, 1> fr;
+    load_factors<K, 1>(f, fr);
+    if (a.late_done) {
+        __asm__ volatile("" : "+v"(done));
+        if (__builtin_amdgcn_readfirstlane(done)) return;
+    }
     if (role == 2 && m == 0) {
         // discrete Gauss law on the axis row of Ez
         double part = 0.0;
@@ -292,8 +363,7 @@ __global__ __launch_bounds__(kSolveThreads) void k_solve(SolveArgs a)
         }
     }
     stage_in<K>(stage, v, nr, d[0]);
-    const FactorT f[1] = {dfactor(g, kind, m)};
-    seg_solve<K, 1>(f, d, red);
+    seg_solve_rows<K, 1>(fr, d, red);
     stage_out<K>(stage, d[0], v);
 #pragma unroll
     for (int kk = 0; kk < K; kk++) {
@@ -318,6 +388,7 @@ struct BfieldArgs {
     double *conv_blk;   // this iteration's half: [workgroup][max old^2, max diff^2]
     IterState *it;
     int l;              // iteration index (1-based)
+    int late_done;      // as in SrcArgs
 };
 
 // thread per node i = 0..nr+1, all modes: B_r, B_phi of b_spe from the
@@ -332,7 +403,12 @@ __global__ __launch_bounds__(256) void k_bfields(BfieldArgs a)
 {
     constexpr int NM = 2 * M + 1;
     __shared__ double red[8];
-    if (a.it->done) return;
+    // tested once the loads are requested (late_done); a vector load
+    int done = flag_vload(&a.it->done);
+    if (!a.late_done) {
+        __asm__ volatile("");
+        if (__builtin_amdgcn_readfirstlane(done)) return;
+    }
     const FieldCommon &g = a.g;
     const int nm = NM, ld = g.ld, nr = g.nr;
     const int t = threadIdx.x;
@@ -361,6 +437,10 @@ __global__ __launch_bounds__(256) void k_bfields(BfieldArgs a)
         pp_2[k] = a.psi[(size_t)p * ld + 2];
     }
     const double ore = a.conv_re[ic], oim = a.conv_im[ic];
+    if (a.late_done) {
+        __asm__ volatile("" : "+v"(done));
+        if (__builtin_amdgcn_readfirstlane(done)) return;
+    }
     // ---- b_spe (inner nodes), b, E_perp ----
 #pragma unroll
     for (int m = 0; m <= M; m++) {
@@ -441,6 +521,7 @@ int launch_iteration_fields(qpad_ctx *c, int j, int l)
     s.b_spe = c->f[QPAD_F_B_SPE];
     s.src = c->src;
     s.it = c->it;
+    s.late_done = c->adaptive;
     const size_t src_lds = sizeof(double) * 8 * c->g.nm * kSrcTL;
     const dim3 sgrid((unsigned)((c->g.nr + 2 + kSrcTile - 1) / kSrcTile));
     switch (c->nspecies) {
@@ -458,6 +539,7 @@ int launch_iteration_fields(qpad_ctx *c, int j, int l)
     a.e = c->f[QPAD_F_E];
     a.cu = c->f[QPAD_F_CU];
     a.it = c->it;
+    a.late_done = c->adaptive;
     const int nblk = 4 * (c->g.M + 1) + 2 * c->g.nm;
     QPAD_DISPATCH_K(c->g.nr, k_solve, dim3(nblk), 0, c->stream, a);
     QPAD_HIP_CHECK(hipGetLastError());
@@ -475,6 +557,7 @@ int launch_iteration_fields(qpad_ctx *c, int j, int l)
     f.conv_blk = c->conv_blk + (size_t)(l & 1) * 2 * conv_blocks(c->g.nr);
     f.it = c->it;
     f.l = l;
+    f.late_done = c->adaptive;
     const dim3 fgrid((unsigned)((c->g.nr + 2 + 255) / 256));
     switch (c->g.M) {
     case 0: hipLaunchKernelGGL(k_bfields<0>, fgrid, dim3(256), 0, c->stream, f); break;

@@ -97,21 +97,37 @@ struct FactorT {
     const double *lo, *inv, *cp;
 };
 
-// Solve NS systems in place on d[s][k] (row r = threadIdx.x * K + k).
+// this thread's factor rows of NS systems (independent of the data, so a
+// caller can request them together with its right-hand sides)
 template <int K, int NS>
-__device__ inline void seg_solve(const FactorT (&f)[NS], double (&d)[NS][K], double *red)
+struct FactorRows {
+    double lo[NS][K], inv[NS][K], cp[NS][K];
+};
+
+template <int K, int NS>
+__device__ inline void load_factors(const FactorT (&f)[NS], FactorRows<K, NS> &r)
 {
     const int t = threadIdx.x;
-    double A[NS], B[NS], ex[NS];
-    double lo[NS][K], inv[NS][K];
 #pragma unroll
     for (int s = 0; s < NS; s++) {
 #pragma unroll
         for (int k = 0; k < K; k++) {
-            lo[s][k] = f[s].lo[k * kSolveThreads + t];
-            inv[s][k] = f[s].inv[k * kSolveThreads + t];
+            r.lo[s][k] = f[s].lo[k * kSolveThreads + t];
+            r.inv[s][k] = f[s].inv[k * kSolveThreads + t];
+            r.cp[s][k] = f[s].cp[k * kSolveThreads + t];
         }
     }
+}
+
+// Solve NS systems in place on d[s][k] (row r = threadIdx.x * K + k), with
+// the factor rows already in registers.
+template <int K, int NS>
+__device__ inline void seg_solve_rows(const FactorRows<K, NS> &fr, double (&d)[NS][K], double *red)
+{
+    double A[NS], B[NS], ex[NS];
+    const auto &lo = fr.lo;
+    const auto &inv = fr.inv;
+    const auto &cp = fr.cp;
 #pragma unroll
     for (int s = 0; s < NS; s++) {
         double a = 1.0, b = 0.0;
@@ -133,12 +149,6 @@ __device__ inline void seg_solve(const FactorT (&f)[NS], double (&d)[NS][K], dou
             y = (d[s][k] - lo[s][k] * y) * inv[s][k];
             d[s][k] = y;
         }
-    }
-    double cp[NS][K];
-#pragma unroll
-    for (int s = 0; s < NS; s++) {
-#pragma unroll
-        for (int k = 0; k < K; k++) cp[s][k] = f[s].cp[k * kSolveThreads + t];
     }
 #pragma unroll
     for (int s = 0; s < NS; s++) {
@@ -162,6 +172,15 @@ __device__ inline void seg_solve(const FactorT (&f)[NS], double (&d)[NS][K], dou
             d[s][k] = x;
         }
     }
+}
+
+// Solve NS systems in place on d[s][k] (row r = threadIdx.x * K + k).
+template <int K, int NS>
+__device__ inline void seg_solve(const FactorT (&f)[NS], double (&d)[NS][K], double *red)
+{
+    FactorRows<K, NS> fr;
+    load_factors<K, NS>(f, fr);
+    seg_solve_rows<K, NS>(fr, d, red);
 }
 
 // ---- padded LDS stage between interleaved and blocked row order ----
