@@ -116,8 +116,9 @@ def batch_c4(reps=2):
     """The C4 scan (16 independent instances of random Nr, modes, beam charge
     and offset, Nxi 512, from a recorded seed): one untimed 3D step each,
     then `reps` timed 3D steps of all 16 stepped one after another, as one
-    eager batch, as one graph batch (after the graphs are captured), and as
-    one graph batch after every instance's deposit launch shape is autotuned.
+    eager batch (event-driven, each instance with adaptive iteration
+    queuing), as one graph batch (after the graphs are captured), and as one
+    graph batch after every instance's deposit launch shape is autotuned.
     Instance 3D steps per second; every form gives the same results (the
     batch tests)."""
     from qpad_amd import Batch, Simulation, configs
@@ -151,6 +152,7 @@ def batch_c4(reps=2):
             "sequential_instance_steps_per_s": n / t_seq, "batch_eager_instance_steps_per_s": n / t_eager,
             "batch_graph_instance_steps_per_s": n / t_graph, "batch_graph_tuned_instance_steps_per_s": n / t_tuned,
             "tuned_dep_cells": shapes, "batch_graph_tuned_speedup_vs_sequential": t_seq / t_tuned,
+            "batch_eager_speedup_vs_sequential": t_seq / t_eager,
             "ms_per_batch_step": {"sequential": 1e3 * t_seq, "eager": 1e3 * t_eager, "graph": 1e3 * t_graph,
                                   "graph_tuned": 1e3 * t_tuned}}
 

@@ -558,9 +558,72 @@ namespace qpad {
 // test has set IterState::done.
 // overlap: also queue the next slice's seed behind this slice's epilogue
 // (qpad_step3d, where nothing can change the fields between slices)
-static int slice_enqueue(qpad_ctx *c, int j, bool overlap)
+// A slice's queuing as a resumable state machine, so that the blocking form
+// (slice_enqueue) and the batch's event-driven form queue the same kernels.
+struct SliceRun {
+    int j = 0;            // slice being queued
+    int queued = 0;       // iterations queued so far
+    int upto = 0;         // iterations to queue before the next decision
+    int known_iters = 0;  // iterations the slice needed, once known
+    bool waiting = false; // a deciding deposit is in flight (dec_ev)
+    bool overlap = false; // network overlap (qpad_step3d only)
+};
+
+static int slice_finish(qpad_ctx *c, SliceRun &st);
+
+// Queue the iterations up to st.upto, then (unless iter_max is reached) the
+// next iteration, whose deposit decides on the one before it and reports to
+// the host behind dec_ev; or finish the slice.
+static int slice_advance(qpad_ctx *c, SliceRun &st)
 {
-    // rho - Jz of every species, psi, Bz from the J predictor, 'record'
+    const int j = st.j, imax = c->g.iter_max;
+    for (int l = st.queued + 1; l <= st.upto; l++) {
+        const bool first = l == 1;
+        for (int s = 0; s < c->nspecies; s++) TRY(launch_amjdeposit(c, s, l, true, first));
+        hipEvent_t stop = nullptr;
+        if (first) time_begin(c, 2, &stop);
+        TRY(launch_iteration_fields(c, j, l));
+        if (first) time_end(c, 2, stop);
+    }
+    st.queued = std::max(st.queued, st.upto);
+    if (st.queued >= imax) return slice_finish(c, st);
+    // iteration queued + 1: its deposit decides on iteration `queued`
+    const int l = st.queued + 1;
+    TRY(launch_amjdeposit(c, 0, l, true, false, true));
+    QPAD_HIP_CHECK(hipEventRecord(c->dec_ev, c->stream));
+    for (int s = 1; s < c->nspecies; s++) TRY(launch_amjdeposit(c, s, l, true, false));
+    TRY(launch_iteration_fields(c, j, l));
+    st.queued = l;
+    st.waiting = true;
+    return 0;
+}
+
+// the deciding deposit has completed: finish the slice, or queue on
+static int slice_decided(qpad_ctx *c, SliceRun &st)
+{
+    st.waiting = false;
+    if (c->it_report->done) {
+        st.known_iters = c->it_report->iters;
+        return slice_finish(c, st);
+    }
+    st.upto = st.queued;
+    return slice_advance(c, st);
+}
+
+// rho - Jz of every species, psi, Bz from the J predictor, 'record', then
+// the first iterations.  Each iteration's kernels return at once once the
+// device-side test has set IterState::done, so queuing more iterations than
+// needed changes no result, only costs launches.  Adaptive mode queues as
+// many iterations as the previous slice needed (plus the margin); after
+// that, the deposit of each further iteration (which decides whether the
+// one before it converged) reports the decision to the host, with that
+// iteration's field kernels already queued behind it, and the host stops
+// queuing at the first converged report.  Fixed mode queues all iter_max
+// iterations.  Timing events (when enabled) bracket only the first
+// iteration, which always does work: an event per launch would cost a
+// quarter of the step.
+static int slice_start(qpad_ctx *c, int j, bool overlap, SliceRun &st)
+{
     // the learned seed (when on) is computed on the network's stream from
     // the fields left by slice j - 1, beside the previous slice's push and
     // sort and this slice's charge deposit
@@ -571,54 +634,40 @@ static int slice_enqueue(qpad_ctx *c, int j, bool overlap)
     if (net) TRY(seednet_before_sources(c, j, &seed));
     TRY(launch_slice_sources(c, seed));
     TRY(launch_slice_solves(c, j));
-    // predictor-corrector iterations, queued without waiting for the device.
-    // Each iteration's kernels return at once once the device-side test has
-    // set IterState::done, so queuing more iterations than needed changes no
-    // result, only costs launches.  Adaptive mode queues as many iterations
-    // as the previous slice needed (plus the margin); after that, the deposit
-    // of each further iteration (which decides whether the one before it
-    // converged) reports the decision to the host, with that iteration's
-    // field kernels already queued behind it, and the host stops queuing at
-    // the first converged report.  Fixed mode queues all iter_max iterations.
-    // Timing events (when enabled) bracket only the first iteration, which
-    // always does work: an event per launch would cost a quarter of the step.
     const int imax = c->g.iter_max;
+    st = SliceRun();
+    st.j = j;
+    st.overlap = overlap;
+    st.known_iters = imax;
     // (at most imax - 1 up front, so that the deposit of iteration imax still
     // reports and the next slice's guess keeps following the iterations)
-    int upto = c->adaptive ? std::max(1, std::min(imax - 1, c->spec_guess + c->spec_margin)) : imax;
-    int queued = 0, known_iters = imax;
-    for (;;) {
-        for (int l = queued + 1; l <= upto; l++) {
-            const bool first = l == 1;
-            for (int s = 0; s < c->nspecies; s++) TRY(launch_amjdeposit(c, s, l, true, first));
-            hipEvent_t stop = nullptr;
-            if (first) time_begin(c, 2, &stop);
-            TRY(launch_iteration_fields(c, j, l));
-            if (first) time_end(c, 2, stop);
-        }
-        queued = upto;
-        if (queued >= imax) break;
-        // iteration queued + 1: its deposit decides on iteration `queued`
-        const int l = queued + 1;
-        TRY(launch_amjdeposit(c, 0, l, true, false, true));
-        QPAD_HIP_CHECK(hipEventRecord(c->dec_ev, c->stream));
-        for (int s = 1; s < c->nspecies; s++) TRY(launch_amjdeposit(c, s, l, true, false));
-        TRY(launch_iteration_fields(c, j, l));
-        queued = l;
-        QPAD_HIP_CHECK(hipEventSynchronize(c->dec_ev));
-        if (c->it_report->done) {
-            known_iters = c->it_report->iters;
-            break;
-        }
-        upto = queued;
-    }
-    if (c->adaptive) c->spec_guess = known_iters;
-    // slice epilogue: cbq, e_spe, history (simulation_class.f03:401-456)
+    st.upto = c->adaptive ? std::max(1, std::min(imax - 1, c->spec_guess + c->spec_margin)) : imax;
+    return slice_advance(c, st);
+}
+
+// slice epilogue: cbq, e_spe, history (simulation_class.f03:401-456), then
+// the push and the sort
+static int slice_finish(qpad_ctx *c, SliceRun &st)
+{
+    const int j = st.j;
+    if (c->adaptive) c->spec_guess = st.known_iters;
     TRY(launch_epilogue(c, j));
-    if (net) TRY(seednet_after_epilogue(c, j, (overlap && j < c->g.nxi) ? j + 1 : 0));
+    if (seednet_active(c)) TRY(seednet_after_epilogue(c, j, (st.overlap && j < c->g.nxi) ? j + 1 : 0));
     for (int s = 0; s < c->nspecies; s++) {
         TRY(launch_push(c, s));
         TRY(launch_sort(c, s));
+    }
+    st.j = 0;
+    return 0;
+}
+
+static int slice_enqueue(qpad_ctx *c, int j, bool overlap)
+{
+    SliceRun st;
+    TRY(slice_start(c, j, overlap, st));
+    while (st.waiting) {
+        QPAD_HIP_CHECK(hipEventSynchronize(c->dec_ev));
+        TRY(slice_decided(c, st));
     }
     return 0;
 }
@@ -1161,9 +1210,10 @@ int qpad_batch_mode(qpad_batch *b, int32_t graph)
 }
 
 // One 3D step of every context: the heads, then the slices of all contexts
-// (round-robin by slice index, each context on its own stream, so their
-// kernels run side by side; or each context's captured slice graph), then
-// the tails.  Every context's results equal its own qpad_step3d.
+// (each context on its own stream, so their kernels run side by side: queued
+// by an event-driven loop over the contexts, or as each context's captured
+// slice graph), then the tails.  Every context's results equal its own
+// qpad_step3d.
 int qpad_batch_step3d(qpad_batch *b, int32_t *total_iters_out)
 {
     if (!b) return fail(QPAD_E_INVALID, "null batch");
@@ -1195,18 +1245,37 @@ int qpad_batch_step3d(qpad_batch *b, int32_t *total_iters_out)
             QPAD_HIP_CHECK(hipGraphLaunch(b->exec[k][i], c->stream));
         }
     } else {
-        // fixed queuing: a host wait for one context would stall the others
-        std::vector<int> adaptive(n);
+        // Each context runs its own slice loop with its own iteration policy.
+        // The host never waits on one context: it polls every context's
+        // pending decision (dec_ev) and serves the ready ones, so that all
+        // streams stay fed.  The kernels each context queues are the same as
+        // in its own qpad_step3d.
+        std::vector<SliceRun> st(n);
+        std::vector<int> next(n, 1);
+        int active = 0;
         for (int i = 0; i < n; i++) {
-            adaptive[i] = b->ctx[i]->adaptive;
-            b->ctx[i]->adaptive = 0;
+            qpad_ctx *c = b->ctx[i];
+            if (c->g.nxi < 1) continue;
+            active++;
+            TRY(slice_start(c, next[i]++, false, st[i]));
         }
-        int rc = 0;
-        for (int j = 1; j <= nxi_max && !rc; j++)
-            for (int i = 0; i < n && !rc; i++)
-                if (j <= b->ctx[i]->g.nxi) rc = slice_enqueue(b->ctx[i], j, false);
-        for (int i = 0; i < n; i++) b->ctx[i]->adaptive = adaptive[i];
-        TRY(rc);
+        while (active > 0) {
+            for (int i = 0; i < n; i++) {
+                qpad_ctx *c = b->ctx[i];
+                if (st[i].j == 0 && next[i] > c->g.nxi) continue;
+                if (st[i].waiting) {
+                    const hipError_t q = hipEventQuery(c->dec_ev);
+                    if (q == hipErrorNotReady) continue;
+                    QPAD_HIP_CHECK(q);
+                    TRY(slice_decided(c, st[i]));
+                }
+                // a slice that finished starts the next one at once
+                while (!st[i].waiting && st[i].j == 0 && next[i] <= c->g.nxi)
+                    TRY(slice_start(c, next[i]++, false, st[i]));
+                if (st[i].j == 0 && next[i] > c->g.nxi) active--;
+            }
+        }
+        (void)nxi_max;
     }
     for (int i = 0; i < n; i++) {
         qpad_ctx *c = b->ctx[i];
