@@ -25,8 +25,8 @@
 
 namespace qpad {
 
-constexpr int kSortThreads = 256;
-constexpr int kSortPer = 8;
+constexpr int kSortThreads = 512;   // 8 waves: two per SIMD with one chunk per CU
+constexpr int kSortPer = 4;
 constexpr int kSortChunk = kSortThreads * kSortPer;
 constexpr int kScanGroups = 16;       // chunk groups per bin in k_sort_scan
 constexpr int kScanBins = kSortThreads / kScanGroups;
@@ -74,33 +74,35 @@ __global__ __launch_bounds__(kSortThreads) void k_sort_hist(SortArgs a)
 }
 
 // workgroup = kScanBins bins x kScanGroups chunk groups; each thread prefixes
-// a contiguous run of at most kScanRun chunks of its bin (held in registers),
-// the groups are then prefixed in LDS; the last workgroup turns the bin
-// totals into cell_start
+// a contiguous run of at most RUN chunks of its bin (held in registers; RUN
+// is the smallest of 8, 16, 32, 64 that covers nchunks / kScanGroups), the
+// groups are then prefixed in LDS; the last workgroup turns the bin totals
+// into cell_start
 constexpr int kScanRun = 64;
 constexpr int kMaxSortChunks = kScanGroups * kScanRun;
 
+template <int RUN>
 __global__ __launch_bounds__(kSortThreads) void k_sort_scan(SortArgs a)
 {
     __shared__ int gsum[kScanGroups][kScanBins];
     const int t = threadIdx.x;
     const int lb = t % kScanBins, grp = t / kScanBins;
     const int b = blockIdx.x * kScanBins + lb;
-    const int per = (a.nchunks + kScanGroups - 1) / kScanGroups;   // <= kScanRun
+    const int per = (a.nchunks + kScanGroups - 1) / kScanGroups;   // <= RUN
     const int c0 = grp * per;
     // clamped (always valid) addresses, masked values: no branches between
     // the loads, which all issue together
     const int bc = min(b, a.nb - 1);
-    int h[kScanRun];
+    int h[RUN];
 #pragma unroll
-    for (int u = 0; u < kScanRun; u++) {
+    for (int u = 0; u < RUN; u++) {
         const int c = min(c0 + u, max(a.nchunks - 1, 0));
         const int x = a.hist[(size_t)c * a.nb + bc];
         h[u] = (b < a.nb && u < per && c0 + u < a.nchunks) ? x : 0;
     }
     int run = 0;
 #pragma unroll
-    for (int u = 0; u < kScanRun; u++) {
+    for (int u = 0; u < RUN; u++) {
         const int v = h[u];
         h[u] = run;
         run += v;
@@ -114,7 +116,7 @@ __global__ __launch_bounds__(kSortThreads) void k_sort_scan(SortArgs a)
         total += v;
     }
 #pragma unroll
-    for (int u = 0; u < kScanRun; u++) {
+    for (int u = 0; u < RUN; u++) {
         const int c = c0 + u;
         if (b < a.nb && u < per && c < a.nchunks) a.hist[(size_t)c * a.nb + b] = h[u] + goff;
     }
@@ -286,6 +288,21 @@ __global__ __launch_bounds__(kSortThreads) void k_sort_scatter(SortArgs a)
     }
 }
 
+// dynamic LDS of k_sort_scatter: bin starts + counts
+static size_t scatter_lds(int nb)
+{
+    return (sizeof(int) + sizeof(unsigned short)) * (size_t)nb;
+}
+
+// allow the scatter's LDS for the largest nr (within the default 64 KB
+// limit today; kept explicit so that a larger count type stays legal)
+int sort_kernels_init()
+{
+    QPAD_HIP_CHECK(hipFuncSetAttribute((const void *)k_sort_scatter, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                      (int)scatter_lds(8192 + 1)));
+    return 0;
+}
+
 int sort_workspace(int64_t npmax, int nr, size_t *hist_ints)
 {
     const int64_t nchunks = (npmax + kSortChunk - 1) / kSortChunk;
@@ -329,10 +346,17 @@ int launch_sort(qpad_ctx *c, int s)
     const size_t lds = sizeof(int) * (size_t)a.nb;
     if (a.nchunks > 0)
         hipLaunchKernelGGL(k_sort_hist, dim3(a.nchunks), dim3(kSortThreads), lds, c->stream, a);
-    hipLaunchKernelGGL(k_sort_scan, dim3((a.nb + kScanBins - 1) / kScanBins), dim3(kSortThreads), 0, c->stream, a);
+    {
+        const dim3 sg((a.nb + kScanBins - 1) / kScanBins), sb(kSortThreads);
+        const int per = (a.nchunks + kScanGroups - 1) / kScanGroups;
+        if (per <= 8) hipLaunchKernelGGL(k_sort_scan<8>, sg, sb, 0, c->stream, a);
+        else if (per <= 16) hipLaunchKernelGGL(k_sort_scan<16>, sg, sb, 0, c->stream, a);
+        else if (per <= 32) hipLaunchKernelGGL(k_sort_scan<32>, sg, sb, 0, c->stream, a);
+        else hipLaunchKernelGGL(k_sort_scan<kScanRun>, sg, sb, 0, c->stream, a);
+    }
     // at least one scatter workgroup: workgroup 0 also writes cell_start
     hipLaunchKernelGGL(k_sort_scatter, dim3(std::max(a.nchunks, 1)), dim3(kSortThreads),
-                       lds + sizeof(unsigned short) * (size_t)a.nb, c->stream, a);
+                       scatter_lds(a.nb), c->stream, a);
     QPAD_HIP_CHECK(hipGetLastError());
     if (n > 0) {
         std::swap(sp.x1, sp.a_x1); std::swap(sp.x2, sp.a_x2);

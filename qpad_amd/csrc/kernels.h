@@ -42,6 +42,7 @@ int launch_iteration_fields(qpad_ctx *c, int j, int l);
 // J_perp predictor for the next slice
 int launch_epilogue(qpad_ctx *c, int j);
 int slice_kernels_init();
+int sort_kernels_init();   // LDS limit of the sort scatter (k_sort.hip)
 // start of a slice after the charge deposit: q_spe, psi, Bz, Ez, 'record',
 // then E_perp and b for the first current deposit
 int launch_slice_sources(qpad_ctx *c, const double *seed);

@@ -253,7 +253,8 @@ int qpad_create(const qpad_grid *gr, qpad_ctx **out)
     if (lds_need + 1024 > prop.sharedMemPerBlock)
         return fail(QPAD_E_INVALID, "nr too large for this device's LDS");
 
-    if (field_kernels_init(lds_need) != 0 || slice_kernels_init() != 0) return QPAD_E_HIP;
+    if (field_kernels_init(lds_need) != 0 || slice_kernels_init() != 0 || sort_kernels_init() != 0)
+        return QPAD_E_HIP;
     qpad_ctx *c = new qpad_ctx{};
     c->device = dev;
     Grid &g = c->g;
