@@ -319,18 +319,22 @@ int qpad_seednet_eval(qpad_ctx *ctx, const float *x, const float *t, int32_t npo
 /* ---- batched instances ----
    No counterpart in QPAD, whose runs are one simulation each (parameter
    scans run as separate jobs).  A batch steps several independent contexts
-   (different nr, modes, beams) in one call: their slices are queued
-   round-robin, each context on its own stream, so the kernels of different
-   instances run side by side on the device.  Every context's results equal
-   its own qpad_step3d.  The contexts stay usable on their own. */
+   (different nr, modes, beams) in one call, each context on its own
+   stream, so the kernels of different instances run side by side on the
+   device.  Every context's results equal its own qpad_step3d.  The
+   contexts stay usable on their own. */
 typedef struct qpad_batch qpad_batch;
 int qpad_batch_create(qpad_ctx *const *ctxs, int32_t n, qpad_batch **out);
 int qpad_batch_destroy(qpad_batch *batch);
-/* graph != 0: each context's slice loop is captured once into a hipGraph
-   (all iter_max iterations queued; the ones after convergence return
-   immediately on the device) and replayed every step.  Needs an even nxi
-   (the particle buffers swap once per slice). */
-int qpad_batch_mode(qpad_batch *batch, int32_t graph);
+/* mode 0 (default): eager, one host thread per context queues its slices
+   with the context's own iteration policy (qpad_iteration_policy);
+   mode 1: each context's slice loop is captured once into a hipGraph (all
+   iter_max iterations queued; the ones after convergence return
+   immediately on the device) and replayed every step; needs an even nxi
+   (the particle buffers swap once per slice);
+   mode 2: eager from the calling thread alone, which polls every context's
+   pending iteration decision and serves the ready ones. */
+int qpad_batch_mode(qpad_batch *batch, int32_t mode);
 /* one 3D step of every context; total_iters_out: n entries or NULL */
 int qpad_batch_step3d(qpad_batch *batch, int32_t *total_iters_out);
 /* qpad_tune_launch on every context (captured graphs are dropped) */

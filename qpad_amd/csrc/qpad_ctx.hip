@@ -4,6 +4,8 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
+#include <thread>
 #include <cmath>
 #include <cstring>
 #include <string>
@@ -1110,7 +1112,7 @@ int qpad_kernel_time_reset(qpad_ctx *c, int32_t enable)
 // ---------------------------------------------------------------------------
 struct qpad_batch {
     std::vector<qpad_ctx *> ctx;
-    int graph = 0;
+    int graph = 0;   // mode: 0 eager (host thread per context), 1 graph, 2 eager (one host thread)
     // per context, its captured slice loops.  The kernels capture the
     // particle buffers by address, and the buffers swap once per sort: an
     // even number of times in a slice loop, once more in the step's tail
@@ -1122,6 +1124,8 @@ struct qpad_batch {
 };
 
 namespace qpad {
+
+constexpr int kBatchMaxThreads = 64;   // host threads of an eager batch step
 
 // the slice loop of one 3D step of c, captured into a graph: all iter_max
 // iterations queued (no host feedback), no timing events, no seed network.
@@ -1202,11 +1206,12 @@ int qpad_batch_tune(qpad_batch *b, int32_t reps)
     return 0;
 }
 
-int qpad_batch_mode(qpad_batch *b, int32_t graph)
+int qpad_batch_mode(qpad_batch *b, int32_t mode)
 {
     if (!b) return fail(QPAD_E_INVALID, "null batch");
-    if ((graph != 0) != (b->graph != 0)) batch_drop_graphs(b);
-    b->graph = graph != 0;
+    if (mode < 0 || mode > 2) return fail(QPAD_E_INVALID, "batch mode must be 0, 1 or 2");
+    if ((mode == 1) != (b->graph == 1)) batch_drop_graphs(b);
+    b->graph = mode;
     return 0;
 }
 
@@ -1229,7 +1234,7 @@ int qpad_batch_step3d(qpad_batch *b, int32_t *total_iters_out)
         memset(c->it_host, 0, sizeof(IterState));
         QPAD_HIP_CHECK(hipMemcpyAsync(c->it, c->it_host, sizeof(IterState), hipMemcpyHostToDevice, c->stream));
     }
-    if (b->graph) {
+    if (b->graph == 1) {
         for (int i = 0; i < n; i++) {
             qpad_ctx *c = b->ctx[i];
             const void *key = c->nspecies > 0 ? (const void *)c->sp[0].x1 : nullptr;
@@ -1245,12 +1250,38 @@ int qpad_batch_step3d(qpad_batch *b, int32_t *total_iters_out)
             }
             QPAD_HIP_CHECK(hipGraphLaunch(b->exec[k][i], c->stream));
         }
+    } else if (b->graph == 0) {
+        // one host thread per context (each runs the context's slice loop
+        // with its own iteration policy, waiting on its own decisions), so
+        // that the launches of all contexts are issued in parallel.  The
+        // kernels each context queues are the same as in its own qpad_step3d.
+        std::vector<int> rcs(n, 0);
+        std::vector<std::string> msgs(n);
+        std::atomic<int> next_ctx{0};
+        auto worker = [&]() {
+            for (;;) {
+                const int i = next_ctx.fetch_add(1);
+                if (i >= n) break;
+                qpad_ctx *c = b->ctx[i];
+                int rc = hipSetDevice(c->device) == hipSuccess ? 0 : fail(QPAD_E_HIP, "hipSetDevice failed");
+                for (int j = 1; j <= c->g.nxi && !rc; j++) rc = slice_enqueue(c, j, false);
+                if (rc) {
+                    rcs[i] = rc;
+                    msgs[i] = qpad_last_error();
+                }
+            }
+        };
+        const int nth = std::min(n, kBatchMaxThreads);
+        std::vector<std::thread> pool;
+        for (int k = 0; k < nth; k++) pool.emplace_back(worker);
+        for (auto &th : pool) th.join();
+        for (int i = 0; i < n; i++)
+            if (rcs[i]) return fail(rcs[i], msgs[i]);
     } else {
-        // Each context runs its own slice loop with its own iteration policy.
-        // The host never waits on one context: it polls every context's
-        // pending decision (dec_ev) and serves the ready ones, so that all
-        // streams stay fed.  The kernels each context queues are the same as
-        // in its own qpad_step3d.
+        // Single host thread: each context runs its own slice loop with its
+        // own iteration policy.  The host never waits on one context: it
+        // polls every context's pending decision (dec_ev) and serves the
+        // ready ones, so that all streams stay fed.
         std::vector<SliceRun> st(n);
         std::vector<int> next(n, 1);
         int active = 0;

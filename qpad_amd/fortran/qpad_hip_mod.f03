@@ -421,10 +421,11 @@ module qpad_hip_mod
       integer(c_int) :: rc
     end function
 
-    function qpad_batch_mode(batch, graph) bind(C, name="qpad_batch_mode") result(rc)
+    ! mode: 0 eager (host thread per context), 1 graph replay, 2 eager (one host thread)
+    function qpad_batch_mode(batch, mode) bind(C, name="qpad_batch_mode") result(rc)
       import :: c_int, c_int32_t, c_ptr
       type(c_ptr), value :: batch
-      integer(c_int32_t), value :: graph
+      integer(c_int32_t), value :: mode
       integer(c_int) :: rc
     end function
 

@@ -358,6 +358,8 @@ class Batch:
     stream (include/qpad_hip.h, qpad_batch_*).  Results equal each
     simulation's own step3d."""
 
+    MODES = {"eager": 0, "graph": 1, "eager_single": 2}
+
     def __init__(self, sims, graph=False):
         self.sims = list(sims)
         self.lib = self.sims[0].lib
@@ -368,7 +370,11 @@ class Batch:
         self.mode(graph)
 
     def mode(self, graph):
-        L.check(self.lib.qpad_batch_mode(self.h, int(bool(graph))))
+        """graph: True/False, or one of 'eager' (host thread per simulation),
+        'graph' (captured slice loops), 'eager_single' (one polling host
+        thread)."""
+        m = self.MODES[graph] if isinstance(graph, str) else int(bool(graph))
+        L.check(self.lib.qpad_batch_mode(self.h, m))
 
     def tune(self, reps=5):
         """Autotune every instance's launch shape; returns the choices."""

@@ -1,11 +1,12 @@
 """Batched instances (qpad_batch_*): several independent simulations of
-different Nr, modes and beams stepped in one call, eagerly (slices queued
-round-robin, one stream per instance) or by replaying each instance's
-captured slice graph.  Every instance must match the same simulation stepped
-alone with qpad_step3d bit for bit, on every step: the kernels and their
-order per instance are the same, only the instances run side by side.  The
-graph is replayed on three consecutive steps with the eager step head and
-tail (and host synchronisations) between replays."""
+different Nr, modes and beams stepped in one call, eagerly (one stream per
+instance, queued by a host thread per instance or by one polling host
+thread) or by replaying each instance's captured slice graph.  Every
+instance must match the same simulation stepped alone with qpad_step3d bit
+for bit, on every step: the kernels and their order per instance are the
+same, only the instances run side by side.  The graph is replayed on three
+consecutive steps with the eager step head and tail (and host
+synchronisations) between replays."""
 import numpy as np
 import pytest
 
@@ -38,7 +39,7 @@ def assert_same(a, b, what):
         assert np.array_equal(a[k], b[k]), f"{what}: {k} differs"
 
 
-@pytest.mark.parametrize("graph", [False, True])
+@pytest.mark.parametrize("graph", ["eager", "graph", "eager_single"])
 def test_batch_matches_instances_stepped_alone(graph):
     nsteps = 3
     ref = []
@@ -57,7 +58,7 @@ def test_batch_matches_instances_stepped_alone(graph):
         for i, sim in enumerate(sims):
             it_ref, st_ref = ref[i][step]
             assert its[i] == it_ref, f"instance {i} step {step}: {its[i]} vs {it_ref} iterations"
-            assert_same(state(sim), st_ref, f"instance {i} step {step} ({'graph' if graph else 'eager'})")
+            assert_same(state(sim), st_ref, f"instance {i} step {step} ({graph})")
     batch.close()
     for sim in sims:
         sim.close()
