@@ -952,8 +952,11 @@ __global__ __launch_bounds__(256) void k_net_apply(const double *seed, double *b
 // B_perp(xi-1), cu = the J predictor of slice j, q_spe of slice j - 1,
 // q_beam of slice j): scales and saved B_perp, then in seed mode the fused
 // inference into seed_buf (keeping the activations when training),
-// otherwise the packed inputs for the training step's forward pass
-static int queue_slice_inputs(qpad_ctx *c, hipStream_t st, int j)
+// otherwise the packed inputs for the training step's forward pass.
+// q_prev: q_spe of slice j - 1; the fused slice passes its history copy,
+// because the next slice's start overwrites the live q_spe while these
+// kernels may still run
+static int queue_slice_inputs(qpad_ctx *c, hipStream_t st, int j, const double *q_prev)
 {
     SeedNet *n = c->net;
     const int nr = c->g.nr, nm = c->g.nm, ld = c->g.ld;
@@ -965,7 +968,7 @@ static int queue_slice_inputs(qpad_ctx *c, hipStream_t st, int j)
         NetScaleArgs a;
         a.b_spe = c->f[QPAD_F_B_SPE];
         a.cu = c->f[QPAD_F_CU];
-        a.q_spe = c->f[QPAD_F_Q_SPE];
+        a.q_spe = q_prev;
         a.q_beam = q_beam;
         a.bsave = bsave;
         a.part = n->scale_part;
@@ -983,7 +986,7 @@ static int queue_slice_inputs(qpad_ctx *c, hipStream_t st, int j)
         a.bsave = bsave;
         a.bold = bold;
         a.cu = c->f[QPAD_F_CU];
-        a.q_spe = c->f[QPAD_F_Q_SPE];
+        a.q_spe = q_prev;
         a.q_beam = q_beam;
         a.part = n->scale_part;
         a.nsp = n->nscale_blk;
@@ -1019,7 +1022,7 @@ static int queue_slice_inputs(qpad_ctx *c, hipStream_t st, int j)
         a.bsave = bsave;
         a.bold = bold;
         a.cu = c->f[QPAD_F_CU];
-        a.q_spe = c->f[QPAD_F_Q_SPE];
+        a.q_spe = q_prev;
         a.q_beam = q_beam;
         a.part = n->scale_part;
         a.nsp = n->nscale_blk;
@@ -1051,15 +1054,24 @@ static int queue_slice_training(qpad_ctx *c, hipStream_t st, int j)
 
 int seednet_active(const qpad_ctx *c) { return c->net && (c->net->seed_on || c->net->train_on); }
 
-int seednet_after_epilogue(qpad_ctx *c, int j, int next)
+int seednet_mark(qpad_ctx *c)
+{
+    QPAD_HIP_CHECK(hipEventRecord(c->net->ev_main, c->stream));
+    return 0;
+}
+
+int seednet_after_epilogue(qpad_ctx *c, int j, int next, bool marked)
 {
     SeedNet *n = c->net;
-    QPAD_HIP_CHECK(hipEventRecord(n->ev_main, c->stream));
+    if (!marked) QPAD_HIP_CHECK(hipEventRecord(n->ev_main, c->stream));
     QPAD_HIP_CHECK(hipStreamWaitEvent(n->stream, n->ev_main, 0));
     if (j > 0 && n->train_on) QPAD_TRY(queue_slice_training(c, n->stream, j));
     c->net_pre = 0;
     if (next > 0) {
-        QPAD_TRY(queue_slice_inputs(c, n->stream, next));
+        // q_spe of slice next - 1 from its history copy (written by that
+        // slice's epilogue; zeroed with every history at the step start)
+        const size_t q_off = (size_t)(next - 1) * c->g.nm * c->g.ld;
+        QPAD_TRY(queue_slice_inputs(c, n->stream, next, c->h_q_spe + q_off));
         c->net_pre = next;
     }
     QPAD_HIP_CHECK(hipEventRecord(n->ev_net, n->stream));
@@ -1286,7 +1298,7 @@ int qpad_seednet_seed(qpad_ctx *c, int32_t j)
     if (j < 1 || j > c->g.nxi) return net_fail(QPAD_E_INVALID, "slice index out of range");
     SeedNet &n = *c->net;
     QPAD_TRY(seednet_quiesce(c));
-    QPAD_TRY(queue_slice_inputs(c, c->stream, j));
+    QPAD_TRY(queue_slice_inputs(c, c->stream, j, c->f[QPAD_F_Q_SPE]));
     if (n.seed_on) {
         const dim3 grid((unsigned)((c->g.nr + 255) / 256), (unsigned)c->g.nm);
         hipLaunchKernelGGL(k_net_apply, grid, dim3(256), 0, c->stream, n.seed_buf, c->f[QPAD_F_B_SPE], c->g.nr,

@@ -689,10 +689,6 @@ struct StartSrcArgs {
     const int32_t *os[kMaxSpecies];
     const double *cu;
     double *q_spe, *src;
-    // learned seed of the iteration (B_perp components 0, 1 of nodes
-    // 1..nr), copied into b_spe here, or null
-    const double *seed;
-    double *b_spe;
 };
 
 // thread per (node i = 0..nr+1, mode array k): q_spe = rho - Jz of every
@@ -748,10 +744,6 @@ __global__ __launch_bounds__(256) void k_start_src(StartSrcArgs a)
         if (a.qn[s]) v = v + qb[s];
     a.q_spe[(size_t)k * ld + i] = v;
     if (i < 1 || i > nr) return;
-    if (a.seed) {
-        FX(a.b_spe, 0, k, i) = FX(a.seed, 0, k, i);
-        FX(a.b_spe, 1, k, i) = FX(a.seed, 1, k, i);
-    }
     const FieldArr cu{a.cu, nm, ld};
     a.src[(size_t)src_psi(M, nm, k) * nr + i - 1] = -1.0 * v;
     a.src[(size_t)src_bz(M, k) * nr + i - 1] = bz_source(g, cu, k, i);
@@ -762,12 +754,10 @@ struct StartSolveArgs {
     FieldCommon g;
     const double *src, *cu;
     double *psi, *b_spe, *e;
-    double *conv_re, *conv_im;
-    IterState *it;
 };
 
-// workgroups: psi of each mode array, Bz of each, Ez of each, then one
-// 'record' workgroup (simulation_class.f03:549-558)
+// workgroups: psi of each mode array, Bz of each, Ez of each (none of them
+// reads B_perp, so they run before the iteration's seed is in place)
 template <int K>
 __global__ __launch_bounds__(kSolveThreads) void k_start_solve(StartSolveArgs a)
 {
@@ -777,24 +767,6 @@ __global__ __launch_bounds__(kSolveThreads) void k_start_solve(StartSolveArgs a)
     const int nm = g.nm, ld = g.ld, nr = g.nr, M = g.M;
     const int t = threadIdx.x;
     const int b = blockIdx.x;
-    if (b == 3 * nm) {
-        // 'record' of sum_m |B_phi,m| (b_spe component 2 is not read)
-        if (t == 0) {
-            a.it->done = 0;
-            a.it->iters = 0;
-            a.it->counter = 0;
-        }
-        for (int i = t + 1; i <= nr; i += kSolveThreads) {
-            double re = 0.0, im = 0.0;
-            for (int mm = 0; mm <= M; mm++) {
-                re = re + fabs(FX(a.b_spe, 1, kre(mm), i));
-                if (mm > 0) im = im + fabs(FX(a.b_spe, 1, kim(mm), i));
-            }
-            a.conv_re[i] = re;
-            a.conv_im[i] = im;
-        }
-        return;
-    }
     const int role = b / nm, k = b % nm, m = mode_of(k);
     const int row = role == 0 ? src_psi(M, nm, k) : (role == 1 ? src_bz(M, k) : src_ez(M, nm, k));
     double v[K], d[1][K];
@@ -830,26 +802,56 @@ __global__ __launch_bounds__(kSolveThreads) void k_start_solve(StartSolveArgs a)
 
 struct EperpArgs {
     FieldCommon g;
-    const double *b_spe, *b_beam, *psi;
+    double *b_spe;
+    const double *b_beam, *psi;
     double *b_out, *e;
+    // learned seed of the iteration (B_perp components 0, 1 of nodes
+    // 1..nr), copied into b_spe here, or null
+    const double *seed;
+    double *conv_re, *conv_im;
+    IterState *it;
 };
 
-// thread per (node i = 0..nr+1, mode array k): b = b_spe + b_beam and
-// E_perp from b and psi (field_e_class.f03:443-504)
+// thread per (node i = 0..nr+1, mode array k): the iteration's seed into
+// b_spe (when given), b = b_spe + b_beam and E_perp from b and psi
+// (field_e_class.f03:443-504); the k = 0 threads also run 'record' of
+// sum_m |B_phi,m| at their node (simulation_class.f03:549-558) and reset
+// the iteration state.  The first kernel of the slice that reads B_perp.
 __global__ __launch_bounds__(256) void k_eperp(EperpArgs a)
 {
     const FieldCommon &g = a.g;
-    const int nm = g.nm, ld = g.ld, nr = g.nr;
+    const int nm = g.nm, ld = g.ld, nr = g.nr, M = g.M;
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     const int k = blockIdx.y;
     if (i > nr + 1) return;
+    const bool inner = i >= 1 && i <= nr;
+    // B_perp of the iteration's start: the seed, or b_spe as it is
+    const double *bperp = (a.seed && inner) ? a.seed : a.b_spe;
+    if (k == 0) {
+        if (i == 0) {
+            a.it->done = 0;
+            a.it->iters = 0;
+            a.it->counter = 0;
+        }
+        if (inner) {
+            double re = 0.0, im = 0.0;
+            for (int mm = 0; mm <= M; mm++) {
+                re = re + fabs(FX(bperp, 1, kre(mm), i));
+                if (mm > 0) im = im + fabs(FX(bperp, 1, kim(mm), i));
+            }
+            a.conv_re[i] = re;
+            a.conv_im[i] = im;
+        }
+    }
     double bv[3];
 #pragma unroll
     for (int c = 0; c < 3; c++) {
-        bv[c] = FX(a.b_spe, c, k, i) + FX(a.b_beam, c, k, i);
+        const double bs = FX(c < 2 ? bperp : a.b_spe, c, k, i);
+        if (c < 2 && a.seed && inner) FX(a.b_spe, c, k, i) = bs;
+        bv[c] = bs + FX(a.b_beam, c, k, i);
         FX(a.b_out, c, k, i) = bv[c];
     }
-    if (i < 1 || i > nr) return;
+    if (!inner) return;
     double ev[2];
     et_node(g, bv[0], bv[1], FieldArr{a.psi, nm, ld}, k, i, ev);
     FX(a.e, 0, k, i) = ev[0];
@@ -858,9 +860,8 @@ __global__ __launch_bounds__(256) void k_eperp(EperpArgs a)
 
 int src_rows_host(int M, int nm) { return src_rows(M, nm); }
 
-// sources of the slice start (q_spe, the psi / Bz / Ez right-hand sides);
-// seed: the learned seed to copy into b_spe first, or null
-int launch_slice_sources(qpad_ctx *c, const double *seed)
+// sources of the slice start (q_spe, the psi / Bz / Ez right-hand sides)
+int launch_slice_sources(qpad_ctx *c)
 {
     const FieldCommon g = field_common(c);
     const int nm = c->g.nm;
@@ -879,8 +880,6 @@ int launch_slice_sources(qpad_ctx *c, const double *seed)
     s.cu = c->f[QPAD_F_CU];
     s.q_spe = c->f[QPAD_F_Q_SPE];
     s.src = c->src;
-    s.seed = seed;
-    s.b_spe = c->f[QPAD_F_B_SPE];
     switch (c->nspecies) {
     case 1: hipLaunchKernelGGL(k_start_src<1>, ngrid, dim3(256), 0, c->stream, s); break;
     case 2: hipLaunchKernelGGL(k_start_src<2>, ngrid, dim3(256), 0, c->stream, s); break;
@@ -891,13 +890,11 @@ int launch_slice_sources(qpad_ctx *c, const double *seed)
     return 0;
 }
 
-// psi, Bz from the J predictor, Ez, the first 'record' and E_perp: everything
-// of the slice start that reads b_spe (the iteration's seed)
-int launch_slice_solves(qpad_ctx *c, int j)
+// psi, Bz from the J predictor and Ez of the slice start (no B_perp read)
+int launch_start_solves(qpad_ctx *c)
 {
     const FieldCommon g = field_common(c);
     const int nm = c->g.nm;
-    const dim3 ngrid((unsigned)((c->g.nr + 2 + 255) / 256), nm);
     StartSolveArgs a;
     a.g = g;
     a.src = c->src;
@@ -905,19 +902,28 @@ int launch_slice_solves(qpad_ctx *c, int j)
     a.psi = c->f[QPAD_F_PSI];
     a.b_spe = c->f[QPAD_F_B_SPE];
     a.e = c->f[QPAD_F_E];
-    a.conv_re = c->conv_re;
-    a.conv_im = c->conv_im;
-    a.it = c->it;
-    QPAD_DISPATCH_K(c->g.nr, k_start_solve, dim3(3 * nm + 1), 0, c->stream, a);
+    QPAD_DISPATCH_K(c->g.nr, k_start_solve, dim3(3 * nm), 0, c->stream, a);
     QPAD_HIP_CHECK(hipGetLastError());
+    return 0;
+}
 
+// the iteration's seed into b_spe (seed: the learned seed or null), the
+// first 'record', b and E_perp for the first deposit
+int launch_start_eperp(qpad_ctx *c, int j, const double *seed)
+{
+    const int nm = c->g.nm;
+    const dim3 ngrid((unsigned)((c->g.nr + 2 + 255) / 256), nm);
     EperpArgs e;
-    e.g = g;
+    e.g = field_common(c);
     e.b_spe = c->f[QPAD_F_B_SPE];
     e.b_beam = c->h_b_beam + (size_t)j * 3 * nm * c->g.ld;
     e.psi = c->f[QPAD_F_PSI];
     e.b_out = c->f[QPAD_F_B];
     e.e = c->f[QPAD_F_E];
+    e.seed = seed;
+    e.conv_re = c->conv_re;
+    e.conv_im = c->conv_im;
+    e.it = c->it;
     hipLaunchKernelGGL(k_eperp, ngrid, dim3(256), 0, c->stream, e);
     QPAD_HIP_CHECK(hipGetLastError());
     return 0;

@@ -43,21 +43,27 @@ int launch_iteration_fields(qpad_ctx *c, int j, int l);
 int launch_epilogue(qpad_ctx *c, int j);
 int slice_kernels_init();
 int sort_kernels_init();   // LDS limit of the sort scatter (k_sort.hip)
-// start of a slice after the charge deposit: q_spe, psi, Bz, Ez, 'record',
-// then E_perp and b for the first current deposit
-int launch_slice_sources(qpad_ctx *c, const double *seed);
-int launch_slice_solves(qpad_ctx *c, int j);
+// start of a slice after the charge deposit: q_spe and the right-hand
+// sides, the psi, Bz and Ez solves (none reads B_perp), then the
+// iteration's seed into b_spe (seed, or null to keep b_spe), 'record', b
+// and E_perp for the first current deposit
+int launch_slice_sources(qpad_ctx *c);
+int launch_start_solves(qpad_ctx *c);
+int launch_start_eperp(qpad_ctx *c, int j, const double *seed);
 
 // learned B_perp seed (k_seednet.hip), on its own stream beside the slice:
 // seednet_before_sources makes the main stream wait for slice j's seed (and
 // queues its inputs and inference first if they were not queued at the end
 // of slice j - 1) and returns the seed to copy into b_spe (null without one);
 // seednet_after_epilogue queues slice j's Adam step (train mode) and slice
-// j + 1's inputs and inference (next > 0), behind the epilogue;
+// j + 1's inputs and inference (next > 0), behind the epilogue: behind the
+// point seednet_mark recorded on the main stream (marked), or behind the
+// main stream's current end;
 // seednet_quiesce makes the main stream wait for all of the network's work
 int seednet_active(const qpad_ctx *c);
 int seednet_before_sources(qpad_ctx *c, int j, const double **seed);
-int seednet_after_epilogue(qpad_ctx *c, int j, int next);
+int seednet_mark(qpad_ctx *c);
+int seednet_after_epilogue(qpad_ctx *c, int j, int next, bool marked = false);
 int seednet_quiesce(qpad_ctx *c);
 int seednet_step_begin(qpad_ctx *c);
 void seednet_destroy(qpad_ctx *c);

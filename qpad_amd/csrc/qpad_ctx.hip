@@ -629,14 +629,16 @@ static int slice_start(qpad_ctx *c, int j, bool overlap, SliceRun &st)
 {
     // the learned seed (when on) is computed on the network's stream from
     // the fields left by slice j - 1, beside the previous slice's push and
-    // sort and this slice's charge deposit
+    // sort and this slice's charge deposit, sources and solves
     const bool net = seednet_active(c);
     const double *seed = nullptr;
     if (net && c->net_pre != j) TRY(seednet_after_epilogue(c, 0, j));
     for (int s = 0; s < c->nspecies; s++) TRY(launch_qdeposit(c, s));
+    TRY(launch_slice_sources(c));
+    TRY(launch_start_solves(c));
+    // the first kernel that reads B_perp waits for the seed
     if (net) TRY(seednet_before_sources(c, j, &seed));
-    TRY(launch_slice_sources(c, seed));
-    TRY(launch_slice_solves(c, j));
+    TRY(launch_start_eperp(c, j, seed));
     const int imax = c->g.iter_max;
     st = SliceRun();
     st.j = j;
@@ -649,17 +651,22 @@ static int slice_start(qpad_ctx *c, int j, bool overlap, SliceRun &st)
 }
 
 // slice epilogue: cbq, e_spe, history (simulation_class.f03:401-456), then
-// the push and the sort
+// the push and the sort.  The network's work behind the epilogue (the next
+// seed, training) is queued after the push and the sort: the host reaches
+// this point only once the slice's convergence is known, and queuing the
+// network first left the main stream idle while the host was busy with it.
 static int slice_finish(qpad_ctx *c, SliceRun &st)
 {
     const int j = st.j;
     if (c->adaptive) c->spec_guess = st.known_iters;
     TRY(launch_epilogue(c, j));
-    if (seednet_active(c)) TRY(seednet_after_epilogue(c, j, (st.overlap && j < c->g.nxi) ? j + 1 : 0));
+    const bool net = seednet_active(c);
+    if (net) TRY(seednet_mark(c));
     for (int s = 0; s < c->nspecies; s++) {
         TRY(launch_push(c, s));
         TRY(launch_sort(c, s));
     }
+    if (net) TRY(seednet_after_epilogue(c, j, (st.overlap && j < c->g.nxi) ? j + 1 : 0, true));
     st.j = 0;
     return 0;
 }
