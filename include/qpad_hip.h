@@ -245,7 +245,7 @@ int qpad_kernel_time_reset(qpad_ctx *ctx, int32_t enable);
    and charge, the J predictor of slice xi) and the beam charge of slice xi
    (the output layer starts at zero: untrained, the seed is the
    extrapolation).  The iteration still runs to the same tolerance and
-   iter_max, so the plain method remains the judge.  Since the inputs are
+   iter_max, so the plain method still decides convergence.  Since the inputs are
    ready when slice xi-1 ends, qpad_step3d computes each seed on a second
    stream beside the previous slice's push and sort.
 

@@ -16,7 +16,7 @@
 
 namespace qpad {
 
-constexpr int kMaxMode = 4;          // modes m = 0..4
+constexpr int kMaxMode = 4;          // modes m = 0..4 (the most any benchmark deck uses)
 constexpr int kMaxNm = 2 * kMaxMode + 1;
 constexpr int kMaxSpecies = 4;
 constexpr int kMaxBeams = 4;

@@ -52,7 +52,7 @@ class ErrorNorm:
     """Norm-wise relative error over a whole run: for each quantity,
     max over slices of max |gpu - oracle| divided by max over slices of
     max |oracle|.  Ahead of the beam the plasma response is round-off noise;
-    judged against the quantity's own scale over the run, that noise counts
+    measured against the quantity's own scale over the run, that noise counts
     as the absolute error it is."""
 
     def __init__(self):
