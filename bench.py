@@ -2,7 +2,7 @@
 
 python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload c2|c1|c3]
                 [--seed learned|plain] [--train-steps T] [--cpu-slices S] [--no-cpu]
-                [--no-c4]
+                [--no-c4 | --only-c4] [--c4-eager eager|eager_single]
 
 A step is one 3D step of the workload: every xi-slice of the plasma response
 (with every B_perp predictor-corrector iteration), the beam deposit, the beam
@@ -51,6 +51,11 @@ def parse():
                     help="extra untimed 3D steps of online training of the seed network")
     ap.add_argument("--no-c4", action="store_true",
                     help="skip the batched-instances scan (C4) section")
+    ap.add_argument("--only-c4", action="store_true",
+                    help="run only the batched-instances scan (C4) and print its section as JSON")
+    ap.add_argument("--c4-eager", default="eager", choices=["eager", "eager_single"],
+                    help="eager batch mode of the C4 scan: a host thread per instance, or one "
+                         "polling host thread")
     ap.add_argument("--profile-slices", type=int, default=0,
                     help="run only a 512 MiB calibration copy and this many slices of the workload "
                          "(for rocprofv3 counter runs), print nothing")
@@ -112,7 +117,7 @@ def algorithmic_bytes_per_step(grid, np_per_slice, iters, nbeam):
     return b
 
 
-def batch_c4(reps=2):
+def batch_c4(reps=2, eager="eager"):
     """The C4 scan (16 independent instances of random Nr, modes, beam charge
     and offset, Nxi 512, from a recorded seed): one untimed 3D step each,
     then `reps` timed 3D steps of all 16 stepped one after another, as one
@@ -134,6 +139,7 @@ def batch_c4(reps=2):
 
     t_seq, its = timed(lambda: [s_.step3d() for s_ in sims])
     batch = Batch(sims)
+    batch.mode(eager)
     t_eager, _ = timed(batch.step3d)
     batch.mode(True)
     batch.step3d()
@@ -148,7 +154,7 @@ def batch_c4(reps=2):
     for s_ in sims:
         s_.close()
     n = len(sims)
-    return {"instances": n, "params": params, "iters_per_slice": sum(its[-1]) / nsl,
+    return {"instances": n, "params": params, "iters_per_slice": sum(its[-1]) / nsl, "eager_mode": eager,
             "sequential_instance_steps_per_s": n / t_seq, "batch_eager_instance_steps_per_s": n / t_eager,
             "batch_graph_instance_steps_per_s": n / t_graph, "batch_graph_tuned_instance_steps_per_s": n / t_tuned,
             "tuned_dep_cells": shapes, "batch_graph_tuned_speedup_vs_sequential": t_seq / t_tuned,
@@ -248,6 +254,9 @@ def main():
         print(json.dumps({"error": "this build targets one MI355X: multi-GPU runs are not "
                                    "part of it (use --gpus 1)", "n_gpus": args.gpus}))
         sys.exit(2)
+    if args.only_c4:
+        print(json.dumps({"batch_c4": batch_c4(eager=args.c4_eager)}))
+        return
     from qpad_amd import SeedNet, Simulation
     deck = deck_for(args.workload)
     legs = {"plain": Simulation(deck), "learned": Simulation(deck)}
@@ -378,7 +387,7 @@ def main():
     }
     legs["learned"].close()
     if not args.no_c4:
-        result["batch_c4"] = batch_c4()
+        result["batch_c4"] = batch_c4(eager=args.c4_eager)
     if not args.no_cpu:
         # the oracle is the plain method: replay window of the plain leg (re-run
         # its step up to j0 - 1 and snapshot there), scaled by its iterations
