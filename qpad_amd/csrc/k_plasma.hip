@@ -511,12 +511,19 @@ __device__ __forceinline__ void deposit_particle(const JDepArgs &a, const NodePa
     }
 }
 
+// the arguments of every species of one launch; workgroup row blockIdx.y
+// deposits species blockIdx.y
+struct JDepMulti {
+    JDepArgs s[kMaxSpecies];
+};
+
 // SPEC: a speculative launch of iteration >= 2 of a slice, which returns at
 // once when the iteration has already converged (a separate instance, so
 // that profiles show the first-iteration launches, which always work, apart)
 template <int M, bool SPEC, int CPW>
-__global__ __launch_bounds__(64 * CPW) void k_amjdeposit(JDepArgs a)
+__global__ __launch_bounds__(64 * CPW) void k_amjdeposit(const JDepMulti m)
 {
+    const JDepArgs a = m.s[__builtin_amdgcn_readfirstlane(blockIdx.y)];
     constexpr int NM = 2 * M + 1;
     constexpr int NV = 8 * NM;
     constexpr int NF = 12 * NM;   // staged field values per cell
@@ -614,7 +621,7 @@ __global__ __launch_bounds__(64 * CPW) void k_amjdeposit(JDepArgs a)
 void time_begin(qpad_ctx *c, int which, hipEvent_t *stop);
 void time_end(qpad_ctx *c, int which, hipEvent_t stop);
 
-int launch_amjdeposit(qpad_ctx *c, int s, int l, bool store_diag, bool timed, bool report)
+static JDepArgs jdep_args(qpad_ctx *c, int s, int l, bool store_diag, bool report)
 {
     const bool speculative = l >= 2;
     Species &sp = c->sp[s];
@@ -646,14 +653,29 @@ int launch_amjdeposit(qpad_ctx *c, int s, int l, bool store_diag, bool timed, bo
     a.O0 = sp.O0;
     a.O1 = sp.O1;
     a.n_ov_max = sp.n_ov_max;
+    return a;
+}
+
+// species s0 .. s1-1 in one launch (grid rows by species)
+static int launch_jdep(qpad_ctx *c, int s0, int s1, int l, bool store_diag, bool timed, bool report)
+{
+    const bool speculative = l >= 2;
+    JDepMulti m;
+    memset(&m, 0, sizeof(m));
     const int cpw = c->dep_cells;
-    dim3 grid(dep_grid(c->g.nr, sp.npp, sp.n_ov_max, cpw)), blk(64 * cpw);
+    int gx = 1;
+    for (int s = s0; s < s1; s++) {
+        // the report (the host's copy of the decision) from the first species only
+        m.s[s - s0] = jdep_args(c, s, l, store_diag, report && s == s0);
+        gx = std::max(gx, dep_grid(c->g.nr, c->sp[s].npp, c->sp[s].n_ov_max, cpw));
+    }
+    dim3 grid((unsigned)gx, (unsigned)(s1 - s0)), blk(64 * cpw);
     hipEvent_t stop = nullptr;
     if (timed) time_begin(c, 0, &stop);
 #define QPAD_DEP2(MM, CC)                                                                              \
     do {                                                                                               \
-        if (speculative) hipLaunchKernelGGL((k_amjdeposit<MM, true, CC>), grid, blk, 0, c->stream, a); \
-        else hipLaunchKernelGGL((k_amjdeposit<MM, false, CC>), grid, blk, 0, c->stream, a);            \
+        if (speculative) hipLaunchKernelGGL((k_amjdeposit<MM, true, CC>), grid, blk, 0, c->stream, m); \
+        else hipLaunchKernelGGL((k_amjdeposit<MM, false, CC>), grid, blk, 0, c->stream, m);            \
     } while (0)
 #define QPAD_DEP(MM)                                 \
     do {                                             \
@@ -673,6 +695,17 @@ int launch_amjdeposit(qpad_ctx *c, int s, int l, bool store_diag, bool timed, bo
     QPAD_HIP_CHECK(hipGetLastError());
     if (timed) time_end(c, 0, stop);
     return 0;
+}
+
+int launch_amjdeposit(qpad_ctx *c, int s, int l, bool store_diag, bool timed, bool report)
+{
+    return launch_jdep(c, s, s + 1, l, store_diag, timed, report);
+}
+
+int launch_amjdeposit_all(qpad_ctx *c, int l, bool store_diag, bool timed, bool report)
+{
+    if (c->nspecies == 0) return 0;
+    return launch_jdep(c, 0, c->nspecies, l, store_diag, timed, report);
 }
 
 // ---------------------------------------------------------------------------

@@ -84,6 +84,9 @@ int launch_combine_q(qpad_ctx *c, int sp, double *out, double scale, bool accumu
 // and returns at once when it has converged; l = 0 for a plain deposit);
 // timed: bracket the launch with the timing events of pool 0
 int launch_amjdeposit(qpad_ctx *c, int sp, int l, bool store_diag, bool timed, bool report = false);
+// the same for every species in one launch (the workgroup rows are the
+// species); report: from species 0
+int launch_amjdeposit_all(qpad_ctx *c, int l, bool store_diag, bool timed, bool report = false);
 // workgroups of the B_perp field kernel (one convergence maximum pair each)
 inline int conv_blocks(int nr) { return (nr + 2 + 255) / 256; }
 int launch_push(qpad_ctx *c, int sp);            // push_u + push_x + keys

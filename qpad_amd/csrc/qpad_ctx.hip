@@ -582,7 +582,7 @@ static int slice_advance(qpad_ctx *c, SliceRun &st)
     const int j = st.j, imax = c->g.iter_max;
     for (int l = st.queued + 1; l <= st.upto; l++) {
         const bool first = l == 1;
-        for (int s = 0; s < c->nspecies; s++) TRY(launch_amjdeposit(c, s, l, true, first));
+        TRY(launch_amjdeposit_all(c, l, true, first));
         hipEvent_t stop = nullptr;
         if (first) time_begin(c, 2, &stop);
         TRY(launch_iteration_fields(c, j, l));
@@ -592,9 +592,8 @@ static int slice_advance(qpad_ctx *c, SliceRun &st)
     if (st.queued >= imax) return slice_finish(c, st);
     // iteration queued + 1: its deposit decides on iteration `queued`
     const int l = st.queued + 1;
-    TRY(launch_amjdeposit(c, 0, l, true, false, true));
+    TRY(launch_amjdeposit_all(c, l, true, false, true));
     QPAD_HIP_CHECK(hipEventRecord(c->dec_ev, c->stream));
-    for (int s = 1; s < c->nspecies; s++) TRY(launch_amjdeposit(c, s, l, true, false));
     TRY(launch_iteration_fields(c, j, l));
     st.queued = l;
     st.waiting = true;
