@@ -246,9 +246,15 @@ inline int dep_grid(int nr, int64_t npp, int n_ov_max, int cpw)
     return (nr + cpw - 1) / cpw + (nov + cpw - 1) / cpw;
 }
 
+// the arguments of every species of one launch (workgroup rows by species)
+struct QDepMulti {
+    QDepArgs s[kMaxSpecies];
+};
+
 template <int M, int CPW>
-__global__ __launch_bounds__(64 * CPW) void k_qdeposit(QDepArgs a)
+__global__ __launch_bounds__(64 * CPW) void k_qdeposit(const QDepMulti m)
 {
+    const QDepArgs a = m.s[__builtin_amdgcn_readfirstlane(blockIdx.y)];
     constexpr int NM = 2 * M + 1;
     const int lane = threadIdx.x & 63;
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
@@ -292,7 +298,7 @@ __global__ __launch_bounds__(64 * CPW) void k_qdeposit(QDepArgs a)
     }
 }
 
-int launch_qdeposit(qpad_ctx *c, int s)
+static QDepArgs qdep_args(qpad_ctx *c, int s)
 {
     Species &sp = c->sp[s];
     QDepArgs a;
@@ -307,9 +313,26 @@ int launch_qdeposit(qpad_ctx *c, int s)
     a.O0 = sp.O0;
     a.O1 = sp.O1;
     a.n_ov_max = sp.n_ov_max;
+    return a;
+}
+
+int launch_qdeposit(qpad_ctx *c, int s) { return launch_qdeposit_range(c, s, s + 1); }
+
+int launch_qdeposit_all(qpad_ctx *c) { return c->nspecies ? launch_qdeposit_range(c, 0, c->nspecies) : 0; }
+
+// species s0 .. s1-1 in one launch
+int launch_qdeposit_range(qpad_ctx *c, int s0, int s1)
+{
+    QDepMulti m;
+    memset(&m, 0, sizeof(m));
     const int cpw = c->dep_cells;
-    dim3 grid(dep_grid(c->g.nr, sp.npp, sp.n_ov_max, cpw)), blk(64 * cpw);
-#define QPAD_QDEP(MM, CC) hipLaunchKernelGGL((k_qdeposit<MM, CC>), grid, blk, 0, c->stream, a)
+    int gx = 1;
+    for (int s = s0; s < s1; s++) {
+        m.s[s - s0] = qdep_args(c, s);
+        gx = std::max(gx, dep_grid(c->g.nr, c->sp[s].npp, c->sp[s].n_ov_max, cpw));
+    }
+    dim3 grid((unsigned)gx, (unsigned)(s1 - s0)), blk(64 * cpw);
+#define QPAD_QDEP(MM, CC) hipLaunchKernelGGL((k_qdeposit<MM, CC>), grid, blk, 0, c->stream, m)
 #define QPAD_QDEP_M(MM)                              \
     do {                                             \
         if (cpw == 2) QPAD_QDEP(MM, 2);              \

@@ -78,6 +78,8 @@ int launch_cell_start(qpad_ctx *c, const uint32_t *key, int64_t n, int ncs, int3
 int launch_sort(qpad_ctx *c, int sp);            // sort by key, permute, cell starts
 int launch_keys(qpad_ctx *c, int sp);            // keys of the current positions
 int launch_qdeposit(qpad_ctx *c, int sp);        // per-cell partials of rho - Jz
+int launch_qdeposit_range(qpad_ctx *c, int s0, int s1);   // species s0 .. s1-1 in one launch
+int launch_qdeposit_all(qpad_ctx *c);            // every species in one launch
 int launch_combine_q(qpad_ctx *c, int sp, double *out, double scale, bool accumulate);
 // l: predictor-corrector iteration of the slice (1-based; l >= 2 is a
 // speculative launch that first evaluates the convergence of iteration l-1

@@ -632,7 +632,7 @@ static int slice_start(qpad_ctx *c, int j, bool overlap, SliceRun &st)
     const bool net = seednet_active(c);
     const double *seed = nullptr;
     if (net && c->net_pre != j) TRY(seednet_after_epilogue(c, 0, j));
-    for (int s = 0; s < c->nspecies; s++) TRY(launch_qdeposit(c, s));
+    TRY(launch_qdeposit_all(c));
     TRY(launch_slice_sources(c));
     TRY(launch_start_solves(c));
     // the first kernel that reads B_perp waits for the seed
