@@ -746,9 +746,15 @@ struct PushArgs {
     const double *e, *b;
 };
 
+// the arguments of every species of one launch (workgroup rows by species)
+struct PushMulti {
+    PushArgs s[kMaxSpecies];
+};
+
 template <int M>
-__global__ __launch_bounds__(256) void k_push(PushArgs a)
+__global__ __launch_bounds__(256) void k_push(const PushMulti m)
 {
+    const PushArgs a = m.s[__builtin_amdgcn_readfirstlane(blockIdx.y)];
     const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= a.nslots) return;
     const int npp = a.cs[a.nr];
@@ -805,10 +811,9 @@ __global__ __launch_bounds__(256) void k_push(PushArgs a)
     a.key[p] = cell_key(x1, x2, a.idr, a.edge, a.nr);
 }
 
-int launch_push(qpad_ctx *c, int s)
+static PushArgs push_args(qpad_ctx *c, int s)
 {
     Species &sp = c->sp[s];
-    if (sp.npp == 0) return 0;
     PushArgs a;
     a.nr = c->g.nr;
     a.ld = c->g.ld;
@@ -823,20 +828,38 @@ int launch_push(qpad_ctx *c, int s)
     a.key = sp.key;
     a.e = c->f[QPAD_F_E];
     a.b = c->f[QPAD_F_B];
-    dim3 grid((unsigned)((sp.npp + 255) / 256)), blk(256);
+    return a;
+}
+
+// species s0 .. s1-1 in one launch (species without particles get idle rows)
+static int launch_push_range(qpad_ctx *c, int s0, int s1)
+{
+    PushMulti m;
+    memset(&m, 0, sizeof(m));
+    int64_t nmax = 0;
+    for (int s = s0; s < s1; s++) {
+        m.s[s - s0] = push_args(c, s);
+        nmax = std::max(nmax, c->sp[s].npp);
+    }
+    if (nmax == 0) return 0;
+    dim3 grid((unsigned)((nmax + 255) / 256), (unsigned)(s1 - s0)), blk(256);
     hipEvent_t stop = nullptr;
     time_begin(c, 1, &stop);
     switch (c->g.M) {
-    case 0: hipLaunchKernelGGL(k_push<0>, grid, blk, 0, c->stream, a); break;
-    case 1: hipLaunchKernelGGL(k_push<1>, grid, blk, 0, c->stream, a); break;
-    case 2: hipLaunchKernelGGL(k_push<2>, grid, blk, 0, c->stream, a); break;
-    case 3: hipLaunchKernelGGL(k_push<3>, grid, blk, 0, c->stream, a); break;
-    default: hipLaunchKernelGGL(k_push<4>, grid, blk, 0, c->stream, a); break;
+    case 0: hipLaunchKernelGGL(k_push<0>, grid, blk, 0, c->stream, m); break;
+    case 1: hipLaunchKernelGGL(k_push<1>, grid, blk, 0, c->stream, m); break;
+    case 2: hipLaunchKernelGGL(k_push<2>, grid, blk, 0, c->stream, m); break;
+    case 3: hipLaunchKernelGGL(k_push<3>, grid, blk, 0, c->stream, m); break;
+    default: hipLaunchKernelGGL(k_push<4>, grid, blk, 0, c->stream, m); break;
     }
     QPAD_HIP_CHECK(hipGetLastError());
     time_end(c, 1, stop);
     return 0;
 }
+
+int launch_push(qpad_ctx *c, int s) { return launch_push_range(c, s, s + 1); }
+
+int launch_push_all(qpad_ctx *c) { return c->nspecies ? launch_push_range(c, 0, c->nspecies) : 0; }
 
 // q_spe += qn (the neutralising background of species s)
 __global__ void k_axpy(double *y, const double *x, int64_t n)
