@@ -50,8 +50,15 @@ struct SortArgs {
     int n_ov_max;
 };
 
-__global__ __launch_bounds__(kSortThreads) void k_sort_hist(SortArgs a)
+// the arguments of every species of one launch (workgroup rows by species)
+struct SortMulti {
+    SortArgs s[kMaxSpecies];
+};
+
+__global__ __launch_bounds__(kSortThreads) void k_sort_hist(const SortMulti m)
 {
+    const SortArgs a = m.s[__builtin_amdgcn_readfirstlane(blockIdx.y)];
+    if ((int)blockIdx.x >= a.nchunks) return;   // the species has fewer chunks than the grid
     extern __shared__ int cnt[];
     const int t = threadIdx.x;
     const int64_t e0 = (int64_t)blockIdx.x * kSortChunk;
@@ -82,8 +89,9 @@ constexpr int kScanRun = 64;
 constexpr int kMaxSortChunks = kScanGroups * kScanRun;
 
 template <int RUN>
-__global__ __launch_bounds__(kSortThreads) void k_sort_scan(SortArgs a)
+__global__ __launch_bounds__(kSortThreads) void k_sort_scan(const SortMulti m)
 {
+    const SortArgs a = m.s[__builtin_amdgcn_readfirstlane(blockIdx.y)];
     __shared__ int gsum[kScanGroups][kScanBins];
     const int t = threadIdx.x;
     const int lb = t % kScanBins, grp = t / kScanBins;
@@ -211,8 +219,11 @@ __device__ inline void overflow_chunks(const SortArgs &a, const int *base)
     if (t == kSortThreads - 1) a.os[nr] = min(run, a.n_ov_max);
 }
 
-__global__ __launch_bounds__(kSortThreads) void k_sort_scatter(SortArgs a)
+__global__ __launch_bounds__(kSortThreads) void k_sort_scatter(const SortMulti m)
 {
+    const SortArgs a = m.s[__builtin_amdgcn_readfirstlane(blockIdx.y)];
+    // at least one workgroup per species (workgroup 0 writes cell_start)
+    if ((int)blockIdx.x >= max(a.nchunks, 1)) return;
     extern __shared__ int base[];                                  // [nb] bin starts
     unsigned short *cnt = reinterpret_cast<unsigned short *>(base + a.nb);   // [nb] < kSortChunk
     const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
@@ -315,7 +326,7 @@ int sort_workspace(int64_t npmax, int nr, size_t *hist_ints)
     return 0;
 }
 
-int launch_sort(qpad_ctx *c, int s)
+static SortArgs sort_args(qpad_ctx *c, int s)
 {
     Species &sp = c->sp[s];
     const int64_t n = sp.npp;
@@ -343,29 +354,50 @@ int launch_sort(qpad_ctx *c, int s)
     a.os = sp.ov_start;
     a.desc = sp.ov_desc;
     a.n_ov_max = sp.n_ov_max;
-    const size_t lds = sizeof(int) * (size_t)a.nb;
-    if (a.nchunks > 0)
-        hipLaunchKernelGGL(k_sort_hist, dim3(a.nchunks), dim3(kSortThreads), lds, c->stream, a);
-    {
-        const dim3 sg((a.nb + kScanBins - 1) / kScanBins), sb(kSortThreads);
-        const int per = (a.nchunks + kScanGroups - 1) / kScanGroups;
-        if (per <= 8) hipLaunchKernelGGL(k_sort_scan<8>, sg, sb, 0, c->stream, a);
-        else if (per <= 16) hipLaunchKernelGGL(k_sort_scan<16>, sg, sb, 0, c->stream, a);
-        else if (per <= 32) hipLaunchKernelGGL(k_sort_scan<32>, sg, sb, 0, c->stream, a);
-        else hipLaunchKernelGGL(k_sort_scan<kScanRun>, sg, sb, 0, c->stream, a);
+    return a;
+}
+
+// species s0 .. s1-1 sorted by the same three launches (workgroup rows by
+// species; each species' rows beyond its own chunk count return at once)
+static int launch_sort_range(qpad_ctx *c, int s0, int s1)
+{
+    SortMulti m;
+    memset(&m, 0, sizeof(m));
+    int maxc = 0;
+    for (int s = s0; s < s1; s++) {
+        m.s[s - s0] = sort_args(c, s);
+        maxc = std::max(maxc, m.s[s - s0].nchunks);
     }
-    // at least one scatter workgroup: workgroup 0 also writes cell_start
-    hipLaunchKernelGGL(k_sort_scatter, dim3(std::max(a.nchunks, 1)), dim3(kSortThreads),
-                       scatter_lds(a.nb), c->stream, a);
+    const int ns = s1 - s0, nb = c->g.nr + 1;
+    const size_t lds = sizeof(int) * (size_t)nb;
+    if (maxc > 0) hipLaunchKernelGGL(k_sort_hist, dim3(maxc, ns), dim3(kSortThreads), lds, c->stream, m);
+    {
+        const dim3 sg((nb + kScanBins - 1) / kScanBins, ns), sb(kSortThreads);
+        const int per = (maxc + kScanGroups - 1) / kScanGroups;
+        if (per <= 8) hipLaunchKernelGGL(k_sort_scan<8>, sg, sb, 0, c->stream, m);
+        else if (per <= 16) hipLaunchKernelGGL(k_sort_scan<16>, sg, sb, 0, c->stream, m);
+        else if (per <= 32) hipLaunchKernelGGL(k_sort_scan<32>, sg, sb, 0, c->stream, m);
+        else hipLaunchKernelGGL(k_sort_scan<kScanRun>, sg, sb, 0, c->stream, m);
+    }
+    // at least one scatter workgroup per species: workgroup 0 also writes cell_start
+    hipLaunchKernelGGL(k_sort_scatter, dim3(std::max(maxc, 1), ns), dim3(kSortThreads), scatter_lds(nb), c->stream,
+                       m);
     QPAD_HIP_CHECK(hipGetLastError());
-    if (n > 0) {
-        std::swap(sp.x1, sp.a_x1); std::swap(sp.x2, sp.a_x2);
-        std::swap(sp.p1, sp.a_p1); std::swap(sp.p2, sp.a_p2); std::swap(sp.p3, sp.a_p3);
-        std::swap(sp.q, sp.a_q); std::swap(sp.gamma, sp.a_gamma); std::swap(sp.psi, sp.a_psi);
-        std::swap(sp.id, sp.a_id);
-        std::swap(sp.key, sp.key_alt);
+    for (int s = s0; s < s1; s++) {
+        Species &sp = c->sp[s];
+        if (sp.npp > 0) {
+            std::swap(sp.x1, sp.a_x1); std::swap(sp.x2, sp.a_x2);
+            std::swap(sp.p1, sp.a_p1); std::swap(sp.p2, sp.a_p2); std::swap(sp.p3, sp.a_p3);
+            std::swap(sp.q, sp.a_q); std::swap(sp.gamma, sp.a_gamma); std::swap(sp.psi, sp.a_psi);
+            std::swap(sp.id, sp.a_id);
+            std::swap(sp.key, sp.key_alt);
+        }
     }
     return 0;
 }
+
+int launch_sort(qpad_ctx *c, int s) { return launch_sort_range(c, s, s + 1); }
+
+int launch_sort_all(qpad_ctx *c) { return c->nspecies ? launch_sort_range(c, 0, c->nspecies) : 0; }
 
 }  // namespace qpad

@@ -76,6 +76,7 @@ int launch_inject(qpad_ctx *c, int sp);
 // cell_start[c] = lower bound of c in the sorted keys, c = 0 .. ncs-1
 int launch_cell_start(qpad_ctx *c, const uint32_t *key, int64_t n, int ncs, int32_t *cs);
 int launch_sort(qpad_ctx *c, int sp);            // sort by key, permute, cell starts
+int launch_sort_all(qpad_ctx *c);                // the same for every species in one set of launches
 int launch_keys(qpad_ctx *c, int sp);            // keys of the current positions
 int launch_qdeposit(qpad_ctx *c, int sp);        // per-cell partials of rho - Jz
 int launch_qdeposit_range(qpad_ctx *c, int s0, int s1);   // species s0 .. s1-1 in one launch

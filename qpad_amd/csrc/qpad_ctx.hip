@@ -662,7 +662,7 @@ static int slice_finish(qpad_ctx *c, SliceRun &st)
     const bool net = seednet_active(c);
     if (net) TRY(seednet_mark(c));
     TRY(launch_push_all(c));
-    for (int s = 0; s < c->nspecies; s++) TRY(launch_sort(c, s));
+    TRY(launch_sort_all(c));
     if (net) TRY(seednet_after_epilogue(c, j, (st.overlap && j < c->g.nxi) ? j + 1 : 0, true));
     st.j = 0;
     return 0;
