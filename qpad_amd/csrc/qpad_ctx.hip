@@ -137,13 +137,16 @@ void time_begin(qpad_ctx *c, int which, hipEvent_t *stop)
     if (!c->timing) return;
     EventPool &p = c->pools[which];
     if (p.used * 2 + 2 > p.ev.size()) {
-        if (p.ev.size() < 8192) {
-            size_t old = p.ev.size();
-            p.ev.resize(old + 512);
-            for (size_t i = old; i < p.ev.size(); i++) (void)hipEventCreate(&p.ev[i]);
-        } else {
-            pool_drain(c, p);
+        // grow the pool up to 8192 events; when full, or when an event cannot
+        // be created, read out the recorded pairs and reuse them
+        const size_t want = std::min<size_t>(8192, p.ev.size() + 512);
+        while (p.ev.size() < want) {
+            hipEvent_t e = nullptr;
+            if (hipEventCreate(&e) != hipSuccess) break;
+            p.ev.push_back(e);
         }
+        if (p.used * 2 + 2 > p.ev.size()) pool_drain(c, p);
+        if (p.ev.size() < 2) return;
     }
     (void)hipEventRecord(p.ev[2 * p.used], c->stream);
     *stop = p.ev[2 * p.used + 1];
@@ -1228,11 +1231,8 @@ int qpad_batch_step3d(qpad_batch *b, int32_t *total_iters_out)
 {
     if (!b) return fail(QPAD_E_INVALID, "null batch");
     const int n = (int)b->ctx.size();
-    int nxi_max = 0;
-    for (qpad_ctx *c : b->ctx) {
+    for (qpad_ctx *c : b->ctx)
         if (seednet_active(c)) return fail(QPAD_E_STATE, "batched contexts step without the seed network");
-        nxi_max = std::max(nxi_max, c->g.nxi);
-    }
     for (qpad_ctx *c : b->ctx) {
         TRY(step_begin_enqueue(c));
         memset(c->it_host, 0, sizeof(IterState));
@@ -1311,7 +1311,6 @@ int qpad_batch_step3d(qpad_batch *b, int32_t *total_iters_out)
                 if (st[i].j == 0 && next[i] > c->g.nxi) active--;
             }
         }
-        (void)nxi_max;
     }
     for (int i = 0; i < n; i++) {
         qpad_ctx *c = b->ctx[i];
