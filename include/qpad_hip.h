@@ -56,7 +56,7 @@ extern "C" {
 #define QPAD_F_Q_BEAM 11
 #define QPAD_NFIELD 12
 
-/* boundary conditions (source/param.f03:69-71) */
+/* boundary conditions (source/param.f03:71-74) */
 #define QPAD_BND_ZERO 2
 #define QPAD_BND_OPEN 3
 
@@ -149,7 +149,7 @@ int qpad_field_solve_djdxi(qpad_ctx *ctx);
 /* field_b%solve(djdxi, jay): plasma B_perp predictor-corrector solve into
    b_spe (field_b_class.f03:836-879) */
 int qpad_field_solve_bt_iter(qpad_ctx *ctx);
-/* add_f1(b_spe, b_beam, b) (ufield_class.f03:817-881) */
+/* add_f1(b_spe, b_beam, b) (ufield_class.f03:842-904, called at simulation_class.f03:375,413) */
 int qpad_field_add_b(qpad_ctx *ctx);
 /* convergence_tester(b_spe, 2, op) (simulation_class.f03:522-606);
    op 0 = 'record', 1 = 'compare' (rel/abs returned for 'compare') */

@@ -14,7 +14,7 @@
 #include <stdlib.h>
 #include <string.h>
 
-#define PI_DB (4.0 * atan(1.0)) /* source/param.f03:35 */
+#define PI_DB (4.0 * atan(1.0)) /* source/param.f03:36 */
 
 static inline int kre(int m) { return m == 0 ? 0 : 2 * m - 1; }
 static inline int kim(int m) { return 2 * m; }

@@ -35,10 +35,10 @@
 extern "C" {
 #endif
 
-/* field kinds of field_solver (source/param.f03:72-80) */
+/* field kinds of field_solver (source/param.f03:75-84) */
 enum { ORA_FK_PSI = 0, ORA_FK_EZ = 1, ORA_FK_BZ = 2, ORA_FK_BT = 3,
        ORA_FK_BPLUS = 4, ORA_FK_BMINUS = 5 };
-/* boundary conditions (source/param.f03:69-71) */
+/* boundary conditions (source/param.f03:71-74) */
 enum { ORA_BND_ZERO = 2, ORA_BND_OPEN = 3 };
 
 /* field ids of the slice state (source/sim_fields_class.f03:147-160) */

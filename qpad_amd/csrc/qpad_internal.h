@@ -28,7 +28,7 @@ constexpr int kDepWaves = 4;         // default radial cells (waves) per deposit
 // cell no longer holds up a whole deposit launch
 constexpr int kDepChunk = 1024;
 
-// field kinds (source/param.f03:72-80)
+// field kinds (source/param.f03:75-84)
 enum { FK_PSI = 0, FK_EZ = 1, FK_BZ = 2, FK_BT = 3, FK_BPLUS = 4, FK_BMINUS = 5, FK_N = 6 };
 
 __host__ __device__ inline int kre(int m) { return m == 0 ? 0 : 2 * m - 1; }
