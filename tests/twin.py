@@ -32,6 +32,78 @@ def make_twin(sim):
     return o
 
 
+def _as_oracle_struct(cls, src):
+    """Copy a library config struct into the oracle's struct of the same
+    layout, field by field (the oracle's fields are a subset)."""
+    out = cls()
+    for name, typ in cls._fields_:
+        v = getattr(src, name)
+        setattr(out, name, typ(*list(v)) if hasattr(typ, "_length_") else v)
+    return out
+
+
+def oracle_from_deck(deck):
+    """The oracle set up from a qpinput deck on its own, as qpad_amd.Simulation
+    sets up the library: same grid, same species and beam seeds; the oracle
+    injects the plasma and the beam itself."""
+    from qpad_amd.input import beam_from_deck, grid_from_deck, species_from_deck
+    g, _, z0 = grid_from_deck(deck)
+    o = po.Oracle(_as_oracle_struct(po.Grid, g))
+    sim = deck["simulation"]
+    seed = int(sim.get("random_seed", 10))
+    nsp = sim.get("nspecies", len(deck.get("species", [])))
+    nbm = sim.get("nbeams", len(deck.get("beam", [])))
+    for k, sp in enumerate(deck.get("species", [])[:nsp]):
+        o.add_species(_as_oracle_struct(po.SpeciesCfg, species_from_deck(sp, seed=seed + 1000 * k)))
+    for k, bm in enumerate(deck.get("beam", [])[:nbm]):
+        o.add_beam(_as_oracle_struct(po.BeamCfg, beam_from_deck(bm, z0, seed=seed + 7 * (k + 1))))
+    return o
+
+
+# what a golden record of one 3D step holds: iteration counts per slice, the
+# histories below (every slice), the plasma particles after the last slice
+# (by id) and the beam after its push (each coordinate sorted: the library
+# keeps the beam sorted by cell)
+GOLDEN_HIST = ("psi", "e", "b", "b_spe", "cu", "q_spe", "q_beam")
+GOLDEN_SPECIES = ("x1", "x2", "p1", "p2", "p3", "gamma", "psi")
+GOLDEN_BEAM = ("x1", "x2", "x3", "p1", "p2", "p3")
+
+
+def _record_particles(out, sp, b):
+    sp = particles_by_id({k: np.array(v) for k, v in sp.items()})
+    out["sp_id"] = sp["id"].astype(np.int32)
+    for k in GOLDEN_SPECIES:
+        out["sp_" + k] = sp[k]
+    for k in GOLDEN_BEAM:
+        out["beam_" + k] = np.sort(np.array(b[k]))
+
+
+def record_oracle_step(o, nr, nxi):
+    """One 3D step of the oracle (species 0, beam 0), recorded."""
+    o.step_begin()
+    out = {"iters": np.array([o.slice(j) for j in range(1, nxi + 1)], np.int32)}
+    for n in GOLDEN_HIST:
+        out["hist_" + n] = np.array(o.hist(ORA_ID[n])[1:nxi + 1, :, :, :nr + 2])
+    sp = o.species(0)
+    sp = {k: np.array(v) for k, v in sp.items()}
+    o.step_end(1)
+    _record_particles(out, sp, o.beam(0))
+    return out
+
+
+def record_gpu_step(sim):
+    """The same record of one 3D step of a qpad_amd.Simulation."""
+    nxi = sim.grid.nxi
+    sim.step3d_begin()
+    out = {"iters": np.array([sim.slice(j) for j in range(1, nxi + 1)], np.int32)}
+    for n in GOLDEN_HIST:
+        out["hist_" + n] = np.stack([sim.fields[n].hist(j) for j in range(1, nxi + 1)])
+    sp = sim.spe[0].get()
+    sim.step3d_end()
+    _record_particles(out, sp, sim.beam[0].get())
+    return out
+
+
 def sync_slice_inputs(sim, o, j):
     """Copy what slice j depends on: particles, qn, cu, b_spe, q_beam(j)."""
     for k, sp in enumerate(sim.spe):
