@@ -1,12 +1,12 @@
-"""Write the golden record of one 3D step of a small deck, from the oracle.
+"""Write the golden records of one 3D step of two small decks, from the oracle.
 
 QPAD ships no golden vectors for this path (its test programs hold no
-assertions), so the fixture is the oracle's own output on a fixed deck,
+assertions), so each fixture is the oracle's own output on a fixed deck,
 committed after the oracle was pinned against mathematics.  It serves two
 checks: the oracle must keep reproducing it (a change to the oracle that moves
 its answer fails a CPU test), and the HIP library must match it on a GPU.
 
-    python tests/golden/make_golden.py      # rewrites step_small.npz
+    python tests/golden/make_golden.py      # rewrites step_small.npz, step_ions_m4.npz
 """
 import os
 import sys
@@ -17,32 +17,51 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(os.path.dirname(HERE))
 sys.path[:0] = [ROOT, os.path.join(ROOT, "oracle"), os.path.join(ROOT, "tests")]
 
-FIXTURE = os.path.join(HERE, "step_small.npz")
 
-
-def deck():
-    """Nr 32, 24 slices, modes 0..2, 8 particles per cell, a 1024-particle
-    beam (sigma_xi 0.1, so that it lies within the box) offset from the axis
-    so that every mode carries a response."""
+def deck_small():
+    """Nr 32, 24 slices, modes 0..2, electrons (8 particles per cell), a
+    1024-particle beam (sigma_xi 0.1, so that it lies within the box) offset
+    from the axis so that every mode carries a response."""
     from qpad_amd import configs
     return configs.small(nr=32, nxi=24, max_mode=2, ppc=(2, 4), total_num=1 << 10, sigma_xi=0.1,
                          offset_x=0.05)
 
 
-def oracle_record():
+def deck_ions_m4():
+    """Nr 24, 20 slices, modes 0..4, electrons and ions (neither neutralised
+    by a background), the same kind of beam."""
+    from qpad_amd import configs
+    return configs.small(nr=24, nxi=20, max_mode=4, ppc=(2, 4), total_num=1 << 10, sigma_xi=0.1,
+                         offset_x=0.05, ions=True)
+
+
+DECKS = {"step_small": deck_small, "step_ions_m4": deck_ions_m4}
+
+
+def fixture_path(name):
+    return os.path.join(HERE, name + ".npz")
+
+
+def oracle_record(name):
     import pyoracle
     import twin
     pyoracle.lib()
-    d = deck()
-    o = twin.oracle_from_deck(d)
+    o = twin.oracle_from_deck(DECKS[name]())
     try:
         return twin.record_oracle_step(o, o.grid.nr, o.grid.nxi)
     finally:
         o.close()
 
 
+def load(name):
+    with np.load(fixture_path(name)) as z:
+        return {k: z[k] for k in z.files}
+
+
 if __name__ == "__main__":
-    rec = oracle_record()
-    np.savez_compressed(FIXTURE, **rec)
-    print(f"wrote {FIXTURE}: iterations {rec['iters'].tolist()}, "
-          f"{len(rec['sp_id'])} plasma and {len(rec['beam_x1'])} beam particles")
+    for name in DECKS:
+        rec = oracle_record(name)
+        np.savez_compressed(fixture_path(name), **rec)
+        nsp = sum(1 for k in rec if k.endswith("_id"))
+        print(f"wrote {fixture_path(name)}: iterations {rec['iters'].tolist()}, {nsp} species, "
+              f"{len(rec['beam_x1'])} beam particles")

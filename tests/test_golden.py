@@ -1,5 +1,5 @@
-"""The oracle against the committed golden record of one 3D step
-(tests/golden/step_small.npz, written by tests/golden/make_golden.py).
+"""The oracle against the committed golden records of one 3D step
+(tests/golden/*.npz, written by tests/golden/make_golden.py).
 
 The record pins the oracle's whole-step answer on a fixed deck: plasma and
 beam injection, every slice's fields and iteration count, the plasma push and
@@ -22,15 +22,14 @@ import make_golden  # noqa: E402
 TOL = 1e-12
 
 
-@pytest.fixture(scope="module")
-def golden():
-    with np.load(make_golden.FIXTURE) as z:
-        return {k: z[k] for k in z.files}
+NAMES = sorted(make_golden.DECKS)
 
 
-def test_fixture_is_a_real_response(golden):
+@pytest.mark.parametrize("name", NAMES)
+def test_fixture_is_a_real_response(name):
     """The deck exercises the iteration (several slices need more than one
     iteration) and every mode of the fields."""
+    golden = make_golden.load(name)
     assert golden["iters"].max() >= 4 and golden["iters"].min() >= 1
     assert len(golden["beam_x1"]) == 1 << 10
     b = golden["hist_b"]
@@ -38,11 +37,15 @@ def test_fixture_is_a_real_response(golden):
         assert np.abs(b[:, :, k, :]).max() > 1e-6 * np.abs(b).max(), f"mode array {k} is empty"
 
 
-def test_oracle_reproduces_golden_step(golden):
-    rec = make_golden.oracle_record()
+@pytest.mark.parametrize("name", NAMES)
+def test_oracle_reproduces_golden_step(name):
+    golden = make_golden.load(name)
+    rec = make_golden.oracle_record(name)
     assert sorted(rec) == sorted(golden)
     assert rec["iters"].tolist() == golden["iters"].tolist()
-    assert np.array_equal(rec["sp_id"], golden["sp_id"])
-    bad = {k: rel_err(rec[k], golden[k]) for k in golden if k not in ("iters", "sp_id")}
+    ids = [k for k in golden if k.endswith("_id")]
+    for k in ids:
+        assert np.array_equal(rec[k], golden[k]), k
+    bad = {k: rel_err(rec[k], golden[k]) for k in golden if k != "iters" and k not in ids}
     bad = {k: v for k, v in bad.items() if not v <= TOL}
     assert not bad, f"beyond {TOL}: {bad}"

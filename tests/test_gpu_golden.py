@@ -1,5 +1,6 @@
-"""The HIP library against the committed golden record of one 3D step
-(tests/golden/step_small.npz: the oracle's answer on a fixed deck).
+"""The HIP library against the committed golden records of one 3D step
+(tests/golden/*.npz: the oracle's answers on fixed decks; electrons at
+modes 0..2, and electrons with ions at modes 0..4).
 
 Unlike the slice-by-slice parity tests, nothing is copied from the library
 into the oracle here: the library injects its own plasma and beam from the
@@ -22,23 +23,26 @@ import make_golden  # noqa: E402
 pytestmark = pytest.mark.gpu
 
 # a whole step without re-synchronisation, held to the slice parity bar: on
-# this deck the library's reordered sums stay within 6e-15 of the record
+# these decks the library's reordered sums stay within 1e-14 of the records
 TOL_STEP = 1e-10
 TOL_BEAM = 1e-10
 
 
-def test_library_matches_golden_step():
+@pytest.mark.parametrize("name", sorted(make_golden.DECKS))
+def test_library_matches_golden_step(name):
     from qpad_amd import Simulation
-    with np.load(make_golden.FIXTURE) as z:
-        gold = {k: z[k] for k in z.files}
-    sim = Simulation(make_golden.deck())
+    gold = make_golden.load(name)
+    sim = Simulation(make_golden.DECKS[name]())
     try:
         rec = twin.record_gpu_step(sim)
     finally:
         sim.close()
+    assert sorted(rec) == sorted(gold)
     assert rec["iters"].tolist() == gold["iters"].tolist()
-    assert np.array_equal(rec["sp_id"], gold["sp_id"])
-    errs = {k: rel_err(rec[k], gold[k]) for k in gold if k not in ("iters", "sp_id")}
+    ids = [k for k in gold if k.endswith("_id")]
+    for k in ids:
+        assert np.array_equal(rec[k], gold[k]), k
+    errs = {k: rel_err(rec[k], gold[k]) for k in gold if k != "iters" and k not in ids}
     print({k: f"{v:.1e}" for k, v in errs.items()})
     bad = {k: v for k, v in errs.items() if not v <= (TOL_BEAM if k.startswith("beam_") else TOL_STEP)}
     assert not bad, f"beyond tolerance: {bad}"

@@ -61,33 +61,33 @@ def oracle_from_deck(deck):
 
 
 # what a golden record of one 3D step holds: iteration counts per slice, the
-# histories below (every slice), the plasma particles after the last slice
-# (by id) and the beam after its push (each coordinate sorted: the library
-# keeps the beam sorted by cell)
+# histories below (every slice), every species' plasma particles after the
+# last slice (by id) and beam 0 after its push (each coordinate sorted: the
+# library keeps the beam sorted by cell)
 GOLDEN_HIST = ("psi", "e", "b", "b_spe", "cu", "q_spe", "q_beam")
 GOLDEN_SPECIES = ("x1", "x2", "p1", "p2", "p3", "gamma", "psi")
 GOLDEN_BEAM = ("x1", "x2", "x3", "p1", "p2", "p3")
 
 
-def _record_particles(out, sp, b):
-    sp = particles_by_id({k: np.array(v) for k, v in sp.items()})
-    out["sp_id"] = sp["id"].astype(np.int32)
-    for k in GOLDEN_SPECIES:
-        out["sp_" + k] = sp[k]
+def _record_particles(out, species, b):
+    for i, sp in enumerate(species):
+        sp = particles_by_id({k: np.array(v) for k, v in sp.items()})
+        out[f"sp{i}_id"] = sp["id"].astype(np.int32)
+        for k in GOLDEN_SPECIES:
+            out[f"sp{i}_{k}"] = sp[k]
     for k in GOLDEN_BEAM:
         out["beam_" + k] = np.sort(np.array(b[k]))
 
 
 def record_oracle_step(o, nr, nxi):
-    """One 3D step of the oracle (species 0, beam 0), recorded."""
+    """One 3D step of the oracle (every species, beam 0), recorded."""
     o.step_begin()
     out = {"iters": np.array([o.slice(j) for j in range(1, nxi + 1)], np.int32)}
     for n in GOLDEN_HIST:
         out["hist_" + n] = np.array(o.hist(ORA_ID[n])[1:nxi + 1, :, :, :nr + 2])
-    sp = o.species(0)
-    sp = {k: np.array(v) for k, v in sp.items()}
+    species = [{k: np.array(v) for k, v in o.species(i).items()} for i in range(o.nspecies)]
     o.step_end(1)
-    _record_particles(out, sp, o.beam(0))
+    _record_particles(out, species, o.beam(0))
     return out
 
 
@@ -98,9 +98,9 @@ def record_gpu_step(sim):
     out = {"iters": np.array([sim.slice(j) for j in range(1, nxi + 1)], np.int32)}
     for n in GOLDEN_HIST:
         out["hist_" + n] = np.stack([sim.fields[n].hist(j) for j in range(1, nxi + 1)])
-    sp = sim.spe[0].get()
+    species = [sp.get() for sp in sim.spe]
     sim.step3d_end()
-    _record_particles(out, sp, sim.beam[0].get())
+    _record_particles(out, species, sim.beam[0].get())
     return out
 
 
