@@ -42,11 +42,12 @@ def fixture_path(name):
     return os.path.join(HERE, name + ".npz")
 
 
-def oracle_record(name):
+def oracle_record(name, deck=None):
+    """The oracle's record of one 3D step of deck `name` (or of `deck`)."""
     import pyoracle
     import twin
     pyoracle.lib()
-    o = twin.oracle_from_deck(DECKS[name]())
+    o = twin.oracle_from_deck(DECKS[name]() if deck is None else deck)
     try:
         return twin.record_oracle_step(o, o.grid.nr, o.grid.nxi)
     finally:

@@ -49,3 +49,15 @@ def test_oracle_reproduces_golden_step(name):
     bad = {k: rel_err(rec[k], golden[k]) for k in golden if k != "iters" and k not in ids}
     bad = {k: v for k, v in bad.items() if not v <= TOL}
     assert not bad, f"beyond {TOL}: {bad}"
+
+
+def test_golden_check_sees_a_small_change():
+    """The 1e-12 bar is tight enough to see a small change of the answer: a
+    beam charge off by 1e-9 (relative) moves the record beyond it."""
+    name = "step_small"
+    golden = make_golden.load(name)
+    d = make_golden.DECKS[name]()
+    d["beam"][0]["total_charge"] *= 1.0 + 1e-9
+    rec = make_golden.oracle_record(name, deck=d)
+    worst = max(rel_err(rec[k], golden[k]) for k in ("hist_psi", "hist_b", "hist_q_beam"))
+    assert worst > 100 * TOL, worst
