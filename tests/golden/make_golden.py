@@ -1,4 +1,4 @@
-"""Write the golden records of one 3D step of two small decks, from the oracle.
+"""Write the golden records of small decks (one or two 3D steps), from the oracle.
 
 QPAD ships no golden vectors for this path (its test programs hold no
 assertions), so each fixture is the oracle's own output on a fixed deck,
@@ -6,7 +6,7 @@ committed after the oracle was pinned against mathematics.  It serves two
 checks: the oracle must keep reproducing it (a change to the oracle that moves
 its answer fails a CPU test), and the HIP library must match it on a GPU.
 
-    python tests/golden/make_golden.py      # rewrites step_small.npz, step_ions_m4.npz
+    python tests/golden/make_golden.py      # rewrites every record (*.npz)
 """
 import os
 import sys
@@ -35,7 +35,10 @@ def deck_ions_m4():
                          offset_x=0.05, ions=True)
 
 
-DECKS = {"step_small": deck_small, "step_ions_m4": deck_ions_m4}
+DECKS = {"step_small": deck_small, "step_ions_m4": deck_ions_m4, "two_steps_small": deck_small}
+# 3D steps per record (default 1): the second step of two_steps_small starts
+# from the beam the first one pushed and from the re-injected plasma
+STEPS = {"two_steps_small": 2}
 
 
 def fixture_path(name):
@@ -43,13 +46,13 @@ def fixture_path(name):
 
 
 def oracle_record(name, deck=None):
-    """The oracle's record of one 3D step of deck `name` (or of `deck`)."""
+    """The oracle's record of deck `name` (or of `deck`)."""
     import pyoracle
     import twin
     pyoracle.lib()
     o = twin.oracle_from_deck(DECKS[name]() if deck is None else deck)
     try:
-        return twin.record_oracle_step(o, o.grid.nr, o.grid.nxi)
+        return twin.record_oracle_step(o, o.grid.nr, o.grid.nxi, STEPS.get(name, 1))
     finally:
         o.close()
 

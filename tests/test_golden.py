@@ -1,4 +1,4 @@
-"""The oracle against the committed golden records of one 3D step
+"""The oracle against the committed golden records of one or two 3D steps
 (tests/golden/*.npz, written by tests/golden/make_golden.py).
 
 The record pins the oracle's whole-step answer on a fixed deck: plasma and

@@ -79,23 +79,34 @@ def _record_particles(out, species, b):
         out["beam_" + k] = np.sort(np.array(b[k]))
 
 
-def record_oracle_step(o, nr, nxi):
-    """One 3D step of the oracle (every species, beam 0), recorded."""
-    o.step_begin()
-    out = {"iters": np.array([o.slice(j) for j in range(1, nxi + 1)], np.int32)}
+def record_oracle_step(o, nr, nxi, nsteps=1):
+    """nsteps 3D steps of the oracle (every species, beam 0): the iteration
+    counts of every step, the rest of the record from the last one."""
+    iters = []
+    for i in range(1, nsteps + 1):
+        o.step_begin()
+        iters += [o.slice(j) for j in range(1, nxi + 1)]
+        if i < nsteps:
+            o.step_end(i)
+    out = {"iters": np.array(iters, np.int32)}
     for n in GOLDEN_HIST:
         out["hist_" + n] = np.array(o.hist(ORA_ID[n])[1:nxi + 1, :, :, :nr + 2])
     species = [{k: np.array(v) for k, v in o.species(i).items()} for i in range(o.nspecies)]
-    o.step_end(1)
+    o.step_end(nsteps)
     _record_particles(out, species, o.beam(0))
     return out
 
 
-def record_gpu_step(sim):
-    """The same record of one 3D step of a qpad_amd.Simulation."""
+def record_gpu_step(sim, nsteps=1):
+    """The same record of nsteps 3D steps of a qpad_amd.Simulation."""
     nxi = sim.grid.nxi
-    sim.step3d_begin()
-    out = {"iters": np.array([sim.slice(j) for j in range(1, nxi + 1)], np.int32)}
+    iters = []
+    for i in range(1, nsteps + 1):
+        sim.step3d_begin()
+        iters += [sim.slice(j) for j in range(1, nxi + 1)]
+        if i < nsteps:
+            sim.step3d_end()
+    out = {"iters": np.array(iters, np.int32)}
     for n in GOLDEN_HIST:
         out["hist_" + n] = np.stack([sim.fields[n].hist(j) for j in range(1, nxi + 1)])
     species = [sp.get() for sp in sim.spe]
