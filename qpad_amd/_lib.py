@@ -19,6 +19,8 @@ FIELD_IDS = {"psi": F_PSI, "e": F_E, "e_spe": F_E_SPE, "b": F_B, "b_spe": F_B_SP
              "q_spe": F_Q_SPE, "q_beam": F_Q_BEAM}
 BND_ZERO, BND_OPEN = 2, 3
 PUSH2_ROBUST = 1
+# status codes (include/qpad_hip.h)
+OK, E_INVALID, E_HIP, E_NOMEM, E_STATE = 0, -1, -2, -3, -4
 
 _dp = C.POINTER(C.c_double)
 _ip = C.POINTER(C.c_int32)

@@ -27,10 +27,10 @@ CFLAGS = ["-O3", "-fPIC", "-std=c++17", f"--offload-arch={ARCH}",
           "-I", os.path.join(HERE, "..", "include")]
 
 
-def _compile(src):
+def _compile(src, force=False):
     obj = os.path.join(BUILD, os.path.basename(src) + ".o")
     deps = [src] + glob.glob(os.path.join(CSRC, "*.h")) + [os.path.join(HERE, "..", "include", "qpad_hip.h")]
-    if os.path.exists(obj) and os.path.getmtime(obj) >= max(os.path.getmtime(d) for d in deps):
+    if not force and os.path.exists(obj) and os.path.getmtime(obj) >= max(os.path.getmtime(d) for d in deps):
         return obj, ""
     cmd = [HIPCC] + CFLAGS + ["-c", src, "-o", obj]
     r = subprocess.run(cmd, capture_output=True, text=True)
@@ -39,11 +39,12 @@ def _compile(src):
     return obj, r.stderr
 
 
-def build(jobs=4, verbose=False):
+def build(jobs=4, verbose=False, force=False):
+    """Compile (every source when force, else the out-of-date ones) and link."""
     os.makedirs(BUILD, exist_ok=True)
     srcs = sorted(glob.glob(os.path.join(CSRC, "*.hip")))
     with cf.ThreadPoolExecutor(max_workers=jobs) as ex:
-        results = list(ex.map(_compile, srcs))
+        results = list(ex.map(lambda s_: _compile(s_, force), srcs))
     objs = [o for o, _ in results]
     if verbose:
         for o, w in results:
@@ -62,5 +63,6 @@ if __name__ == "__main__":
     ap = argparse.ArgumentParser()
     ap.add_argument("--jobs", type=int, default=4)
     ap.add_argument("-v", "--verbose", action="store_true")
+    ap.add_argument("--force", action="store_true", help="recompile every source")
     args = ap.parse_args()
-    print(build(args.jobs, args.verbose))
+    print(build(args.jobs, args.verbose, args.force))

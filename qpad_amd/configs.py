@@ -99,11 +99,14 @@ def c4(n=16, seed=2024, total_num=1 << 18, nxi=512):
 
 
 def small(nr=64, nxi=48, max_mode=1, ppc=(2, 4), total_num=1 << 14, rmax=4.0, dxi=0.05,
-          n_peak=4.0, sigma_r=0.3, sigma_xi=0.4, offset_x=0.0, ions=False):
-    """A reduced deck with the same physics for fast parity checks."""
+          n_peak=4.0, sigma_r=0.3, sigma_xi=0.4, offset_x=0.0, ions=False, ion_ppc=None):
+    """A reduced deck with the same physics for fast parity checks; with
+    ions, ion_ppc (default: the electrons' ppc) sets the ions' (r, phi)
+    particles per cell, so that the two species can differ in count."""
     species = [_electrons(ppc[0], ppc[1], neutralized=not ions)]
     if ions:
-        ion = _electrons(ppc[0], ppc[1], neutralized=False)
+        ip = ppc if ion_ppc is None else ion_ppc
+        ion = _electrons(ip[0], ip[1], neutralized=False)
         ion.update({"q": 1.0, "m": 1836.0})
         species.append(ion)
     return _deck(nr, rmax, max_mode, nxi, dxi, species,

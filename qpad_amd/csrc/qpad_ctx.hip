@@ -225,6 +225,28 @@ static int species_charge_setup(qpad_ctx *c, int s)
     return 0;
 }
 
+static void species_free(Species &sp)
+{
+    dfree(sp.x1); dfree(sp.x2); dfree(sp.p1); dfree(sp.p2); dfree(sp.p3); dfree(sp.q);
+    dfree(sp.gamma); dfree(sp.psi); dfree(sp.id);
+    dfree(sp.a_x1); dfree(sp.a_x2); dfree(sp.a_p1); dfree(sp.a_p2); dfree(sp.a_p3); dfree(sp.a_q);
+    dfree(sp.a_gamma); dfree(sp.a_psi); dfree(sp.a_id);
+    dfree(sp.key); dfree(sp.key_alt);
+    dfree(sp.cell_start); dfree(sp.qn); dfree(sp.P0); dfree(sp.P1);
+    dfree(sp.ov_start); dfree(sp.ov_desc); dfree(sp.O0); dfree(sp.O1);
+    dfree(sp.hist); dfree(sp.tot);
+}
+
+static void beam_free(Beam &bm)
+{
+    dfree(bm.x1); dfree(bm.x2); dfree(bm.x3); dfree(bm.p1); dfree(bm.p2); dfree(bm.p3); dfree(bm.q);
+    dfree(bm.a_x1); dfree(bm.a_x2); dfree(bm.a_x3); dfree(bm.a_p1); dfree(bm.a_p2); dfree(bm.a_p3);
+    dfree(bm.a_q);
+    dfree(bm.key); dfree(bm.key_alt); dfree(bm.idx); dfree(bm.idx_alt); dfree(bm.cell_start);
+    if (bm.sort_tmp) (void)hipFree(bm.sort_tmp);
+    bm.sort_tmp = nullptr;
+}
+
 }  // namespace qpad
 
 using namespace qpad;
@@ -384,25 +406,8 @@ int qpad_destroy(qpad_ctx *c)
     if (c->it_host) (void)hipHostFree(c->it_host);
     if (c->it_report) (void)hipHostFree(c->it_report);
     if (c->dec_ev) (void)hipEventDestroy(c->dec_ev);
-    for (int s = 0; s < c->nspecies; s++) {
-        Species &sp = c->sp[s];
-        dfree(sp.x1); dfree(sp.x2); dfree(sp.p1); dfree(sp.p2); dfree(sp.p3); dfree(sp.q);
-        dfree(sp.gamma); dfree(sp.psi); dfree(sp.id);
-        dfree(sp.a_x1); dfree(sp.a_x2); dfree(sp.a_p1); dfree(sp.a_p2); dfree(sp.a_p3); dfree(sp.a_q);
-        dfree(sp.a_gamma); dfree(sp.a_psi); dfree(sp.a_id);
-        dfree(sp.key); dfree(sp.key_alt);
-        dfree(sp.cell_start); dfree(sp.qn); dfree(sp.P0); dfree(sp.P1);
-        dfree(sp.ov_start); dfree(sp.ov_desc); dfree(sp.O0); dfree(sp.O1);
-        dfree(sp.hist); dfree(sp.tot);
-    }
-    for (int b = 0; b < c->nbeams; b++) {
-        Beam &bm = c->bm[b];
-        dfree(bm.x1); dfree(bm.x2); dfree(bm.x3); dfree(bm.p1); dfree(bm.p2); dfree(bm.p3); dfree(bm.q);
-        dfree(bm.a_x1); dfree(bm.a_x2); dfree(bm.a_x3); dfree(bm.a_p1); dfree(bm.a_p2); dfree(bm.a_p3);
-        dfree(bm.a_q);
-        dfree(bm.key); dfree(bm.key_alt); dfree(bm.idx); dfree(bm.idx_alt); dfree(bm.cell_start);
-        if (bm.sort_tmp) (void)hipFree(bm.sort_tmp);
-    }
+    for (int s = 0; s < c->nspecies; s++) species_free(c->sp[s]);
+    for (int b = 0; b < c->nbeams; b++) beam_free(c->bm[b]);
     for (auto &p : c->pools)
         for (auto e : p.ev) (void)hipEventDestroy(e);
     if (c->stream) (void)hipStreamDestroy(c->stream);
@@ -411,6 +416,10 @@ int qpad_destroy(qpad_ctx *c)
 }
 
 
+// Every check runs before the context is touched; the species becomes
+// visible (nspecies grows) only once all of its buffers exist and its
+// plasma is injected.  On any failure its buffers are freed and the context
+// is left exactly as it was.
 int qpad_species_new(qpad_ctx *c, const qpad_species_cfg *cfg, int32_t *sp_id)
 {
     if (!c || !cfg) return fail(QPAD_E_INVALID, "null argument");
@@ -420,17 +429,18 @@ int qpad_species_new(qpad_ctx *c, const qpad_species_cfg *cfg, int32_t *sp_id)
     if (cfg->push_type != QPAD_PUSH2_ROBUST)
         return fail(QPAD_E_INVALID, "only push_type 'robust' is implemented on the GPU path");
     if (cfg->mass == 0.0 || cfg->qm == 0.0) return fail(QPAD_E_INVALID, "q and m must be non-zero");
+    // npmax: fdist2d_class.f03:258-262 (xtra = 2)
+    const int64_t nlat = (int64_t)c->g.nr * cfg->ppc1 * cfg->ppc2 * cfg->num_theta;
+    if (nlat > 0x7fffffff / 2) return fail(QPAD_E_INVALID, "too many particles per species");
+    size_t hist_ints = 0;
+    TRY(sort_workspace(2 * nlat, c->g.nr, &hist_ints));
     const int s = c->nspecies;
     Species &sp = c->sp[s];
     memset((void *)&sp, 0, sizeof(Species));
     sp.cfg = *cfg;
     sp.qbm = cfg->qm / cfg->mass;
-    // npmax: fdist2d_class.f03:258-262 (xtra = 2)
-    const int64_t nlat = (int64_t)c->g.nr * cfg->ppc1 * cfg->ppc2 * cfg->num_theta;
-    if (nlat > 0x7fffffff / 2) return fail(QPAD_E_INVALID, "too many particles per species");
     sp.npmax = 2 * nlat;
     const size_t n = (size_t)sp.npmax;
-    c->nspecies++;
     int rc = 0;
     double **dp[] = {&sp.x1, &sp.x2, &sp.p1, &sp.p2, &sp.p3, &sp.q, &sp.gamma, &sp.psi,
                      &sp.a_x1, &sp.a_x2, &sp.a_p1, &sp.a_p2, &sp.a_p3, &sp.a_q, &sp.a_gamma, &sp.a_psi};
@@ -452,15 +462,23 @@ int qpad_species_new(qpad_ctx *c, const qpad_species_cfg *cfg, int32_t *sp_id)
     if (!rc && (hipMemsetAsync(sp.ov_start, 0, sizeof(int32_t) * ((size_t)c->g.nr + 1), c->stream) != hipSuccess ||
                 hipMemsetAsync(sp.ov_desc, 0, sizeof(int4) * (size_t)sp.n_ov_max, c->stream) != hipSuccess))
         rc = fail(QPAD_E_HIP, "hipMemsetAsync of the overflow rows failed");
-    size_t hist_ints = 0;
-    if (!rc) rc = sort_workspace(sp.npmax, c->g.nr, &hist_ints);
     if (!rc) rc = dalloc(&sp.hist, hist_ints);
     if (!rc) rc = dalloc(&sp.tot, (size_t)c->g.nr + 1);
     if (!rc) rc = zero_dev(c, sp.qn, f1_size(c, 1));
     if (!rc) rc = launch_inject(c, s);
     if (!rc) rc = species_charge_setup(c, s);
     if (!rc) rc = sync(c);
-    if (rc) return rc;
+    if (rc) {
+        // keep the message of the first failure; wait for queued work on the
+        // buffers before they are freed
+        const std::string msg = g_err;
+        (void)hipStreamSynchronize(c->stream);
+        species_free(sp);
+        memset((void *)&sp, 0, sizeof(Species));
+        g_err = msg;
+        return rc;
+    }
+    c->nspecies++;
     if (sp_id) *sp_id = s;
     return 0;
 }
@@ -489,7 +507,6 @@ int qpad_beam_new(qpad_ctx *c, const qpad_beam_cfg *cfg, int32_t *beam_id)
     bm.cfg = *cfg;
     bm.qbm = cfg->qm / cfg->mass;
     bm.npmax = std::max<int64_t>(1, cfg->total_num * (cfg->quiet ? 2 : 1));
-    c->nbeams++;
     const size_t n = (size_t)bm.npmax;
     int rc = 0;
     double **dp[] = {&bm.x1, &bm.x2, &bm.x3, &bm.p1, &bm.p2, &bm.p3, &bm.q,
@@ -513,7 +530,15 @@ int qpad_beam_new(qpad_ctx *c, const qpad_beam_cfg *cfg, int32_t *beam_id)
     if (!rc) rc = launch_beam_inject(c, b);
     if (!rc) rc = launch_beam_sort(c, b);
     if (!rc) rc = refresh_beam_npp(c, b);
-    if (rc) return rc;
+    if (rc) {
+        const std::string msg = g_err;
+        (void)hipStreamSynchronize(c->stream);
+        beam_free(bm);
+        memset((void *)&bm, 0, sizeof(Beam));
+        g_err = msg;
+        return rc;
+    }
+    c->nbeams++;
     if (beam_id) *beam_id = b;
     return 0;
 }
@@ -744,9 +769,10 @@ int qpad_set_launch_shape(qpad_ctx *c, const qpad_launch_shape *shape)
     return 0;
 }
 
-// Every candidate shape runs the species' current-deposit (first-iteration
-// instance, no gamma/psi stores) and charge-deposit kernels on the current
-// particles and fields, `reps` timed launches each after one untimed.  Both
+// Every candidate shape runs the current deposit of all species (first-
+// iteration instance, no gamma/psi stores) and their charge deposit, the
+// same launches the slice queues, on the current particles and fields,
+// `reps` timed launches each after one untimed.  Both
 // kernels only write the per-cell partials, which every slice recomputes
 // before use, so tuning changes no result.
 int qpad_tune_launch(qpad_ctx *c, int32_t reps, double *ms_out)
@@ -767,10 +793,9 @@ int qpad_tune_launch(qpad_ctx *c, int32_t reps, double *ms_out)
         c->dep_cells = cand[k];
         for (int r = 0; r <= reps && !rc; r++) {
             if (r == 1 && hipEventRecord(e0, c->stream) != hipSuccess) rc = fail(QPAD_E_HIP, "hipEventRecord");
-            for (int s = 0; s < c->nspecies && !rc; s++) {
-                rc = launch_amjdeposit(c, s, 1, false, false);
-                if (!rc) rc = launch_qdeposit(c, s);
-            }
+            // the all-species launches slice_advance and slice_start queue
+            rc = launch_amjdeposit_all(c, 1, false, false);
+            if (!rc) rc = launch_qdeposit_all(c);
         }
         float ms = 0.0f;
         if (!rc && (hipEventRecord(e1, c->stream) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||

@@ -35,7 +35,17 @@ def deck_ions_m4():
                          offset_x=0.05, ions=True)
 
 
-DECKS = {"step_small": deck_small, "step_ions_m4": deck_ions_m4, "two_steps_small": deck_small}
+def deck_unequal_species():
+    """Nr 24, 20 slices, modes 0..2, electrons at 3 x 5 and ions at 1 x 2
+    particles per cell (neither neutralised): the species differ in count,
+    so the all-species launches run rows past the smaller species' end."""
+    from qpad_amd import configs
+    return configs.small(nr=24, nxi=20, max_mode=2, ppc=(3, 5), total_num=1 << 10, sigma_xi=0.1,
+                         offset_x=0.05, ions=True, ion_ppc=(1, 2))
+
+
+DECKS = {"step_small": deck_small, "step_ions_m4": deck_ions_m4, "two_steps_small": deck_small,
+         "step_unequal_species": deck_unequal_species}
 # 3D steps per record (default 1): the second step of two_steps_small starts
 # from the beam the first one pushed and from the re-injected plasma
 STEPS = {"two_steps_small": 2}
@@ -63,7 +73,7 @@ def load(name):
 
 
 if __name__ == "__main__":
-    for name in DECKS:
+    for name in (sys.argv[1:] or DECKS):
         rec = oracle_record(name)
         np.savez_compressed(fixture_path(name), **rec)
         nsp = sum(1 for k in rec if k.endswith("_id"))

@@ -1,0 +1,112 @@
+"""The benchmark workloads themselves on the GPU, at their full sizes.
+
+C1 (Nr 256, Nxi 256, modes 0..1, 4,096 plasma particles per slice, 2^18 beam
+particles): a whole 3D step compared with the oracle slice by slice on
+identical inputs, then the beam push.
+
+C4 (16 independent instances, Nr in {512 .. 4096}, modes 0..1 to 0..4, Nxi
+512): one 3D step of the whole scan as an eager batch, every instance bit
+for bit equal to the same deck stepped alone; and one slice inside the wake
+of its Nr 4096, modes 0..4 instance compared with the oracle.
+
+QPAD's loop: simulation_class.f03:294-512 (3D step), :342-469 (slice)."""
+import numpy as np
+import pytest
+
+import twin
+from conftest import rel_err
+from qpad_amd import Batch, Simulation, configs
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-10
+
+
+def test_c1_step_matches_oracle_slice_by_slice():
+    sim = Simulation(configs.c1())
+    g = sim.grid
+    assert (g.nr, g.nxi, g.max_mode) == (256, 256, 1)
+    assert sim.spe[0].count() == 256 * 2 * 8 and sim.beam[0].count() == 1 << 18
+    o = twin.make_twin(sim)
+    sim.step3d_begin()
+    o.step_begin()
+    qb = max(rel_err(sim.fields["q_beam"].hist(s), o.hist(twin.po.Q_BEAM)[s]) for s in range(1, g.nxi + 1))
+    assert qb <= TOL, f"q_beam history {qb}"
+    norm = twin.ErrorNorm()
+    iters = []
+    for j in range(1, g.nxi + 1):
+        twin.sync_slice_inputs(sim, o, j)
+        it_g, it_o = sim.slice(j), o.slice(j)
+        iters.append((it_g, it_o))
+        twin.compare_fields(sim, o, norm, j)
+        if j % 16 == 0 or j == g.nxi:
+            twin.compare_species(sim, o, norm, j)
+    print({n: f"{e:.1e}" for n, e in sorted(norm.errors().items())})
+    print("iterations per slice:", np.mean([a for a, _ in iters]))
+    assert all(a == b for a, b in iters), [(j + 1, a, b) for j, (a, b) in enumerate(iters) if a != b]
+    bad = norm.bad(TOL)
+    assert not bad, f"beyond {TOL}: {bad}"
+    # the beam push on each side's own field histories
+    sim.step3d_end()
+    o.step_end(1)
+    gb, ob = sim.beam[0].get(), o.beam(0)
+    assert len(gb["x1"]) == len(ob["x1"])
+    for k in ("x1", "x2", "x3", "p1", "p2", "p3"):
+        e = rel_err(np.sort(gb[k]), np.sort(ob[k]))
+        assert e <= 1e-9, (k, e)
+    sim.close()
+
+
+def _state(sim, every=8):
+    nxi = sim.grid.nxi
+    out = {n: np.stack([sim.fields[n].hist(j) for j in range(1, nxi + 1, every)])
+           for n in ("psi", "b_spe", "e", "cu")}
+    for k, sp in enumerate(sim.spe):
+        out.update({f"sp{k}.{n}": v for n, v in sp.get().items()})
+    out.update({"beam." + n: v for n, v in sim.beam[0].get().items()})
+    return out
+
+
+def test_c4_scan_batch_matches_instances_alone():
+    decks, params = configs.c4()
+    assert len(decks) == 16
+    sims = [Simulation(d) for d in decks]
+    batch = Batch(sims, graph="eager")
+    its = batch.step3d()
+    batch.close()
+    got = []
+    for s_ in sims:
+        got.append(_state(s_))
+        s_.close()
+    for i, d in enumerate(decks):
+        alone = Simulation(d)
+        it = alone.step3d()
+        ref = _state(alone)
+        alone.close()
+        assert its[i] == it, f"instance {i} ({params[i]}): {its[i]} vs {it} iterations"
+        for k in ref:
+            assert np.array_equal(got[i][k], ref[k]), f"instance {i} ({params[i]}): {k} differs"
+    print("iterations per slice:", [round(int(t) / 512, 2) for t in its])
+
+
+def test_c4_largest_instance_slice_matches_oracle():
+    decks, params = configs.c4()
+    i = next(k for k, p in enumerate(params) if p["nr"] == 4096 and p["max_mode"] == 4)
+    sim = Simulation(decks[i])
+    o = twin.make_twin(sim)
+    sim.step3d_begin()
+    o.step_begin()
+    j0 = 150   # behind the beam centre (slice 125): plasma response of every mode
+    for j in range(1, j0):
+        sim.slice(j)
+    norm = twin.ErrorNorm()
+    for j in (j0, j0 + 1):
+        twin.sync_slice_inputs(sim, o, j)
+        it_g, it_o = sim.slice(j), o.slice(j)
+        assert it_g == it_o, (j, it_g, it_o)
+        twin.compare_fields(sim, o, norm, j)
+        twin.compare_species(sim, o, norm, j)
+    sim.close()
+    print(params[i], {n: f"{e:.1e}" for n, e in sorted(norm.errors().items())})
+    bad = norm.bad(TOL)
+    assert not bad, f"beyond {TOL}: {bad}"
