@@ -123,6 +123,15 @@ int qpad_species_renew(qpad_ctx *ctx, int32_t sp);
    background) into q_spe (species2d_class.f03:186-208,
    part2d_class.f03:231-359).  The caller zeroes q_spe first, as QPAD does. */
 int qpad_species_qdp(qpad_ctx *ctx, int32_t sp);
+/* qpad_species_qdp with no device allocation: the deposit's per-cell partial
+   rows live in a 2 MiB static device array instead of the species' own
+   buffers; result bit for bit that of qpad_species_qdp.  No counterpart in
+   QPAD: it is for callers that cannot allocate (code under stream capture,
+   serving loops, harnesses that forbid hipMalloc).  Every species the
+   library accepts fits (16 * nm * (nr + npmax / 1024 + 1) bytes); a larger
+   one is refused with QPAD_E_INVALID.  The array is shared by every call on
+   a GPU: calls that overlap on one GPU race. */
+int qpad_species_qdp_noalloc(qpad_ctx *ctx, int32_t sp);
 /* species2d%amjdp: deposit cu, acu (dJ/dxi part) and amu of species sp into
    the totals using fields e and b (species2d_class.f03:233-280,
    part2d_class.f03:746-1010, robust pusher) */

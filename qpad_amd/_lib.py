@@ -79,6 +79,7 @@ SIGNATURES = {
     "qpad_species_new": (C.c_int, [_CTX, C.POINTER(SpeciesCfg), C.POINTER(C.c_int32)]),
     "qpad_species_renew": (C.c_int, [_CTX, C.c_int32]),
     "qpad_species_qdp": (C.c_int, [_CTX, C.c_int32]),
+    "qpad_species_qdp_noalloc": (C.c_int, [_CTX, C.c_int32]),
     "qpad_species_amjdp": (C.c_int, [_CTX, C.c_int32, C.c_double]),
     "qpad_species_push": (C.c_int, [_CTX, C.c_int32, C.c_double]),
     "qpad_field_solve_psi": (C.c_int, [_CTX]),

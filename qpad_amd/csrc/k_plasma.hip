@@ -320,18 +320,11 @@ int launch_qdeposit(qpad_ctx *c, int s) { return launch_qdeposit_range(c, s, s +
 
 int launch_qdeposit_all(qpad_ctx *c) { return c->nspecies ? launch_qdeposit_range(c, 0, c->nspecies) : 0; }
 
-// species s0 .. s1-1 in one launch
-int launch_qdeposit_range(qpad_ctx *c, int s0, int s1)
+// the charge deposit of the species in m.s[0 .. ns-1], gx workgroups per species
+static int qdeposit_launch(qpad_ctx *c, const QDepMulti &m, int ns, int gx)
 {
-    QDepMulti m;
-    memset(&m, 0, sizeof(m));
     const int cpw = c->dep_cells;
-    int gx = 1;
-    for (int s = s0; s < s1; s++) {
-        m.s[s - s0] = qdep_args(c, s);
-        gx = std::max(gx, dep_grid(c->g.nr, c->sp[s].npp, c->sp[s].n_ov_max, cpw));
-    }
-    dim3 grid((unsigned)gx, (unsigned)(s1 - s0)), blk(64 * cpw);
+    dim3 grid((unsigned)gx, (unsigned)ns), blk(64 * cpw);
 #define QPAD_QDEP(MM, CC) hipLaunchKernelGGL((k_qdeposit<MM, CC>), grid, blk, 0, c->stream, m)
 #define QPAD_QDEP_M(MM)                              \
     do {                                             \
@@ -352,6 +345,36 @@ int launch_qdeposit_range(qpad_ctx *c, int s0, int s1)
     return 0;
 }
 
+// species s0 .. s1-1 in one launch
+int launch_qdeposit_range(qpad_ctx *c, int s0, int s1)
+{
+    QDepMulti m;
+    memset(&m, 0, sizeof(m));
+    int gx = 1;
+    for (int s = s0; s < s1; s++) {
+        m.s[s - s0] = qdep_args(c, s);
+        gx = std::max(gx, dep_grid(c->g.nr, c->sp[s].npp, c->sp[s].n_ov_max, c->dep_cells));
+    }
+    return qdeposit_launch(c, m, s1 - s0, gx);
+}
+
+// species s with its partial rows in caller-given memory instead of the
+// species' own: rows.P0/P1 hold nr rows and rows.O0/O1 n_ov_max rows of nm
+// values each.  Every row a consumer reads is written (empty cells write
+// zeros), so the rows need no clearing.
+int launch_qdeposit_rows(qpad_ctx *c, int s, const DepRows &rows)
+{
+    QDepMulti m;
+    memset(&m, 0, sizeof(m));
+    m.s[0] = qdep_args(c, s);
+    m.s[0].P0 = rows.P0;
+    m.s[0].P1 = rows.P1;
+    m.s[0].O0 = rows.O0;
+    m.s[0].O1 = rows.O1;
+    const int gx = std::max(1, dep_grid(c->g.nr, c->sp[s].npp, c->sp[s].n_ov_max, c->dep_cells));
+    return qdeposit_launch(c, m, 1, gx);
+}
+
 // one species' q (axis rules, 1/r) times `scale` into out (1 x nm x ld)
 __global__ void k_combine_q(const double *P0, const double *P1, const double *O0, const double *O1,
                             const int32_t *os, double *out, int nr, int nm, int ld, double scale, int accumulate)
@@ -369,14 +392,20 @@ __global__ void k_combine_q(const double *P0, const double *P1, const double *O0
     *o = accumulate ? *o + scale * v : scale * v;
 }
 
-int launch_combine_q(qpad_ctx *c, int s, double *out, double scale, bool accumulate)
+int launch_combine_q_rows(qpad_ctx *c, int s, const DepRows &rows, double *out, double scale, bool accumulate)
 {
     int64_t n = (int64_t)c->g.nm * (c->g.nr + 2);
-    const Species &sp = c->sp[s];
-    hipLaunchKernelGGL(k_combine_q, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, c->stream, sp.P0, sp.P1,
-                       sp.O0, sp.O1, sp.ov_start, out, c->g.nr, c->g.nm, c->g.ld, scale, accumulate ? 1 : 0);
+    hipLaunchKernelGGL(k_combine_q, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, c->stream, rows.P0, rows.P1,
+                       rows.O0, rows.O1, c->sp[s].ov_start, out, c->g.nr, c->g.nm, c->g.ld, scale,
+                       accumulate ? 1 : 0);
     QPAD_HIP_CHECK(hipGetLastError());
     return 0;
+}
+
+int launch_combine_q(qpad_ctx *c, int s, double *out, double scale, bool accumulate)
+{
+    const Species &sp = c->sp[s];
+    return launch_combine_q_rows(c, s, DepRows{sp.P0, sp.P1, sp.O0, sp.O1}, out, scale, accumulate);
 }
 
 // ---------------------------------------------------------------------------

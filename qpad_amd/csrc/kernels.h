@@ -82,6 +82,12 @@ int launch_qdeposit(qpad_ctx *c, int sp);        // per-cell partials of rho - J
 int launch_qdeposit_range(qpad_ctx *c, int s0, int s1);   // species s0 .. s1-1 in one launch
 int launch_qdeposit_all(qpad_ctx *c);            // every species in one launch
 int launch_combine_q(qpad_ctx *c, int sp, double *out, double scale, bool accumulate);
+// deposit partial rows of one species: P0/P1 nr rows, O0/O1 n_ov_max rows
+struct DepRows {
+    double *P0, *P1, *O0, *O1;
+};
+int launch_qdeposit_rows(qpad_ctx *c, int sp, const DepRows &rows);
+int launch_combine_q_rows(qpad_ctx *c, int sp, const DepRows &rows, double *out, double scale, bool accumulate);
 // l: predictor-corrector iteration of the slice (1-based; l >= 2 is a
 // speculative launch that first evaluates the convergence of iteration l-1
 // and returns at once when it has converged; l = 0 for a plain deposit);
@@ -104,6 +110,9 @@ int launch_beam_qdeposit(qpad_ctx *c);           // all beams into h_q_beam
 int launch_beam_push(qpad_ctx *c, int ib);
 
 // ---- generic ----
+// device address of the static scratch of qpad_species_qdp_noalloc on the
+// current device dev (noalloc.hip; once per device, from qpad_create)
+int noalloc_init_device(int dev);
 int field_kernels_init(size_t dyn_lds);
 int sort_scratch(size_t n, int bits, size_t *bytes);
 // histogram entries of the plasma counting sort for up to npmax particles

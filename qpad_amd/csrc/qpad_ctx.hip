@@ -282,6 +282,7 @@ int qpad_create(const qpad_grid *gr, qpad_ctx **out)
 
     if (field_kernels_init(lds_need) != 0 || slice_kernels_init() != 0 || sort_kernels_init() != 0)
         return QPAD_E_HIP;
+    TRY(noalloc_init_device(dev));
     qpad_ctx *c = new qpad_ctx{};
     c->device = dev;
     Grid &g = c->g;

@@ -124,6 +124,15 @@ module qpad_hip_mod
       integer(c_int) :: rc
     end function
 
+    ! the same with the deposit's partial rows in a static device array (no
+    ! device allocation; calls that overlap on one GPU race)
+    function qpad_species_qdp_noalloc(ctx, sp) bind(C, name="qpad_species_qdp_noalloc") result(rc)
+      import :: c_int, c_int32_t, c_ptr
+      type(c_ptr), value :: ctx
+      integer(c_int32_t), value :: sp
+      integer(c_int) :: rc
+    end function
+
     function qpad_species_amjdp(ctx, sp, dxi) bind(C, name="qpad_species_amjdp") result(rc)
       import :: c_int, c_int32_t, c_ptr, c_double
       type(c_ptr), value :: ctx
@@ -507,7 +516,8 @@ module qpad_hip_mod
   end interface
 
   public :: qpad_last_error, qpad_abi_version, qpad_create, qpad_destroy, &
-    qpad_species_new, qpad_species_renew, qpad_species_qdp, qpad_species_amjdp, &
+    qpad_species_new, qpad_species_renew, qpad_species_qdp, qpad_species_qdp_noalloc, &
+    qpad_species_amjdp, &
     qpad_species_push, qpad_field_solve_psi, qpad_field_solve_ez, &
     qpad_field_solve_et, qpad_field_solve_bz, qpad_field_solve_bt_beam, &
     qpad_field_solve_djdxi, qpad_field_solve_bt_iter, qpad_field_add_b, &
