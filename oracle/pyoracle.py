@@ -81,52 +81,58 @@ def lib():
     if _lib is None:
         if not os.path.exists(LIB_PATH):
             build()
-        L = C.CDLL(LIB_PATH)
-        L.ora_create.restype = C.c_void_p
-        L.ora_create.argtypes = [C.POINTER(Grid)]
-        L.ora_destroy.argtypes = [C.c_void_p]
-        L.ora_add_species.argtypes = [C.c_void_p, C.POINTER(SpeciesCfg)]
-        L.ora_add_beam.argtypes = [C.c_void_p, C.POINTER(BeamCfg), C.c_int64]
-        L.ora_beam_set.argtypes = [C.c_void_p, C.c_int, C.c_int64] + [_dp] * 7
-        L.ora_step3d_begin.argtypes = [C.c_void_p]
-        L.ora_slice.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_int)]
-        L.ora_step3d_end.argtypes = [C.c_void_p, C.c_int]
-        L.ora_run.argtypes = [C.c_void_p, C.c_int, C.c_int]
-        L.ora_field.restype = _dp
-        L.ora_field.argtypes = [C.c_void_p, C.c_int]
-        L.ora_hist.restype = _dp
-        L.ora_hist.argtypes = [C.c_void_p, C.c_int]
-        L.ora_get_species.restype = C.POINTER(_Species)
-        L.ora_get_species.argtypes = [C.c_void_p, C.c_int]
-        L.ora_get_beam.restype = C.POINTER(_Beam)
-        L.ora_get_beam.argtypes = [C.c_void_p, C.c_int]
-        L.ora_matrix.argtypes = [C.c_int, C.c_int, C.c_int, C.c_double, C.c_double, C.c_int,
-                                 _dp, _dp, _dp]
-        L.ora_thomas_factor.argtypes = [C.c_int, _dp, _dp, _dp, _dp, _dp, _dp]
-        L.ora_thomas_solve.argtypes = [C.c_int, _dp, _dp, _dp, _dp]
-        for name in ("ora_solve_psi", "ora_solve_ez", "ora_solve_bz", "ora_solve_bt_beam"):
-            getattr(L, name).argtypes = [C.c_void_p, _dp, _dp]
-        for name in ("ora_solve_et", "ora_solve_bt_iter", "ora_solve_djdxi"):
-            getattr(L, name).argtypes = [C.c_void_p, _dp, _dp, _dp]
-        L.ora_conv_record.argtypes = [C.c_void_p, _dp]
-        L.ora_conv_compare.argtypes = [C.c_void_p, _dp, _dp, _dp]
-        L.ora_qdeposit.argtypes = [C.c_void_p, C.POINTER(_Species), _dp]
-        L.ora_amjdeposit_robust.argtypes = [C.c_void_p, C.POINTER(_Species), _dp, _dp, _dp, _dp,
-                                            _dp, C.c_double]
-        L.ora_push_u_robust.argtypes = [C.c_void_p, C.POINTER(_Species), _dp, _dp, C.c_double]
-        L.ora_push_x.argtypes = [C.POINTER(_Species), C.c_double]
-        L.ora_update_bound.argtypes = [C.c_void_p, C.POINTER(_Species)]
-        L.ora_interp_point.argtypes = [C.c_void_p, _dp, C.c_int, C.c_double, C.c_double, _dp]
-        L.ora_beam_qdeposit.argtypes = [C.c_void_p, C.c_int, _dp]
-        L.ora_beam_push.argtypes = [C.c_void_p, C.c_int]
-        L.ora_species_inject.argtypes = [C.c_void_p, C.c_int]
-        L.ora_species_init_q.argtypes = [C.c_void_p, C.c_int]
-        L.ora_last_iters.argtypes = [C.c_void_p]
-        L.ora_philox4x32.argtypes = [C.POINTER(C.c_uint32), C.POINTER(C.c_uint32),
-                                     C.POINTER(C.c_uint32)]
-        L.ora_normal4.argtypes = [C.c_uint64, C.c_uint64, C.c_uint32, _dp]
-        _lib = L
+        _lib = load_library(LIB_PATH)
     return _lib
+
+
+def load_library(path):
+    """A ctypes handle on an oracle library at `path` (the built oracle, or
+    a copy compiled from modified sources by the pin tests)."""
+    L = C.CDLL(path)
+    L.ora_create.restype = C.c_void_p
+    L.ora_create.argtypes = [C.POINTER(Grid)]
+    L.ora_destroy.argtypes = [C.c_void_p]
+    L.ora_add_species.argtypes = [C.c_void_p, C.POINTER(SpeciesCfg)]
+    L.ora_add_beam.argtypes = [C.c_void_p, C.POINTER(BeamCfg), C.c_int64]
+    L.ora_beam_set.argtypes = [C.c_void_p, C.c_int, C.c_int64] + [_dp] * 7
+    L.ora_step3d_begin.argtypes = [C.c_void_p]
+    L.ora_slice.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_int)]
+    L.ora_step3d_end.argtypes = [C.c_void_p, C.c_int]
+    L.ora_run.argtypes = [C.c_void_p, C.c_int, C.c_int]
+    L.ora_field.restype = _dp
+    L.ora_field.argtypes = [C.c_void_p, C.c_int]
+    L.ora_hist.restype = _dp
+    L.ora_hist.argtypes = [C.c_void_p, C.c_int]
+    L.ora_get_species.restype = C.POINTER(_Species)
+    L.ora_get_species.argtypes = [C.c_void_p, C.c_int]
+    L.ora_get_beam.restype = C.POINTER(_Beam)
+    L.ora_get_beam.argtypes = [C.c_void_p, C.c_int]
+    L.ora_matrix.argtypes = [C.c_int, C.c_int, C.c_int, C.c_double, C.c_double, C.c_int,
+                             _dp, _dp, _dp]
+    L.ora_thomas_factor.argtypes = [C.c_int, _dp, _dp, _dp, _dp, _dp, _dp]
+    L.ora_thomas_solve.argtypes = [C.c_int, _dp, _dp, _dp, _dp]
+    for name in ("ora_solve_psi", "ora_solve_ez", "ora_solve_bz", "ora_solve_bt_beam"):
+        getattr(L, name).argtypes = [C.c_void_p, _dp, _dp]
+    for name in ("ora_solve_et", "ora_solve_bt_iter", "ora_solve_djdxi"):
+        getattr(L, name).argtypes = [C.c_void_p, _dp, _dp, _dp]
+    L.ora_conv_record.argtypes = [C.c_void_p, _dp]
+    L.ora_conv_compare.argtypes = [C.c_void_p, _dp, _dp, _dp]
+    L.ora_qdeposit.argtypes = [C.c_void_p, C.POINTER(_Species), _dp]
+    L.ora_amjdeposit_robust.argtypes = [C.c_void_p, C.POINTER(_Species), _dp, _dp, _dp, _dp,
+                                        _dp, C.c_double]
+    L.ora_push_u_robust.argtypes = [C.c_void_p, C.POINTER(_Species), _dp, _dp, C.c_double]
+    L.ora_push_x.argtypes = [C.POINTER(_Species), C.c_double]
+    L.ora_update_bound.argtypes = [C.c_void_p, C.POINTER(_Species)]
+    L.ora_interp_point.argtypes = [C.c_void_p, _dp, C.c_int, C.c_double, C.c_double, _dp]
+    L.ora_beam_qdeposit.argtypes = [C.c_void_p, C.c_int, _dp]
+    L.ora_beam_push.argtypes = [C.c_void_p, C.c_int]
+    L.ora_species_inject.argtypes = [C.c_void_p, C.c_int]
+    L.ora_species_init_q.argtypes = [C.c_void_p, C.c_int]
+    L.ora_last_iters.argtypes = [C.c_void_p]
+    L.ora_philox4x32.argtypes = [C.POINTER(C.c_uint32), C.POINTER(C.c_uint32),
+                                 C.POINTER(C.c_uint32)]
+    L.ora_normal4.argtypes = [C.c_uint64, C.c_uint64, C.c_uint32, _dp]
+    return L
 
 
 def ptr(a):
@@ -154,9 +160,9 @@ def beam_cfg(total_num, total_charge, center, sigma, range_lo, range_hi, qm=-1.0
                    d3(*range_lo), d3(*range_hi), d3(*uth), int(quiet), int(evol), seed)
 
 
-def matrix(kind, mode, nr, dr, relax_fac, bnd=BND_OPEN):
+def matrix(kind, mode, nr, dr, relax_fac, bnd=BND_OPEN, L=None):
     a, b, c = (np.zeros(nr) for _ in range(3))
-    lib().ora_matrix(kind, mode, nr, dr, relax_fac, bnd, ptr(a), ptr(b), ptr(c))
+    (L or lib()).ora_matrix(kind, mode, nr, dr, relax_fac, bnd, ptr(a), ptr(b), ptr(c))
     return a, b, c
 
 
@@ -185,11 +191,13 @@ def normal4(seed, stream, sub):
 
 
 class Oracle:
-    """One oracle simulation instance (simulation_class restated)."""
+    """One oracle simulation instance (simulation_class restated).  `L`: the
+    library to run (default: the built oracle)."""
 
-    def __init__(self, grid):
+    def __init__(self, grid, L=None):
+        self.L = L or lib()
         self.grid = grid
-        self.h = lib().ora_create(C.byref(grid))
+        self.h = self.L.ora_create(C.byref(grid))
         self.nm = 2 * grid.max_mode + 1
         self.ld = grid.nr + 2
         self.nspecies = 0
@@ -197,7 +205,7 @@ class Oracle:
 
     def close(self):
         if self.h:
-            lib().ora_destroy(self.h)
+            self.L.ora_destroy(self.h)
             self.h = None
 
     def __del__(self):
@@ -208,49 +216,49 @@ class Oracle:
 
     # -- construction
     def add_species(self, cfg):
-        r = lib().ora_add_species(self.h, C.byref(cfg))
+        r = self.L.ora_add_species(self.h, C.byref(cfg))
         self.nspecies += 1
         return r
 
     def add_beam(self, cfg, npmax=0):
-        r = lib().ora_add_beam(self.h, C.byref(cfg), npmax)
+        r = self.L.ora_add_beam(self.h, C.byref(cfg), npmax)
         self.nbeams += 1
         return r
 
     def set_beam(self, ib, x1, x2, x3, p1, p2, p3, q):
         arrs = [np.ascontiguousarray(a, dtype=np.float64) for a in (x1, x2, x3, p1, p2, p3, q)]
-        rc = lib().ora_beam_set(self.h, ib, len(arrs[0]), *[ptr(a) for a in arrs])
+        rc = self.L.ora_beam_set(self.h, ib, len(arrs[0]), *[ptr(a) for a in arrs])
         assert rc == 0
 
     # -- stepping
     def step_begin(self):
-        lib().ora_step3d_begin(self.h)
+        self.L.ora_step3d_begin(self.h)
 
     def slice(self, j):
         n = C.c_int(0)
-        lib().ora_slice(self.h, j, C.byref(n))
+        self.L.ora_slice(self.h, j, C.byref(n))
         return n.value
 
     def step_end(self, i=1):
-        lib().ora_step3d_end(self.h, i)
+        self.L.ora_step3d_end(self.h, i)
 
     def run(self, nstep3d, nslices=0):
-        lib().ora_run(self.h, nstep3d, nslices)
+        self.L.ora_run(self.h, nstep3d, nslices)
 
     # -- state access (views)
     def field(self, fid):
-        p = lib().ora_field(self.h, fid)
+        p = self.L.ora_field(self.h, fid)
         n = NCOMP[fid] * self.nm * self.ld
         return np.ctypeslib.as_array(p, shape=(n,)).reshape(NCOMP[fid], self.nm, self.ld)
 
     def hist(self, fid):
-        p = lib().ora_hist(self.h, fid)
+        p = self.L.ora_hist(self.h, fid)
         ns = self.grid.nxi + 2
         n = ns * NCOMP[fid] * self.nm * self.ld
         return np.ctypeslib.as_array(p, shape=(n,)).reshape(ns, NCOMP[fid], self.nm, self.ld)
 
     def species_raw(self, isp):
-        return lib().ora_get_species(self.h, isp)
+        return self.L.ora_get_species(self.h, isp)
 
     def species(self, isp):
         s = self.species_raw(isp).contents
@@ -281,7 +289,7 @@ class Oracle:
                 np.ctypeslib.as_array(s.qn, shape=(n,)).reshape(1, self.nm, self.ld))
 
     def beam(self, ib):
-        b = lib().ora_get_beam(self.h, ib).contents
+        b = self.L.ora_get_beam(self.h, ib).contents
         n = int(b.npp)
         out = {}
         for name in ("x1", "x2", "x3", "p1", "p2", "p3", "q"):
@@ -290,12 +298,12 @@ class Oracle:
         return out
 
     def last_iters(self):
-        return lib().ora_last_iters(self.h)
+        return self.L.ora_last_iters(self.h)
 
     # -- building blocks on this instance's grid
     def new_field(self, ncomp):
         return np.zeros((ncomp, self.nm, self.ld))
 
     def solve(self, name, *arrays):
-        fn = getattr(lib(), "ora_solve_" + name)
+        fn = getattr(self.L, "ora_solve_" + name)
         fn(self.h, *[ptr(a) for a in arrays])

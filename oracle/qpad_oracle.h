@@ -18,8 +18,16 @@
  *                (s = 1..nxi are QPAD's f2 slices, s = nxi+1 is the upper
  *                 guard slice, s = 0 is unused)
  *
- * Parity pins (see the tests): closed forms, manufactured solutions,
- * invariants and dense solves.  HYPRE StructCycRed (the reference's
+ * Parity pins (tests/test_oracle_pins.py, tests/oracle_pins.py): every
+ * routine is checked against what the mathematics fixes -- closed forms,
+ * polynomial exactness of each stencil (axis and edge rows included),
+ * manufactured solutions with second-order convergence, vacuum fall-off of
+ * the boundary folds, invariants, dense solves, known-answer vectors -- and
+ * tests/oracle_mutants.py shows each check failing on a copy of this file
+ * with one plausible mistake planted.  Restated without a mathematical pin:
+ * the B- outer fold using m + 1 (field_solver_class.f03:532-540), the m = 0
+ * zero ghost of psi/Ez/Bz, and the order of the slice driver (checked only
+ * by the linear-wake test).  HYPRE StructCycRed (the reference's
  * tridiagonal solver, third-party, unversioned) is replaced by Thomas'
  * algorithm on the same matrices: parity unpinned against HYPRE's rounding.
  * gfortran's random_number stream is replaced by Philox4x32-10: thermal

@@ -1,0 +1,111 @@
+"""Copies of the oracle with one plausible mistake planted each.
+
+Each mutant names a text edit of oracle/qpad_oracle.c (a dropped term, a
+wrong sign, a transposed component, a wrong axis factor or boundary fold)
+and the pins of tests/oracle_pins.py that must fail on it.  A mistake in a
+matrix is named only for the matrix pin: the source pins recover the
+right-hand side with the library's own matrix, so they test the sources
+given the matrix.  The copies are compiled with the oracle's own flags into
+a temporary directory; nothing here touches the built oracle.  Mistakes are
+planted only in this CPU code.
+"""
+import concurrent.futures as cf
+import os
+import subprocess
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+SRC = os.path.join(HERE, "..", "oracle", "qpad_oracle.c")
+CFLAGS = ["-O2", "-fPIC", "-std=c11", "-ffp-contract=off", "-shared"]
+
+# name: (original text, planted text, pins that must fail)
+MUTANTS = {
+    "bplus_index_m": ("double t = (double)(m + 1) / j;", "double t = (double)(m) / j;",
+                      ["matrices"]),
+    "bminus_axis_relax": ("if (m == 1) { a[0] = 0.0; b[0] = -4.0 - relax; c[0] = 4.0; }",
+                          "if (m == 1) { a[0] = 0.0; b[0] = -4.0; c[0] = 4.0; }",
+                          ["matrices"]),
+    "bt_log_fold_sign": ("if (m == 0) coef = 1.0 + 1.0 / (jmax * log(jmax * dr));",
+                         "if (m == 0) coef = 1.0 - 1.0 / (jmax * log(jmax * dr));", ["matrices"]),
+    "open_fold_m_plus_1": ("else coef = 1.0 - (double)m / jmax;\n            break;\n        case ORA_FK_BT:",
+                           "else coef = 1.0 - (double)(m + 1) / jmax;\n            break;\n        case ORA_FK_BT:",
+                           ["matrices"]),
+    "ez_coupling_sign": ("bre[i] = idrh * (JRE(1, i + 1) - JRE(1, i - 1)) + ir * JRE(1, i) - dm * ir * JIM(2, i);",
+                         "bre[i] = idrh * (JRE(1, i + 1) - JRE(1, i - 1)) + ir * JRE(1, i) + dm * ir * JIM(2, i);",
+                         ["ez"]),
+    "ez_gauss_edge": ("* ((double)nr - 2.5);", "* ((double)nr - 2.0);", ["ez"]),
+    "ez_m1_node2_drop_term": (
+        "bre[2] = idr * (JRE(1, 3) - JRE(1, 2)) + ir * JRE(1, 2) - dm * ir * JIM(2, 2);",
+        "bre[2] = idr * (JRE(1, 3) - JRE(1, 2)) - dm * ir * JIM(2, 2);", ["ez"]),
+    "bz_coupling_sign": ("bim[i] = -idrh * (JIM(2, i + 1) - JIM(2, i - 1)) - ir * JIM(2, i) + dm * ir * JRE(1, i);",
+                         "bim[i] = -idrh * (JIM(2, i + 1) - JIM(2, i - 1)) - ir * JIM(2, i) - dm * ir * JRE(1, i);",
+                         ["bz"]),
+    "bz_axis_factor": ("bre[1] = -2.0 * idr * JRE(2, 2);", "bre[1] = -1.0 * idr * JRE(2, 2);", ["bz"]),
+    "eperp_coupling_sign": ("FA(e, 1, r, i) = -FA(b, 0, r, i) + ir * dm * FA(psi, 0, q, i);",
+                            "FA(e, 1, r, i) = -FA(b, 0, r, i) - ir * dm * FA(psi, 0, q, i);", ["eperp"]),
+    "eperp_m1_axis_half": ("FA(e, 0, r, 1) = FA(b, 1, r, 1) - idr * FA(psi, 0, r, 2);",
+                           "FA(e, 0, r, 1) = FA(b, 1, r, 1) - idrh * FA(psi, 0, r, 2);", ["eperp"]),
+    "djdxi_transposed_flux": (
+        "DR(1, i) = AR(1, i) - idrh * (MR(1, i + 1) - MR(1, i - 1)) - ir * MR(1, i) + dm * ir * MI(2, i);",
+        "DR(1, i) = AR(1, i) - idrh * (MR(1, i + 1) - MR(1, i - 1)) - ir * MR(1, i) + dm * ir * MI(3, i);",
+        ["djdxi"]),
+    "djdxi_m2_node2_drop_term": (
+        "DR(1, 2) = AR(1, 2) - idr * (MR(1, 3) - MR(1, 2)) - ir * MR(1, 2) + dm * ir * MI(2, 2);",
+        "DR(1, 2) = AR(1, 2) - idr * (MR(1, 3) - MR(1, 2)) + dm * ir * MI(2, 2);", ["djdxi"]),
+    "bt_iter_relax_sign": ("b1r[i] = -F1R(2, i) - F3R(1, i) * rx;", "b1r[i] = -F1R(2, i) + F3R(1, i) * rx;",
+                           ["bt_iter"]),
+    "bt_iter_recombination": ("FA(bf, 1, kr, i) = 0.5 * (b1i[i] - b2i[i]);",
+                              "FA(bf, 1, kr, i) = 0.5 * (b2i[i] - b1i[i]);", ["bt_iter"]),
+    "bt_iter_jz_coupling_sign": ("s1r = -F1R(2, i) + dm * F2I(3, i) * ir;",
+                                 "s1r = -F1R(2, i) - dm * F2I(3, i) * ir;", ["bt_iter"]),
+    "bt_beam_br_sign": ("FA(bf, 0, r, i) = -ir * dm * bim[i];", "FA(bf, 0, r, i) = ir * dm * bim[i];",
+                        ["bt_beam"]),
+    "amj_axis_m1_factor": ("FA(cu, 0, k, 1) = 8.0 * FA(cu, 0, k, 1); FA(cu, 1, k, 1) = 8.0 * FA(cu, 1, k, 1);",
+                           "FA(cu, 0, k, 1) = 4.0 * FA(cu, 0, k, 1); FA(cu, 1, k, 1) = 8.0 * FA(cu, 1, k, 1);",
+                           ["amjdeposit"]),
+    "amj_axis_m2_flux_zeroed": ("for (int c = 0; c < 3; c++) FA(amu, c, k, 1) = 8.0 * FA(amu, c, k, 1);",
+                                "for (int c = 0; c < 3; c++) FA(amu, c, k, 1) = 0.0 * FA(amu, c, k, 1);",
+                                ["amjdeposit"]),
+    "amj_flux_missing_ipsi": ("u2[0] = u[0] * u[0] * ipsi;", "u2[0] = u[0] * u[0];", ["amjdeposit"]),
+    "amj_phase_not_conjugated": ("double ph0r = pc, ph0i = -ps;", "double ph0r = pc, ph0i = ps;",
+                                 ["amjdeposit"]),
+    "beam_xi_drift": ("bm->x3[p] = bm->x3[p] - bm->p3[p] * dt_gam + bm->dt;",
+                      "bm->x3[p] = bm->x3[p] - bm->p3[p] * dt_gam;", ["beam_push"]),
+    "beam_force_sign": ("wp[0] = ep[0] - bp[1];\n        wp[1] = ep[1] + bp[0];\n        wp[2] = ep[2];\n"
+                        "        bm->p1[p]",
+                        "wp[0] = ep[0] + bp[1];\n        wp[1] = ep[1] + bp[0];\n        wp[2] = ep[2];\n"
+                        "        bm->p1[p]", ["beam_push"]),
+    "beam_rotation": ("double t = ep[0] * cc - ep[1] * ss; double u = ep[0] * ss + ep[1] * cc;",
+                      "double t = ep[0] * cc + ep[1] * ss; double u = ep[0] * ss + ep[1] * cc;", ["beam_push"]),
+    "inject_cell_edge_lattice": ("double rn = ((double)i1 - 0.5) / (double)c->ppc1 + (double)(i - 1);",
+                                 "double rn = ((double)i1) / (double)c->ppc1 + (double)(i - 1);", ["inject"]),
+    "beam_quiet_charge": ("bm->q[i] = 0.5 * bm->q[i];", "bm->q[i] = 1.0 * bm->q[i];", ["beam_inject"]),
+    "beam_sigma_swapped": ("xt[d] = g[d] * c->sigma[d] + c->center[d];",
+                           "xt[d] = g[d] * c->sigma[2 - d] + c->center[d];", ["beam_inject"]),
+    "update_bound_skips_one": ("while (i < sp->npp - 1) {\n        double pos = sqrt(sp->x1[i]",
+                               "while (i < sp->npp - 2) {\n        double pos = sqrt(sp->x1[i]",
+                               ["update_bound"]),
+}
+
+
+def source_of(name):
+    text = open(SRC).read()
+    old, new, _ = MUTANTS[name]
+    assert text.count(old) == 1, f"mutant {name}: the original text must occur exactly once"
+    return text.replace(old, new)
+
+
+def build_all(outdir, names=None):
+    """compile every mutant (in parallel) into outdir; returns {name: path}"""
+    names = sorted(MUTANTS) if names is None else names
+    inc = os.path.join(HERE, "..", "oracle")
+
+    def one(name):
+        src = os.path.join(outdir, f"{name}.c")
+        with open(src, "w") as f:
+            f.write(source_of(name))
+        lib = os.path.join(outdir, f"lib_{name}.so")
+        subprocess.run(["gcc"] + CFLAGS + ["-I", inc, "-o", lib, src, "-lm"], check=True)
+        return name, lib
+
+    with cf.ThreadPoolExecutor(max_workers=8) as ex:
+        return dict(ex.map(one, names))

@@ -1,0 +1,634 @@
+"""Pins of the oracle's field and particle routines, fixed by mathematics.
+
+Each pin_* function takes an oracle library handle (pyoracle.load_library)
+and raises AssertionError when the routine it pins departs from what the
+mathematics of the discretisation fixes.  tests/test_oracle_pins_math.py runs
+every pin on the built oracle, and on copies of the oracle compiled with one
+plausible mistake planted each (tests/oracle_mutants.py), to show that the
+pin sees that mistake.
+
+What fixes each routine (reference lines are the QPAD source it restates):
+
+* Tridiagonal matrices (field_solver_class.f03:256-561): applied to a
+  quadratic polynomial, an interior row is exactly (L_k - alpha) P at the
+  node, L_k = d2/dr2 + (1/r) d/dr - k^2/r^2, k = m (psi, Ez, Bz, beam bt),
+  m + 1 (B+), m - 1 (B-), alpha = relax_fac / dr^2 on B+-; the axis rows
+  [0, -4, 4] (k = 0) are exactly L_0 P(0) for even P; open-boundary folds
+  x_{N+1} = c x_N match the ratio of the vacuum solution (r^-k, or ln r for
+  the beam potential at m = 0) between the last two nodes to O(1/N^2).
+* Sources of Ez, Bz, E_perp, dJ_perp/dxi and B+-: recovered as A x from the
+  solution (the matrix itself pinned as above) or read directly, they equal
+  the continuous operator on polynomial inputs at every node where the
+  stencil is exact for that degree (central and three-point one-sided
+  differences: quadratics; two-point: linears), including the special axis
+  rows, whose values follow from regularity (lim_{r->0} P(r)/r = P'(0)):
+    Ez  : div J_perp   = (1/r) d(r J_r)/dr + (im/r) J_phi
+    Bz  : -curl J_perp = -(1/r) d(r J_phi)/dr + (im/r) J_r
+    E_perp = (B_phi - dpsi/dr, -B_r - (im/r) psi)
+    dJ/dxi = acu - ((1/r) d(r amu_1)/dr + (im/r) amu_2,
+                    (1/r) d(r amu_2)/dr + (im/r) amu_3)
+    (L_{m+-1} - alpha) B+- = s1 +- i s2 - alpha B+-_old, B+- = B_r +- i B_phi,
+      s1 = -dJ_phi/dxi - (im/r) J_z, s2 = dJ_r/dxi + dJ_z/dr
+  (field_e_class.f03:148-514, field_b_class.f03:205-307,360-506,703-758,
+  field_src_class.f03:273-405).  The m = 0 Ez axis value is the discrete
+  Gauss law, exact for J_r linear in r.
+* Beam B_perp (field_b_class.f03:545-701): a manufactured potential
+  A = r^m exp(-r^2/w^2), B_r = i (m/r) A, B_phi = -dA/dr, converges at
+  second order.
+* Current deposit (part2d_class.f03:746-1010): one particle in zero fields
+  keeps u = u0, so every deposit is closed form: J = q u / (gamma - u_z),
+  momentum flux q u_a u_b / (gamma - u_z)^2, times the linear weights and
+  e^{-im phi}, divided by the node index; on the axis node a component is
+  kept (x8, the axis cell's volume 1/8 against j - 1 elsewhere) only for the
+  mode in which it is non-zero on the axis: scalars (J_z) at m = 0, vectors
+  (J_perp, dJ_perp/dxi) at m = 1, the flux tensor at m = 2 (:916-960).  A
+  particle at rest in a uniform E_z gets the closed-form kick
+  u_z = q/m dxi E_z of the robust pusher.
+* Beam push (part3d_class.f03:477-576): uniform Cartesian fields, written in
+  the mode representation (m = 1 for transverse fields), give the
+  closed-form reduced-pusher kick p += q/m dt (E_x - B_y, E_y + B_x, E_z)
+  and drift x_perp += p_perp dt / gamma, xi += dt - p_z dt / gamma;
+  particles at r >= r_max or xi >= xi_max are removed.
+* Plasma injection (fdist2d_class.f03:289-357): the lattice deposits a
+  uniform density exactly (rho = sign(q) n at nodes 2..nr), no m > 0
+  content, and a total charge per unit length of sign(q) n pi R^2.
+* update_bound (part2d_class.f03:2307-2353): hand-worked examples of
+  swap-with-last removal.
+* Random beam injection (fdist3d_rnd_class.f03:360-513): sample moments of
+  the truncated Gaussian, the ranges, p_z from gamma, the quiet-start
+  mirror and its halved charges.
+"""
+import math
+
+import numpy as np
+from scipy.stats import truncnorm
+
+import pyoracle as po
+
+NR, DR = 48, 0.05
+TOL = 1e-9
+
+
+def kre(m):
+    return 0 if m == 0 else 2 * m - 1
+
+
+def kim(m):
+    return 2 * m
+
+
+def nodes(nr, dr):
+    """radius of node i = 0..nr+1 (node 1 on the axis)"""
+    return (np.arange(nr + 2) - 1.0) * dr
+
+
+def put(f, c, m, v):
+    f[c, kre(m), :] = np.real(v)
+    if m > 0:
+        f[c, kim(m), :] = np.imag(v)
+
+
+def get(f, c, m):
+    v = f[c, kre(m), :].astype(complex)
+    if m > 0:
+        v = v + 1j * f[c, kim(m), :]
+    return v
+
+
+def poly(c, r):
+    return c[0] + c[1] * r + c[2] * r * r
+
+
+def dpoly(c, r):
+    return c[1] + 2 * c[2] * r
+
+
+def coeffs(rng, m, deg, zero0=False):
+    c = rng.uniform(-1, 1, 3) + (1j * rng.uniform(-1, 1, 3) if m > 0 else 0)
+    c = np.asarray(c, dtype=complex)
+    if deg < 2:
+        c[2] = 0
+    if zero0:
+        c[0] = 0
+    return c
+
+
+def apply_matrix(L, kind, m, x, nr, dr, relax):
+    """A x for the oracle's matrix of (kind, mode); x at nodes 1..nr"""
+    a, b, c = po.matrix(kind, m, nr, dr, relax, L=L)
+    y = b * x
+    y[1:] += a[1:] * x[:-1]
+    y[:-1] += c[:-1] * x[1:]
+    return y
+
+
+def close(got, want, rows, what, tol=TOL):
+    got, want = np.asarray(got)[rows], np.asarray(want)[rows]
+    scale = max(np.max(np.abs(want)), 1e-300)
+    err = np.max(np.abs(got - want)) / scale
+    assert err <= tol, f"{what}: relative error {err:.3e}"
+
+
+def oracle(L, nr=NR, dr=DR, M=3):
+    return po.Oracle(po.make_grid(nr, 4, M, dr, 0.05), L)
+
+
+def _over_r(v, r):
+    with np.errstate(divide="ignore", invalid="ignore"):
+        return np.where(r > 0, v / np.where(r > 0, r, 1.0), 0.0)
+
+
+# ---------------------------------------------------------------------------
+# matrices
+# ---------------------------------------------------------------------------
+KINDS = [(po.FK_PSI, 0), (po.FK_EZ, 0), (po.FK_BZ, 0), (po.FK_BT, 0), (po.FK_BPLUS, 1), (po.FK_BMINUS, -1)]
+
+
+def pin_matrices(L):
+    nr, dr = NR, DR
+    relax = 1e-3 * (dr / 0.02) ** 2
+    alpha = relax / dr ** 2
+    r = nodes(nr, dr)[1:nr + 1]
+    rng = np.random.default_rng(5)
+    for kind, shift in KINDS:
+        for m in range(4):
+            k = m + shift
+            al = alpha if kind in (po.FK_BPLUS, po.FK_BMINUS) else 0.0
+            a, b, c = po.matrix(kind, m, nr, dr, relax, L=L)
+            ident = not (k == 0 and (kind != po.FK_BPLUS))
+            # interior rows: exact for quadratics (P(0) = 0 where node 1 is
+            # decoupled, i.e. a Dirichlet axis)
+            cf = rng.uniform(-1, 1, 3)
+            if ident:
+                cf[0] = 0.0
+            x = poly(cf, r)
+            y = apply_matrix(L, kind, m, x, nr, dr, relax)
+            exact = 2 * cf[2] + _over_r(dpoly(cf, r), r) - k * k * _over_r(_over_r(x, r), r) - al * x
+            close(y, exact, slice(1, nr - 1), f"kind {kind} m {m} interior rows")
+            if ident:
+                # identity axis row, node 2 decoupled from node 1
+                assert a[0] == 0.0 and c[0] == 0.0 and b[0] * dr * dr == 1.0 and a[1] == 0.0, (kind, m)
+            else:
+                # [0, -4(-relax), 4] / dr^2: L_0 P(0) - alpha P(0) for even P
+                ce = np.array([rng.uniform(-1, 1), 0.0, rng.uniform(-1, 1)])
+                ye = apply_matrix(L, kind, m, poly(ce, r), nr, dr, relax)
+                assert abs(ye[0] - (4 * ce[2] - al * ce[0])) <= 1e-9 * abs(4 * ce[2]), (kind, m, ye[0])
+            # outer boundary: fold coefficient = vacuum ratio x(R) / x(R - dr)
+            jl = nr - 1
+            fold = (b[-1] * dr * dr + 2.0 + k * k / jl ** 2 + al * dr * dr) / (1.0 + 0.5 / jl)
+            if kind == po.FK_BT and m == 0:
+                u, v = math.log(jl * dr), math.log(nr * dr)
+                assert abs(fold - v / u) <= 2.0 / (nr * nr * abs(u * v)), ("bt m0 log fold", fold, v / u)
+            elif kind in (po.FK_PSI, po.FK_EZ, po.FK_BZ) and m == 0:
+                assert abs(fold) <= 1e-12, ("m = 0 zero ghost", kind, fold)
+            elif kind == po.FK_BMINUS:
+                # QPAD folds B- with the same m + 1 as B+
+                # (field_solver_class.f03:532-540): restated, not derivable
+                a2, b2, c2 = po.matrix(po.FK_BPLUS, m, nr, dr, relax, L=L)
+                kp = m + 1
+                fp = (b2[-1] * dr * dr + 2.0 + kp * kp / jl ** 2 + al * dr * dr) / (1.0 + 0.5 / jl)
+                assert abs(fold - fp) <= 1e-12, ("B- fold equals B+ fold", m, fold, fp)
+            else:
+                ratio = (jl / nr) ** abs(k)
+                assert abs(fold - ratio) <= k * k / nr ** 2 + 1e-12, ("vacuum fold", kind, m, fold, ratio)
+
+
+# ---------------------------------------------------------------------------
+# Ez and Bz sources through the solve
+# ---------------------------------------------------------------------------
+def _ez_bz(L, which):
+    o = oracle(L)
+    nr, dr = NR, DR
+    rall = nodes(nr, dr)
+    r = rall[1:nr + 1]
+    rng = np.random.default_rng(7 if which == "ez" else 8)
+    kind, comp = (po.FK_EZ, 2) if which == "ez" else (po.FK_BZ, 2)
+    for m in range(4):
+        for deg in (1, 2):
+            # the differentiated component vanishes on the axis at m = 0
+            # (regular J_perp); the other one enters without derivative
+            ca = coeffs(rng, m, deg, zero0=(m == 0))
+            cb = coeffs(rng, m, deg, zero0=(m == 0))
+            jay = o.new_field(3)
+            put(jay, 0, m, poly(ca, rall))
+            put(jay, 1, m, poly(cb, rall))
+            out = o.new_field(3)
+            o.solve(which, jay, out)
+            x = get(out, comp, m)[1:nr + 1]
+            y = apply_matrix(L, kind, m, x, nr, dr, 0.0)
+            jr, jp = poly(ca, r), poly(cb, r)
+            if which == "ez":
+                s = dpoly(ca, r) + _over_r(jr, r) + 1j * m * _over_r(jp, r)
+                s0 = 2 * ca[1]          # lim div J at r = 0 (m = 0)
+            else:
+                s = -(dpoly(cb, r) + _over_r(jp, r)) + 1j * m * _over_r(jr, r)
+                s0 = -2 * cb[1]
+            if m == 0:
+                s[0] = s0
+                if deg == 1:
+                    rows = slice(0, nr)                    # axis (Gauss law) and edge exact
+                else:
+                    # interior; Ez's last row is two-point, Bz's three-point
+                    rows = slice(1, nr - 1) if which == "ez" else slice(1, nr)
+            else:
+                # node 1 is the identity row (solution forced to 0); m = 1
+                # takes a forward difference at node 2 (exact for linears)
+                rows = slice(2 if (m == 1 and deg == 2) else 1, nr)
+                assert get(out, comp, m)[1] == 0.0, f"{which} m {m}: non-zero on the axis"
+            close(y, s, rows, f"{which} m {m} degree {deg}")
+
+
+def pin_ez(L):
+    _ez_bz(L, "ez")
+
+
+def pin_bz(L):
+    _ez_bz(L, "bz")
+
+
+# ---------------------------------------------------------------------------
+# E_perp from B and psi
+# ---------------------------------------------------------------------------
+def pin_eperp(L):
+    o = oracle(L)
+    nr, dr = NR, DR
+    rall = nodes(nr, dr)
+    rng = np.random.default_rng(9)
+    for m in range(4):
+        for deg in (1, 2):
+            if m == 0:
+                cp = np.array([rng.uniform(-1, 1), 0.0, rng.uniform(-1, 1)], dtype=complex)   # even psi
+                cbr = np.array([0.0, rng.uniform(-1, 1), 0.0], dtype=complex)              # B ~ r
+                cbp = np.array([0.0, rng.uniform(-1, 1), 0.0], dtype=complex)
+            else:
+                cp = coeffs(rng, m, deg, zero0=True)
+                if m >= 2:
+                    cp[1] = 0.0          # psi ~ r^2: d psi/dr and psi/r vanish on the axis
+                cbr = coeffs(rng, m, 1, zero0=(m >= 2))
+                cbp = coeffs(rng, m, 1, zero0=(m >= 2))
+            psi = o.new_field(1)
+            b = o.new_field(3)
+            put(psi, 0, m, poly(cp, rall))
+            put(b, 0, m, poly(cbr, rall))
+            put(b, 1, m, poly(cbp, rall))
+            e = o.new_field(3)
+            o.solve("et", b, psi, e)
+            er_want = poly(cbp, rall) - dpoly(cp, rall)
+            ep_want = -poly(cbr, rall) - 1j * m * _over_r(poly(cp, rall), rall)
+            if m >= 1:
+                ep_want[1] = -poly(cbr, rall)[1] - 1j * m * cp[1]    # lim psi/r = psi'(0)
+            rows = slice(1, nr + 1)
+            if m == 1 and deg == 2:
+                rows = slice(2, nr + 1)     # axis: forward difference, exact for linears
+            close(get(e, 0, m), er_want, rows, f"E_r m {m} degree {deg}")
+            close(get(e, 1, m), ep_want, rows, f"E_phi m {m} degree {deg}")
+
+
+# ---------------------------------------------------------------------------
+# dJ_perp / dxi
+# ---------------------------------------------------------------------------
+def pin_djdxi(L):
+    o = oracle(L)
+    nr, dr = NR, DR
+    rall = nodes(nr, dr)
+    rng = np.random.default_rng(10)
+    for m in range(4):
+        for deg in (1, 2):
+            ca = [coeffs(rng, m, deg) for _ in range(2)]
+            cm = [coeffs(rng, m, deg, zero0=(m == 1)) for _ in range(3)]
+            acu, amu = o.new_field(2), o.new_field(3)
+            for c_ in range(2):
+                put(acu, c_, m, poly(ca[c_], rall))
+            for c_ in range(3):
+                put(amu, c_, m, poly(cm[c_], rall))
+            d = o.new_field(2)
+            o.solve("djdxi", acu, amu, d)
+            want = []
+            for c_ in range(2):
+                m1, m2 = cm[c_], cm[c_ + 1]
+                w = (poly(ca[c_], rall) - (dpoly(m1, rall) + _over_r(poly(m1, rall), rall))
+                     - 1j * m * _over_r(poly(m2, rall), rall))
+                if m == 1:      # regular limits at r = 0 (m1, m2 vanish there)
+                    w[1] = ca[c_][0] - 2 * m1[1] - 1j * m * m2[1]
+                want.append(w)
+            if m == 1:
+                rows = slice(1, nr + 1) if deg == 1 else slice(2, nr + 1)
+            elif m == 2:
+                rows = slice(2 if deg == 1 else 3, nr + 1)   # node 2: forward difference
+                assert np.all(get(d, 0, m)[1] == 0) and np.all(get(d, 1, m)[1] == 0), "m = 2 axis"
+            else:
+                rows = slice(2, nr + 1)
+                assert get(d, 0, m)[1] == 0 and get(d, 1, m)[1] == 0, f"m {m} axis"
+            for c_ in range(2):
+                close(get(d, c_, m), want[c_], rows, f"dJ/dxi comp {c_} m {m} degree {deg}")
+
+
+# ---------------------------------------------------------------------------
+# B_perp predictor-corrector solve
+# ---------------------------------------------------------------------------
+def pin_bt_iter(L):
+    o = oracle(L)
+    nr, dr = NR, DR
+    relax = o.grid.relax_fac
+    alpha = relax / dr ** 2
+    rall = nodes(nr, dr)
+    r = rall[1:nr + 1]
+    rng = np.random.default_rng(11)
+    for m in range(4):
+        for deg in (1, 2):
+            cd = [coeffs(rng, m, 2) for _ in range(2)]          # dJ_r/dxi, dJ_phi/dxi
+            cz = coeffs(rng, m, deg, zero0=(m >= 1))            # J_z (regular: ~ r for m >= 1)
+            cbo = [coeffs(rng, m, 2) for _ in range(2)]         # B_old r, phi
+            dj, jay, bf = o.new_field(2), o.new_field(3), o.new_field(3)
+            put(dj, 0, m, poly(cd[0], rall))
+            put(dj, 1, m, poly(cd[1], rall))
+            put(jay, 2, m, poly(cz, rall))
+            put(bf, 0, m, poly(cbo[0], rall))
+            put(bf, 1, m, poly(cbo[1], rall))
+            o.solve("bt_iter", dj, jay, bf)
+            br, bp = get(bf, 0, m)[1:nr + 1], get(bf, 1, m)[1:nr + 1]
+            jz, djr, djp = poly(cz, r), poly(cd[0], r), poly(cd[1], r)
+            s1 = -djp - 1j * m * _over_r(jz, r)
+            s2 = djr + dpoly(cz, r)
+            bro, bpo = poly(cbo[0], r), poly(cbo[1], r)
+            if m == 0:
+                yp = apply_matrix(L, po.FK_BPLUS, 0, br, nr, dr, relax)
+                ym = apply_matrix(L, po.FK_BMINUS, 0, bp, nr, dr, relax)
+                wp = -djp - alpha * bro
+                wm = djr + dpoly(cz, r) - alpha * bpo
+                # node 2 of B_phi: forward difference of J_z (exact for linears)
+                close(yp, wp, slice(1, nr), f"B_r m 0 degree {deg}")
+                close(ym, wm, slice(1 if deg == 1 else 2, nr), f"B_phi m 0 degree {deg}")
+                assert br[0] == 0 and bp[0] == 0, "m = 0 B_perp on the axis"
+                continue
+            bplus, bminus = br + 1j * bp, br - 1j * bp
+            yp = apply_matrix(L, po.FK_BPLUS, m, bplus, nr, dr, relax)
+            ym = apply_matrix(L, po.FK_BMINUS, m, bminus, nr, dr, relax)
+            wp = s1 + 1j * s2 - alpha * (bro + 1j * bpo)
+            wm = s1 - 1j * s2 - alpha * (bro - 1j * bpo)
+            close(yp, wp, slice(1, nr), f"B+ m {m} degree {deg}")
+            rows_m = slice(1, nr)
+            if m == 1 and deg == 1:
+                # B- at m = 1 is non-zero on the axis: row [0, -4 - relax, 4],
+                # source limits (im/r) J_z -> i m J_z'(0), dJ_z/dr -> J_z'(0)
+                s1_0 = -poly(cd[1], 0.0) - 1j * m * cz[1]
+                s2_0 = poly(cd[0], 0.0) + cz[1]
+                wm[0] = s1_0 - 1j * s2_0 - alpha * (cbo[0][0] - 1j * cbo[1][0])
+                rows_m = slice(0, nr)
+            close(ym, wm, rows_m, f"B- m {m} degree {deg}")
+            if m >= 2:
+                assert br[0] == 0 and bp[0] == 0, f"m {m} B_perp on the axis"
+
+
+# ---------------------------------------------------------------------------
+# beam B_perp: manufactured potential, second order
+# ---------------------------------------------------------------------------
+def _lap_gauss(r, m, w):
+    s = (r / w) ** 2
+    return 4.0 / w ** 2 * r ** m * np.exp(-s) * (s - m - 1.0)
+
+
+def pin_bt_beam(L):
+    w, R = 1.0, 6.0
+    for m in (1, 2, 3):
+        errs = []
+        for nr in (64, 128, 256):
+            dr = R / nr
+            o = oracle(L, nr=nr, dr=dr)
+            r = np.maximum(nodes(nr, dr), 0.0)
+            amp = 0.8 - 0.6j
+            A = amp * r ** m * np.exp(-(r / w) ** 2)
+            dA = amp * (m * r ** max(m - 1, 0) - 2 * r ** (m + 1) / w ** 2) * np.exp(-(r / w) ** 2)
+            q = o.new_field(1)
+            put(q, 0, m, -amp * _lap_gauss(r, m, w))
+            q[0, :, 0] = 0.0
+            b = o.new_field(3)
+            o.solve("bt_beam", q, b)
+            br_want = 1j * m * _over_r(A, r)
+            if m == 1:
+                br_want[1] = 1j * amp
+            bp_want = -dA
+            sl = slice(1, nr)
+            e = max(np.max(np.abs(get(b, 0, m)[sl] - br_want[sl])), np.max(np.abs(get(b, 1, m)[sl] - bp_want[sl])))
+            errs.append(e)
+        assert errs[-1] < 5e-3, ("beam B_perp", m, errs)
+        assert 3.0 < errs[0] / errs[1] < 5.0 and 3.0 < errs[1] / errs[2] < 5.0, ("second order", m, errs)
+
+
+# ---------------------------------------------------------------------------
+# particles
+# ---------------------------------------------------------------------------
+def _one_particle(o, x1, x2, p, q=-0.7):
+    arrs = {"x1": np.array([x1]), "x2": np.array([x2]), "p1": np.array([p[0]]),
+            "p2": np.array([p[1]]), "p3": np.array([p[2]]), "q": np.array([q]),
+            "gamma": np.array([math.sqrt(1 + p[0] ** 2 + p[1] ** 2 + p[2] ** 2)]), "psi": np.zeros(1)}
+    o.set_species(0, arrs)
+
+
+def _deposit(L, o, e, b, dxi):
+    cu, amu, acu = o.new_field(3), o.new_field(3), o.new_field(2)
+    L.ora_amjdeposit_robust(o.h, o.species_raw(0), po.ptr(e), po.ptr(b), po.ptr(cu), po.ptr(amu),
+                            po.ptr(acu), dxi)
+    return cu, amu, acu
+
+
+# components kept (x8) on the axis node, by mode: scalars at m = 0, vectors
+# at m = 1, the momentum-flux tensor at m = 2
+AXIS_KEEP = {"cu": {0: [2], 1: [0, 1]}, "acu": {1: [0, 1]}, "amu": {2: [0, 1, 2]}}
+
+
+def _expected(nr, dr, M, x1, x2, vals):
+    """vals[name] = per-component values of one particle; linear weights,
+    e^{-im phi}, axis rules, division by the node index"""
+    rp = math.hypot(x1, x2)
+    phi = math.atan2(x2, x1)
+    pos = rp / dr
+    ix = int(math.floor(pos)) + 1
+    d = pos - math.floor(pos)
+    nm = 2 * M + 1
+    out = {}
+    for name, v in vals.items():
+        f = np.zeros((len(v), nm, nr + 2))
+        for m in range(M + 1):
+            ph = np.exp(-1j * m * phi)
+            for node, wgt in ((ix, 1 - d), (ix + 1, d)):
+                for c_, val in enumerate(v):
+                    z = wgt * val * ph
+                    if node == 1:
+                        z = 8 * z if c_ in AXIS_KEEP[name].get(m, []) else 0.0
+                    else:
+                        z = z / (node - 1)
+                    f[c_, kre(m), node] += np.real(z)
+                    if m > 0:
+                        f[c_, kim(m), node] += np.imag(z)
+        out[name] = f
+    return out
+
+
+def pin_amjdeposit(L):
+    nr, dr, M, dxi = 16, 0.1, 3, 0.05
+    o = po.Oracle(po.make_grid(nr, 4, M, dr, dxi), L)
+    o.add_species(po.species_cfg(1, 1, 4))
+    e, b = o.new_field(3), o.new_field(3)
+    q = -0.7
+    for (rp, phi) in ((0.37 * dr, 0.9), (5.6 * dr, 2.2)):
+        p = (0.3, -0.2, 0.4)
+        x1, x2 = rp * math.cos(phi), rp * math.sin(phi)
+        _one_particle(o, x1, x2, p, q)
+        cu, amu, acu = _deposit(L, o, e, b, dxi)
+        ur = p[0] * math.cos(phi) + p[1] * math.sin(phi)
+        up = -p[0] * math.sin(phi) + p[1] * math.cos(phi)
+        uz = p[2]
+        g = math.sqrt(1 + ur * ur + up * up + uz * uz)
+        ipsi = 1.0 / (g - uz)
+        want = _expected(nr, dr, M, x1, x2, {
+            "cu": [q * ipsi * ur, q * ipsi * up, q * ipsi * uz],
+            "amu": [q * ipsi ** 2 * ur * ur, q * ipsi ** 2 * ur * up, q * ipsi ** 2 * up * up],
+            "acu": [0.0, 0.0]})
+        for name, got in (("cu", cu), ("amu", amu), ("acu", acu)):
+            err = np.max(np.abs(got - want[name]))
+            assert err <= 1e-13, f"{name}, particle at r = {rp / dr:.2f} dr: {err:.2e}"
+        s = o.species(0)
+        assert abs(s["gamma"][0] - g) <= 1e-14 and abs(s["psi"][0] - (1 - 1 / ipsi) / -1.0) <= 1e-14
+    # a particle at rest in a uniform E_z: robust kick u_z = q/m dxi E_z,
+    # time-centred u_z / 2 in the deposit
+    ez = 0.8
+    e[2, 0, :] = ez
+    rp, phi = 3.3 * dr, 0.4
+    x1, x2 = rp * math.cos(phi), rp * math.sin(phi)
+    _one_particle(o, x1, x2, (0.0, 0.0, 0.0), q)
+    cu, amu, acu = _deposit(L, o, e, b, dxi)
+    uz = -1.0 * dxi * ez / 2
+    g = math.sqrt(1 + uz * uz)
+    want = _expected(nr, dr, M, x1, x2, {"cu": [0.0, 0.0, q * uz / (g - uz)], "amu": [0.0] * 3, "acu": [0.0] * 2})
+    for name, got in (("cu", cu), ("amu", amu), ("acu", acu)):
+        assert np.max(np.abs(got - want[name])) <= 1e-13, f"E_z kick: {name}"
+
+
+def pin_beam_push(L):
+    nr, nxi, dr, dxi, dt = 16, 20, 0.1, 0.1, 3.0
+    M = 2
+    o = po.Oracle(po.make_grid(nr, nxi, M, dr, dxi, dt=dt), L)
+    o.add_beam(po.beam_cfg(0, 1.0, (0, 0, 1), (0.1, 0.1, 0.1), (-1, -1, 0), (1, 1, 2), qm=-1.0, mass=2.0),
+               npmax=8)
+    ex, ey, ez, bx, by = 0.3, -0.5, 0.7, 0.2, -0.4
+    he, hb = o.hist(po.E), o.hist(po.B)
+    # uniform Cartesian fields in modes: E_r = E_x cos + E_y sin, E_phi =
+    # -E_x sin + E_y cos, i.e. m = 1 with Re(E_r) = E_x/2, Im(E_r) = -E_y/2,
+    # Re(E_phi) = E_y/2, Im(E_phi) = E_x/2 (f = f0 + 2 (Re f cos - Im f sin))
+    for h, (fx, fy) in ((he, (ex, ey)), (hb, (bx, by))):
+        h[:, 0, 1, :] = fx / 2
+        h[:, 0, 2, :] = -fy / 2
+        h[:, 1, 1, :] = fy / 2
+        h[:, 1, 2, :] = fx / 2
+    he[:, 2, 0, :] = ez
+    # particles 3 (r) and 4 (xi) leave the box
+    x1 = np.array([0.31, -0.52, 0.05, 1.595, 0.2])
+    x2 = np.array([0.12, 0.33, -0.6, 0.0, 0.1])
+    x3 = np.array([0.45, 1.21, 0.77, 0.5, 1.95])
+    p1 = np.array([0.5, -1.0, 0.2, 5.0, 0.0])
+    p2 = np.array([0.1, 0.3, -0.4, 0.0, 0.0])
+    p3 = np.array([200.0, 150.0, 400.0, 300.0, 2.0])
+    q = np.full(5, -1.0)
+    o.set_beam(0, x1, x2, x3, p1, p2, p3, q)
+    L.ora_beam_push(o.h, 0)
+    qbm = -1.0 / 2.0
+    kick = qbm * dt * np.array([ex - by, ey + bx, ez])
+    np1, np2, np3 = p1 + kick[0], p2 + kick[1], p3 + kick[2]
+    gam = np.sqrt(1 + np1 ** 2 + np2 ** 2 + np3 ** 2)
+    nx1, nx2 = x1 + np1 * dt / gam, x2 + np2 * dt / gam
+    nx3 = x3 + dt - np3 * dt / gam
+    keep = (np.hypot(nx1, nx2) < nr * dr) & (nx3 < nxi * dxi)
+    assert keep.tolist() == [True, True, True, False, False]
+    got = o.beam(0)
+    assert sorted(got["id"].tolist()) == sorted(np.nonzero(keep)[0].tolist()), ("removed", got["id"])
+    for k, want in (("p1", np1), ("p2", np2), ("p3", np3), ("x1", nx1), ("x2", nx2), ("x3", nx3)):
+        g = got[k][np.argsort(got["id"])]
+        w = want[keep]
+        assert np.max(np.abs(g - w)) <= 1e-12 * max(1.0, np.max(np.abs(w))), (k, g, w)
+
+
+def pin_inject(L):
+    nr, dr, M = 20, 0.1, 2
+    for qm, dens, ppc1, ppc2, nth in ((-1.0, 1.0, 3, 2, 8), (1.0, 2.5, 2, 1, 6)):
+        o = po.Oracle(po.make_grid(nr, 4, M, dr, 0.05), L)
+        o.add_species(po.species_cfg(ppc1, ppc2, nth, qm=qm, density=dens))
+        s = o.species(0)
+        assert len(s["x1"]) == nr * ppc1 * ppc2 * nth
+        assert np.all(s["p1"] == 0) and np.all(s["p3"] == 0) and np.all(s["gamma"] == 1) and np.all(s["psi"] == 0)
+        q = o.new_field(1)
+        L.ora_qdeposit(o.h, o.species_raw(0), po.ptr(q))
+        rho = math.copysign(dens, qm)
+        assert np.max(np.abs(q[0, 0, 2:nr + 1] - rho)) <= 1e-12 * dens, ("uniform density", q[0, 0, 2:nr + 1])
+        assert np.max(np.abs(q[0, 1:, :])) <= 1e-12 * dens, "m > 0 content of a uniform plasma"
+        total = np.sum(s["q"]) * 2 * math.pi * dr * dr
+        assert abs(total - rho * math.pi * (nr * dr) ** 2) <= 1e-12 * abs(total), ("charge per length", total)
+
+
+def pin_update_bound(L):
+    """r >= nr dr (= 1.0 here) leaves; the vacated slot takes the last
+    particle (swap-with-last), worked by hand:
+      r [1.2, 0.3, 1.0, 0.5, 1.5] (0, 2, 4 out) -> ids [3, 1]
+      r [0.3, 0.4, 1.2, 0.5]      (2 out)       -> ids [0, 1, 3]"""
+    nr, dr = 10, 0.1
+    o = po.Oracle(po.make_grid(nr, 4, 1, dr, 0.05), L)
+    o.add_species(po.species_cfg(1, 1, 4))
+    for r, want in (([1.2, 0.3, 1.0, 0.5, 1.5], [3, 1]), ([0.3, 0.4, 1.2, 0.5], [0, 1, 3])):
+        n = len(r)
+        arrs = {"x1": np.array(r), "x2": np.zeros(n), "p1": np.zeros(n), "p2": np.zeros(n),
+                "p3": np.zeros(n), "q": np.arange(float(n)), "gamma": np.ones(n), "psi": np.zeros(n),
+                "id": np.arange(n)}
+        o.set_species(0, arrs)
+        L.ora_update_bound(o.h, o.species_raw(0))
+        s = o.species(0)
+        assert s["id"].tolist() == want, (r, s["id"].tolist())
+        assert s["q"].tolist() == [float(i) for i in want]
+
+
+def pin_beam_inject(L):
+    """Gaussian beam from the generator (fdist3d_rnd_class.f03:360-513):
+    sample moments within five standard errors of (center, sigma), every
+    particle inside the truncation ranges, p_z from gamma and the thermal
+    spread: sqrt((uth_z g + gamma)^2 - p_perp^2 - 1); quiet start mirrors
+    (-x, -y, -p_perp) with half charges."""
+    nr, nxi, dr, dxi = 64, 64, 0.05, 0.05
+    o = po.Oracle(po.make_grid(nr, nxi, 1, dr, dxi), L)
+    n = 1 << 15
+    cen, sig = (0.2, -0.1, 1.6), (0.3, 0.25, 0.4)
+    lo, hi = (-0.9, -0.8, 0.5), (1.3, 0.6, 2.9)
+    o.add_beam(po.beam_cfg(n, 2.0, cen, sig, lo, hi, gamma=1000.0, uth=(2.0, 1.0, 0.0), quiet=True))
+    bm = o.beam(0)
+    half = len(bm["x1"]) // 2
+    x = [bm["x1"][:half], bm["x2"][:half], bm["x3"][:half]]
+    for d in range(3):
+        assert np.all(x[d] >= lo[d]) and np.all(x[d] < hi[d]), f"range {d}"
+        # truncated Gaussian: compare with the moments of the truncated law
+        a, b = (lo[d] - cen[d]) / sig[d], (hi[d] - cen[d]) / sig[d]
+        mu, var = truncnorm.stats(a, b, loc=cen[d], scale=sig[d], moments="mv")
+        se = math.sqrt(var / half)
+        assert abs(np.mean(x[d]) - mu) < 5 * se, ("mean", d, np.mean(x[d]), mu)
+        assert abs(np.var(x[d]) / var - 1.0) < 5 * math.sqrt(2.0 / half), ("variance", d)
+    p1, p2, p3 = bm["p1"][:half], bm["p2"][:half], bm["p3"][:half]
+    assert abs(np.std(p1) / 2.0 - 1) < 0.05 and abs(np.std(p2) / 1.0 - 1) < 0.05
+    assert np.max(np.abs(p3 - np.sqrt(1000.0 ** 2 - p1 ** 2 - p2 ** 2 - 1.0))) <= 1e-9
+    for k, sgn in (("x1", -1), ("x2", -1), ("x3", 1), ("p1", -1), ("p2", -1), ("p3", 1)):
+        assert np.array_equal(bm[k][half:], sgn * bm[k][:half]), f"quiet-start mirror {k}"
+    q_full = -2.0 / (2 * math.pi * n * dr * dr * dxi)
+    assert np.allclose(bm["q"], 0.5 * q_full, rtol=1e-14), "quiet start halves the charge"
+
+
+PINS = {
+    "matrices": pin_matrices,
+    "ez": pin_ez,
+    "bz": pin_bz,
+    "eperp": pin_eperp,
+    "djdxi": pin_djdxi,
+    "bt_iter": pin_bt_iter,
+    "bt_beam": pin_bt_beam,
+    "amjdeposit": pin_amjdeposit,
+    "beam_push": pin_beam_push,
+    "beam_inject": pin_beam_inject,
+    "inject": pin_inject,
+    "update_bound": pin_update_bound,
+}
+
