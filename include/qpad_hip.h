@@ -102,6 +102,45 @@ typedef struct {
     uint64_t seed;
 } qpad_beam_cfg;
 
+/* beam(i) section for profile_type "standard" (the default): the lattice
+   beam of fdist3d_std_class.f03:150-436 (settings) and :439-613
+   (injection).  Per direction a density profile (fdist3d_std_lib.f03:
+   uniform, gaussian exp(-(x - c)^2 / (2 s^2)), piecewise-linear in pw_x /
+   pw_fx), applied to x, y, xi (geometry cartesian) or r, theta, xi
+   (cylindrical).  Every xi position (gauss_center[2], pw_x[2], range, the
+   perp_offset origin) is relative to the box start, as QPAD stores them.
+   alpha != 0 tilts the transverse phase space (Twiss; twiss_beta = gamma
+   sigma / uth as QPAD computes it); perp_offset adds sum_k c_k (xi -
+   xi0)^k to x (index 0) or y (index 1).  Thermal momenta draw from
+   Philox4x32-10 (counter = particle index) instead of gfortran's
+   random_number. */
+#define QPAD_PROF_UNIFORM 0
+#define QPAD_PROF_GAUSSIAN 1
+#define QPAD_PROF_PIECEWISE 2
+#define QPAD_GEOM_CARTESIAN 0
+#define QPAD_GEOM_CYLINDRICAL 1
+#define QPAD_PW_MAX 32
+#define QPAD_OFFSET_MAX 8
+typedef struct {
+    int32_t ppc[3];
+    int32_t num_theta;
+    int32_t geometry;
+    int32_t profile[3];
+    double qm, mass, density, gamma, den_min;
+    double gauss_center[3], gauss_sigma[3];
+    int32_t pw_n[3];
+    double pw_x[3][QPAD_PW_MAX], pw_fx[3][QPAD_PW_MAX];
+    double range_lo[3], range_hi[3];
+    double uth[3];
+    double alpha[2], twiss_ctr[2], twiss_beta[2];
+    int32_t offset_n[2];
+    double offset_xi0[2];
+    double offset_c[2][QPAD_OFFSET_MAX];
+    int32_t quiet;
+    int32_t evol;
+    uint64_t seed;
+} qpad_beam_std_cfg;
+
 typedef struct qpad_ctx qpad_ctx;
 
 const char *qpad_last_error(void);
@@ -173,6 +212,10 @@ int qpad_beam_push(qpad_ctx *ctx);
 /* beam3d%new with a random Gaussian profile (beam3d_class.f03:121-177,
    fdist3d_rnd_class.f03:360-513); draws from Philox4x32-10 */
 int qpad_beam_new(qpad_ctx *ctx, const qpad_beam_cfg *cfg, int32_t *beam_id);
+/* beam3d%new with the lattice profile ("standard", every shipped QPAD deck):
+   beam3d_class.f03:121-177, fdist3d_std_class.f03:439-613; the beam holds
+   exactly the injected particles (QPAD's npmax is not needed) */
+int qpad_beam_new_std(qpad_ctx *ctx, const qpad_beam_std_cfg *cfg, int32_t *beam_id);
 
 /* The fused forms of simulation%run (simulation_class.f03:294-512):
    qpad_step3d_begin = beam deposit, beam B for every slice, field zeroing

@@ -50,6 +50,20 @@ class BeamCfg(C.Structure):
                 ("evol", C.c_int), ("seed", C.c_uint64)]
 
 
+class BeamStdCfg(C.Structure):
+    _fields_ = [("ppc", C.c_int * 3), ("num_theta", C.c_int), ("geometry", C.c_int),
+                ("profile", C.c_int * 3), ("qm", C.c_double), ("mass", C.c_double),
+                ("density", C.c_double), ("gamma", C.c_double), ("den_min", C.c_double),
+                ("gauss_center", C.c_double * 3), ("gauss_sigma", C.c_double * 3),
+                ("pw_n", C.c_int * 3), ("pw_x", (C.c_double * 32) * 3),
+                ("pw_fx", (C.c_double * 32) * 3), ("range_lo", C.c_double * 3),
+                ("range_hi", C.c_double * 3), ("uth", C.c_double * 3), ("alpha", C.c_double * 2),
+                ("twiss_ctr", C.c_double * 2), ("twiss_beta", C.c_double * 2),
+                ("offset_n", C.c_int * 2), ("offset_xi0", C.c_double * 2),
+                ("offset_c", (C.c_double * 8) * 2), ("quiet", C.c_int), ("evol", C.c_int),
+                ("seed", C.c_uint64)]
+
+
 class _Species(C.Structure):
     _fields_ = [("npp", C.c_int64), ("npmax", C.c_int64),
                 ("x1", _dp), ("x2", _dp), ("p1", _dp), ("p2", _dp), ("p3", _dp),
@@ -94,6 +108,7 @@ def load_library(path):
     L.ora_destroy.argtypes = [C.c_void_p]
     L.ora_add_species.argtypes = [C.c_void_p, C.POINTER(SpeciesCfg)]
     L.ora_add_beam.argtypes = [C.c_void_p, C.POINTER(BeamCfg), C.c_int64]
+    L.ora_add_beam_std.argtypes = [C.c_void_p, C.POINTER(BeamStdCfg)]
     L.ora_beam_set.argtypes = [C.c_void_p, C.c_int, C.c_int64] + [_dp] * 7
     L.ora_step3d_begin.argtypes = [C.c_void_p]
     L.ora_slice.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_int)]
@@ -222,6 +237,11 @@ class Oracle:
 
     def add_beam(self, cfg, npmax=0):
         r = self.L.ora_add_beam(self.h, C.byref(cfg), npmax)
+        self.nbeams += 1
+        return r
+
+    def add_beam_std(self, cfg):
+        r = self.L.ora_add_beam_std(self.h, C.byref(cfg))
         self.nbeams += 1
         return r
 

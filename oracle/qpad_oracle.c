@@ -1014,6 +1014,145 @@ void ora_beam_inject(ora_sim *s, int ib, const ora_beam_cfg *c)
     bm->npp = n;
 }
 
+/* density profiles of fdist3d_std_lib.f03: get_den_uniform (:33-39),
+   get_den_gaussian (:114-127), get_den_pw_linear (:332-363) */
+static double std_profile(const ora_beam_std_cfg *c, int d, double x)
+{
+    if (c->profile[d] == 1) {
+        double mu = c->gauss_center[d], sigma = c->gauss_sigma[d];
+        double t = (x - mu) / sigma;
+        return exp(-0.5 * (t * t));
+    }
+    if (c->profile[d] == 2) {
+        int n = c->pw_n[d];
+        const double *xa = c->pw_x[d], *fa = c->pw_fx[d];
+        if (x < xa[0] || x > xa[n - 1]) return 0.0;
+        for (int i = 1; i < n; i++)
+            if (x <= xa[i]) return fa[i - 1] + (fa[i] - fa[i - 1]) / (xa[i] - xa[i - 1]) * (x - xa[i - 1]);
+        return 0.0;
+    }
+    return 1.0;
+}
+
+/* inject_fdist3d_std: source/beam/fdist3d_std_class.f03:439-613 (one radial
+   and longitudinal partition).  Pass 0 counts the kept lattice points, pass
+   1 stores them; momenta draw from Philox (counter = kept index) instead of
+   gfortran's rand_norm. */
+static int64_t std_lattice(ora_sim *s, const ora_beam_std_cfg *c, ora_beam *bm, int store)
+{
+    int nrp = s->g.nr, nzp = s->g.nxi, nth = c->num_theta;
+    double dr = s->g.dr, dz = s->g.dxi;
+    double dtheta = 2.0 * PI_DB / (double)nth;
+    int ppc_tot = c->ppc[0] * c->ppc[1] * c->ppc[2];
+    double coef = copysign(1.0, c->qm) / ((double)ppc_tot * (double)nth);
+    int64_t ip = 0;
+    for (int k = 1; k <= nzp; k++)
+        for (int j = 1; j <= nth; j++)
+            for (int i = 1; i <= nrp; i++)
+                for (int i3 = 1; i3 <= c->ppc[2]; i3++) {
+                    double zn = ((double)i3 - 0.5) / (double)c->ppc[2] + (double)(k - 1);
+                    for (int i2 = 1; i2 <= c->ppc[1]; i2++) {
+                        double theta = (((double)i2 - 0.5) / (double)c->ppc[1] + (double)j - 1.0) * dtheta;
+                        for (int i1 = 1; i1 <= c->ppc[0]; i1++) {
+                            double rn = ((double)i1 - 0.5) / (double)c->ppc[0] + (double)(i - 1);
+                            double xt[3];
+                            if (c->geometry == 1) {
+                                xt[0] = rn * dr;
+                                xt[1] = theta;
+                            } else {
+                                xt[0] = rn * dr * cos(theta);
+                                xt[1] = rn * dr * sin(theta);
+                            }
+                            xt[2] = zn * dz;
+                            int out = 0;
+                            for (int d = 0; d < 3; d++)
+                                if (xt[d] < c->range_lo[d] || xt[d] > c->range_hi[d]) out = 1;
+                            if (out) continue;
+                            double dv[3];
+                            for (int d = 0; d < 3; d++) dv[d] = std_profile(c, d, xt[d]);
+                            double den_loc = dv[0] * dv[1] * dv[2] * c->density;
+                            if (den_loc < c->den_min) continue;
+                            if (store) {
+                                if (c->geometry == 1) {
+                                    bm->x1[ip] = xt[0] * cos(xt[1]);
+                                    bm->x2[ip] = xt[0] * sin(xt[1]);
+                                } else {
+                                    bm->x1[ip] = xt[0];
+                                    bm->x2[ip] = xt[1];
+                                }
+                                bm->x3[ip] = xt[2];
+                                double g[4];
+                                ora_normal4(c->seed, (uint64_t)ip, 0x80000000u, g);
+                                bm->p1[ip] = c->uth[0] * g[0];
+                                bm->p2[ip] = c->uth[1] * g[1];
+                                bm->p3[ip] = c->uth[2] * g[2] + c->gamma;
+                                bm->p3[ip] = sqrt(bm->p3[ip] * bm->p3[ip] - bm->p1[ip] * bm->p1[ip]
+                                                  - bm->p2[ip] * bm->p2[ip] - 1.0);
+                                bm->q[ip] = rn * den_loc * coef;
+                                bm->id[ip] = (int32_t)ip;
+                            }
+                            ip++;
+                        }
+                    }
+                }
+    return ip;
+}
+
+int ora_add_beam_std(ora_sim *s, const ora_beam_std_cfg *c)
+{
+    if (s->nbeams >= 8) return -1;
+    int64_t n = std_lattice(s, c, NULL, 0);
+    int ib = s->nbeams++;
+    ora_beam *bm = &s->bm[ib];
+    memset(bm, 0, sizeof(*bm));
+    int64_t npmax = n * (c->quiet ? 2 : 1);
+    if (npmax < 1) npmax = 1;
+    bm->npmax = npmax;
+    bm->x1 = calloc(npmax, sizeof(double)); bm->x2 = calloc(npmax, sizeof(double));
+    bm->x3 = calloc(npmax, sizeof(double)); bm->p1 = calloc(npmax, sizeof(double));
+    bm->p2 = calloc(npmax, sizeof(double)); bm->p3 = calloc(npmax, sizeof(double));
+    bm->q = calloc(npmax, sizeof(double)); bm->id = calloc(npmax, sizeof(int32_t));
+    bm->qbm = c->qm / c->mass;
+    bm->dt = s->g.dt;
+    bm->evol = c->evol;
+    bm->q_own_hist = calloc(((size_t)s->g.nxi + 2) * s->nm * s->ld, sizeof(double));
+    std_lattice(s, c, bm, 1);
+    /* quiet start (:554-579) */
+    if (c->quiet) {
+        for (int64_t i = 0; i < n; i++) {
+            bm->x1[n + i] = -bm->x1[i]; bm->x2[n + i] = -bm->x2[i]; bm->x3[n + i] = bm->x3[i];
+            bm->p1[n + i] = -bm->p1[i]; bm->p2[n + i] = -bm->p2[i]; bm->p3[n + i] = bm->p3[i];
+            bm->q[i] = 0.5 * bm->q[i];
+            bm->q[n + i] = bm->q[i];
+            bm->id[n + i] = bm->id[i] + (int32_t)n;
+        }
+        n *= 2;
+    }
+    /* Twiss tilt (:581-592) */
+    for (int d = 0; d < 2; d++) {
+        if (c->alpha[d] == 0.0) continue;
+        double cf = c->gamma * c->alpha[d] / c->twiss_beta[d];
+        double *x = d == 0 ? bm->x1 : bm->x2, *p = d == 0 ? bm->p1 : bm->p2;
+        for (int64_t i = 0; i < n; i++) p[i] = p[i] - cf * (x[i] - c->twiss_ctr[d]);
+    }
+    /* xi-dependent transverse offset, eval_polynomial (math_module.f03:
+       134-149) (:594-608) */
+    for (int d = 0; d < 2; d++) {
+        if (c->offset_n[d] <= 0) continue;
+        double *x = d == 0 ? bm->x1 : bm->x2;
+        for (int64_t i = 0; i < n; i++) {
+            double t = bm->x3[i] - c->offset_xi0[d], y = 0.0, tp = 1.0;
+            for (int k = 0; k < c->offset_n[d]; k++) {
+                y = y + c->offset_c[d][k] * tp;
+                tp = tp * t;
+            }
+            x[i] = x[i] + y;
+        }
+    }
+    bm->npp = n;
+    return ib;
+}
+
 /* qdeposit_part3d: source/beam/part3d_class.f03:221-356 into the beam's f2 */
 void ora_beam_qdeposit(ora_sim *s, int ib, double *h)
 {

@@ -86,6 +86,30 @@ typedef struct {
     uint64_t seed;
 } ora_beam_cfg;
 
+/* beam(i) section of profile_type "standard" (fdist3d_std_class.f03:
+   150-436): profiles per direction 0 uniform, 1 gaussian, 2 piecewise-
+   linear; geometry 0 cartesian, 1 cylindrical; xi positions relative to the
+   box start.  Same layout as the library's qpad_beam_std_cfg. */
+typedef struct {
+    int ppc[3];
+    int num_theta;
+    int geometry;
+    int profile[3];
+    double qm, mass, density, gamma, den_min;
+    double gauss_center[3], gauss_sigma[3];
+    int pw_n[3];
+    double pw_x[3][32], pw_fx[3][32];
+    double range_lo[3], range_hi[3];
+    double uth[3];
+    double alpha[2], twiss_ctr[2], twiss_beta[2];
+    int offset_n[2];
+    double offset_xi0[2];
+    double offset_c[2][8];
+    int quiet;
+    int evol;
+    uint64_t seed;
+} ora_beam_std_cfg;
+
 typedef struct {
     int64_t npp, npmax;
     double *x1, *x2, *p1, *p2, *p3, *q, *gamma, *psi;
@@ -165,6 +189,7 @@ ora_sim *ora_create(const ora_grid *g);
 void ora_destroy(ora_sim *s);
 int ora_add_species(ora_sim *s, const ora_species_cfg *cfg);
 int ora_add_beam(ora_sim *s, const ora_beam_cfg *cfg, int64_t npmax);
+int ora_add_beam_std(ora_sim *s, const ora_beam_std_cfg *cfg);
 int ora_beam_set(ora_sim *s, int ib, int64_t n, const double *x1,
                  const double *x2, const double *x3, const double *p1,
                  const double *p2, const double *p3, const double *q);

@@ -69,6 +69,25 @@ class BeamCfg(C.Structure):
                 ("evol", C.c_int32), ("seed", C.c_uint64)]
 
 
+PROF_UNIFORM, PROF_GAUSSIAN, PROF_PIECEWISE = 0, 1, 2
+GEOM_CARTESIAN, GEOM_CYLINDRICAL = 0, 1
+PW_MAX, OFFSET_MAX = 32, 8
+
+
+class BeamStdCfg(C.Structure):
+    _fields_ = [("ppc", C.c_int32 * 3), ("num_theta", C.c_int32), ("geometry", C.c_int32),
+                ("profile", C.c_int32 * 3), ("qm", C.c_double), ("mass", C.c_double),
+                ("density", C.c_double), ("gamma", C.c_double), ("den_min", C.c_double),
+                ("gauss_center", C.c_double * 3), ("gauss_sigma", C.c_double * 3),
+                ("pw_n", C.c_int32 * 3), ("pw_x", (C.c_double * PW_MAX) * 3),
+                ("pw_fx", (C.c_double * PW_MAX) * 3), ("range_lo", C.c_double * 3),
+                ("range_hi", C.c_double * 3), ("uth", C.c_double * 3), ("alpha", C.c_double * 2),
+                ("twiss_ctr", C.c_double * 2), ("twiss_beta", C.c_double * 2),
+                ("offset_n", C.c_int32 * 2), ("offset_xi0", C.c_double * 2),
+                ("offset_c", (C.c_double * OFFSET_MAX) * 2), ("quiet", C.c_int32), ("evol", C.c_int32),
+                ("seed", C.c_uint64)]
+
+
 # name -> (restype, argtypes); every entry point returns int status
 _CTX = C.c_void_p
 SIGNATURES = {
@@ -94,6 +113,7 @@ SIGNATURES = {
     "qpad_beam_qdp": (C.c_int, [_CTX]),
     "qpad_beam_push": (C.c_int, [_CTX]),
     "qpad_beam_new": (C.c_int, [_CTX, C.POINTER(BeamCfg), C.POINTER(C.c_int32)]),
+    "qpad_beam_new_std": (C.c_int, [_CTX, C.POINTER(BeamStdCfg), C.POINTER(C.c_int32)]),
     "qpad_step3d_begin": (C.c_int, [_CTX]),
     "qpad_slice": (C.c_int, [_CTX, C.c_int32, C.POINTER(C.c_int32)]),
     "qpad_step3d_end": (C.c_int, [_CTX]),

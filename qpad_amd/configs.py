@@ -113,6 +113,36 @@ def small(nr=64, nxi=48, max_mode=1, ppc=(2, 4), total_num=1 << 14, rmax=4.0, dx
                  _beam(nxi * dxi, sigma_r, sigma_xi, n_peak, total_num, offset_x=offset_x), nstep=1)
 
 
+def tri_gaussian():
+    """QPAD's own blowout-regime example, restated as data
+    (input_file/blowout_regime/qpinput_tri-gaussian.json, diagnostics and
+    restart settings left out): Nr 250, r_max 5, Nxi 500 over [-5, 5], modes
+    0..1, electrons at ppc 2 x 2 x 16, a lattice ("standard") tri-Gaussian
+    beam with quiet start, n_b/n_p = 4, sigma (0.25, 0.25, 0.5), one 3D step
+    of dt 10."""
+    return {
+        "simulation": {
+            "nodes": [1, 1], "grid": [250, 500], "box": {"r": [0.0, 5.0], "z": [-5.0, 5.0]},
+            "field_boundary": "open", "max_mode": 1, "interpolation": "linear", "n0": 1.0e16,
+            "time": 10.1, "dt": 10.0, "nbeams": 1, "nspecies": 1, "nneutrals": 0, "nlasers": 0,
+            "iter_max": 10, "iter_reltol": 1.0e-3, "iter_abstol": 1.0e-3, "random_seed": 10,
+            "algorithm": "standard",
+        },
+        "beam": [{
+            "geometry": "cartesian", "profile": ["gaussian", "gaussian", "gaussian"], "evolution": True,
+            "push_type": "reduced", "has_spin": False, "ppc": [2, 2, 2], "num_theta": 16,
+            "npmax": 1000000, "q": -1.0, "m": 1.0, "gamma": 20000, "density": 4.0, "quiet_start": True,
+            "gauss_center": [0.0, 0.0, -2.5], "gauss_sigma": [0.25, 0.25, 0.5],
+            "range1": [-1.25, 1.25], "range2": [-1.25, 1.25], "range3": [-5.0, 0.0],
+            "uth": [5.0, 5.0, 0.0], "den_min": 1e-10,
+        }],
+        "species": [{
+            "profile": ["uniform", "uniform"], "ppc": [2, 2], "num_theta": 16, "q": -1.0, "m": 1.0,
+            "density": 1.0, "push_type": "robust",
+        }],
+    }
+
+
 def with_overrides(deck, **sim):
     d = copy.deepcopy(deck)
     d["simulation"].update(sim)

@@ -163,6 +163,234 @@ int launch_beam_inject(qpad_ctx *c, int ib)
 }
 
 // ---------------------------------------------------------------------------
+// lattice injection: fdist3d_std_class.f03:439-613.  QPAD walks the lattice
+// k (xi cell) > j (theta sector) > i (r cell) > i3 > i2 > i1 and keeps the
+// points inside the ranges whose density reaches den_min; the kept points
+// are numbered in that order.  Here the walk is restricted to the cells the
+// ranges allow (a box of the same order), one thread per lattice point: a
+// counting pass flags the kept points, an exclusive scan numbers them, and
+// the writing pass stores each one at its number.
+// ---------------------------------------------------------------------------
+struct LatticeArgs {
+    qpad_beam_std_cfg cf;
+    int64_t n;              // lattice points in the box
+    int k0, ni;             // xi cells k0 .. (1-based), r cells 1 .. ni
+    double dr, dz, dtheta, coef;
+    uint32_t *flag;
+    const uint32_t *pos;
+    double *x1, *x2, *x3, *p1, *p2, *p3, *q;
+};
+
+// density profile of direction d at coordinate x (fdist3d_std_lib.f03)
+__device__ double lattice_profile(const qpad_beam_std_cfg &c, int d, double x)
+{
+    if (c.profile[d] == QPAD_PROF_GAUSSIAN) {
+        const double t = (x - c.gauss_center[d]) / c.gauss_sigma[d];
+        return exp(-0.5 * (t * t));
+    }
+    if (c.profile[d] == QPAD_PROF_PIECEWISE) {
+        const int n = c.pw_n[d];
+        const double *xs = c.pw_x[d], *fx = c.pw_fx[d];
+        if (x < xs[0] || x > xs[n - 1]) return 0.0;
+        for (int i = 1; i < n; i++)
+            if (x <= xs[i]) return fx[i - 1] + (fx[i] - fx[i - 1]) / (xs[i] - xs[i - 1]) * (x - xs[i - 1]);
+        return 0.0;
+    }
+    return 1.0;
+}
+
+// lattice point L of the box: position (geometry coordinates), rn, density;
+// false when QPAD's loop skips it
+__device__ bool lattice_point(const LatticeArgs &a, int64_t L, double xt[3], double *rn, double *den)
+{
+    const qpad_beam_std_cfg &c = a.cf;
+    const int i1 = (int)(L % c.ppc[0]) + 1;
+    L /= c.ppc[0];
+    const int i2 = (int)(L % c.ppc[1]) + 1;
+    L /= c.ppc[1];
+    const int i3 = (int)(L % c.ppc[2]) + 1;
+    L /= c.ppc[2];
+    const int i = (int)(L % a.ni) + 1;
+    L /= a.ni;
+    const int j = (int)(L % c.num_theta) + 1;
+    const int k = (int)(L / c.num_theta) + a.k0;
+    const double zn = ((double)i3 - 0.5) / (double)c.ppc[2] + (double)(k - 1);
+    const double theta = (((double)i2 - 0.5) / (double)c.ppc[1] + (double)j - 1.0) * a.dtheta;
+    *rn = ((double)i1 - 0.5) / (double)c.ppc[0] + (double)(i - 1);
+    if (c.geometry == QPAD_GEOM_CYLINDRICAL) {
+        xt[0] = *rn * a.dr;
+        xt[1] = theta;
+    } else {
+        xt[0] = *rn * a.dr * cos(theta);
+        xt[1] = *rn * a.dr * sin(theta);
+    }
+    xt[2] = zn * a.dz;
+    for (int d = 0; d < 3; d++)
+        if (xt[d] < c.range_lo[d] || xt[d] > c.range_hi[d]) return false;
+    double dv[3];
+    for (int d = 0; d < 3; d++) dv[d] = lattice_profile(c, d, xt[d]);
+    *den = dv[0] * dv[1] * dv[2] * c.density;
+    return !(*den < c.den_min);
+}
+
+__global__ void k_lattice_flag(LatticeArgs a)
+{
+    const int64_t L = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (L >= a.n) return;
+    double xt[3], rn, den;
+    a.flag[L] = lattice_point(a, L, xt, &rn, &den) ? 1u : 0u;
+}
+
+__global__ void k_lattice_write(LatticeArgs a)
+{
+    const int64_t L = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (L >= a.n || !a.flag[L]) return;
+    double xt[3], rn, den;
+    lattice_point(a, L, xt, &rn, &den);
+    const uint32_t ip = a.pos[L];
+    const qpad_beam_std_cfg &c = a.cf;
+    if (c.geometry == QPAD_GEOM_CYLINDRICAL) {
+        a.x1[ip] = xt[0] * cos(xt[1]);
+        a.x2[ip] = xt[0] * sin(xt[1]);
+    } else {
+        a.x1[ip] = xt[0];
+        a.x2[ip] = xt[1];
+    }
+    a.x3[ip] = xt[2];
+    double g[4];
+    normal4(c.seed, (uint64_t)ip, 0x80000000u, g);
+    const double p1 = c.uth[0] * g[0], p2 = c.uth[1] * g[1];
+    double p3 = c.uth[2] * g[2] + c.gamma;
+    p3 = sqrt(p3 * p3 - p1 * p1 - p2 * p2 - 1.0);
+    a.p1[ip] = p1;
+    a.p2[ip] = p2;
+    a.p3[ip] = p3;
+    a.q[ip] = rn * den * a.coef;
+}
+
+// Twiss tilt of the transverse phase space, then the xi-dependent offset
+// (fdist3d_std_class.f03:580-608), over every particle including the
+// quiet-start mirrors
+__global__ void k_lattice_shape(qpad_beam_std_cfg c, int64_t n, double *x1, double *x2, const double *x3,
+                                double *p1, double *p2)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    if (c.alpha[0] != 0.0) p1[i] = p1[i] - c.gamma * c.alpha[0] / c.twiss_beta[0] * (x1[i] - c.twiss_ctr[0]);
+    if (c.alpha[1] != 0.0) p2[i] = p2[i] - c.gamma * c.alpha[1] / c.twiss_beta[1] * (x2[i] - c.twiss_ctr[1]);
+    double *xs[2] = {x1, x2};
+    for (int d = 0; d < 2; d++) {
+        if (c.offset_n[d] <= 0) continue;
+        const double t = x3[i] - c.offset_xi0[d];
+        double y = 0.0, tp = 1.0;
+        for (int k = 0; k < c.offset_n[d]; k++) {
+            y = y + c.offset_c[d][k] * tp;
+            tp = tp * t;
+        }
+        xs[d][i] = xs[d][i] + y;
+    }
+}
+
+static int lattice_args(qpad_ctx *c, const qpad_beam_std_cfg &cf, LatticeArgs *a)
+{
+    memset((void *)a, 0, sizeof(*a));
+    a->cf = cf;
+    a->dr = c->g.dr;
+    a->dz = c->g.dxi;
+    const double pi = 4.0 * atan(1.0);
+    a->dtheta = 2.0 * pi / (double)cf.num_theta;
+    a->coef = copysign(1.0, cf.qm) / ((double)(cf.ppc[0] * cf.ppc[1] * cf.ppc[2]) * (double)cf.num_theta);
+    // the radial cells the ranges allow: r <= rmax
+    double rmax;
+    if (cf.geometry == QPAD_GEOM_CYLINDRICAL) {
+        rmax = cf.range_hi[0];
+    } else {
+        const double mx = fmax(fabs(cf.range_lo[0]), fabs(cf.range_hi[0]));
+        const double my = fmax(fabs(cf.range_lo[1]), fabs(cf.range_hi[1]));
+        rmax = sqrt(mx * mx + my * my);
+    }
+    const double imax = floor(rmax / c->g.dr) + 2.0;
+    a->ni = (int)fmin((double)c->g.nr, fmax(imax, 0.0));
+    const double k0 = floor(cf.range_lo[2] / c->g.dxi) - 1.0, k1 = floor(cf.range_hi[2] / c->g.dxi) + 2.0;
+    const int kk0 = (int)fmax(1.0, fmin(k0, (double)c->g.nxi + 1.0));
+    const int kk1 = (int)fmax(0.0, fmin(k1, (double)c->g.nxi));
+    a->k0 = kk0;
+    const int64_t nk = kk1 >= kk0 ? (int64_t)(kk1 - kk0 + 1) : 0;
+    a->n = nk * cf.num_theta * a->ni * cf.ppc[0] * cf.ppc[1] * cf.ppc[2];
+    if (a->ni <= 0) a->n = 0;
+    return 0;
+}
+
+int lattice_count(qpad_ctx *c, const qpad_beam_std_cfg &cf, LatticePlan *pl)
+{
+    memset((void *)pl, 0, sizeof(*pl));
+    LatticeArgs a;
+    QPAD_TRY(lattice_args(c, cf, &a));
+    pl->n = a.n;
+    if (a.n == 0) return 0;
+    if (a.n >= ((int64_t)1 << 31)) {
+        set_error("lattice beam: more than 2^31 lattice points inside the ranges");
+        return QPAD_E_INVALID;
+    }
+    QPAD_HIP_CHECK(hipMalloc((void **)&pl->flag, sizeof(uint32_t) * (size_t)a.n));
+    QPAD_HIP_CHECK(hipMalloc((void **)&pl->pos, sizeof(uint32_t) * (size_t)a.n));
+    size_t bytes = 0;
+    QPAD_HIP_CHECK(rocprim::exclusive_scan(nullptr, bytes, pl->flag, pl->pos, 0u, (size_t)a.n,
+                                           rocprim::plus<uint32_t>()));
+    pl->tmp_bytes = bytes + 256;
+    QPAD_HIP_CHECK(hipMalloc(&pl->tmp, pl->tmp_bytes));
+    a.flag = pl->flag;
+    const unsigned nb = (unsigned)((a.n + 255) / 256);
+    hipLaunchKernelGGL(k_lattice_flag, dim3(nb), dim3(256), 0, c->stream, a);
+    QPAD_HIP_CHECK(hipGetLastError());
+    bytes = pl->tmp_bytes;
+    QPAD_HIP_CHECK(rocprim::exclusive_scan(pl->tmp, bytes, pl->flag, pl->pos, 0u, (size_t)a.n,
+                                           rocprim::plus<uint32_t>(), c->stream));
+    uint32_t last_pos = 0, last_flag = 0;
+    QPAD_HIP_CHECK(hipMemcpyAsync(&last_pos, pl->pos + (a.n - 1), sizeof(uint32_t), hipMemcpyDeviceToHost,
+                                  c->stream));
+    QPAD_HIP_CHECK(hipMemcpyAsync(&last_flag, pl->flag + (a.n - 1), sizeof(uint32_t), hipMemcpyDeviceToHost,
+                                  c->stream));
+    QPAD_HIP_CHECK(hipStreamSynchronize(c->stream));
+    pl->kept = (int64_t)last_pos + last_flag;
+    return 0;
+}
+
+int lattice_write(qpad_ctx *c, const qpad_beam_std_cfg &cf, const LatticePlan &pl, Beam &bm)
+{
+    int64_t nk = pl.kept;
+    if (nk > 0) {
+        LatticeArgs a;
+        QPAD_TRY(lattice_args(c, cf, &a));
+        a.flag = pl.flag;
+        a.pos = pl.pos;
+        a.x1 = bm.x1; a.x2 = bm.x2; a.x3 = bm.x3;
+        a.p1 = bm.p1; a.p2 = bm.p2; a.p3 = bm.p3; a.q = bm.q;
+        hipLaunchKernelGGL(k_lattice_write, dim3((unsigned)((a.n + 255) / 256)), dim3(256), 0, c->stream, a);
+        QPAD_HIP_CHECK(hipGetLastError());
+        if (cf.quiet) {
+            hipLaunchKernelGGL(k_beam_quiet, dim3((unsigned)((nk + 255) / 256)), dim3(256), 0, c->stream,
+                               bm.x1, bm.x2, bm.x3, bm.p1, bm.p2, bm.p3, bm.q, nk);
+            QPAD_HIP_CHECK(hipGetLastError());
+            nk *= 2;
+        }
+        hipLaunchKernelGGL(k_lattice_shape, dim3((unsigned)((nk + 255) / 256)), dim3(256), 0, c->stream, cf, nk,
+                           bm.x1, bm.x2, bm.x3, bm.p1, bm.p2);
+        QPAD_HIP_CHECK(hipGetLastError());
+    }
+    bm.npp = nk;
+    return 0;
+}
+
+void lattice_free(LatticePlan *pl)
+{
+    if (pl->flag) (void)hipFree(pl->flag);
+    if (pl->pos) (void)hipFree(pl->pos);
+    if (pl->tmp) (void)hipFree(pl->tmp);
+    memset((void *)pl, 0, sizeof(*pl));
+}
+
+// ---------------------------------------------------------------------------
 // sort by (xi, r) cell: key = mm * nr + nn (0-based cell indices); particles
 // outside the box get key = nr * nxi and fall past cell_start[ncell]
 // ---------------------------------------------------------------------------

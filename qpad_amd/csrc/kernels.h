@@ -104,6 +104,19 @@ int launch_add_qn_into_qspe(qpad_ctx *c, int sp);
 
 // ---- beam (k_beam.hip) ----
 int launch_beam_inject(qpad_ctx *c, int ib);
+// lattice beam (profile "standard"): lattice_count flags and numbers the
+// kept lattice points (temporary device buffers, released by lattice_free);
+// lattice_write stores them into the beam, mirrors them for a quiet start,
+// tilts and offsets them, and sets npp
+struct LatticePlan {
+    int64_t n, kept;         // lattice points in the range box, points kept
+    uint32_t *flag, *pos;
+    void *tmp;
+    size_t tmp_bytes;
+};
+int lattice_count(qpad_ctx *c, const qpad_beam_std_cfg &cf, LatticePlan *pl);
+int lattice_write(qpad_ctx *c, const qpad_beam_std_cfg &cf, const LatticePlan &pl, Beam &bm);
+void lattice_free(LatticePlan *pl);
 int launch_beam_sort(qpad_ctx *c, int ib);
 int launch_beam_keys(qpad_ctx *c, int ib);
 int launch_beam_qdeposit(qpad_ctx *c);           // all beams into h_q_beam

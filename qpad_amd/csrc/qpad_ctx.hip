@@ -492,22 +492,14 @@ int qpad_species_renew(qpad_ctx *c, int32_t s)
     return sync(c);
 }
 
-int qpad_beam_new(qpad_ctx *c, const qpad_beam_cfg *cfg, int32_t *beam_id)
+}  // extern "C"
+
+namespace qpad {
+// the buffers of a beam of npmax particles; on failure everything allocated
+// so far is released by the caller (beam_free)
+static int beam_alloc(qpad_ctx *c, Beam &bm, int64_t npmax)
 {
-    if (!c || !cfg) return fail(QPAD_E_INVALID, "null argument");
-    if (c->nbeams >= kMaxBeams) return fail(QPAD_E_INVALID, "too many beams");
-    if (cfg->mass == 0.0) return fail(QPAD_E_INVALID, "beam mass must be non-zero");
-    if (cfg->total_num < 0 || cfg->total_num > 0x3fffffff) return fail(QPAD_E_INVALID, "bad total_num");
-    for (int d = 0; d < 3; d++)
-        if (!(cfg->sigma[d] > 0.0) || !(cfg->range_hi[d] > cfg->range_lo[d]))
-            return fail(QPAD_E_INVALID, "beam sigma and ranges must be positive");
-    if ((int64_t)c->g.nr * c->g.nxi >= 0x7fffffff) return fail(QPAD_E_INVALID, "grid too large for beam cells");
-    const int b = c->nbeams;
-    Beam &bm = c->bm[b];
-    memset((void *)&bm, 0, sizeof(Beam));
-    bm.cfg = *cfg;
-    bm.qbm = cfg->qm / cfg->mass;
-    bm.npmax = std::max<int64_t>(1, cfg->total_num * (cfg->quiet ? 2 : 1));
+    bm.npmax = std::max<int64_t>(1, npmax);
     const size_t n = (size_t)bm.npmax;
     int rc = 0;
     double **dp[] = {&bm.x1, &bm.x2, &bm.x3, &bm.p1, &bm.p2, &bm.p3, &bm.q,
@@ -528,7 +520,14 @@ int qpad_beam_new(qpad_ctx *c, const qpad_beam_cfg *cfg, int32_t *beam_id)
             if (e != hipSuccess) rc = fail(QPAD_E_NOMEM, "beam sort scratch allocation failed");
         }
     }
-    if (!rc) rc = launch_beam_inject(c, b);
+    return rc;
+}
+
+// sort the injected beam by cell and publish it as beam b; on failure free
+// its buffers and leave the context as it was
+static int beam_finish(qpad_ctx *c, int b, int rc, int32_t *beam_id)
+{
+    Beam &bm = c->bm[b];
     if (!rc) rc = launch_beam_sort(c, b);
     if (!rc) rc = refresh_beam_npp(c, b);
     if (rc) {
@@ -542,6 +541,81 @@ int qpad_beam_new(qpad_ctx *c, const qpad_beam_cfg *cfg, int32_t *beam_id)
     c->nbeams++;
     if (beam_id) *beam_id = b;
     return 0;
+}
+}  // namespace qpad
+
+extern "C" {
+
+int qpad_beam_new(qpad_ctx *c, const qpad_beam_cfg *cfg, int32_t *beam_id)
+{
+    if (!c || !cfg) return fail(QPAD_E_INVALID, "null argument");
+    if (c->nbeams >= kMaxBeams) return fail(QPAD_E_INVALID, "too many beams");
+    if (cfg->mass == 0.0) return fail(QPAD_E_INVALID, "beam mass must be non-zero");
+    if (cfg->total_num < 0 || cfg->total_num > 0x3fffffff) return fail(QPAD_E_INVALID, "bad total_num");
+    for (int d = 0; d < 3; d++)
+        if (!(cfg->sigma[d] > 0.0) || !(cfg->range_hi[d] > cfg->range_lo[d]))
+            return fail(QPAD_E_INVALID, "beam sigma and ranges must be positive");
+    if ((int64_t)c->g.nr * c->g.nxi >= 0x7fffffff) return fail(QPAD_E_INVALID, "grid too large for beam cells");
+    const int b = c->nbeams;
+    Beam &bm = c->bm[b];
+    memset((void *)&bm, 0, sizeof(Beam));
+    bm.cfg = *cfg;
+    bm.qbm = cfg->qm / cfg->mass;
+    int rc = beam_alloc(c, bm, cfg->total_num * (cfg->quiet ? 2 : 1));
+    if (!rc) rc = launch_beam_inject(c, b);
+    return beam_finish(c, b, rc, beam_id);
+}
+
+int qpad_beam_new_std(qpad_ctx *c, const qpad_beam_std_cfg *cfg, int32_t *beam_id)
+{
+    if (!c || !cfg) return fail(QPAD_E_INVALID, "null argument");
+    if (c->nbeams >= kMaxBeams) return fail(QPAD_E_INVALID, "too many beams");
+    if (cfg->mass == 0.0 || cfg->qm == 0.0) return fail(QPAD_E_INVALID, "beam q and m must be non-zero");
+    if (cfg->ppc[0] < 1 || cfg->ppc[1] < 1 || cfg->ppc[2] < 1 || cfg->num_theta < 1)
+        return fail(QPAD_E_INVALID, "beam ppc and num_theta must be >= 1");
+    if (cfg->geometry != QPAD_GEOM_CARTESIAN && cfg->geometry != QPAD_GEOM_CYLINDRICAL)
+        return fail(QPAD_E_INVALID, "beam geometry must be cartesian or cylindrical");
+    for (int d = 0; d < 3; d++) {
+        const int pt = cfg->profile[d];
+        if (pt != QPAD_PROF_UNIFORM && pt != QPAD_PROF_GAUSSIAN && pt != QPAD_PROF_PIECEWISE)
+            return fail(QPAD_E_INVALID, "beam profile must be uniform, gaussian or piecewise-linear");
+        if (pt == QPAD_PROF_GAUSSIAN && !(cfg->gauss_sigma[d] > 0.0))
+            return fail(QPAD_E_INVALID, "gauss_sigma must be > 0");
+        if (pt == QPAD_PROF_PIECEWISE) {
+            if (cfg->pw_n[d] < 2 || cfg->pw_n[d] > QPAD_PW_MAX)
+                return fail(QPAD_E_INVALID, "piecewise-linear profiles take 2 .. 32 points");
+            for (int i = 1; i < cfg->pw_n[d]; i++)
+                if (!(cfg->pw_x[d][i] > cfg->pw_x[d][i - 1]))
+                    return fail(QPAD_E_INVALID, "Position array must be monotonically increasing!");
+        }
+        if (!(cfg->range_hi[d] >= cfg->range_lo[d])) return fail(QPAD_E_INVALID, "beam range upper < lower");
+    }
+    for (int d = 0; d < 2; d++) {
+        if (cfg->offset_n[d] < 0 || cfg->offset_n[d] > QPAD_OFFSET_MAX)
+            return fail(QPAD_E_INVALID, "perp_offset takes at most 8 coefficients");
+        if (cfg->alpha[d] != 0.0 && !(cfg->twiss_beta[d] != 0.0))
+            return fail(QPAD_E_INVALID, "Twiss alpha needs a non-zero beta");
+    }
+    if ((int64_t)c->g.nr * c->g.nxi >= 0x7fffffff) return fail(QPAD_E_INVALID, "grid too large for beam cells");
+    const int b = c->nbeams;
+    Beam &bm = c->bm[b];
+    memset((void *)&bm, 0, sizeof(Beam));
+    // the fields the push and the host mirror read (charge sign, mass, evolution)
+    bm.cfg.qm = cfg->qm;
+    bm.cfg.mass = cfg->mass;
+    bm.cfg.gamma = cfg->gamma;
+    bm.cfg.quiet = cfg->quiet;
+    bm.cfg.evol = cfg->evol;
+    bm.cfg.seed = cfg->seed;
+    bm.qbm = cfg->qm / cfg->mass;
+    LatticePlan pl;
+    int rc = lattice_count(c, *cfg, &pl);
+    if (!rc && pl.kept > 0x3fffffff) rc = fail(QPAD_E_INVALID, "lattice beam: too many particles");
+    if (!rc) rc = beam_alloc(c, bm, pl.kept * (cfg->quiet ? 2 : 1));
+    if (!rc) rc = lattice_write(c, *cfg, pl, bm);
+    if (!rc) rc = sync(c);
+    lattice_free(&pl);
+    return beam_finish(c, b, rc, beam_id);
 }
 
 // ---- fused step ----

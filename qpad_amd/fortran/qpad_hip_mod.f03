@@ -44,6 +44,28 @@ module qpad_hip_mod
     integer(c_int32_t) :: push_type
   end type qpad_species_cfg
 
+  integer(c_int), parameter, public :: QPAD_PROF_UNIFORM = 0, QPAD_PROF_GAUSSIAN = 1, &
+    QPAD_PROF_PIECEWISE = 2, QPAD_GEOM_CARTESIAN = 0, QPAD_GEOM_CYLINDRICAL = 1, &
+    QPAD_PW_MAX = 32, QPAD_OFFSET_MAX = 8
+
+  ! lattice beam (profile_type "standard"); C arrays [3][32] are Fortran
+  ! (32, 3), so pw_x(:, d) is direction d
+  type, bind(C), public :: qpad_beam_std_cfg
+    integer(c_int32_t) :: ppc(3), num_theta, geometry, profile(3)
+    real(c_double) :: qm, mass, density, gamma, den_min
+    real(c_double) :: gauss_center(3), gauss_sigma(3)
+    integer(c_int32_t) :: pw_n(3)
+    real(c_double) :: pw_x(32, 3), pw_fx(32, 3)
+    real(c_double) :: range_lo(3), range_hi(3)
+    real(c_double) :: uth(3)
+    real(c_double) :: alpha(2), twiss_ctr(2), twiss_beta(2)
+    integer(c_int32_t) :: offset_n(2)
+    real(c_double) :: offset_xi0(2)
+    real(c_double) :: offset_c(8, 2)
+    integer(c_int32_t) :: quiet, evol
+    integer(c_int64_t) :: seed
+  end type qpad_beam_std_cfg
+
   type, bind(C), public :: qpad_beam_cfg
     integer(c_int64_t) :: total_num
     real(c_double) :: total_charge
@@ -229,6 +251,15 @@ module qpad_hip_mod
       import :: c_int, c_int32_t, c_ptr, qpad_beam_cfg
       type(c_ptr), value :: ctx
       type(qpad_beam_cfg), intent(in) :: cfg
+      integer(c_int32_t), intent(out) :: beam_id
+      integer(c_int) :: rc
+    end function
+
+    ! beam3d%new with the lattice ("standard") profile
+    function qpad_beam_new_std(ctx, cfg, beam_id) bind(C, name="qpad_beam_new_std") result(rc)
+      import :: c_int, c_int32_t, c_ptr, qpad_beam_std_cfg
+      type(c_ptr), value :: ctx
+      type(qpad_beam_std_cfg), intent(in) :: cfg
       integer(c_int32_t), intent(out) :: beam_id
       integer(c_int) :: rc
     end function
@@ -521,7 +552,7 @@ module qpad_hip_mod
     qpad_species_push, qpad_field_solve_psi, qpad_field_solve_ez, &
     qpad_field_solve_et, qpad_field_solve_bz, qpad_field_solve_bt_beam, &
     qpad_field_solve_djdxi, qpad_field_solve_bt_iter, qpad_field_add_b, &
-    qpad_convergence_tester, qpad_beam_qdp, qpad_beam_push, qpad_beam_new, &
+    qpad_convergence_tester, qpad_beam_qdp, qpad_beam_push, qpad_beam_new, qpad_beam_new_std, &
     qpad_step3d_begin, qpad_slice, qpad_step3d_end, qpad_step3d, &
     qpad_get_field, qpad_set_field, qpad_get_hist, qpad_set_hist, &
     qpad_species_count, qpad_get_species, qpad_set_species, &

@@ -8,7 +8,9 @@ else raises QpadError, as QPAD's write_err would stop on an invalid deck.
 import json
 import math
 
-from ._lib import BND_OPEN, BND_ZERO, PUSH2_ROBUST, BeamCfg, Grid, QpadError, SpeciesCfg
+from ._lib import (BND_OPEN, BND_ZERO, GEOM_CARTESIAN, GEOM_CYLINDRICAL, OFFSET_MAX, PROF_GAUSSIAN,
+                   PROF_PIECEWISE, PROF_UNIFORM, PUSH2_ROBUST, PW_MAX, BeamCfg, BeamStdCfg, Grid, QpadError,
+                   SpeciesCfg)
 
 
 def _get(d, key, default=None, required=True):
@@ -19,9 +21,29 @@ def _get(d, key, default=None, required=True):
     return default
 
 
+def strip_comments(text):
+    """QPAD reads its deck with json-fortran and comment_char '!'
+    (input_class.f03:63): drop '!' to the end of a line outside strings."""
+    out = []
+    for line in text.splitlines():
+        in_str, esc = False, False
+        for i, ch in enumerate(line):
+            if esc:
+                esc = False
+            elif ch == "\\":
+                esc = in_str
+            elif ch == '"':
+                in_str = not in_str
+            elif ch == "!" and not in_str:
+                line = line[:i]
+                break
+        out.append(line)
+    return "\n".join(out)
+
+
 def load_deck(path):
     with open(path) as f:
-        return json.load(f)
+        return json.loads(strip_comments(f.read()))
 
 
 def grid_from_deck(deck):
@@ -76,20 +98,26 @@ def ctypes_d3(v):
 
 
 def beam_from_deck(bm, z0, seed=1):
-    """fdist3d_rnd_class.f03:104-356 (profile_type 'random', Cartesian Gaussian)."""
-    if _get(bm, "profile_type", "standard", required=False) != "random":
-        raise QpadError("only profile_type 'random' beams are on the GPU path")
-    if _get(bm, "geometry", "cartesian", required=False) != "cartesian":
-        raise QpadError("only Cartesian beam geometry is on the GPU path")
-    if list(_get(bm, "profile")) != ["gaussian", "gaussian", "gaussian"]:
-        raise QpadError("only Gaussian beam profiles are on the GPU path")
+    """beam(i) section: profile_type "standard" (the default: the lattice of
+    fdist3d_std_class.f03:150-436) or "random" (fdist3d_rnd_class.f03:
+    104-356).  xi positions are made relative to the box start, as QPAD
+    stores them."""
+    kind = _get(bm, "profile_type", "standard", required=False)
     if _get(bm, "push_type", "reduced", required=False) != "reduced":
         raise QpadError('only the "reduced" beam pusher is on the GPU path')
     if _get(bm, "has_spin", False, required=False):
         raise QpadError("spin push is not on the GPU path")
+    if kind == "standard":
+        return _lattice_beam(bm, z0, seed)
+    if kind != "random":
+        raise QpadError(f"beam profile_type '{kind}' is not on the GPU path")
+    if _get(bm, "geometry", "cartesian", required=False) != "cartesian":
+        raise QpadError("only Cartesian random beams are on the GPU path")
+    if list(_get(bm, "profile")) != ["gaussian", "gaussian", "gaussian"]:
+        raise QpadError("only Gaussian random beam profiles are on the GPU path")
     for key in ("alpha", "perp_offset_x", "perp_offset_y"):
         if key in bm:
-            raise QpadError(f"beam key '{key}' is not on the GPU path")
+            raise QpadError(f"beam key '{key}' is not on the GPU path for random beams")
     ctr = list(_get(bm, "gauss_center"))
     ctr[2] = ctr[2] - z0
     lo = [bm["range1"][0], bm["range2"][0], bm["range3"][0] - z0]
@@ -100,6 +128,77 @@ def beam_from_deck(bm, z0, seed=1):
                    ctypes_d3(_get(bm, "uth", [0.0, 0.0, 0.0], required=False)),
                    int(bool(_get(bm, "quiet_start", False, required=False))),
                    int(bool(_get(bm, "evolution", True, required=False))), int(seed))
+
+
+_PROFILES = {"uniform": PROF_UNIFORM, "gaussian": PROF_GAUSSIAN, "piecewise-linear": PROF_PIECEWISE}
+
+
+def _lattice_beam(bm, z0, seed):
+    """fdist3d_std_class.f03:150-436 (settings read by init_fdist3d_std) and
+    fdist3d_std_lib.f03 (profile parameters)."""
+    c = BeamStdCfg()
+    geom = _get(bm, "geometry", "cartesian", required=False)
+    if geom not in ("cartesian", "cylindrical"):
+        raise QpadError(f"Invalid beam geometry '{geom}'")
+    c.geometry = GEOM_CARTESIAN if geom == "cartesian" else GEOM_CYLINDRICAL
+    prof = list(_get(bm, "profile"))
+    for d, name in enumerate(prof):
+        if name not in _PROFILES:
+            raise QpadError(f"beam density profile '{name}' is not on the GPU path "
+                            "(uniform, gaussian and piecewise-linear are)")
+        c.profile[d] = _PROFILES[name]
+        shift = z0 if d == 2 else 0.0
+        if name == "gaussian":
+            c.gauss_center[d] = float(bm["gauss_center"][d]) - shift
+            c.gauss_sigma[d] = float(bm["gauss_sigma"][d])
+        elif name == "piecewise-linear":
+            xs = [float(v) - shift for v in _get(bm, f"piecewise_x{d + 1}")]
+            fx = [float(v) for v in _get(bm, f"piecewise_fx{d + 1}")]
+            if len(xs) != len(fx) or not 2 <= len(xs) <= PW_MAX:
+                raise QpadError(f"piecewise_x{d + 1} / piecewise_fx{d + 1}: 2 .. {PW_MAX} points of equal count")
+            c.pw_n[d] = len(xs)
+            for i, (x, f) in enumerate(zip(xs, fx)):
+                c.pw_x[d][i] = x
+                c.pw_fx[d][i] = f
+    ppc = _get(bm, "ppc")
+    for d in range(3):
+        c.ppc[d] = int(ppc[d])
+    c.num_theta = int(_get(bm, "num_theta"))
+    c.qm = float(_get(bm, "q"))
+    c.mass = float(_get(bm, "m"))
+    c.density = float(_get(bm, "density"))
+    c.gamma = float(_get(bm, "gamma"))
+    for d in range(3):
+        r = _get(bm, f"range{d + 1}")
+        shift = z0 if d == 2 else 0.0
+        c.range_lo[d] = float(r[0]) - shift
+        c.range_hi[d] = float(r[1]) - shift
+    c.evol = int(bool(_get(bm, "evolution", True, required=False)))
+    c.quiet = int(bool(_get(bm, "quiet_start", False, required=False)))
+    c.den_min = float(_get(bm, "den_min", 1.0e-10, required=False))
+    uth = _get(bm, "uth", [0.0, 0.0, 0.0], required=False)
+    for d in range(3):
+        c.uth[d] = float(uth[d])
+    if "alpha" in bm:
+        # Twiss parameters (fdist3d_std_class.f03:367-386)
+        for d in range(2):
+            if prof[d] != "gaussian" or geom != "cartesian":
+                raise QpadError("Twiss parameter alpha is only available for Gaussian profile in the "
+                                "Cartesian geometry.")
+            c.alpha[d] = float(bm["alpha"][d])
+            c.twiss_ctr[d] = float(bm["gauss_center"][d])
+            c.twiss_beta[d] = c.gamma * float(bm["gauss_sigma"][d]) / float(uth[d])
+    for d, key in enumerate(("perp_offset_x", "perp_offset_y")):
+        if key in bm:
+            v = [float(x) for x in bm[key]]
+            if not 2 <= len(v) <= OFFSET_MAX + 1:
+                raise QpadError(f"{key}: origin and 1 .. {OFFSET_MAX} coefficients")
+            c.offset_xi0[d] = v[0] - z0
+            c.offset_n[d] = len(v) - 1
+            for k, x in enumerate(v[1:]):
+                c.offset_c[d][k] = x
+    c.seed = int(seed)
+    return c
 
 
 def tri_gaussian_charge(n_peak, sigma):

@@ -156,12 +156,14 @@ class Species2D:
 
 
 class Beam3D:
-    """beam3d (beam3d_class.f03:51-52): random Gaussian beam, reduced pusher."""
+    """beam3d (beam3d_class.f03:51-52): a lattice ("standard") or random
+    Gaussian beam, reduced pusher."""
 
     def __init__(self, sim, cfg):
         self.sim = sim
         bid = C.c_int32(0)
-        L.check(sim.lib.qpad_beam_new(sim.ctx, C.byref(cfg), C.byref(bid)))
+        new = sim.lib.qpad_beam_new_std if isinstance(cfg, L.BeamStdCfg) else sim.lib.qpad_beam_new
+        L.check(new(sim.ctx, C.byref(cfg), C.byref(bid)))
         self.id = bid.value
         self.cfg = cfg
 

@@ -78,9 +78,22 @@ MUTANTS = {
                       "double t = ep[0] * cc + ep[1] * ss; double u = ep[0] * ss + ep[1] * cc;", ["beam_push"]),
     "inject_cell_edge_lattice": ("double rn = ((double)i1 - 0.5) / (double)c->ppc1 + (double)(i - 1);",
                                  "double rn = ((double)i1) / (double)c->ppc1 + (double)(i - 1);", ["inject"]),
-    "beam_quiet_charge": ("bm->q[i] = 0.5 * bm->q[i];", "bm->q[i] = 1.0 * bm->q[i];", ["beam_inject"]),
+    "beam_quiet_charge": ("bm->q[i] = 0.5 * bm->q[i];\n            bm->q[n + i] = bm->q[i];\n"
+                          "            bm->id[n + i] = bm->id[i] + (int32_t)c->total_num;",
+                          "bm->q[i] = 1.0 * bm->q[i];\n            bm->q[n + i] = bm->q[i];\n"
+                          "            bm->id[n + i] = bm->id[i] + (int32_t)c->total_num;", ["beam_inject"]),
+    "lattice_quiet_charge": ("bm->q[i] = 0.5 * bm->q[i];\n            bm->q[n + i] = bm->q[i];\n"
+                             "            bm->id[n + i] = bm->id[i] + (int32_t)n;",
+                             "bm->q[i] = 1.0 * bm->q[i];\n            bm->q[n + i] = bm->q[i];\n"
+                             "            bm->id[n + i] = bm->id[i] + (int32_t)n;", ["beam_lattice"]),
     "beam_sigma_swapped": ("xt[d] = g[d] * c->sigma[d] + c->center[d];",
                            "xt[d] = g[d] * c->sigma[2 - d] + c->center[d];", ["beam_inject"]),
+    "lattice_rn_cell_edge": ("double rn = ((double)i1 - 0.5) / (double)c->ppc[0] + (double)(i - 1);",
+                             "double rn = ((double)i1) / (double)c->ppc[0] + (double)(i - 1);",
+                             ["beam_lattice"]),
+    "lattice_gauss_sigma_squared": ("double t = (x - mu) / sigma;\n        return exp(-0.5 * (t * t));",
+                                    "double t = (x - mu) / sigma;\n        return exp(-1.0 * (t * t));",
+                                    ["beam_lattice"]),
     "update_bound_skips_one": ("while (i < sp->npp - 1) {\n        double pos = sqrt(sp->x1[i]",
                                "while (i < sp->npp - 2) {\n        double pos = sqrt(sp->x1[i]",
                                ["update_bound"]),

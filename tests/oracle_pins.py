@@ -54,6 +54,9 @@ What fixes each routine (reference lines are the QPAD source it restates):
   content, and a total charge per unit length of sign(q) n pi R^2.
 * update_bound (part2d_class.f03:2307-2353): hand-worked examples of
   swap-with-last removal.
+* Lattice beam injection (fdist3d_std_class.f03:439-613): a uniform
+  lattice deposits its density exactly at interior (r, xi) nodes; a
+  Gaussian lattice carries n_b (2 pi)^(3/2) s_x s_y s_z; quiet start.
 * Random beam injection (fdist3d_rnd_class.f03:360-513): sample moments of
   the truncated Gaussian, the ranges, p_z from gamma, the quiet-start
   mirror and its halved charges.
@@ -617,6 +620,59 @@ def pin_beam_inject(L):
     assert np.allclose(bm["q"], 0.5 * q_full, rtol=1e-14), "quiet start halves the charge"
 
 
+def pin_beam_lattice(L):
+    """Lattice beam (fdist3d_std_class.f03:439-613): a uniform profile over
+    the whole box deposits its density exactly at every interior (r, xi)
+    node (the linear-weight identity of the ring lattice, in r and in xi)
+    with no m > 0 content; a Gaussian profile carries the total charge
+    n_b (2 pi)^(3/2) s_x s_y s_z (fdist3d_std_lib.f03:114-127) to the
+    lattice's second-order accuracy; quiet start mirrors with half charges."""
+    nr, nxi, dr, dxi = 12, 10, 0.1, 0.1
+    o = po.Oracle(po.make_grid(nr, nxi, 2, dr, dxi), L)
+    c = po.BeamStdCfg()
+    c.ppc[:] = [2, 3, 2]
+    c.num_theta = 8
+    c.qm, c.mass, c.density, c.gamma, c.den_min = -1.0, 1.0, 2.5, 1000.0, 1e-10
+    c.range_lo[:] = [-5.0, -5.0, -1.0]
+    c.range_hi[:] = [5.0, 5.0, 5.0]
+    c.evol = 1
+    o.add_beam_std(c)
+    b = o.beam(0)
+    assert len(b["x1"]) == nr * nxi * 2 * 3 * 2 * 8, "every lattice point kept"
+    h = np.zeros((nxi + 2, 1, o.nm, o.ld))
+    L.ora_beam_qdeposit(o.h, 0, po.ptr(h))
+    assert np.max(np.abs(h[2:nxi + 1, 0, 0, 2:nr + 1] + 2.5)) <= 1e-12, "uniform lattice density"
+    assert np.max(np.abs(h[2:nxi + 1, 0, 1:, 2:nr + 1])) <= 1e-12, "m > 0 content of a uniform lattice"
+    # charge of the uniform cylinder: -n pi R^2 L
+    total = np.sum(b["q"]) * 2 * math.pi * dr * dr * dxi
+    assert abs(total + 2.5 * math.pi * (nr * dr) ** 2 * nxi * dxi) <= 1e-12 * abs(total), ("cylinder charge", total)
+    # Gaussian beam: total charge and quiet-start mirror
+    nr, nxi, dr, dxi = 60, 60, 0.05, 0.05
+    o = po.Oracle(po.make_grid(nr, nxi, 1, dr, dxi), L)
+    sig, ctr = (0.3, 0.3, 0.4), (0.0, 0.0, 1.5)
+    c = po.BeamStdCfg()
+    c.ppc[:] = [2, 2, 2]
+    c.num_theta = 16
+    c.profile[:] = [1, 1, 1]
+    c.gauss_center[:] = ctr
+    c.gauss_sigma[:] = sig
+    c.qm, c.mass, c.density, c.gamma, c.den_min = -1.0, 1.0, 4.0, 1000.0, 1e-10
+    c.range_lo[:] = [-1.5, -1.5, 0.0]
+    c.range_hi[:] = [1.5, 1.5, 3.0]
+    c.uth[:] = [2.0, 2.0, 0.0]
+    c.quiet, c.evol = 1, 1
+    o.add_beam_std(c)
+    b = o.beam(0)
+    n = len(b["x1"]) // 2
+    total = np.sum(b["q"]) * 2 * math.pi * dr * dr * dxi
+    want = -4.0 * (2 * math.pi) ** 1.5 * sig[0] * sig[1] * sig[2]
+    assert abs(total / want - 1.0) < 2e-3, ("Gaussian lattice charge", total, want)
+    for k, sgn in (("x1", -1), ("x2", -1), ("x3", 1), ("p1", -1), ("p2", -1), ("p3", 1)):
+        assert np.array_equal(b[k][n:], sgn * b[k][:n]), f"quiet-start mirror {k}"
+    assert np.array_equal(b["q"][n:], b["q"][:n])
+    assert np.max(np.abs(b["p3"] - np.sqrt(1000.0 ** 2 - b["p1"] ** 2 - b["p2"] ** 2 - 1.0))) <= 1e-9
+
+
 PINS = {
     "matrices": pin_matrices,
     "ez": pin_ez,
@@ -628,6 +684,7 @@ PINS = {
     "amjdeposit": pin_amjdeposit,
     "beam_push": pin_beam_push,
     "beam_inject": pin_beam_inject,
+    "beam_lattice": pin_beam_lattice,
     "inject": pin_inject,
     "update_bound": pin_update_bound,
 }

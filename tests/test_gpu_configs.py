@@ -110,3 +110,46 @@ def test_c4_largest_instance_slice_matches_oracle():
     print(params[i], {n: f"{e:.1e}" for n, e in sorted(norm.errors().items())})
     bad = norm.bad(TOL)
     assert not bad, f"beyond {TOL}: {bad}"
+
+
+def test_tri_gaussian_deck_lattice_beam_and_step_match_oracle():
+    """QPAD's blowout-regime example deck (lattice beam with quiet start,
+    input_file/blowout_regime/qpinput_tri-gaussian.json): the library
+    injects the same 4.4 million beam particles as the oracle's restatement
+    of fdist3d_std_class.f03:439-613, and one 3D step agrees slice by slice,
+    then through the beam push."""
+    deck = configs.tri_gaussian()
+    sim = Simulation(deck)
+    ob = twin.oracle_from_deck(deck)
+    gb, rb = sim.beam[0].get(), ob.beam(0)
+    assert len(gb["x1"]) == len(rb["x1"]) == 4405856
+    for k in ("x1", "x2", "x3", "p1", "p2", "p3", "q"):
+        e = rel_err(np.sort(gb[k]), np.sort(rb[k]))
+        assert e <= 1e-12, (k, e)
+    ob.close()
+    g = sim.grid
+    o = twin.make_twin(sim)
+    sim.step3d_begin()
+    o.step_begin()
+    norm = twin.ErrorNorm()
+    iters = []
+    for j in range(1, g.nxi + 1):
+        twin.sync_slice_inputs(sim, o, j)
+        it_g, it_o = sim.slice(j), o.slice(j)
+        iters.append((it_g, it_o))
+        twin.compare_fields(sim, o, norm, j)
+        if j % 50 == 0 or j == g.nxi:
+            twin.compare_species(sim, o, norm, j)
+    print({n: f"{e:.1e}" for n, e in sorted(norm.errors().items())})
+    print("iterations per slice:", np.mean([a for a, _ in iters]))
+    assert all(a == b for a, b in iters), [(j + 1, a, b) for j, (a, b) in enumerate(iters) if a != b]
+    bad = norm.bad(TOL)
+    assert not bad, f"beyond {TOL}: {bad}"
+    sim.step3d_end()
+    o.step_end(1)
+    gb, ob = sim.beam[0].get(), o.beam(0)
+    assert len(gb["x1"]) == len(ob["x1"])
+    for k in ("x1", "x2", "x3", "p1", "p2", "p3"):
+        e = rel_err(np.sort(gb[k]), np.sort(ob[k]))
+        assert e <= 1e-9, (k, e)
+    sim.close()

@@ -1,5 +1,7 @@
 """Oracle twin of a GPU Simulation: same grid, same species, the GPU's beam,
 and helpers to copy the GPU's slice inputs into the oracle and compare."""
+import ctypes as C
+
 import numpy as np
 
 import pyoracle as po
@@ -38,7 +40,10 @@ def _as_oracle_struct(cls, src):
     out = cls()
     for name, typ in cls._fields_:
         v = getattr(src, name)
-        setattr(out, name, typ(*list(v)) if hasattr(typ, "_length_") else v)
+        if hasattr(typ, "_length_"):
+            C.memmove(C.addressof(getattr(out, name)), C.addressof(v), C.sizeof(typ))
+        else:
+            setattr(out, name, v)
     return out
 
 
@@ -56,7 +61,11 @@ def oracle_from_deck(deck):
     for k, sp in enumerate(deck.get("species", [])[:nsp]):
         o.add_species(_as_oracle_struct(po.SpeciesCfg, species_from_deck(sp, seed=seed + 1000 * k)))
     for k, bm in enumerate(deck.get("beam", [])[:nbm]):
-        o.add_beam(_as_oracle_struct(po.BeamCfg, beam_from_deck(bm, z0, seed=seed + 7 * (k + 1))))
+        cfg = beam_from_deck(bm, z0, seed=seed + 7 * (k + 1))
+        if type(cfg).__name__ == "BeamStdCfg":
+            o.add_beam_std(_as_oracle_struct(po.BeamStdCfg, cfg))
+        else:
+            o.add_beam(_as_oracle_struct(po.BeamCfg, cfg))
     return o
 
 
