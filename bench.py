@@ -13,7 +13,8 @@ HBM before the timed region.  Prints one JSON line.
 Two legs run from identical initial states, to the same iteration
 tolerance: the plain method (QPAD's seed b_spe(xi-1)) and the learned seed
 (first-order extrapolation plus the conv network's correction, trained
-online during the W + T untimed warm-up steps, frozen in the timed steps).
+during the W + T untimed warm-up steps on the plain solver's (slice,
+converged B_perp) pairs, then frozen and seeding in the timed steps).
 Both time the same K steps (3D steps W+T+1 .. W+T+K of the same physics),
 interleaved step by step.
 `value` is the leg named by --seed (plain by default); the other leg is
@@ -42,13 +43,14 @@ def parse():
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", default="c2", choices=["c1", "c2", "c3"])
-    ap.add_argument("--cpu-slices", type=int, default=24)
+    ap.add_argument("--cpu-slices", type=int, default=12)
     ap.add_argument("--cpu-start", type=int, default=100)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--seed", default="plain", choices=["learned", "plain"],
                     help="the leg reported as value (both legs always run)")
     ap.add_argument("--train-steps", type=int, default=2,
                     help="extra untimed 3D steps of online training of the seed network")
+    ap.add_argument("--lr", type=float, default=1e-5, help="Adam learning rate of the seed network")
     ap.add_argument("--no-c4", action="store_true",
                     help="skip the batched-instances scan (C4) section")
     ap.add_argument("--only-c4", action="store_true",
@@ -168,12 +170,13 @@ def batch_c4(reps=2, eager="eager"):
 # replayed from a GPU snapshot, one process per core
 # ---------------------------------------------------------------------------
 def _cpu_worker(args):
-    snap_path, nslices, out_q = args
+    snap_path, nslices, lib_path, out_q = args
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import pyoracle as po
+    L = po.load_library(lib_path) if lib_path else po.lib()
     z = np.load(snap_path, allow_pickle=False)
     g = po.Grid(*[z["grid_" + f][()] for f, _ in po.Grid._fields_])
-    o = po.Oracle(g)
+    o = po.Oracle(g, L)
     nsp = int(z["nspecies"])
     for k in range(nsp):
         c = z[f"sp{k}_cfg"]
@@ -186,13 +189,61 @@ def _cpu_worker(args):
     o.field(po.B_SPE)[...] = z["b_spe"]
     j0 = int(z["j0"])
     o.hist(po.Q_BEAM)[j0:j0 + nslices] = z["q_beam"][:nslices]
+    po.timing(True, L)
     t0 = time.perf_counter()
     its = [o.slice(j0 + s) for s in range(nslices)]
     dt = time.perf_counter() - t0
-    out_q.put((dt, its))
+    out_q.put((dt, its, po.timers(L)))
 
 
-def cpu_baseline(sim, per_slice_iters, j0, nslices, cores):
+def _collect(q, procs, limit_s=900.0):
+    """one result per worker; raise if a worker dies or the sample overruns"""
+    import queue
+    out, t0 = [], time.perf_counter()
+    while len(out) < len(procs):
+        try:
+            out.append(q.get(timeout=2.0))
+        except queue.Empty:
+            dead = [p.exitcode for p in procs if p.exitcode not in (None, 0)]
+            if dead or time.perf_counter() - t0 > limit_s:
+                for p in procs:
+                    if p.is_alive():
+                        p.terminate()
+                raise RuntimeError(f"CPU baseline worker failed (exit codes {dead})")
+    for p in procs:
+        p.join()
+    return out
+
+
+def _cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def _fast_oracle():
+    """The oracle built at -O3 -march=native on this host (the timing build
+    of the CPU baseline; oracle/Makefile target 'fast'), or None."""
+    import subprocess
+    out = os.path.join(tempfile.gettempdir(), f"qpad_oracle_fast_{os.getpid()}")
+    r = subprocess.run(["make", "-s", "-C", os.path.join(ROOT, "oracle"), "fast", f"FAST_OUT={out}"],
+                       capture_output=True, text=True)
+    path = os.path.join(out, "libqpad_oracle_fast.so")
+    return path if r.returncode == 0 and os.path.exists(path) else None
+
+
+def cpu_baseline(sim, per_slice_iters, j0, nslices, runs=3):
+    """The oracle (serial FP64 C restatement of QPAD's slice step) timed on
+    this host: one core, and one independent instance per core of this
+    process's CPU set running concurrently (the upper bound of QPAD's MPI
+    throughput on the host); each `runs` times, median reported.  The sample
+    is a window of nslices slices replayed from a snapshot of the plain leg
+    at slice j0, scaled to a 3D step by particle passes (2 + N_it per
+    slice)."""
     from qpad_amd import _lib as L
     g = sim.grid
     snap = {"j0": j0, "nspecies": len(sim.spe)}
@@ -211,38 +262,74 @@ def cpu_baseline(sim, per_slice_iters, j0, nslices, cores):
     fd, path = tempfile.mkstemp(suffix=".npz", dir=os.environ.get("TMPDIR", "/tmp"))
     os.close(fd)
     np.savez(path, **snap)
+    fast = _fast_oracle()
+    # the cores this job may use: its CPU set, limited to the CPU share the
+    # host allots to one GPU job where it states one (OMP_NUM_THREADS)
+    cpu_set = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+    share = int(os.environ.get("OMP_NUM_THREADS", "0") or 0)
+    cores = min(cpu_set, share) if share > 0 else cpu_set
     ctx = mp.get_context("spawn")
-    results = {}
+    passes_step = sum(2 + i for i in per_slice_iters)
+    one, allc, breakdown = [], [], None
     try:
-        for ncore in sorted(set([1, cores])):
-            q = ctx.Queue()
-            procs = [ctx.Process(target=_cpu_worker, args=((path, nslices, q),)) for _ in range(ncore)]
-            for p in procs:
-                p.start()
-            out = [q.get() for _ in procs]
-            for p in procs:
-                p.join()
-            results[ncore] = out
+        for _ in range(runs):
+            for ncore in (1, cores):
+                q = ctx.Queue()
+                procs = [ctx.Process(target=_cpu_worker, args=((path, nslices, fast, q),)) for _ in range(ncore)]
+                for p in procs:
+                    p.start()
+                out = _collect(q, procs)
+                # seconds per 3D step of each instance, by particle passes
+                per = [dt / sum(2 + i for i in its) * passes_step for dt, its, _ in out]
+                if ncore == 1:
+                    one.append(per[0])
+                    dt, its, tm = out[0]
+                    sample_its = its
+                    scale = passes_step / sum(2 + i for i in its)
+                    breakdown = {k: v * scale for k, v in tm.items()}
+                else:
+                    allc.append(sum(1.0 / t for t in per))
     finally:
         os.unlink(path)
-    # cost model: particle passes per slice = 2 + N_it (qdeposit, N_it
-    # deposits, push); scale the sample to the full 3D step's iteration profile
-    dt1, its1 = results[1][0]
-    passes_sample = sum(2 + i for i in its1)
-    passes_step = sum(2 + i for i in per_slice_iters)
-    t_step_1 = dt1 / passes_sample * passes_step
-    per_core = [dt / sum(2 + i for i in its) * passes_step for dt, its in results[cores]]
-    agg = sum(1.0 / t for t in per_core)   # 3D steps/s of `cores` concurrent instances
+    t1 = float(np.median(one))
+    agg = float(np.median(allc))
     np_slice = sum(sp.count() for sp in sim.spe)
+    build = "gcc -O3 -march=native" if fast else "gcc -O2 (the -O3 -march=native build failed)"
     return {
         "value": agg, "unit": "3D steps/s", "cores": cores, "kind": "port",
-        "sample": (f"oracle (serial FP64 C, gcc -O2) replaying slices {j0}..{j0 + nslices - 1} of "
-                   f"the workload from a GPU snapshot, {cores} concurrent instances; scaled to a "
-                   f"3D step by particle passes (2 + N_it per slice)"),
-        "one_core_3d_steps_per_s": 1.0 / t_step_1,
-        "one_core_particle_slice_pushes_per_s": np_slice * nslices / dt1,
-        "sample_iters": its1,
+        "sample": (f"oracle (serial FP64 C, {build}) replaying slices {j0}..{j0 + nslices - 1} of the "
+                   f"workload from a snapshot of the plain leg; {cores} concurrent instances (one per core "
+                   f"this job may use); median of {runs} runs; scaled to a 3D step by particle "
+                   f"passes (2 + N_it per slice)"),
+        "host_cpu_count": os.cpu_count(), "cpu_set": cpu_set, "cpu_share_omp_num_threads": share or None,
+        "cpu_model": _cpu_model(),
+        "one_core_3d_steps_per_s": 1.0 / t1,
+        "one_core_particle_slice_pushes_per_s": np_slice * g.nxi / t1,
+        "one_core_runs_s_per_step": one, "all_core_runs_3d_steps_per_s": allc,
+        "one_core_breakdown_s_per_step": breakdown,
+        "sample_iters": sample_its,
     }
+
+
+def hbm_copy_bandwidth():
+    """Device-to-device copy of 1 GiB (read 1 GiB + write 1 GiB), its source
+    first evicted from the caches; bytes moved / time of the best of 5."""
+    import torch
+    n = 1 << 27
+    src = torch.ones(n, dtype=torch.float64, device="cuda")
+    dst = torch.empty_like(src)
+    evict = torch.empty(n, dtype=torch.float64, device="cuda")
+    best = float("inf")
+    for _ in range(5):
+        evict.fill_(2.0)
+        s0, s1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        s0.record()
+        dst.copy_(src)
+        s1.record()
+        torch.cuda.synchronize()
+        best = min(best, s0.elapsed_time(s1) / 1e3)
+    del src, dst, evict
+    return 2.0 * 8 * n / best / 1e9
 
 
 def main():
@@ -257,47 +344,59 @@ def main():
     if args.only_c4:
         print(json.dumps({"batch_c4": batch_c4(eager=args.c4_eager)}))
         return
+    # torch's HIP runtime starts before the library's: measured first
+    copy_gbs = hbm_copy_bandwidth()
     from qpad_amd import SeedNet, Simulation
     deck = deck_for(args.workload)
     legs = {"plain": Simulation(deck), "learned": Simulation(deck)}
-    net = SeedNet(legs["learned"])
-    net.mode(seed=True, train=True)
+    # the network learns from (slice, converged B_perp) pairs of the plain
+    # solver (seed off during the warm-up steps, one Adam step per slice):
+    # seeded labels would carry the network's own seed at the tolerance level
+    net = SeedNet(legs["learned"], lr=args.lr)
+    net.mode(seed=False, train=True)
     sim = legs[args.seed]
     g = sim.grid
     np_slice = sum(sp.count() for sp in sim.spe)
     nbeam = sum(b.count() for b in sim.beam)
-    # first warm-up step of both legs from the same state.  The learned seed's
-    # converged fields against the plain method's, per quarter of the box in
-    # xi, beside the same comparison for the plain method at half the
-    # tolerance (the wake behind the driver amplifies any tolerance-level
-    # change along xi, so that control is the scale to read the first by)
+    # the plain method at half the tolerance, one step from the initial
+    # state, against the plain leg's first step: the scale of a
+    # tolerance-level change, which the wake behind the driver amplifies
+    # along xi
     import copy
     deck_half = copy.deepcopy(deck)
     deck_half["simulation"]["iter_reltol"] = 0.5 * deck["simulation"]["iter_reltol"]
     deck_half["simulation"]["iter_abstol"] = 0.5 * deck["simulation"]["iter_abstol"]
     control = Simulation(deck_half)
-    warm_iters = {k: s_.step3d() / g.nxi for k, s_ in legs.items()}
     control.step3d()
-    field_diff = {"learned_vs_plain": {}, "plain_half_tol_vs_plain": {}}
+    legs["plain"].step3d()
+
+    def quarter_diff(fa, fb):
+        rows = []
+        for qtr in range(4):
+            num = den = 0.0
+            for j in range(1 + qtr * g.nxi // 4, 1 + (qtr + 1) * g.nxi // 4, 8):
+                a, b = fa.hist(j), fb.hist(j)
+                num = max(num, float(np.max(np.abs(a - b))))
+                den = max(den, float(np.max(np.abs(a))))
+            rows.append(num / den if den > 0 else num)
+        return rows
+
+    field_diff = {"learned_vs_plain": {}, "plain_half_tol_vs_plain_first_step": {}}
     for name in ("psi", "b_spe"):
-        fp = legs["plain"].fields[name]
-        for key, other in (("learned_vs_plain", legs["learned"].fields[name]),
-                           ("plain_half_tol_vs_plain", control.fields[name])):
-            rows = []
-            for qtr in range(4):
-                num = den = 0.0
-                for j in range(1 + qtr * g.nxi // 4, 1 + (qtr + 1) * g.nxi // 4, 8):
-                    a, b = fp.hist(j), other.hist(j)
-                    num = max(num, float(np.max(np.abs(a - b))))
-                    den = max(den, float(np.max(np.abs(a))))
-                rows.append(num / den if den > 0 else num)
-            field_diff[key][name] = rows
+        field_diff["plain_half_tol_vs_plain_first_step"][name] = quarter_diff(legs["plain"].fields[name],
+                                                                              control.fields[name])
     control.close()
+    legs["learned"].step3d()
     for _ in range(args.warmup + args.train_steps - 1):
         for s_ in legs.values():
             s_.step3d()
+    # frozen network, seed on: the first seeded step (untimed) against the
+    # plain leg's step from the same state
     net.mode(seed=True, train=False)
     train_losses = net.losses()
+    warm_iters = {k: s_.step3d() / g.nxi for k, s_ in legs.items()}
+    for name in ("psi", "b_spe"):
+        field_diff["learned_vs_plain"][name] = quarter_diff(legs["plain"].fields[name], legs["learned"].fields[name])
     # timed region: K whole 3D steps of each leg, interleaved (plain, learned,
     # then learned, plain, ...) so that both legs see the same device
     # conditions; each step3d call returns when its step is done.  HIP events
@@ -327,9 +426,10 @@ def main():
     legs_out["learned"].update({
         "train_steps": int(net.info().steps), "nparams": int(net.nparams),
         "loss_median_last_step": float(np.median(train_losses[-g.nxi:])) if len(train_losses) else None,
-        "first_step_iters_per_slice": warm_iters["learned"],
-        "first_step_field_max_rel_diff_per_xi_quarter": field_diff})
-    legs_out["plain"]["first_step_iters_per_slice"] = warm_iters["plain"]
+        "lr": args.lr, "loss_median_first_step": float(np.median(train_losses[:g.nxi])) if len(train_losses) else None,
+        "first_seeded_step_iters_per_slice": warm_iters["learned"],
+        "field_max_rel_diff_per_xi_quarter": field_diff})
+    legs_out["plain"]["first_seeded_step_iters_per_slice"] = warm_iters["plain"]
     # per-slice iteration profiles (one instrumented, untimed step per leg)
     prof = {}
     for name, s_ in legs.items():
@@ -338,15 +438,17 @@ def main():
         s_.step3d_end()
     per_slice = prof[args.seed]
     j0 = max(1, min(args.cpu_start, g.nxi - args.cpu_slices + 1))
-    # deposit kernel roofline: 48 B read (x1, x2, p1, p2, p3, q) + 16 B written
-    # (gamma, psi) per plasma particle per launch.  The HIP events bracket the
+    # deposit kernel roofline: 48 B read (x1, x2, p1, p2, p3, q) per plasma
+    # particle per launch, the algorithmic bytes of an iteration pass; the
+    # gamma and psi stores (16 B) are reported beside it.  The HIP events bracket the
     # first predictor-corrector iteration's deposit of every slice (always a
     # working launch; the speculative later ones are a separate kernel
     # instance, k_amjdeposit<M, true>, which returns at once after
     # convergence).
     dep_avg_s = dep_ms / max(dep_n, 1) / 1e3
-    dep_bytes = 64.0 * np_slice
+    dep_bytes = 48.0 * np_slice          # algorithmic: x1, x2, p1, p2, p3, q read per pass
     achieved = dep_bytes / dep_avg_s / 1e9 if dep_avg_s > 0 else 0.0
+    traffic = measured_traffic(args.workload, g.max_mode)
     step_bytes = algorithmic_bytes_per_step(g, np_slice, per_slice, nbeam)
     result = {
         "metric": "3D steps/s & plasma particle-slice pushes/s; learned-seed vs plain, same tol",
@@ -373,11 +475,16 @@ def main():
         "learned_vs_plain_speedup": legs_out["learned"]["value"] / legs_out["plain"]["value"],
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS,
-                     "traffic": measured_traffic(args.workload, g.max_mode),
+                     "traffic": traffic,
                      "kernel": f"k_amjdeposit<{g.max_mode}, false> (first iteration of each slice: gather + "
                                "robust predictor + J/dJ/momentum-flux deposit)",
                      "avg_launch_us": dep_avg_s * 1e6, "timed_launches": dep_n,
-                     "bytes_per_launch": dep_bytes},
+                     "algorithmic_bytes_per_launch": dep_bytes,
+                     "algorithmic_bytes_per_particle": 48,
+                     "gamma_psi_store_bytes_per_launch": 16.0 * np_slice,
+                     "traffic_over_algorithmic": traffic / dep_bytes if traffic else None,
+                     "measured_copy_GBs": copy_gbs,
+                     "frac_of_measured_copy": achieved / copy_gbs},
         "step_roofline": {"bytes_per_step": step_bytes,
                           "achieved_GBs": step_bytes * steps_per_s / 1e9,
                           "frac": step_bytes * steps_per_s / 1e9 / HBM_PEAK_GBS},
@@ -395,8 +502,7 @@ def main():
         sp.step3d_begin()
         for j in range(1, j0):
             sp.slice(j)
-        cores = min(os.cpu_count() or 1, 16)
-        result["cpu_baseline"] = cpu_baseline(sp, prof["plain"], j0, args.cpu_slices, cores)
+        result["cpu_baseline"] = cpu_baseline(sp, prof["plain"], j0, args.cpu_slices)
     legs["plain"].close()
     print(json.dumps(result))
 

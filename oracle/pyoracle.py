@@ -22,6 +22,10 @@ NCOMP = [1, 3, 3, 3, 3, 3, 3, 2, 3, 2, 1, 1]
 FIELD_NAMES = ["psi", "e", "e_spe", "b", "b_spe", "b_beam", "cu", "acu", "amu",
                "dcu", "q_spe", "q_beam"]
 FK_PSI, FK_EZ, FK_BZ, FK_BT, FK_BPLUS, FK_BMINUS = range(6)
+# QPAD's timer event names (sysutil_module.f03:307-330), in ora_timers order
+TIMER_NAMES = ["solve psi", "solve bz", "solve beam bt", "solve plasma bt", "solve ez",
+               "solve plasma et", "set source", "copy slices", "arithmetics", "deposit 2D particles",
+               "push 2D particles", "move 2D particles", "deposit 3D particles", "push 3D particles"]
 BND_ZERO, BND_OPEN = 2, 3
 
 _dp = C.POINTER(C.c_double)
@@ -147,7 +151,20 @@ def load_library(path):
     L.ora_philox4x32.argtypes = [C.POINTER(C.c_uint32), C.POINTER(C.c_uint32),
                                  C.POINTER(C.c_uint32)]
     L.ora_normal4.argtypes = [C.c_uint64, C.c_uint64, C.c_uint32, _dp]
+    L.ora_timing.argtypes = [C.c_int]
+    L.ora_timers.argtypes = [_dp]
     return L
+
+
+def timers(L=None):
+    """seconds per QPAD timer event since timing(True) (this process)"""
+    out = np.zeros(len(TIMER_NAMES))
+    (L or lib()).ora_timers(ptr(out))
+    return dict(zip(TIMER_NAMES, out.tolist()))
+
+
+def timing(enable, L=None):
+    (L or lib()).ora_timing(int(enable))
 
 
 def ptr(a):

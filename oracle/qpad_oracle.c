@@ -8,13 +8,41 @@
  * right, as gfortran evaluates it without fast-math) so that the restatement
  * is faithful operation by operation.  Compile with -ffp-contract=off.
  */
+#define _POSIX_C_SOURCE 199309L   /* clock_gettime under -std=c11 */
 #include "qpad_oracle.h"
 
 #include <math.h>
 #include <stdlib.h>
 #include <string.h>
+#include <time.h>
 
 #define PI_DB (4.0 * atan(1.0)) /* source/param.f03:36 */
+
+/* Timers by QPAD's own event names (source/sysutil_module.f03:307-330 and
+   the start_tprof calls of the routines restated here), for the CPU
+   baseline's breakdown.  Off unless ora_timing(1); they do not touch the
+   arithmetic. */
+static int g_timing = 0;
+static double g_timer[ORA_NTIMER];
+static double now_s(void)
+{
+    struct timespec t;
+    clock_gettime(CLOCK_MONOTONIC, &t);
+    return (double)t.tv_sec + 1e-9 * (double)t.tv_nsec;
+}
+#define TSTART() double t_ = g_timing ? now_s() : 0.0
+#define TSTOP(k) do { if (g_timing) { double e_ = now_s(); g_timer[k] += e_ - t_; t_ = e_; } } while (0)
+
+void ora_timing(int enable)
+{
+    g_timing = enable;
+    for (int k = 0; k < ORA_NTIMER; k++) g_timer[k] = 0.0;
+}
+
+void ora_timers(double *out)
+{
+    for (int k = 0; k < ORA_NTIMER; k++) out[k] = g_timer[k];
+}
 
 static inline int kre(int m) { return m == 0 ? 0 : 2 * m - 1; }
 static inline int kim(int m) { return 2 * m; }
@@ -1428,12 +1456,14 @@ void ora_step3d_begin(ora_sim *s)
     size_t n1 = (size_t)s->nm * s->ld, ns = (size_t)s->g.nxi + 2;
     memset(s->h_q_beam, 0, ns * n1 * sizeof(double));   /* q_beam%as(0.0) */
     memset(s->h_q_spe, 0, ns * n1 * sizeof(double));    /* q_spe%as(0.0)  */
+    TSTART();
     for (int ib = 0; ib < s->nbeams; ib++) {            /* beam3d%qdp     */
         ora_beam *bm = &s->bm[ib];
         memset(bm->q_own_hist, 0, ns * n1 * sizeof(double));
         ora_beam_qdeposit(s, ib, bm->q_own_hist);
         for (size_t t = 0; t < ns * n1; t++) s->h_q_beam[t] = s->h_q_beam[t] + bm->q_own_hist[t];
     }
+    TSTOP(ORA_T_DEPOSIT_3D);
     int zero[] = {ORA_B, ORA_E, ORA_B_SPE, ORA_E_SPE, ORA_PSI, ORA_CU, ORA_ACU, ORA_AMU};
     for (int t = 0; t < 8; t++) memset(s->f[zero[t]], 0, n1 * s->ncomp[zero[t]] * sizeof(double));
 }
@@ -1453,8 +1483,11 @@ void ora_slice(ora_sim *s, int j, int *n_iter)
     double *cu = s->f[ORA_CU], *acu = s->f[ORA_ACU], *amu = s->f[ORA_AMU];
     double *dcu = s->f[ORA_DCU], *q_spe = s->f[ORA_Q_SPE], *q_beam = s->f[ORA_Q_BEAM];
 
+    TSTART();
     memcpy(q_beam, s->h_q_beam + (size_t)j * n1, n1 * sizeof(double));  /* copy_slice 2to1 */
+    TSTOP(ORA_T_COPY_SLICES);
     ora_solve_bt_beam(s, q_beam, b_beam);
+    TSTOP(ORA_T_SOLVE_BEAM_BT);
     memset(q_spe, 0, n1 * sizeof(double));
     for (int k = 0; k < s->nspecies; k++) {                              /* species2d%qdp */
         ora_species *sp = &s->sp[k];
@@ -1463,8 +1496,11 @@ void ora_slice(ora_sim *s, int j, int *n_iter)
         add_into(q_spe, sp->q_own, n1);
         if (sp->cfg.neutralized) add_into(q_spe, sp->qn, n1);
     }
+    TSTOP(ORA_T_DEPOSIT_2D);
     ora_solve_psi(s, q_spe, psi);
+    TSTOP(ORA_T_SOLVE_PSI);
     ora_solve_bz(s, cu, b_spe);
+    TSTOP(ORA_T_SOLVE_BZ);
 
     int l, nit = 0;
     double rel = 0.0, ab = 0.0;
@@ -1472,8 +1508,11 @@ void ora_slice(ora_sim *s, int j, int *n_iter)
         nit = l;
         ora_conv_record(s, b_spe);
         for (size_t t = 0; t < 3 * n1; t++) b[t] = b_spe[t] + b_beam[t];
+        TSTOP(ORA_T_ARITHMETICS);
         ora_solve_ez(s, cu, e);
+        TSTOP(ORA_T_SOLVE_EZ);
         ora_solve_et(s, b, psi, e);
+        TSTOP(ORA_T_SOLVE_PLASMA_ET);
         memset(cu, 0, 3 * n1 * sizeof(double));
         memset(acu, 0, 2 * n1 * sizeof(double));
         memset(amu, 0, 3 * n1 * sizeof(double));
@@ -1487,10 +1526,15 @@ void ora_slice(ora_sim *s, int j, int *n_iter)
             add_into(acu, sp->acu_own, 2 * n1);
             add_into(amu, sp->amu_own, 3 * n1);
         }
+        TSTOP(ORA_T_DEPOSIT_2D);
         ora_solve_djdxi(s, acu, amu, dcu);
+        TSTOP(ORA_T_SET_SOURCE);
         ora_solve_bt_iter(s, dcu, cu, b_spe);
+        TSTOP(ORA_T_SOLVE_PLASMA_BT);
         ora_solve_bz(s, cu, b_spe);
+        TSTOP(ORA_T_SOLVE_BZ);
         ora_conv_compare(s, b_spe, &rel, &ab);
+        TSTOP(ORA_T_ARITHMETICS);
         if (rel < s->g.iter_reltol || ab < s->g.iter_abstol) break;
     }
     s->last_iters = nit;
@@ -1508,24 +1552,32 @@ void ora_slice(ora_sim *s, int j, int *n_iter)
     memcpy(s->h_q_spe + (size_t)j * n1, q_spe, n1 * sizeof(double));
 
     for (size_t t = 0; t < 3 * n1; t++) b[t] = b_spe[t] + b_beam[t];
+    TSTOP(ORA_T_COPY_SLICES);
     ora_solve_et(s, b_spe, psi, e_spe);
+    TSTOP(ORA_T_SOLVE_PLASMA_ET);
     ora_solve_ez(s, cu, e);
+    TSTOP(ORA_T_SOLVE_EZ);
     ora_solve_et(s, b, psi, e);
+    TSTOP(ORA_T_SOLVE_PLASMA_ET);
 
     for (size_t t = 0; t < 2 * n1; t++) dcu[t] = s->g.dxi * dcu[t];      /* dot_f1 */
     for (size_t t = 0; t < 2 * n1; t++) cu[t] = cu[t] + dcu[t];          /* J_perp predictor */
+    TSTOP(ORA_T_ARITHMETICS);
 
     for (int k = 0; k < s->nspecies; k++) {
         ora_species *sp = &s->sp[k];
         ora_push_u_robust(s, sp, e, b, s->g.dxi);
         ora_push_x(sp, s->g.dxi);
+        TSTOP(ORA_T_PUSH_2D);
         ora_update_bound(s, sp);
+        TSTOP(ORA_T_MOVE_2D);
     }
     memcpy(s->h_e + (size_t)j * 3 * n1, e, 3 * n1 * sizeof(double));
     memcpy(s->h_b + (size_t)j * 3 * n1, b, 3 * n1 * sizeof(double));
     memcpy(s->h_psi + (size_t)j * n1, psi, n1 * sizeof(double));
     memcpy(s->h_b_spe + (size_t)j * 3 * n1, b_spe, 3 * n1 * sizeof(double));
     memcpy(s->h_e_spe + (size_t)j * 3 * n1, e_spe, 3 * n1 * sizeof(double));
+    TSTOP(ORA_T_COPY_SLICES);
 }
 
 /* 3D step epilogue: source/simulation_class.f03:485-501 (beam push, species
@@ -1533,7 +1585,9 @@ void ora_slice(ora_sim *s, int j, int *n_iter)
 void ora_step3d_end(ora_sim *s, int istep)
 {
     (void)istep;
+    TSTART();
     for (int ib = 0; ib < s->nbeams; ib++) ora_beam_push(s, ib);
+    TSTOP(ORA_T_PUSH_3D);
     for (int k = 0; k < s->nspecies; k++) {
         ora_species_inject(s, k);
         ora_species_init_q(s, k);

@@ -212,6 +212,15 @@ ora_species *ora_get_species(ora_sim *s, int isp);
 ora_beam *ora_get_beam(ora_sim *s, int ib);
 int ora_last_iters(const ora_sim *s);
 
+/* timers by QPAD's event names (sysutil_module.f03:307-330; 'set source'
+   is field_src_class.f03:293): seconds accumulated since ora_timing(1) */
+enum { ORA_T_SOLVE_PSI = 0, ORA_T_SOLVE_BZ, ORA_T_SOLVE_BEAM_BT, ORA_T_SOLVE_PLASMA_BT,
+       ORA_T_SOLVE_EZ, ORA_T_SOLVE_PLASMA_ET, ORA_T_SET_SOURCE, ORA_T_COPY_SLICES,
+       ORA_T_ARITHMETICS, ORA_T_DEPOSIT_2D, ORA_T_PUSH_2D, ORA_T_MOVE_2D,
+       ORA_T_DEPOSIT_3D, ORA_T_PUSH_3D, ORA_NTIMER };
+void ora_timing(int enable);
+void ora_timers(double *out);
+
 /* Philox4x32-10 (Salmon et al., SC'11) and the Gaussian draw built on it */
 void ora_philox4x32(const uint32_t ctr[4], const uint32_t key[2],
                     uint32_t out[4]);
