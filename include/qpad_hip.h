@@ -151,6 +151,9 @@ int qpad_abi_version(void);
 int qpad_create(const qpad_grid *grid, qpad_ctx **out);
 /* simulation%del (simulation_class.f03:200-224) */
 int qpad_destroy(qpad_ctx *ctx);
+/* the grid the context was created with (options_class.f03:44-117 sizes
+   and simulation settings), for host-side writers and drivers */
+int qpad_get_grid(qpad_ctx *ctx, qpad_grid *out);
 
 /* species2d%new: inject, initial charge, neutralising background
    (species2d_class.f03:75-134, part2d_class.f03:108-142,
@@ -171,6 +174,18 @@ int qpad_species_qdp(qpad_ctx *ctx, int32_t sp);
    one is refused with QPAD_E_INVALID.  The array is shared by every call on
    a GPU: calls that overlap on one GPU race. */
 int qpad_species_qdp_noalloc(qpad_ctx *ctx, int32_t sp);
+/* Per-species charge (species2d%q, its history written by species2d%cbq,
+   species2d_class.f03:186-208,396-410), which QPAD's species charge_cyl_m
+   diagnostic writes.  Off by default (it costs two small launches per slice
+   and species and one history per species); qpad_species_charge_history(1)
+   turns it on for every species, present and later.  With it on,
+   qpad_slice / qpad_step3d keep it, qpad_species_qdp sets the species' q,
+   qpad_species_cbq(sp, pos) adds the species' J_z and stores history slice
+   pos, and qpad_get_species_charge reads history slice `slice`
+   (1 x nm x (nr+2), as qpad_get_hist). */
+int qpad_species_charge_history(qpad_ctx *ctx, int32_t enable);
+int qpad_species_cbq(qpad_ctx *ctx, int32_t sp, int32_t pos);
+int qpad_get_species_charge(qpad_ctx *ctx, int32_t sp, int32_t slice, double *out);
 /* species2d%amjdp: deposit cu, acu (dJ/dxi part) and amu of species sp into
    the totals using fields e and b (species2d_class.f03:233-280,
    part2d_class.f03:746-1010, robust pusher) */

@@ -319,6 +319,13 @@ class Oracle:
         ids[:n] = arrs.get("id", np.arange(n))
         self.species_raw(isp).contents.npp = n
 
+    def species_q_hist(self, isp):
+        """species isp's charge history (species2d%q written by cbq), [nxi + 2, 1, nm, ld]"""
+        s = self.species_raw(isp).contents
+        ns = self.grid.nxi + 2
+        n = ns * self.nm * self.ld
+        return np.ctypeslib.as_array(s.q_hist, shape=(n,)).reshape(ns, 1, self.nm, self.ld)
+
     def species_q_own(self, isp):
         s = self.species_raw(isp).contents
         n = self.nm * self.ld

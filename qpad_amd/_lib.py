@@ -95,10 +95,14 @@ SIGNATURES = {
     "qpad_abi_version": (C.c_int, []),
     "qpad_create": (C.c_int, [C.POINTER(Grid), C.POINTER(C.c_void_p)]),
     "qpad_destroy": (C.c_int, [_CTX]),
+    "qpad_get_grid": (C.c_int, [_CTX, C.POINTER(Grid)]),
     "qpad_species_new": (C.c_int, [_CTX, C.POINTER(SpeciesCfg), C.POINTER(C.c_int32)]),
     "qpad_species_renew": (C.c_int, [_CTX, C.c_int32]),
     "qpad_species_qdp": (C.c_int, [_CTX, C.c_int32]),
     "qpad_species_qdp_noalloc": (C.c_int, [_CTX, C.c_int32]),
+    "qpad_species_charge_history": (C.c_int, [_CTX, C.c_int32]),
+    "qpad_species_cbq": (C.c_int, [_CTX, C.c_int32, C.c_int32]),
+    "qpad_get_species_charge": (C.c_int, [_CTX, C.c_int32, C.c_int32, _dp]),
     "qpad_species_amjdp": (C.c_int, [_CTX, C.c_int32, C.c_double]),
     "qpad_species_push": (C.c_int, [_CTX, C.c_int32, C.c_double]),
     "qpad_field_solve_psi": (C.c_int, [_CTX]),

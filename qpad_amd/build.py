@@ -56,7 +56,36 @@ def build(jobs=4, verbose=False, force=False):
     if r.returncode != 0:
         raise RuntimeError(f"link failed:\n{r.stdout}\n{r.stderr}")
     os.replace(tmp, LIB)
+    build_diag()
     return LIB
+
+
+DIAG_SRC = os.path.join(HERE, "diag", "qpad_diag.cpp")
+DIAG_LIB = os.path.join(HERE, "libqpad_diag.so")
+HDF5_ROOT = os.environ.get("QPAD_HDF5_ROOT", "/opt/conda")
+
+
+def build_diag():
+    """libqpad_diag.so: QPAD's HDF5 diagnostics (host C++ on HDF5 1.10 and
+    libqpad_hip's C ABI).  Skipped, with a message, where no HDF5 is
+    installed; the core library does not depend on it."""
+    hdr = os.path.join(HDF5_ROOT, "include", "hdf5.h")
+    if not os.path.exists(hdr):
+        print(f"libqpad_diag.so not built: no HDF5 headers at {hdr}", file=sys.stderr)
+        return None
+    hlib = os.path.join(HDF5_ROOT, "lib")
+    tmp = DIAG_LIB + ".tmp"
+    # libstdc++ and libgcc linked in, and a RUNPATH (direct dependencies
+    # only): the HDF5 prefix also holds an older libstdc++, which must not
+    # shadow the system one that the ROCm runtime needs
+    cmd = ["g++", "-O2", "-fPIC", "-shared", "-std=c++17", "-Wall", "-I", os.path.join(HDF5_ROOT, "include"),
+           DIAG_SRC, "-o", tmp, "-static-libstdc++", "-static-libgcc", "-L", HERE, "-lqpad_hip", "-L", hlib,
+           "-lhdf5", "-Wl,--enable-new-dtags", f"-Wl,-rpath,$ORIGIN:{hlib}"]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"libqpad_diag.so build failed:\n{r.stdout}\n{r.stderr}")
+    os.replace(tmp, DIAG_LIB)
+    return DIAG_LIB
 
 
 if __name__ == "__main__":

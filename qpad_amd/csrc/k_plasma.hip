@@ -897,6 +897,50 @@ __global__ void k_axpy(double *y, const double *x, int64_t n)
     if (t < n) y[t] = y[t] + x[t];
 }
 
+// ---------------------------------------------------------------------------
+// species2d%q and cbq (species2d_class.f03:186-208, :396-410): the species'
+// own rho - Jz (its deposit with the axis rules and 1/r; the neutralising
+// background enters only the total), then J_z of its last executed current
+// deposit added, and the sum copied into the species' history slice
+// ---------------------------------------------------------------------------
+int launch_species_q_start(qpad_ctx *c, int s)
+{
+    return launch_combine_q(c, s, c->sp[s].qsl, 1.0, false);
+}
+
+// thread per (node i = 0..nr+1, mode array k): q += J_z of the species
+// (comp 2 of its current-deposit partials, axis x8 at m = 0 and 0 above),
+// then history slice = q
+__global__ void k_species_cbq(const double *P0, const double *P1, const double *O0, const double *O1,
+                              const int32_t *os, double *q, double *hist, int nr, int nm, int ld)
+{
+    const int64_t tid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (tid >= (int64_t)nm * (nr + 2)) return;
+    const int i = (int)(tid % (nr + 2)), k = (int)(tid / (nr + 2));
+    const int NV = 8 * nm, v = 2 * nm + k;
+    double val = 0.0;
+    if (i >= 1 && i <= nr) val += cell_partial(P0, O0, os, i - 1, NV, v);
+    if (i >= 2) val += cell_partial(P1, O1, os, i - 2, NV, v);
+    if (i == 0) val = 0.0;
+    else if (i == 1) val = (k == 0) ? 8.0 * val : 0.0;
+    else val = val * (1.0 / (double)(i - 1));
+    const size_t o = (size_t)k * ld + i;
+    const double x = q[o] + val;
+    q[o] = x;
+    hist[o] = x;
+}
+
+int launch_species_cbq(qpad_ctx *c, int s, int j)
+{
+    const Species &sp = c->sp[s];
+    const int64_t n = (int64_t)c->g.nm * (c->g.nr + 2);
+    double *hist = sp.h_q + (size_t)j * c->g.nm * c->g.ld;
+    hipLaunchKernelGGL(k_species_cbq, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, c->stream, sp.P0, sp.P1,
+                       sp.O0, sp.O1, sp.ov_start, sp.qsl, hist, c->g.nr, c->g.nm, c->g.ld);
+    QPAD_HIP_CHECK(hipGetLastError());
+    return 0;
+}
+
 int launch_add_qn_into_qspe(qpad_ctx *c, int s)
 {
     int64_t n = (int64_t)c->g.nm * c->g.ld;

@@ -101,6 +101,12 @@ inline int conv_blocks(int nr) { return (nr + 2 + 255) / 256; }
 int launch_push(qpad_ctx *c, int sp);            // push_u + push_x + keys
 int launch_push_all(qpad_ctx *c);                // the same for every species in one launch
 int launch_add_qn_into_qspe(qpad_ctx *c, int sp);
+// per-species charge (species2d%q, cbq): q of species sp from its charge
+// deposit partials (at the slice start, before the current deposit reuses
+// the rows), and at the slice end its J_z added and the result stored in
+// history slice j
+int launch_species_q_start(qpad_ctx *c, int sp);
+int launch_species_cbq(qpad_ctx *c, int sp, int j);
 
 // ---- beam (k_beam.hip) ----
 int launch_beam_inject(qpad_ctx *c, int ib);

@@ -235,6 +235,19 @@ static void species_free(Species &sp)
     dfree(sp.cell_start); dfree(sp.qn); dfree(sp.P0); dfree(sp.P1);
     dfree(sp.ov_start); dfree(sp.ov_desc); dfree(sp.O0); dfree(sp.O1);
     dfree(sp.hist); dfree(sp.tot);
+    dfree(sp.qsl); dfree(sp.h_q);
+}
+
+// the per-species charge arrays of species s (zeroed)
+static int species_q_alloc(qpad_ctx *c, int s)
+{
+    Species &sp = c->sp[s];
+    if (sp.h_q) return 0;
+    const size_t n1 = f1_size(c, 1), nh = ((size_t)c->g.nxi + 2) * n1;
+    TRY(dalloc(&sp.qsl, n1));
+    TRY(dalloc(&sp.h_q, nh));
+    TRY(zero_dev(c, sp.qsl, n1));
+    return zero_dev(c, sp.h_q, nh);
 }
 
 static void beam_free(Beam &bm)
@@ -466,6 +479,7 @@ int qpad_species_new(qpad_ctx *c, const qpad_species_cfg *cfg, int32_t *sp_id)
     if (!rc) rc = dalloc(&sp.hist, hist_ints);
     if (!rc) rc = dalloc(&sp.tot, (size_t)c->g.nr + 1);
     if (!rc) rc = zero_dev(c, sp.qn, f1_size(c, 1));
+    if (!rc && c->species_q) rc = species_q_alloc(c, s);
     if (!rc) rc = launch_inject(c, s);
     if (!rc) rc = species_charge_setup(c, s);
     if (!rc) rc = sync(c);
@@ -736,6 +750,8 @@ static int slice_start(qpad_ctx *c, int j, bool overlap, SliceRun &st)
     const double *seed = nullptr;
     if (net && c->net_pre != j) TRY(seednet_after_epilogue(c, 0, j));
     TRY(launch_qdeposit_all(c));
+    if (c->species_q)
+        for (int s = 0; s < c->nspecies; s++) TRY(launch_species_q_start(c, s));
     TRY(launch_slice_sources(c));
     TRY(launch_start_solves(c));
     // the first kernel that reads B_perp waits for the seed
@@ -762,6 +778,8 @@ static int slice_finish(qpad_ctx *c, SliceRun &st)
     const int j = st.j;
     if (c->adaptive) c->spec_guess = st.known_iters;
     TRY(launch_epilogue(c, j));
+    if (c->species_q)
+        for (int s = 0; s < c->nspecies; s++) TRY(launch_species_cbq(c, s, j));
     const bool net = seednet_active(c);
     if (net) TRY(seednet_mark(c));
     TRY(launch_push_all(c));
@@ -904,9 +922,49 @@ int qpad_species_qdp(qpad_ctx *c, int32_t s)
 {
     if (!c || s < 0 || s >= c->nspecies) return fail(QPAD_E_INVALID, "bad species");
     TRY(launch_qdeposit(c, s));
+    if (c->species_q) TRY(launch_species_q_start(c, s));
     TRY(launch_combine_q(c, s, c->f[QPAD_F_Q_SPE], 1.0, true));
     if (c->sp[s].cfg.neutralized) TRY(launch_add_qn_into_qspe(c, s));
     return sync(c);
+}
+
+static int copy_rows(qpad_ctx *c, double *dev, int ncomp, double *host_out, const double *host_in);
+
+int qpad_species_cbq(qpad_ctx *c, int32_t s, int32_t pos)
+{
+    if (!c || s < 0 || s >= c->nspecies) return fail(QPAD_E_INVALID, "bad species");
+    if (!c->species_q) return fail(QPAD_E_STATE, "per-species charge is off (qpad_species_charge_history)");
+    if (pos < 1 || pos > c->g.nxi) return fail(QPAD_E_INVALID, "slice index out of range");
+    TRY(launch_species_cbq(c, s, pos));
+    return sync(c);
+}
+
+int qpad_species_charge_history(qpad_ctx *c, int32_t enable)
+{
+    if (!c) return fail(QPAD_E_INVALID, "null context");
+    if (enable && !c->species_q) {
+        for (int s = 0; s < c->nspecies; s++) {
+            const int rc = species_q_alloc(c, s);
+            if (rc) {
+                for (int t = 0; t <= s; t++) {
+                    dfree(c->sp[t].qsl);
+                    dfree(c->sp[t].h_q);
+                    c->sp[t].qsl = c->sp[t].h_q = nullptr;
+                }
+                return rc;
+            }
+        }
+    }
+    c->species_q = enable != 0;
+    return sync(c);
+}
+
+int qpad_get_species_charge(qpad_ctx *c, int32_t s, int32_t slice, double *out)
+{
+    if (!c || s < 0 || s >= c->nspecies || !out) return fail(QPAD_E_INVALID, "bad species");
+    if (!c->sp[s].h_q) return fail(QPAD_E_STATE, "per-species charge is off (qpad_species_charge_history)");
+    if (slice < 0 || slice > c->g.nxi + 1) return fail(QPAD_E_INVALID, "bad history request");
+    return copy_rows(c, c->sp[s].h_q + (size_t)slice * f1_size(c, 1), 1, out, nullptr);
 }
 
 int qpad_species_amjdp(qpad_ctx *c, int32_t s, double dxi)
@@ -1049,6 +1107,24 @@ static int copy_rows(qpad_ctx *c, double *dev, int ncomp, double *host_out, cons
                                         w * sizeof(double), rows, hipMemcpyHostToDevice, c->stream));
     }
     return sync(c);
+}
+
+int qpad_get_grid(qpad_ctx *c, qpad_grid *out)
+{
+    if (!c || !out) return fail(QPAD_E_INVALID, "null argument");
+    const Grid &g = c->g;
+    out->nr = g.nr;
+    out->nxi = g.nxi;
+    out->max_mode = g.M;
+    out->dr = g.dr;
+    out->dxi = g.dxi;
+    out->relax_fac = g.relax_fac;
+    out->bnd = g.bnd;
+    out->iter_max = g.iter_max;
+    out->iter_reltol = g.reltol;
+    out->iter_abstol = g.abstol;
+    out->dt = g.dt;
+    return 0;
 }
 
 int qpad_get_field(qpad_ctx *c, int32_t fid, double *out)

@@ -63,6 +63,10 @@ struct Species {
     double *O0, *O1;
     int n_ov_max;            // rows allocated: npmax / kDepChunk + 1
     int *hist, *tot;         // counting-sort workspace: [chunks][nr+1], [nr+1]
+    // species2d%q and its history (cbq), kept when the context tracks the
+    // per-species charge (qpad_species_charge_history): 1 x nm x ld and
+    // [slice 0..nxi+1][nm][ld]
+    double *qsl, *h_q;
 };
 
 struct Beam {
@@ -129,6 +133,7 @@ struct qpad_ctx {
     bool timing;
     qpad::EventPool pools[3];
     bool step_open;
+    bool species_q;           // per-species charge history on (Species::qsl, h_q)
     qpad::SeedNet *net;       // null unless qpad_seednet_create
     int net_pre;              // slice whose seed is already queued (0: none)
 };

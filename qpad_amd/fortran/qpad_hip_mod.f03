@@ -123,6 +123,13 @@ module qpad_hip_mod
       integer(c_int) :: rc
     end function
 
+    function qpad_get_grid(ctx, grid) bind(C, name="qpad_get_grid") result(rc)
+      import :: c_int, c_ptr, qpad_grid
+      type(c_ptr), value :: ctx
+      type(qpad_grid), intent(out) :: grid
+      integer(c_int) :: rc
+    end function
+
     ! species2d%new / %renew / %qdp / %amjdp / %push_u + %push_x
     function qpad_species_new(ctx, cfg, sp_id) bind(C, name="qpad_species_new") result(rc)
       import :: c_int, c_int32_t, c_ptr, qpad_species_cfg
@@ -152,6 +159,29 @@ module qpad_hip_mod
       import :: c_int, c_int32_t, c_ptr
       type(c_ptr), value :: ctx
       integer(c_int32_t), value :: sp
+      integer(c_int) :: rc
+    end function
+
+    ! species2d%q history (cbq) for the species charge_cyl_m diagnostic
+    function qpad_species_charge_history(ctx, enable) bind(C, name="qpad_species_charge_history") result(rc)
+      import :: c_int, c_int32_t, c_ptr
+      type(c_ptr), value :: ctx
+      integer(c_int32_t), value :: enable
+      integer(c_int) :: rc
+    end function
+
+    function qpad_species_cbq(ctx, sp, pos) bind(C, name="qpad_species_cbq") result(rc)
+      import :: c_int, c_int32_t, c_ptr
+      type(c_ptr), value :: ctx
+      integer(c_int32_t), value :: sp, pos
+      integer(c_int) :: rc
+    end function
+
+    function qpad_get_species_charge(ctx, sp, slice, out) bind(C, name="qpad_get_species_charge") result(rc)
+      import :: c_int, c_int32_t, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      integer(c_int32_t), value :: sp, slice
+      real(c_double), intent(out) :: out(*)
       integer(c_int) :: rc
     end function
 
@@ -546,9 +576,9 @@ module qpad_hip_mod
 
   end interface
 
-  public :: qpad_last_error, qpad_abi_version, qpad_create, qpad_destroy, &
+  public :: qpad_last_error, qpad_abi_version, qpad_create, qpad_destroy, qpad_get_grid, &
     qpad_species_new, qpad_species_renew, qpad_species_qdp, qpad_species_qdp_noalloc, &
-    qpad_species_amjdp, &
+    qpad_species_charge_history, qpad_species_cbq, qpad_get_species_charge, qpad_species_amjdp, &
     qpad_species_push, qpad_field_solve_psi, qpad_field_solve_ez, &
     qpad_field_solve_et, qpad_field_solve_bz, qpad_field_solve_bt_beam, &
     qpad_field_solve_djdxi, qpad_field_solve_bt_iter, qpad_field_add_b, &

@@ -145,6 +145,12 @@ class Species2D:
             self.sim.ctx, self.id, n, *[L.dptr(a[k]) for k in ("x1", "x2", "p1", "p2", "p3", "q",
                                                               "gamma", "psi")], L.iptr(ids)))
 
+    def charge_hist(self, s):
+        """species2d%q history slice s (qpad_species_charge_history on)"""
+        out = np.zeros((1, self.sim.nm, self.sim.grid.nr + 2))
+        L.check(self.sim.lib.qpad_get_species_charge(self.sim.ctx, self.id, int(s), L.dptr(out)))
+        return out
+
     def qn(self):
         out = np.zeros((1, self.sim.nm, self.sim.grid.nr + 2))
         L.check(self.sim.lib.qpad_get_species_qn(self.sim.ctx, self.id, L.dptr(out)))
@@ -294,10 +300,30 @@ class Simulation:
         self.tstep += 1
         return it.value
 
-    def run(self, nstep3d=None):
+    def run(self, nstep3d=None, diag=None):
+        """simulation%run (simulation_class.f03:226-520).  With diag (a
+        qpad_amd.diag.Diagnostics): on a fresh run the beam charge is
+        deposited and the step-0 files written (:276-289); each 3D step
+        writes its files after the beam push and before the species are
+        renewed (:487-501)."""
+        n = self.nstep3d if nstep3d is None else nstep3d
         total = 0
-        for _ in range(self.nstep3d if nstep3d is None else nstep3d):
-            total += self.step3d()
+        if diag is None:
+            for _ in range(n):
+                total += self.step3d()
+            return total
+        if self.tstep == 0:
+            L.check(self.lib.qpad_beam_qdp(self.ctx))
+            diag.run(0)
+        for _ in range(n):
+            self.step3d_begin()
+            for j in range(1, self.grid.nxi + 1):
+                total += self.slice(j)
+            L.check(self.lib.qpad_beam_push(self.ctx))
+            self.tstep += 1
+            diag.run(self.tstep)
+            for sp in self.spe:
+                sp.renew()
         return total
 
     def iteration_policy(self, adaptive=True, margin=0):
