@@ -70,6 +70,13 @@ int qpad_diag_species_raw(qpad_ctx *ctx, int32_t sp, int32_t psample, const char
 int qpad_diag_beam_raw(qpad_ctx *ctx, int32_t ib, int32_t psample, const char *dir, int32_t iter,
                        const qpad_diag_box *box);
 
+/* the particle file of a "file" beam (fdist3d_file_class.f03:154-343):
+   datasets x1, x2, x3, p1, p2, p3, q of the root group, each holding n
+   values (rank 1, or rank 2 with one extent 1).  With x1 null only n is
+   returned; otherwise every array must hold *n values. */
+int qpad_diag_read_beam_file(const char *path, int64_t *n, double *x1, double *x2, double *x3, double *p1,
+                             double *p2, double *p3, double *q);
+
 #ifdef __cplusplus
 }
 #endif

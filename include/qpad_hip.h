@@ -231,6 +231,12 @@ int qpad_beam_new(qpad_ctx *ctx, const qpad_beam_cfg *cfg, int32_t *beam_id);
    beam3d_class.f03:121-177, fdist3d_std_class.f03:439-613; the beam holds
    exactly the injected particles (QPAD's npmax is not needed) */
 int qpad_beam_new_std(qpad_ctx *ctx, const qpad_beam_std_cfg *cfg, int32_t *beam_id);
+/* beam3d%new with given particles: the "file" profile
+   (fdist3d_file_class.f03:154-343), whose host side reads and places them
+   (qpad_diag_read_beam_file); x3 relative to the box start */
+int qpad_beam_new_arrays(qpad_ctx *ctx, double qm, double mass, int32_t evol, int64_t n, const double *x1,
+                         const double *x2, const double *x3, const double *p1, const double *p2, const double *p3,
+                         const double *q, int32_t *beam_id);
 
 /* The fused forms of simulation%run (simulation_class.f03:294-512):
    qpad_step3d_begin = beam deposit, beam B for every slice, field zeroing

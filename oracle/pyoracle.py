@@ -192,6 +192,30 @@ def beam_cfg(total_num, total_charge, center, sigma, range_lo, range_hi, qm=-1.0
                    d3(*range_lo), d3(*range_hi), d3(*uth), int(quiet), int(evol), seed)
 
 
+def file_beam_place(raw, beam_center, file_center, lconv, qconv, nr, dr, nxi, dxi):
+    """Particles of a "file" beam in box coordinates, one particle at a time
+    as inject_fdist3d_file does (fdist3d_file_class.f03:300-330): centre on
+    file_center, scale by lconv, shift to beam_center (x3 mirrored), keep
+    0 <= r < nr*dr and 0 <= xi < nxi*dxi in file order, q scaled by qconv.
+    beam_center[2] is relative to the box start (init_fdist3d_file :84-85)."""
+    import math
+    keys = ("x1", "x2", "x3", "p1", "p2", "p3", "q")
+    out = {k: [] for k in keys}
+    n = len(np.asarray(raw["x1"]).ravel())
+    cols = {k: np.asarray(raw[k], dtype=np.float64).ravel() for k in keys}
+    for i in range(n):
+        x = [(float(cols[k][i]) - file_center[d]) * lconv for d, k in enumerate(("x1", "x2", "x3"))]
+        x = [beam_center[0] + x[0], beam_center[1] + x[1], beam_center[2] - x[2]]
+        r = math.sqrt(x[0] ** 2 + x[1] ** 2)
+        if 0.0 <= r < nr * dr and 0.0 <= x[2] < nxi * dxi:
+            for d, k in enumerate(("x1", "x2", "x3")):
+                out[k].append(x[d])
+            for k in ("p1", "p2", "p3"):
+                out[k].append(float(cols[k][i]))
+            out["q"].append(float(cols["q"][i]) * qconv)
+    return {k: np.array(v, dtype=np.float64) for k, v in out.items()}
+
+
 def matrix(kind, mode, nr, dr, relax_fac, bnd=BND_OPEN, L=None):
     a, b, c = (np.zeros(nr) for _ in range(3))
     (L or lib()).ora_matrix(kind, mode, nr, dr, relax_fac, bnd, ptr(a), ptr(b), ptr(c))

@@ -118,6 +118,8 @@ SIGNATURES = {
     "qpad_beam_push": (C.c_int, [_CTX]),
     "qpad_beam_new": (C.c_int, [_CTX, C.POINTER(BeamCfg), C.POINTER(C.c_int32)]),
     "qpad_beam_new_std": (C.c_int, [_CTX, C.POINTER(BeamStdCfg), C.POINTER(C.c_int32)]),
+    "qpad_beam_new_arrays": (C.c_int, [_CTX, C.c_double, C.c_double, C.c_int32, C.c_int64] + [_dp] * 7
+                             + [C.POINTER(C.c_int32)]),
     "qpad_step3d_begin": (C.c_int, [_CTX]),
     "qpad_slice": (C.c_int, [_CTX, C.c_int32, C.POINTER(C.c_int32)]),
     "qpad_step3d_end": (C.c_int, [_CTX]),

@@ -60,7 +60,7 @@ def build(jobs=4, verbose=False, force=False):
     return LIB
 
 
-DIAG_SRC = os.path.join(HERE, "diag", "qpad_diag.cpp")
+DIAG_SRC = os.path.join(CSRC, "qpad_diag.cpp")
 DIAG_LIB = os.path.join(HERE, "libqpad_diag.so")
 HDF5_ROOT = os.environ.get("QPAD_HDF5_ROOT", "/opt/conda")
 

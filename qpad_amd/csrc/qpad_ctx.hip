@@ -580,6 +580,34 @@ int qpad_beam_new(qpad_ctx *c, const qpad_beam_cfg *cfg, int32_t *beam_id)
     return beam_finish(c, b, rc, beam_id);
 }
 
+int qpad_beam_new_arrays(qpad_ctx *c, double qm, double mass, int32_t evol, int64_t n, const double *x1,
+                         const double *x2, const double *x3, const double *p1, const double *p2, const double *p3,
+                         const double *q, int32_t *beam_id)
+{
+    if (!c) return fail(QPAD_E_INVALID, "null argument");
+    if (c->nbeams >= kMaxBeams) return fail(QPAD_E_INVALID, "too many beams");
+    if (mass == 0.0 || qm == 0.0) return fail(QPAD_E_INVALID, "beam q and m must be non-zero");
+    if (n < 0 || n > 0x3fffffff) return fail(QPAD_E_INVALID, "bad beam particle count");
+    if (n > 0 && (!x1 || !x2 || !x3 || !p1 || !p2 || !p3 || !q)) return fail(QPAD_E_INVALID, "null beam array");
+    if ((int64_t)c->g.nr * c->g.nxi >= 0x7fffffff) return fail(QPAD_E_INVALID, "grid too large for beam cells");
+    const int b = c->nbeams;
+    Beam &bm = c->bm[b];
+    memset((void *)&bm, 0, sizeof(Beam));
+    bm.cfg.qm = qm;
+    bm.cfg.mass = mass;
+    bm.cfg.evol = evol;
+    bm.qbm = qm / mass;
+    int rc = beam_alloc(c, bm, n);
+    const double *srcs[7] = {x1, x2, x3, p1, p2, p3, q};
+    double *dsts[7] = {bm.x1, bm.x2, bm.x3, bm.p1, bm.p2, bm.p3, bm.q};
+    for (int f = 0; !rc && f < 7 && n; f++)
+        if (hipMemcpyAsync(dsts[f], srcs[f], n * sizeof(double), hipMemcpyHostToDevice, c->stream) != hipSuccess)
+            rc = fail(QPAD_E_HIP, "copy of the beam particles failed");
+    if (!rc) rc = sync(c);
+    bm.npp = n;
+    return beam_finish(c, b, rc, beam_id);
+}
+
 int qpad_beam_new_std(qpad_ctx *c, const qpad_beam_std_cfg *cfg, int32_t *beam_id)
 {
     if (!c || !cfg) return fail(QPAD_E_INVALID, "null argument");

@@ -425,4 +425,45 @@ int qpad_diag_beam_raw(qpad_ctx *c, int32_t ib, int32_t psample, const char *dir
     return ok ? 0 : fail(QPAD_E_INVALID, "HDF5 write failed for " + path);
 }
 
+int qpad_diag_read_beam_file(const char *path, int64_t *n, double *x1, double *x2, double *x3, double *p1,
+                             double *p2, double *p3, double *q)
+{
+    if (!path || !n) return fail(QPAD_E_INVALID, "bad beam-file arguments");
+    const hid_t file = H5Fopen(path, H5F_ACC_RDONLY, H5P_DEFAULT);
+    if (file < 0) return fail(QPAD_E_INVALID, std::string("cannot open ") + path);
+    const char *names[7] = {"x1", "x2", "x3", "p1", "p2", "p3", "q"};
+    double *outs[7] = {x1, x2, x3, p1, p2, p3, q};
+    int rc = 0;
+    int64_t count = -1;
+    for (int f = 0; f < 7 && !rc; f++) {
+        const hid_t d = H5Dopen2(file, names[f], H5P_DEFAULT);
+        if (d < 0) {
+            rc = fail(QPAD_E_INVALID, std::string("no dataset ") + names[f] + " in " + path);
+            break;
+        }
+        const hid_t sp = H5Dget_space(d);
+        const int rank = H5Sget_simple_extent_ndims(sp);
+        hsize_t dims[2] = {0, 0};
+        int64_t len = -1;
+        if (rank == 1 || rank == 2) {
+            H5Sget_simple_extent_dims(sp, dims, nullptr);
+            if (rank == 1) len = (int64_t)dims[0];
+            else if (dims[0] == 1 || dims[1] == 1) len = (int64_t)(dims[0] * dims[1]);
+        }
+        if (len < 0) rc = fail(QPAD_E_INVALID, std::string("dataset ") + names[f] + " is not a list of values");
+        else if (count >= 0 && len != count) rc = fail(QPAD_E_INVALID, "the beam datasets differ in length");
+        count = len;
+        if (!rc && x1) {
+            if (len != *n) rc = fail(QPAD_E_INVALID, "array length differs from the file's particle count");
+            else if (len > 0 && H5Dread(d, H5T_NATIVE_DOUBLE, H5S_ALL, H5S_ALL, H5P_DEFAULT, outs[f]) < 0)
+                rc = fail(QPAD_E_INVALID, std::string("reading ") + names[f] + " failed");
+        }
+        H5Sclose(sp);
+        H5Dclose(d);
+    }
+    H5Fclose(file);
+    if (!rc && !x1) *n = count;
+    return rc;
+}
+
 }  // extern "C"

@@ -45,6 +45,7 @@ def load():
             getattr(d, name).argtypes = [C.c_void_p, C.c_int32, C.c_char_p, C.c_int32, C.POINTER(Box)]
         for name in ("qpad_diag_species_raw", "qpad_diag_beam_raw"):
             getattr(d, name).argtypes = [C.c_void_p, C.c_int32, C.c_int32, C.c_char_p, C.c_int32, C.POINTER(Box)]
+        d.qpad_diag_read_beam_file.argtypes = [C.c_char_p, C.POINTER(C.c_int64)] + [L._dp] * 7
         _dlib = d
     return _dlib
 
@@ -67,6 +68,19 @@ def write_grid(path, dataname, data, iteration, box, units="", label=""):
     nxi, nr = a.shape
     check(load().qpad_diag_write_grid(path.encode(), dataname.encode(), L.dptr(a), nxi, nr, int(iteration),
                                       C.byref(box), units.encode(), label.encode()))
+
+
+BEAM_FILE_KEYS = ("x1", "x2", "x3", "p1", "p2", "p3", "q")
+
+
+def read_beam_file(path):
+    """the datasets x1..x3, p1..p3, q of a "file" beam's particle file
+    (fdist3d_file_class.f03:200-290) as float64 arrays"""
+    d, n = load(), C.c_int64(0)
+    check(d.qpad_diag_read_beam_file(path.encode(), C.byref(n), *([None] * 7)))
+    out = {k: np.zeros(n.value) for k in BEAM_FILE_KEYS}
+    check(d.qpad_diag_read_beam_file(path.encode(), C.byref(n), *[L.dptr(out[k]) for k in BEAM_FILE_KEYS]))
+    return out
 
 
 def _entries(section):

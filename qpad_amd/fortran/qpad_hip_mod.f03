@@ -285,6 +285,19 @@ module qpad_hip_mod
       integer(c_int) :: rc
     end function
 
+    ! beam3d%new with given particles (the "file" profile)
+    function qpad_beam_new_arrays(ctx, qm, mass, evol, n, x1, x2, x3, p1, p2, p3, q, beam_id) &
+        bind(C, name="qpad_beam_new_arrays") result(rc)
+      import :: c_int, c_int32_t, c_int64_t, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      real(c_double), value :: qm, mass
+      integer(c_int32_t), value :: evol
+      integer(c_int64_t), value :: n
+      real(c_double), intent(in) :: x1(*), x2(*), x3(*), p1(*), p2(*), p3(*), q(*)
+      integer(c_int32_t), intent(out) :: beam_id
+      integer(c_int) :: rc
+    end function
+
     ! beam3d%new with the lattice ("standard") profile
     function qpad_beam_new_std(ctx, cfg, beam_id) bind(C, name="qpad_beam_new_std") result(rc)
       import :: c_int, c_int32_t, c_ptr, qpad_beam_std_cfg
@@ -582,7 +595,7 @@ module qpad_hip_mod
     qpad_species_push, qpad_field_solve_psi, qpad_field_solve_ez, &
     qpad_field_solve_et, qpad_field_solve_bz, qpad_field_solve_bt_beam, &
     qpad_field_solve_djdxi, qpad_field_solve_bt_iter, qpad_field_add_b, &
-    qpad_convergence_tester, qpad_beam_qdp, qpad_beam_push, qpad_beam_new, qpad_beam_new_std, &
+    qpad_convergence_tester, qpad_beam_qdp, qpad_beam_push, qpad_beam_new, qpad_beam_new_std, qpad_beam_new_arrays, &
     qpad_step3d_begin, qpad_slice, qpad_step3d_end, qpad_step3d, &
     qpad_get_field, qpad_set_field, qpad_get_hist, qpad_set_hist, &
     qpad_species_count, qpad_get_species, qpad_set_species, &

@@ -5,8 +5,11 @@ source/input_class.f03:49-104 and the init routines it feeds) onto the C ABI
 structures.  Only the features on the accelerated path are accepted; anything
 else raises QpadError, as QPAD's write_err would stop on an invalid deck.
 """
+import dataclasses
 import json
 import math
+
+import numpy as np
 
 from ._lib import (BND_OPEN, BND_ZERO, GEOM_CARTESIAN, GEOM_CYLINDRICAL, OFFSET_MAX, PROF_GAUSSIAN,
                    PROF_PIECEWISE, PROF_UNIFORM, PUSH2_ROBUST, PW_MAX, BeamCfg, BeamStdCfg, Grid, QpadError,
@@ -99,9 +102,9 @@ def ctypes_d3(v):
 
 def beam_from_deck(bm, z0, seed=1):
     """beam(i) section: profile_type "standard" (the default: the lattice of
-    fdist3d_std_class.f03:150-436) or "random" (fdist3d_rnd_class.f03:
-    104-356).  xi positions are made relative to the box start, as QPAD
-    stores them."""
+    fdist3d_std_class.f03:150-436), "random" (fdist3d_rnd_class.f03:
+    104-356) or "file" (fdist3d_file_class.f03:52-343).  xi positions are
+    made relative to the box start, as QPAD stores them."""
     kind = _get(bm, "profile_type", "standard", required=False)
     if _get(bm, "push_type", "reduced", required=False) != "reduced":
         raise QpadError('only the "reduced" beam pusher is on the GPU path')
@@ -109,8 +112,10 @@ def beam_from_deck(bm, z0, seed=1):
         raise QpadError("spin push is not on the GPU path")
     if kind == "standard":
         return _lattice_beam(bm, z0, seed)
+    if kind == "file":
+        return _file_beam(bm, z0)
     if kind != "random":
-        raise QpadError(f"beam profile_type '{kind}' is not on the GPU path")
+        raise QpadError(f"Invalid beam profile type '{kind}'")
     if _get(bm, "geometry", "cartesian", required=False) != "cartesian":
         raise QpadError("only Cartesian random beams are on the GPU path")
     if list(_get(bm, "profile")) != ["gaussian", "gaussian", "gaussian"]:
@@ -128,6 +133,51 @@ def beam_from_deck(bm, z0, seed=1):
                    ctypes_d3(_get(bm, "uth", [0.0, 0.0, 0.0], required=False)),
                    int(bool(_get(bm, "quiet_start", False, required=False))),
                    int(bool(_get(bm, "evolution", True, required=False))), int(seed))
+
+
+@dataclasses.dataclass
+class FileBeam:
+    """A beam read from a particle file (init_fdist3d_file,
+    fdist3d_file_class.f03:52-130): datasets x1..x3, p1..p3, q of the file's
+    root group; beam_center[2] relative to the box start."""
+    filename: str
+    beam_center: tuple
+    file_center: tuple
+    length_conv: float
+    charge_conv: float
+    qm: float
+    mass: float
+    evol: bool
+
+
+def _file_beam(bm, z0):
+    if _get(bm, "has_spin", False, required=False):
+        raise QpadError("spin is not on the GPU path")
+    bc = [float(v) for v in _get(bm, "beam_center")]
+    bc[2] -= z0
+    return FileBeam(str(_get(bm, "filename")), tuple(bc), tuple(float(v) for v in _get(bm, "file_center")),
+                    float(_get(bm, "length_conv_fac", 1.0, required=False)),
+                    float(_get(bm, "charge_conv_fac", 1.0, required=False)), float(_get(bm, "q")),
+                    float(_get(bm, "m")), bool(_get(bm, "evolution", True, required=False)))
+
+
+def place_file_beam(raw, fb, nr, dr, nxi, dxi):
+    """Particles of the file in box coordinates (inject_fdist3d_file,
+    fdist3d_file_class.f03:300-330): x_perp = center + (x - file_center) *
+    length_conv, xi = center_3 - (x3 - file_center_3) * length_conv (the
+    file's x3 grows toward the head), q * charge_conv; only particles with
+    0 <= r < nr*dr and 0 <= xi < nxi*dxi are kept, in file order."""
+    c, f, k = fb.beam_center, fb.file_center, fb.length_conv
+    x1 = c[0] + (np.asarray(raw["x1"], dtype=np.float64).ravel() - f[0]) * k
+    x2 = c[1] + (np.asarray(raw["x2"], dtype=np.float64).ravel() - f[1]) * k
+    x3 = c[2] - (np.asarray(raw["x3"], dtype=np.float64).ravel() - f[2]) * k
+    r = np.sqrt(x1 * x1 + x2 * x2)
+    keep = (r >= 0.0) & (r < nr * dr) & (x3 >= 0.0) & (x3 < nxi * dxi)
+    out = {"x1": x1[keep], "x2": x2[keep], "x3": x3[keep]}
+    for name in ("p1", "p2", "p3"):
+        out[name] = np.asarray(raw[name], dtype=np.float64).ravel()[keep]
+    out["q"] = np.asarray(raw["q"], dtype=np.float64).ravel()[keep] * fb.charge_conv
+    return out
 
 
 _PROFILES = {"uniform": PROF_UNIFORM, "gaussian": PROF_GAUSSIAN, "piecewise-linear": PROF_PIECEWISE}

@@ -11,7 +11,7 @@ import ctypes as C
 import numpy as np
 
 from . import _lib as L
-from .input import beam_from_deck, grid_from_deck, species_from_deck
+from .input import FileBeam, beam_from_deck, grid_from_deck, place_file_beam, species_from_deck
 
 
 class Field:
@@ -162,14 +162,22 @@ class Species2D:
 
 
 class Beam3D:
-    """beam3d (beam3d_class.f03:51-52): a lattice ("standard") or random
-    Gaussian beam, reduced pusher."""
+    """beam3d (beam3d_class.f03:51-52): a lattice ("standard"), random
+    Gaussian or file beam, reduced pusher."""
 
     def __init__(self, sim, cfg):
         self.sim = sim
         bid = C.c_int32(0)
-        new = sim.lib.qpad_beam_new_std if isinstance(cfg, L.BeamStdCfg) else sim.lib.qpad_beam_new
-        L.check(new(sim.ctx, C.byref(cfg), C.byref(bid)))
+        if isinstance(cfg, FileBeam):
+            from . import diag
+            g = sim.grid
+            a = place_file_beam(diag.read_beam_file(cfg.filename), cfg, g.nr, g.dr, g.nxi, g.dxi)
+            L.check(sim.lib.qpad_beam_new_arrays(sim.ctx, cfg.qm, cfg.mass, int(cfg.evol), len(a["x1"]),
+                                                 *[L.dptr(a[k]) for k in ("x1", "x2", "x3", "p1", "p2", "p3",
+                                                                          "q")], C.byref(bid)))
+        else:
+            new = sim.lib.qpad_beam_new_std if isinstance(cfg, L.BeamStdCfg) else sim.lib.qpad_beam_new
+            L.check(new(sim.ctx, C.byref(cfg), C.byref(bid)))
         self.id = bid.value
         self.cfg = cfg
 

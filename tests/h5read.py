@@ -1,6 +1,7 @@
 """A reader of the HDF5 files the diagnostics write, on the HDF5 C library
 of the image (ctypes; test infrastructure): datasets as numpy arrays,
-attributes as str / float / int, and the objects of a group."""
+attributes as str / float / int, and the objects of a group; and a writer
+of plain double datasets (the particle files of "file" beams)."""
 import ctypes as C
 import os
 
@@ -31,7 +32,11 @@ def lib():
                 ("H5Tclose", herr_t, [hid_t]), ("H5Aget_num_attrs", C.c_int, [hid_t]),
                 ("H5Gget_num_objs", herr_t, [hid_t, C.POINTER(hsize_t)]),
                 ("H5Gget_objname_by_idx", C.c_ssize_t, [hid_t, hsize_t, C.c_char_p, C.c_size_t]),
-                ("H5Gopen2", hid_t, [hid_t, C.c_char_p, hid_t]), ("H5Gclose", herr_t, [hid_t])):
+                ("H5Gopen2", hid_t, [hid_t, C.c_char_p, hid_t]), ("H5Gclose", herr_t, [hid_t]),
+                ("H5Fcreate", hid_t, [C.c_char_p, C.c_uint, hid_t, hid_t]),
+                ("H5Screate_simple", hid_t, [C.c_int, C.POINTER(hsize_t), C.POINTER(hsize_t)]),
+                ("H5Dcreate2", hid_t, [hid_t, C.c_char_p, hid_t, hid_t, hid_t, hid_t, hid_t]),
+                ("H5Dwrite", herr_t, [hid_t, hid_t, hid_t, hid_t, hid_t, C.c_void_p])):
             f = getattr(h, name)
             f.restype, f.argtypes = res, args
         _h = h
@@ -40,6 +45,24 @@ def lib():
 
 def _native(name):
     return hid_t.in_dll(lib(), name).value
+
+
+def write_datasets(path, arrays):
+    """a new file at path with one float64 dataset per item of arrays (the
+    array's own shape)"""
+    h = lib()
+    f = h.H5Fcreate(path.encode(), 2, 0, 0)      # H5F_ACC_TRUNC
+    assert f >= 0, path
+    dbl = _native("H5T_NATIVE_DOUBLE_g")
+    for name, a in arrays.items():
+        a = np.ascontiguousarray(a, dtype=np.float64)
+        dims = (hsize_t * a.ndim)(*a.shape)
+        sp = h.H5Screate_simple(a.ndim, dims, None)
+        d = h.H5Dcreate2(f, name.encode(), _native("H5T_IEEE_F64LE_g"), sp, 0, 0, 0)
+        assert d >= 0 and h.H5Dwrite(d, dbl, 0, 0, 0, a.ctypes.data_as(C.c_void_p)) >= 0, name
+        h.H5Dclose(d)
+        h.H5Sclose(sp)
+    h.H5Fclose(f)
 
 
 class File:

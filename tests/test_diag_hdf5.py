@@ -49,3 +49,39 @@ def test_deck_diag_sections_are_read():
 def test_bad_grid_arguments_are_refused(tmp_path):
     with pytest.raises(QpadError, match="bad grid-file"):
         diag.write_grid(str(tmp_path / "x.h5"), "psi", np.zeros((2, 2)), 0, diag.Box(1.0, 0.0, 0.0, 1.0, 1.0))
+
+
+def test_beam_file_reader(tmp_path):
+    """qpad_diag_read_beam_file reads QPAD's particle-file datasets, rank 1
+    or rank 2 with one extent 1 (QPAD reads them as (n, 1) columns,
+    fdist3d_file_class.f03:204-215), and refuses ragged or missing ones."""
+    rng = np.random.default_rng(5)
+    cols = {k: rng.standard_normal(37) for k in diag.BEAM_FILE_KEYS}
+    p1 = str(tmp_path / "flat.h5")
+    h5read.write_datasets(p1, cols)
+    got = diag.read_beam_file(p1)
+    for k in diag.BEAM_FILE_KEYS:
+        assert np.array_equal(got[k], cols[k]), k
+    p2 = str(tmp_path / "column.h5")
+    h5read.write_datasets(p2, {k: v.reshape(1, -1) if k < "p" else v.reshape(-1, 1) for k, v in cols.items()})
+    got = diag.read_beam_file(p2)
+    for k in diag.BEAM_FILE_KEYS:
+        assert np.array_equal(got[k], cols[k]), k
+    p3 = str(tmp_path / "empty.h5")
+    h5read.write_datasets(p3, {k: np.zeros(0) for k in diag.BEAM_FILE_KEYS})
+    assert all(len(v) == 0 for v in diag.read_beam_file(p3).values())
+    ragged = dict(cols, q=cols["q"][:-1])
+    p4 = str(tmp_path / "ragged.h5")
+    h5read.write_datasets(p4, ragged)
+    with pytest.raises(QpadError, match="differ in length"):
+        diag.read_beam_file(p4)
+    p5 = str(tmp_path / "missing.h5")
+    h5read.write_datasets(p5, {k: v for k, v in cols.items() if k != "p2"})
+    with pytest.raises(QpadError, match="no dataset p2"):
+        diag.read_beam_file(p5)
+    p6 = str(tmp_path / "matrix.h5")
+    h5read.write_datasets(p6, dict(cols, x1=np.zeros((3, 4))))
+    with pytest.raises(QpadError, match="not a list"):
+        diag.read_beam_file(p6)
+    with pytest.raises(QpadError, match="cannot open"):
+        diag.read_beam_file(str(tmp_path / "absent.h5"))

@@ -153,3 +153,63 @@ def test_tri_gaussian_deck_lattice_beam_and_step_match_oracle():
         e = rel_err(np.sort(gb[k]), np.sort(ob[k]))
         assert e <= 1e-9, (k, e)
     sim.close()
+
+
+def test_file_beam_deck_matches_oracle(tmp_path):
+    """A "file" beam (fdist3d_file_class.f03:154-343): particles written to
+    an HDF5 file in other units (micrometres, charge in other units, x3
+    growing toward the head, a few outside the box), read back by the
+    library, placed and filtered as QPAD places them; the oracle reads the
+    same file with its own reader and placement.  Same particles, then one
+    3D step slice by slice and the beam push."""
+    import h5read
+    from qpad_amd.input import grid_from_deck
+    src = Simulation(configs.small())
+    b = src.beam[0].get()
+    src.close()
+    deck = configs.small()
+    _, _, z0 = grid_from_deck(deck)
+    bc, fc, lconv, qconv = (0.03, -0.02, z0 + 1.1), (5.0, 7.0, 300.0), 1e-2, 1e-3
+    raw = {"x1": fc[0] + (b["x1"] - bc[0]) / lconv, "x2": fc[1] + (b["x2"] - bc[1]) / lconv,
+           "x3": fc[2] - (b["x3"] - (bc[2] - z0)) / lconv, "p1": b["p1"], "p2": b["p2"], "p3": b["p3"],
+           "q": b["q"] / qconv}
+    n_in = len(raw["x1"])
+    # box r < 4, 0 <= xi < 2.4 (xi of the file centre 1.1): r 4.53 dropped,
+    # xi 3.1 dropped, xi -0.1 dropped, xi 2.3 kept
+    extra = {"x1": [fc[0] + 4.5 / lconv, fc[0], fc[0], fc[0]], "x2": [fc[1]] * 4,
+             "x3": [fc[2], fc[2] - 2.0 / lconv, fc[2] + 1.2 / lconv, fc[2] - 1.2 / lconv], "p1": [0.0] * 4,
+             "p2": [0.0] * 4, "p3": [2000.0] * 4, "q": [-1.0] * 4}
+    raw = {k: np.concatenate([raw[k], extra[k]]).reshape(-1, 1) for k in raw}
+    path = str(tmp_path / "beam.h5")
+    h5read.write_datasets(path, raw)
+    deck["beam"][0] = {"profile_type": "file", "filename": path, "beam_center": list(bc), "file_center": list(fc),
+                       "length_conv_fac": lconv, "charge_conv_fac": qconv, "q": -1.0, "m": 1.0,
+                       "evolution": True}
+    sim = Simulation(deck)
+    ob = twin.oracle_from_deck(deck)
+    gb, rb = sim.beam[0].get(), ob.beam(0)
+    assert len(gb["x1"]) == len(rb["x1"]) == n_in + 1
+    for k in ("x1", "x2", "x3", "p1", "p2", "p3", "q"):
+        assert np.array_equal(np.sort(gb[k]), np.sort(rb[k])), k
+    ob.close()
+    g = sim.grid
+    o = twin.make_twin(sim)
+    sim.step3d_begin()
+    o.step_begin()
+    norm = twin.ErrorNorm()
+    for j in range(1, g.nxi + 1):
+        twin.sync_slice_inputs(sim, o, j)
+        assert sim.slice(j) == o.slice(j), j
+        twin.compare_fields(sim, o, norm, j)
+        twin.compare_species(sim, o, norm, j)
+    print({n: f"{e:.1e}" for n, e in sorted(norm.errors().items())})
+    bad = norm.bad(TOL)
+    assert not bad, f"beyond {TOL}: {bad}"
+    sim.step3d_end()
+    o.step_end(1)
+    gb, ob = sim.beam[0].get(), o.beam(0)
+    assert len(gb["x1"]) == len(ob["x1"])
+    for k in ("x1", "x2", "x3", "p1", "p2", "p3"):
+        e = rel_err(np.sort(gb[k]), np.sort(ob[k]))
+        assert e <= 1e-9, (k, e)
+    sim.close()

@@ -7,8 +7,8 @@ import pytest
 
 from qpad_amd import QpadError, configs
 from qpad_amd._lib import BeamStdCfg
-from qpad_amd.input import (beam_from_deck, grid_from_deck, load_deck, species_from_deck, strip_comments,
-                            tri_gaussian_charge)
+from qpad_amd.input import (FileBeam, beam_from_deck, grid_from_deck, load_deck, place_file_beam,
+                            species_from_deck, strip_comments, tri_gaussian_charge)
 
 # QPAD's own example decks, where the reference tree is present (this
 # container; the GPU runs use configs.tri_gaussian(), which restates one)
@@ -49,7 +49,7 @@ def test_total_charge_definition():
     (lambda d: d["species"][0].update(profile=["parabolic-channel", "uniform"]), "uniform"),
     (lambda d: d["simulation"].update(nlasers=1), "nlasers"),
     (lambda d: d["simulation"].update(field_boundary="periodic"), "boundary"),
-    (lambda d: d["beam"][0].update(profile_type="file"), "profile_type"),
+    (lambda d: d["beam"][0].update(profile_type="gaussian"), "Invalid beam profile type"),
     (lambda d: d["beam"][0].update(profile_type="standard", profile=["analytic"] * 3), "analytic"),
     (lambda d: d["beam"][0].update(push_type="boris"), "reduced"),
 ])
@@ -119,3 +119,55 @@ def test_tri_gaussian_config_restates_the_deck():
             assert {k: v for k, v in a.items() if k != "diag"} == b
     for k, v in mine["simulation"].items():
         assert real["simulation"][k] == v, k
+
+
+def test_file_beam_section():
+    """beam(i) with profile_type "file" (init_fdist3d_file,
+    fdist3d_file_class.f03:79-107): beam_center(3) made relative to the box
+    start, conversion factors defaulting to 1."""
+    bm = {"profile_type": "file", "filename": "beam.h5", "beam_center": [0.1, -0.2, 5.0],
+          "file_center": [1.0, 2.0, 3.0], "q": -1.0, "m": 1.0, "charge_conv_fac": 2.5}
+    fb = beam_from_deck(bm, z0=-2.0)
+    assert isinstance(fb, FileBeam)
+    assert fb.beam_center == (0.1, -0.2, 7.0) and fb.file_center == (1.0, 2.0, 3.0)
+    assert (fb.length_conv, fb.charge_conv, fb.qm, fb.mass, fb.evol) == (1.0, 2.5, -1.0, 1.0, True)
+    with pytest.raises(QpadError, match="spin"):
+        beam_from_deck(dict(bm, has_spin=True), z0=0.0)
+
+
+def test_file_beam_placement_by_hand():
+    """inject_fdist3d_file's transform and box filter (fdist3d_file_class.
+    f03:300-330) on particles whose images are worked out by hand: box
+    r < 4 * 0.5 = 2, 0 <= xi < 10 * 0.25 = 2.5; centre (0.5, 0, 1), file
+    centre (10, 20, 30), length factor 0.5, charge factor 3."""
+    fb = FileBeam("f.h5", (0.5, 0.0, 1.0), (10.0, 20.0, 30.0), 0.5, 3.0, -1.0, 1.0, True)
+    raw = {"x1": [10.0, 12.0, 10.0, 10.0, 10.0, 13.0, 9.0],
+           "x2": [20.0, 20.0, 22.0, 20.0, 20.0, 20.0, 20.0],
+           "x3": [30.0, 30.0, 30.0, 32.0, 28.0, 30.0, 28.0 - 1.0],
+           "p1": [1, 2, 3, 4, 5, 6, 7], "p2": [0] * 7, "p3": [100.0] * 7, "q": [1, 2, 3, 4, 5, 6, 7]}
+    # 0: at the file centre -> (0.5, 0, 1)
+    # 1: x1 + 2 -> x1 = 0.5 + 1 = 1.5, r 1.5 kept
+    # 2: x2 + 2 -> (0.5, 1), r = 1.118 kept
+    # 3: x3 + 2 (toward the head) -> xi = 1 - 1 = 0, on the lower edge, kept
+    # 4: x3 - 2 -> xi = 2 kept
+    # 5: x1 + 3 -> x1 = 2.0, r = 2 on the upper edge, dropped
+    # 6: x1 - 1 -> x1 = 0, x3 - 3 -> xi = 2.5 on the upper edge, dropped
+    a = place_file_beam(raw, fb, nr=4, dr=0.5, nxi=10, dxi=0.25)
+    assert a["x1"].tolist() == [0.5, 1.5, 0.5, 0.5, 0.5]
+    assert a["x2"].tolist() == [0.0, 0.0, 1.0, 0.0, 0.0]
+    assert a["x3"].tolist() == [1.0, 1.0, 1.0, 0.0, 2.0]
+    assert a["p1"].tolist() == [1, 2, 3, 4, 5]
+    assert a["q"].tolist() == [3.0, 6.0, 9.0, 12.0, 15.0]
+
+
+def test_file_beam_placement_matches_the_oracle(oracle_mod):
+    import numpy as np
+    rng = np.random.default_rng(11)
+    n = 4000
+    raw = {k: rng.normal(0.0, 1.0, n) for k in ("x1", "x2", "x3", "p1", "p2", "p3", "q")}
+    fb = FileBeam("f.h5", (0.2, -0.1, 1.5), (0.1, 0.0, -0.3), 0.7, 1e-3, -1.0, 1.0, True)
+    mine = place_file_beam(raw, fb, nr=32, dr=0.05, nxi=64, dxi=0.05)
+    ref = oracle_mod.file_beam_place(raw, fb.beam_center, fb.file_center, 0.7, 1e-3, 32, 0.05, 64, 0.05)
+    assert 0 < len(mine["x1"]) < n
+    for k in ref:
+        assert np.array_equal(mine[k], ref[k]), k

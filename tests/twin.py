@@ -62,7 +62,17 @@ def oracle_from_deck(deck):
         o.add_species(_as_oracle_struct(po.SpeciesCfg, species_from_deck(sp, seed=seed + 1000 * k)))
     for k, bm in enumerate(deck.get("beam", [])[:nbm]):
         cfg = beam_from_deck(bm, z0, seed=seed + 7 * (k + 1))
-        if type(cfg).__name__ == "BeamStdCfg":
+        if type(cfg).__name__ == "FileBeam":
+            import h5read
+            with h5read.File(cfg.filename) as f:
+                raw = {key: f.dataset(key) for key in ("x1", "x2", "x3", "p1", "p2", "p3", "q")}
+            a = po.file_beam_place(raw, cfg.beam_center, cfg.file_center, cfg.length_conv, cfg.charge_conv,
+                                   g.nr, g.dr, g.nxi, g.dxi)
+            ib = o.add_beam(po.beam_cfg(0, 1.0, (0.0, 0.0, 0.0), (1.0, 1.0, 1.0), (-1.0, -1.0, -1.0),
+                                        (1.0, 1.0, 1.0), qm=cfg.qm, mass=cfg.mass, evol=cfg.evol),
+                            npmax=max(1, len(a["x1"])))
+            o.set_beam(ib, a["x1"], a["x2"], a["x3"], a["p1"], a["p2"], a["p3"], a["q"])
+        elif type(cfg).__name__ == "BeamStdCfg":
             o.add_beam_std(_as_oracle_struct(po.BeamStdCfg, cfg))
         else:
             o.add_beam(_as_oracle_struct(po.BeamCfg, cfg))
