@@ -57,6 +57,7 @@ def build(jobs=4, verbose=False, force=False):
         raise RuntimeError(f"link failed:\n{r.stdout}\n{r.stderr}")
     os.replace(tmp, LIB)
     build_diag()
+    build_fortran_driver()
     return LIB
 
 
@@ -86,6 +87,32 @@ def build_diag():
         raise RuntimeError(f"libqpad_diag.so build failed:\n{r.stdout}\n{r.stderr}")
     os.replace(tmp, DIAG_LIB)
     return DIAG_LIB
+
+
+FORTRAN = os.path.join(HERE, "fortran")
+DRIVER = os.path.join(FORTRAN, "qpad_slice_driver")
+
+
+def build_fortran_driver():
+    """qpad_amd/fortran/qpad_slice_driver: the Fortran module and a program
+    that drives a 3D step through it in QPAD's order (amdflang).  Skipped,
+    with a message, where amdflang is missing."""
+    fc = shutil.which("amdflang") or "/opt/rocm/bin/amdflang"
+    if not os.path.exists(fc):
+        print("qpad_slice_driver not built: no amdflang", file=sys.stderr)
+        return None
+    work = os.path.join(BUILD, "fortran")
+    os.makedirs(work, exist_ok=True)
+    mod_o = os.path.join(work, "qpad_hip_mod.o")
+    tmp = DRIVER + ".tmp"
+    for cmd in ([fc, "-O2", "-c", os.path.join(FORTRAN, "qpad_hip_mod.f03"), "-o", mod_o, "-J", work],
+                [fc, "-O2", "-I", work, os.path.join(FORTRAN, "qpad_slice_driver.f03"), mod_o, "-L", HERE,
+                 "-lqpad_hip", "-Wl,-rpath,$ORIGIN/..", "-o", tmp]):
+        r = subprocess.run(cmd, capture_output=True, text=True)
+        if r.returncode != 0:
+            raise RuntimeError(f"Fortran build failed:\n{' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
+    os.replace(tmp, DRIVER)
+    return DRIVER
 
 
 if __name__ == "__main__":

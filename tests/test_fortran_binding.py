@@ -70,3 +70,18 @@ def test_fortran_module_declares_every_entry_point():
     text = open(MOD).read().lower()
     bound = set(re.findall(r'bind\(c, name="(qpad_[a-z0-9_]+)"\)', text))
     assert bound == set(L.SIGNATURES)
+
+
+def test_fortran_driver_reports_library_errors_through_write_err(tmp_path):
+    """The Fortran driver (qpad_amd/fortran/qpad_slice_driver.f03) stops
+    through its write_err stand-in with the library's message when a call
+    fails (CPU: the grid is refused before any device call)."""
+    import fortran_driver
+    from qpad_amd import configs
+    if not os.path.exists(fortran_driver.DRIVER):
+        pytest.skip("Fortran driver not built (no amdflang)")
+    deck = configs.small(nr=4)
+    with pytest.raises(subprocess.CalledProcessError) as e:
+        fortran_driver.run(deck, str(tmp_path))
+    assert e.value.returncode == 1
+    assert e.value.stderr.strip().startswith("HIP: qpad_create: nr"), e.value.stderr
