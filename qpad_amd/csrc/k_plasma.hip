@@ -266,8 +266,19 @@ __global__ __launch_bounds__(64 * CPW) void k_qdeposit(const QDepMulti m)
     double acc[16 * NB];
 #pragma unroll
     for (int v = 0; v < 16 * NB; v++) acc[v] = 0.0;
-    for (int p = p0 + lane; p < p1; p += 64) {
-        const double x1 = a.x1[p], x2 = a.x2[p], q = a.q[p];
+    // wave-uniform passes, software-pipelined as in k_amjdeposit; lanes past
+    // the chunk's end deposit zero
+    const int npass = __builtin_amdgcn_readfirstlane((p1 - p0 + 63) >> 6);
+    const int pl = max(p1 - 1, p0);
+    int p = p0 + lane;
+    int pc = min(p, pl);
+    double nx1 = a.x1[pc], nx2 = a.x2[pc], nq = a.q[pc];
+    for (int k = 0; k < npass; k++, p += 64) {
+        const double x1 = nx1, x2 = nx2, q = p < p1 ? nq : 0.0;
+        pc = min(p + 64, pl);
+        nx1 = a.x1[pc];
+        nx2 = a.x2[pc];
+        nq = a.q[pc];
         double pos = __dmul_rn(radius(x1, x2), a.idr);
         double ph0r = x1 / pos * a.idr, ph0i = -x2 / pos * a.idr;
         double w1 = pos - (double)c, w0 = 1.0 - w1;
@@ -484,16 +495,31 @@ struct JDepArgs {
     int n_ov_max;
 };
 
-// robust predictor and current deposit of one particle of cell c into this
-// lane's accumulators: acc[v] node ix, acc[NV + v] node ix + 1
+// What one particle deposits (robust predictor, part2d_class.f03:877-960):
+// the eight deposited quantities val[comp] (cu 0..2, dcu 3..4, amu 5..7) and
+// the weight of node ix + n times the mode phase of mode array k,
+// wph[n][k].  Value v = n * 8NM + comp * NM + k of the cell's deposit is the
+// sum over its particles of wph[n][k] * val[comp].
 template <int M>
-__device__ __forceinline__ void deposit_particle(const JDepArgs &a, const NodePairLds<2 * M + 1> &fe,
-                                                 const NodePairLds<2 * M + 1> &fb, int c, int p, double x1,
-                                                 double x2, double pp1, double pp2, double pp3, double qp,
-                                                 double (&acc)[16 * (2 * M + 1)])
+struct DepTerms {
+    double val[8];
+    double wph[2][2 * M + 1];
+    __device__ __forceinline__ double operator()(int v) const
+    {
+        constexpr int NM = 2 * M + 1;
+        const int n = v / (8 * NM), r = v % (8 * NM);
+        return wph[n][r % NM] * val[r / NM];
+    }
+};
+
+// gather and robust predictor of particle p of cell c; lanes past the
+// chunk's end (valid false) deposit zero and store nothing
+template <int M>
+__device__ __forceinline__ void particle_terms(const JDepArgs &a, const NodePairLds<2 * M + 1> &fe,
+                                               const NodePairLds<2 * M + 1> &fb, int c, int p, bool valid,
+                                               double x1, double x2, double pp1, double pp2, double pp3,
+                                               double qp, DepTerms<M> &t)
 {
-    constexpr int NM = 2 * M + 1;
-    constexpr int NV = 8 * NM;
     double r = radius(x1, x2);
     const double pc = x1 / r, ps = x2 / r;
     double pos = __dmul_rn(r, a.idr);
@@ -528,39 +554,49 @@ __device__ __forceinline__ void deposit_particle(const JDepArgs &a, const NodePa
     for (int cc = 0; cc < 3; cc++) u[cc] = 0.5 * (u[cc] + u0[cc]);
     double g2 = sqrt(1.0 + u[0] * u[0] + u[1] * u[1] + u[2] * u[2]);
     double ipsi = 1.0 / (g2 - u[2]);
-    if (a.store_diag) {
+    if (a.store_diag && valid) {
         a.gamma[p] = g2;
         a.psi[p] = (1.0 - 1.0 / ipsi) / a.qbm;
     }
     double dpsi = a.qbm * (wp2 - (wp0 * u[0] + wp1 * u[1]) * ipsi);
     du0 = du0 + u[0] * dpsi * ipsi;
     du1 = du1 + u[1] * dpsi * ipsi;
-    const double val[8] = {u[0], u[1], u[2], du0, du1, u[0] * u[0] * ipsi, u[0] * u[1] * ipsi,
-                           u[1] * u[1] * ipsi};
-    double phr = 1.0 * qp * ipsi, phi = 0.0;
-    {
-        const double wa = w0 * phr, wb = w1 * phr;
-#pragma unroll
-        for (int cc = 0; cc < 8; cc++) {
-            acc[cc * NM] += wa * val[cc];
-            acc[NV + cc * NM] += wb * val[cc];
-        }
-    }
+    t.val[0] = u[0];
+    t.val[1] = u[1];
+    t.val[2] = u[2];
+    t.val[3] = du0;
+    t.val[4] = du1;
+    t.val[5] = u[0] * u[0] * ipsi;
+    t.val[6] = u[0] * u[1] * ipsi;
+    t.val[7] = u[1] * u[1] * ipsi;
+    double phr = 1.0 * (valid ? qp : 0.0) * ipsi, phi = 0.0;
+    t.wph[0][0] = w0 * phr;
+    t.wph[1][0] = w1 * phr;
     const double ph0r = pc, ph0i = -ps;
 #pragma unroll
     for (int m = 1; m <= M; m++) {
         double tr = phr * ph0r - phi * ph0i, ti = phr * ph0i + phi * ph0r;
         phr = tr;
         phi = ti;
-        const double war = w0 * phr, wbr = w1 * phr, wai = w0 * phi, wbi = w1 * phi;
-#pragma unroll
-        for (int cc = 0; cc < 8; cc++) {
-            acc[cc * NM + 2 * m - 1] += war * val[cc];
-            acc[NV + cc * NM + 2 * m - 1] += wbr * val[cc];
-            acc[cc * NM + 2 * m] += wai * val[cc];
-            acc[NV + cc * NM + 2 * m] += wbi * val[cc];
-        }
+        t.wph[0][2 * m - 1] = w0 * phr;
+        t.wph[1][2 * m - 1] = w1 * phr;
+        t.wph[0][2 * m] = w0 * phi;
+        t.wph[1][2 * m] = w1 * phi;
     }
+}
+
+// Halving steps of the wave reduce-scatter applied to each pass's
+// contributions before they are accumulated: a lane keeps 16NM >> S
+// accumulators instead of 16NM, and the remaining 4 - S steps run once per
+// chunk.  Each step costs a chain of dependent cross-lane moves per pass, so
+// it pays only where the full set of accumulators would leave one wave per
+// SIMD: measured on MI355X (C2, M = 2: 20.6 us per launch with S = 0 and two
+// waves per SIMD, 23.1 us with S = 3 and four; C3, M = 4: 41.0 us with
+// S = 0 and one wave, 28.0 us with S = 1 and two).
+template <int M>
+constexpr int dep_pass_steps()
+{
+    return M <= 2 ? 0 : 1;
 }
 
 // the arguments of every species of one launch; workgroup row blockIdx.y
@@ -639,25 +675,32 @@ __global__ __launch_bounds__(64 * CPW) void k_amjdeposit(const JDepMulti m)
     const double *__restrict__ PP2 = a.p2;
     const double *__restrict__ PP3 = a.p3;
     const double *__restrict__ Q = a.q;
-    double acc[2 * NV];
+    constexpr int S = dep_pass_steps<M>();
+    constexpr int NA = (2 * NV) >> S;
+    double acc[NA];
 #pragma unroll
-    for (int v = 0; v < 2 * NV; v++) acc[v] = 0.0;
-    // software-pipelined: the next pass's particle loads are in flight while
-    // this pass computes
-    int p = p0 + lane;
+    for (int v = 0; v < NA; v++) acc[v] = 0.0;
+    // every pass has all 64 lanes active (the exchanges need them); lanes
+    // past the chunk's end deposit zero.  Software-pipelined: the next pass's
+    // particle loads are in flight while this pass computes.
+    const int npass = __builtin_amdgcn_readfirstlane((p1 - p0 + 63) >> 6);
     const int pl = max(p1 - 1, p0);
+    int p = p0 + lane;
     int pc = min(p, pl);
     double x1 = X1[pc], x2 = X2[pc], pp1 = PP1[pc], pp2 = PP2[pc], pp3 = PP3[pc], qp = Q[pc];
-    for (; p < p1; p += 64) {
+    for (int k = 0; k < npass; k++, p += 64) {
         // the staged fields are re-read from LDS (broadcast) every particle
         // rather than held in registers: the accumulators need the VGPRs
         __asm__ volatile("" ::: "memory");
         const int pn = min(p + 64, pl);
         const double nx1 = X1[pn], nx2 = X2[pn], np1 = PP1[pn], np2 = PP2[pn], np3 = PP3[pn], nq = Q[pn];
-        deposit_particle<M>(a, fe, fbf, c, p, x1, x2, pp1, pp2, pp3, qp, acc);
+        DepTerms<M> t;
+        particle_terms<M>(a, fe, fbf, c, p, p < p1, x1, x2, pp1, pp2, pp3, qp, t);
+#pragma unroll
+        for (int i = 0; i < NA; i++) acc[i] += reduced_value<NM, S>(t, i, lane);
         x1 = nx1; x2 = nx2; pp1 = np1; pp2 = np2; pp3 = np3; qp = nq;
     }
-    wave_reduce_scatter<NM>(acc);
+    wave_reduce_from<NM, S>(acc);
     const int g = lane >> 2;
     double *D0 = ch.ovf ? a.O0 : a.P0, *D1 = ch.ovf ? a.O1 : a.P1;
 #pragma unroll

@@ -69,39 +69,69 @@ constexpr int kDppRowHalfMirror = 0x141;
 constexpr int kDppQuadXor1 = 0xB1;   // quad_perm [1, 0, 3, 2]
 constexpr int kDppQuadXor2 = 0x4E;   // quad_perm [2, 3, 0, 1]
 
-// In place on acc[0 .. 16 NB): on return acc[0 .. NB) of lane l hold the
-// 64-lane sums of values [(l >> 2) * NB, (l >> 2 + 1) * NB).  Requires all
-// 64 lanes active.
+// One exchange step of the reduce-scatter.  Level L (0..4) holds
+// 16 NB >> L values per lane; step L -> L + 1 pairs value i with value
+// i + half (half = 8 NB >> L) as described above and keeps one sum of each
+// pair.  lo, hi: the pair's values of this lane.
+template <int L>
+__device__ __forceinline__ double reduce_pair(double lo, double hi, int lane)
+{
+    static_assert(L >= 0 && L < 4, "four halving steps");
+    if constexpr (L == 0) {
+        return swap32_sum(lo, hi);
+    } else if constexpr (L == 1) {
+        return swap16_sum(lo, hi);
+    } else if constexpr (L == 2) {
+        const bool up = (lane & 8) != 0;
+        return (up ? hi : lo) + dpp<kDppRowMirror>(up ? lo : hi);
+    } else {
+        const bool up = (lane & 4) != 0;
+        return (up ? hi : lo) + dpp<kDppRowHalfMirror>(up ? lo : hi);
+    }
+}
+
+// Steps FROM .. 3 and the quad butterfly, in place on the 16 NB >> FROM
+// values a lane holds at level FROM.  On return acc[0 .. NB) of lane l hold
+// the 64-lane sums of values [(l >> 2) * NB, (l >> 2 + 1) * NB) of level 0.
+// Requires all 64 lanes active.
+template <int NB, int FROM>
+__device__ __forceinline__ void wave_reduce_from(double (&acc)[(16 * NB) >> FROM])
+{
+    const int lane = threadIdx.x & 63;
+    if constexpr (FROM < 4) {
+        constexpr int half = (8 * NB) >> FROM;
+#pragma unroll
+        for (int i = 0; i < half; i++) acc[i] = reduce_pair<FROM>(acc[i], acc[half + i], lane);
+        wave_reduce_from<NB, FROM + 1>(*reinterpret_cast<double(*)[(16 * NB) >> (FROM + 1)]>(&acc[0]));
+    } else {
+#pragma unroll
+        for (int i = 0; i < NB; i++) {
+            acc[i] = acc[i] + dpp<kDppQuadXor1>(acc[i]);
+            acc[i] = acc[i] + dpp<kDppQuadXor2>(acc[i]);
+        }
+    }
+}
+
+// the whole reduce-scatter of 16 NB values per lane
 template <int NB>
 __device__ __forceinline__ void wave_reduce_scatter(double (&acc)[16 * NB])
 {
-    const int lane = threadIdx.x & 63;
-#pragma unroll
-    for (int i = 0; i < 8 * NB; i++) acc[i] = swap32_sum(acc[i], acc[8 * NB + i]);
-#pragma unroll
-    for (int i = 0; i < 4 * NB; i++) acc[i] = swap16_sum(acc[i], acc[4 * NB + i]);
-    {
-        const bool up = (lane & 8) != 0;
-#pragma unroll
-        for (int i = 0; i < 2 * NB; i++) {
-            const double lo = acc[i], hi = acc[2 * NB + i];
-            const double recv = dpp<kDppRowMirror>(up ? lo : hi);
-            acc[i] = (up ? hi : lo) + recv;
-        }
-    }
-    {
-        const bool up = (lane & 4) != 0;
-#pragma unroll
-        for (int i = 0; i < NB; i++) {
-            const double lo = acc[i], hi = acc[NB + i];
-            const double recv = dpp<kDppRowHalfMirror>(up ? lo : hi);
-            acc[i] = (up ? hi : lo) + recv;
-        }
-    }
-#pragma unroll
-    for (int i = 0; i < NB; i++) {
-        acc[i] = acc[i] + dpp<kDppQuadXor1>(acc[i]);
-        acc[i] = acc[i] + dpp<kDppQuadXor2>(acc[i]);
+    wave_reduce_from<NB, 0>(acc);
+}
+
+// Level-L value i of values generated on the fly: t0(v), v < 16 NB, is a
+// lane's level-0 value v.  Only the values feeding value i are live, so a
+// lane can add a pass's contributions into level-L accumulators without
+// holding all 16 NB of them.
+template <int NB, int L, class T0>
+__device__ __forceinline__ double reduced_value(const T0 &t0, int i, int lane)
+{
+    if constexpr (L == 0) {
+        return t0(i);
+    } else {
+        constexpr int half = (8 * NB) >> (L - 1);
+        return reduce_pair<L - 1>(reduced_value<NB, L - 1>(t0, i, lane),
+                                  reduced_value<NB, L - 1>(t0, half + i, lane), lane);
     }
 }
 

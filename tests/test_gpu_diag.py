@@ -48,13 +48,18 @@ def modes(h, nm):
 
 
 def check_mode_files(folder, dataname, want, it, tol=1e-10):
+    """every mode file against the oracle, norm-wise over the diagnostic: the
+    error of each mode array relative to the largest value of any mode (a
+    mode far smaller than m = 0, such as the ions' Im1 charge, holds only the
+    round-off of the free-running step at that scale)"""
+    scale = max(float(np.max(np.abs(a))) for a in want.values())
     for sub, arr in want.items():
         path = os.path.join(folder, sub, f"{dataname}_{it:08d}.h5")
         with h5read.File(path) as f:
             got = f.dataset(dataname)
             assert f.attr("/", "ITER") == it
         assert got.shape == arr.shape, (path, got.shape, arr.shape)
-        e = rel_err(got, arr, floor=1e-6)
+        e = rel_err(got, arr, floor=max(scale, 1e-6))
         assert e <= tol, (path, e)
 
 

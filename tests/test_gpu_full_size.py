@@ -2,7 +2,8 @@
 
 * C2 and C3 slices inside the beam, and C2 slices deep in the wake,
   compared with the oracle on identical inputs (a few slices: the serial
-  oracle takes ~0.3 s per C2 slice);
+  oracle takes ~0.3 s per C2 slice), each against the oracle's own
+  sensitivity to a round-off change of its input where that is larger;
 * size-independent properties over whole C2 3D steps: particles stay sorted
   by radial cell with consistent cell starts, the particle count is
   conserved (inside + left the box), and a rerun is bit-identical.
@@ -24,7 +25,14 @@ def run_to(sim, j_end):
 
 
 # C2 slices 520 and 900 are deep in the wake, where the iteration runs to
-# iter_max and B_perp reaches O(1e3): the GPU follows the oracle there too
+# iter_max and does not contract: there a 1e-15 change of the slice's input
+# moves the oracle's own result by up to 1e-4 (particles) and 1e-6 (fields).
+# Each slice is therefore held to TOL or to SENS_FACTOR times the oracle's
+# own sensitivity to such a change, whichever is larger; in the beam and
+# ahead of the wake that is TOL.
+SENS_FACTOR = 10.0
+
+
 @pytest.mark.parametrize("name,j0,n", [("c2", 100, 3), ("c2", 520, 1), ("c2", 900, 1), ("c3", 100, 2)])
 def test_full_size_slices_match_oracle(name, j0, n):
     deck = getattr(configs, name)()
@@ -34,15 +42,21 @@ def test_full_size_slices_match_oracle(name, j0, n):
     o.step_begin()
     run_to(sim, j0)
     norm = twin.ErrorNorm()
+    bound = {}
     for j in range(j0, j0 + n):
         twin.sync_slice_inputs(sim, o, j)
-        it_g, it_o = sim.slice(j), o.slice(j)
+        it_o, sens = twin.oracle_self_sensitivity(o, j, len(sim.spe))
+        it_g = sim.slice(j)
         assert it_g == it_o, f"{name} slice {j}: {it_g} iterations (GPU) vs {it_o} (oracle)"
         twin.compare_fields(sim, o, norm, j)
         twin.compare_species(sim, o, norm, j)
+        for q, e in sens.errors().items():
+            bound[q] = max(bound.get(q, TOL), SENS_FACTOR * e)
     sim.close()
-    bad = norm.bad(TOL)
-    assert not bad, f"{name}: beyond {TOL}: {bad}"
+    errs = norm.errors()
+    print(name, j0, {q: f"{e:.1e} (bound {bound[q]:.1e})" for q, e in sorted(errs.items())})
+    bad = {q: (e, bound[q]) for q, e in errs.items() if not e <= bound[q]}
+    assert not bad, f"{name}: beyond the bound: {bad}"
 
 
 def cell_keys(p, g):

@@ -194,3 +194,48 @@ def compare_fields(sim, o, norm, j, names=FIELDS):
         norm.add(name, sim.fields[name].get(), o.field(ORA_ID[name]), j)
     for name in HIST:
         norm.add("h_" + name, sim.fields[name].hist(j), o.hist(ORA_ID[name])[j], j)
+
+
+def _oracle_capture(o, nspecies, j):
+    out = {name: o.field(ORA_ID[name]).copy() for name in FIELDS}
+    for name in HIST:
+        out["h_" + name] = o.hist(ORA_ID[name])[j].copy()
+    for k in range(nspecies):
+        r = particles_by_id({n: np.array(v) for n, v in o.species(k).items()})
+        for name in ("x1", "x2", "p1", "p2", "p3", "gamma", "psi"):
+            out[f"sp{k}.{name}"] = r[name]
+    return out
+
+
+def oracle_self_sensitivity(o, j, nspecies, eps=1e-15, seed=1):
+    """How far the oracle's own slice j moves when its input J_perp (cu, the
+    predictor the slice starts from) is perturbed by a relative eps (seeded
+    normal): slice j is run from the oracle's current state once perturbed,
+    then once as it is, which leaves the oracle in the unperturbed state
+    after slice j.  Returns (iterations, ErrorNorm of the perturbed run
+    against the unperturbed one over the quantities compare_fields and
+    compare_species check).  Where the predictor-corrector iteration does not
+    contract (deep in a strong wake) this is far above round-off: no port
+    can agree with the oracle more closely than the oracle agrees with
+    itself."""
+    spec = [{n: np.array(v) for n, v in o.species(k).items()} for k in range(nspecies)]
+    fields = {fid: o.field(fid).copy() for fid in range(len(po.NCOMP))}
+
+    def restore():
+        for k in range(nspecies):
+            o.set_species(k, {n: v.copy() for n, v in spec[k].items()})
+        for fid, v in fields.items():
+            o.field(fid)[...] = v
+
+    restore()
+    cu = o.field(po.CU)
+    cu *= 1.0 + eps * np.random.default_rng(seed).standard_normal(cu.shape)
+    o.slice(j)
+    pert = _oracle_capture(o, nspecies, j)
+    restore()
+    its = o.slice(j)
+    base = _oracle_capture(o, nspecies, j)
+    norm = ErrorNorm()
+    for name, v in base.items():
+        norm.add(name, pert[name], v, j)
+    return its, norm
