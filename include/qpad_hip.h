@@ -60,7 +60,9 @@ extern "C" {
 #define QPAD_BND_ZERO 2
 #define QPAD_BND_OPEN 3
 
-/* plasma pusher (source/param.f03:63-64); only "robust" is on this path */
+/* plasma pusher (source/param.f03:63-64): "standard" and "robust" (the PGC
+   pushers need the laser envelope, which is not on this path) */
+#define QPAD_PUSH2_STD 0
 #define QPAD_PUSH2_ROBUST 1
 
 /* grid and iteration settings: options_class.f03:44-117 (nr, nxi, dr, dxi),
@@ -187,12 +189,19 @@ int qpad_species_charge_history(qpad_ctx *ctx, int32_t enable);
 int qpad_species_cbq(qpad_ctx *ctx, int32_t sp, int32_t pos);
 int qpad_get_species_charge(qpad_ctx *ctx, int32_t sp, int32_t slice, double *out);
 /* species2d%amjdp: deposit cu, acu (dJ/dxi part) and amu of species sp into
-   the totals using fields e and b (species2d_class.f03:233-280,
-   part2d_class.f03:746-1010, robust pusher) */
+   the totals using fields e and b (species2d_class.f03:233-280;
+   part2d_class.f03:746-1010 robust pusher, :478-744 standard pusher) */
 int qpad_species_amjdp(qpad_ctx *ctx, int32_t sp, double dxi);
 /* species2d%push_u + %push_x (+ update_bound, move_part2d_comm):
-   part2d_class.f03:1879-1965, :2221-2262, :2307-2353 */
+   part2d_class.f03:1879-1965 (robust) or :1790-1877 (standard),
+   :2221-2262, :2307-2353 */
 int qpad_species_push(qpad_ctx *ctx, int32_t sp, double dxi);
+/* species2d%interp_psi (species2d_class.f03:439-452, part2d_class.f03:
+   2264-2305): psi of the current slice at the particles of a standard-pusher
+   species, which its amjdp and push read; no effect on a robust species.
+   QPAD calls it after field_psi%solve (simulation_class.f03:357-359); the
+   fused forms do it at the same point. */
+int qpad_species_interp_psi(qpad_ctx *ctx, int32_t sp);
 
 /* field_psi%solve(q_spe) (field_psi_class.f03:218-254) */
 int qpad_field_solve_psi(qpad_ctx *ctx);

@@ -43,7 +43,10 @@ class SpeciesCfg(C.Structure):
     _fields_ = [("ppc1", C.c_int), ("ppc2", C.c_int), ("num_theta", C.c_int),
                 ("qm", C.c_double), ("mass", C.c_double), ("density", C.c_double),
                 ("uth", C.c_double * 3), ("neutralized", C.c_int), ("den_min", C.c_double),
-                ("seed", C.c_uint64)]
+                ("seed", C.c_uint64), ("push_type", C.c_int)]
+
+
+PUSH2_STD, PUSH2_ROBUST = 0, 1
 
 
 class BeamCfg(C.Structure):
@@ -141,6 +144,9 @@ def load_library(path):
                                         _dp, C.c_double]
     L.ora_push_u_robust.argtypes = [C.c_void_p, C.POINTER(_Species), _dp, _dp, C.c_double]
     L.ora_push_x.argtypes = [C.POINTER(_Species), C.c_double]
+    L.ora_interp_psi.argtypes = [C.c_void_p, C.POINTER(_Species), _dp]
+    L.ora_amjdeposit_std.argtypes = [C.c_void_p, C.POINTER(_Species), _dp, _dp, _dp, _dp, _dp, C.c_double]
+    L.ora_push_u_std.argtypes = [C.c_void_p, C.POINTER(_Species), _dp, _dp, C.c_double]
     L.ora_update_bound.argtypes = [C.c_void_p, C.POINTER(_Species)]
     L.ora_interp_point.argtypes = [C.c_void_p, _dp, C.c_int, C.c_double, C.c_double, _dp]
     L.ora_beam_qdeposit.argtypes = [C.c_void_p, C.c_int, _dp]
@@ -180,9 +186,9 @@ def make_grid(nr, nxi, max_mode, dr, dxi, relax_fac=None, bnd=BND_OPEN, iter_max
 
 
 def species_cfg(ppc1, ppc2, num_theta, qm=-1.0, mass=1.0, density=1.0, uth=(0.0, 0.0, 0.0),
-                neutralized=True, den_min=1e-10, seed=0):
+                neutralized=True, den_min=1e-10, seed=0, push_type=PUSH2_ROBUST):
     return SpeciesCfg(ppc1, ppc2, num_theta, qm, mass, density, (C.c_double * 3)(*uth),
-                      int(neutralized), den_min, seed)
+                      int(neutralized), den_min, seed, push_type)
 
 
 def beam_cfg(total_num, total_charge, center, sigma, range_lo, range_hi, qm=-1.0, mass=1.0,

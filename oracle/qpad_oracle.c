@@ -707,12 +707,56 @@ void ora_qdeposit(const ora_sim *s, const ora_species *sp, double *q)
     }
 }
 
+/* axis rules (part2d_class.f03:914-960) and division by the node index
+   (:962-1004) of the current deposits; amjdeposit_std_part2d has the same
+   (:683-742) */
+static void deposit_axis_and_radius(const ora_sim *s, double *cu, double *amu, double *dcu)
+{
+    int nr = s->g.nr, nm = s->nm, ld = s->ld, M = s->g.max_mode;
+    for (int c = 0; c < 3; c++) { FA(cu, c, 0, 0) = 0.0; FA(amu, c, 0, 0) = 0.0; }
+    for (int c = 0; c < 2; c++) FA(dcu, c, 0, 0) = 0.0;
+    FA(cu, 0, 0, 1) = 0.0; FA(cu, 1, 0, 1) = 0.0; FA(cu, 2, 0, 1) = 8.0 * FA(cu, 2, 0, 1);
+    FA(dcu, 0, 0, 1) = 0.0; FA(dcu, 1, 0, 1) = 0.0;
+    for (int c = 0; c < 3; c++) FA(amu, c, 0, 1) = 0.0;
+    for (int m = 1; m <= M; m++) {
+        int ks[2] = {kre(m), kim(m)};
+        for (int t = 0; t < 2; t++) {
+            int k = ks[t];
+            for (int c = 0; c < 3; c++) { FA(cu, c, k, 0) = 0.0; FA(amu, c, k, 0) = 0.0; }
+            for (int c = 0; c < 2; c++) FA(dcu, c, k, 0) = 0.0;
+            if (m == 1) {
+                FA(cu, 0, k, 1) = 8.0 * FA(cu, 0, k, 1); FA(cu, 1, k, 1) = 8.0 * FA(cu, 1, k, 1);
+                FA(cu, 2, k, 1) = 0.0;
+                FA(dcu, 0, k, 1) = 8.0 * FA(dcu, 0, k, 1); FA(dcu, 1, k, 1) = 8.0 * FA(dcu, 1, k, 1);
+                for (int c = 0; c < 3; c++) FA(amu, c, k, 1) = 0.0;
+            } else if (m == 2) {
+                for (int c = 0; c < 3; c++) FA(cu, c, k, 1) = 0.0;
+                for (int c = 0; c < 2; c++) FA(dcu, c, k, 1) = 0.0;
+                for (int c = 0; c < 3; c++) FA(amu, c, k, 1) = 8.0 * FA(amu, c, k, 1);
+            } else {
+                for (int c = 0; c < 3; c++) FA(cu, c, k, 1) = 0.0;
+                for (int c = 0; c < 2; c++) FA(dcu, c, k, 1) = 0.0;
+                for (int c = 0; c < 3; c++) FA(amu, c, k, 1) = 0.0;
+            }
+        }
+    }
+    /* divide by the node index (:962-1004) */
+    for (int k = 0; k < nm; k++) {
+        for (int j = 2; j <= nr + 1; j++) {
+            double ir = 1.0 / (double)(j - 1);
+            for (int c = 0; c < 3; c++) FA(cu, c, k, j) = FA(cu, c, k, j) * ir;
+            for (int c = 0; c < 2; c++) FA(dcu, c, k, j) = FA(dcu, c, k, j) * ir;
+            for (int c = 0; c < 3; c++) FA(amu, c, k, j) = FA(amu, c, k, j) * ir;
+        }
+    }
+}
+
 /* amjdeposit_robust_part2d: source/species/part2d_class.f03:746-1010 */
 void ora_amjdeposit_robust(const ora_sim *s, ora_species *sp, const double *ef,
                            const double *bf, double *cu, double *amu,
                            double *dcu, double dt)
 {
-    int nr = s->g.nr, nm = s->nm, ld = s->ld, M = s->g.max_mode;
+    int nm = s->nm, ld = s->ld, M = s->g.max_mode;
     double idr = 1.0 / s->g.dr;
     double idt = 1.0 / dt;
     double qtmh = 0.5 * sp->qbm * dt;
@@ -781,43 +825,7 @@ void ora_amjdeposit_robust(const ora_sim *s, ora_species *sp, const double *ef,
             }
         }
     }
-    /* axis rules (:914-960) */
-    for (int c = 0; c < 3; c++) { FA(cu, c, 0, 0) = 0.0; FA(amu, c, 0, 0) = 0.0; }
-    for (int c = 0; c < 2; c++) FA(dcu, c, 0, 0) = 0.0;
-    FA(cu, 0, 0, 1) = 0.0; FA(cu, 1, 0, 1) = 0.0; FA(cu, 2, 0, 1) = 8.0 * FA(cu, 2, 0, 1);
-    FA(dcu, 0, 0, 1) = 0.0; FA(dcu, 1, 0, 1) = 0.0;
-    for (int c = 0; c < 3; c++) FA(amu, c, 0, 1) = 0.0;
-    for (int m = 1; m <= M; m++) {
-        int ks[2] = {kre(m), kim(m)};
-        for (int t = 0; t < 2; t++) {
-            int k = ks[t];
-            for (int c = 0; c < 3; c++) { FA(cu, c, k, 0) = 0.0; FA(amu, c, k, 0) = 0.0; }
-            for (int c = 0; c < 2; c++) FA(dcu, c, k, 0) = 0.0;
-            if (m == 1) {
-                FA(cu, 0, k, 1) = 8.0 * FA(cu, 0, k, 1); FA(cu, 1, k, 1) = 8.0 * FA(cu, 1, k, 1);
-                FA(cu, 2, k, 1) = 0.0;
-                FA(dcu, 0, k, 1) = 8.0 * FA(dcu, 0, k, 1); FA(dcu, 1, k, 1) = 8.0 * FA(dcu, 1, k, 1);
-                for (int c = 0; c < 3; c++) FA(amu, c, k, 1) = 0.0;
-            } else if (m == 2) {
-                for (int c = 0; c < 3; c++) FA(cu, c, k, 1) = 0.0;
-                for (int c = 0; c < 2; c++) FA(dcu, c, k, 1) = 0.0;
-                for (int c = 0; c < 3; c++) FA(amu, c, k, 1) = 8.0 * FA(amu, c, k, 1);
-            } else {
-                for (int c = 0; c < 3; c++) FA(cu, c, k, 1) = 0.0;
-                for (int c = 0; c < 2; c++) FA(dcu, c, k, 1) = 0.0;
-                for (int c = 0; c < 3; c++) FA(amu, c, k, 1) = 0.0;
-            }
-        }
-    }
-    /* divide by the node index (:962-1004) */
-    for (int k = 0; k < nm; k++) {
-        for (int j = 2; j <= nr + 1; j++) {
-            double ir = 1.0 / (double)(j - 1);
-            for (int c = 0; c < 3; c++) FA(cu, c, k, j) = FA(cu, c, k, j) * ir;
-            for (int c = 0; c < 2; c++) FA(dcu, c, k, j) = FA(dcu, c, k, j) * ir;
-            for (int c = 0; c < 3; c++) FA(amu, c, k, j) = FA(amu, c, k, j) * ir;
-        }
-    }
+    deposit_axis_and_radius(s, cu, amu, dcu);
 }
 
 /* push_u_robust_part2d: source/species/part2d_class.f03:1879-1965 */
@@ -839,6 +847,134 @@ void ora_push_u_robust(const ora_sim *s, ora_species *sp, const double *ef,
         double gam = sqrt(1.0 + pp[0] * pp[0] + pp[1] * pp[1] + pp[2] * pp[2]);
         double qtmh1 = qtmh / (gam - pp[2]);
         double qtmh2 = qtmh1 * gam;
+        for (int c = 0; c < 3; c++) { ep[c] = ep[c] * qtmh2; bp[c] = bp[c] * qtmh1; }
+        for (int c = 0; c < 3; c++) ut[c] = pp[c] + ep[c];
+        pp[0] = ut[0] + ut[1] * bp[2] - ut[2] * bp[1];
+        pp[1] = ut[1] + ut[2] * bp[0] - ut[0] * bp[2];
+        pp[2] = ut[2] + ut[0] * bp[1] - ut[1] * bp[0];
+        double ostq = 2.0 / (1.0 + bp[0] * bp[0] + bp[1] * bp[1] + bp[2] * bp[2]);
+        for (int c = 0; c < 3; c++) bp[c] = bp[c] * ostq;
+        ut[0] = ut[0] + pp[1] * bp[2] - pp[2] * bp[1];
+        ut[1] = ut[1] + pp[2] * bp[0] - pp[0] * bp[2];
+        ut[2] = ut[2] + pp[0] * bp[1] - pp[1] * bp[0];
+        for (int c = 0; c < 3; c++) pp[c] = ut[c] + ep[c];
+        sp->p1[p] = pp[0]; sp->p2[p] = pp[1]; sp->p3[p] = pp[2];
+        sp->gamma[p] = sqrt(1.0 + pp[0] * pp[0] + pp[1] * pp[1] + pp[2] * pp[2]);
+    }
+}
+
+/* interp_psi_part2d: source/species/part2d_class.f03:2264-2305 -- the
+   field psi at each particle, which the standard pusher uses in place of
+   the robust pusher's gamma - u_z */
+void ora_interp_psi(const ora_sim *s, ora_species *sp, const double *psi)
+{
+    double idr = 1.0 / s->g.dr;
+    for (int64_t p = 0; p < sp->npp; p++) {
+        int ix; double w0, w1, pc, ps;
+        interp_info(sp->x1[p], sp->x2[p], idr, &ix, &w0, &w1, &pc, &ps);
+        interp_vec(s, psi, 1, ix, w0, w1, pc, ps, &sp->psi[p]);
+    }
+}
+
+/* amjdeposit_std_part2d: source/species/part2d_class.f03:478-744.  The
+   predictor uses the interpolated psi of each particle (interp_psi) and
+   stores the time-centred gamma; the axis rules and the division by the
+   node index are the robust deposit's (:683-742 = :914-1004) */
+void ora_amjdeposit_std(const ora_sim *s, ora_species *sp, const double *ef,
+                        const double *bf, double *cu, double *amu,
+                        double *dcu, double dt)
+{
+    int nm = s->nm, ld = s->ld, M = s->g.max_mode;
+    double idr = 1.0 / s->g.dr;
+    double idt = 1.0 / dt;
+    double qtmh = 0.5 * sp->qbm * dt;
+    for (int64_t p = 0; p < sp->npp; p++) {
+        int ix; double w0, w1, pc, ps;
+        interp_info(sp->x1[p], sp->x2[p], idr, &ix, &w0, &w1, &pc, &ps);
+        double ep[3], bp[3], wp[3], u0[3], u[3], ut[3], du[3], u2[3];
+        interp_vec(s, ef, 3, ix, w0, w1, pc, ps, ep);
+        interp_vec(s, bf, 3, ix, w0, w1, pc, ps, bp);
+        wp[0] = ep[0] - bp[1];
+        wp[1] = ep[1] + bp[0];
+        wp[2] = ep[2];
+        u0[0] = sp->p1[p] * pc + sp->p2[p] * ps;
+        u0[1] = sp->p2[p] * pc - sp->p1[p] * ps;
+        u0[2] = sp->p3[p];
+        double gam = sqrt(1.0 + u0[0] * u0[0] + u0[1] * u0[1] + u0[2] * u0[2]);
+        double qtmh2 = qtmh / (1.0 - sp->qbm * sp->psi[p]);
+        double qtmh1 = qtmh2 * gam;
+        for (int c = 0; c < 3; c++) bp[c] = bp[c] * qtmh2;
+        for (int c = 0; c < 3; c++) ut[c] = u0[c] + ep[c] * qtmh1;
+        u[0] = ut[0] + ut[1] * bp[2] - ut[2] * bp[1];
+        u[1] = ut[1] + ut[2] * bp[0] - ut[0] * bp[2];
+        u[2] = ut[2] + ut[0] * bp[1] - ut[1] * bp[0];
+        double ostq = 2.0 / (1.0 + bp[0] * bp[0] + bp[1] * bp[1] + bp[2] * bp[2]);
+        for (int c = 0; c < 3; c++) bp[c] = bp[c] * ostq;
+        ut[0] = ut[0] + u[1] * bp[2] - u[2] * bp[1];
+        ut[1] = ut[1] + u[2] * bp[0] - u[0] * bp[2];
+        ut[2] = ut[2] + u[0] * bp[1] - u[1] * bp[0];
+        gam = sqrt(1.0 + ut[0] * ut[0] + ut[1] * ut[1] + ut[2] * ut[2]);
+        qtmh1 = qtmh2 * gam;
+        for (int c = 0; c < 3; c++) u[c] = ut[c] + ep[c] * qtmh1;
+
+        du[0] = idt * (u[0] - u0[0]);
+        du[1] = idt * (u[1] - u0[1]);
+        for (int c = 0; c < 3; c++) u[c] = 0.5 * (u[c] + u0[c]);
+        sp->gamma[p] = sqrt(1.0 + u[0] * u[0] + u[1] * u[1] + u[2] * u[2]);
+        double ipsi = 1.0 / (1.0 - sp->qbm * sp->psi[p]);
+        double dpsi = sp->qbm * (wp[2] - (wp[0] * u[0] + wp[1] * u[1]) * ipsi);
+        du[0] = du[0] + u[0] * dpsi * ipsi;
+        du[1] = du[1] + u[1] * dpsi * ipsi;
+        u2[0] = u[0] * u[0] * ipsi;
+        u2[1] = u[0] * u[1] * ipsi;
+        u2[2] = u[1] * u[1] * ipsi;
+        double ph0r = pc, ph0i = -ps;
+        double phr = 1.0 * sp->q[p] * ipsi, phi = 0.0 * sp->q[p] * ipsi;
+        double wt[2] = {w0, w1};
+        for (int j = 0; j < 2; j++) {
+            double w = wt[j] * phr;
+            for (int c = 0; c < 3; c++) FA(cu, c, 0, ix + j) = FA(cu, c, 0, ix + j) + w * u[c];
+            for (int c = 0; c < 2; c++) FA(dcu, c, 0, ix + j) = FA(dcu, c, 0, ix + j) + w * du[c];
+            for (int c = 0; c < 3; c++) FA(amu, c, 0, ix + j) = FA(amu, c, 0, ix + j) + w * u2[c];
+        }
+        for (int m = 1; m <= M; m++) {
+            double a = phr * ph0r - phi * ph0i, b = phr * ph0i + phi * ph0r;
+            phr = a; phi = b;
+            int kr = kre(m), ki = kim(m);
+            for (int j = 0; j < 2; j++) {
+                double w = wt[j] * phr;
+                for (int c = 0; c < 3; c++) FA(cu, c, kr, ix + j) = FA(cu, c, kr, ix + j) + w * u[c];
+                for (int c = 0; c < 2; c++) FA(dcu, c, kr, ix + j) = FA(dcu, c, kr, ix + j) + w * du[c];
+                for (int c = 0; c < 3; c++) FA(amu, c, kr, ix + j) = FA(amu, c, kr, ix + j) + w * u2[c];
+                w = wt[j] * phi;
+                for (int c = 0; c < 3; c++) FA(cu, c, ki, ix + j) = FA(cu, c, ki, ix + j) + w * u[c];
+                for (int c = 0; c < 2; c++) FA(dcu, c, ki, ix + j) = FA(dcu, c, ki, ix + j) + w * du[c];
+                for (int c = 0; c < 3; c++) FA(amu, c, ki, ix + j) = FA(amu, c, ki, ix + j) + w * u2[c];
+            }
+        }
+    }
+    deposit_axis_and_radius(s, cu, amu, dcu);
+}
+
+/* push_u_std_part2d: source/species/part2d_class.f03:1790-1877 -- the
+   interpolated psi and the gamma stored by the last current deposit in
+   place of the robust pusher's own gamma - p_z and gamma */
+void ora_push_u_std(const ora_sim *s, ora_species *sp, const double *ef,
+                    const double *bf, double dt)
+{
+    double idr = 1.0 / s->g.dr;
+    double qtmh = sp->qbm * dt * 0.5;
+    for (int64_t p = 0; p < sp->npp; p++) {
+        int ix; double w0, w1, pc, ps;
+        interp_info(sp->x1[p], sp->x2[p], idr, &ix, &w0, &w1, &pc, &ps);
+        double ep[3], bp[3], ut[3], pp[3];
+        interp_vec(s, ef, 3, ix, w0, w1, pc, ps, ep);
+        interp_vec(s, bf, 3, ix, w0, w1, pc, ps, bp);
+        double t = ep[0] * pc - ep[1] * ps; ep[1] = ep[0] * ps + ep[1] * pc; ep[0] = t;
+        t = bp[0] * pc - bp[1] * ps; bp[1] = bp[0] * ps + bp[1] * pc; bp[0] = t;
+        pp[0] = sp->p1[p]; pp[1] = sp->p2[p]; pp[2] = sp->p3[p];
+        double qtmh1 = qtmh / (1.0 - sp->qbm * sp->psi[p]);
+        double qtmh2 = qtmh1 * sp->gamma[p];
         for (int c = 0; c < 3; c++) { ep[c] = ep[c] * qtmh2; bp[c] = bp[c] * qtmh1; }
         for (int c = 0; c < 3; c++) ut[c] = pp[c] + ep[c];
         pp[0] = ut[0] + ut[1] * bp[2] - ut[2] * bp[1];
@@ -1499,6 +1635,8 @@ void ora_slice(ora_sim *s, int j, int *n_iter)
     TSTOP(ORA_T_DEPOSIT_2D);
     ora_solve_psi(s, q_spe, psi);
     TSTOP(ORA_T_SOLVE_PSI);
+    for (int k = 0; k < s->nspecies; k++)                                /* species2d%interp_psi */
+        if (s->sp[k].cfg.push_type == ORA_PUSH2_STD) ora_interp_psi(s, &s->sp[k], psi);
     ora_solve_bz(s, cu, b_spe);
     TSTOP(ORA_T_SOLVE_BZ);
 
@@ -1521,7 +1659,10 @@ void ora_slice(ora_sim *s, int j, int *n_iter)
             memset(sp->cu_own, 0, 3 * n1 * sizeof(double));
             memset(sp->acu_own, 0, 2 * n1 * sizeof(double));
             memset(sp->amu_own, 0, 3 * n1 * sizeof(double));
-            ora_amjdeposit_robust(s, sp, e, b, sp->cu_own, sp->amu_own, sp->acu_own, s->g.dxi);
+            if (sp->cfg.push_type == ORA_PUSH2_STD)
+                ora_amjdeposit_std(s, sp, e, b, sp->cu_own, sp->amu_own, sp->acu_own, s->g.dxi);
+            else
+                ora_amjdeposit_robust(s, sp, e, b, sp->cu_own, sp->amu_own, sp->acu_own, s->g.dxi);
             add_into(cu, sp->cu_own, 3 * n1);
             add_into(acu, sp->acu_own, 2 * n1);
             add_into(amu, sp->amu_own, 3 * n1);
@@ -1566,7 +1707,8 @@ void ora_slice(ora_sim *s, int j, int *n_iter)
 
     for (int k = 0; k < s->nspecies; k++) {
         ora_species *sp = &s->sp[k];
-        ora_push_u_robust(s, sp, e, b, s->g.dxi);
+        if (sp->cfg.push_type == ORA_PUSH2_STD) ora_push_u_std(s, sp, e, b, s->g.dxi);
+        else ora_push_u_robust(s, sp, e, b, s->g.dxi);
         ora_push_x(sp, s->g.dxi);
         TSTOP(ORA_T_PUSH_2D);
         ora_update_bound(s, sp);

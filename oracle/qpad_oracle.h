@@ -71,7 +71,14 @@ typedef struct {
     int neutralized;           /* species(i).neutralized (default true)     */
     double den_min;            /* species(i).den_min (default 1e-10)        */
     uint64_t seed;             /* Philox key for thermal draws (uth != 0)   */
+    int push_type;             /* species(i).push_type: ORA_PUSH2_STD
+                                  ("standard") or ORA_PUSH2_ROBUST
+                                  (param.f03:63, sim_plasma_class.f03:
+                                  123-137)                                   */
 } ora_species_cfg;
+
+#define ORA_PUSH2_STD 0
+#define ORA_PUSH2_ROBUST 1
 
 typedef struct {
     int64_t total_num;         /* beam(i).total_num                         */
@@ -179,6 +186,12 @@ void ora_amjdeposit_robust(const ora_sim *s, ora_species *sp,
                            double *amu, double *dcu, double dt);
 void ora_push_u_robust(const ora_sim *s, ora_species *sp, const double *e,
                        const double *b, double dt);
+void ora_interp_psi(const ora_sim *s, ora_species *sp, const double *psi);
+void ora_amjdeposit_std(const ora_sim *s, ora_species *sp,
+                        const double *e, const double *b, double *cu,
+                        double *amu, double *dcu, double dt);
+void ora_push_u_std(const ora_sim *s, ora_species *sp, const double *e,
+                    const double *b, double dt);
 void ora_push_x(ora_species *sp, double dt);
 void ora_update_bound(const ora_sim *s, ora_species *sp);
 void ora_interp_point(const ora_sim *s, const double *f, int ncomp,

@@ -18,7 +18,7 @@ FIELD_IDS = {"psi": F_PSI, "e": F_E, "e_spe": F_E_SPE, "b": F_B, "b_spe": F_B_SP
              "b_beam": F_B_BEAM, "cu": F_CU, "acu": F_ACU, "amu": F_AMU, "dcu": F_DCU,
              "q_spe": F_Q_SPE, "q_beam": F_Q_BEAM}
 BND_ZERO, BND_OPEN = 2, 3
-PUSH2_ROBUST = 1
+PUSH2_STD, PUSH2_ROBUST = 0, 1
 # status codes (include/qpad_hip.h)
 OK, E_INVALID, E_HIP, E_NOMEM, E_STATE = 0, -1, -2, -3, -4
 
@@ -105,6 +105,7 @@ SIGNATURES = {
     "qpad_get_species_charge": (C.c_int, [_CTX, C.c_int32, C.c_int32, _dp]),
     "qpad_species_amjdp": (C.c_int, [_CTX, C.c_int32, C.c_double]),
     "qpad_species_push": (C.c_int, [_CTX, C.c_int32, C.c_double]),
+    "qpad_species_interp_psi": (C.c_int, [_CTX, C.c_int32]),
     "qpad_field_solve_psi": (C.c_int, [_CTX]),
     "qpad_field_solve_ez": (C.c_int, [_CTX, C.c_int32, C.c_int32]),
     "qpad_field_solve_et": (C.c_int, [_CTX, C.c_int32, C.c_int32]),

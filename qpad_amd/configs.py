@@ -55,9 +55,9 @@ def _beam(zlen, sigma_r, sigma_xi, n_peak, total_num, offset_x=0.0, gamma=20000.
     }
 
 
-def _electrons(ppc_r, ppc_phi, neutralized=True):
+def _electrons(ppc_r, ppc_phi, neutralized=True, push_type="robust"):
     return {"profile": ["uniform", "uniform"], "ppc": [ppc_r, 1], "num_theta": ppc_phi,
-            "q": -1.0, "m": 1.0, "density": 1.0, "push_type": "robust",
+            "q": -1.0, "m": 1.0, "density": 1.0, "push_type": push_type,
             "neutralized": neutralized}
 
 
@@ -99,14 +99,16 @@ def c4(n=16, seed=2024, total_num=1 << 18, nxi=512):
 
 
 def small(nr=64, nxi=48, max_mode=1, ppc=(2, 4), total_num=1 << 14, rmax=4.0, dxi=0.05,
-          n_peak=4.0, sigma_r=0.3, sigma_xi=0.4, offset_x=0.0, ions=False, ion_ppc=None):
+          n_peak=4.0, sigma_r=0.3, sigma_xi=0.4, offset_x=0.0, ions=False, ion_ppc=None,
+          push_type="robust", ion_push_type=None):
     """A reduced deck with the same physics for fast parity checks; with
     ions, ion_ppc (default: the electrons' ppc) sets the ions' (r, phi)
-    particles per cell, so that the two species can differ in count."""
-    species = [_electrons(ppc[0], ppc[1], neutralized=not ions)]
+    particles per cell, so that the two species can differ in count.
+    push_type / ion_push_type (default: push_type): "robust" or "standard"."""
+    species = [_electrons(ppc[0], ppc[1], neutralized=not ions, push_type=push_type)]
     if ions:
         ip = ppc if ion_ppc is None else ion_ppc
-        ion = _electrons(ip[0], ip[1], neutralized=False)
+        ion = _electrons(ip[0], ip[1], neutralized=False, push_type=ion_push_type or push_type)
         ion.update({"q": 1.0, "m": 1836.0})
         species.append(ion)
     return _deck(nr, rmax, max_mode, nxi, dxi, species,

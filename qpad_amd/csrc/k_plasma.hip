@@ -512,13 +512,16 @@ struct DepTerms {
     }
 };
 
-// gather and robust predictor of particle p of cell c; lanes past the
-// chunk's end (valid false) deposit zero and store nothing
-template <int M>
+// gather and predictor of particle p of cell c; lanes past the chunk's end
+// (valid false) deposit zero and store nothing.  STD: the standard pusher
+// (amjdeposit_std_part2d, part2d_class.f03:478-744), which normalises by the
+// particle's interpolated psi (psip) where the robust one uses gamma - u_z,
+// and always stores the time-centred gamma.
+template <int M, bool STD>
 __device__ __forceinline__ void particle_terms(const JDepArgs &a, const NodePairLds<2 * M + 1> &fe,
                                                const NodePairLds<2 * M + 1> &fb, int c, int p, bool valid,
                                                double x1, double x2, double pp1, double pp2, double pp3,
-                                               double qp, DepTerms<M> &t)
+                                               double qp, double psip, DepTerms<M> &t)
 {
     double r = radius(x1, x2);
     const double pc = x1 / r, ps = x2 / r;
@@ -530,12 +533,20 @@ __device__ __forceinline__ void particle_terms(const JDepArgs &a, const NodePair
     double wp0 = ep[0] - bp[1], wp1 = ep[1] + bp[0], wp2 = ep[2];
     double u0[3] = {pp1 * pc + pp2 * ps, pp2 * pc - pp1 * ps, pp3};
     double gam = sqrt(1.0 + u0[0] * u0[0] + u0[1] * u0[1] + u0[2] * u0[2]);
-    double qtmh1 = a.qtmh * gam / (gam - u0[2]);
     double ut[3], u[3];
+    double qtmh2;
+    if constexpr (STD) {
+        qtmh2 = a.qtmh / (1.0 - a.qbm * psip);
+        const double qtmh1 = qtmh2 * gam;
 #pragma unroll
-    for (int cc = 0; cc < 3; cc++) { ep[cc] = ep[cc] * qtmh1; ut[cc] = u0[cc] + ep[cc]; }
-    gam = sqrt(1.0 + ut[0] * ut[0] + ut[1] * ut[1] + ut[2] * ut[2]);
-    double qtmh2 = a.qtmh / (gam - ut[2]);
+        for (int cc = 0; cc < 3; cc++) ut[cc] = u0[cc] + ep[cc] * qtmh1;
+    } else {
+        const double qtmh1 = a.qtmh * gam / (gam - u0[2]);
+#pragma unroll
+        for (int cc = 0; cc < 3; cc++) { ep[cc] = ep[cc] * qtmh1; ut[cc] = u0[cc] + ep[cc]; }
+        gam = sqrt(1.0 + ut[0] * ut[0] + ut[1] * ut[1] + ut[2] * ut[2]);
+        qtmh2 = a.qtmh / (gam - ut[2]);
+    }
 #pragma unroll
     for (int cc = 0; cc < 3; cc++) bp[cc] = bp[cc] * qtmh2;
     u[0] = ut[0] + ut[1] * bp[2] - ut[2] * bp[1];
@@ -547,16 +558,29 @@ __device__ __forceinline__ void particle_terms(const JDepArgs &a, const NodePair
     ut[0] = ut[0] + u[1] * bp[2] - u[2] * bp[1];
     ut[1] = ut[1] + u[2] * bp[0] - u[0] * bp[2];
     ut[2] = ut[2] + u[0] * bp[1] - u[1] * bp[0];
+    if constexpr (STD) {
+        // second half kick normalised by the gamma after the rotation
+        const double qtmh1 = qtmh2 * sqrt(1.0 + ut[0] * ut[0] + ut[1] * ut[1] + ut[2] * ut[2]);
 #pragma unroll
-    for (int cc = 0; cc < 3; cc++) u[cc] = ut[cc] + ep[cc];
+        for (int cc = 0; cc < 3; cc++) u[cc] = ut[cc] + ep[cc] * qtmh1;
+    } else {
+#pragma unroll
+        for (int cc = 0; cc < 3; cc++) u[cc] = ut[cc] + ep[cc];
+    }
     double du0 = a.idt * (u[0] - u0[0]), du1 = a.idt * (u[1] - u0[1]);
 #pragma unroll
     for (int cc = 0; cc < 3; cc++) u[cc] = 0.5 * (u[cc] + u0[cc]);
     double g2 = sqrt(1.0 + u[0] * u[0] + u[1] * u[1] + u[2] * u[2]);
-    double ipsi = 1.0 / (g2 - u[2]);
-    if (a.store_diag && valid) {
-        a.gamma[p] = g2;
-        a.psi[p] = (1.0 - 1.0 / ipsi) / a.qbm;
+    double ipsi;
+    if constexpr (STD) {
+        ipsi = 1.0 / (1.0 - a.qbm * psip);
+        if (valid) a.gamma[p] = g2;
+    } else {
+        ipsi = 1.0 / (g2 - u[2]);
+        if (a.store_diag && valid) {
+            a.gamma[p] = g2;
+            a.psi[p] = (1.0 - 1.0 / ipsi) / a.qbm;
+        }
     }
     double dpsi = a.qbm * (wp2 - (wp0 * u[0] + wp1 * u[1]) * ipsi);
     du0 = du0 + u[0] * dpsi * ipsi;
@@ -608,7 +632,7 @@ struct JDepMulti {
 // SPEC: a speculative launch of iteration >= 2 of a slice, which returns at
 // once when the iteration has already converged (a separate instance, so
 // that profiles show the first-iteration launches, which always work, apart)
-template <int M, bool SPEC, int CPW>
+template <int M, bool SPEC, int CPW, bool STD>
 __global__ __launch_bounds__(64 * CPW) void k_amjdeposit(const JDepMulti m)
 {
     const JDepArgs a = m.s[__builtin_amdgcn_readfirstlane(blockIdx.y)];
@@ -688,17 +712,19 @@ __global__ __launch_bounds__(64 * CPW) void k_amjdeposit(const JDepMulti m)
     int p = p0 + lane;
     int pc = min(p, pl);
     double x1 = X1[pc], x2 = X2[pc], pp1 = PP1[pc], pp2 = PP2[pc], pp3 = PP3[pc], qp = Q[pc];
+    double psip = STD ? a.psi[pc] : 0.0;
     for (int k = 0; k < npass; k++, p += 64) {
         // the staged fields are re-read from LDS (broadcast) every particle
         // rather than held in registers: the accumulators need the VGPRs
         __asm__ volatile("" ::: "memory");
         const int pn = min(p + 64, pl);
         const double nx1 = X1[pn], nx2 = X2[pn], np1 = PP1[pn], np2 = PP2[pn], np3 = PP3[pn], nq = Q[pn];
+        const double npsi = STD ? a.psi[pn] : 0.0;
         DepTerms<M> t;
-        particle_terms<M>(a, fe, fbf, c, p, p < p1, x1, x2, pp1, pp2, pp3, qp, t);
+        particle_terms<M, STD>(a, fe, fbf, c, p, p < p1, x1, x2, pp1, pp2, pp3, qp, psip, t);
 #pragma unroll
         for (int i = 0; i < NA; i++) acc[i] += reduced_value<NM, S>(t, i, lane);
-        x1 = nx1; x2 = nx2; pp1 = np1; pp2 = np2; pp3 = np3; qp = nq;
+        x1 = nx1; x2 = nx2; pp1 = np1; pp2 = np2; pp3 = np3; qp = nq; psip = npsi;
     }
     wave_reduce_from<NM, S>(acc);
     const int g = lane >> 2;
@@ -751,26 +777,29 @@ static JDepArgs jdep_args(qpad_ctx *c, int s, int l, bool store_diag, bool repor
     return a;
 }
 
-// species s0 .. s1-1 in one launch (grid rows by species)
-static int launch_jdep(qpad_ctx *c, int s0, int s1, int l, bool store_diag, bool timed, bool report)
+// the species of species-list entries in one launch (grid rows by list
+// entry), all of push type STD
+template <bool STD>
+static int launch_jdep_list(qpad_ctx *c, const int *list, int n, int l, bool store_diag, bool timed, bool report)
 {
     const bool speculative = l >= 2;
     JDepMulti m;
     memset(&m, 0, sizeof(m));
     const int cpw = c->dep_cells;
     int gx = 1;
-    for (int s = s0; s < s1; s++) {
-        // the report (the host's copy of the decision) from the first species only
-        m.s[s - s0] = jdep_args(c, s, l, store_diag, report && s == s0);
+    for (int i = 0; i < n; i++) {
+        const int s = list[i];
+        // the report (the host's copy of the decision) from the first entry only
+        m.s[i] = jdep_args(c, s, l, store_diag, report && i == 0);
         gx = std::max(gx, dep_grid(c->g.nr, c->sp[s].npp, c->sp[s].n_ov_max, cpw));
     }
-    dim3 grid((unsigned)gx, (unsigned)(s1 - s0)), blk(64 * cpw);
+    dim3 grid((unsigned)gx, (unsigned)n), blk(64 * cpw);
     hipEvent_t stop = nullptr;
     if (timed) time_begin(c, 0, &stop);
-#define QPAD_DEP2(MM, CC)                                                                              \
-    do {                                                                                               \
-        if (speculative) hipLaunchKernelGGL((k_amjdeposit<MM, true, CC>), grid, blk, 0, c->stream, m); \
-        else hipLaunchKernelGGL((k_amjdeposit<MM, false, CC>), grid, blk, 0, c->stream, m);            \
+#define QPAD_DEP2(MM, CC)                                                                                   \
+    do {                                                                                                    \
+        if (speculative) hipLaunchKernelGGL((k_amjdeposit<MM, true, CC, STD>), grid, blk, 0, c->stream, m); \
+        else hipLaunchKernelGGL((k_amjdeposit<MM, false, CC, STD>), grid, blk, 0, c->stream, m);            \
     } while (0)
 #define QPAD_DEP(MM)                                 \
     do {                                             \
@@ -792,6 +821,21 @@ static int launch_jdep(qpad_ctx *c, int s0, int s1, int l, bool store_diag, bool
     return 0;
 }
 
+// species s0 .. s1-1: one launch per push type present (robust first, which
+// carries the host report when there is one)
+static int launch_jdep(qpad_ctx *c, int s0, int s1, int l, bool store_diag, bool timed, bool report)
+{
+    int rob[kMaxSpecies], std_[kMaxSpecies], nr_ = 0, ns_ = 0;
+    for (int s = s0; s < s1; s++) {
+        if (c->sp[s].cfg.push_type == QPAD_PUSH2_STD) std_[ns_++] = s;
+        else rob[nr_++] = s;
+    }
+    int rc = 0;
+    if (nr_) rc = launch_jdep_list<false>(c, rob, nr_, l, store_diag, timed, report);
+    if (!rc && ns_) rc = launch_jdep_list<true>(c, std_, ns_, l, store_diag, timed && !nr_, report && !nr_);
+    return rc;
+}
+
 int launch_amjdeposit(qpad_ctx *c, int s, int l, bool store_diag, bool timed, bool report)
 {
     return launch_jdep(c, s, s + 1, l, store_diag, timed, report);
@@ -811,8 +855,9 @@ int launch_amjdeposit_all(qpad_ctx *c, int l, bool store_diag, bool timed, bool 
 struct PushArgs {
     int nr, ld, nm;
     int64_t nslots;
-    double idr, edge, qtmh, dt;
+    double idr, edge, qtmh, dt, qbm;
     double *x1, *x2, *p1, *p2, *p3, *gamma;
+    const double *psi;   // interpolated psi (standard pusher)
     const int32_t *cs;
     uint32_t *key;
     const double *e, *b;
@@ -823,7 +868,10 @@ struct PushMulti {
     PushArgs s[kMaxSpecies];
 };
 
-template <int M>
+// STD: push_u_std_part2d (part2d_class.f03:1790-1877), normalised by the
+// interpolated psi and the gamma the last current deposit stored, where
+// push_u_robust_part2d (:1879-1965) uses the particle's own gamma - p_z
+template <int M, bool STD>
 __global__ __launch_bounds__(256) void k_push(const PushMulti m)
 {
     const PushArgs a = m.s[__builtin_amdgcn_readfirstlane(blockIdx.y)];
@@ -850,9 +898,15 @@ __global__ __launch_bounds__(256) void k_push(const PushMulti m)
     bp[1] = bp[0] * ps + bp[1] * pc;
     bp[0] = t;
     double pp[3] = {a.p1[p], a.p2[p], a.p3[p]}, ut[3];
-    double gam = sqrt(1.0 + pp[0] * pp[0] + pp[1] * pp[1] + pp[2] * pp[2]);
-    double qtmh1 = a.qtmh / (gam - pp[2]);
-    double qtmh2 = qtmh1 * gam;
+    double qtmh1, qtmh2;
+    if constexpr (STD) {
+        qtmh1 = a.qtmh / (1.0 - a.qbm * a.psi[p]);
+        qtmh2 = qtmh1 * a.gamma[p];
+    } else {
+        const double gam = sqrt(1.0 + pp[0] * pp[0] + pp[1] * pp[1] + pp[2] * pp[2]);
+        qtmh1 = a.qtmh / (gam - pp[2]);
+        qtmh2 = qtmh1 * gam;
+    }
 #pragma unroll
     for (int cc = 0; cc < 3; cc++) {
         ep[cc] = ep[cc] * qtmh2;
@@ -895,7 +949,9 @@ static PushArgs push_args(qpad_ctx *c, int s)
     a.edge = (double)c->g.nr * c->g.dr;
     a.qtmh = sp.qbm * c->g.dxi * 0.5;
     a.dt = c->g.dxi;
+    a.qbm = sp.qbm;
     a.x1 = sp.x1; a.x2 = sp.x2; a.p1 = sp.p1; a.p2 = sp.p2; a.p3 = sp.p3; a.gamma = sp.gamma;
+    a.psi = sp.psi;
     a.cs = sp.cell_start;
     a.key = sp.key;
     a.e = c->f[QPAD_F_E];
@@ -903,29 +959,121 @@ static PushArgs push_args(qpad_ctx *c, int s)
     return a;
 }
 
-// species s0 .. s1-1 in one launch (species without particles get idle rows)
-static int launch_push_range(qpad_ctx *c, int s0, int s1)
+// the species of list entries in one launch (species without particles get
+// idle rows), all of push type STD
+template <bool STD>
+static int launch_push_list(qpad_ctx *c, const int *list, int n)
 {
     PushMulti m;
     memset(&m, 0, sizeof(m));
     int64_t nmax = 0;
-    for (int s = s0; s < s1; s++) {
-        m.s[s - s0] = push_args(c, s);
-        nmax = std::max(nmax, c->sp[s].npp);
+    for (int i = 0; i < n; i++) {
+        m.s[i] = push_args(c, list[i]);
+        nmax = std::max(nmax, c->sp[list[i]].npp);
     }
     if (nmax == 0) return 0;
-    dim3 grid((unsigned)((nmax + 255) / 256), (unsigned)(s1 - s0)), blk(256);
+    dim3 grid((unsigned)((nmax + 255) / 256), (unsigned)n), blk(256);
     hipEvent_t stop = nullptr;
     time_begin(c, 1, &stop);
     switch (c->g.M) {
-    case 0: hipLaunchKernelGGL(k_push<0>, grid, blk, 0, c->stream, m); break;
-    case 1: hipLaunchKernelGGL(k_push<1>, grid, blk, 0, c->stream, m); break;
-    case 2: hipLaunchKernelGGL(k_push<2>, grid, blk, 0, c->stream, m); break;
-    case 3: hipLaunchKernelGGL(k_push<3>, grid, blk, 0, c->stream, m); break;
-    default: hipLaunchKernelGGL(k_push<4>, grid, blk, 0, c->stream, m); break;
+    case 0: hipLaunchKernelGGL((k_push<0, STD>), grid, blk, 0, c->stream, m); break;
+    case 1: hipLaunchKernelGGL((k_push<1, STD>), grid, blk, 0, c->stream, m); break;
+    case 2: hipLaunchKernelGGL((k_push<2, STD>), grid, blk, 0, c->stream, m); break;
+    case 3: hipLaunchKernelGGL((k_push<3, STD>), grid, blk, 0, c->stream, m); break;
+    default: hipLaunchKernelGGL((k_push<4, STD>), grid, blk, 0, c->stream, m); break;
     }
     QPAD_HIP_CHECK(hipGetLastError());
     time_end(c, 1, stop);
+    return 0;
+}
+
+// species s0 .. s1-1: one launch per push type present
+static int launch_push_range(qpad_ctx *c, int s0, int s1)
+{
+    int rob[kMaxSpecies], std_[kMaxSpecies], nr_ = 0, ns_ = 0;
+    for (int s = s0; s < s1; s++) {
+        if (c->sp[s].cfg.push_type == QPAD_PUSH2_STD) std_[ns_++] = s;
+        else rob[nr_++] = s;
+    }
+    int rc = 0;
+    if (nr_) rc = launch_push_list<false>(c, rob, nr_);
+    if (!rc && ns_) rc = launch_push_list<true>(c, std_, ns_);
+    return rc;
+}
+
+// ---------------------------------------------------------------------------
+// interp_psi_part2d (part2d_class.f03:2264-2305): the field psi at every
+// particle of a standard-pusher species, after the slice's psi solve
+// (simulation_class.f03:357-359).  Thread per particle slot; particles past
+// cell_start[nr] have left the box.
+// ---------------------------------------------------------------------------
+struct InterpPsiArgs {
+    int nr, ld;
+    int64_t nslots;
+    double idr;
+    const double *x1, *x2, *f;
+    double *psi;
+    const int32_t *cs;
+};
+
+template <int M>
+__global__ __launch_bounds__(256) void k_interp_psi(const InterpPsiArgs a)
+{
+    const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= a.nslots || p >= a.cs[a.nr]) return;
+    constexpr int NM = 2 * M + 1;
+    const double x1 = a.x1[p], x2 = a.x2[p];
+    const double r = radius(x1, x2);
+    const double pc = x1 / r, ps = x2 / r;
+    double pos = __dmul_rn(r, a.idr);
+    if (!(pos >= 0.0)) pos = 0.0;
+    const int ic = (int)fmin(pos, (double)(a.nr - 1));
+    const int ix = ic + 1;
+    const double w1 = pos - (double)ic, w0 = 1.0 - w1;
+    // the gather of interp_field (interp_part2d.f03:66-110) for one
+    // component, in gather3's order of operations
+    const double *f = a.f + ix;
+    double v = 0.0;
+    v = v + f[0] * w0;
+    v = v + f[1] * w1;
+    double phr = 1.0, phi = 0.0;
+#pragma unroll
+    for (int m = 1; m <= M; m++) {
+        const double tr = phr * pc - phi * ps, ti = phr * ps + phi * pc;
+        phr = tr;
+        phi = ti;
+        const double ph_re = 2.0 * phr, ph_im = 2.0 * phi;
+        const double *fr = f + (size_t)(2 * m - 1) * a.ld, *fi = f + (size_t)(2 * m) * a.ld;
+        v = v + (fr[0] * ph_re - fi[0] * ph_im) * w0;
+        v = v + (fr[1] * ph_re - fi[1] * ph_im) * w1;
+    }
+    (void)NM;
+    a.psi[p] = v;
+}
+
+int launch_interp_psi(qpad_ctx *c, int s)
+{
+    Species &sp = c->sp[s];
+    if (sp.cfg.push_type != QPAD_PUSH2_STD || sp.npp == 0) return 0;
+    InterpPsiArgs a;
+    a.nr = c->g.nr;
+    a.ld = c->g.ld;
+    a.nslots = sp.npp;
+    a.idr = 1.0 / c->g.dr;
+    a.x1 = sp.x1;
+    a.x2 = sp.x2;
+    a.f = c->f[QPAD_F_PSI];
+    a.psi = sp.psi;
+    a.cs = sp.cell_start;
+    const dim3 grid((unsigned)((sp.npp + 255) / 256)), blk(256);
+    switch (c->g.M) {
+    case 0: hipLaunchKernelGGL(k_interp_psi<0>, grid, blk, 0, c->stream, a); break;
+    case 1: hipLaunchKernelGGL(k_interp_psi<1>, grid, blk, 0, c->stream, a); break;
+    case 2: hipLaunchKernelGGL(k_interp_psi<2>, grid, blk, 0, c->stream, a); break;
+    case 3: hipLaunchKernelGGL(k_interp_psi<3>, grid, blk, 0, c->stream, a); break;
+    default: hipLaunchKernelGGL(k_interp_psi<4>, grid, blk, 0, c->stream, a); break;
+    }
+    QPAD_HIP_CHECK(hipGetLastError());
     return 0;
 }
 

@@ -99,6 +99,7 @@ int launch_amjdeposit_all(qpad_ctx *c, int l, bool store_diag, bool timed, bool 
 // workgroups of the B_perp field kernel (one convergence maximum pair each)
 inline int conv_blocks(int nr) { return (nr + 2 + 255) / 256; }
 int launch_push(qpad_ctx *c, int sp);            // push_u + push_x + keys
+int launch_interp_psi(qpad_ctx *c, int sp);      // psi at the particles (standard pusher only)
 int launch_push_all(qpad_ctx *c);                // the same for every species in one launch
 int launch_add_qn_into_qspe(qpad_ctx *c, int sp);
 // per-species charge (species2d%q, cbq): q of species sp from its charge

@@ -440,8 +440,8 @@ int qpad_species_new(qpad_ctx *c, const qpad_species_cfg *cfg, int32_t *sp_id)
     if (c->nspecies >= kMaxSpecies) return fail(QPAD_E_INVALID, "too many species");
     if (cfg->ppc1 < 1 || cfg->ppc2 < 1 || cfg->num_theta < 1)
         return fail(QPAD_E_INVALID, "ppc and num_theta must be >= 1");
-    if (cfg->push_type != QPAD_PUSH2_ROBUST)
-        return fail(QPAD_E_INVALID, "only push_type 'robust' is implemented on the GPU path");
+    if (cfg->push_type != QPAD_PUSH2_ROBUST && cfg->push_type != QPAD_PUSH2_STD)
+        return fail(QPAD_E_INVALID, "push_type must be QPAD_PUSH2_STD or QPAD_PUSH2_ROBUST");
     if (cfg->mass == 0.0 || cfg->qm == 0.0) return fail(QPAD_E_INVALID, "q and m must be non-zero");
     // npmax: fdist2d_class.f03:258-262 (xtra = 2)
     const int64_t nlat = (int64_t)c->g.nr * cfg->ppc1 * cfg->ppc2 * cfg->num_theta;
@@ -782,6 +782,9 @@ static int slice_start(qpad_ctx *c, int j, bool overlap, SliceRun &st)
         for (int s = 0; s < c->nspecies; s++) TRY(launch_species_q_start(c, s));
     TRY(launch_slice_sources(c));
     TRY(launch_start_solves(c));
+    // species2d%interp_psi of the standard-pusher species
+    // (simulation_class.f03:357-359)
+    for (int s = 0; s < c->nspecies; s++) TRY(launch_interp_psi(c, s));
     // the first kernel that reads B_perp waits for the seed
     if (net) TRY(seednet_before_sources(c, j, &seed));
     TRY(launch_start_eperp(c, j, seed));
@@ -1002,6 +1005,13 @@ int qpad_species_amjdp(qpad_ctx *c, int32_t s, double dxi)
     TRY(launch_amjdeposit(c, s, 0, true, false));
     // add this species' cu/acu/amu into the totals
     TRY(launch_combine_j_species(c, s));
+    return sync(c);
+}
+
+int qpad_species_interp_psi(qpad_ctx *c, int32_t s)
+{
+    if (!c || s < 0 || s >= c->nspecies) return fail(QPAD_E_INVALID, "bad species");
+    TRY(launch_interp_psi(c, s));
     return sync(c);
 }
 

@@ -75,6 +75,9 @@ program qpad_slice_driver
       call chk(qpad_species_qdp(ctx, k - 1), 'species2d%qdp')
     end do
     call chk(qpad_field_solve_psi(ctx), 'field_psi%solve')
+    do k = 1, nsp
+      call chk(qpad_species_interp_psi(ctx, k - 1), 'species2d%interp_psi')
+    end do
     call chk(qpad_field_solve_bz(ctx, QPAD_F_B_SPE, QPAD_F_CU), 'field_b%solve(cu)')
     ! predictor-corrector iteration (simulation_class.f03:370-399)
     do l = 1, g%iter_max

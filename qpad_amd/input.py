@@ -12,7 +12,7 @@ import math
 import numpy as np
 
 from ._lib import (BND_OPEN, BND_ZERO, GEOM_CARTESIAN, GEOM_CYLINDRICAL, OFFSET_MAX, PROF_GAUSSIAN,
-                   PROF_PIECEWISE, PROF_UNIFORM, PUSH2_ROBUST, PW_MAX, BeamCfg, BeamStdCfg, Grid, QpadError,
+                   PROF_PIECEWISE, PROF_UNIFORM, PUSH2_ROBUST, PUSH2_STD, PW_MAX, BeamCfg, BeamStdCfg, Grid, QpadError,
                    SpeciesCfg)
 
 
@@ -77,14 +77,19 @@ def grid_from_deck(deck):
     return g, nstep3d, z0
 
 
+# species(i).push_type (sim_plasma_class.f03:123-137)
+_PUSHERS = {"standard": PUSH2_STD, "robust": PUSH2_ROBUST}
+
+
 def species_from_deck(sp, seed=0):
     """fdist2d_class.f03:117-272, sim_plasma_class.f03:116-157."""
     prof = _get(sp, "profile")
     if list(prof) != ["uniform", "uniform"]:
         raise QpadError("only uniform plasma profiles are on the GPU path")
-    push = _get(sp, "push_type", "robust", required=False)
-    if push != "robust":
-        raise QpadError('Invalid pusher type! only "robust" is on the GPU path')
+    push = _get(sp, "push_type")
+    if push not in _PUSHERS:
+        raise QpadError(f'Invalid pusher type "{push}": "standard" and "robust" are on the GPU path '
+                        '(the PGC pushers need the laser envelope)')
     ppc = _get(sp, "ppc")
     uth = _get(sp, "uth", [0.0, 0.0, 0.0], required=False)
     if _get(sp, "random_theta", False, required=False):
@@ -92,7 +97,7 @@ def species_from_deck(sp, seed=0):
     return SpeciesCfg(int(ppc[0]), int(ppc[1]), int(_get(sp, "num_theta")), float(_get(sp, "q")),
                       float(_get(sp, "m")), float(_get(sp, "density")), (ctypes_d3(uth)),
                       int(bool(_get(sp, "neutralized", True, required=False))),
-                      float(_get(sp, "den_min", 1.0e-10, required=False)), int(seed), PUSH2_ROBUST)
+                      float(_get(sp, "den_min", 1.0e-10, required=False)), int(seed), _PUSHERS[push])
 
 
 def ctypes_d3(v):

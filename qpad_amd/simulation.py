@@ -96,7 +96,7 @@ class FieldDjdxi(Field):
 
 
 class Species2D:
-    """species2d (species2d_class.f03:24-54), robust pusher."""
+    """species2d (species2d_class.f03:24-54), standard or robust pusher."""
 
     def __init__(self, sim, cfg):
         self.sim = sim
@@ -117,6 +117,13 @@ class Species2D:
         if (ef.name, bf.name, cu.name, amu.name, acu.name) != ("e", "b", "cu", "amu", "acu"):
             raise L.QpadError("species2d%amjdp(e, b, cu, amu, acu, dxi)")
         L.check(self.sim.lib.qpad_species_amjdp(self.sim.ctx, self.id, float(dxi)))
+
+    def interp_psi(self, psi):
+        """species2d%interp_psi (species2d_class.f03:439-452): psi at the
+        particles of a standard-pusher species (no effect on a robust one)"""
+        if psi.name != "psi":
+            raise L.QpadError("species2d%interp_psi takes psi")
+        L.check(self.sim.lib.qpad_species_interp_psi(self.sim.ctx, self.id))
 
     def push(self, dxi):
         """push_u + push_x (+ update_bound)."""

@@ -65,9 +65,23 @@ MUTANTS = {
     "amj_axis_m2_flux_zeroed": ("for (int c = 0; c < 3; c++) FA(amu, c, k, 1) = 8.0 * FA(amu, c, k, 1);",
                                 "for (int c = 0; c < 3; c++) FA(amu, c, k, 1) = 0.0 * FA(amu, c, k, 1);",
                                 ["amjdeposit"]),
-    "amj_flux_missing_ipsi": ("u2[0] = u[0] * u[0] * ipsi;", "u2[0] = u[0] * u[0];", ["amjdeposit"]),
+    # text shared by the robust and the standard deposit: planted in both
+    "amj_flux_missing_ipsi": ("u2[0] = u[0] * u[0] * ipsi;", "u2[0] = u[0] * u[0];",
+                              ["amjdeposit", "amjdeposit_std"], 2),
     "amj_phase_not_conjugated": ("double ph0r = pc, ph0i = -ps;", "double ph0r = pc, ph0i = ps;",
-                                 ["amjdeposit"]),
+                                 ["amjdeposit", "amjdeposit_std"], 2),
+    "std_normalisation_sign": ("double qtmh2 = qtmh / (1.0 - sp->qbm * sp->psi[p]);",
+                               "double qtmh2 = qtmh / (1.0 + sp->qbm * sp->psi[p]);", ["amjdeposit_std"]),
+    "std_weight_uses_gamma": ("double ipsi = 1.0 / (1.0 - sp->qbm * sp->psi[p]);",
+                              "double ipsi = 1.0 / (sp->gamma[p] - u[2]);", ["amjdeposit_std"]),
+    "std_second_kick_old_gamma": ("qtmh1 = qtmh2 * gam;\n        for (int c = 0; c < 3; c++) u[c] = ut[c] + ep[c] * qtmh1;",
+                                  "for (int c = 0; c < 3; c++) u[c] = ut[c] + ep[c] * qtmh1;",
+                                  ["amjdeposit_std"]),
+    "std_push_own_gamma": ("double qtmh2 = qtmh1 * sp->gamma[p];",
+                           "double qtmh2 = qtmh1 * sqrt(1.0 + pp[0] * pp[0] + pp[1] * pp[1] + pp[2] * pp[2]);",
+                           ["push_std"]),
+    "interp_psi_weights_swapped": ("interp_vec(s, psi, 1, ix, w0, w1, pc, ps, &sp->psi[p]);",
+                                   "interp_vec(s, psi, 1, ix, w1, w0, pc, ps, &sp->psi[p]);", ["interp_psi"]),
     "beam_xi_drift": ("bm->x3[p] = bm->x3[p] - bm->p3[p] * dt_gam + bm->dt;",
                       "bm->x3[p] = bm->x3[p] - bm->p3[p] * dt_gam;", ["beam_push"]),
     "beam_force_sign": ("wp[0] = ep[0] - bp[1];\n        wp[1] = ep[1] + bp[0];\n        wp[2] = ep[2];\n"
@@ -101,9 +115,13 @@ MUTANTS = {
 
 
 def source_of(name):
+    """the oracle's source with mutant `name` planted: its text must occur as
+    often as the mutant says (once, unless a fourth entry gives the count of
+    places that share it), and every occurrence is replaced"""
     text = open(SRC).read()
-    old, new, _ = MUTANTS[name]
-    assert text.count(old) == 1, f"mutant {name}: the original text must occur exactly once"
+    old, new = MUTANTS[name][:2]
+    count = MUTANTS[name][3] if len(MUTANTS[name]) > 3 else 1
+    assert text.count(old) == count, f"mutant {name}: the original text must occur exactly {count} time(s)"
     return text.replace(old, new)
 
 

@@ -508,6 +508,160 @@ def pin_amjdeposit(L):
         assert np.max(np.abs(got - want[name])) <= 1e-13, f"E_z kick: {name}"
 
 
+def _deposit_std(L, o, e, b, dxi):
+    cu, amu, acu = o.new_field(3), o.new_field(3), o.new_field(2)
+    L.ora_amjdeposit_std(o.h, o.species_raw(0), po.ptr(e), po.ptr(b), po.ptr(cu), po.ptr(amu), po.ptr(acu), dxi)
+    return cu, amu, acu
+
+
+def pin_interp_psi(L):
+    """psi at a particle is the mode sum f0 + 2 sum_m Re(f_m e^{i m phi}) of
+    the linear radial interpolation: exact for a field linear in r per mode"""
+    nr, dr, M = 16, 0.1, 3
+    o = po.Oracle(po.make_grid(nr, 4, M, dr, 0.05), L)
+    o.add_species(po.species_cfg(1, 1, 4, push_type=po.PUSH2_STD))
+    psi = o.new_field(1)
+    rng = np.random.default_rng(3)
+    a, c = rng.standard_normal(2 * M + 1), rng.standard_normal(2 * M + 1)
+    r_node = (np.arange(nr + 2) - 1) * dr
+    for k in range(2 * M + 1):
+        psi[0, k, :] = a[k] + c[k] * r_node
+    for rp, phi in ((0.37 * dr, 0.9), (5.6 * dr, -2.2), (11.05 * dr, 3.0)):
+        x1, x2 = rp * math.cos(phi), rp * math.sin(phi)
+        _one_particle(o, x1, x2, (0.1, 0.2, 0.3))
+        L.ora_interp_psi(o.h, o.species_raw(0), po.ptr(psi))
+        want = a[0] + c[0] * rp
+        for m in range(1, M + 1):
+            fre, fim = a[kre(m)] + c[kre(m)] * rp, a[kim(m)] + c[kim(m)] * rp
+            want += 2 * (fre * math.cos(m * phi) - fim * math.sin(m * phi))
+        got = o.species(0)["psi"][0]
+        assert abs(got - want) <= 1e-13 * max(1.0, abs(want)), (rp, phi, got, want)
+
+
+def pin_amjdeposit_std(L):
+    """The standard deposit (part2d_class.f03:478-744): in zero fields a
+    particle deposits u / (1 - q/m psi) with its interpolated psi, and where
+    that psi equals the quasi-static (1 - (gamma - u_z)) / (q/m) it deposits
+    exactly what the robust deposit does; the stored gamma is the
+    time-centred one.  In a uniform B_z alone the Boris rotation keeps |u_perp|
+    and turns it by 2 atan(t), t = q/m dxi/2 B_z / (1 - q/m psi)."""
+    nr, dr, M, dxi = 16, 0.1, 3, 0.05
+    o = po.Oracle(po.make_grid(nr, 4, M, dr, dxi), L)
+    o.add_species(po.species_cfg(1, 1, 4, push_type=po.PUSH2_STD))
+    e, b = o.new_field(3), o.new_field(3)
+    q, qbm = -0.7, -1.0
+    for (rp, phi, psi_p) in ((0.37 * dr, 0.9, 0.3), (5.6 * dr, 2.2, -0.45)):
+        p = (0.3, -0.2, 0.4)
+        x1, x2 = rp * math.cos(phi), rp * math.sin(phi)
+        _one_particle(o, x1, x2, p, q)
+        sp = o.species(0)
+        sp["psi"][0] = psi_p
+        cu, amu, acu = _deposit_std(L, o, e, b, dxi)
+        ur = p[0] * math.cos(phi) + p[1] * math.sin(phi)
+        up = -p[0] * math.sin(phi) + p[1] * math.cos(phi)
+        uz = p[2]
+        ipsi = 1.0 / (1.0 - qbm * psi_p)
+        want = _expected(nr, dr, M, x1, x2, {
+            "cu": [q * ipsi * ur, q * ipsi * up, q * ipsi * uz],
+            "amu": [q * ipsi ** 2 * ur * ur, q * ipsi ** 2 * ur * up, q * ipsi ** 2 * up * up],
+            "acu": [0.0, 0.0]})
+        for name, got in (("cu", cu), ("amu", amu), ("acu", acu)):
+            err = np.max(np.abs(got - want[name]))
+            assert err <= 1e-13, f"{name}, particle at r = {rp / dr:.2f} dr: {err:.2e}"
+        assert abs(o.species(0)["gamma"][0] - math.sqrt(1 + ur * ur + up * up + uz * uz)) <= 1e-14
+        # psi of the quasi-static relation: the robust deposit's result
+        g = math.sqrt(1 + ur * ur + up * up + uz * uz)
+        _one_particle(o, x1, x2, p, q)
+        rob = _deposit(L, o, e, b, dxi)
+        _one_particle(o, x1, x2, p, q)
+        o.species(0)["psi"][0] = (1.0 - (g - uz)) / qbm
+        std = _deposit_std(L, o, e, b, dxi)
+        for got, ref in zip(std, rob):
+            assert np.max(np.abs(got - ref)) <= 1e-13 * max(1.0, np.max(np.abs(ref)))
+    # uniform E_z from rest: the first half kick is normalised by the old
+    # gamma (1), the second by the gamma after the first, each over
+    # (1 - q/m psi); time-centred u_z / 2 in the deposit
+    ez, psi_p = 0.8, 0.3
+    e[2, 0, :] = ez
+    rp, phi = 3.3 * dr, 0.4
+    x1, x2 = rp * math.cos(phi), rp * math.sin(phi)
+    _one_particle(o, x1, x2, (0.0, 0.0, 0.0), q)
+    o.species(0)["psi"][0] = psi_p
+    cu, amu, acu = _deposit_std(L, o, e, b, dxi)
+    k = 0.5 * qbm * dxi / (1.0 - qbm * psi_p)
+    u1 = k * ez
+    uz = 0.5 * (u1 + k * math.sqrt(1 + u1 * u1) * ez)
+    ipsi = 1.0 / (1.0 - qbm * psi_p)
+    want = _expected(nr, dr, M, x1, x2, {"cu": [0.0, 0.0, q * ipsi * uz], "amu": [0.0] * 3,
+                                         "acu": [0.0, 0.0]})
+    # acu: du_perp = 0 and u_perp = 0, so the dpsi terms vanish too
+    for name, got in (("cu", cu), ("amu", amu), ("acu", acu)):
+        assert np.max(np.abs(got - want[name])) <= 1e-15, f"E_z kick: {name}"
+    assert abs(o.species(0)["gamma"][0] - math.sqrt(1 + uz * uz)) <= 1e-15
+    e[2, 0, :] = 0.0
+    # uniform B_z: norm-preserving rotation by 2 atan(t); the deposited dcu is
+    # the change of u_perp over dxi (time-centred flux of u_perp)
+    bz, psi_p = 2.0, 0.25
+    b[2, 0, :] = bz
+    rp, phi = 3.3 * dr, 0.0
+    p = (0.5, 0.0, 0.2)
+    _one_particle(o, rp, 0.0, p, q)
+    o.species(0)["psi"][0] = psi_p
+    cu, amu, acu = _deposit_std(L, o, e, b, dxi)
+    t = 0.5 * qbm * dxi * bz / (1.0 - qbm * psi_p)
+    ang = 2.0 * math.atan(t)
+    # u_perp after the step, rotated by -ang about z (the Boris sense for t > 0)
+    ux, uy = p[0] * math.cos(ang), -p[0] * math.sin(ang)
+    ipsi = 1.0 / (1.0 - qbm * psi_p)
+    du = ((ux - p[0]) / dxi, (uy - p[1]) / dxi)
+    um = (0.5 * (ux + p[0]), 0.5 * (uy + p[1]), p[2])
+    dpsi = 0.0   # no E and no B_perp: the wake force w = (E_r - B_phi, E_phi + B_r, E_z) is zero
+    want = _expected(nr, dr, M, rp, 0.0, {
+        "cu": [q * ipsi * um[0], q * ipsi * um[1], q * ipsi * um[2]],
+        "acu": [q * ipsi * (du[0] + um[0] * dpsi * ipsi), q * ipsi * (du[1] + um[1] * dpsi * ipsi)],
+        "amu": [q * ipsi ** 2 * um[0] * um[0], q * ipsi ** 2 * um[0] * um[1], q * ipsi ** 2 * um[1] * um[1]]})
+    for name, got in (("cu", cu), ("amu", amu), ("acu", acu)):
+        err = np.max(np.abs(got - want[name])) / max(1e-300, np.max(np.abs(want[name])))
+        assert err <= 1e-12, f"B_z rotation: {name} {err:.2e}"
+
+
+def pin_push_std(L):
+    """push_u_std_part2d (part2d_class.f03:1790-1877): both half kicks use
+    the gamma stored by the last deposit, so from rest in a uniform E_z
+    p_z = 2 (q/m dxi/2) gamma_s E_z / (1 - q/m psi); a uniform B_z turns p_perp
+    by 2 atan(t) at constant |p_perp|; the new gamma is sqrt(1 + p^2)"""
+    nr, dr, M, dxi = 16, 0.1, 2, 0.05
+    o = po.Oracle(po.make_grid(nr, 4, M, dr, dxi), L)
+    o.add_species(po.species_cfg(1, 1, 4, push_type=po.PUSH2_STD))
+    e, b = o.new_field(3), o.new_field(3)
+    qbm, ez, psi_p, gam_s = -1.0, 0.8, 0.3, 1.7
+    e[2, 0, :] = ez
+    rp, phi = 4.3 * dr, 1.1
+    _one_particle(o, rp * math.cos(phi), rp * math.sin(phi), (0.0, 0.0, 0.0))
+    sp = o.species(0)
+    sp["psi"][0] = psi_p
+    sp["gamma"][0] = gam_s
+    L.ora_push_u_std(o.h, o.species_raw(0), po.ptr(e), po.ptr(b), dxi)
+    s2 = o.species(0)
+    pz = 2 * (0.5 * qbm * dxi) * gam_s * ez / (1.0 - qbm * psi_p)
+    assert abs(s2["p3"][0] - pz) <= 1e-14 and abs(s2["p1"][0]) <= 1e-15 and abs(s2["p2"][0]) <= 1e-15
+    assert abs(s2["gamma"][0] - math.sqrt(1 + pz * pz)) <= 1e-14
+    e[2, 0, :] = 0.0
+    bz = 1.5
+    b[2, 0, :] = bz
+    p0 = (0.4, 0.3, 0.2)
+    _one_particle(o, rp * math.cos(phi), rp * math.sin(phi), p0)
+    o.species(0)["psi"][0] = psi_p
+    L.ora_push_u_std(o.h, o.species_raw(0), po.ptr(e), po.ptr(b), dxi)
+    s3 = o.species(0)
+    t = 0.5 * qbm * dxi * bz / (1.0 - qbm * psi_p)
+    ang = 2.0 * math.atan(t)
+    c_, s_ = math.cos(ang), math.sin(ang)
+    want = (p0[0] * c_ + p0[1] * s_, -p0[0] * s_ + p0[1] * c_)
+    assert abs(s3["p1"][0] - want[0]) <= 1e-14 and abs(s3["p2"][0] - want[1]) <= 1e-14
+    assert abs(s3["p3"][0] - p0[2]) <= 1e-15
+
+
 def pin_beam_push(L):
     nr, nxi, dr, dxi, dt = 16, 20, 0.1, 0.1, 3.0
     M = 2
@@ -682,6 +836,9 @@ PINS = {
     "bt_iter": pin_bt_iter,
     "bt_beam": pin_bt_beam,
     "amjdeposit": pin_amjdeposit,
+    "interp_psi": pin_interp_psi,
+    "amjdeposit_std": pin_amjdeposit_std,
+    "push_std": pin_push_std,
     "beam_push": pin_beam_push,
     "beam_inject": pin_beam_inject,
     "beam_lattice": pin_beam_lattice,

@@ -25,6 +25,8 @@ def fine_grained_slice(sim, j):
     for sp in sim.spe:
         sp.qdp(f["q_spe"])
     f["psi"].solve(f["q_spe"])
+    for sp in sim.spe:
+        sp.interp_psi(f["psi"])                              # standard pusher only
     f["b_spe"].solve(f["cu"])                                # Bz from the J predictor
     iters = 0
     for l in range(1, g.iter_max + 1):
@@ -63,10 +65,10 @@ def fine_grained_slice(sim, j):
     return iters
 
 
-@pytest.mark.parametrize("max_mode,ions", [(1, False), (2, True)])
-def test_fine_grained_slices_match_oracle(max_mode, ions):
+@pytest.mark.parametrize("max_mode,ions,push", [(1, False, "robust"), (2, True, "robust"), (2, True, "standard")])
+def test_fine_grained_slices_match_oracle(max_mode, ions, push):
     deck = configs.small(nr=40, nxi=48, max_mode=max_mode, ppc=(2, 4), total_num=1 << 12, ions=ions,
-                         offset_x=0.03)
+                         offset_x=0.03, push_type=push)
     sim = Simulation(deck)
     o = twin.make_twin(sim)
     sim.step3d_begin()

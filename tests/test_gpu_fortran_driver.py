@@ -19,10 +19,10 @@ pytestmark = [pytest.mark.gpu,
 TOL = 1e-10
 
 
-@pytest.mark.parametrize("max_mode,ions", [(1, False), (2, True)])
-def test_fortran_driver_step_matches_fused_path(tmp_path, max_mode, ions):
+@pytest.mark.parametrize("max_mode,ions,push", [(1, False, "robust"), (2, True, "robust"), (2, True, "standard")])
+def test_fortran_driver_step_matches_fused_path(tmp_path, max_mode, ions, push):
     deck = configs.small(nr=48, nxi=40, max_mode=max_mode, ppc=(2, 4), total_num=1 << 12, ions=ions,
-                         offset_x=0.03)
+                         offset_x=0.03, push_type=push)
     print("fortran driver: max_mode", max_mode, "ions", ions, flush=True)
     g, iters, fields, parts = fortran_driver.run(deck, str(tmp_path))
     sim = Simulation(deck)

@@ -45,7 +45,7 @@ def test_total_charge_definition():
 
 
 @pytest.mark.parametrize("edit,match", [
-    (lambda d: d["species"][0].update(push_type="standard"), "pusher"),
+    (lambda d: d["species"][0].update(push_type="standard_pgc"), "pusher"),
     (lambda d: d["species"][0].update(profile=["parabolic-channel", "uniform"]), "uniform"),
     (lambda d: d["simulation"].update(nlasers=1), "nlasers"),
     (lambda d: d["simulation"].update(field_boundary="periodic"), "boundary"),
@@ -171,3 +171,16 @@ def test_file_beam_placement_matches_the_oracle(oracle_mod):
     assert 0 < len(mine["x1"]) < n
     for k in ref:
         assert np.array_equal(mine[k], ref[k]), k
+
+
+def test_pusher_types():
+    """species(i).push_type (sim_plasma_class.f03:123-137): "standard" and
+    "robust" map onto the library's pushers; the key is required, as in QPAD"""
+    from qpad_amd._lib import PUSH2_ROBUST, PUSH2_STD
+    d = configs.small(push_type="standard", ions=True, ion_push_type="robust")
+    assert species_from_deck(d["species"][0]).push_type == PUSH2_STD
+    assert species_from_deck(d["species"][1]).push_type == PUSH2_ROBUST
+    sp = dict(d["species"][0])
+    del sp["push_type"]
+    with pytest.raises(QpadError, match="push_type"):
+        species_from_deck(sp)

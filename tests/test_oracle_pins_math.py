@@ -30,5 +30,5 @@ def test_pins_see_planted_mistake(name, mutant_libs):
 
 
 def test_every_pin_has_a_mutant():
-    caught = {p for _, _, pins in oracle_mutants.MUTANTS.values() for p in pins}
+    caught = {p for m in oracle_mutants.MUTANTS.values() for p in m[2]}
     assert caught == set(oracle_pins.PINS)

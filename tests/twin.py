@@ -23,7 +23,7 @@ def make_twin(sim):
     for sp in sim.spe:
         c = sp.cfg
         o.add_species(po.species_cfg(c.ppc1, c.ppc2, c.num_theta, c.qm, c.mass, c.density,
-                                     tuple(c.uth), bool(c.neutralized), c.den_min, c.seed))
+                                     tuple(c.uth), bool(c.neutralized), c.den_min, c.seed, c.push_type))
     for b in sim.beam:
         arr = b.get()
         c = b.cfg
