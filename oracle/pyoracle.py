@@ -154,6 +154,7 @@ def load_library(path):
     L.ora_species_inject.argtypes = [C.c_void_p, C.c_int]
     L.ora_species_init_q.argtypes = [C.c_void_p, C.c_int]
     L.ora_last_iters.argtypes = [C.c_void_p]
+    L.ora_last_residuals.argtypes = [C.c_void_p, _dp, _dp]
     L.ora_philox4x32.argtypes = [C.POINTER(C.c_uint32), C.POINTER(C.c_uint32),
                                  C.POINTER(C.c_uint32)]
     L.ora_normal4.argtypes = [C.c_uint64, C.c_uint64, C.c_uint32, _dp]
@@ -373,6 +374,12 @@ class Oracle:
 
     def last_iters(self):
         return self.L.ora_last_iters(self.h)
+
+    def last_residuals(self):
+        """(relative, absolute) residual of the last slice's final 'compare'"""
+        r, a = np.zeros(1), np.zeros(1)
+        self.L.ora_last_residuals(self.h, ptr(r), ptr(a))
+        return float(r[0]), float(a[0])
 
     # -- building blocks on this instance's grid
     def new_field(self, ncomp):

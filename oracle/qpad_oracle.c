@@ -1768,3 +1768,8 @@ int ora_ld(const ora_sim *s) { return s->ld; }
 ora_species *ora_get_species(ora_sim *s, int isp) { return &s->sp[isp]; }
 ora_beam *ora_get_beam(ora_sim *s, int ib) { return &s->bm[ib]; }
 int ora_last_iters(const ora_sim *s) { return s->last_iters; }
+void ora_last_residuals(const ora_sim *s, double *rel, double *abs_)
+{
+    *rel = s->last_rel;
+    *abs_ = s->last_abs;
+}

@@ -224,6 +224,8 @@ int ora_ld(const ora_sim *s);
 ora_species *ora_get_species(ora_sim *s, int isp);
 ora_beam *ora_get_beam(ora_sim *s, int ib);
 int ora_last_iters(const ora_sim *s);
+/* the relative and absolute residuals of the last slice's final 'compare' */
+void ora_last_residuals(const ora_sim *s, double *rel, double *abs_);
 
 /* timers by QPAD's event names (sysutil_module.f03:307-330; 'set source'
    is field_src_class.f03:293): seconds accumulated since ora_timing(1) */

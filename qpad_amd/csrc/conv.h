@@ -12,14 +12,30 @@
 
 namespace qpad {
 
-// this lane's pair of workgroup maxima (-inf past nblk); nblk <= 64
+// most workgroup maxima a wave reduces (three per lane)
+constexpr int kConvMaxBlocks = 192;
+
+// this lane's maxima over the workgroup maxima lane, lane + 64, lane + 128
+// (-inf past nblk); nblk <= kConvMaxBlocks.  All loads are issued first.
 __device__ inline void conv_load(const double *blk, int nblk, double &o, double &d)
 {
     const int lane = threadIdx.x & 63;
-    const int q = min(lane, nblk - 1);
-    const double o1 = blk[2 * q], d1 = blk[2 * q + 1];
-    o = lane < nblk ? o1 : -INFINITY;
-    d = lane < nblk ? d1 : -INFINITY;
+    double ov[3], dv[3];
+#pragma unroll
+    for (int u = 0; u < 3; u++) {
+        const int q = min(lane + 64 * u, nblk - 1);
+        ov[u] = blk[2 * q];
+        dv[u] = blk[2 * q + 1];
+    }
+    o = -INFINITY;
+    d = -INFINITY;
+#pragma unroll
+    for (int u = 0; u < 3; u++) {
+        if (lane + 64 * u < nblk) {
+            o = fmax(o, ov[u]);
+            d = fmax(d, dv[u]);
+        }
+    }
 }
 
 // rel and abs residuals: ||old - new||_inf and that over ||old||_inf (huge

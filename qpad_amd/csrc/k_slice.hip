@@ -24,6 +24,12 @@
 
 namespace qpad {
 
+// threads per workgroup of the once-per-slice node kernels (k_start_src,
+// k_eperp, k_epilogue: a thread per node and mode array); small workgroups
+// spread the few thousand nodes over many CUs
+constexpr int kNodeThreads = 256;
+inline int node_blocks(int nr) { return (nr + 2 + kNodeThreads - 1) / kNodeThreads; }
+
 // src buffer rows: B+/B- of mode m at 4m + s (s = Re B+, Im B+, Re B-,
 // Im B-), Bz of mode array k at 4(M+1) + k, Ez at 4(M+1) + nm + k, psi at
 // 4(M+1) + 2nm + k; row r = i - 1 of each system at [sys * nr + r]
@@ -48,7 +54,7 @@ struct SrcArgs {
 
 // nodes per k_src workgroup; its LDS tile also holds a halo of 2 nodes on
 // each side (the sources use nodes i-2 .. i+1)
-constexpr int kSrcTile = 32;
+constexpr int kSrcTile = 16;
 constexpr int kSrcHalo = 2;
 constexpr int kSrcTL = kSrcTile + 2 * kSrcHalo;
 static_assert(kSrcTile * (kMaxMode + 1) <= 256, "one B+/B- item per k_src thread");
@@ -398,8 +404,11 @@ struct BfieldArgs {
 // recorded one, max over the workgroup's nodes (the next iteration's deposit
 // reduces the workgroup maxima).  Loads are issued for all modes before any
 // arithmetic.
+static_assert((16 * kSolveThreads + 2 + kBfThreads - 1) / kBfThreads <= kConvMaxBlocks,
+              "the next deposit reduces every B_perp workgroup's maxima");
+
 template <int M>
-__global__ __launch_bounds__(256) void k_bfields(BfieldArgs a)
+__global__ __launch_bounds__(kBfThreads) void k_bfields(BfieldArgs a)
 {
     constexpr int NM = 2 * M + 1;
     __shared__ double red[8];
@@ -558,13 +567,13 @@ int launch_iteration_fields(qpad_ctx *c, int j, int l)
     f.it = c->it;
     f.l = l;
     f.late_done = c->adaptive;
-    const dim3 fgrid((unsigned)((c->g.nr + 2 + 255) / 256));
+    const dim3 fgrid((unsigned)conv_blocks(c->g.nr)), fblk(kBfThreads);
     switch (c->g.M) {
-    case 0: hipLaunchKernelGGL(k_bfields<0>, fgrid, dim3(256), 0, c->stream, f); break;
-    case 1: hipLaunchKernelGGL(k_bfields<1>, fgrid, dim3(256), 0, c->stream, f); break;
-    case 2: hipLaunchKernelGGL(k_bfields<2>, fgrid, dim3(256), 0, c->stream, f); break;
-    case 3: hipLaunchKernelGGL(k_bfields<3>, fgrid, dim3(256), 0, c->stream, f); break;
-    default: hipLaunchKernelGGL(k_bfields<4>, fgrid, dim3(256), 0, c->stream, f); break;
+    case 0: hipLaunchKernelGGL(k_bfields<0>, fgrid, fblk, 0, c->stream, f); break;
+    case 1: hipLaunchKernelGGL(k_bfields<1>, fgrid, fblk, 0, c->stream, f); break;
+    case 2: hipLaunchKernelGGL(k_bfields<2>, fgrid, fblk, 0, c->stream, f); break;
+    case 3: hipLaunchKernelGGL(k_bfields<3>, fgrid, fblk, 0, c->stream, f); break;
+    default: hipLaunchKernelGGL(k_bfields<4>, fgrid, fblk, 0, c->stream, f); break;
     }
     QPAD_HIP_CHECK(hipGetLastError());
     return 0;
@@ -589,7 +598,7 @@ struct EpiArgs {
 };
 
 // thread per (node i = 0..nr+1, mode array k)
-__global__ __launch_bounds__(256) void k_epilogue(EpiArgs a)
+__global__ __launch_bounds__(kNodeThreads) void k_epilogue(EpiArgs a)
 {
     const FieldCommon &g = a.g;
     const int nm = g.nm, ld = g.ld, nr = g.nr;
@@ -672,8 +681,8 @@ int launch_epilogue(qpad_ctx *c, int j)
     a.it = c->it;
     a.conv_blk = c->conv_blk;
     a.nblk = conv_blocks(c->g.nr);
-    hipLaunchKernelGGL(k_epilogue, dim3((unsigned)((c->g.nr + 2 + 255) / 256), c->g.nm), dim3(256), 0, c->stream,
-                       a);
+    hipLaunchKernelGGL(k_epilogue, dim3((unsigned)node_blocks(c->g.nr), c->g.nm), dim3(kNodeThreads), 0,
+                       c->stream, a);
     QPAD_HIP_CHECK(hipGetLastError());
     return 0;
 }
@@ -696,7 +705,7 @@ struct StartSrcArgs {
 // qdeposit_part2d, part2d_class.f03:313-336) plus the neutralising
 // backgrounds; the psi, Bz and Ez right-hand sides at node i
 template <int NS>
-__global__ __launch_bounds__(256) void k_start_src(StartSrcArgs a)
+__global__ __launch_bounds__(kNodeThreads) void k_start_src(StartSrcArgs a)
 {
     const FieldCommon &g = a.g;
     const int nm = g.nm, ld = g.ld, nr = g.nr, M = g.M;
@@ -817,7 +826,7 @@ struct EperpArgs {
 // (field_e_class.f03:443-504); the k = 0 threads also run 'record' of
 // sum_m |B_phi,m| at their node (simulation_class.f03:549-558) and reset
 // the iteration state.  The first kernel of the slice that reads B_perp.
-__global__ __launch_bounds__(256) void k_eperp(EperpArgs a)
+__global__ __launch_bounds__(kNodeThreads) void k_eperp(EperpArgs a)
 {
     const FieldCommon &g = a.g;
     const int nm = g.nm, ld = g.ld, nr = g.nr, M = g.M;
@@ -865,7 +874,7 @@ int launch_slice_sources(qpad_ctx *c)
 {
     const FieldCommon g = field_common(c);
     const int nm = c->g.nm;
-    const dim3 ngrid((unsigned)((c->g.nr + 2 + 255) / 256), nm);
+    const dim3 ngrid((unsigned)node_blocks(c->g.nr), nm);
     StartSrcArgs s;
     memset(&s, 0, sizeof(s));
     s.g = g;
@@ -881,10 +890,10 @@ int launch_slice_sources(qpad_ctx *c)
     s.q_spe = c->f[QPAD_F_Q_SPE];
     s.src = c->src;
     switch (c->nspecies) {
-    case 1: hipLaunchKernelGGL(k_start_src<1>, ngrid, dim3(256), 0, c->stream, s); break;
-    case 2: hipLaunchKernelGGL(k_start_src<2>, ngrid, dim3(256), 0, c->stream, s); break;
-    case 3: hipLaunchKernelGGL(k_start_src<3>, ngrid, dim3(256), 0, c->stream, s); break;
-    default: hipLaunchKernelGGL(k_start_src<4>, ngrid, dim3(256), 0, c->stream, s); break;
+    case 1: hipLaunchKernelGGL(k_start_src<1>, ngrid, dim3(kNodeThreads), 0, c->stream, s); break;
+    case 2: hipLaunchKernelGGL(k_start_src<2>, ngrid, dim3(kNodeThreads), 0, c->stream, s); break;
+    case 3: hipLaunchKernelGGL(k_start_src<3>, ngrid, dim3(kNodeThreads), 0, c->stream, s); break;
+    default: hipLaunchKernelGGL(k_start_src<4>, ngrid, dim3(kNodeThreads), 0, c->stream, s); break;
     }
     QPAD_HIP_CHECK(hipGetLastError());
     return 0;
@@ -912,7 +921,7 @@ int launch_start_solves(qpad_ctx *c)
 int launch_start_eperp(qpad_ctx *c, int j, const double *seed)
 {
     const int nm = c->g.nm;
-    const dim3 ngrid((unsigned)((c->g.nr + 2 + 255) / 256), nm);
+    const dim3 ngrid((unsigned)node_blocks(c->g.nr), nm);
     EperpArgs e;
     e.g = field_common(c);
     e.b_spe = c->f[QPAD_F_B_SPE];
@@ -924,7 +933,7 @@ int launch_start_eperp(qpad_ctx *c, int j, const double *seed)
     e.conv_re = c->conv_re;
     e.conv_im = c->conv_im;
     e.it = c->it;
-    hipLaunchKernelGGL(k_eperp, ngrid, dim3(256), 0, c->stream, e);
+    hipLaunchKernelGGL(k_eperp, ngrid, dim3(kNodeThreads), 0, c->stream, e);
     QPAD_HIP_CHECK(hipGetLastError());
     return 0;
 }

@@ -96,8 +96,10 @@ int launch_amjdeposit(qpad_ctx *c, int sp, int l, bool store_diag, bool timed, b
 // the same for every species in one launch (the workgroup rows are the
 // species); report: from species 0
 int launch_amjdeposit_all(qpad_ctx *c, int l, bool store_diag, bool timed, bool report = false);
-// workgroups of the B_perp field kernel (one convergence maximum pair each)
-inline int conv_blocks(int nr) { return (nr + 2 + 255) / 256; }
+// threads per workgroup of the B_perp field kernel, and its workgroups (one
+// convergence maximum pair each; at most kConvMaxBlocks)
+constexpr int kBfThreads = 64;
+inline int conv_blocks(int nr) { return (nr + 2 + kBfThreads - 1) / kBfThreads; }
 int launch_push(qpad_ctx *c, int sp);            // push_u + push_x + keys
 int launch_interp_psi(qpad_ctx *c, int sp);      // psi at the particles (standard pusher only)
 int launch_push_all(qpad_ctx *c);                // the same for every species in one launch
