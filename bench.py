@@ -119,16 +119,21 @@ def algorithmic_bytes_per_step(grid, np_per_slice, iters, nbeam):
     return b
 
 
-def batch_c4(reps=2, eager="eager"):
+def batch_c4(reps=2, eager="eager", train_steps=2, lr=1e-5):
     """The C4 scan (16 independent instances of random Nr, modes, beam charge
     and offset, Nxi 512, from a recorded seed): one untimed 3D step each,
     then `reps` timed 3D steps of all 16 stepped one after another, as one
     eager batch (event-driven, each instance with adaptive iteration
     queuing), as one graph batch (after the graphs are captured), and as one
     graph batch after every instance's deposit launch shape is autotuned.
-    Instance 3D steps per second; every form gives the same results (the
-    batch tests)."""
-    from qpad_amd import Batch, Simulation, configs
+    A second set of the same 16 instances carries the learned seed: after
+    the same untimed step, each instance's network trains on its own
+    plain-solver slices for `train_steps` eager batch steps, then seeds
+    (frozen) in `reps` timed eager batch steps, which are the same 3D steps
+    as the plain eager leg's.  Instance 3D steps per second, iterations per
+    slice and the share of slices that ended at iter_max per leg; every
+    plain form gives the same results (the batch tests)."""
+    from qpad_amd import Batch, SeedNet, Simulation, configs
     decks, params = configs.c4()
     sims = [Simulation(d) for d in decks]
     for s_ in sims:
@@ -139,10 +144,15 @@ def batch_c4(reps=2, eager="eager"):
         out = [fn() for _ in range(reps)]
         return (time.perf_counter() - t0) / reps, out
 
+    def at_max_share(group):
+        st = [s_.step_iterations() for s_ in group]
+        return sum(m for _, m in st) / sum(s_.grid.nxi for s_ in group)
+
     t_seq, its = timed(lambda: [s_.step3d() for s_ in sims])
     batch = Batch(sims)
     batch.mode(eager)
-    t_eager, _ = timed(batch.step3d)
+    t_eager, its_eager = timed(batch.step3d)
+    share_eager = at_max_share(sims)
     batch.mode(True)
     batch.step3d()
     batch.step3d()          # both particle-buffer assignments captured
@@ -155,8 +165,33 @@ def batch_c4(reps=2, eager="eager"):
     nsl = sum(s_.grid.nxi for s_ in sims)
     for s_ in sims:
         s_.close()
+    # the learned leg: a second set of the instances, each with its network
+    lsims = [Simulation(d) for d in decks]
+    for s_ in lsims:
+        s_.step3d()
+    nets = [SeedNet(s_, seed=3, lr=lr) for s_ in lsims]
+    lbatch = Batch(lsims)
+    lbatch.mode(eager)
+    for net in nets:
+        net.mode(seed=False, train=True)
+    for _ in range(train_steps):
+        lbatch.step3d()
+    for net in nets:
+        net.mode(seed=True, train=False)
+    t_learned, its_learned = timed(lbatch.step3d)
+    share_learned = at_max_share(lsims)
+    lbatch.close()
+    for s_ in lsims:
+        s_.close()
     n = len(sims)
     return {"instances": n, "params": params, "iters_per_slice": sum(its[-1]) / nsl, "eager_mode": eager,
+            "learned_seed_eager": {"instance_steps_per_s": n / t_learned,
+                                   "speedup_vs_plain_eager": t_eager / t_learned,
+                                   "iters_per_slice": float(sum(its_learned[-1])) / nsl,
+                                   "plain_eager_iters_per_slice": float(sum(its_eager[-1])) / nsl,
+                                   "share_of_slices_at_iter_max": share_learned,
+                                   "plain_eager_share_of_slices_at_iter_max": share_eager,
+                                   "train_steps": train_steps, "lr": lr},
             "sequential_instance_steps_per_s": n / t_seq, "batch_eager_instance_steps_per_s": n / t_eager,
             "batch_graph_instance_steps_per_s": n / t_graph, "batch_graph_tuned_instance_steps_per_s": n / t_tuned,
             "tuned_dep_cells": shapes, "batch_graph_tuned_speedup_vs_sequential": t_seq / t_tuned,
@@ -342,7 +377,7 @@ def main():
                                    "part of it (use --gpus 1)", "n_gpus": args.gpus}))
         sys.exit(2)
     if args.only_c4:
-        print(json.dumps({"batch_c4": batch_c4(eager=args.c4_eager)}))
+        print(json.dumps({"batch_c4": batch_c4(eager=args.c4_eager, lr=args.lr)}))
         return
     # torch's HIP runtime starts before the library's: measured first
     copy_gbs = hbm_copy_bandwidth()
@@ -494,7 +529,7 @@ def main():
     }
     legs["learned"].close()
     if not args.no_c4:
-        result["batch_c4"] = batch_c4(eager=args.c4_eager)
+        result["batch_c4"] = batch_c4(eager=args.c4_eager, lr=args.lr)
     if not args.no_cpu:
         # the oracle is the plain method: replay window of the plain leg (re-run
         # its step up to j0 - 1 and snapshot there), scaled by its iterations

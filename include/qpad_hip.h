@@ -256,6 +256,11 @@ int qpad_step3d_begin(qpad_ctx *ctx);
 int qpad_slice(qpad_ctx *ctx, int32_t j, int32_t *iters_out);
 int qpad_step3d_end(qpad_ctx *ctx);
 int qpad_step3d(qpad_ctx *ctx, int32_t *total_iters_out);
+/* iterations of the last 3D step of qpad_step3d or qpad_batch_step3d:
+   summed over its slices, and the number of its slices that ended at
+   iter_max without meeting the tolerance (no counterpart in QPAD, which
+   logs neither) */
+int qpad_step_iterations(qpad_ctx *ctx, int32_t *total_iters, int32_t *slices_at_iter_max);
 /* How qpad_slice / qpad_step3d queue the predictor-corrector iterations (no
    counterpart in QPAD, whose loop is serial on the host; results do not
    depend on it).  adaptive = 0: all iter_max iterations are queued at once

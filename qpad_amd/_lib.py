@@ -106,6 +106,7 @@ SIGNATURES = {
     "qpad_species_amjdp": (C.c_int, [_CTX, C.c_int32, C.c_double]),
     "qpad_species_push": (C.c_int, [_CTX, C.c_int32, C.c_double]),
     "qpad_species_interp_psi": (C.c_int, [_CTX, C.c_int32]),
+    "qpad_step_iterations": (C.c_int, [_CTX, C.POINTER(C.c_int32), C.POINTER(C.c_int32)]),
     "qpad_field_solve_psi": (C.c_int, [_CTX]),
     "qpad_field_solve_ez": (C.c_int, [_CTX, C.c_int32, C.c_int32]),
     "qpad_field_solve_et": (C.c_int, [_CTX, C.c_int32, C.c_int32]),

@@ -595,6 +595,7 @@ struct EpiArgs {
     IterState *it;
     const double *conv_blk;   // both parities
     int nblk;
+    int iter_max;
 };
 
 // thread per (node i = 0..nr+1, mode array k)
@@ -613,6 +614,7 @@ __global__ __launch_bounds__(kNodeThreads) void k_epilogue(EpiArgs a)
             a.it->rel = rel;
             a.it->abs_ = ab;
             a.it->total_iters += iters;
+            if (iters >= a.iter_max) a.it->at_max += 1;
         }
     }
     if (i > nr + 1) return;
@@ -681,6 +683,7 @@ int launch_epilogue(qpad_ctx *c, int j)
     a.it = c->it;
     a.conv_blk = c->conv_blk;
     a.nblk = conv_blocks(c->g.nr);
+    a.iter_max = c->g.iter_max;
     hipLaunchKernelGGL(k_epilogue, dim3((unsigned)node_blocks(c->g.nr), c->g.nm), dim3(kNodeThreads), 0,
                        c->stream, a);
     QPAD_HIP_CHECK(hipGetLastError());

@@ -863,11 +863,20 @@ int qpad_step3d_end(qpad_ctx *c)
     return sync(c);
 }
 
+int qpad_step_iterations(qpad_ctx *c, int32_t *total_iters, int32_t *slices_at_iter_max)
+{
+    if (!c || !total_iters || !slices_at_iter_max) return fail(QPAD_E_INVALID, "null argument");
+    *total_iters = c->it_host->total_iters;
+    *slices_at_iter_max = c->it_host->at_max;
+    return 0;
+}
+
 int qpad_step3d(qpad_ctx *c, int32_t *total_iters_out)
 {
     TRY(qpad_step3d_begin(c));
     TRY(read_iter_state(c));
     c->it_host->total_iters = 0;
+    c->it_host->at_max = 0;
     QPAD_HIP_CHECK(hipMemcpyAsync(c->it, c->it_host, sizeof(IterState), hipMemcpyHostToDevice, c->stream));
     for (int j = 1; j <= c->g.nxi; j++) TRY(slice_enqueue(c, j, true));
     TRY(read_iter_state(c));
@@ -1445,8 +1454,13 @@ int qpad_batch_step3d(qpad_batch *b, int32_t *total_iters_out)
 {
     if (!b) return fail(QPAD_E_INVALID, "null batch");
     const int n = (int)b->ctx.size();
-    for (qpad_ctx *c : b->ctx)
-        if (seednet_active(c)) return fail(QPAD_E_STATE, "batched contexts step without the seed network");
+    // the seed network's stream joins a context's slice loop through events
+    // the eager modes queue like qpad_step3d does; a captured slice graph
+    // holds no such second stream
+    if (b->graph == 1)
+        for (qpad_ctx *c : b->ctx)
+            if (seednet_active(c))
+                return fail(QPAD_E_STATE, "graph batches step without the seed network (use an eager mode)");
     for (qpad_ctx *c : b->ctx) {
         TRY(step_begin_enqueue(c));
         memset(c->it_host, 0, sizeof(IterState));
@@ -1482,7 +1496,7 @@ int qpad_batch_step3d(qpad_batch *b, int32_t *total_iters_out)
                 if (i >= n) break;
                 qpad_ctx *c = b->ctx[i];
                 int rc = hipSetDevice(c->device) == hipSuccess ? 0 : fail(QPAD_E_HIP, "hipSetDevice failed");
-                for (int j = 1; j <= c->g.nxi && !rc; j++) rc = slice_enqueue(c, j, false);
+                for (int j = 1; j <= c->g.nxi && !rc; j++) rc = slice_enqueue(c, j, true);
                 if (rc) {
                     rcs[i] = rc;
                     msgs[i] = qpad_last_error();
@@ -1507,7 +1521,7 @@ int qpad_batch_step3d(qpad_batch *b, int32_t *total_iters_out)
             qpad_ctx *c = b->ctx[i];
             if (c->g.nxi < 1) continue;
             active++;
-            TRY(slice_start(c, next[i]++, false, st[i]));
+            TRY(slice_start(c, next[i]++, true, st[i]));
         }
         while (active > 0) {
             for (int i = 0; i < n; i++) {
@@ -1521,7 +1535,7 @@ int qpad_batch_step3d(qpad_batch *b, int32_t *total_iters_out)
                 }
                 // a slice that finished starts the next one at once
                 while (!st[i].waiting && st[i].j == 0 && next[i] <= c->g.nxi)
-                    TRY(slice_start(c, next[i]++, false, st[i]));
+                    TRY(slice_start(c, next[i]++, true, st[i]));
                 if (st[i].j == 0 && next[i] > c->g.nxi) active--;
             }
         }

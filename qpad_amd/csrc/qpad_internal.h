@@ -94,7 +94,8 @@ struct IterState {        // device-resident iteration bookkeeping
     int counter;          // last-block-done counter of the B-perp solve
     int iters;            // iterations executed in the current slice
     int total_iters;      // iterations summed over the slices of a 3D step
-    int pad[2];
+    int at_max;           // slices of the 3D step that ended at iter_max
+    int pad;
 };
 
 }  // namespace qpad

@@ -201,6 +201,15 @@ module qpad_hip_mod
       integer(c_int) :: rc
     end function
 
+    ! iterations of the last 3D step: total, and slices that ended at iter_max
+    function qpad_step_iterations(ctx, total_iters, slices_at_iter_max) &
+        bind(C, name="qpad_step_iterations") result(rc)
+      import :: c_int, c_int32_t, c_ptr
+      type(c_ptr), value :: ctx
+      integer(c_int32_t), intent(out) :: total_iters, slices_at_iter_max
+      integer(c_int) :: rc
+    end function
+
     ! species2d%interp_psi (standard pusher; no effect on a robust species)
     function qpad_species_interp_psi(ctx, sp) bind(C, name="qpad_species_interp_psi") result(rc)
       import :: c_int, c_int32_t, c_ptr
@@ -600,7 +609,7 @@ module qpad_hip_mod
   public :: qpad_last_error, qpad_abi_version, qpad_create, qpad_destroy, qpad_get_grid, &
     qpad_species_new, qpad_species_renew, qpad_species_qdp, qpad_species_qdp_noalloc, &
     qpad_species_charge_history, qpad_species_cbq, qpad_get_species_charge, qpad_species_amjdp, &
-    qpad_species_push, qpad_species_interp_psi, qpad_field_solve_psi, qpad_field_solve_ez, &
+    qpad_species_push, qpad_species_interp_psi, qpad_step_iterations, qpad_field_solve_psi, qpad_field_solve_ez, &
     qpad_field_solve_et, qpad_field_solve_bz, qpad_field_solve_bt_beam, &
     qpad_field_solve_djdxi, qpad_field_solve_bt_iter, qpad_field_add_b, &
     qpad_convergence_tester, qpad_beam_qdp, qpad_beam_push, qpad_beam_new, qpad_beam_new_std, qpad_beam_new_arrays, &

@@ -369,6 +369,13 @@ class Simulation:
                                                  C.byref(rel), C.byref(ab)))
         return rel.value, ab.value
 
+    def step_iterations(self):
+        """(iterations summed over the last 3D step's slices, slices of that
+        step that ended at iter_max)"""
+        t, m = C.c_int32(0), C.c_int32(0)
+        L.check(self.lib.qpad_step_iterations(self.ctx, C.byref(t), C.byref(m)))
+        return t.value, m.value
+
     def kernel_time(self, which):
         ms, n = C.c_double(0.0), C.c_int64(0)
         L.check(self.lib.qpad_kernel_time(self.ctx, int(which), C.byref(ms), C.byref(n)))

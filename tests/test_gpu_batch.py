@@ -119,3 +119,41 @@ def test_graph_mode_needs_even_nxi():
         batch.step3d()
     batch.close()
     sim.close()
+
+
+def test_seeded_eager_batch_matches_instances_alone():
+    """Instances carrying the learned seed (trained for one step from
+    plain-solver labels, then seeding frozen) step in an eager batch exactly
+    as each steps alone with qpad_step3d; the graph mode refuses them.  The
+    slices that ended at iter_max are counted per step."""
+    from qpad_amd import SeedNet
+
+    def seeded(d):
+        sim = Simulation(d)
+        net = SeedNet(sim, seed=5, lr=1e-5)
+        net.mode(seed=False, train=True)
+        sim.step3d()
+        net.mode(seed=True, train=False)
+        return sim
+
+    ref = []
+    for d in scan_decks():
+        sim = seeded(d)
+        ref.append([(sim.step3d(), sim.step_iterations(), state(sim)) for _ in range(2)])
+        sim.close()
+    for mode in ("eager", "eager_single"):
+        sims = [seeded(d) for d in scan_decks()]
+        batch = Batch(sims, graph=mode)
+        for step in range(2):
+            its = batch.step3d()
+            for i, sim in enumerate(sims):
+                it_ref, stats_ref, st_ref = ref[i][step]
+                assert its[i] == it_ref and sim.step_iterations() == stats_ref, (mode, i, step)
+                assert stats_ref[0] == it_ref and 0 <= stats_ref[1] <= NXI
+                assert_same(state(sim), st_ref, f"seeded {mode} batch instance {i} step {step}")
+        batch.mode("graph")
+        with pytest.raises(Exception, match="seed network"):
+            batch.step3d()
+        batch.close()
+        for sim in sims:
+            sim.close()
