@@ -46,7 +46,8 @@ class SpeciesCfg(C.Structure):
                 ("seed", C.c_uint64), ("push_type", C.c_int)]
 
 
-PUSH2_STD, PUSH2_ROBUST = 0, 1
+PUSH2_STD, PUSH2_ROBUST, PUSH2_STD_PGC, PUSH2_ROBUST_PGC = 0, 1, 4, 5
+LASER_AR, LASER_AI, LASER_AR_GRAD, LASER_AI_GRAD, LASER_CHI, LASER_H_CHI, LASER_HIST_AR, LASER_HIST_AI = range(8)
 
 
 class BeamCfg(C.Structure):
@@ -147,6 +148,14 @@ def load_library(path):
     L.ora_interp_psi.argtypes = [C.c_void_p, C.POINTER(_Species), _dp]
     L.ora_amjdeposit_std.argtypes = [C.c_void_p, C.POINTER(_Species), _dp, _dp, _dp, _dp, _dp, C.c_double]
     L.ora_push_u_std.argtypes = [C.c_void_p, C.POINTER(_Species), _dp, _dp, C.c_double]
+    L.ora_set_laser.argtypes = [C.c_void_p, _dp, _dp]
+    L.ora_laser_slice.argtypes = [C.c_void_p, C.c_int]
+    L.ora_amjdeposit_pgc.argtypes = [C.c_void_p, C.POINTER(_Species), _dp, _dp, _dp, _dp, _dp, C.c_double,
+                                     C.c_int]
+    L.ora_push_u_pgc.argtypes = [C.c_void_p, C.POINTER(_Species), _dp, _dp, C.c_double]
+    L.ora_deposit_chi.argtypes = [C.c_void_p, C.POINTER(_Species), _dp]
+    L.ora_laser_array.restype = _dp
+    L.ora_laser_array.argtypes = [C.c_void_p, C.c_int]
     L.ora_update_bound.argtypes = [C.c_void_p, C.POINTER(_Species)]
     L.ora_interp_point.argtypes = [C.c_void_p, _dp, C.c_int, C.c_double, C.c_double, _dp]
     L.ora_beam_qdeposit.argtypes = [C.c_void_p, C.c_int, _dp]
@@ -324,6 +333,31 @@ class Oracle:
         ns = self.grid.nxi + 2
         n = ns * NCOMP[fid] * self.nm * self.ld
         return np.ctypeslib.as_array(p, shape=(n,)).reshape(ns, NCOMP[fid], self.nm, self.ld)
+
+    # -- laser (PGC)
+    def set_laser(self, ar_hist, ai_hist):
+        """The laser envelope history (real and imaginary part of the complex
+        envelope), [nxi + 3, nm, ld] each: slices -1 .. nxi + 1."""
+        shape = (self.grid.nxi + 3, self.nm, self.ld)
+        a = np.ascontiguousarray(ar_hist, dtype=np.float64)
+        b = np.ascontiguousarray(ai_hist, dtype=np.float64)
+        assert a.shape == shape and b.shape == shape, (a.shape, shape)
+        assert self.L.ora_set_laser(self.h, ptr(a), ptr(b)) == 0
+
+    def laser(self, which):
+        """View of a laser array (LASER_*); None before set_laser or a PGC
+        species"""
+        p = self.L.ora_laser_array(self.h, which)
+        if not p:
+            return None
+        nm, ld, nxi = self.nm, self.ld, self.grid.nxi
+        shape = {LASER_AR: (1, nm, ld), LASER_AI: (1, nm, ld), LASER_AR_GRAD: (3, nm, ld),
+                 LASER_AI_GRAD: (3, nm, ld), LASER_CHI: (1, nm, ld), LASER_H_CHI: (nxi + 2, 1, nm, ld),
+                 LASER_HIST_AR: (nxi + 3, nm, ld), LASER_HIST_AI: (nxi + 3, nm, ld)}[which]
+        return np.ctypeslib.as_array(p, shape=(int(np.prod(shape)),)).reshape(shape)
+
+    def laser_slice(self, j):
+        self.L.ora_laser_slice(self.h, j)
 
     def species_raw(self, isp):
         return self.L.ora_get_species(self.h, isp)

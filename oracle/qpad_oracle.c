@@ -991,6 +991,308 @@ void ora_push_u_std(const ora_sim *s, ora_species *sp, const double *ef,
     }
 }
 
+/* ------------------------------------------------------------------------ */
+/* Laser (PGC) path                                                          */
+/* ------------------------------------------------------------------------ */
+
+/* the envelope history of the laser (field_laser's cfr, cfi f2; with
+   several lasers, their sum), slices -1 .. nxi+1: field_laser keeps two
+   guard slices below the box (sim_lasers_class.f03:105-110) */
+static int laser_alloc(ora_sim *s)
+{
+    const size_t n1 = (size_t)s->nm * s->ld, nh = (size_t)(s->g.nxi + 3) * n1;
+    if (!s->laser_on) {
+        s->lz_ar = calloc(nh, sizeof(double));
+        s->lz_ai = calloc(nh, sizeof(double));
+        s->ar = calloc(n1, sizeof(double));
+        s->ai = calloc(n1, sizeof(double));
+        s->ar_grad = calloc(3 * n1, sizeof(double));
+        s->ai_grad = calloc(3 * n1, sizeof(double));
+        s->chi = calloc(n1, sizeof(double));
+        s->h_chi = calloc((size_t)(s->g.nxi + 2) * n1, sizeof(double));
+        if (!s->lz_ar || !s->lz_ai || !s->ar || !s->ai || !s->ar_grad || !s->ai_grad || !s->chi || !s->h_chi)
+            return -1;
+        s->laser_on = 1;
+    }
+    return 0;
+}
+
+int ora_set_laser(ora_sim *s, const double *ar_hist, const double *ai_hist)
+{
+    const size_t nh = (size_t)(s->g.nxi + 3) * s->nm * s->ld;
+    if (laser_alloc(s)) return -1;
+    memcpy(s->lz_ar, ar_hist, nh * sizeof(double));
+    memcpy(s->lz_ai, ai_hist, nh * sizeof(double));
+    return 0;
+}
+
+/* laser_all of slice j with one laser: zero_field_laser(only_f1)
+   (field_laser_class.f03:952-1012), copy_slice of the envelope, the
+   gradients of set_grad_field_laser (:637-750) and gather (:929-946).
+   Gradient components are (r, phi, z): central differences in r, the
+   (1/r) d/dphi of the mode (-m Im, m Re over r) and a three-point backward
+   difference in xi over slices j, j-1, j-2.  For m > 0 the axis values of
+   the first rank are written after the radial loop with its index, that is
+   at node nr + 1 (:710-724), and node 1 keeps zero: restated as is. */
+void ora_laser_slice(ora_sim *s, int j)
+{
+    const int nm = s->nm, ld = s->ld, nr = s->g.nr, M = s->g.max_mode;
+    const size_t n1 = (size_t)nm * ld;
+    const double idrh = 0.5 / s->g.dr, idzh = 0.5 / s->g.dxi;
+#define LZ(h, sl, k, i) (h)[((size_t)((sl) + 1) * nm + (k)) * ld + (i)]
+    memset(s->ar_grad, 0, 3 * n1 * sizeof(double));
+    memset(s->ai_grad, 0, 3 * n1 * sizeof(double));
+    for (int k = 0; k < nm; k++)
+        for (int i = 0; i <= nr + 1; i++) {
+            FA(s->ar, 0, k, i) = LZ(s->lz_ar, j, k, i);
+            FA(s->ai, 0, k, i) = LZ(s->lz_ai, j, k, i);
+        }
+    const double *hs[2] = {s->lz_ar, s->lz_ai};
+    double *gs[2] = {s->ar_grad, s->ai_grad};
+    for (int t = 0; t < 2; t++) {
+        const double *h = hs[t];
+        double *gr = gs[t];
+        for (int i = 1; i <= nr; i++)
+            FA(gr, 2, 0, i) = idzh * (3.0 * LZ(h, j, 0, i) - 4.0 * LZ(h, j - 1, 0, i) + LZ(h, j - 2, 0, i));
+        for (int i = 2; i <= nr; i++) {
+            FA(gr, 0, 0, i) = idrh * (LZ(h, j, 0, i + 1) - LZ(h, j, 0, i - 1));
+            FA(gr, 1, 0, i) = 0.0;
+        }
+        FA(gr, 0, 0, 1) = 0.0;
+        FA(gr, 1, 0, 1) = 0.0;
+        for (int m = 1; m <= M; m++) {
+            const int kr = kre(m), ki = kim(m);
+            for (int i = 1; i <= nr; i++) {
+                FA(gr, 2, kr, i) = idzh * (3.0 * LZ(h, j, kr, i) - 4.0 * LZ(h, j - 1, kr, i) + LZ(h, j - 2, kr, i));
+                FA(gr, 2, ki, i) = idzh * (3.0 * LZ(h, j, ki, i) - 4.0 * LZ(h, j - 1, ki, i) + LZ(h, j - 2, ki, i));
+            }
+            for (int i = 2; i <= nr; i++) {
+                const double ir = 1.0 / ((double)(i - 1) * s->g.dr);
+                FA(gr, 0, kr, i) = idrh * (LZ(h, j, kr, i + 1) - LZ(h, j, kr, i - 1));
+                FA(gr, 0, ki, i) = idrh * (LZ(h, j, ki, i + 1) - LZ(h, j, ki, i - 1));
+                FA(gr, 1, kr, i) = -ir * m * LZ(h, j, ki, i);
+                FA(gr, 1, ki, i) = ir * m * LZ(h, j, kr, i);
+            }
+            const int ie = nr + 1;   /* the loop index after "do i = 2, nrp" */
+            if (m % 2 == 1) {
+                FA(gr, 0, kr, ie) = 2.0 * idrh * LZ(h, j, kr, 2);
+                FA(gr, 0, ki, ie) = 2.0 * idrh * LZ(h, j, ki, 2);
+                FA(gr, 1, kr, ie) = -m * FA(gr, 0, ki, ie);
+                FA(gr, 1, ki, ie) = m * FA(gr, 0, kr, ie);
+            } else {
+                FA(gr, 0, kr, ie) = 0.0;
+                FA(gr, 0, ki, ie) = 0.0;
+                FA(gr, 1, kr, ie) = 0.0;
+                FA(gr, 1, ki, ie) = 0.0;
+            }
+        }
+    }
+#undef LZ
+}
+
+/* interp_field of the laser envelope and its gradients at a particle
+   (part2d_class.f03:1406-1409, interp_part2d.f03:66-110) */
+static void laser_gather(const ora_sim *s, int ix, double w0, double w1, double pc, double ps,
+                         double *apr, double *api, double *aprg, double *apig)
+{
+    interp_vec(s, s->ar, 1, ix, w0, w1, pc, ps, apr);
+    interp_vec(s, s->ai, 1, ix, w0, w1, pc, ps, api);
+    interp_vec(s, s->ar_grad, 3, ix, w0, w1, pc, ps, aprg);
+    interp_vec(s, s->ai_grad, 3, ix, w0, w1, pc, ps, apig);
+}
+
+/* amjdeposit_robust_pgc_part2d (part2d_class.f03:1310-1602) and, std_,
+   amjdeposit_std_pgc_part2d (:1012-1308): the ponderomotive correction
+   gam_corr = (q/m)^2 |a|^2 / 2 in every gamma, and the effective field
+   E -+ (q/m)/(2 gamma) grad |a|^2 / 2 (r, phi components minus, z plus)
+   with the initial gamma; the robust form normalises each half kick by its
+   own gamma - u_z, the standard one by the interpolated psi */
+void ora_amjdeposit_pgc(const ora_sim *s, ora_species *sp, const double *ef, const double *bf,
+                        double *cu, double *amu, double *dcu, double dt, int std_)
+{
+    int nm = s->nm, ld = s->ld, M = s->g.max_mode;
+    double idr = 1.0 / s->g.dr, idt = 1.0 / dt, qtmh = 0.5 * sp->qbm * dt;
+    for (int64_t p = 0; p < sp->npp; p++) {
+        int ix; double w0, w1, pc, ps;
+        interp_info(sp->x1[p], sp->x2[p], idr, &ix, &w0, &w1, &pc, &ps);
+        double ep[3], bp[3], wp[3], u0[3], u[3], ut[3], du[3], u2[3];
+        double apr, api, aprg[3], apig[3];
+        interp_vec(s, ef, 3, ix, w0, w1, pc, ps, ep);
+        interp_vec(s, bf, 3, ix, w0, w1, pc, ps, bp);
+        laser_gather(s, ix, w0, w1, pc, ps, &apr, &api, aprg, apig);
+        double gam_corr = 0.5 * sp->qbm * sp->qbm * (apr * apr + api * api);
+        u0[0] = sp->p1[p] * pc + sp->p2[p] * ps;
+        u0[1] = sp->p2[p] * pc - sp->p1[p] * ps;
+        u0[2] = sp->p3[p];
+        double gam = sqrt(1.0 + u0[0] * u0[0] + u0[1] * u0[1] + u0[2] * u0[2] + gam_corr);
+        wp[0] = ep[0] - bp[1];
+        wp[1] = ep[1] + bp[0];
+        wp[2] = ep[2];
+        double tmp = 0.5 * sp->qbm / gam;
+        ep[0] = ep[0] - tmp * (apr * aprg[0] + api * apig[0]);
+        ep[1] = ep[1] - tmp * (apr * aprg[1] + api * apig[1]);
+        ep[2] = ep[2] + tmp * (apr * aprg[2] + api * apig[2]);
+        double qtmh_e, qtmh_b = 0.0;
+        if (std_) {
+            qtmh_b = qtmh / (1.0 - sp->qbm * sp->psi[p]);
+            qtmh_e = qtmh_b * gam;
+        } else {
+            qtmh_e = qtmh * gam / (gam - u0[2]);
+        }
+        for (int c = 0; c < 3; c++) ut[c] = u0[c] + ep[c] * qtmh_e;
+        if (!std_) {
+            gam = sqrt(1.0 + ut[0] * ut[0] + ut[1] * ut[1] + ut[2] * ut[2] + gam_corr);
+            qtmh_b = qtmh / (gam - ut[2]);
+        }
+        for (int c = 0; c < 3; c++) bp[c] = bp[c] * qtmh_b;
+        u[0] = ut[0] + ut[1] * bp[2] - ut[2] * bp[1];
+        u[1] = ut[1] + ut[2] * bp[0] - ut[0] * bp[2];
+        u[2] = ut[2] + ut[0] * bp[1] - ut[1] * bp[0];
+        double ostq = 2.0 / (1.0 + bp[0] * bp[0] + bp[1] * bp[1] + bp[2] * bp[2]);
+        for (int c = 0; c < 3; c++) bp[c] = bp[c] * ostq;
+        ut[0] = ut[0] + u[1] * bp[2] - u[2] * bp[1];
+        ut[1] = ut[1] + u[2] * bp[0] - u[0] * bp[2];
+        ut[2] = ut[2] + u[0] * bp[1] - u[1] * bp[0];
+        gam = sqrt(1.0 + ut[0] * ut[0] + ut[1] * ut[1] + ut[2] * ut[2] + gam_corr);
+        qtmh_e = std_ ? qtmh_b * gam : qtmh * gam / (gam - ut[2]);
+        for (int c = 0; c < 3; c++) u[c] = ut[c] + ep[c] * qtmh_e;
+        du[0] = idt * (u[0] - u0[0]);
+        du[1] = idt * (u[1] - u0[1]);
+        for (int c = 0; c < 3; c++) u[c] = 0.5 * (u[c] + u0[c]);
+        sp->gamma[p] = sqrt(1.0 + u[0] * u[0] + u[1] * u[1] + u[2] * u[2] + gam_corr);
+        double ipsi;
+        if (std_) {
+            ipsi = 1.0 / (1.0 - sp->qbm * sp->psi[p]);
+        } else {
+            ipsi = 1.0 / (sp->gamma[p] - u[2]);
+            sp->psi[p] = (1.0 - 1.0 / ipsi) / sp->qbm;
+        }
+        double dpsi = sp->qbm * (wp[2] - (wp[0] * u[0] + wp[1] * u[1]) * ipsi);
+        du[0] = du[0] + u[0] * dpsi * ipsi;
+        du[1] = du[1] + u[1] * dpsi * ipsi;
+        u2[0] = u[0] * u[0] * ipsi;
+        u2[1] = u[0] * u[1] * ipsi;
+        u2[2] = u[1] * u[1] * ipsi;
+        double ph0r = pc, ph0i = -ps;
+        double phr = 1.0 * sp->q[p] * ipsi, phi = 0.0 * sp->q[p] * ipsi;
+        double wt[2] = {w0, w1};
+        for (int jj = 0; jj < 2; jj++) {
+            double w = wt[jj] * phr;
+            for (int c = 0; c < 3; c++) FA(cu, c, 0, ix + jj) = FA(cu, c, 0, ix + jj) + w * u[c];
+            for (int c = 0; c < 2; c++) FA(dcu, c, 0, ix + jj) = FA(dcu, c, 0, ix + jj) + w * du[c];
+            for (int c = 0; c < 3; c++) FA(amu, c, 0, ix + jj) = FA(amu, c, 0, ix + jj) + w * u2[c];
+        }
+        for (int m = 1; m <= M; m++) {
+            double a = phr * ph0r - phi * ph0i, b = phr * ph0i + phi * ph0r;
+            phr = a; phi = b;
+            int kr = kre(m), ki = kim(m);
+            for (int jj = 0; jj < 2; jj++) {
+                double w = wt[jj] * phr;
+                for (int c = 0; c < 3; c++) FA(cu, c, kr, ix + jj) = FA(cu, c, kr, ix + jj) + w * u[c];
+                for (int c = 0; c < 2; c++) FA(dcu, c, kr, ix + jj) = FA(dcu, c, kr, ix + jj) + w * du[c];
+                for (int c = 0; c < 3; c++) FA(amu, c, kr, ix + jj) = FA(amu, c, kr, ix + jj) + w * u2[c];
+                w = wt[jj] * phi;
+                for (int c = 0; c < 3; c++) FA(cu, c, ki, ix + jj) = FA(cu, c, ki, ix + jj) + w * u[c];
+                for (int c = 0; c < 2; c++) FA(dcu, c, ki, ix + jj) = FA(dcu, c, ki, ix + jj) + w * du[c];
+                for (int c = 0; c < 3; c++) FA(amu, c, ki, ix + jj) = FA(amu, c, ki, ix + jj) + w * u2[c];
+            }
+        }
+    }
+    deposit_axis_and_radius(s, cu, amu, dcu);
+}
+
+/* push_u_robust_pgc_part2d (part2d_class.f03:1967-2092) and
+   push_u_std_pgc_part2d (:2094-2219), the same steps: both normalise by
+   the time-centred gamma and psi the last current deposit stored; the new
+   gamma carries gam_corr extrapolated over dxi with the xi gradient of the
+   envelope */
+void ora_push_u_pgc(const ora_sim *s, ora_species *sp, const double *ef, const double *bf, double dt)
+{
+    double idr = 1.0 / s->g.dr;
+    double qtmh = sp->qbm * dt * 0.5, qbm2_hf = sp->qbm * sp->qbm * 0.5;
+    for (int64_t p = 0; p < sp->npp; p++) {
+        int ix; double w0, w1, pc, ps;
+        interp_info(sp->x1[p], sp->x2[p], idr, &ix, &w0, &w1, &pc, &ps);
+        double ep[3], bp[3], ut[3], pp[3], apr, api, aprg[3], apig[3];
+        interp_vec(s, ef, 3, ix, w0, w1, pc, ps, ep);
+        interp_vec(s, bf, 3, ix, w0, w1, pc, ps, bp);
+        laser_gather(s, ix, w0, w1, pc, ps, &apr, &api, aprg, apig);
+        double t = ep[0] * pc - ep[1] * ps; ep[1] = ep[0] * ps + ep[1] * pc; ep[0] = t;
+        t = bp[0] * pc - bp[1] * ps; bp[1] = bp[0] * ps + bp[1] * pc; bp[0] = t;
+        t = aprg[0] * pc - aprg[1] * ps; aprg[1] = aprg[0] * ps + aprg[1] * pc; aprg[0] = t;
+        t = apig[0] * pc - apig[1] * ps; apig[1] = apig[0] * ps + apig[1] * pc; apig[0] = t;
+        double gam_corr = qbm2_hf * (apr * apr + api * api);
+        double tmp = 0.5 * sp->qbm / sp->gamma[p];
+        ep[0] = ep[0] - tmp * (apr * aprg[0] + api * apig[0]);
+        ep[1] = ep[1] - tmp * (apr * aprg[1] + api * apig[1]);
+        ep[2] = ep[2] + tmp * (apr * aprg[2] + api * apig[2]);
+        double qtmh_b = qtmh / (1.0 - sp->qbm * sp->psi[p]);
+        double qtmh_e = qtmh_b * sp->gamma[p];
+        for (int c = 0; c < 3; c++) { ep[c] = ep[c] * qtmh_e; bp[c] = bp[c] * qtmh_b; }
+        pp[0] = sp->p1[p]; pp[1] = sp->p2[p]; pp[2] = sp->p3[p];
+        for (int c = 0; c < 3; c++) ut[c] = pp[c] + ep[c];
+        pp[0] = ut[0] + ut[1] * bp[2] - ut[2] * bp[1];
+        pp[1] = ut[1] + ut[2] * bp[0] - ut[0] * bp[2];
+        pp[2] = ut[2] + ut[0] * bp[1] - ut[1] * bp[0];
+        double ostq = 2.0 / (1.0 + bp[0] * bp[0] + bp[1] * bp[1] + bp[2] * bp[2]);
+        for (int c = 0; c < 3; c++) bp[c] = bp[c] * ostq;
+        ut[0] = ut[0] + pp[1] * bp[2] - pp[2] * bp[1];
+        ut[1] = ut[1] + pp[2] * bp[0] - pp[0] * bp[2];
+        ut[2] = ut[2] + pp[0] * bp[1] - pp[1] * bp[0];
+        for (int c = 0; c < 3; c++) pp[c] = ut[c] + ep[c];
+        sp->p1[p] = pp[0]; sp->p2[p] = pp[1]; sp->p3[p] = pp[2];
+        tmp = aprg[2] * dt; gam_corr = gam_corr + qbm2_hf * (apr + 0.25 * tmp) * tmp;
+        tmp = apig[2] * dt; gam_corr = gam_corr + qbm2_hf * (api + 0.25 * tmp) * tmp;
+        sp->gamma[p] = sqrt(1.0 + pp[0] * pp[0] + pp[1] * pp[1] + pp[2] * pp[2] + gam_corr);
+    }
+}
+
+/* deposit_chi_part2d (part2d_class.f03:361-476): the susceptibility
+   -(q/m) q / (1 - (q/m) psi) of each particle with linear weights and
+   e^{-im phi}; then, on the whole array, node 0 zero, the m = 0 axis node
+   times get_deposit_ax_corr(ppc_r) = 12 ppc^2 / (1 + 2 ppc^2) (:2581-2586),
+   m > 0 axis zero, node j divided by j - 1.  The rescaling acts on
+   whatever chi holds, so a second species rescales the first one's
+   deposit again (sim_lasers_class.f03:176-200 calls it per species into
+   one array): restated as is. */
+void ora_deposit_chi(const ora_sim *s, const ora_species *sp, double *chi)
+{
+    int nr = s->g.nr, nm = s->nm, ld = s->ld, M = s->g.max_mode;
+    double idr = 1.0 / s->g.dr;
+    for (int64_t p = 0; p < sp->npp; p++) {
+        double pos = sqrt(sp->x1[p] * sp->x1[p] + sp->x2[p] * sp->x2[p]) * idr;
+        double ph0r = sp->x1[p] / pos * idr, ph0i = -sp->x2[p] / pos * idr;
+        int nn = (int)floor(pos);
+        pos = pos - (double)nn;
+        nn = nn + 1;
+        double wt[2] = {1.0 - pos, pos};
+        double phr = -1.0 * sp->qbm * sp->q[p] / (1.0 - sp->qbm * sp->psi[p]), phi = 0.0;
+        for (int jj = 0; jj < 2; jj++) FA(chi, 0, 0, nn + jj) = FA(chi, 0, 0, nn + jj) + wt[jj] * phr;
+        for (int m = 1; m <= M; m++) {
+            double a = phr * ph0r - phi * ph0i, b = phr * ph0i + phi * ph0r;
+            phr = a; phi = b;
+            for (int jj = 0; jj < 2; jj++) {
+                FA(chi, 0, kre(m), nn + jj) = FA(chi, 0, kre(m), nn + jj) + wt[jj] * phr;
+                FA(chi, 0, kim(m), nn + jj) = FA(chi, 0, kim(m), nn + jj) + wt[jj] * phi;
+            }
+        }
+    }
+    const double ppc = (double)sp->cfg.ppc1;
+    const double corr = (12.0 * ppc * ppc) / (1.0 + 2.0 * ppc * ppc);
+    FA(chi, 0, 0, 0) = 0.0;
+    FA(chi, 0, 0, 1) = FA(chi, 0, 0, 1) * corr;
+    for (int j = 2; j <= nr + 1; j++) FA(chi, 0, 0, j) = FA(chi, 0, 0, j) * (1.0 / (double)(j - 1));
+    for (int m = 1; m <= M; m++) {
+        FA(chi, 0, kre(m), 0) = 0.0; FA(chi, 0, kim(m), 0) = 0.0;
+        FA(chi, 0, kre(m), 1) = 0.0; FA(chi, 0, kim(m), 1) = 0.0;
+        for (int j = 2; j <= nr + 1; j++) {
+            double ir = 1.0 / (double)(j - 1);
+            FA(chi, 0, kre(m), j) = FA(chi, 0, kre(m), j) * ir;
+            FA(chi, 0, kim(m), j) = FA(chi, 0, kim(m), j) * ir;
+        }
+    }
+}
+
 /* push_x_part2d: source/species/part2d_class.f03:2221-2262 */
 void ora_push_x(ora_species *sp, double dt)
 {
@@ -1501,6 +1803,8 @@ void ora_destroy(ora_sim *s)
     free(s->h_psi); free(s->h_e); free(s->h_b); free(s->h_b_spe); free(s->h_e_spe);
     free(s->h_cu); free(s->h_q_spe); free(s->h_q_beam);
     free(s->conv_re); free(s->conv_im);
+    free(s->lz_ar); free(s->lz_ai); free(s->ar); free(s->ai); free(s->ar_grad); free(s->ai_grad);
+    free(s->chi); free(s->h_chi);
     for (int t = 0; t < 6; t++)
         for (int m = 0; m <= s->g.max_mode; m++) {
             free(s->fac[t][m].lo); free(s->fac[t][m].inv); free(s->fac[t][m].cp);
@@ -1527,6 +1831,10 @@ int ora_add_species(ora_sim *s, const ora_species_cfg *cfg)
     ora_species *sp = &s->sp[isp];
     memset(sp, 0, sizeof(*sp));
     sp->cfg = *cfg;
+    /* a PGC species with no laser pushes in a zero envelope (laser_all of
+       sim_lasers exists with nlasers = 0) */
+    if ((cfg->push_type == ORA_PUSH2_STD_PGC || cfg->push_type == ORA_PUSH2_ROBUST_PGC) && laser_alloc(s))
+        return -1;
     /* npmax: fdist2d_class.f03:259-262 (xtra = 2) */
     sp->npmax = (int64_t)s->g.nr * cfg->ppc1 * cfg->ppc2 * cfg->num_theta * 2;
     int64_t n = sp->npmax;
@@ -1635,8 +1943,11 @@ void ora_slice(ora_sim *s, int j, int *n_iter)
     TSTOP(ORA_T_DEPOSIT_2D);
     ora_solve_psi(s, q_spe, psi);
     TSTOP(ORA_T_SOLVE_PSI);
-    for (int k = 0; k < s->nspecies; k++)                                /* species2d%interp_psi */
-        if (s->sp[k].cfg.push_type == ORA_PUSH2_STD) ora_interp_psi(s, &s->sp[k], psi);
+    for (int k = 0; k < s->nspecies; k++) {                              /* species2d%interp_psi */
+        const int pt = s->sp[k].cfg.push_type;
+        if (pt == ORA_PUSH2_STD || pt == ORA_PUSH2_STD_PGC) ora_interp_psi(s, &s->sp[k], psi);
+    }
+    if (s->laser_on) ora_laser_slice(s, j);                              /* laser_all of slice j */
     ora_solve_bz(s, cu, b_spe);
     TSTOP(ORA_T_SOLVE_BZ);
 
@@ -1659,8 +1970,12 @@ void ora_slice(ora_sim *s, int j, int *n_iter)
             memset(sp->cu_own, 0, 3 * n1 * sizeof(double));
             memset(sp->acu_own, 0, 2 * n1 * sizeof(double));
             memset(sp->amu_own, 0, 3 * n1 * sizeof(double));
-            if (sp->cfg.push_type == ORA_PUSH2_STD)
+            const int pt = sp->cfg.push_type;
+            if (pt == ORA_PUSH2_STD)
                 ora_amjdeposit_std(s, sp, e, b, sp->cu_own, sp->amu_own, sp->acu_own, s->g.dxi);
+            else if (pt == ORA_PUSH2_STD_PGC || pt == ORA_PUSH2_ROBUST_PGC)
+                ora_amjdeposit_pgc(s, sp, e, b, sp->cu_own, sp->amu_own, sp->acu_own, s->g.dxi,
+                                   pt == ORA_PUSH2_STD_PGC);
             else
                 ora_amjdeposit_robust(s, sp, e, b, sp->cu_own, sp->amu_own, sp->acu_own, s->g.dxi);
             add_into(cu, sp->cu_own, 3 * n1);
@@ -1683,6 +1998,11 @@ void ora_slice(ora_sim *s, int j, int *n_iter)
     s->last_abs = ab;
     if (n_iter) *n_iter = nit;
 
+    if (s->laser_on) {                                                   /* sim_lasers%deposit_chi */
+        memset(s->chi, 0, n1 * sizeof(double));
+        for (int k = 0; k < s->nspecies; k++) ora_deposit_chi(s, &s->sp[k], s->chi);
+        memcpy(s->h_chi + (size_t)j * n1, s->chi, n1 * sizeof(double));
+    }
     for (int k = 0; k < s->nspecies; k++) {                              /* species2d%cbq */
         ora_species *sp = &s->sp[k];
         for (size_t t = 0; t < n1; t++) sp->q_own[t] = sp->q_own[t] + sp->cu_own[2 * n1 + t];
@@ -1707,7 +2027,9 @@ void ora_slice(ora_sim *s, int j, int *n_iter)
 
     for (int k = 0; k < s->nspecies; k++) {
         ora_species *sp = &s->sp[k];
-        if (sp->cfg.push_type == ORA_PUSH2_STD) ora_push_u_std(s, sp, e, b, s->g.dxi);
+        const int pt = sp->cfg.push_type;
+        if (pt == ORA_PUSH2_STD) ora_push_u_std(s, sp, e, b, s->g.dxi);
+        else if (pt == ORA_PUSH2_STD_PGC || pt == ORA_PUSH2_ROBUST_PGC) ora_push_u_pgc(s, sp, e, b, s->g.dxi);
         else ora_push_u_robust(s, sp, e, b, s->g.dxi);
         ora_push_x(sp, s->g.dxi);
         TSTOP(ORA_T_PUSH_2D);
@@ -1747,6 +2069,21 @@ void ora_run(ora_sim *s, int nstep3d, int nslices)
 }
 
 double *ora_field(ora_sim *s, int fid) { return (fid >= 0 && fid < ORA_NFIELD) ? s->f[fid] : NULL; }
+
+double *ora_laser_array(ora_sim *s, int which)
+{
+    switch (which) {
+    case 0: return s->ar;
+    case 1: return s->ai;
+    case 2: return s->ar_grad;
+    case 3: return s->ai_grad;
+    case 4: return s->chi;
+    case 5: return s->h_chi;
+    case 6: return s->lz_ar;
+    case 7: return s->lz_ai;
+    default: return NULL;
+    }
+}
 
 double *ora_hist(ora_sim *s, int fid)
 {

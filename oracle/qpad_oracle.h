@@ -72,13 +72,15 @@ typedef struct {
     double den_min;            /* species(i).den_min (default 1e-10)        */
     uint64_t seed;             /* Philox key for thermal draws (uth != 0)   */
     int push_type;             /* species(i).push_type: ORA_PUSH2_STD
-                                  ("standard") or ORA_PUSH2_ROBUST
-                                  (param.f03:63, sim_plasma_class.f03:
-                                  123-137)                                   */
+                                  ("standard"), ORA_PUSH2_ROBUST, ORA_PUSH2_
+                                  STD_PGC, ORA_PUSH2_ROBUST_PGC (param.f03:
+                                  63-64, sim_plasma_class.f03:123-137)       */
 } ora_species_cfg;
 
 #define ORA_PUSH2_STD 0
 #define ORA_PUSH2_ROBUST 1
+#define ORA_PUSH2_STD_PGC 4
+#define ORA_PUSH2_ROBUST_PGC 5
 
 typedef struct {
     int64_t total_num;         /* beam(i).total_num                         */
@@ -157,6 +159,15 @@ typedef struct {
     /* diagnostics of the last slice */
     int last_iters;
     double last_rel, last_abs;
+    /* laser (PGC): the envelope history (cfr, cfi of field_laser, slices
+       -1 .. nxi+1, [(nxi + 3)][nm][ld]), the slice's envelope and gradients
+       (laser_all: ar, ai [nm][ld]; ar_grad, ai_grad [3][nm][ld], components
+       r, phi, z) and the plasma susceptibility chi with its history
+       [(nxi + 2)][nm][ld] */
+    int laser_on;
+    double *lz_ar, *lz_ai;
+    double *ar, *ai, *ar_grad, *ai_grad;
+    double *chi, *h_chi;
 } ora_sim;
 
 /* ---- low-level building blocks (exported for unit tests) ---- */
@@ -193,6 +204,13 @@ void ora_amjdeposit_std(const ora_sim *s, ora_species *sp,
 void ora_push_u_std(const ora_sim *s, ora_species *sp, const double *e,
                     const double *b, double dt);
 void ora_push_x(ora_species *sp, double dt);
+/* laser (PGC) */
+int ora_set_laser(ora_sim *s, const double *ar_hist, const double *ai_hist);
+void ora_laser_slice(ora_sim *s, int j);
+void ora_amjdeposit_pgc(const ora_sim *s, ora_species *sp, const double *e, const double *b,
+                        double *cu, double *amu, double *dcu, double dt, int std_);
+void ora_push_u_pgc(const ora_sim *s, ora_species *sp, const double *e, const double *b, double dt);
+void ora_deposit_chi(const ora_sim *s, const ora_species *sp, double *chi);
 void ora_update_bound(const ora_sim *s, ora_species *sp);
 void ora_interp_point(const ora_sim *s, const double *f, int ncomp,
                       double x1, double x2, double *out);
@@ -219,6 +237,9 @@ void ora_run(ora_sim *s, int nstep3d, int nslices);
 
 double *ora_field(ora_sim *s, int fid);
 double *ora_hist(ora_sim *s, int fid);
+/* laser arrays (NULL before ora_set_laser or a PGC species): 0 ar, 1 ai,
+   2 ar_grad, 3 ai_grad, 4 chi, 5 h_chi, 6 the ar history, 7 the ai history */
+double *ora_laser_array(ora_sim *s, int which);
 int ora_nm(const ora_sim *s);
 int ora_ld(const ora_sim *s);
 ora_species *ora_get_species(ora_sim *s, int isp);

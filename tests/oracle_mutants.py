@@ -67,9 +67,9 @@ MUTANTS = {
                                 ["amjdeposit"]),
     # text shared by the robust and the standard deposit: planted in both
     "amj_flux_missing_ipsi": ("u2[0] = u[0] * u[0] * ipsi;", "u2[0] = u[0] * u[0];",
-                              ["amjdeposit", "amjdeposit_std"], 2),
+                              ["amjdeposit", "amjdeposit_std", "amjdeposit_pgc"], 3),
     "amj_phase_not_conjugated": ("double ph0r = pc, ph0i = -ps;", "double ph0r = pc, ph0i = ps;",
-                                 ["amjdeposit", "amjdeposit_std"], 2),
+                                 ["amjdeposit", "amjdeposit_std", "amjdeposit_pgc"], 3),
     "std_normalisation_sign": ("double qtmh2 = qtmh / (1.0 - sp->qbm * sp->psi[p]);",
                                "double qtmh2 = qtmh / (1.0 + sp->qbm * sp->psi[p]);", ["amjdeposit_std"]),
     "std_weight_uses_gamma": ("double ipsi = 1.0 / (1.0 - sp->qbm * sp->psi[p]);",
@@ -108,6 +108,37 @@ MUTANTS = {
     "lattice_gauss_sigma_squared": ("double t = (x - mu) / sigma;\n        return exp(-0.5 * (t * t));",
                                     "double t = (x - mu) / sigma;\n        return exp(-1.0 * (t * t));",
                                     ["beam_lattice"]),
+    "laser_grad_phi_sign": ("FA(gr, 1, kr, i) = -ir * m * LZ(h, j, ki, i);",
+                            "FA(gr, 1, kr, i) = ir * m * LZ(h, j, ki, i);", ["laser_grad"]),
+    "laser_grad_xi_stencil": ("- 4.0 * LZ(h, j - 1, 0, i) + LZ(h, j - 2, 0, i));",
+                              "- 4.0 * LZ(h, j - 1, 0, i) - LZ(h, j - 2, 0, i));", ["laser_grad"]),
+    "laser_grad_axis_on_node1": ("const int ie = nr + 1;", "const int ie = 1;", ["laser_grad"]),
+    "laser_grad_axis_factor": ("FA(gr, 0, kr, ie) = 2.0 * idrh * LZ(h, j, kr, 2);",
+                               "FA(gr, 0, kr, ie) = idrh * LZ(h, j, kr, 2);", ["laser_grad"]),
+    "pgc_quiver_factor": ("double gam_corr = 0.5 * sp->qbm * sp->qbm * (apr * apr + api * api);",
+                          "double gam_corr = sp->qbm * sp->qbm * (apr * apr + api * api);", ["amjdeposit_pgc"]),
+    "pgc_ponderomotive_xi_sign": ("ep[2] = ep[2] + tmp * (apr * aprg[2] + api * apig[2]);",
+                                  "ep[2] = ep[2] - tmp * (apr * aprg[2] + api * apig[2]);",
+                                  ["amjdeposit_pgc", "push_pgc"], 2),
+    "pgc_drops_imaginary_envelope": ("ep[0] = ep[0] - tmp * (apr * aprg[0] + api * apig[0]);",
+                                     "ep[0] = ep[0] - tmp * (apr * aprg[0]);", ["amjdeposit_pgc", "push_pgc"], 2),
+    "pgc_robust_second_kick_initial_u": ("qtmh_e = std_ ? qtmh_b * gam : qtmh * gam / (gam - ut[2]);",
+                                         "qtmh_e = std_ ? qtmh_b * gam : qtmh * gam / (gam - u0[2]);",
+                                         ["amjdeposit_pgc"]),
+    "pgc_std_kick_without_gamma": ("qtmh_b = qtmh / (1.0 - sp->qbm * sp->psi[p]);\n            qtmh_e = qtmh_b * gam;",
+                                   "qtmh_b = qtmh / (1.0 - sp->qbm * sp->psi[p]);\n            qtmh_e = qtmh_b;",
+                                   ["amjdeposit_pgc"]),
+    "pgc_push_half_step_ahead": ("gam_corr = gam_corr + qbm2_hf * (apr + 0.25 * tmp) * tmp;",
+                                 "gam_corr = gam_corr + qbm2_hf * (apr + 0.5 * tmp) * tmp;", ["push_pgc"]),
+    "pgc_push_force_without_gamma": ("double tmp = 0.5 * sp->qbm / sp->gamma[p];",
+                                     "double tmp = 0.5 * sp->qbm;", ["push_pgc"]),
+    "chi_psi_sign": ("double phr = -1.0 * sp->qbm * sp->q[p] / (1.0 - sp->qbm * sp->psi[p]), phi = 0.0;",
+                     "double phr = -1.0 * sp->qbm * sp->q[p] / (1.0 + sp->qbm * sp->psi[p]), phi = 0.0;",
+                     ["deposit_chi"]),
+    "chi_axis_factor_8": ("const double corr = (12.0 * ppc * ppc) / (1.0 + 2.0 * ppc * ppc);",
+                          "const double corr = 8.0;", ["deposit_chi"]),
+    "chi_uses_charge_not_q_over_m": ("double phr = -1.0 * sp->qbm * sp->q[p]",
+                                     "double phr = -1.0 * sp->qm * sp->q[p]", ["deposit_chi"]),
     "update_bound_skips_one": ("while (i < sp->npp - 1) {\n        double pos = sqrt(sp->x1[i]",
                                "while (i < sp->npp - 2) {\n        double pos = sqrt(sp->x1[i]",
                                ["update_bound"]),

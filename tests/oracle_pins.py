@@ -827,6 +827,369 @@ def pin_beam_lattice(L):
     assert np.max(np.abs(b["p3"] - np.sqrt(1000.0 ** 2 - b["p1"] ** 2 - b["p2"] ** 2 - 1.0))) <= 1e-9
 
 
+# ---------------------------------------------------------------------------
+# laser (PGC)
+# ---------------------------------------------------------------------------
+def _envelope_hist(o, fn):
+    """[nxi + 3, nm, ld] history of fn(k, r, xi) over slices -1 .. nxi + 1
+    (slice s at xi = s dxi) and nodes 0 .. nr + 1"""
+    g = o.grid
+    r = nodes(g.nr, g.dr)
+    xi = np.arange(-1, g.nxi + 2) * g.dxi
+    out = np.zeros((g.nxi + 3, o.nm, o.ld))
+    for k in range(o.nm):
+        out[:, k, :] = fn(k, r[None, :], xi[:, None])
+    return out
+
+
+def _mode_of(k):
+    return 0 if k == 0 else (k + 1) // 2
+
+
+def pin_laser_grad(L):
+    """set_grad_field_laser (field_laser_class.f03:637-750): for an envelope
+    quadratic in r and in xi per mode, the central r difference and the
+    three-point backward xi difference are exact, and the phi component is
+    the mode's (1/r) d/dphi: (-m Im, m Re) / r.  Components (r, phi, z) of
+    each mode at nodes 2..nr (xi: 1..nr).  On the axis the m > 0 values are
+    written at node nr + 1 (the loop index after its loop), node 1 keeping
+    zero: for an odd mode linear in r they are the axis limits dA/dr and
+    (i m / r) A at r -> 0; even modes get zero there."""
+    nr, nxi, M, dr, dxi = 12, 6, 3, 0.1, 0.05
+    o = po.Oracle(po.make_grid(nr, nxi, M, dr, dxi), L)
+    rng = np.random.default_rng(11)
+    co = rng.standard_normal((2, 2 * M + 1, 6))
+    for t in range(2):
+        for k in range(2 * M + 1):
+            if _mode_of(k) % 2 == 1:
+                co[t, k, 0] = co[t, k, 2] = 0.0   # odd modes: linear in r
+
+    def f(c, r, xi):
+        return (c[0] + c[1] * r + c[2] * r * r) * (c[3] + c[4] * xi + c[5] * xi * xi)
+
+    def f_r(c, r, xi):
+        return (c[1] + 2 * c[2] * r) * (c[3] + c[4] * xi + c[5] * xi * xi)
+
+    def f_xi(c, r, xi):
+        return (c[0] + c[1] * r + c[2] * r * r) * (c[4] + 2 * c[5] * xi)
+
+    hist = [_envelope_hist(o, lambda k, r, xi, t=t: f(co[t, k], r, xi)) for t in range(2)]
+    o.set_laser(hist[0], hist[1])
+    r = nodes(nr, dr)
+    inner, ri = slice(2, nr + 1), r[2:nr + 1]
+    for j in (1, 2, nxi):
+        xj = j * dxi
+        o.laser_slice(j)
+        for t, (wa, wg) in enumerate(((po.LASER_AR, po.LASER_AR_GRAD), (po.LASER_AI, po.LASER_AI_GRAD))):
+            assert np.array_equal(o.laser(wa)[0], hist[t][j + 1]), "envelope of the slice"
+            gr = o.laser(wg)
+            for k in range(2 * M + 1):
+                m, c = _mode_of(k), co[t, k]
+                what = f"{'ar' if t == 0 else 'ai'} grad, slice {j}, mode row {k}"
+                close(gr[2, k], f_xi(c, r, xj), slice(1, nr + 1), what + " xi")
+                close(gr[0, k], f_r(c, r, xj), inner, what + " r")
+                if m == 0:
+                    want_phi = np.zeros(nr - 1)
+                elif k == kre(m):
+                    want_phi = -m * f(co[t, kim(m)], ri, xj) / ri
+                else:
+                    want_phi = m * f(co[t, kre(m)], ri, xj) / ri
+                assert np.max(np.abs(gr[1, k, inner] - want_phi)) <= TOL * max(1.0, np.max(np.abs(want_phi))), \
+                    what + " phi"
+                assert np.all(gr[:, k, 0] == 0.0), what + ": node 0"
+                assert gr[0, k, 1] == 0.0 and gr[1, k, 1] == 0.0, what + ": axis node r, phi"
+                assert gr[2, k, nr + 1] == 0.0, what + ": xi at node nr + 1"
+                if m == 0 or m % 2 == 0:
+                    assert gr[0, k, nr + 1] == 0.0 and gr[1, k, nr + 1] == 0.0, what + ": node nr + 1"
+                    continue
+                # odd m: the axis limits land on node nr + 1
+                d_re, d_im = f_r(co[t, kre(m)], 0.0, xj), f_r(co[t, kim(m)], 0.0, xj)
+                want_r = d_re if k == kre(m) else d_im
+                want_p = -m * d_im if k == kre(m) else m * d_re
+                assert abs(gr[0, k, nr + 1] - want_r) <= TOL * max(1.0, abs(want_r)), what + ": axis dA/dr"
+                assert abs(gr[1, k, nr + 1] - want_p) <= TOL * max(1.0, abs(want_p)), what + ": axis (im/r) A"
+
+
+def _pgc_oracle(L, push_type, nr=16, nxi=6, M=2, dr=0.1, dxi=0.05, ppc1=1):
+    o = po.Oracle(po.make_grid(nr, nxi, M, dr, dxi), L)
+    o.add_species(po.species_cfg(ppc1, 1, 4, push_type=push_type))
+    return o
+
+
+def _deposit_pgc(L, o, e, b, dxi, std):
+    cu, amu, acu = o.new_field(3), o.new_field(3), o.new_field(2)
+    L.ora_amjdeposit_pgc(o.h, o.species_raw(0), po.ptr(e), po.ptr(b), po.ptr(cu), po.ptr(amu), po.ptr(acu),
+                         dxi, int(std))
+    return cu, amu, acu
+
+
+def _lin(fn, rp, dr):
+    """linear interpolation of node values fn(r) at radius rp"""
+    pos = rp / dr
+    i = math.floor(pos)
+    d = pos - i
+    return (1 - d) * fn(i * dr) + d * fn((i + 1) * dr)
+
+
+def _random_fields(o, rng):
+    e, b = o.new_field(3), o.new_field(3)
+    for f in (e, b):
+        f[:, :, 1:] = 0.3 * rng.standard_normal(f[:, :, 1:].shape)
+    return e, b
+
+
+def pin_amjdeposit_pgc(L):
+    """amjdeposit_robust_pgc / _std_pgc (part2d_class.f03:1012-1602).  In a
+    zero envelope the standard one is the standard deposit (pinned above);
+    the robust one is the robust deposit where E = 0 and differs from it in
+    the second half kick, normalised by gamma / (gamma - u_z) of the rotated
+    momentum rather than of the initial one: from rest in a uniform E_z,
+    u_z = k E_z + k E_z g1 / (g1 - k E_z), k = q/m dxi / 2,
+    g1 = sqrt(1 + (k E_z)^2).  In a
+    uniform envelope a particle at rest stays at rest with the quiver
+    gamma sqrt(1 + (q/m)^2 |a|^2 / 2) (robust: psi = (1 - gamma) / (q/m)).
+    For a weak envelope a(r, xi) at rest the step is the kick of the
+    ponderomotive field F = -(q/m)/(2 gamma) (d_r, (1/r) d_phi, -d_xi) |a|^2/2
+    (d_z = -d_xi): u = q/m dxi F (robust, gamma / (gamma - u_z) = 1), or
+    q/m dxi gamma F / (1 - q/m psi) (standard), to first order in u."""
+    nr, M, dr, dxi, q = 16, 2, 0.1, 0.05, -0.7
+    rng = np.random.default_rng(21)
+    for pt, std, ref in ((po.PUSH2_ROBUST_PGC, 0, "ora_amjdeposit_robust"),
+                         (po.PUSH2_STD_PGC, 1, "ora_amjdeposit_std")):
+        # zero envelope: the deposit without laser
+        o = _pgc_oracle(L, pt)
+        o.laser_slice(2)
+        e, b = _random_fields(o, rng)
+        if not std:
+            e[...] = 0.0
+        for rp, phi in ((0.37 * dr, 0.9), (5.6 * dr, 2.2)):
+            p = tuple(0.4 * rng.standard_normal(3))
+            x1, x2 = rp * math.cos(phi), rp * math.sin(phi)
+            outs = []
+            for fn in ("pgc", ref):
+                _one_particle(o, x1, x2, p, q)
+                o.species(0)["psi"][0] = -0.2
+                if fn == "pgc":
+                    dep = _deposit_pgc(L, o, e, b, dxi, std)
+                else:
+                    dep = (o.new_field(3), o.new_field(3), o.new_field(2))
+                    getattr(L, fn)(o.h, o.species_raw(0), po.ptr(e), po.ptr(b), po.ptr(dep[0]), po.ptr(dep[1]),
+                                   po.ptr(dep[2]), dxi)
+                s = o.species(0)
+                outs.append(list(dep) + [s["gamma"].copy(), s["psi"].copy()])
+            for name, got, want in zip(("cu", "amu", "acu", "gamma", "psi"), *outs):
+                err = np.max(np.abs(got - want)) / max(1e-300, np.max(np.abs(want)))
+                assert err <= 1e-13, f"zero envelope, push type {pt}: {name} {err:.2e}"
+        if not std:
+            ez, qbm = 0.8, -1.0
+            e, b = o.new_field(3), o.new_field(3)
+            e[2, 0, :] = ez
+            rp, phi = 3.3 * dr, 0.4
+            x1, x2 = rp * math.cos(phi), rp * math.sin(phi)
+            _one_particle(o, x1, x2, (0.0, 0.0, 0.0), q)
+            cu, amu, acu = _deposit_pgc(L, o, e, b, dxi, std)
+            k = 0.5 * qbm * dxi
+            g1 = math.sqrt(1 + (k * ez) ** 2)
+            uz = 0.5 * (k * ez + k * ez * g1 / (g1 - k * ez))
+            gc = math.sqrt(1 + uz * uz)
+            want = _expected(nr, dr, M, x1, x2, {"cu": [0.0, 0.0, q * uz / (gc - uz)], "amu": [0.0] * 3,
+                                                 "acu": [0.0] * 2})
+            for name, got in (("cu", cu), ("amu", amu), ("acu", acu)):
+                assert np.max(np.abs(got - want[name])) <= 1e-14, f"robust E_z kick: {name}"
+            assert abs(o.species(0)["gamma"][0] - gc) <= 1e-14
+        o.close()
+        # uniform envelope: the quiver gamma
+        o = _pgc_oracle(L, pt)
+        a_r, a_i, qbm = 0.8, -0.5, -1.0
+        o.set_laser(np.full((o.grid.nxi + 3, o.nm, o.ld), a_r) * (np.arange(o.nm) == 0)[None, :, None],
+                    np.full((o.grid.nxi + 3, o.nm, o.ld), a_i) * (np.arange(o.nm) == 0)[None, :, None])
+        o.laser_slice(3)
+        e, b = o.new_field(3), o.new_field(3)
+        _one_particle(o, 4.3 * dr * math.cos(0.7), 4.3 * dr * math.sin(0.7), (0.0, 0.0, 0.0), q)
+        o.species(0)["psi"][0] = 0.1
+        cu, amu, acu = _deposit_pgc(L, o, e, b, dxi, std)
+        gq = math.sqrt(1 + qbm * qbm * (a_r * a_r + a_i * a_i) / 2)
+        s = o.species(0)
+        assert abs(s["gamma"][0] - gq) <= 1e-14, ("quiver gamma", s["gamma"][0], gq)
+        want_psi = 0.1 if std else (1 - gq) / qbm
+        assert abs(s["psi"][0] - want_psi) <= 1e-14, ("psi", s["psi"][0], want_psi)
+        # at rest up to the rounding of the xi difference of a constant
+        assert np.max(np.abs(cu)) <= 1e-14 and np.max(np.abs(amu)) <= 1e-14 and np.max(np.abs(acu)) <= 1e-14
+        o.close()
+        # weak envelope varying in r and xi: the ponderomotive kick
+        o = _pgc_oracle(L, pt)
+        eps, j = 1e-3, 3
+        cr, ci = (0.9, 0.0, -1.7, 1.0, 2.0, 0.0), (0.4, 0.0, 2.3, 1.0, -3.0, 0.0)
+
+        def env(c, r, xi):
+            return eps * (c[0] + c[2] * r * r) * (c[3] + c[4] * xi)
+
+        o.set_laser(_envelope_hist(o, lambda k, r, xi: env(cr, r, xi) * (k == 0)),
+                    _envelope_hist(o, lambda k, r, xi: env(ci, r, xi) * (k == 0)))
+        o.laser_slice(j)
+        xj = j * dxi
+        psi_p = 0.3
+        for rp, phi in ((4.3 * dr, 0.7), (9.8 * dr, -2.0)):
+            _one_particle(o, rp * math.cos(phi), rp * math.sin(phi), (0.0, 0.0, 0.0), q)
+            o.species(0)["psi"][0] = psi_p
+            cu, amu, acu = _deposit_pgc(L, o, e, b, dxi, std)
+            A = [_lin(lambda r, c=c: env(c, r, xj), rp, dr) for c in (cr, ci)]
+            Gr = [eps * 2 * c[2] * rp * (c[3] + c[4] * xj) for c in (cr, ci)]
+            Gz = [_lin(lambda r, c=c: eps * (c[0] + c[2] * r * r) * c[4], rp, dr) for c in (cr, ci)]
+            g0 = math.sqrt(1 + qbm * qbm * (A[0] ** 2 + A[1] ** 2) / 2)
+            fr = -qbm / (2 * g0) * (A[0] * Gr[0] + A[1] * Gr[1])
+            fz = qbm / (2 * g0) * (A[0] * Gz[0] + A[1] * Gz[1])
+            k = qbm * dxi * (g0 / (1 - qbm * psi_p) if std else 1.0)
+            u = (k * fr, 0.0, k * fz)
+            uc = tuple(0.5 * x for x in u)
+            ipsi = 1 / (1 - qbm * psi_p) if std else 1 / g0
+            want = _expected(nr, dr, M, rp * math.cos(phi), rp * math.sin(phi), {
+                "cu": [q * ipsi * x for x in uc],
+                "acu": [q * ipsi * u[0] / dxi, 0.0],
+                "amu": [q * ipsi ** 2 * uc[0] * uc[0], 0.0, 0.0]})
+            for name, got in (("cu", cu), ("acu", acu), ("amu", amu)):
+                err = np.max(np.abs(got - want[name])) / np.max(np.abs(want[name]))
+                assert err <= 1e-6, f"ponderomotive kick, push type {pt}, r = {rp / dr:.1f} dr: {name} {err:.2e}"
+        o.close()
+
+
+def pin_push_pgc(L):
+    """push_u_robust_pgc / _std_pgc (part2d_class.f03:1967-2219), the same
+    steps.  In a zero envelope they are push_u_std (pinned above).  From rest
+    in E = B = 0 both half kicks use the stored gamma and psi, so
+    p = q/m dxi gamma_s F / (1 - q/m psi) exactly, F the ponderomotive field
+    (Cartesian), and the new gamma is sqrt(1 + p^2 + (q/m)^2 |a|^2 / 2) with
+    the envelope taken half a step ahead, a(xi + dxi/2) (exact for a linear
+    in xi)."""
+    nr, M, dr, dxi = 16, 2, 0.1, 0.05
+    rng = np.random.default_rng(31)
+    for pt in (po.PUSH2_ROBUST_PGC, po.PUSH2_STD_PGC):
+        o = _pgc_oracle(L, pt)
+        o.laser_slice(2)
+        e, b = _random_fields(o, rng)
+        for rp, phi in ((0.37 * dr, 0.9), (5.6 * dr, 2.2)):
+            p = tuple(0.4 * rng.standard_normal(3))
+            outs = []
+            for fn in (L.ora_push_u_pgc, L.ora_push_u_std):
+                _one_particle(o, rp * math.cos(phi), rp * math.sin(phi), p)
+                o.species(0)["psi"][0] = -0.2
+                o.species(0)["gamma"][0] = 1.9
+                fn(o.h, o.species_raw(0), po.ptr(e), po.ptr(b), dxi)
+                s = o.species(0)
+                outs.append([s[k].copy() for k in ("p1", "p2", "p3", "gamma")])
+            for name, got, want in zip(("p1", "p2", "p3", "gamma"), *outs):
+                assert abs(got[0] - want[0]) <= 1e-14 * max(1.0, abs(want[0])), f"zero envelope: {name}"
+        o.close()
+        qbm, gs, psi_p = -1.0, 1.6, 0.3
+        # envelope linear in xi, uniform in r (m = 0): a z kick
+        o = _pgc_oracle(L, pt)
+        cr, ci = (0.6, 1.4), (-0.3, 2.2)
+        o.set_laser(_envelope_hist(o, lambda k, r, xi: (cr[0] + cr[1] * xi + 0 * r) * (k == 0)),
+                    _envelope_hist(o, lambda k, r, xi: (ci[0] + ci[1] * xi + 0 * r) * (k == 0)))
+        j = 3
+        xj = j * dxi
+        o.laser_slice(j)
+        e, b = o.new_field(3), o.new_field(3)
+        _one_particle(o, 4.3 * dr * math.cos(1.1), 4.3 * dr * math.sin(1.1), (0.0, 0.0, 0.0))
+        o.species(0)["psi"][0] = psi_p
+        o.species(0)["gamma"][0] = gs
+        L.ora_push_u_pgc(o.h, o.species_raw(0), po.ptr(e), po.ptr(b), dxi)
+        s = o.species(0)
+        A = (cr[0] + cr[1] * xj, ci[0] + ci[1] * xj)
+        fz = qbm / (2 * gs) * (A[0] * cr[1] + A[1] * ci[1])
+        pz = qbm * dxi * gs * fz / (1 - qbm * psi_p)
+        assert abs(s["p3"][0] - pz) <= 1e-14 * abs(pz) and s["p1"][0] == 0 and s["p2"][0] == 0, \
+            ("xi kick", s["p3"][0], pz)
+        ah = (cr[0] + cr[1] * (xj + dxi / 2), ci[0] + ci[1] * (xj + dxi / 2))
+        gn = math.sqrt(1 + pz * pz + qbm * qbm * (ah[0] ** 2 + ah[1] ** 2) / 2)
+        assert abs(s["gamma"][0] - gn) <= 1e-14 * gn, ("gamma half a step ahead", s["gamma"][0], gn)
+        o.close()
+        # envelope quadratic in r, constant in xi: a radial kick along (cos, sin)
+        o = _pgc_oracle(L, pt)
+        cr, ci = (0.5, -1.3), (0.2, 0.8)
+        o.set_laser(_envelope_hist(o, lambda k, r, xi: (cr[0] + cr[1] * r * r + 0 * xi) * (k == 0)),
+                    _envelope_hist(o, lambda k, r, xi: (ci[0] + ci[1] * r * r + 0 * xi) * (k == 0)))
+        o.laser_slice(j)
+        for rp, phi in ((4.3 * dr, 1.1), (10.6 * dr, -2.4)):
+            _one_particle(o, rp * math.cos(phi), rp * math.sin(phi), (0.0, 0.0, 0.0))
+            o.species(0)["psi"][0] = psi_p
+            o.species(0)["gamma"][0] = gs
+            L.ora_push_u_pgc(o.h, o.species_raw(0), po.ptr(e), po.ptr(b), dxi)
+            s = o.species(0)
+            A = [_lin(lambda r, c=c: c[0] + c[1] * r * r, rp, dr) for c in (cr, ci)]
+            G = [2 * c[1] * rp for c in (cr, ci)]
+            fr = -qbm / (2 * gs) * (A[0] * G[0] + A[1] * G[1])
+            pr = qbm * dxi * gs * fr / (1 - qbm * psi_p)
+            want = (pr * math.cos(phi), pr * math.sin(phi))
+            assert abs(s["p1"][0] - want[0]) <= 1e-13 * abs(pr) and abs(s["p2"][0] - want[1]) <= 1e-13 * abs(pr), \
+                ("radial kick", s["p1"][0], s["p2"][0], want)
+            # the xi difference 3a - 4a + a of a constant is zero up to rounding
+            assert abs(s["p3"][0]) <= 1e-14 * abs(pr)
+        o.close()
+
+
+def pin_deposit_chi(L):
+    """deposit_chi_part2d (part2d_class.f03:361-476): chi = -(q/m) q /
+    (1 - q/m psi) per particle, so a uniform plasma of density n has
+    chi = -n (q/m) |q/e| = -n q^2/m in these units, at every node up to the
+    edge: the axis factor get_deposit_ax_corr = 12 ppc^2 / (1 + 2 ppc^2)
+    makes the ppc lattice exact on the axis too, for any ppc; m > 0 empty;
+    a uniform psi divides by (1 - q/m psi).  One particle deposits its value
+    with the linear weights and e^{-im phi} over the node index, m > 0
+    axis values zero.  Called once per species on one array, the rescaling
+    applies to the sum so far: the first species' deposit is divided by the
+    node index twice."""
+    nr, M, dr = 16, 2, 0.1
+    for qm, mass, dens, ppc1 in ((-1.0, 1.0, 1.0, 1), (1.0, 2.0, 1.7, 2), (-1.0, 1.0, 0.6, 3)):
+        o = po.Oracle(po.make_grid(nr, 4, M, dr, 0.05), L)
+        o.add_species(po.species_cfg(ppc1, 2, 8, qm=qm, mass=mass, density=dens, push_type=po.PUSH2_ROBUST_PGC))
+        chi = o.new_field(1)
+        L.ora_deposit_chi(o.h, o.species_raw(0), po.ptr(chi))
+        want = -dens * qm * qm / mass
+        assert np.max(np.abs(chi[0, 0, 1:nr] - want)) <= 1e-12 * abs(want), ("uniform plasma", ppc1, chi[0, 0, :4])
+        assert chi[0, 0, 0] == 0 and np.max(np.abs(chi[0, 1:])) <= 1e-12 * abs(want), "m > 0 of a uniform plasma"
+        o.species(0)["psi"][:] = 0.35
+        chi2 = o.new_field(1)
+        L.ora_deposit_chi(o.h, o.species_raw(0), po.ptr(chi2))
+        assert np.max(np.abs(chi2[0, 0, 1:nr] - want / (1 - qm / mass * 0.35))) <= 1e-12 * abs(want), "psi"
+        o.close()
+    o = po.Oracle(po.make_grid(nr, 4, M, dr, 0.05), L)
+    o.add_species(po.species_cfg(1, 1, 4, push_type=po.PUSH2_STD_PGC))
+    q, qbm, psi_p = -0.7, -1.0, 0.2
+    for rp, phi in ((3.3 * dr, 0.4), (0.6 * dr, -2.5), (8.1 * dr, 2.9)):
+        x1, x2 = rp * math.cos(phi), rp * math.sin(phi)
+        _one_particle(o, x1, x2, (0.0, 0.0, 0.0), q)
+        o.species(0)["psi"][0] = psi_p
+        chi = o.new_field(1)
+        L.ora_deposit_chi(o.h, o.species_raw(0), po.ptr(chi))
+        val = -qbm * q / (1 - qbm * psi_p)
+        pos = rp / dr
+        ix, d = math.floor(pos) + 1, pos - math.floor(pos)
+        for m in range(M + 1):
+            for node, w in ((ix, 1 - d), (ix + 1, d)):
+                if node == 1:
+                    if m > 0:
+                        assert chi[0, kre(m), 1] == 0 and chi[0, kim(m), 1] == 0, "m > 0 on the axis"
+                    continue
+                z = w * val * np.exp(-1j * m * phi) / (node - 1)
+                assert abs(chi[0, kre(m), node] - z.real) <= 1e-14, ("one particle", m, node)
+                if m > 0:
+                    assert abs(chi[0, kim(m), node] - z.imag) <= 1e-14, ("one particle im", m, node)
+    o.close()
+    # two species into one array
+    o = po.Oracle(po.make_grid(nr, 4, M, dr, 0.05), L)
+    o.add_species(po.species_cfg(2, 2, 8, push_type=po.PUSH2_ROBUST_PGC))
+    o.add_species(po.species_cfg(1, 2, 6, qm=1.0, mass=3.0, density=0.5, push_type=po.PUSH2_ROBUST))
+    c1, c2, both = o.new_field(1), o.new_field(1), o.new_field(1)
+    L.ora_deposit_chi(o.h, o.species_raw(0), po.ptr(c1))
+    L.ora_deposit_chi(o.h, o.species_raw(1), po.ptr(c2))
+    L.ora_deposit_chi(o.h, o.species_raw(0), po.ptr(both))
+    L.ora_deposit_chi(o.h, o.species_raw(1), po.ptr(both))
+    jj = np.arange(2, nr + 2)
+    want = c1[0, :, 2:] / (jj - 1)[None, :] + c2[0, :, 2:]
+    assert np.max(np.abs(both[0, :, 2:] - want)) <= 1e-14, "the first species is rescaled twice"
+
+
 PINS = {
     "matrices": pin_matrices,
     "ez": pin_ez,
@@ -844,5 +1207,9 @@ PINS = {
     "beam_lattice": pin_beam_lattice,
     "inject": pin_inject,
     "update_bound": pin_update_bound,
+    "laser_grad": pin_laser_grad,
+    "amjdeposit_pgc": pin_amjdeposit_pgc,
+    "push_pgc": pin_push_pgc,
+    "deposit_chi": pin_deposit_chi,
 }
 
