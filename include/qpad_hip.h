@@ -54,16 +54,29 @@ extern "C" {
 #define QPAD_F_DCU 9
 #define QPAD_F_Q_SPE 10
 #define QPAD_F_Q_BEAM 11
-#define QPAD_NFIELD 12
+/* the laser envelope of the slice and its gradients (field_laser laser_all:
+   cfr, cfi and ar_grad, ai_grad with components r, phi, z;
+   field_laser_class.f03:952-1012, :637-750) and the plasma susceptibility
+   chi (sim_lasers%chi, sim_lasers_class.f03:176-200); chi has a history */
+#define QPAD_F_AR 12
+#define QPAD_F_AI 13
+#define QPAD_F_AR_GRAD 14
+#define QPAD_F_AI_GRAD 15
+#define QPAD_F_CHI 16
+#define QPAD_NFIELD 17
 
 /* boundary conditions (source/param.f03:71-74) */
 #define QPAD_BND_ZERO 2
 #define QPAD_BND_OPEN 3
 
-/* plasma pusher (source/param.f03:63-64): "standard" and "robust" (the PGC
-   pushers need the laser envelope, which is not on this path) */
+/* plasma pusher (source/param.f03:63-64, sim_plasma_class.f03:123-137):
+   "standard", "robust", "standard_pgc", "robust_pgc" (the ponderomotive
+   guiding centre pushers, which push in the laser envelope of
+   qpad_set_laser; a zero envelope until one is set) */
 #define QPAD_PUSH2_STD 0
 #define QPAD_PUSH2_ROBUST 1
+#define QPAD_PUSH2_STD_PGC 4
+#define QPAD_PUSH2_ROBUST_PGC 5
 
 /* grid and iteration settings: options_class.f03:44-117 (nr, nxi, dr, dxi),
    simulation_class.f03:141-172 (iter_max/reltol/abstol, dt),
@@ -202,6 +215,28 @@ int qpad_species_push(qpad_ctx *ctx, int32_t sp, double dxi);
    QPAD calls it after field_psi%solve (simulation_class.f03:357-359); the
    fused forms do it at the same point. */
 int qpad_species_interp_psi(qpad_ctx *ctx, int32_t sp);
+
+/* ---- laser (PGC pushers) ----
+   The laser envelope the PGC pushers push in: the real and imaginary part
+   of the complex envelope a of every slice, [(nxi + 3)][nm][nr + 2] each,
+   slices -1 .. nxi + 1 (QPAD's field_laser cfr, cfi f2, field_laser_class.f03:
+   104-180, with two guard slices below the box and one above,
+   sim_lasers_class.f03:107-108; with several lasers their sum).  Replaces
+   the envelope QPAD's laser solver keeps (sim_lasers%advance,
+   sim_lasers_class.f03:202-223, which stays with the caller between 3D
+   steps).  The first call (or the first PGC species)
+   switches the context's slices to the laser path: every slice then takes
+   its envelope and gradients (qpad_laser_slice) and deposits chi
+   (qpad_deposit_chi) into the QPAD_F_CHI history the laser solver reads. */
+int qpad_set_laser(qpad_ctx *ctx, const double *ar_hist, const double *ai_hist);
+/* laser_all of slice j: zero, copy_slice(j), set_grad(j), gather
+   (simulation_class.f03:362-367, field_laser_class.f03:637-750, :929-1012):
+   fields QPAD_F_AR, QPAD_F_AI, QPAD_F_AR_GRAD, QPAD_F_AI_GRAD */
+int qpad_laser_slice(qpad_ctx *ctx, int32_t j);
+/* sim_lasers%deposit_chi(spe, j) (sim_lasers_class.f03:176-200,
+   part2d_class.f03:361-476): chi of every species into QPAD_F_CHI and its
+   history slice j */
+int qpad_deposit_chi(qpad_ctx *ctx, int32_t j);
 
 /* field_psi%solve(q_spe) (field_psi_class.f03:218-254) */
 int qpad_field_solve_psi(qpad_ctx *ctx);

@@ -12,13 +12,15 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libqpad_hip.so")
 
 F_PSI, F_E, F_E_SPE, F_B, F_B_SPE, F_B_BEAM, F_CU, F_ACU, F_AMU, F_DCU, F_Q_SPE, F_Q_BEAM = range(12)
-NFIELD = 12
-NCOMP = [1, 3, 3, 3, 3, 3, 3, 2, 3, 2, 1, 1]
+F_AR, F_AI, F_AR_GRAD, F_AI_GRAD, F_CHI = range(12, 17)
+NFIELD = 17
+NCOMP = [1, 3, 3, 3, 3, 3, 3, 2, 3, 2, 1, 1, 1, 1, 3, 3, 1]
 FIELD_IDS = {"psi": F_PSI, "e": F_E, "e_spe": F_E_SPE, "b": F_B, "b_spe": F_B_SPE,
              "b_beam": F_B_BEAM, "cu": F_CU, "acu": F_ACU, "amu": F_AMU, "dcu": F_DCU,
-             "q_spe": F_Q_SPE, "q_beam": F_Q_BEAM}
+             "q_spe": F_Q_SPE, "q_beam": F_Q_BEAM, "ar": F_AR, "ai": F_AI, "ar_grad": F_AR_GRAD,
+             "ai_grad": F_AI_GRAD, "chi": F_CHI}
 BND_ZERO, BND_OPEN = 2, 3
-PUSH2_STD, PUSH2_ROBUST = 0, 1
+PUSH2_STD, PUSH2_ROBUST, PUSH2_STD_PGC, PUSH2_ROBUST_PGC = 0, 1, 4, 5
 # status codes (include/qpad_hip.h)
 OK, E_INVALID, E_HIP, E_NOMEM, E_STATE = 0, -1, -2, -3, -4
 
@@ -106,6 +108,9 @@ SIGNATURES = {
     "qpad_species_amjdp": (C.c_int, [_CTX, C.c_int32, C.c_double]),
     "qpad_species_push": (C.c_int, [_CTX, C.c_int32, C.c_double]),
     "qpad_species_interp_psi": (C.c_int, [_CTX, C.c_int32]),
+    "qpad_set_laser": (C.c_int, [_CTX, _dp, _dp]),
+    "qpad_laser_slice": (C.c_int, [_CTX, C.c_int32]),
+    "qpad_deposit_chi": (C.c_int, [_CTX, C.c_int32]),
     "qpad_step_iterations": (C.c_int, [_CTX, C.POINTER(C.c_int32), C.POINTER(C.c_int32)]),
     "qpad_field_solve_psi": (C.c_int, [_CTX]),
     "qpad_field_solve_ez": (C.c_int, [_CTX, C.c_int32, C.c_int32]),

@@ -181,6 +181,9 @@ struct QDepArgs {
     int nr;
     double idr;
     const double *x1, *x2, *q;
+    // susceptibility deposit (CHI): the particle's psi and q/m
+    const double *psi;
+    double qbm;
     const int32_t *cs;
     double *P0, *P1;
     // overflow chunks (listed by the sort) and their partial rows
@@ -251,7 +254,9 @@ struct QDepMulti {
     QDepArgs s[kMaxSpecies];
 };
 
-template <int M, int CPW>
+// CHI: deposit_chi_part2d (part2d_class.f03:361-476) instead: each particle
+// deposits -(q/m) q / (1 - (q/m) psi) in place of q
+template <int M, int CPW, bool CHI = false>
 __global__ __launch_bounds__(64 * CPW) void k_qdeposit(const QDepMulti m)
 {
     const QDepArgs a = m.s[__builtin_amdgcn_readfirstlane(blockIdx.y)];
@@ -273,12 +278,16 @@ __global__ __launch_bounds__(64 * CPW) void k_qdeposit(const QDepMulti m)
     int p = p0 + lane;
     int pc = min(p, pl);
     double nx1 = a.x1[pc], nx2 = a.x2[pc], nq = a.q[pc];
+    double npsi = CHI ? a.psi[pc] : 0.0;
     for (int k = 0; k < npass; k++, p += 64) {
-        const double x1 = nx1, x2 = nx2, q = p < p1 ? nq : 0.0;
+        const double x1 = nx1, x2 = nx2;
+        double q = p < p1 ? nq : 0.0;
+        if constexpr (CHI) q = p < p1 ? -1.0 * a.qbm * nq / (1.0 - a.qbm * npsi) : 0.0;
         pc = min(p + 64, pl);
         nx1 = a.x1[pc];
         nx2 = a.x2[pc];
         nq = a.q[pc];
+        if constexpr (CHI) npsi = a.psi[pc];
         double pos = __dmul_rn(radius(x1, x2), a.idr);
         double ph0r = x1 / pos * a.idr, ph0i = -x2 / pos * a.idr;
         double w1 = pos - (double)c, w0 = 1.0 - w1;
@@ -316,6 +325,8 @@ static QDepArgs qdep_args(qpad_ctx *c, int s)
     a.nr = c->g.nr;
     a.idr = 1.0 / c->g.dr;
     a.x1 = sp.x1; a.x2 = sp.x2; a.q = sp.q;
+    a.psi = sp.psi;
+    a.qbm = sp.qbm;
     a.cs = sp.cell_start;
     a.P0 = sp.P0;
     a.P1 = sp.P1;
@@ -331,12 +342,17 @@ int launch_qdeposit(qpad_ctx *c, int s) { return launch_qdeposit_range(c, s, s +
 
 int launch_qdeposit_all(qpad_ctx *c) { return c->nspecies ? launch_qdeposit_range(c, 0, c->nspecies) : 0; }
 
-// the charge deposit of the species in m.s[0 .. ns-1], gx workgroups per species
-static int qdeposit_launch(qpad_ctx *c, const QDepMulti &m, int ns, int gx)
+// the charge deposit of the species in m.s[0 .. ns-1], gx workgroups per
+// species (chi: the susceptibility deposit)
+static int qdeposit_launch(qpad_ctx *c, const QDepMulti &m, int ns, int gx, bool chi = false)
 {
     const int cpw = c->dep_cells;
     dim3 grid((unsigned)gx, (unsigned)ns), blk(64 * cpw);
-#define QPAD_QDEP(MM, CC) hipLaunchKernelGGL((k_qdeposit<MM, CC>), grid, blk, 0, c->stream, m)
+#define QPAD_QDEP(MM, CC)                                                                        \
+    do {                                                                                         \
+        if (chi) hipLaunchKernelGGL((k_qdeposit<MM, CC, true>), grid, blk, 0, c->stream, m);     \
+        else hipLaunchKernelGGL((k_qdeposit<MM, CC, false>), grid, blk, 0, c->stream, m);        \
+    } while (0)
 #define QPAD_QDEP_M(MM)                              \
     do {                                             \
         if (cpw == 2) QPAD_QDEP(MM, 2);              \
@@ -386,6 +402,21 @@ int launch_qdeposit_rows(qpad_ctx *c, int s, const DepRows &rows)
     return qdeposit_launch(c, m, 1, gx);
 }
 
+// the susceptibility deposit of species s into caller-given rows (as
+// launch_qdeposit_rows)
+int launch_chi_deposit_rows(qpad_ctx *c, int s, const DepRows &rows)
+{
+    QDepMulti m;
+    memset(&m, 0, sizeof(m));
+    m.s[0] = qdep_args(c, s);
+    m.s[0].P0 = rows.P0;
+    m.s[0].P1 = rows.P1;
+    m.s[0].O0 = rows.O0;
+    m.s[0].O1 = rows.O1;
+    const int gx = std::max(1, dep_grid(c->g.nr, c->sp[s].npp, c->sp[s].n_ov_max, c->dep_cells));
+    return qdeposit_launch(c, m, 1, gx, true);
+}
+
 // one species' q (axis rules, 1/r) times `scale` into out (1 x nm x ld)
 __global__ void k_combine_q(const double *P0, const double *P1, const double *O0, const double *O1,
                             const int32_t *os, double *out, int nr, int nm, int ld, double scale, int accumulate)
@@ -422,12 +453,13 @@ int launch_combine_q(qpad_ctx *c, int s, double *out, double scale, bool accumul
 // ---------------------------------------------------------------------------
 // gather of a 3-component field at nodes ix, ix+1 (interp_part2d.f03:66-110)
 // ---------------------------------------------------------------------------
-// f(cc, k, n): component cc of mode array k at node ix + n (n = 0, 1)
-template <int M, class F>
-__device__ __forceinline__ void gather3(const F &f, double w0, double w1, double pc, double ps, double out[3])
+// f(cc, k, n): component cc of mode array k at node ix + n (n = 0, 1); NC
+// components (1: interp_field_part2d_scalar, :110-153)
+template <int M, int NC, class F>
+__device__ __forceinline__ void gather_n(const F &f, double w0, double w1, double pc, double ps, double out[NC])
 {
 #pragma unroll
-    for (int cc = 0; cc < 3; cc++) {
+    for (int cc = 0; cc < NC; cc++) {
         double v = 0.0;
         v = v + f(cc, 0, 0) * w0;
         v = v + f(cc, 0, 1) * w1;
@@ -441,11 +473,17 @@ __device__ __forceinline__ void gather3(const F &f, double w0, double w1, double
         phi = ti;
         const double ph_re = 2.0 * phr, ph_im = 2.0 * phi;
 #pragma unroll
-        for (int cc = 0; cc < 3; cc++) {
+        for (int cc = 0; cc < NC; cc++) {
             out[cc] = out[cc] + (f(cc, 2 * m - 1, 0) * ph_re - f(cc, 2 * m, 0) * ph_im) * w0;
             out[cc] = out[cc] + (f(cc, 2 * m - 1, 1) * ph_re - f(cc, 2 * m, 1) * ph_im) * w1;
         }
     }
+}
+
+template <int M, class F>
+__device__ __forceinline__ void gather3(const F &f, double w0, double w1, double pc, double ps, double out[3])
+{
+    gather_n<M, 3>(f, w0, w1, pc, ps, out);
 }
 
 // nodes ix, ix + 1 of a slice field in HBM
@@ -458,12 +496,32 @@ struct NodePair {
     }
 };
 
-// nodes ix, ix + 1 of a slice field staged in LDS as [n][cc][k]
-template <int NM>
+// nodes ix, ix + 1 of a slice field staged in LDS as [n][cc][k] (NC
+// components)
+template <int NM, int NC = 3>
 struct NodePairLds {
     const double *s;
-    __device__ __forceinline__ double operator()(int cc, int k, int n) const { return s[(n * 3 + cc) * NM + k]; }
+    __device__ __forceinline__ double operator()(int cc, int k, int n) const { return s[(n * NC + cc) * NM + k]; }
 };
+
+// the laser envelope (ar, ai) and its gradients (ar_grad, ai_grad: r, phi,
+// z) at a particle, from nodes f(field, cc, k, n) (interp_field calls of
+// part2d_class.f03:1100-1105, :1406-1409, :1996-2001)
+struct LaserAt {
+    double apr, api, aprg[3], apig[3];
+};
+
+template <int M, class FR, class FI, class GR, class GI>
+__device__ __forceinline__ LaserAt gather_laser(const FR &far, const FI &fai, const GR &gar, const GI &gai, double w0,
+                                               double w1, double pc, double ps)
+{
+    LaserAt l;
+    gather_n<M, 1>(far, w0, w1, pc, ps, &l.apr);
+    gather_n<M, 1>(fai, w0, w1, pc, ps, &l.api);
+    gather_n<M, 3>(gar, w0, w1, pc, ps, l.aprg);
+    gather_n<M, 3>(gai, w0, w1, pc, ps, l.apig);
+    return l;
+}
 
 // ---------------------------------------------------------------------------
 // amjdeposit_robust_part2d (part2d_class.f03:746-1010): gather E, B; robust
@@ -478,6 +536,8 @@ struct JDepArgs {
     int store_diag;
     const int32_t *cs;
     const double *e, *b;
+    // the laser envelope and its gradients of the slice (PGC pushers)
+    const double *ar, *ai, *arg, *aig;
     double *P0, *P1;
     IterState *it;
     // SPEC launches: convergence of the previous iteration from the field
@@ -512,27 +572,82 @@ struct DepTerms {
     }
 };
 
+// the values a deposit workgroup stages in LDS per wave: E, B ([n][cc][k]),
+// then for the PGC pushers ar, ai ([n][k]) and ar_grad, ai_grad ([n][cc][k])
+template <int NM, bool PGC>
+constexpr int dep_staged() { return PGC ? 28 * NM : 12 * NM; }
+
+// staged value l of the PGC layout of the cell whose left node is ix
+template <int NM>
+__device__ __forceinline__ double dep_stage_load_pgc(const double *e, const double *b, const double *ar,
+                                                     const double *ai, const double *arg, const double *aig, int nm,
+                                                     int ld, int l, int ix)
+{
+    const double *f;
+    int nc, base;
+    if (l < 12 * NM) {
+        f = l < 6 * NM ? e : b;
+        nc = 3;
+        base = l < 6 * NM ? 0 : 6 * NM;
+    } else if (l < 16 * NM) {
+        f = l < 14 * NM ? ar : ai;
+        nc = 1;
+        base = l < 14 * NM ? 12 * NM : 14 * NM;
+    } else {
+        f = l < 22 * NM ? arg : aig;
+        nc = 3;
+        base = l < 22 * NM ? 16 * NM : 22 * NM;
+    }
+    const int r = l - base, n = r / (nc * NM), comp = (r / NM) % nc, k = r % NM;
+    return f[((size_t)comp * nm + k) * ld + ix + n];
+}
+
 // gather and predictor of particle p of cell c; lanes past the chunk's end
 // (valid false) deposit zero and store nothing.  STD: the standard pusher
 // (amjdeposit_std_part2d, part2d_class.f03:478-744), which normalises by the
 // particle's interpolated psi (psip) where the robust one uses gamma - u_z,
-// and always stores the time-centred gamma.
-template <int M, bool STD>
-__device__ __forceinline__ void particle_terms(const JDepArgs &a, const NodePairLds<2 * M + 1> &fe,
-                                               const NodePairLds<2 * M + 1> &fb, int c, int p, bool valid,
+// and always stores the time-centred gamma.  PGC: the laser pushers
+// (amjdeposit_std_pgc_part2d :1012-1308, amjdeposit_robust_pgc_part2d
+// :1310-1602): every gamma carries gam_corr = (q/m)^2 |a|^2 / 2, the
+// electric field the ponderomotive term -+ (q/m)/(2 gamma) a grad a
+// (r, phi minus, z plus, with the initial gamma), and the robust form
+// normalises its second half kick by gamma / (gamma - u_z) of the rotated
+// momentum.  sf: the wave's staged values (dep_staged layout).
+template <int M, bool STD, bool PGC>
+__device__ __forceinline__ void particle_terms(const JDepArgs &a, const double *sf, int c, int p, bool valid,
                                                double x1, double x2, double pp1, double pp2, double pp3,
                                                double qp, double psip, DepTerms<M> &t)
 {
+    constexpr int NM = 2 * M + 1;
     double r = radius(x1, x2);
     const double pc = x1 / r, ps = x2 / r;
     double pos = __dmul_rn(r, a.idr);
     const double w1 = pos - (double)c, w0 = 1.0 - w1;
     double ep[3], bp[3];
-    gather3<M>(fe, w0, w1, pc, ps, ep);
-    gather3<M>(fb, w0, w1, pc, ps, bp);
+    gather3<M>(NodePairLds<NM>{sf}, w0, w1, pc, ps, ep);
+    gather3<M>(NodePairLds<NM>{sf + 6 * NM}, w0, w1, pc, ps, bp);
+    double gam_corr = 0.0;
+    LaserAt la;
+    if constexpr (PGC) {
+        la = gather_laser<M>(NodePairLds<NM, 1>{sf + 12 * NM}, NodePairLds<NM, 1>{sf + 14 * NM},
+                             NodePairLds<NM>{sf + 16 * NM}, NodePairLds<NM>{sf + 22 * NM}, w0, w1, pc, ps);
+        gam_corr = 0.5 * a.qbm * a.qbm * (la.apr * la.apr + la.api * la.api);
+    }
+    // sqrt(1 + v^2) (+ gam_corr for the laser pushers)
+    auto gamma_of = [&](const double *v) {
+        double g2 = 1.0 + v[0] * v[0] + v[1] * v[1] + v[2] * v[2];
+        if constexpr (PGC) g2 = g2 + gam_corr;
+        return sqrt(g2);
+    };
     double wp0 = ep[0] - bp[1], wp1 = ep[1] + bp[0], wp2 = ep[2];
     double u0[3] = {pp1 * pc + pp2 * ps, pp2 * pc - pp1 * ps, pp3};
-    double gam = sqrt(1.0 + u0[0] * u0[0] + u0[1] * u0[1] + u0[2] * u0[2]);
+    double gam = gamma_of(u0);
+    if constexpr (PGC) {
+        const double tmp = 0.5 * a.qbm / gam;
+        ep[0] = ep[0] - tmp * (la.apr * la.aprg[0] + la.api * la.apig[0]);
+        ep[1] = ep[1] - tmp * (la.apr * la.aprg[1] + la.api * la.apig[1]);
+        ep[2] = ep[2] + tmp * (la.apr * la.aprg[2] + la.api * la.apig[2]);
+    }
     double ut[3], u[3];
     double qtmh2;
     if constexpr (STD) {
@@ -540,6 +655,12 @@ __device__ __forceinline__ void particle_terms(const JDepArgs &a, const NodePair
         const double qtmh1 = qtmh2 * gam;
 #pragma unroll
         for (int cc = 0; cc < 3; cc++) ut[cc] = u0[cc] + ep[cc] * qtmh1;
+    } else if constexpr (PGC) {
+        const double qtmh1 = a.qtmh * gam / (gam - u0[2]);
+#pragma unroll
+        for (int cc = 0; cc < 3; cc++) ut[cc] = u0[cc] + ep[cc] * qtmh1;
+        gam = gamma_of(ut);
+        qtmh2 = a.qtmh / (gam - ut[2]);
     } else {
         const double qtmh1 = a.qtmh * gam / (gam - u0[2]);
 #pragma unroll
@@ -560,7 +681,12 @@ __device__ __forceinline__ void particle_terms(const JDepArgs &a, const NodePair
     ut[2] = ut[2] + u[0] * bp[1] - u[1] * bp[0];
     if constexpr (STD) {
         // second half kick normalised by the gamma after the rotation
-        const double qtmh1 = qtmh2 * sqrt(1.0 + ut[0] * ut[0] + ut[1] * ut[1] + ut[2] * ut[2]);
+        const double qtmh1 = qtmh2 * gamma_of(ut);
+#pragma unroll
+        for (int cc = 0; cc < 3; cc++) u[cc] = ut[cc] + ep[cc] * qtmh1;
+    } else if constexpr (PGC) {
+        const double g1 = gamma_of(ut);
+        const double qtmh1 = a.qtmh * g1 / (g1 - ut[2]);
 #pragma unroll
         for (int cc = 0; cc < 3; cc++) u[cc] = ut[cc] + ep[cc] * qtmh1;
     } else {
@@ -570,7 +696,7 @@ __device__ __forceinline__ void particle_terms(const JDepArgs &a, const NodePair
     double du0 = a.idt * (u[0] - u0[0]), du1 = a.idt * (u[1] - u0[1]);
 #pragma unroll
     for (int cc = 0; cc < 3; cc++) u[cc] = 0.5 * (u[cc] + u0[cc]);
-    double g2 = sqrt(1.0 + u[0] * u[0] + u[1] * u[1] + u[2] * u[2]);
+    double g2 = gamma_of(u);
     double ipsi;
     if constexpr (STD) {
         ipsi = 1.0 / (1.0 - a.qbm * psip);
@@ -632,15 +758,15 @@ struct JDepMulti {
 // SPEC: a speculative launch of iteration >= 2 of a slice, which returns at
 // once when the iteration has already converged (a separate instance, so
 // that profiles show the first-iteration launches, which always work, apart)
-template <int M, bool SPEC, int CPW, bool STD>
+template <int M, bool SPEC, int CPW, bool STD, bool PGC = false>
 __global__ __launch_bounds__(64 * CPW) void k_amjdeposit(const JDepMulti m)
 {
     const JDepArgs a = m.s[__builtin_amdgcn_readfirstlane(blockIdx.y)];
     constexpr int NM = 2 * M + 1;
     constexpr int NV = 8 * NM;
-    constexpr int NF = 12 * NM;   // staged field values per cell
-    // E and B at the two nodes of each wave's cell: [wave][E|B][n][cc][k]
-    __shared__ double sf[CPW][2][2 * 3 * NM];
+    constexpr int NF = dep_staged<NM, PGC>();   // staged field values per cell
+    // the fields at the two nodes of each wave's cell (dep_staged layout)
+    __shared__ double sf[CPW][NF];
     const int lane = threadIdx.x & 63;
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     if (dep_group_idle<CPW>(a.nr, a.os)) return;
@@ -663,10 +789,14 @@ __global__ __launch_bounds__(64 * CPW) void k_amjdeposit(const JDepMulti m)
 #pragma unroll
     for (int u = 0; u < (NF + 63) / 64; u++) {
         const int l = min(lane + 64 * u, NF - 1);
-        const int fb = l / (6 * NM), r = l % (6 * NM);
-        const int n = r / (3 * NM), comp = (r / NM) % 3, k = r % NM;
-        const double *f = fb ? a.b : a.e;
-        fv[u] = f[((size_t)comp * nm + k) * ld + ix + n];
+        if constexpr (PGC) {
+            fv[u] = dep_stage_load_pgc<NM>(a.e, a.b, a.ar, a.ai, a.arg, a.aig, nm, ld, l, ix);
+        } else {
+            const int fb = l / (6 * NM), r = l % (6 * NM);
+            const int n = r / (3 * NM), comp = (r / NM) % 3, k = r % NM;
+            const double *f = fb ? a.b : a.e;
+            fv[u] = f[((size_t)comp * nm + k) * ld + ix + n];
+        }
     }
     if (SPEC) {
         int stop = done;
@@ -688,11 +818,10 @@ __global__ __launch_bounds__(64 * CPW) void k_amjdeposit(const JDepMulti m)
 #pragma unroll
     for (int u = 0; u < (NF + 63) / 64; u++) {
         const int l = lane + 64 * u;
-        if (l < NF) sf[wave][l / (6 * NM)][l % (6 * NM)] = fv[u];
+        if (l < NF) sf[wave][l] = fv[u];
     }
     __syncthreads();
     if (!ch.live) return;
-    const NodePairLds<NM> fe{sf[wave][0]}, fbf{sf[wave][1]};
     const double *__restrict__ X1 = a.x1;
     const double *__restrict__ X2 = a.x2;
     const double *__restrict__ PP1 = a.p1;
@@ -721,7 +850,7 @@ __global__ __launch_bounds__(64 * CPW) void k_amjdeposit(const JDepMulti m)
         const double nx1 = X1[pn], nx2 = X2[pn], np1 = PP1[pn], np2 = PP2[pn], np3 = PP3[pn], nq = Q[pn];
         const double npsi = STD ? a.psi[pn] : 0.0;
         DepTerms<M> t;
-        particle_terms<M, STD>(a, fe, fbf, c, p, p < p1, x1, x2, pp1, pp2, pp3, qp, psip, t);
+        particle_terms<M, STD, PGC>(a, sf[wave], c, p, p < p1, x1, x2, pp1, pp2, pp3, qp, psip, t);
 #pragma unroll
         for (int i = 0; i < NA; i++) acc[i] += reduced_value<NM, S>(t, i, lane);
         x1 = nx1; x2 = nx2; pp1 = np1; pp2 = np2; pp3 = np3; qp = nq; psip = npsi;
@@ -761,6 +890,10 @@ static JDepArgs jdep_args(qpad_ctx *c, int s, int l, bool store_diag, bool repor
     a.cs = sp.cell_start;
     a.e = c->f[QPAD_F_E];
     a.b = c->f[QPAD_F_B];
+    a.ar = c->f[QPAD_F_AR];
+    a.ai = c->f[QPAD_F_AI];
+    a.arg = c->f[QPAD_F_AR_GRAD];
+    a.aig = c->f[QPAD_F_AI_GRAD];
     a.P0 = sp.P0;
     a.P1 = sp.P1;
     a.it = c->it;
@@ -778,8 +911,8 @@ static JDepArgs jdep_args(qpad_ctx *c, int s, int l, bool store_diag, bool repor
 }
 
 // the species of species-list entries in one launch (grid rows by list
-// entry), all of push type STD
-template <bool STD>
+// entry), all of the push type STD, PGC names
+template <bool STD, bool PGC>
 static int launch_jdep_list(qpad_ctx *c, const int *list, int n, int l, bool store_diag, bool timed, bool report)
 {
     const bool speculative = l >= 2;
@@ -798,8 +931,9 @@ static int launch_jdep_list(qpad_ctx *c, const int *list, int n, int l, bool sto
     if (timed) time_begin(c, 0, &stop);
 #define QPAD_DEP2(MM, CC)                                                                                   \
     do {                                                                                                    \
-        if (speculative) hipLaunchKernelGGL((k_amjdeposit<MM, true, CC, STD>), grid, blk, 0, c->stream, m); \
-        else hipLaunchKernelGGL((k_amjdeposit<MM, false, CC, STD>), grid, blk, 0, c->stream, m);            \
+        if (speculative)                                                                                    \
+            hipLaunchKernelGGL((k_amjdeposit<MM, true, CC, STD, PGC>), grid, blk, 0, c->stream, m);          \
+        else hipLaunchKernelGGL((k_amjdeposit<MM, false, CC, STD, PGC>), grid, blk, 0, c->stream, m);        \
     } while (0)
 #define QPAD_DEP(MM)                                 \
     do {                                             \
@@ -821,18 +955,38 @@ static int launch_jdep_list(qpad_ctx *c, const int *list, int n, int l, bool sto
     return 0;
 }
 
-// species s0 .. s1-1: one launch per push type present (robust first, which
-// carries the host report when there is one)
+// the four pushers in launch order: robust, standard, robust_pgc,
+// standard_pgc
+static int push_group(int push_type)
+{
+    switch (push_type) {
+    case QPAD_PUSH2_STD: return 1;
+    case QPAD_PUSH2_ROBUST_PGC: return 2;
+    case QPAD_PUSH2_STD_PGC: return 3;
+    default: return 0;
+    }
+}
+
+// species s0 .. s1-1: one launch per push type present (the first launch
+// carries the timing and the host report when there is one)
 static int launch_jdep(qpad_ctx *c, int s0, int s1, int l, bool store_diag, bool timed, bool report)
 {
-    int rob[kMaxSpecies], std_[kMaxSpecies], nr_ = 0, ns_ = 0;
+    int list[4][kMaxSpecies], n[4] = {0, 0, 0, 0};
     for (int s = s0; s < s1; s++) {
-        if (c->sp[s].cfg.push_type == QPAD_PUSH2_STD) std_[ns_++] = s;
-        else rob[nr_++] = s;
+        const int g = push_group(c->sp[s].cfg.push_type);
+        list[g][n[g]++] = s;
     }
+    bool first = true;
     int rc = 0;
-    if (nr_) rc = launch_jdep_list<false>(c, rob, nr_, l, store_diag, timed, report);
-    if (!rc && ns_) rc = launch_jdep_list<true>(c, std_, ns_, l, store_diag, timed && !nr_, report && !nr_);
+    for (int g = 0; g < 4 && !rc; g++) {
+        if (!n[g]) continue;
+        const bool tm = timed && first, rp = report && first;
+        if (g == 0) rc = launch_jdep_list<false, false>(c, list[g], n[g], l, store_diag, tm, rp);
+        else if (g == 1) rc = launch_jdep_list<true, false>(c, list[g], n[g], l, store_diag, tm, rp);
+        else if (g == 2) rc = launch_jdep_list<false, true>(c, list[g], n[g], l, store_diag, tm, rp);
+        else rc = launch_jdep_list<true, true>(c, list[g], n[g], l, store_diag, tm, rp);
+        first = false;
+    }
     return rc;
 }
 
@@ -861,6 +1015,7 @@ struct PushArgs {
     const int32_t *cs;
     uint32_t *key;
     const double *e, *b;
+    const double *ar, *ai, *arg, *aig;   // laser envelope and gradients (PGC)
 };
 
 // the arguments of every species of one launch (workgroup rows by species)
@@ -870,8 +1025,12 @@ struct PushMulti {
 
 // STD: push_u_std_part2d (part2d_class.f03:1790-1877), normalised by the
 // interpolated psi and the gamma the last current deposit stored, where
-// push_u_robust_part2d (:1879-1965) uses the particle's own gamma - p_z
-template <int M, bool STD>
+// push_u_robust_part2d (:1879-1965) uses the particle's own gamma - p_z.
+// PGC: push_u_robust_pgc_part2d / push_u_std_pgc_part2d (:1967-2219, the
+// same steps): the standard normalisation, the ponderomotive term in E with
+// the stored gamma, and the new gamma with gam_corr extrapolated half a step
+// in xi (the envelope's xi gradient)
+template <int M, bool STD, bool PGC = false>
 __global__ __launch_bounds__(256) void k_push(const PushMulti m)
 {
     const PushArgs a = m.s[__builtin_amdgcn_readfirstlane(blockIdx.y)];
@@ -897,9 +1056,30 @@ __global__ __launch_bounds__(256) void k_push(const PushMulti m)
     t = bp[0] * pc - bp[1] * ps;
     bp[1] = bp[0] * ps + bp[1] * pc;
     bp[0] = t;
+    double gam_corr = 0.0, apr_z = 0.0, api_z = 0.0, apr = 0.0, api = 0.0;
+    if constexpr (PGC) {
+        LaserAt la = gather_laser<M>(NodePair{a.ar, nm, ld, ix}, NodePair{a.ai, nm, ld, ix},
+                                     NodePair{a.arg, nm, ld, ix}, NodePair{a.aig, nm, ld, ix}, w0, w1, pc, ps);
+        t = la.aprg[0] * pc - la.aprg[1] * ps;
+        la.aprg[1] = la.aprg[0] * ps + la.aprg[1] * pc;
+        la.aprg[0] = t;
+        t = la.apig[0] * pc - la.apig[1] * ps;
+        la.apig[1] = la.apig[0] * ps + la.apig[1] * pc;
+        la.apig[0] = t;
+        const double qbm2_hf = a.qbm * a.qbm * 0.5;
+        gam_corr = qbm2_hf * (la.apr * la.apr + la.api * la.api);
+        const double tmp = 0.5 * a.qbm / a.gamma[p];
+        ep[0] = ep[0] - tmp * (la.apr * la.aprg[0] + la.api * la.apig[0]);
+        ep[1] = ep[1] - tmp * (la.apr * la.aprg[1] + la.api * la.apig[1]);
+        ep[2] = ep[2] + tmp * (la.apr * la.aprg[2] + la.api * la.apig[2]);
+        apr = la.apr;
+        api = la.api;
+        apr_z = la.aprg[2];
+        api_z = la.apig[2];
+    }
     double pp[3] = {a.p1[p], a.p2[p], a.p3[p]}, ut[3];
     double qtmh1, qtmh2;
-    if constexpr (STD) {
+    if constexpr (STD || PGC) {
         qtmh1 = a.qtmh / (1.0 - a.qbm * a.psi[p]);
         qtmh2 = qtmh1 * a.gamma[p];
     } else {
@@ -924,7 +1104,17 @@ __global__ __launch_bounds__(256) void k_push(const PushMulti m)
     ut[2] = ut[2] + pp[0] * bp[1] - pp[1] * bp[0];
 #pragma unroll
     for (int cc = 0; cc < 3; cc++) pp[cc] = ut[cc] + ep[cc];
-    double g2 = sqrt(1.0 + pp[0] * pp[0] + pp[1] * pp[1] + pp[2] * pp[2]);
+    double g2;
+    if constexpr (PGC) {
+        const double qbm2_hf = a.qbm * a.qbm * 0.5;
+        double tmp = apr_z * a.dt;
+        gam_corr = gam_corr + qbm2_hf * (apr + 0.25 * tmp) * tmp;
+        tmp = api_z * a.dt;
+        gam_corr = gam_corr + qbm2_hf * (api + 0.25 * tmp) * tmp;
+        g2 = sqrt(1.0 + pp[0] * pp[0] + pp[1] * pp[1] + pp[2] * pp[2] + gam_corr);
+    } else {
+        g2 = sqrt(1.0 + pp[0] * pp[0] + pp[1] * pp[1] + pp[2] * pp[2]);
+    }
     double dtc = a.dt / (g2 - pp[2]);
     x1 = x1 + pp[0] * dtc;
     x2 = x2 + pp[1] * dtc;
@@ -956,12 +1146,16 @@ static PushArgs push_args(qpad_ctx *c, int s)
     a.key = sp.key;
     a.e = c->f[QPAD_F_E];
     a.b = c->f[QPAD_F_B];
+    a.ar = c->f[QPAD_F_AR];
+    a.ai = c->f[QPAD_F_AI];
+    a.arg = c->f[QPAD_F_AR_GRAD];
+    a.aig = c->f[QPAD_F_AI_GRAD];
     return a;
 }
 
 // the species of list entries in one launch (species without particles get
-// idle rows), all of push type STD
-template <bool STD>
+// idle rows), all of the push type STD, PGC names
+template <bool STD, bool PGC>
 static int launch_push_list(qpad_ctx *c, const int *list, int n)
 {
     PushMulti m;
@@ -976,28 +1170,30 @@ static int launch_push_list(qpad_ctx *c, const int *list, int n)
     hipEvent_t stop = nullptr;
     time_begin(c, 1, &stop);
     switch (c->g.M) {
-    case 0: hipLaunchKernelGGL((k_push<0, STD>), grid, blk, 0, c->stream, m); break;
-    case 1: hipLaunchKernelGGL((k_push<1, STD>), grid, blk, 0, c->stream, m); break;
-    case 2: hipLaunchKernelGGL((k_push<2, STD>), grid, blk, 0, c->stream, m); break;
-    case 3: hipLaunchKernelGGL((k_push<3, STD>), grid, blk, 0, c->stream, m); break;
-    default: hipLaunchKernelGGL((k_push<4, STD>), grid, blk, 0, c->stream, m); break;
+    case 0: hipLaunchKernelGGL((k_push<0, STD, PGC>), grid, blk, 0, c->stream, m); break;
+    case 1: hipLaunchKernelGGL((k_push<1, STD, PGC>), grid, blk, 0, c->stream, m); break;
+    case 2: hipLaunchKernelGGL((k_push<2, STD, PGC>), grid, blk, 0, c->stream, m); break;
+    case 3: hipLaunchKernelGGL((k_push<3, STD, PGC>), grid, blk, 0, c->stream, m); break;
+    default: hipLaunchKernelGGL((k_push<4, STD, PGC>), grid, blk, 0, c->stream, m); break;
     }
     QPAD_HIP_CHECK(hipGetLastError());
     time_end(c, 1, stop);
     return 0;
 }
 
-// species s0 .. s1-1: one launch per push type present
+// species s0 .. s1-1: one launch per push present (the two PGC pushers
+// push alike)
 static int launch_push_range(qpad_ctx *c, int s0, int s1)
 {
-    int rob[kMaxSpecies], std_[kMaxSpecies], nr_ = 0, ns_ = 0;
+    int list[3][kMaxSpecies], n[3] = {0, 0, 0};
     for (int s = s0; s < s1; s++) {
-        if (c->sp[s].cfg.push_type == QPAD_PUSH2_STD) std_[ns_++] = s;
-        else rob[nr_++] = s;
+        const int g = std::min(push_group(c->sp[s].cfg.push_type), 2);
+        list[g][n[g]++] = s;
     }
     int rc = 0;
-    if (nr_) rc = launch_push_list<false>(c, rob, nr_);
-    if (!rc && ns_) rc = launch_push_list<true>(c, std_, ns_);
+    if (n[0]) rc = launch_push_list<false, false>(c, list[0], n[0]);
+    if (!rc && n[1]) rc = launch_push_list<true, false>(c, list[1], n[1]);
+    if (!rc && n[2]) rc = launch_push_list<false, true>(c, list[2], n[2]);
     return rc;
 }
 
@@ -1054,7 +1250,8 @@ __global__ __launch_bounds__(256) void k_interp_psi(const InterpPsiArgs a)
 int launch_interp_psi(qpad_ctx *c, int s)
 {
     Species &sp = c->sp[s];
-    if (sp.cfg.push_type != QPAD_PUSH2_STD || sp.npp == 0) return 0;
+    const int pt = sp.cfg.push_type;
+    if ((pt != QPAD_PUSH2_STD && pt != QPAD_PUSH2_STD_PGC) || sp.npp == 0) return 0;
     InterpPsiArgs a;
     a.nr = c->g.nr;
     a.ld = c->g.ld;

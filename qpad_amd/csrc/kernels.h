@@ -88,6 +88,8 @@ struct DepRows {
 };
 int launch_qdeposit_rows(qpad_ctx *c, int sp, const DepRows &rows);
 int launch_combine_q_rows(qpad_ctx *c, int sp, const DepRows &rows, double *out, double scale, bool accumulate);
+// the susceptibility deposit (deposit_chi) of species sp into rows
+int launch_chi_deposit_rows(qpad_ctx *c, int sp, const DepRows &rows);
 // l: predictor-corrector iteration of the slice (1-based; l >= 2 is a
 // speculative launch that first evaluates the convergence of iteration l-1
 // and returns at once when it has converged; l = 0 for a plain deposit);
@@ -110,6 +112,13 @@ int launch_add_qn_into_qspe(qpad_ctx *c, int sp);
 // history slice j
 int launch_species_q_start(qpad_ctx *c, int sp);
 int launch_species_cbq(qpad_ctx *c, int sp, int j);
+
+// ---- laser (k_laser.hip) ----
+// the envelope and its gradients of slice j from the envelope history
+int launch_laser_slice(qpad_ctx *c, int j);
+// chi of every species (with the running rescaling) into the chi field and
+// history slice j
+int launch_deposit_chi(qpad_ctx *c, int j);
 
 // ---- beam (k_beam.hip) ----
 int launch_beam_inject(qpad_ctx *c, int ib);

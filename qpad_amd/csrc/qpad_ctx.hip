@@ -236,6 +236,7 @@ static void species_free(Species &sp)
     dfree(sp.ov_start); dfree(sp.ov_desc); dfree(sp.O0); dfree(sp.O1);
     dfree(sp.hist); dfree(sp.tot);
     dfree(sp.qsl); dfree(sp.h_q);
+    dfree(sp.X0); dfree(sp.X1); dfree(sp.XO0); dfree(sp.XO1);
 }
 
 // the per-species charge arrays of species s (zeroed)
@@ -248,6 +249,47 @@ static int species_q_alloc(qpad_ctx *c, int s)
     TRY(dalloc(&sp.h_q, nh));
     TRY(zero_dev(c, sp.qsl, n1));
     return zero_dev(c, sp.h_q, nh);
+}
+
+static bool is_pgc(int push_type) { return push_type == QPAD_PUSH2_STD_PGC || push_type == QPAD_PUSH2_ROBUST_PGC; }
+
+// the susceptibility deposit rows of species s (every row a consumer reads
+// is written by the deposit, so they need no clearing)
+static int species_chi_alloc(qpad_ctx *c, int s)
+{
+    Species &sp = c->sp[s];
+    if (sp.X0) return 0;
+    const size_t nm = (size_t)c->g.nm;
+    TRY(dalloc(&sp.X0, (size_t)c->g.nr * nm));
+    TRY(dalloc(&sp.X1, (size_t)c->g.nr * nm));
+    TRY(dalloc(&sp.XO0, (size_t)sp.n_ov_max * nm));
+    return dalloc(&sp.XO1, (size_t)sp.n_ov_max * nm);
+}
+
+// the laser state (a zero envelope history, the chi history, every
+// species' chi rows); on failure the context is left without it
+static int laser_ensure(qpad_ctx *c)
+{
+    if (c->laser) return 0;
+    const size_t n1 = f1_size(c, 1);
+    const size_t nh = ((size_t)c->g.nxi + 3) * n1, nc = ((size_t)c->g.nxi + 2) * n1;
+    int rc = dalloc(&c->lz_ar, nh);
+    if (!rc) rc = dalloc(&c->lz_ai, nh);
+    if (!rc) rc = dalloc(&c->h_chi, nc);
+    for (int s = 0; s < c->nspecies && !rc; s++) rc = species_chi_alloc(c, s);
+    if (!rc) rc = zero_dev(c, c->lz_ar, nh);
+    if (!rc) rc = zero_dev(c, c->lz_ai, nh);
+    if (!rc) rc = zero_dev(c, c->h_chi, nc);
+    if (!rc) rc = sync(c);
+    if (rc) {
+        const std::string msg = g_err;
+        (void)hipStreamSynchronize(c->stream);
+        dfree(c->lz_ar); dfree(c->lz_ai); dfree(c->h_chi);
+        g_err = msg;
+        return rc;
+    }
+    c->laser = true;
+    return 0;
 }
 
 static void beam_free(Beam &bm)
@@ -319,7 +361,7 @@ int qpad_create(const qpad_grid *gr, qpad_ctx **out)
         delete c;
         return fail(QPAD_E_HIP, std::string("hipStreamCreate: ") + hipGetErrorString(he));
     }
-    static const int ncomp[QPAD_NFIELD] = {1, 3, 3, 3, 3, 3, 3, 2, 3, 2, 1, 1};
+    static const int ncomp[QPAD_NFIELD] = {1, 3, 3, 3, 3, 3, 3, 2, 3, 2, 1, 1, 1, 1, 3, 3, 1};
     for (int f = 0; f < QPAD_NFIELD && !rc; f++) {
         c->ncomp[f] = ncomp[f];
         rc = dalloc(&c->f[f], f1_size(c, ncomp[f]));
@@ -417,6 +459,7 @@ int qpad_destroy(qpad_ctx *c)
     dfree(c->h_psi); dfree(c->h_e); dfree(c->h_b); dfree(c->h_b_spe); dfree(c->h_e_spe);
     dfree(c->h_cu); dfree(c->h_q_spe); dfree(c->h_q_beam); dfree(c->h_b_beam);
     dfree(c->conv_re); dfree(c->conv_im); dfree(c->conv_blk); dfree(c->it); dfree(c->fac); dfree(c->src);
+    dfree(c->lz_ar); dfree(c->lz_ai); dfree(c->h_chi);
     if (c->it_host) (void)hipHostFree(c->it_host);
     if (c->it_report) (void)hipHostFree(c->it_report);
     if (c->dec_ev) (void)hipEventDestroy(c->dec_ev);
@@ -440,8 +483,9 @@ int qpad_species_new(qpad_ctx *c, const qpad_species_cfg *cfg, int32_t *sp_id)
     if (c->nspecies >= kMaxSpecies) return fail(QPAD_E_INVALID, "too many species");
     if (cfg->ppc1 < 1 || cfg->ppc2 < 1 || cfg->num_theta < 1)
         return fail(QPAD_E_INVALID, "ppc and num_theta must be >= 1");
-    if (cfg->push_type != QPAD_PUSH2_ROBUST && cfg->push_type != QPAD_PUSH2_STD)
-        return fail(QPAD_E_INVALID, "push_type must be QPAD_PUSH2_STD or QPAD_PUSH2_ROBUST");
+    if (cfg->push_type != QPAD_PUSH2_ROBUST && cfg->push_type != QPAD_PUSH2_STD && !is_pgc(cfg->push_type))
+        return fail(QPAD_E_INVALID, "push_type must be QPAD_PUSH2_STD, QPAD_PUSH2_ROBUST, QPAD_PUSH2_STD_PGC or "
+                                    "QPAD_PUSH2_ROBUST_PGC");
     if (cfg->mass == 0.0 || cfg->qm == 0.0) return fail(QPAD_E_INVALID, "q and m must be non-zero");
     // npmax: fdist2d_class.f03:258-262 (xtra = 2)
     const int64_t nlat = (int64_t)c->g.nr * cfg->ppc1 * cfg->ppc2 * cfg->num_theta;
@@ -480,9 +524,13 @@ int qpad_species_new(qpad_ctx *c, const qpad_species_cfg *cfg, int32_t *sp_id)
     if (!rc) rc = dalloc(&sp.tot, (size_t)c->g.nr + 1);
     if (!rc) rc = zero_dev(c, sp.qn, f1_size(c, 1));
     if (!rc && c->species_q) rc = species_q_alloc(c, s);
+    if (!rc && (c->laser || is_pgc(cfg->push_type))) rc = species_chi_alloc(c, s);
     if (!rc) rc = launch_inject(c, s);
     if (!rc) rc = species_charge_setup(c, s);
     if (!rc) rc = sync(c);
+    // a PGC species pushes in the laser envelope: a zero one until
+    // qpad_set_laser (sim_lasers exists with nlasers = 0)
+    if (!rc && is_pgc(cfg->push_type)) rc = laser_ensure(c);
     if (rc) {
         // keep the message of the first failure; wait for queued work on the
         // buffers before they are freed
@@ -780,6 +828,8 @@ static int slice_start(qpad_ctx *c, int j, bool overlap, SliceRun &st)
     TRY(launch_qdeposit_all(c));
     if (c->species_q)
         for (int s = 0; s < c->nspecies; s++) TRY(launch_species_q_start(c, s));
+    // laser_all of slice j (simulation_class.f03:362-367)
+    if (c->laser) TRY(launch_laser_slice(c, j));
     TRY(launch_slice_sources(c));
     TRY(launch_start_solves(c));
     // species2d%interp_psi of the standard-pusher species
@@ -811,6 +861,9 @@ static int slice_finish(qpad_ctx *c, SliceRun &st)
     TRY(launch_epilogue(c, j));
     if (c->species_q)
         for (int s = 0; s < c->nspecies; s++) TRY(launch_species_cbq(c, s, j));
+    // sim_lasers%deposit_chi (simulation_class.f03:400-401): the particles'
+    // psi of the last current deposit, before the push
+    if (c->laser) TRY(launch_deposit_chi(c, j));
     const bool net = seednet_active(c);
     if (net) TRY(seednet_mark(c));
     TRY(launch_push_all(c));
@@ -1017,6 +1070,37 @@ int qpad_species_amjdp(qpad_ctx *c, int32_t s, double dxi)
     return sync(c);
 }
 
+int qpad_set_laser(qpad_ctx *c, const double *ar_hist, const double *ai_hist)
+{
+    if (!c || !ar_hist || !ai_hist) return fail(QPAD_E_INVALID, "null argument");
+    TRY(laser_ensure(c));
+    // packed host [(nxi + 3)][nm][nr + 2] -> padded device rows of length ld
+    const size_t rows = ((size_t)c->g.nxi + 3) * c->g.nm, w = (size_t)c->g.nr + 2;
+    QPAD_HIP_CHECK(hipMemcpy2DAsync(c->lz_ar, c->g.ld * sizeof(double), ar_hist, w * sizeof(double),
+                                    w * sizeof(double), rows, hipMemcpyHostToDevice, c->stream));
+    QPAD_HIP_CHECK(hipMemcpy2DAsync(c->lz_ai, c->g.ld * sizeof(double), ai_hist, w * sizeof(double),
+                                    w * sizeof(double), rows, hipMemcpyHostToDevice, c->stream));
+    return sync(c);
+}
+
+int qpad_laser_slice(qpad_ctx *c, int32_t j)
+{
+    if (!c) return fail(QPAD_E_INVALID, "null context");
+    if (j < 1 || j > c->g.nxi) return fail(QPAD_E_INVALID, "slice index out of range");
+    if (!c->laser) return fail(QPAD_E_STATE, "no laser: call qpad_set_laser or add a PGC species first");
+    TRY(launch_laser_slice(c, j));
+    return sync(c);
+}
+
+int qpad_deposit_chi(qpad_ctx *c, int32_t j)
+{
+    if (!c) return fail(QPAD_E_INVALID, "null context");
+    if (j < 0 || j > c->g.nxi + 1) return fail(QPAD_E_INVALID, "slice index out of range");
+    if (!c->laser) return fail(QPAD_E_STATE, "no laser: call qpad_set_laser or add a PGC species first");
+    TRY(launch_deposit_chi(c, j));
+    return sync(c);
+}
+
 int qpad_species_interp_psi(qpad_ctx *c, int32_t s)
 {
     if (!c || s < 0 || s >= c->nspecies) return fail(QPAD_E_INVALID, "bad species");
@@ -1138,6 +1222,7 @@ static double *hist_ptr(qpad_ctx *c, int fid, int *ncomp)
     case QPAD_F_Q_SPE: *ncomp = 1; return c->h_q_spe;
     case QPAD_F_Q_BEAM: *ncomp = 1; return c->h_q_beam;
     case QPAD_F_B_BEAM: *ncomp = 3; return c->h_b_beam;
+    case QPAD_F_CHI: *ncomp = 1; return c->h_chi;   // null before the laser state exists
     default: return nullptr;
     }
 }

@@ -67,6 +67,9 @@ struct Species {
     // per-species charge (qpad_species_charge_history): 1 x nm x ld and
     // [slice 0..nxi+1][nm][ld]
     double *qsl, *h_q;
+    // partial rows of the susceptibility deposit (laser contexts only):
+    // [nr][nm] and [n_ov_max][nm]
+    double *X0, *X1, *XO0, *XO1;
 };
 
 struct Beam {
@@ -137,6 +140,13 @@ struct qpad_ctx {
     bool species_q;           // per-species charge history on (Species::qsl, h_q)
     qpad::SeedNet *net;       // null unless qpad_seednet_create
     int net_pre;              // slice whose seed is already queued (0: none)
+    // laser (PGC pushers): the envelope history [(nxi + 3)][nm][ld], row
+    // s + 1 = slice s (-1 .. nxi + 1), and the susceptibility history
+    // [(nxi + 2)][nm][ld]; allocated by the first qpad_set_laser or PGC
+    // species (laser true), after which every slice takes the envelope and
+    // deposits chi
+    double *lz_ar, *lz_ai, *h_chi;
+    bool laser;
 };
 
 namespace qpad {

@@ -19,10 +19,12 @@ module qpad_hip_mod
   integer(c_int), parameter, public :: QPAD_F_PSI = 0, QPAD_F_E = 1, &
     QPAD_F_E_SPE = 2, QPAD_F_B = 3, QPAD_F_B_SPE = 4, QPAD_F_B_BEAM = 5, &
     QPAD_F_CU = 6, QPAD_F_ACU = 7, QPAD_F_AMU = 8, QPAD_F_DCU = 9, &
-    QPAD_F_Q_SPE = 10, QPAD_F_Q_BEAM = 11
+    QPAD_F_Q_SPE = 10, QPAD_F_Q_BEAM = 11, QPAD_F_AR = 12, QPAD_F_AI = 13, &
+    QPAD_F_AR_GRAD = 14, QPAD_F_AI_GRAD = 15, QPAD_F_CHI = 16
 
   integer(c_int), parameter, public :: QPAD_BND_ZERO = 2, QPAD_BND_OPEN = 3
-  integer(c_int), parameter, public :: QPAD_PUSH2_STD = 0, QPAD_PUSH2_ROBUST = 1
+  integer(c_int), parameter, public :: QPAD_PUSH2_STD = 0, QPAD_PUSH2_ROBUST = 1, &
+    QPAD_PUSH2_STD_PGC = 4, QPAD_PUSH2_ROBUST_PGC = 5
 
   type, bind(C), public :: qpad_grid
     integer(c_int32_t) :: nr, nxi, max_mode
@@ -215,6 +217,29 @@ module qpad_hip_mod
       import :: c_int, c_int32_t, c_ptr
       type(c_ptr), value :: ctx
       integer(c_int32_t), value :: sp
+      integer(c_int) :: rc
+    end function
+
+    ! the laser envelope history (field_laser cfr, cfi f2), each
+    ! ar(0:nr+1, nm, -1:nxi+1) in Fortran order
+    function qpad_set_laser(ctx, ar, ai) bind(C, name="qpad_set_laser") result(rc)
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      real(c_double), intent(in) :: ar(*), ai(*)
+      integer(c_int) :: rc
+    end function
+
+    function qpad_laser_slice(ctx, j) bind(C, name="qpad_laser_slice") result(rc)
+      import :: c_int, c_int32_t, c_ptr
+      type(c_ptr), value :: ctx
+      integer(c_int32_t), value :: j
+      integer(c_int) :: rc
+    end function
+
+    function qpad_deposit_chi(ctx, j) bind(C, name="qpad_deposit_chi") result(rc)
+      import :: c_int, c_int32_t, c_ptr
+      type(c_ptr), value :: ctx
+      integer(c_int32_t), value :: j
       integer(c_int) :: rc
     end function
 
@@ -610,6 +635,7 @@ module qpad_hip_mod
     qpad_species_new, qpad_species_renew, qpad_species_qdp, qpad_species_qdp_noalloc, &
     qpad_species_charge_history, qpad_species_cbq, qpad_get_species_charge, qpad_species_amjdp, &
     qpad_species_push, qpad_species_interp_psi, qpad_step_iterations, qpad_field_solve_psi, qpad_field_solve_ez, &
+    qpad_set_laser, qpad_laser_slice, qpad_deposit_chi, &
     qpad_field_solve_et, qpad_field_solve_bz, qpad_field_solve_bt_beam, &
     qpad_field_solve_djdxi, qpad_field_solve_bt_iter, qpad_field_add_b, &
     qpad_convergence_tester, qpad_beam_qdp, qpad_beam_push, qpad_beam_new, qpad_beam_new_std, qpad_beam_new_arrays, &

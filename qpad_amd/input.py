@@ -12,8 +12,8 @@ import math
 import numpy as np
 
 from ._lib import (BND_OPEN, BND_ZERO, GEOM_CARTESIAN, GEOM_CYLINDRICAL, OFFSET_MAX, PROF_GAUSSIAN,
-                   PROF_PIECEWISE, PROF_UNIFORM, PUSH2_ROBUST, PUSH2_STD, PW_MAX, BeamCfg, BeamStdCfg, Grid, QpadError,
-                   SpeciesCfg)
+                   PROF_PIECEWISE, PROF_UNIFORM, PUSH2_ROBUST, PUSH2_ROBUST_PGC, PUSH2_STD, PUSH2_STD_PGC,
+                   PW_MAX, BeamCfg, BeamStdCfg, Grid, QpadError, SpeciesCfg)
 
 
 def _get(d, key, default=None, required=True):
@@ -65,9 +65,11 @@ def grid_from_deck(deck):
         raise QpadError('Invalid interpolation type! Only "linear" are supported currently.')
     if _get(sim, "algorithm", "standard", required=False) != "standard":
         raise QpadError("only algorithm 'standard' is on the GPU path")
-    for key in ("nneutrals", "nlasers"):
-        if _get(sim, key, 0, required=False) != 0:
-            raise QpadError(f"{key} > 0 is not on the GPU path")
+    if _get(sim, "nneutrals", 0, required=False) != 0:
+        raise QpadError("nneutrals > 0 is not on the GPU path")
+    if _get(sim, "nlasers", 0, required=False) != 0:
+        raise QpadError("nlasers > 0: the laser solver is not on the GPU path; give the envelope to "
+                        "Simulation.set_laser (the PGC pushers and the chi deposit are)")
     relax = _get(sim, "relax_fac", 1.0e-3 * (dr / 0.02) ** 2, required=False)  # sim_fields_class.f03:137
     g = Grid(nr, nz, _get(sim, "max_mode"), dr, dxi, relax,
              BND_OPEN if bnd_name == "open" else BND_ZERO,
@@ -78,7 +80,8 @@ def grid_from_deck(deck):
 
 
 # species(i).push_type (sim_plasma_class.f03:123-137)
-_PUSHERS = {"standard": PUSH2_STD, "robust": PUSH2_ROBUST}
+_PUSHERS = {"standard": PUSH2_STD, "robust": PUSH2_ROBUST, "standard_pgc": PUSH2_STD_PGC,
+            "robust_pgc": PUSH2_ROBUST_PGC}
 
 
 def species_from_deck(sp, seed=0):
@@ -88,8 +91,8 @@ def species_from_deck(sp, seed=0):
         raise QpadError("only uniform plasma profiles are on the GPU path")
     push = _get(sp, "push_type")
     if push not in _PUSHERS:
-        raise QpadError(f'Invalid pusher type "{push}": "standard" and "robust" are on the GPU path '
-                        '(the PGC pushers need the laser envelope)')
+        raise QpadError('Invalid pusher type! Only "standard", "robust", "standard_pgc" and "robust_pgc" '
+                        'are supported currently.')
     ppc = _get(sp, "ppc")
     uth = _get(sp, "uth", [0.0, 0.0, 0.0], required=False)
     if _get(sp, "random_theta", False, required=False):

@@ -96,7 +96,8 @@ class FieldDjdxi(Field):
 
 
 class Species2D:
-    """species2d (species2d_class.f03:24-54), standard or robust pusher."""
+    """species2d (species2d_class.f03:24-54): standard, robust, standard_pgc
+    or robust_pgc pusher."""
 
     def __init__(self, sim, cfg):
         self.sim = sim
@@ -287,7 +288,8 @@ class Simulation:
         self.fields = {name: cls(self, name) for name, cls in (
             ("psi", FieldPsi), ("e", FieldE), ("e_spe", FieldE), ("b", FieldB),
             ("b_spe", FieldB), ("b_beam", FieldB), ("cu", Field), ("acu", Field), ("amu", Field),
-            ("dcu", FieldDjdxi), ("q_spe", Field), ("q_beam", Field))}
+            ("dcu", FieldDjdxi), ("q_spe", Field), ("q_beam", Field),
+            ("ar", Field), ("ai", Field), ("ar_grad", Field), ("ai_grad", Field), ("chi", Field))}
         nsp = deck["simulation"].get("nspecies", len(deck.get("species", [])))
         nbm = deck["simulation"].get("nbeams", len(deck.get("beam", [])))
         self.spe = [Species2D(self, species_from_deck(sp, seed=seed + 1000 * k))
@@ -295,6 +297,28 @@ class Simulation:
         self.beam = [Beam3D(self, beam_from_deck(bm, z0, seed=seed + 7 * (k + 1)))
                      for k, bm in enumerate(deck.get("beam", [])[:nbm])]
         self.tstep = 0
+
+    # -- laser (PGC pushers)
+    def set_laser(self, ar_hist, ai_hist):
+        """The laser envelope of every slice -1 .. nxi + 1 (field_laser's
+        cfr, cfi f2), real and imaginary part, [nxi + 3, nm, nr + 2] each.
+        From then on every slice pushes the PGC species in it and deposits
+        chi (fields["chi"], with a history)."""
+        shape = (self.grid.nxi + 3, self.nm, self.grid.nr + 2)
+        a = np.ascontiguousarray(ar_hist, dtype=np.float64)
+        b = np.ascontiguousarray(ai_hist, dtype=np.float64)
+        if a.shape != shape or b.shape != shape:
+            raise L.QpadError(f"laser envelope history must be {shape}")
+        L.check(self.lib.qpad_set_laser(self.ctx, L.dptr(a), L.dptr(b)))
+
+    def laser_slice(self, j):
+        """laser_all of slice j (simulation_class.f03:362-367): fields ar,
+        ai, ar_grad, ai_grad"""
+        L.check(self.lib.qpad_laser_slice(self.ctx, int(j)))
+
+    def deposit_chi(self, j):
+        """sim_lasers%deposit_chi(spe, j) (sim_lasers_class.f03:176-200)"""
+        L.check(self.lib.qpad_deposit_chi(self.ctx, int(j)))
 
     # -- fused run loop (simulation_class.f03:226-520)
     def step3d_begin(self):

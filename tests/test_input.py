@@ -45,7 +45,7 @@ def test_total_charge_definition():
 
 
 @pytest.mark.parametrize("edit,match", [
-    (lambda d: d["species"][0].update(push_type="standard_pgc"), "pusher"),
+    (lambda d: d["species"][0].update(push_type="boris"), "pusher"),
     (lambda d: d["species"][0].update(profile=["parabolic-channel", "uniform"]), "uniform"),
     (lambda d: d["simulation"].update(nlasers=1), "nlasers"),
     (lambda d: d["simulation"].update(field_boundary="periodic"), "boundary"),
@@ -174,12 +174,16 @@ def test_file_beam_placement_matches_the_oracle(oracle_mod):
 
 
 def test_pusher_types():
-    """species(i).push_type (sim_plasma_class.f03:123-137): "standard" and
-    "robust" map onto the library's pushers; the key is required, as in QPAD"""
-    from qpad_amd._lib import PUSH2_ROBUST, PUSH2_STD
+    """species(i).push_type (sim_plasma_class.f03:123-137): "standard",
+    "robust", "standard_pgc" and "robust_pgc" map onto the library's
+    pushers (param.f03:63-64); the key is required, as in QPAD"""
+    from qpad_amd._lib import PUSH2_ROBUST, PUSH2_ROBUST_PGC, PUSH2_STD, PUSH2_STD_PGC
     d = configs.small(push_type="standard", ions=True, ion_push_type="robust")
     assert species_from_deck(d["species"][0]).push_type == PUSH2_STD
     assert species_from_deck(d["species"][1]).push_type == PUSH2_ROBUST
+    d = configs.small(push_type="standard_pgc", ions=True, ion_push_type="robust_pgc")
+    assert species_from_deck(d["species"][0]).push_type == PUSH2_STD_PGC == 4
+    assert species_from_deck(d["species"][1]).push_type == PUSH2_ROBUST_PGC == 5
     sp = dict(d["species"][0])
     del sp["push_type"]
     with pytest.raises(QpadError, match="push_type"):

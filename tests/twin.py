@@ -79,6 +79,46 @@ def oracle_from_deck(deck):
     return o
 
 
+def laser_envelope(g, nm, a0=1.0, w0=1.0, xi_c=None, length=0.6, m1=0.15, curv=0.3):
+    """A synthetic laser envelope history for the laser-path tests (the laser
+    solver that evolves QPAD's envelope is not on the path): real and
+    imaginary parts [nxi + 3, nm, nr + 2] over slices -1 .. nxi + 1 (slice s
+    at xi = s dxi).  Gaussian in r (waist w0, peak a0) with a wavefront phase
+    curv r^2 in the imaginary part, sin^2 in xi over 2 length centred at
+    xi_c (default: the middle of the box); m = 1 content m1 (r / w0) times
+    the m = 0 envelope where the grid has modes."""
+    r = (np.arange(g.nr + 2) - 1.0) * g.dr
+    xi = np.arange(-1, g.nxi + 2) * g.dxi
+    if xi_c is None:
+        xi_c = 0.5 * g.nxi * g.dxi
+    d = xi - xi_c
+    lon = np.where(np.abs(d) < length, np.cos(0.5 * np.pi * d / length) ** 2, 0.0)
+    perp = a0 * np.exp(-(r / w0) ** 2)
+    ar = np.zeros((g.nxi + 3, nm, g.nr + 2))
+    ai = np.zeros_like(ar)
+    ar[:, 0, :] = lon[:, None] * (perp * np.cos(curv * r * r))[None, :]
+    ai[:, 0, :] = lon[:, None] * (perp * np.sin(curv * r * r))[None, :]
+    if nm >= 3:
+        ar[:, 1, :] = m1 * (r / w0)[None, :] * ar[:, 0, :]
+        ai[:, 2, :] = m1 * (r / w0)[None, :] * ar[:, 0, :]
+        ar[:, 2, :] = -0.5 * m1 * (r / w0)[None, :] * ai[:, 0, :]
+    return ar, ai
+
+
+LASER_FIELDS = (("ar", po.LASER_AR), ("ai", po.LASER_AI), ("ar_grad", po.LASER_AR_GRAD),
+                ("ai_grad", po.LASER_AI_GRAD), ("chi", po.LASER_CHI))
+
+
+def compare_laser(sim, o, norm, j, chi=True):
+    """the slice's envelope, gradients (and chi with its history slice j)"""
+    for name, which in LASER_FIELDS:
+        if name == "chi" and not chi:
+            continue
+        norm.add("laser." + name, sim.fields[name].get(), o.laser(which), j)
+    if chi:
+        norm.add("h_chi", sim.fields["chi"].hist(j), o.laser(po.LASER_H_CHI)[j], j)
+
+
 # what a golden record of one 3D step holds: iteration counts per slice, the
 # histories below (every slice), every species' plasma particles after the
 # last slice (by id) and beam 0 after its push (each coordinate sorted: the
