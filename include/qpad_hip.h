@@ -32,7 +32,7 @@
 extern "C" {
 #endif
 
-#define QPAD_HIP_ABI_VERSION 1
+#define QPAD_HIP_ABI_VERSION 2
 
 /* error codes */
 #define QPAD_OK 0
@@ -305,18 +305,25 @@ int qpad_step_iterations(qpad_ctx *ctx, int32_t *total_iters, int32_t *slices_at
    time while the device reports no convergence (one host wait per further
    iteration, overlapped with that iteration's field solves). */
 int qpad_iteration_policy(qpad_ctx *ctx, int32_t adaptive, int32_t margin);
-/* Launch shapes of the dominant kernels (no counterpart in QPAD; results do
-   not depend on them): dep_cells = radial cells, one wave each, per
-   workgroup of the current and charge deposits (2, 4 or 8; default 4). */
+/* Launch shapes of the dominant kernels (no counterpart in QPAD):
+   dep_cells = radial cells, one wave each, per workgroup of the current and
+   charge deposits (2, 4 or 8; default 4; results do not depend on it);
+   dep_mfma = 1: the current deposit sums each cell's particles on the matrix
+   cores (FP64 MFMA), 0: with the wave reduce-scatter; the two forms add the
+   same terms in different orders (results equal to rounding), so the form
+   is chosen here and never by the autotuner (default: 1 for max_mode >= 3,
+   where it measured faster on MI355X, else 0). */
 typedef struct {
     int32_t dep_cells;
+    int32_t dep_mfma;
 } qpad_launch_shape;
 int qpad_get_launch_shape(qpad_ctx *ctx, qpad_launch_shape *out);
 int qpad_set_launch_shape(qpad_ctx *ctx, const qpad_launch_shape *shape);
-/* Autotuner: times every candidate shape on the context's current particles
-   and fields (reps launches each; the deposits only rewrite scratch partials,
-   so no result changes) and keeps the fastest.  ms_out: mean ms per
-   candidate dep_cells = 2, 4, 8 (3 entries) or NULL. */
+/* Autotuner: times every candidate dep_cells on the context's current
+   particles and fields in the chosen deposit form (reps launches each; the
+   deposits only rewrite scratch partials, so no result changes) and keeps
+   the fastest.  ms_out: mean ms per candidate dep_cells = 2, 4, 8
+   (3 entries) or NULL. */
 int qpad_tune_launch(qpad_ctx *ctx, int32_t reps, double *ms_out);
 
 /* state exchange with the host (diagnostics, restart, parity checks) */

@@ -1314,9 +1314,12 @@ void ora_update_bound(const ora_sim *s, ora_species *sp)
         sp->p1[dst] = sp->p1[src]; sp->p2[dst] = sp->p2[src]; sp->p3[dst] = sp->p3[src]; \
         sp->gamma[dst] = sp->gamma[src]; sp->psi[dst] = sp->psi[src]; sp->q[dst] = sp->q[src]; \
         sp->id[dst] = sp->id[src]; } while (0)
+    /* !(pos < edge): a particle whose position is no longer a number leaves
+       too (QPAD's pos >= edge keeps it, and its next interpolation indexes
+       with int(NaN)) */
     while (i < sp->npp - 1) {
         double pos = sqrt(sp->x1[i] * sp->x1[i] + sp->x2[i] * sp->x2[i]);
-        if (pos >= edge) {
+        if (!(pos < edge)) {
             MOVE(i, sp->npp - 1);
             sp->npp--;
             continue;
@@ -1325,7 +1328,7 @@ void ora_update_bound(const ora_sim *s, ora_species *sp)
     }
     int64_t l = sp->npp - 1;
     double pos = sqrt(sp->x1[l] * sp->x1[l] + sp->x2[l] * sp->x2[l]);
-    if (pos >= edge) sp->npp--;
+    if (!(pos < edge)) sp->npp--;
 #undef MOVE
 }
 

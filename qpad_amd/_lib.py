@@ -60,7 +60,7 @@ class SeedNetInfo(C.Structure):
 
 
 class LaunchShape(C.Structure):
-    _fields_ = [("dep_cells", C.c_int32)]
+    _fields_ = [("dep_cells", C.c_int32), ("dep_mfma", C.c_int32)]
 
 
 class BeamCfg(C.Structure):

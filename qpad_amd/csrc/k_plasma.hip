@@ -556,14 +556,33 @@ struct JDepArgs {
 };
 
 // What one particle deposits (robust predictor, part2d_class.f03:877-960):
-// the eight deposited quantities val[comp] (cu 0..2, dcu 3..4, amu 5..7) and
-// the weight of node ix + n times the mode phase of mode array k,
-// wph[n][k].  Value v = n * 8NM + comp * NM + k of the cell's deposit is the
-// sum over its particles of wph[n][k] * val[comp].
+// the eight deposited quantities val[comp] (cu 0..2, dcu 3..4, amu 5..7),
+// the linear weights w[n] of nodes ix + n and the mode phases of mode array
+// k times q / (gamma - u_z), ph[k].  Value v = n * 8NM + comp * NM + k of
+// the cell's deposit is the sum over its particles of
+// w[n] ph[k] val[comp].
+template <int M>
+struct PartTerms {
+    double val[8];
+    double w[2];
+    double ph[2 * M + 1];
+};
+
+// the products the reduce-scatter form accumulates: value v is
+// (w[n] ph[k]) val[comp]
 template <int M>
 struct DepTerms {
     double val[8];
     double wph[2][2 * M + 1];
+    __device__ __forceinline__ explicit DepTerms(const PartTerms<M> &t)
+    {
+#pragma unroll
+        for (int c = 0; c < 8; c++) val[c] = t.val[c];
+#pragma unroll
+        for (int n = 0; n < 2; n++)
+#pragma unroll
+            for (int k = 0; k < 2 * M + 1; k++) wph[n][k] = t.w[n] * t.ph[k];
+    }
     __device__ __forceinline__ double operator()(int v) const
     {
         constexpr int NM = 2 * M + 1;
@@ -616,7 +635,7 @@ __device__ __forceinline__ double dep_stage_load_pgc(const double *e, const doub
 template <int M, bool STD, bool PGC>
 __device__ __forceinline__ void particle_terms(const JDepArgs &a, const double *sf, int c, int p, bool valid,
                                                double x1, double x2, double pp1, double pp2, double pp3,
-                                               double qp, double psip, DepTerms<M> &t)
+                                               double qp, double psip, PartTerms<M> &t)
 {
     constexpr int NM = 2 * M + 1;
     double r = radius(x1, x2);
@@ -720,18 +739,82 @@ __device__ __forceinline__ void particle_terms(const JDepArgs &a, const double *
     t.val[6] = u[0] * u[1] * ipsi;
     t.val[7] = u[1] * u[1] * ipsi;
     double phr = 1.0 * (valid ? qp : 0.0) * ipsi, phi = 0.0;
-    t.wph[0][0] = w0 * phr;
-    t.wph[1][0] = w1 * phr;
+    t.w[0] = w0;
+    t.w[1] = w1;
+    t.ph[0] = phr;
     const double ph0r = pc, ph0i = -ps;
 #pragma unroll
     for (int m = 1; m <= M; m++) {
         double tr = phr * ph0r - phi * ph0i, ti = phr * ph0i + phi * ph0r;
         phr = tr;
         phi = ti;
-        t.wph[0][2 * m - 1] = w0 * phr;
-        t.wph[1][2 * m - 1] = w1 * phr;
-        t.wph[0][2 * m] = w0 * phi;
-        t.wph[1][2 * m] = w1 * phi;
+        t.ph[2 * m - 1] = phr;
+        t.ph[2 * m] = phi;
+    }
+}
+
+// The matrix form of a wave's cell deposit: the cell's values are the
+// 16 x NM product D[n * 8 + comp][k] = sum_p (w[n] val[comp])_p ph[k]_p, a
+// GEMM with the particles as the inner dimension, which
+// v_mfma_f64_16x16x4f64 accumulates four particles at a time (FP64 on the
+// matrix cores; no cross-lane reduction and four accumulators per lane
+// instead of 16 NM).  Operand layout of the instruction: lane l supplies
+// A[l % 16][l / 16] and B[l / 16][l % 16] and holds D[4 r + l / 16][l % 16]
+// in its accumulator r.  The pass's particle terms reach that layout
+// through the wave's own LDS rows ([particle][A 0..15, ph 0..NM-1, 0],
+// stride mf_stride; lanes past the NM columns read the zero), one half-wave
+// of 32 particles at a time; each half's operands are read before its eight
+// instructions are issued.
+typedef double dep_acc4 __attribute__((ext_vector_type(4)));
+
+template <int NM>
+constexpr int mf_stride() { return (17 + NM) | 1; }
+
+// LDS writes of this wave visible to its other lanes (waves of a workgroup
+// run different pass counts, so no workgroup barrier here)
+__device__ __forceinline__ void wave_lds_sync()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// one accumulator chain per wave: independent chains (summed at the end)
+// would overlap the instruction's latency, but their registers cost a wave
+// per SIMD, which measured slower on MI355X (C2: 25.7 against 27.5 us with
+// four chains, C3: 23.7 against 18.6 us)
+template <int M>
+__device__ __forceinline__ void mf_accumulate(double *rows, const PartTerms<M> &t, int lane, dep_acc4 &acc)
+{
+    constexpr int NM = 2 * M + 1, STR = mf_stride<NM>();
+    const int gi = lane >> 4, i = lane & 15;
+    const int fb = 16 + min(i, NM);
+#pragma unroll
+    for (int h = 0; h < 2; h++) {
+        if ((lane >> 5) == h) {
+            double *mine = rows + (lane & 31) * STR;
+#pragma unroll
+            for (int n = 0; n < 2; n++)
+#pragma unroll
+                for (int c = 0; c < 8; c++) mine[n * 8 + c] = t.w[n] * t.val[c];
+#pragma unroll
+            for (int k = 0; k < NM; k++) mine[16 + k] = t.ph[k];
+            mine[16 + NM] = 0.0;
+        }
+        wave_lds_sync();
+        double av[8], bv[8];
+#pragma unroll
+        for (int s = 0; s < 8; s++) {
+            const double *r = rows + (4 * s + gi) * STR;
+            av[s] = r[i];
+            bv[s] = r[fb];
+        }
+#pragma unroll
+        for (int s = 0; s < 8; s++) acc = __builtin_amdgcn_mfma_f64_16x16x4f64(av[s], bv[s], acc, 0, 0, 0);
+        // all sixteen reads ahead of the eight instructions
+        __builtin_amdgcn_sched_group_barrier(0x0100, 16, 0);
+        __builtin_amdgcn_sched_group_barrier(0x0008, 8, 0);
+        wave_lds_sync();
     }
 }
 
@@ -758,7 +841,9 @@ struct JDepMulti {
 // SPEC: a speculative launch of iteration >= 2 of a slice, which returns at
 // once when the iteration has already converged (a separate instance, so
 // that profiles show the first-iteration launches, which always work, apart)
-template <int M, bool SPEC, int CPW, bool STD, bool PGC = false>
+// MF: the matrix form of the accumulation (mf_accumulate) instead of the
+// reduce-scatter; the same values up to the order of the sums
+template <int M, bool SPEC, int CPW, bool STD, bool PGC = false, bool MF = false>
 __global__ __launch_bounds__(64 * CPW) void k_amjdeposit(const JDepMulti m)
 {
     const JDepArgs a = m.s[__builtin_amdgcn_readfirstlane(blockIdx.y)];
@@ -767,6 +852,8 @@ __global__ __launch_bounds__(64 * CPW) void k_amjdeposit(const JDepMulti m)
     constexpr int NF = dep_staged<NM, PGC>();   // staged field values per cell
     // the fields at the two nodes of each wave's cell (dep_staged layout)
     __shared__ double sf[CPW][NF];
+    // the matrix form's particle rows, per wave
+    __shared__ double mrows[MF ? CPW * 32 * mf_stride<NM>() : 1];
     const int lane = threadIdx.x & 63;
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     if (dep_group_idle<CPW>(a.nr, a.os)) return;
@@ -829,10 +916,12 @@ __global__ __launch_bounds__(64 * CPW) void k_amjdeposit(const JDepMulti m)
     const double *__restrict__ PP3 = a.p3;
     const double *__restrict__ Q = a.q;
     constexpr int S = dep_pass_steps<M>();
-    constexpr int NA = (2 * NV) >> S;
+    constexpr int NA = MF ? 1 : (2 * NV) >> S;
     double acc[NA];
 #pragma unroll
     for (int v = 0; v < NA; v++) acc[v] = 0.0;
+    dep_acc4 macc = {0.0, 0.0, 0.0, 0.0};
+    double *const wrows = mrows + (MF ? wave * 32 * mf_stride<NM>() : 0);
     // every pass has all 64 lanes active (the exchanges need them); lanes
     // past the chunk's end deposit zero.  Software-pipelined: the next pass's
     // particle loads are in flight while this pass computes.
@@ -849,21 +938,40 @@ __global__ __launch_bounds__(64 * CPW) void k_amjdeposit(const JDepMulti m)
         const int pn = min(p + 64, pl);
         const double nx1 = X1[pn], nx2 = X2[pn], np1 = PP1[pn], np2 = PP2[pn], np3 = PP3[pn], nq = Q[pn];
         const double npsi = STD ? a.psi[pn] : 0.0;
-        DepTerms<M> t;
-        particle_terms<M, STD, PGC>(a, sf[wave], c, p, p < p1, x1, x2, pp1, pp2, pp3, qp, psip, t);
+        PartTerms<M> pt;
+        particle_terms<M, STD, PGC>(a, sf[wave], c, p, p < p1, x1, x2, pp1, pp2, pp3, qp, psip, pt);
+        if constexpr (MF) {
+            mf_accumulate<M>(wrows, pt, lane, macc);
+        } else {
+            const DepTerms<M> t(pt);
 #pragma unroll
-        for (int i = 0; i < NA; i++) acc[i] += reduced_value<NM, S>(t, i, lane);
+            for (int i = 0; i < NA; i++) acc[i] += reduced_value<NM, S>(t, i, lane);
+        }
         x1 = nx1; x2 = nx2; pp1 = np1; pp2 = np2; pp3 = np3; qp = nq; psip = npsi;
     }
-    wave_reduce_from<NM, S>(acc);
-    const int g = lane >> 2;
     double *D0 = ch.ovf ? a.O0 : a.P0, *D1 = ch.ovf ? a.O1 : a.P1;
+    if constexpr (MF) {
+        // lane l holds D[4 r + l / 16][l % 16]: row = node * 8 + comp, column = mode array
+        const int k = lane & 15, rb = lane >> 4;
+        const dep_acc4 sum = macc;
+        if (k < NM) {
 #pragma unroll
-    for (int i = 0; i < NM; i++) {
-        const int v = g * NM + i;
-        if ((lane & 3) == (i & 3)) {
-            if (v < NV) D0[(size_t)ch.row * NV + v] = acc[i];
-            else D1[(size_t)ch.row * NV + v - NV] = acc[i];
+            for (int r = 0; r < 4; r++) {
+                const int row = 4 * r + rb, comp = row & 7;
+                double *D = row < 8 ? D0 : D1;
+                D[(size_t)ch.row * NV + comp * NM + k] = sum[r];
+            }
+        }
+    } else {
+        wave_reduce_from<NM, S>(acc);
+        const int g = lane >> 2;
+#pragma unroll
+        for (int i = 0; i < NM; i++) {
+            const int v = g * NM + i;
+            if ((lane & 3) == (i & 3)) {
+                if (v < NV) D0[(size_t)ch.row * NV + v] = acc[i];
+                else D1[(size_t)ch.row * NV + v - NV] = acc[i];
+            }
         }
     }
 }
@@ -929,11 +1037,17 @@ static int launch_jdep_list(qpad_ctx *c, const int *list, int n, int l, bool sto
     dim3 grid((unsigned)gx, (unsigned)n), blk(64 * cpw);
     hipEvent_t stop = nullptr;
     if (timed) time_begin(c, 0, &stop);
-#define QPAD_DEP2(MM, CC)                                                                                   \
-    do {                                                                                                    \
-        if (speculative)                                                                                    \
-            hipLaunchKernelGGL((k_amjdeposit<MM, true, CC, STD, PGC>), grid, blk, 0, c->stream, m);          \
-        else hipLaunchKernelGGL((k_amjdeposit<MM, false, CC, STD, PGC>), grid, blk, 0, c->stream, m);        \
+    const bool mf = c->dep_mfma != 0;
+#define QPAD_DEP3(MM, CC, MFV)                                                                                \
+    do {                                                                                                      \
+        if (speculative)                                                                                      \
+            hipLaunchKernelGGL((k_amjdeposit<MM, true, CC, STD, PGC, MFV>), grid, blk, 0, c->stream, m);       \
+        else hipLaunchKernelGGL((k_amjdeposit<MM, false, CC, STD, PGC, MFV>), grid, blk, 0, c->stream, m);     \
+    } while (0)
+#define QPAD_DEP2(MM, CC)                    \
+    do {                                     \
+        if (mf) QPAD_DEP3(MM, CC, true);     \
+        else QPAD_DEP3(MM, CC, false);       \
     } while (0)
 #define QPAD_DEP(MM)                                 \
     do {                                             \
@@ -950,6 +1064,7 @@ static int launch_jdep_list(qpad_ctx *c, const int *list, int n, int l, bool sto
     }
 #undef QPAD_DEP
 #undef QPAD_DEP2
+#undef QPAD_DEP3
     QPAD_HIP_CHECK(hipGetLastError());
     if (timed) time_end(c, 0, stop);
     return 0;

@@ -404,6 +404,10 @@ int qpad_create(const qpad_grid *gr, qpad_ctx **out)
     c->spec_margin = 0;
     c->spec_guess = 2;
     c->dep_cells = kDepWaves;
+    // the matrix form where it measured faster on MI355X: modes 0..3 and up
+    // (C3, modes 0..4: 18.6 against 26.0 us per current deposit); the
+    // reduce-scatter up to modes 0..2 (C2: 21.5 against 27.5 us)
+    c->dep_mfma = gr->max_mode >= 3 ? 1 : 0;
     if (rc) {
         qpad_destroy(c);
         return rc;
@@ -943,6 +947,7 @@ int qpad_get_launch_shape(qpad_ctx *c, qpad_launch_shape *out)
 {
     if (!c || !out) return fail(QPAD_E_INVALID, "null argument");
     out->dep_cells = c->dep_cells;
+    out->dep_mfma = c->dep_mfma;
     return 0;
 }
 
@@ -951,7 +956,9 @@ int qpad_set_launch_shape(qpad_ctx *c, const qpad_launch_shape *shape)
     if (!c || !shape) return fail(QPAD_E_INVALID, "null argument");
     if (shape->dep_cells != 2 && shape->dep_cells != 4 && shape->dep_cells != 8)
         return fail(QPAD_E_INVALID, "dep_cells must be 2, 4 or 8");
+    if (shape->dep_mfma != 0 && shape->dep_mfma != 1) return fail(QPAD_E_INVALID, "dep_mfma must be 0 or 1");
     c->dep_cells = shape->dep_cells;
+    c->dep_mfma = shape->dep_mfma;
     return 0;
 }
 

@@ -131,6 +131,7 @@ struct qpad_ctx {
     hipEvent_t dec_ev;
     int adaptive, spec_margin, spec_guess;
     int dep_cells;            // radial cells per deposit workgroup (2, 4, 8)
+    int dep_mfma;             // current deposit in the matrix form (1) or the reduce-scatter form (0)
     int nspecies, nbeams;
     qpad::Species sp[qpad::kMaxSpecies];
     qpad::Beam bm[qpad::kMaxBeams];

@@ -11,7 +11,7 @@ module qpad_hip_mod
   implicit none
   private
 
-  integer(c_int), parameter, public :: QPAD_HIP_ABI_VERSION = 1
+  integer(c_int), parameter, public :: QPAD_HIP_ABI_VERSION = 2
 
   integer(c_int), parameter, public :: QPAD_OK = 0, QPAD_E_INVALID = -1, &
     QPAD_E_HIP = -2, QPAD_E_NOMEM = -3, QPAD_E_STATE = -4
@@ -94,7 +94,7 @@ module qpad_hip_mod
   end type qpad_seednet_info
 
   type, bind(C), public :: qpad_launch_shape
-    integer(c_int32_t) :: dep_cells
+    integer(c_int32_t) :: dep_cells, dep_mfma
   end type qpad_launch_shape
 
   public :: qpad_hip_check, qpad_error_message

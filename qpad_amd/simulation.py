@@ -373,11 +373,21 @@ class Simulation:
     def launch_shape(self, dep_cells=None):
         """Get (or set) the deposit workgroup shape (include/qpad_hip.h)."""
         sh = L.LaunchShape()
+        L.check(self.lib.qpad_get_launch_shape(self.ctx, C.byref(sh)))
         if dep_cells is not None:
             sh.dep_cells = int(dep_cells)
             L.check(self.lib.qpad_set_launch_shape(self.ctx, C.byref(sh)))
-        L.check(self.lib.qpad_get_launch_shape(self.ctx, C.byref(sh)))
         return sh.dep_cells
+
+    def deposit_form(self, mfma=None):
+        """Get (or set) the current deposit's accumulation form: True = FP64
+        matrix cores, False = wave reduce-scatter (include/qpad_hip.h)."""
+        sh = L.LaunchShape()
+        L.check(self.lib.qpad_get_launch_shape(self.ctx, C.byref(sh)))
+        if mfma is not None:
+            sh.dep_mfma = int(bool(mfma))
+            L.check(self.lib.qpad_set_launch_shape(self.ctx, C.byref(sh)))
+        return bool(sh.dep_mfma)
 
     def tune_launch(self, reps=5):
         """Autotune the launch shape on the current state; returns the mean

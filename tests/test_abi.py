@@ -40,6 +40,7 @@ def test_struct_layout_matches_header(tmp_path):
         "qpad_grid": [f for f, _ in L.Grid._fields_],
         "qpad_species_cfg": [f for f, _ in L.SpeciesCfg._fields_],
         "qpad_beam_cfg": [f for f, _ in L.BeamCfg._fields_],
+        "qpad_launch_shape": [f for f, _ in L.LaunchShape._fields_],
     }
     lines = ['#include <stdio.h>', '#include <stddef.h>', f'#include "{HEADER}"', "int main(void){"]
     for st, fs in fields.items():
@@ -53,14 +54,15 @@ def test_struct_layout_matches_header(tmp_path):
     out = dict(line.rsplit(" ", 1) for line in subprocess.run([str(exe)], check=True, capture_output=True,
                                                                text=True).stdout.split("\n") if line)
     for st, cls in (("qpad_grid", L.Grid), ("qpad_species_cfg", L.SpeciesCfg),
-                    ("qpad_beam_cfg", L.BeamCfg)):
+                    ("qpad_beam_cfg", L.BeamCfg), ("qpad_launch_shape", L.LaunchShape)):
         assert int(out[st]) == C.sizeof(cls), st
         for f, _ in cls._fields_:
             assert int(out[f"{st}.{f}"]) == getattr(cls, f).offset, f"{st}.{f}"
 
 
 def test_abi_version():
-    assert L.load().qpad_abi_version() == 1
+    """2: qpad_launch_shape gained dep_mfma (the struct grew)"""
+    assert L.load().qpad_abi_version() == 2
 
 
 def test_invalid_grid_is_rejected_with_message():

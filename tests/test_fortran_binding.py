@@ -55,7 +55,7 @@ def test_fortran_module_binds_the_library(tmp_path):
                     f"-Wl,-rpath,{libdir}", "-o", "t"], check=True, cwd=tmp_path)
     out = subprocess.run([str(tmp_path / "t")], check=True, capture_output=True, text=True).stdout
     kv = dict(line.strip().split(" ", 1) for line in out.splitlines() if line.strip())
-    assert int(kv["abi"]) == lib.qpad_abi_version() == 1
+    assert int(kv["abi"]) == lib.qpad_abi_version() == 2
     assert int(kv["grid"]) == C.sizeof(L.Grid)
     assert int(kv["species"]) == C.sizeof(L.SpeciesCfg)
     assert int(kv["beam"]) == C.sizeof(L.BeamCfg)
