@@ -79,7 +79,23 @@ class _Species(C.Structure):
                 ("qbm", C.c_double), ("qm", C.c_double), ("dt", C.c_double),
                 ("cfg", SpeciesCfg),
                 ("q_own", _dp), ("qn", _dp), ("cu_own", _dp), ("acu_own", _dp),
-                ("amu_own", _dp), ("q_hist", _dp)]
+                ("amu_own", _dp), ("q_hist", _dp),
+                ("is_neutral", C.c_int), ("multi_max", C.c_int), ("wp", C.c_double), ("rate", _dp),
+                ("multi_ion", _dp), ("ion_old", _dp), ("ppc_add", C.POINTER(C.c_int)),
+                ("add_x1", _dp), ("add_x2", _dp), ("add_q", _dp), ("n_add", C.c_int64),
+                ("n_added", C.c_int64), ("rho_hist", _dp)]
+
+
+class NeutralCfg(C.Structure):
+    _fields_ = [("ppc1", C.c_int), ("ppc2", C.c_int), ("num_theta", C.c_int),
+                ("qm", C.c_double), ("mass", C.c_double), ("density", C.c_double),
+                ("den_min", C.c_double), ("element", C.c_int), ("ion_max", C.c_int),
+                ("push_type", C.c_int), ("n0", C.c_double)]
+
+
+def neutral_cfg(ppc1, ppc2, num_theta, element, ion_max, n0, qm=-1.0, mass=1.0, density=1.0,
+                den_min=1e-10, push_type=PUSH2_ROBUST):
+    return NeutralCfg(ppc1, ppc2, num_theta, qm, mass, density, den_min, element, ion_max, push_type, n0)
 
 
 class _Beam(C.Structure):
@@ -157,6 +173,13 @@ def load_library(path):
     L.ora_laser_array.restype = _dp
     L.ora_laser_array.argtypes = [C.c_void_p, C.c_int]
     L.ora_update_bound.argtypes = [C.c_void_p, C.POINTER(_Species)]
+    L.ora_add_neutral.argtypes = [C.c_void_p, C.POINTER(NeutralCfg)]
+    L.ora_adk.restype = _dp
+    L.ora_adk.argtypes = [C.c_int, C.c_int]
+    L.ora_neutral_ionize.argtypes = [C.c_void_p, C.POINTER(_Species), _dp]
+    L.ora_neutral_add.argtypes = [C.c_void_p, C.POINTER(_Species)]
+    L.ora_neutral_ion_deposit.argtypes = [C.c_void_p, C.POINTER(_Species)]
+    L.ora_neutral_renew.argtypes = [C.c_void_p, C.POINTER(_Species)]
     L.ora_interp_point.argtypes = [C.c_void_p, _dp, C.c_int, C.c_double, C.c_double, _dp]
     L.ora_beam_qdeposit.argtypes = [C.c_void_p, C.c_int, _dp]
     L.ora_beam_push.argtypes = [C.c_void_p, C.c_int]
@@ -291,6 +314,32 @@ class Oracle:
         r = self.L.ora_add_species(self.h, C.byref(cfg))
         self.nspecies += 1
         return r
+
+    def add_neutral(self, cfg):
+        r = self.L.ora_add_neutral(self.h, C.byref(cfg))
+        assert r >= 0, "neutral refused"
+        self.nspecies += 1
+        return r
+
+    def neutral_levels(self, isp):
+        """multi_ion of neutral isp as [num_theta, nr, multi_max + 2] (levels
+        1..L, the neutral residue, the discrete total ion charge)"""
+        sp = self.species_raw(isp).contents
+        nth, nr, L = sp.cfg.num_theta, self.grid.nr, sp.multi_max
+        n = nth * nr * (L + 2)
+        return np.ctypeslib.as_array(sp.multi_ion, shape=(n,)).reshape(nth, nr, L + 2)
+
+    def neutral_ppc_add(self, isp):
+        sp = self.species_raw(isp).contents
+        n = sp.cfg.num_theta * self.grid.nr
+        return np.ctypeslib.as_array(sp.ppc_add, shape=(n,)).reshape(sp.cfg.num_theta, self.grid.nr)
+
+    def neutral_rho_hist(self, isp):
+        """rho_ion history [nxi + 2, 1, nm, ld] (neutral%cbq)"""
+        sp = self.species_raw(isp).contents
+        ns = self.grid.nxi + 2
+        n = ns * self.nm * self.ld
+        return np.ctypeslib.as_array(sp.rho_hist, shape=(n,)).reshape(ns, 1, self.nm, self.ld)
 
     def add_beam(self, cfg, npmax=0):
         r = self.L.ora_add_beam(self.h, C.byref(cfg), npmax)

@@ -1818,6 +1818,8 @@ void ora_destroy(ora_sim *s)
         free(sp->q); free(sp->gamma); free(sp->psi); free(sp->id);
         free(sp->q_own); free(sp->qn); free(sp->cu_own); free(sp->acu_own); free(sp->amu_own);
         free(sp->q_hist);
+        free(sp->multi_ion); free(sp->ion_old); free(sp->ppc_add);
+        free(sp->add_x1); free(sp->add_x2); free(sp->add_q); free(sp->rho_hist);
     }
     for (int i = 0; i < s->nbeams; i++) {
         ora_beam *bm = &s->bm[i];
@@ -1859,6 +1861,291 @@ int ora_add_species(ora_sim *s, const ora_species_cfg *cfg)
     ora_species_inject(s, isp);
     ora_species_init_q(s, isp);
     return isp;
+}
+
+/* ------------------------------------------------------------------------ */
+/* Neutral gas and field ionisation (neutral_class.f03)                      */
+/* ------------------------------------------------------------------------ */
+
+/* ADK coefficients per level, w = C1 E^-C3 exp(-C2 / E) with E in GV/m and
+   w in 1/s (neutral_class.f03:23-299, the elements of atomic number 1, 2, 3,
+   6, 7, 8, 10, 18).  Each row is the ADK rate (l = 0, m = 0) of charge
+   state i with its ionisation energy U: n* = i sqrt(U_H / U), C3 = 2n* - 1,
+   C2 = (2/3) (U/U_H)^(3/2) E_a, C1 = w_a 2^(2n*) / (n* Gamma(2n*)) (U/2U_H)
+   (2 (U/U_H)^(3/2) E_a)^(2n* - 1). */
+static const double ADK_H[1][3] = {{8.522542995398661e19, 342.53947239007687, 1.0005337056631487}};
+static const double ADK_He[2][3] = {{7.2207661763501e18, 832.809878216992, 0.48776427204592254},
+                                    {2.7226733893691e21, 2742.1316798375965, 1.0000920088118899}};
+static const double ADK_Li[3][3] = {{3.460272990838495e21, 85.51998980232813, 2.1770706138013733},
+                                    {3.6365138642921554e20, 4493.713340713575, 0.6964625952167312},
+                                    {2.0659396971422902e22, 9256.32561931876, 0.9999745128918196}};
+static const double ADK_C[6][3] = {{1.8809310466441196e20, 258.1083104918942, 1.1984440550739555},
+                                   {5.509352426405289e23, 822.523017116882, 1.987881643259267},
+                                   {4.572825176241475e25, 2263.6763526780164, 2.198152910400111},
+                                   {9.835564484116034e27, 3537.9525592618193, 2.674447703635764},
+                                   {7.536123777733801e22, 53034.28979013096, 0.8628040191262976},
+                                   {6.587972799911127e23, 74090.46603762524, 0.9996157827449552}};
+static const double ADK_N[7][3] = {{6.441568477283914e19, 378.4956025896395, 0.9350660865589433},
+                                   {1.4701035287257558e23, 1100.1258100528623, 1.711847679118431},
+                                   {4.963404971976876e25, 2232.374603536868, 2.2130314611564854},
+                                   {1.429510866064162e27, 4658.081063546513, 2.3525381223431663},
+                                   {1.3012849307544791e29, 6615.849778853221, 2.7281285044063783},
+                                   {2.1409091975134166e23, 88606.4475724123, 0.8838466900060211},
+                                   {1.4212916247536375e24, 117682.27130021283, 0.9994495038042972}};
+static const double ADK_O[8][3] = {{8.471015773053202e19, 343.2810702373722, 0.9990920676510977},
+                                   {4.659951179950614e22, 1421.770921017062, 1.4896379815554006},
+                                   {1.375943467286961e25, 2781.3610873184325, 1.985966133301126},
+                                   {1.4410401568574448e27, 4652.670876621719, 2.353837077475246},
+                                   {2.1812887948673846e28, 8303.41171218499, 2.4562134926799617},
+                                   {1.0859737168339012e30, 11088.09514588445, 2.766300924086769},
+                                   {5.135265231597041e23, 137319.4144041063, 0.8991975448040144},
+                                   {2.7649244731504574e24, 175715.8649323391, 0.999259077262769}};
+static const double ADK_Ne[10][3] = {{1.2380498982127483e19, 684.0495717312216, 0.5886207199036224},
+                                     {1.6861743227006676e22, 1790.87087951275, 1.3052851455522987},
+                                     {4.012462791033353e24, 3450.2505445538854, 1.7789909006599105},
+                                     {1.4251452589514637e26, 6544.999998027671, 1.9932260907792414},
+                                     {6.670617838240815e27, 9689.667572133209, 2.282840577038655},
+                                     {1.9391656057538128e29, 13557.847041347119, 2.522116807457813},
+                                     {1.3881927947789378e30, 20383.797283570835, 2.586898525643846},
+                                     {3.1075134088626144e31, 25254.466329809107, 2.8167466774677434},
+                                     {2.1020658703689154e24, 282468.0585535283, 0.9200040384519763},
+                                     {8.391062111505151e24, 343429.9456106982, 0.9988031600227598}};
+static const double ADK_Ar[18][3] = {{4.583155091154178e19, 427.3616288942586, 0.858307468844768},
+                                     {2.347069522730368e23, 992.0665907024354, 1.8069357289301329},
+                                     {1.931637188507353e26, 1775.9423286421918, 2.4675897958480384},
+                                     {2.3004409750295724e28, 3141.43462246603, 2.8229627269683206},
+                                     {3.468592152886133e30, 4422.602894394151, 3.263766733070865},
+                                     {2.965675081118506e32, 5958.158993507227, 3.6326551045811497},
+                                     {2.1249901067576396e33, 9479.178143644607, 3.629746649292696},
+                                     {8.991061313075681e34, 11737.03857874398, 3.92742350307885},
+                                     {7.611312705906995e29, 59342.85460567607, 2.229751529947494},
+                                     {6.602179339057654e30, 71781.6325368281, 2.3680482459080237},
+                                     {4.815600903407338e31, 85819.59318777094, 2.490706079578982},
+                                     {2.055113377686043e32, 105186.65111959413, 2.558310025830369},
+                                     {1.1945429165015238e33, 122591.38628416909, 2.663021333332675},
+                                     {6.212435859768618e33, 141745.69057711778, 2.7584389081812857},
+                                     {1.6060852787486324e34, 170916.68217921766, 2.783373249974532},
+                                     {9.327897444360857e34, 190110.7378894357, 2.894937691775023},
+                                     {6.6878740700780965e25, 1.8068758775178685e6, 0.9536895901987263},
+                                     {1.5247916822996254e26, 2.0115330349032478e6, 0.9959340925415332}};
+
+const double *ora_adk(int element, int level)
+{
+    const double(*t)[3] = NULL;
+    int n = 0;
+    switch (element) {
+    case 1: t = ADK_H; n = 1; break;
+    case 2: t = ADK_He; n = 2; break;
+    case 3: t = ADK_Li; n = 3; break;
+    case 6: t = ADK_C; n = 6; break;
+    case 7: t = ADK_N; n = 7; break;
+    case 8: t = ADK_O; n = 8; break;
+    case 10: t = ADK_Ne; n = 10; break;
+    case 18: t = ADK_Ar; n = 18; break;
+    default: return NULL;
+    }
+    return (level >= 1 && level <= n) ? t[level - 1] : NULL;
+}
+
+/* renew_neutral (neutral_class.f03:839-878): no electrons, every cell back
+   to the neutral gas, no ion charge */
+void ora_neutral_renew(const ora_sim *s, ora_species *sp)
+{
+    const int L = sp->multi_max, nc = sp->cfg.num_theta * s->g.nr;
+    sp->npp = 0;
+    sp->n_add = 0;
+    sp->n_added = 0;
+    memset(sp->multi_ion, 0, (size_t)nc * (L + 2) * sizeof(double));
+    for (int c = 0; c < nc; c++) sp->multi_ion[(size_t)c * (L + 2) + L] = 1.0;
+    memset(sp->ion_old, 0, (size_t)nc * sizeof(double));
+    memset(sp->ppc_add, 0, (size_t)nc * sizeof(int));
+    memset(sp->qn, 0, (size_t)s->nm * s->ld * sizeof(double));
+}
+
+/* init_neutral (neutral_class.f03:408-574, sim_plasma_class.f03:148-176):
+   an electron species with no particles yet, qbm = q/m, the ADK rows of
+   the first multi_max = ion_max levels, wp = sqrt(n0) 5.641460231180626e4.
+   QPAD sets the level indices from ion_max before clipping it to the
+   element's levels; ion_max beyond them is refused here. */
+int ora_add_neutral(ora_sim *s, const ora_neutral_cfg *c)
+{
+    if (s->nspecies >= 8 || c->ion_max < 1 || !ora_adk(c->element, c->ion_max)) return -1;
+    if (c->density * 1.0 * 1.0 < c->den_min) return -1;
+    ora_species_cfg sc;
+    memset(&sc, 0, sizeof(sc));
+    sc.ppc1 = c->ppc1;
+    sc.ppc2 = c->ppc2;
+    sc.num_theta = c->num_theta;
+    sc.qm = c->qm;
+    sc.mass = c->mass;
+    sc.density = c->density;
+    sc.neutralized = 0;
+    sc.den_min = c->den_min;
+    sc.push_type = c->push_type;
+    const int isp = s->nspecies++;
+    ora_species *sp = &s->sp[isp];
+    memset(sp, 0, sizeof(*sp));
+    sp->cfg = sc;
+    sp->is_neutral = 1;
+    sp->multi_max = c->ion_max;
+    sp->rate = ora_adk(c->element, 1);
+    sp->wp = sqrt(c->n0) * 5.641460231180626e4;
+    sp->npmax = (int64_t)s->g.nr * c->ppc1 * c->ppc2 * c->num_theta * 2;
+    const int64_t n = sp->npmax;
+    sp->x1 = calloc(n, sizeof(double)); sp->x2 = calloc(n, sizeof(double));
+    sp->p1 = calloc(n, sizeof(double)); sp->p2 = calloc(n, sizeof(double));
+    sp->p3 = calloc(n, sizeof(double)); sp->q = calloc(n, sizeof(double));
+    sp->gamma = calloc(n, sizeof(double)); sp->psi = calloc(n, sizeof(double));
+    sp->id = calloc(n, sizeof(int32_t));
+    sp->add_x1 = calloc(n, sizeof(double)); sp->add_x2 = calloc(n, sizeof(double));
+    sp->add_q = calloc(n, sizeof(double));
+    sp->qm = c->qm;
+    sp->qbm = c->qm / c->mass;
+    sp->dt = s->g.dxi;
+    const size_t n1 = (size_t)s->nm * s->ld, nc = (size_t)c->num_theta * s->g.nr;
+    sp->q_own = calloc(n1, sizeof(double));
+    sp->qn = calloc(n1, sizeof(double));
+    sp->cu_own = calloc(3 * n1, sizeof(double));
+    sp->acu_own = calloc(2 * n1, sizeof(double));
+    sp->amu_own = calloc(3 * n1, sizeof(double));
+    sp->q_hist = calloc(((size_t)s->g.nxi + 2) * n1, sizeof(double));
+    sp->rho_hist = calloc(((size_t)s->g.nxi + 2) * n1, sizeof(double));
+    sp->multi_ion = calloc(nc * (c->ion_max + 2), sizeof(double));
+    sp->ion_old = calloc(nc, sizeof(double));
+    sp->ppc_add = calloc(nc, sizeof(int));
+    ora_neutral_renew(s, sp);
+    return isp;
+}
+
+/* ionize_neutral (neutral_class.f03:600-753): per theta bin k and radial
+   cell j, |E| at the cell centre (the mean of nodes j, j+1, modes summed
+   at theta_k; for m > 0 QPAD reads the real part where it means the
+   imaginary one, e_im => e%rf_re(m): restated as is), in GV/m via
+   wp 1.708e-12; where it exceeds 1e-6 and the cell is not fully ionised,
+   the ADK rate of every level, the second-order (as QPAD notes, not quite
+   right) update of the level fractions with the overshoot rules, and the
+   total ion charge rounded to the particles per cell it stands for */
+void ora_neutral_ionize(const ora_sim *s, ora_species *sp, const double *ef)
+{
+    const int nr = s->g.nr, nm = s->nm, ld = s->ld, M = s->g.max_mode;
+    const int nth = sp->cfg.num_theta, L = sp->multi_max, ppc_tot = sp->cfg.ppc1 * sp->cfg.ppc2;
+    const int in = L, ii = L + 1;   /* neutral residue, total ion charge */
+    const double pi2_ntheta = 2.0 * PI_DB / (double)nth, dt = sp->dt, wp = sp->wp;
+    double w_ion[20];
+    for (int k = 1; k <= nth; k++) {
+        const double theta = pi2_ntheta * (double)(k - 1);
+        const double cr = cos(theta), ci = sin(theta);
+        for (int j = 1; j <= nr; j++) {
+            double e1 = 0.5 * (FA(ef, 0, 0, j) + FA(ef, 0, 0, j + 1));
+            double e2 = 0.5 * (FA(ef, 1, 0, j) + FA(ef, 1, 0, j + 1));
+            double e3 = 0.5 * (FA(ef, 2, 0, j) + FA(ef, 2, 0, j + 1));
+            double phr = 1.0, phi = 0.0;
+            for (int m = 1; m <= M; m++) {
+                const double a = phr * cr - phi * ci, b = phr * ci + phi * cr;
+                phr = a;
+                phi = b;
+                const int kr = kre(m), kq = kre(m);   /* the quirk: the real part twice */
+                e1 = e1 + (FA(ef, 0, kr, j) + FA(ef, 0, kr, j + 1)) * phr - (FA(ef, 0, kq, j) + FA(ef, 0, kq, j + 1)) * phi;
+                e2 = e2 + (FA(ef, 1, kr, j) + FA(ef, 1, kr, j + 1)) * phr - (FA(ef, 1, kq, j) + FA(ef, 1, kq, j + 1)) * phi;
+                e3 = e3 + (FA(ef, 2, kr, j) + FA(ef, 2, kr, j + 1)) * phr - (FA(ef, 2, kq, j) + FA(ef, 2, kq, j + 1)) * phi;
+            }
+            (void)nm; (void)ld;
+            const double eij = sqrt(e1 * e1 + e2 * e2 + e3 * e3) * wp * 1.708e-12;
+            double *mi = sp->multi_ion + ((size_t)(k - 1) * nr + (j - 1)) * (L + 2);
+            if (!(eij > 1.0e-6)) continue;
+            if (!(mi[ii] < (double)L)) continue;
+            int shoot = 0;
+            for (int i = 0; i < L; i++) {
+                const double *r = sp->rate + 3 * i;
+                w_ion[i] = r[0] * pow(eij, -r[2]) * exp(-r[1] / eij) / wp;
+            }
+            double cons = mi[in] * w_ion[0] * dt * (1.0 + 0.5 * w_ion[0] * dt);
+            if (cons > mi[in]) {
+                shoot = 1;
+                cons = mi[in];
+            }
+            mi[in] = mi[in] - cons;
+            for (int i = 1; i <= L - 1; i++) {
+                const double inj = cons;
+                double dens_temp = 0.0;
+                if (shoot) {
+                    dens_temp = inj * 0.5;
+                    shoot = 0;
+                }
+                cons = (mi[i - 1] + dens_temp) * w_ion[i] * dt * (1.0 + 0.5 * w_ion[i] * dt);
+                if (cons > mi[i - 1] + dens_temp) {
+                    shoot = 1;
+                    cons = mi[i - 1] + dens_temp;
+                }
+                mi[i - 1] = fmin(mi[i - 1] - cons + inj, 1.0);
+            }
+            mi[L - 1] = fmin(mi[L - 1] + cons, 1.0);
+            mi[ii] = 0.0;
+            for (int i = 1; i <= L; i++) mi[ii] = mi[ii] + (double)i * mi[i - 1];
+            mi[ii] = (double)L / (double)ppc_tot * (double)(int)(mi[ii] * (double)ppc_tot / (double)L + 0.5);
+        }
+    }
+}
+
+/* add_particles (neutral_class.f03:755-837): per theta bin k and radial
+   cell j, as many electrons as the rounded ion charge grew by, at rest on
+   an even radial lattice of the cell at theta_k, charge rn n coef,
+   coef = multi_max sign(q) / (ppc nth); part_add keeps their positions
+   with the opposite charge for the ion deposit.  Ids count the electrons
+   released since the renew. */
+void ora_neutral_add(const ora_sim *s, ora_species *sp)
+{
+    const int nr = s->g.nr, nth = sp->cfg.num_theta, L = sp->multi_max;
+    const int ppc_tot = sp->cfg.ppc1 * sp->cfg.ppc2;
+    const double dr = s->g.dr, dtheta = 2.0 * PI_DB / (double)nth;
+    const double den_lon = 1.0, den_perp = 1.0;
+    const double coef = (double)L * copysign(1.0, sp->cfg.qm) / ((double)ppc_tot * (double)nth);
+    sp->n_add = 0;
+    for (int k = 1; k <= nth; k++)
+        for (int j = 1; j <= nr; j++) {
+            const size_t c = (size_t)(k - 1) * nr + (j - 1);
+            const double *mi = sp->multi_ion + c * (L + 2);
+            const int ppc_add = (int)((mi[L + 1] - sp->ion_old[c]) / (double)L * (double)ppc_tot + 0.5);
+            sp->ppc_add[c] = ppc_add;
+            for (int i = 1; i <= ppc_add; i++) {
+                const double rn = (double)(j - 1) + ((double)i - 0.5) / (double)ppc_add;
+                const double theta = (double)(k - 1) * dtheta;
+                const double x1 = rn * dr * cos(theta), x2 = rn * dr * sin(theta);
+                const int64_t p = sp->npp++;
+                sp->x1[p] = x1;
+                sp->x2[p] = x2;
+                sp->q[p] = rn * den_lon * den_perp * sp->cfg.density * coef;
+                sp->p1[p] = 0.0;
+                sp->p2[p] = 0.0;
+                sp->p3[p] = 0.0;
+                sp->gamma[p] = 1.0;
+                sp->psi[p] = 0.0;
+                sp->id[p] = (int32_t)(sp->n_added++);
+                sp->add_x1[sp->n_add] = x1;
+                sp->add_x2[sp->n_add] = x2;
+                sp->add_q[sp->n_add] = -sp->q[p];
+                sp->n_add++;
+            }
+        }
+}
+
+/* ion_deposit_neutral (neutral_class.f03:904-930): the charge deposit of
+   part_add into rho_ion_add, added to rho_ion (kept in qn, which the slice
+   adds to q_spe after the electrons' charge); part_add emptied */
+void ora_neutral_ion_deposit(const ora_sim *s, ora_species *sp)
+{
+    const size_t n1 = (size_t)s->nm * s->ld;
+    double *add = calloc(n1, sizeof(double));
+    ora_species buf;
+    memset(&buf, 0, sizeof(buf));
+    buf.npp = sp->n_add;
+    buf.x1 = sp->add_x1;
+    buf.x2 = sp->add_x2;
+    buf.q = sp->add_q;
+    ora_qdeposit(s, &buf, add);
+    for (size_t t = 0; t < n1; t++) sp->qn[t] = sp->qn[t] + add[t];
+    free(add);
+    sp->n_add = 0;
 }
 
 int ora_add_beam(ora_sim *s, const ora_beam_cfg *cfg, int64_t npmax)
@@ -1941,7 +2228,12 @@ void ora_slice(ora_sim *s, int j, int *n_iter)
         memset(sp->q_own, 0, n1 * sizeof(double));
         ora_qdeposit(s, sp, sp->q_own);
         add_into(q_spe, sp->q_own, n1);
-        if (sp->cfg.neutralized) add_into(q_spe, sp->qn, n1);
+        if (sp->is_neutral) {                                            /* neutral%ion_deposit */
+            ora_neutral_ion_deposit(s, sp);
+            add_into(q_spe, sp->qn, n1);
+        } else if (sp->cfg.neutralized) {
+            add_into(q_spe, sp->qn, n1);
+        }
     }
     TSTOP(ORA_T_DEPOSIT_2D);
     ora_solve_psi(s, q_spe, psi);
@@ -2003,13 +2295,15 @@ void ora_slice(ora_sim *s, int j, int *n_iter)
 
     if (s->laser_on) {                                                   /* sim_lasers%deposit_chi */
         memset(s->chi, 0, n1 * sizeof(double));
-        for (int k = 0; k < s->nspecies; k++) ora_deposit_chi(s, &s->sp[k], s->chi);
+        for (int k = 0; k < s->nspecies; k++)
+            if (!s->sp[k].is_neutral) ora_deposit_chi(s, &s->sp[k], s->chi);
         memcpy(s->h_chi + (size_t)j * n1, s->chi, n1 * sizeof(double));
     }
     for (int k = 0; k < s->nspecies; k++) {                              /* species2d%cbq */
         ora_species *sp = &s->sp[k];
         for (size_t t = 0; t < n1; t++) sp->q_own[t] = sp->q_own[t] + sp->cu_own[2 * n1 + t];
         memcpy(sp->q_hist + (size_t)j * n1, sp->q_own, n1 * sizeof(double));
+        if (sp->is_neutral) memcpy(sp->rho_hist + (size_t)j * n1, sp->qn, n1 * sizeof(double));
     }
     memcpy(s->h_cu + (size_t)j * 3 * n1, cu, 3 * n1 * sizeof(double));
     for (size_t t = 0; t < n1; t++) q_spe[t] = q_spe[t] + cu[2 * n1 + t];
@@ -2030,6 +2324,12 @@ void ora_slice(ora_sim *s, int j, int *n_iter)
 
     for (int k = 0; k < s->nspecies; k++) {
         ora_species *sp = &s->sp[k];
+        if (sp->is_neutral) {                                            /* neutral%update */
+            const int nc = sp->cfg.num_theta * s->g.nr, L = sp->multi_max;
+            for (int c = 0; c < nc; c++) sp->ion_old[c] = sp->multi_ion[(size_t)c * (L + 2) + L + 1];
+            ora_neutral_ionize(s, sp, e);
+            ora_neutral_add(s, sp);
+        }
         const int pt = sp->cfg.push_type;
         if (pt == ORA_PUSH2_STD) ora_push_u_std(s, sp, e, b, s->g.dxi);
         else if (pt == ORA_PUSH2_STD_PGC || pt == ORA_PUSH2_ROBUST_PGC) ora_push_u_pgc(s, sp, e, b, s->g.dxi);
@@ -2056,6 +2356,10 @@ void ora_step3d_end(ora_sim *s, int istep)
     for (int ib = 0; ib < s->nbeams; ib++) ora_beam_push(s, ib);
     TSTOP(ORA_T_PUSH_3D);
     for (int k = 0; k < s->nspecies; k++) {
+        if (s->sp[k].is_neutral) {
+            ora_neutral_renew(s, &s->sp[k]);
+            continue;
+        }
         ora_species_inject(s, k);
         ora_species_init_q(s, k);
     }

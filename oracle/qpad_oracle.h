@@ -128,7 +128,34 @@ typedef struct {
     double *q_own, *qn;       /* species%q, species%qn slice arrays        */
     double *cu_own, *acu_own, *amu_own;
     double *q_hist;           /* species q history (cbq)                   */
+    /* a neutral gas (neutral_class.f03): the particles are the electrons
+       ionisation has released; qn is the ion charge rho_ion it has left */
+    int is_neutral;
+    int multi_max;            /* ionisation levels followed               */
+    double wp;                /* plasma frequency (SI) of simulation.n0    */
+    const double *rate;       /* [multi_max][3] ADK (C1, C2, C3) per level */
+    double *multi_ion;        /* [n_theta * nr][multi_max + 2], cell
+                                 c = k * nr + j (theta bin k, radial cell
+                                 j): levels 1..multi_max, the neutral
+                                 residue, the discrete total ion charge    */
+    double *ion_old;          /* [n_theta * nr]                            */
+    int *ppc_add;             /* [n_theta * nr] electrons released          */
+    double *add_x1, *add_x2, *add_q;   /* part_add: the released electrons'
+                                          positions and ion charges        */
+    int64_t n_add, n_added;   /* in part_add; released since the renew     */
+    double *rho_hist;         /* rho_ion history [(nxi + 2)][nm][ld]       */
 } ora_species;
+
+/* neutrals(i) section (sim_plasma_class.f03:148-176, fdist2d for the
+   'neutrals' section): uniform profile, element by atomic number (H, He,
+   Li, C, N, O, Ne, Ar), ion_max levels at most, pusher; n0 of the
+   simulation section gives the plasma frequency */
+typedef struct {
+    int ppc1, ppc2, num_theta;
+    double qm, mass, density, den_min;
+    int element, ion_max, push_type;
+    double n0;
+} ora_neutral_cfg;
 
 typedef struct {
     int64_t npp, npmax;
@@ -212,6 +239,15 @@ void ora_amjdeposit_pgc(const ora_sim *s, ora_species *sp, const double *e, cons
 void ora_push_u_pgc(const ora_sim *s, ora_species *sp, const double *e, const double *b, double dt);
 void ora_deposit_chi(const ora_sim *s, const ora_species *sp, double *chi);
 void ora_update_bound(const ora_sim *s, ora_species *sp);
+/* neutrals */
+int ora_add_neutral(ora_sim *s, const ora_neutral_cfg *cfg);
+/* ADK coefficients (C1, C2, C3) of level `level` (1-based) of element Z;
+   NULL where the table has none */
+const double *ora_adk(int element, int level);
+void ora_neutral_ionize(const ora_sim *s, ora_species *sp, const double *e);
+void ora_neutral_add(const ora_sim *s, ora_species *sp);
+void ora_neutral_ion_deposit(const ora_sim *s, ora_species *sp);
+void ora_neutral_renew(const ora_sim *s, ora_species *sp);
 void ora_interp_point(const ora_sim *s, const double *f, int ncomp,
                       double x1, double x2, double *out);
 

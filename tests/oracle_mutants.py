@@ -139,6 +139,24 @@ MUTANTS = {
                           "const double corr = 8.0;", ["deposit_chi"]),
     "chi_uses_charge_not_q_over_m": ("double phr = -1.0 * sp->qbm * sp->q[p]",
                                      "double phr = -1.0 * sp->qm * sp->q[p]", ["deposit_chi"]),
+    "adk_row_digit": ("{2.7226733893691e21, 2742.1316798375965, 1.0000920088118899}",
+                      "{2.7226733893691e21, 2724.1316798375965, 1.0000920088118899}", ["adk_table"]),
+    "ionize_first_order_only": ("double cons = mi[in] * w_ion[0] * dt * (1.0 + 0.5 * w_ion[0] * dt);",
+                                "double cons = mi[in] * w_ion[0] * dt;", ["neutral_ionize"]),
+    "ionize_no_overshoot_cap": ("            if (cons > mi[in]) {\n                shoot = 1;\n                cons = mi[in];",
+                                "            if (cons > 2.0 * mi[in]) {\n                shoot = 1;\n                cons = mi[in];",
+                                ["neutral_ionize"]),
+    "ionize_reads_imaginary_part": ("const int kr = kre(m), kq = kre(m);", "const int kr = kre(m), kq = kim(m);",
+                                    ["neutral_ionize"]),
+    "ionize_total_truncated": ("(double)(int)(mi[ii] * (double)ppc_tot / (double)L + 0.5);",
+                               "(double)(int)(mi[ii] * (double)ppc_tot / (double)L);", ["neutral_ionize"]),
+    "add_lattice_cell_edge": ("const double rn = (double)(j - 1) + ((double)i - 0.5) / (double)ppc_add;",
+                              "const double rn = (double)(j - 1) + ((double)i - 1.0) / (double)ppc_add;",
+                              ["neutral_add"]),
+    "add_charge_without_levels": ("const double coef = (double)L * copysign(1.0, sp->cfg.qm)",
+                                  "const double coef = 1.0 * copysign(1.0, sp->cfg.qm)", ["neutral_add"]),
+    "ion_charge_same_sign": ("sp->add_q[sp->n_add] = -sp->q[p];", "sp->add_q[sp->n_add] = sp->q[p];",
+                             ["neutral_add"]),
     "update_bound_skips_one": ("while (i < sp->npp - 1) {\n        double pos = sqrt(sp->x1[i]",
                                "while (i < sp->npp - 2) {\n        double pos = sqrt(sp->x1[i]",
                                ["update_bound"]),

@@ -1190,6 +1190,176 @@ def pin_deposit_chi(L):
     assert np.max(np.abs(both[0, :, 2:] - want)) <= 1e-14, "the first species is rescaled twice"
 
 
+# ---------------------------------------------------------------------------
+# neutral gas and ionisation
+# ---------------------------------------------------------------------------
+ADK_ELEMENTS = {1: 1, 2: 2, 3: 3, 6: 6, 7: 7, 8: 8, 10: 10, 18: 18}
+E_ATOMIC = 514.220674763      # GV/m
+W_ATOMIC = 4.1341373336e16    # 1/s
+
+
+def pin_adk_table(L):
+    """Every ADK row (C1, C2, C3) is the tunnelling rate (l = 0, m = 0) of its
+    charge state i for some ionisation energy U: with n* = (C3 + 1)/2 and
+    U/U_H = (3 C2 / (2 E_a))^(2/3), n* sqrt(U/U_H) equals i, and C1 equals
+    w_a 2^(2n*) / (n* Gamma(2n*)) (U / 2U_H) (2 (U/U_H)^(3/2) E_a)^(2n* - 1);
+    the energies increase with i.  Beyond the table: NULL."""
+    for el, n in ADK_ELEMENTS.items():
+        prev = 0.0
+        for i in range(1, n + 1):
+            r = L.ora_adk(el, i)
+            assert r, (el, i)
+            c1, c2, c3 = r[0], r[1], r[2]
+            ns = 0.5 * (c3 + 1.0)
+            u = (1.5 * c2 / E_ATOMIC) ** (2.0 / 3.0)
+            assert abs(ns * math.sqrt(u) - i) <= 1e-5 * i, ("charge state", el, i, ns * math.sqrt(u))
+            want = W_ATOMIC * 2.0 ** (2 * ns) / (ns * math.gamma(2 * ns)) * (0.5 * u) \
+                * (2.0 * u ** 1.5 * E_ATOMIC) ** (2 * ns - 1)
+            assert abs(c1 / want - 1.0) <= 1e-6, ("C1", el, i, c1 / want)
+            assert u > prev, ("energies increase", el, i)
+            prev = u
+        assert not L.ora_adk(el, n + 1)
+    assert not L.ora_adk(11, 1) and not L.ora_adk(4, 1)
+
+
+def _neutral_oracle(L, element, ion_max, M=1, nr=12, nth=4, ppc=(2, 2), n0=1e17):
+    o = po.Oracle(po.make_grid(nr, 4, M, 0.1, 0.05), L)
+    isp = o.add_neutral(po.neutral_cfg(ppc[0], ppc[1], nth, element, ion_max, n0))
+    return o, isp
+
+
+def _adk_rate(L, el, i, e_gv, wp):
+    r = L.ora_adk(el, i)
+    return r[0] * e_gv ** (-r[2]) * math.exp(-r[1] / e_gv) / wp
+
+
+def pin_neutral_ionize(L):
+    """ionize_neutral (neutral_class.f03:600-753).  Uniform E_z = E (m = 0):
+    hydrogen keeps the neutral fraction 1 - w dxi (1 + w dxi / 2) of the
+    ADK rate w at E wp 1.708e-12 GV/m; the level fractions and the residue
+    add to one; the total ion charge is rounded to multi_max / ppc; a field
+    strong enough to overshoot ionises a level completely; a field at or
+    below 1e-6 GV/m ionises nothing.  The cell field is the mean of its two
+    nodes with the modes summed at theta_k, 2 (Re cos m theta - Im sin m
+    theta); QPAD reads the real part for the imaginary one too (e_im =>
+    e%rf_re), so an imaginary m = 1 part adds nothing and a real one adds
+    S (cos theta - sin theta), S the sum of its two nodes."""
+    n0, dxi = 1e17, 0.05
+    wp = math.sqrt(n0) * 5.641460231180626e4
+    # hydrogen in a uniform E_z, chosen so that the rounding of the total
+    # ion charge rounds up (fraction of a unit in 0.6 .. 0.9)
+    def neu_of(ez):
+        w = _adk_rate(L, 1, 1, ez * wp * 1.708e-12, wp)
+        return 1.0 - w * dxi * (1.0 + 0.5 * w * dxi)
+
+    ez = next(x for x in np.linspace(0.5, 3.0, 2501)
+              if 0.05 < 1.0 - neu_of(x) < 0.95 and 0.6 < ((1.0 - neu_of(x)) * 6) % 1.0 < 0.9)
+    o, isp = _neutral_oracle(L, 1, 1, ppc=(2, 3))
+    e = o.new_field(3)
+    e[2, 0, :] = ez
+    L.ora_neutral_ionize(o.h, o.species_raw(isp), po.ptr(e))
+    lv = o.neutral_levels(isp)
+    w = _adk_rate(L, 1, 1, ez * wp * 1.708e-12, wp)
+    neu = 1.0 - w * dxi * (1.0 + 0.5 * w * dxi)
+    assert 0.0 < 1.0 - neu < 1.0, "the test field must ionise part of the gas"
+    assert np.allclose(lv[:, :, 1], neu, rtol=1e-12, atol=0), ("neutral residue", lv[0, 0], neu)
+    assert np.allclose(lv[:, :, 0], 1.0 - neu, rtol=1e-12, atol=0)
+    assert np.all(lv[:, :, 2] == math.floor((1.0 - neu) * 6 + 0.5) / 6.0), ("discrete total", lv[0, 0, 2])
+    # helium, two levels, weak field: fractions add to one, first order in w1
+    o, isp = _neutral_oracle(L, 2, 2, ppc=(2, 2))
+    e = o.new_field(3)
+    ez = 2.2
+    e[2, 0, :] = ez
+    L.ora_neutral_ionize(o.h, o.species_raw(isp), po.ptr(e))
+    lv = o.neutral_levels(isp)
+    w1 = _adk_rate(L, 2, 1, ez * wp * 1.708e-12, wp)
+    assert np.allclose(lv[:, :, 0] + lv[:, :, 1] + lv[:, :, 2], 1.0, rtol=0, atol=1e-15), "fractions add to one"
+    assert 1e-6 < w1 * dxi < 0.1
+    assert np.allclose(lv[:, :, 0], w1 * dxi * (1 + 0.5 * w1 * dxi), rtol=1e-12), "first level"
+    # overshoot: where w dxi (1 + w dxi / 2) exceeds the neutral fraction
+    # (here between 1.05 and 1.5) the whole residue ionises
+    ez = next(x for x in np.linspace(1.0, 400.0, 40001) if 1.05 < 1.0 - neu_of(x) < 1.5)
+    o, isp = _neutral_oracle(L, 1, 1)
+    e = o.new_field(3)
+    e[2, 0, :] = ez
+    L.ora_neutral_ionize(o.h, o.species_raw(isp), po.ptr(e))
+    lv = o.neutral_levels(isp)
+    assert np.all(lv[:, :, 1] == 0.0) and np.all(lv[:, :, 0] == 1.0) and np.all(lv[:, :, 2] == 1.0)
+    # below the threshold nothing happens
+    o, isp = _neutral_oracle(L, 1, 1)
+    e = o.new_field(3)
+    e[2, 0, :] = 0.5e-6 / (wp * 1.708e-12)
+    L.ora_neutral_ionize(o.h, o.species_raw(isp), po.ptr(e))
+    assert np.all(o.neutral_levels(isp)[:, :, 1] == 1.0)
+    # m = 1: the cell mean of the nodes, 2 Re cos(theta_k); the imaginary part unread
+    nth = 4
+    for part in ("re", "im"):
+        o, isp = _neutral_oracle(L, 1, 1, M=1, nth=nth, ppc=(1, 1))
+        e = o.new_field(3)
+        nodes_r = np.arange(o.ld) * 1.0
+        e[2, kre(1) if part == "re" else kim(1), :] = 0.4 + 0.01 * nodes_r   # varies from node to node
+        L.ora_neutral_ionize(o.h, o.species_raw(isp), po.ptr(e))
+        lv = o.neutral_levels(isp)
+        for k in range(nth):
+            th = 2 * math.pi / nth * k
+            for j in (1, 5):
+                if part == "im":
+                    assert lv[k, j - 1, 1] == 1.0, "imaginary m = 1 part read"
+                    continue
+                # 2 (Re cos - Im sin) with the real part read for both
+                ecell = ((0.4 + 0.01 * j) + (0.4 + 0.01 * (j + 1))) * (math.cos(th) - math.sin(th))
+                eg = abs(ecell) * wp * 1.708e-12
+                if eg <= 1e-6:
+                    assert lv[k, j - 1, 1] == 1.0
+                    continue
+                w = _adk_rate(L, 1, 1, eg, wp)
+                want = 1.0 - w * dxi * (1 + 0.5 * w * dxi)
+                assert abs(lv[k, j - 1, 1] - max(want, 0.0)) <= 1e-12, (k, j, lv[k, j - 1, 1], want)
+
+
+def pin_neutral_add(L):
+    """add_particles (neutral_class.f03:755-837): where the rounded ion charge
+    of cell (theta_k, j) grew by D, round(D / multi_max ppc) electrons at
+    rest on the radial lattice (j - 1 + (i - 1/2)/n) dr at theta_k, each of
+    charge rn density multi_max sign(q) / (ppc num_theta), ids counting up;
+    part_add holds the same positions with the opposite charges, and their
+    charge deposit cancels the electrons' exactly (ionisation is neutral)."""
+    nr, nth = 12, 4
+    o, isp = _neutral_oracle(L, 3, 3, nr=nr, nth=nth, ppc=(2, 3))
+    ppc = 6
+    lv = o.neutral_levels(isp)
+    # prescribe the totals: cell (k, j) grows by (k + j) % 4 units of multi_max / ppc
+    grow = np.array([[(k + j) % 4 for j in range(nr)] for k in range(nth)])
+    lv[:, :, 4] = grow * 3.0 / ppc
+    sp = o.species_raw(isp).contents
+    L.ora_neutral_add(o.h, o.species_raw(isp))
+    assert np.array_equal(o.neutral_ppc_add(isp), grow), "electrons per cell"
+    s = o.species(isp)
+    assert len(s["x1"]) == grow.sum() == sp.n_add
+    assert np.array_equal(s["id"], np.arange(grow.sum()))
+    p = 0
+    dr = o.grid.dr
+    for k in range(nth):
+        th = 2 * math.pi / nth * k
+        for j in range(1, nr + 1):
+            n = grow[k, j - 1]
+            for i in range(1, n + 1):
+                rn = (j - 1) + (i - 0.5) / n
+                assert abs(s["x1"][p] - rn * dr * math.cos(th)) <= 1e-15 and abs(s["x2"][p] - rn * dr * math.sin(th)) <= 1e-15
+                assert abs(s["q"][p] - rn * 1.0 * 3 * -1.0 / (ppc * nth)) <= 1e-15
+                assert s["p1"][p] == 0 and s["p3"][p] == 0 and s["gamma"][p] == 1 and s["psi"][p] == 0
+                p += 1
+    ax = np.ctypeslib.as_array(sp.add_x1, shape=(p,))
+    aq = np.ctypeslib.as_array(sp.add_q, shape=(p,))
+    assert np.array_equal(ax, s["x1"]) and np.array_equal(aq, -s["q"])
+    qe = o.new_field(1)
+    L.ora_qdeposit(o.h, o.species_raw(isp), po.ptr(qe))
+    L.ora_neutral_ion_deposit(o.h, o.species_raw(isp))
+    _, qn = o.species_q_own(isp)
+    assert np.max(np.abs(qe + qn)) == 0.0, "ion deposit = - electron deposit"
+    assert np.max(np.abs(qn)) > 0 and o.species_raw(isp).contents.n_add == 0
+
+
 PINS = {
     "matrices": pin_matrices,
     "ez": pin_ez,
@@ -1211,5 +1381,8 @@ PINS = {
     "amjdeposit_pgc": pin_amjdeposit_pgc,
     "push_pgc": pin_push_pgc,
     "deposit_chi": pin_deposit_chi,
+    "adk_table": pin_adk_table,
+    "neutral_ionize": pin_neutral_ionize,
+    "neutral_add": pin_neutral_add,
 }
 
