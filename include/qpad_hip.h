@@ -216,6 +216,38 @@ int qpad_species_push(qpad_ctx *ctx, int32_t sp, double dxi);
    fused forms do it at the same point. */
 int qpad_species_interp_psi(qpad_ctx *ctx, int32_t sp);
 
+/* ---- neutral gas and field ionisation ----
+   neutrals(i) section (sim_plasma_class.f03:148-176, neutral_class.f03:
+   408-574): a uniform gas of `element` (atomic number 1, 2, 3, 6, 7, 8, 10
+   or 18) followed up to ion_max ionisation levels, ppc and num_theta as a
+   species (the release lattice), q and m of its electrons, push type
+   QPAD_PUSH2_STD or QPAD_PUSH2_ROBUST; n0 (simulation.n0, cm^-3) gives the
+   plasma frequency of the ADK rates. */
+typedef struct {
+    int32_t ppc1, ppc2, num_theta;
+    double qm, mass, density, den_min;
+    int32_t element, ion_max, push_type;
+    double n0;
+} qpad_neutral_cfg;
+/* neutral%new: a species whose particles are the electrons the field
+   releases (none yet; qpad_species_* calls take its id, qpad_get_species_qn
+   its ion charge rho_ion); the fused slice ionises it with the slice's
+   final E and deposits the ions at the next slice's start */
+int qpad_neutral_new(qpad_ctx *ctx, const qpad_neutral_cfg *cfg, int32_t *sp_id);
+/* neutral%ion_deposit(q_spe) (neutral_class.f03:904-930): rho_ion += the ion
+   charge of the electrons the last update released; q_spe += rho_ion */
+int qpad_neutral_ion_deposit(qpad_ctx *ctx, int32_t sp);
+/* neutral%update(e, psi, s) (neutral_class.f03:576-598, 600-837): the level
+   fractions from E (QPAD_F_E), and the released electrons appended at rest */
+int qpad_neutral_update(qpad_ctx *ctx, int32_t sp);
+/* the level fractions [num_theta][nr][ion_max + 2] (levels 1..ion_max, the
+   neutral residue, the discrete total ion charge; neutral%multi_ion) */
+int qpad_get_neutral_levels(qpad_ctx *ctx, int32_t sp, double *out);
+/* rho_ion of history slice s (neutral%cbq, neutral_class.f03:1158-1176) */
+int qpad_get_ion_density(qpad_ctx *ctx, int32_t sp, int32_t s, double *out);
+/* the ADK coefficients (C1, C2, C3) of a level (neutral_class.f03:23-299) */
+int qpad_neutral_adk(int32_t element, int32_t level, double *out);
+
 /* ---- laser (PGC pushers) ----
    The laser envelope the PGC pushers push in: the real and imaginary part
    of the complex envelope a of every slice, [(nxi + 3)][nm][nr + 2] each,

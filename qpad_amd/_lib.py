@@ -59,6 +59,13 @@ class SeedNetInfo(C.Structure):
                 ("nparams", C.c_int64), ("steps", C.c_int64)]
 
 
+class NeutralCfg(C.Structure):
+    _fields_ = [("ppc1", C.c_int32), ("ppc2", C.c_int32), ("num_theta", C.c_int32),
+                ("qm", C.c_double), ("mass", C.c_double), ("density", C.c_double),
+                ("den_min", C.c_double), ("element", C.c_int32), ("ion_max", C.c_int32),
+                ("push_type", C.c_int32), ("n0", C.c_double)]
+
+
 class LaunchShape(C.Structure):
     _fields_ = [("dep_cells", C.c_int32), ("dep_mfma", C.c_int32)]
 
@@ -109,6 +116,12 @@ SIGNATURES = {
     "qpad_species_push": (C.c_int, [_CTX, C.c_int32, C.c_double]),
     "qpad_species_interp_psi": (C.c_int, [_CTX, C.c_int32]),
     "qpad_set_laser": (C.c_int, [_CTX, _dp, _dp]),
+    "qpad_neutral_new": (C.c_int, [_CTX, C.POINTER(NeutralCfg), C.POINTER(C.c_int32)]),
+    "qpad_neutral_ion_deposit": (C.c_int, [_CTX, C.c_int32]),
+    "qpad_neutral_update": (C.c_int, [_CTX, C.c_int32]),
+    "qpad_get_neutral_levels": (C.c_int, [_CTX, C.c_int32, _dp]),
+    "qpad_get_ion_density": (C.c_int, [_CTX, C.c_int32, C.c_int32, _dp]),
+    "qpad_neutral_adk": (C.c_int, [C.c_int32, C.c_int32, _dp]),
     "qpad_laser_slice": (C.c_int, [_CTX, C.c_int32]),
     "qpad_deposit_chi": (C.c_int, [_CTX, C.c_int32]),
     "qpad_step_iterations": (C.c_int, [_CTX, C.POINTER(C.c_int32), C.POINTER(C.c_int32)]),

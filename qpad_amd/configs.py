@@ -115,6 +115,54 @@ def small(nr=64, nxi=48, max_mode=1, ppc=(2, 4), total_num=1 << 14, rmax=4.0, dx
                  _beam(nxi * dxi, sigma_r, sigma_xi, n_peak, total_num, offset_x=offset_x), nstep=1)
 
 
+def ionization(nr=48, nxi=40, max_mode=1, element=3, ion_max=3, ppc=(2, 2), num_theta=8, plasma=False,
+               n_peak=6.0, sigma_r=0.25, total_num=1 << 12, n0=1.0e17, push_type="robust", offset_x=0.0):
+    """A reduced ionisation deck (QPAD's input_file/ionization example in
+    small): a uniform neutral gas of `element` followed up to ion_max
+    levels behind a Gaussian driver, with (plasma) or without a plasma
+    species; n0 sets the field unit of the ADK rates."""
+    species = [_electrons(2, 4, push_type="robust")] if plasma else []
+    d = _deck(nr, 4.0, max_mode, nxi, 0.05, species,
+              _beam(nxi * 0.05, sigma_r, 0.4, n_peak, total_num, offset_x=offset_x), nstep=1)
+    d["simulation"]["n0"] = n0
+    d["simulation"]["nneutrals"] = 1
+    d["neutrals"] = [{"profile": ["uniform", "uniform"], "ppc": list(ppc), "num_theta": num_theta,
+                      "q": -1.0, "m": 1.0, "density": 1.0, "element": element, "ion_max": ion_max,
+                      "push_type": push_type}]
+    return d
+
+
+def ionization_example():
+    """QPAD's own ionisation example, restated as data
+    (input_file/ionization/qpinput.json, diagnostics left out): Nr 500,
+    r_max 5, Nxi 1000 over [0, 10], modes 0..1, n0 1e17, no plasma species,
+    a lithium gas (ion_max 3, ppc 8 x 8, 32 theta bins, robust pusher), a
+    lattice tri-Gaussian beam (n_b/n_p = 4, sigma (0.25, 0.25, 0.5), quiet
+    start), one 3D step of dt 10."""
+    return {
+        "simulation": {
+            "nodes": [1, 2], "grid": [500, 1000], "box": {"r": [0.0, 5.0], "z": [0.0, 10.0]},
+            "field_boundary": "open", "max_mode": 1, "interpolation": "linear", "n0": 1.0e17,
+            "time": 10.1, "dt": 10.0, "nbeams": 1, "nspecies": 0, "nneutrals": 1, "nlasers": 0,
+            "iter_max": 10, "iter_reltol": 1.0e-3, "iter_abstol": 1.0e-3, "random_seed": 10,
+            "algorithm": "standard",
+        },
+        "beam": [{
+            "geometry": "cartesian", "profile": ["gaussian", "gaussian", "gaussian"], "evolution": True,
+            "push_type": "reduced", "has_spin": False, "ppc": [1, 1, 1], "num_theta": 16,
+            "npmax": 2000000, "q": -1.0, "m": 1.0, "gamma": 20000, "density": 4.0, "quiet_start": True,
+            "gauss_center": [0.0, 0.0, 2.5], "gauss_sigma": [0.25, 0.25, 0.5],
+            "range1": [-1.25, 1.25], "range2": [-1.25, 1.25], "range3": [0.0, 5.0],
+            "uth": [5.0, 5.0, 0.0], "den_min": 1e-10,
+        }],
+        "species": [],
+        "neutrals": [{
+            "profile": ["uniform", "uniform"], "ppc": [8, 8], "num_theta": 32, "q": -1.0, "m": 1.0,
+            "density": 1.0, "element": 3, "ion_max": 3, "push_type": "robust",
+        }],
+    }
+
+
 def tri_gaussian():
     """QPAD's own blowout-regime example, restated as data
     (input_file/blowout_regime/qpinput_tri-gaussian.json, diagnostics and

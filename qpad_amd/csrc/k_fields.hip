@@ -155,7 +155,7 @@ static PsiBzArgs psibz_base(qpad_ctx *c)
     for (int s = 0; s < kMaxSpecies; s++) {
         a.P0[s] = s < c->nspecies ? c->sp[s].P0 : nullptr;
         a.P1[s] = s < c->nspecies ? c->sp[s].P1 : nullptr;
-        a.qn[s] = (s < c->nspecies && c->sp[s].cfg.neutralized) ? c->sp[s].qn : nullptr;
+        a.qn[s] = (s < c->nspecies && (c->sp[s].cfg.neutralized || c->sp[s].neutral)) ? c->sp[s].qn : nullptr;
         a.O0[s] = s < c->nspecies ? c->sp[s].O0 : nullptr;
         a.O1[s] = s < c->nspecies ? c->sp[s].O1 : nullptr;
         a.os[s] = s < c->nspecies ? c->sp[s].ov_start : nullptr;

@@ -125,6 +125,7 @@ int launch_deposit_chi(qpad_ctx *c, int j)
     QPAD_TRY(launch_fill(c, chi, n1, 0.0));
     for (int s = 0; s < c->nspecies; s++) {
         Species &sp = c->sp[s];
+        if (sp.neutral) continue;   // sim_lasers%deposit_chi takes the plasma species only
         const DepRows rows{sp.X0, sp.X1, sp.XO0, sp.XO1};
         QPAD_TRY(launch_chi_deposit_rows(c, s, rows));
         const double ppc = (double)sp.cfg.ppc1;

@@ -884,7 +884,8 @@ int launch_slice_sources(qpad_ctx *c)
     for (int q = 0; q < c->nspecies; q++) {
         s.P0[q] = c->sp[q].P0;
         s.P1[q] = c->sp[q].P1;
-        s.qn[q] = c->sp[q].cfg.neutralized ? c->sp[q].qn : nullptr;
+        // the neutralising background, or a neutral gas's ion charge
+        s.qn[q] = (c->sp[q].cfg.neutralized || c->sp[q].neutral) ? c->sp[q].qn : nullptr;
         s.O0[q] = c->sp[q].O0;
         s.O1[q] = c->sp[q].O1;
         s.os[q] = c->sp[q].ov_start;

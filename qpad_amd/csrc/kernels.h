@@ -120,6 +120,19 @@ int launch_laser_slice(qpad_ctx *c, int j);
 // history slice j
 int launch_deposit_chi(qpad_ctx *c, int j);
 
+// ---- neutral gas (k_neutral.hip) ----
+int neutral_scan_bytes(int ncells, size_t *bytes);
+// every cell back to the neutral gas, no ion charge
+int launch_neutral_reset(qpad_ctx *c, int sp);
+// rho_ion (qn) += the ion charge of the electrons released at the end of
+// the previous slice
+int launch_ion_deposit(qpad_ctx *c, int sp);
+// levels from the slice's final E, the released electrons appended (the
+// host waits for their count, which sizes the push and the sort)
+int launch_neutral_update(qpad_ctx *c, int sp);
+// the ADK row (C1, C2, C3) of level `level` of element Z, or null
+const double *adk_row(int element, int level);
+
 // ---- beam (k_beam.hip) ----
 int launch_beam_inject(qpad_ctx *c, int ib);
 // lattice beam (profile "standard"): lattice_count flags and numbers the

@@ -237,6 +237,26 @@ static void species_free(Species &sp)
     dfree(sp.hist); dfree(sp.tot);
     dfree(sp.qsl); dfree(sp.h_q);
     dfree(sp.X0); dfree(sp.X1); dfree(sp.XO0); dfree(sp.XO1);
+    dfree(sp.rate); dfree(sp.levels); dfree(sp.nadd); dfree(sp.noff);
+    dfree(sp.add_x1); dfree(sp.add_x2); dfree(sp.add_q); dfree(sp.nstate); dfree(sp.h_rho);
+    if (sp.scan_tmp) (void)hipFree(sp.scan_tmp);
+    sp.scan_tmp = nullptr;
+    if (sp.nstate_host) (void)hipHostFree(sp.nstate_host);
+    sp.nstate_host = nullptr;
+}
+
+// a species back to its state at the 3D-step start: the plasma injected
+// (and its charge set up), or a neutral gas with nothing ionised
+static int species_restart(qpad_ctx *c, int s)
+{
+    Species &sp = c->sp[s];
+    if (!sp.neutral) {
+        TRY(launch_inject(c, s));
+        return species_charge_setup(c, s);
+    }
+    sp.npp = 0;
+    TRY(launch_neutral_reset(c, s));
+    return launch_sort(c, s);   // no particles: every cell start 0
 }
 
 // the per-species charge arrays of species s (zeroed)
@@ -264,6 +284,36 @@ static int species_chi_alloc(qpad_ctx *c, int s)
     TRY(dalloc(&sp.X1, (size_t)c->g.nr * nm));
     TRY(dalloc(&sp.XO0, (size_t)sp.n_ov_max * nm));
     return dalloc(&sp.XO1, (size_t)sp.n_ov_max * nm);
+}
+
+// the neutral state of species s (its buffers freed with the species)
+static int neutral_alloc(qpad_ctx *c, int s, const qpad_neutral_cfg &nz)
+{
+    Species &sp = c->sp[s];
+    sp.neutral = true;
+    sp.multi_max = nz.ion_max;
+    sp.wp = sqrt(nz.n0) * 5.641460231180626e4;   // sim_plasma_class.f03:84
+    const int nc = sp.cfg.num_theta * c->g.nr, L = sp.multi_max;
+    TRY(dalloc(&sp.rate, (size_t)3 * L));
+    TRY(dalloc(&sp.levels, (size_t)nc * (L + 2)));
+    TRY(dalloc(&sp.nadd, (size_t)nc + 1));
+    TRY(dalloc(&sp.noff, (size_t)nc + 1));
+    TRY(dalloc(&sp.add_x1, (size_t)sp.npmax));
+    TRY(dalloc(&sp.add_x2, (size_t)sp.npmax));
+    TRY(dalloc(&sp.add_q, (size_t)sp.npmax));
+    TRY(dalloc(&sp.nstate, 3));
+    TRY(dalloc(&sp.h_rho, ((size_t)c->g.nxi + 2) * f1_size(c, 1)));
+    TRY(zero_dev(c, sp.h_rho, ((size_t)c->g.nxi + 2) * f1_size(c, 1)));
+    TRY(neutral_scan_bytes(nc, &sp.scan_bytes));
+    if (hipMalloc(&sp.scan_tmp, std::max<size_t>(sp.scan_bytes, 1)) != hipSuccess)
+        return fail(QPAD_E_NOMEM, "hipMalloc of the neutral scan scratch failed");
+    if (hipHostMalloc((void **)&sp.nstate_host, 3 * sizeof(int32_t), hipHostMallocDefault) != hipSuccess)
+        return fail(QPAD_E_NOMEM, "hipHostMalloc failed");
+    std::vector<double> r(3 * L);
+    for (int i = 0; i < L; i++)
+        for (int t = 0; t < 3; t++) r[3 * i + t] = adk_row(nz.element, i + 1)[t];
+    QPAD_HIP_CHECK(hipMemcpyAsync(sp.rate, r.data(), r.size() * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    return sync(c);
 }
 
 // the laser state (a zero envelope history, the chi history, every
@@ -481,7 +531,22 @@ int qpad_destroy(qpad_ctx *c)
 // visible (nspecies grows) only once all of its buffers exist and its
 // plasma is injected.  On any failure its buffers are freed and the context
 // is left exactly as it was.
+}  // extern "C"
+
+static int species_create(qpad_ctx *c, const qpad_species_cfg *cfg, const qpad_neutral_cfg *nz, int32_t *sp_id);
+
+extern "C" {
+
 int qpad_species_new(qpad_ctx *c, const qpad_species_cfg *cfg, int32_t *sp_id)
+{
+    return species_create(c, cfg, nullptr, sp_id);
+}
+
+}  // extern "C"
+
+// a plasma species (nz null) or a neutral gas (nz: its element, levels and
+// n0)
+static int species_create(qpad_ctx *c, const qpad_species_cfg *cfg, const qpad_neutral_cfg *nz, int32_t *sp_id)
 {
     if (!c || !cfg) return fail(QPAD_E_INVALID, "null argument");
     if (c->nspecies >= kMaxSpecies) return fail(QPAD_E_INVALID, "too many species");
@@ -529,8 +594,8 @@ int qpad_species_new(qpad_ctx *c, const qpad_species_cfg *cfg, int32_t *sp_id)
     if (!rc) rc = zero_dev(c, sp.qn, f1_size(c, 1));
     if (!rc && c->species_q) rc = species_q_alloc(c, s);
     if (!rc && (c->laser || is_pgc(cfg->push_type))) rc = species_chi_alloc(c, s);
-    if (!rc) rc = launch_inject(c, s);
-    if (!rc) rc = species_charge_setup(c, s);
+    if (!rc && nz) rc = neutral_alloc(c, s, *nz);
+    if (!rc) rc = species_restart(c, s);
     if (!rc) rc = sync(c);
     // a PGC species pushes in the laser envelope: a zero one until
     // qpad_set_laser (sim_lasers exists with nlasers = 0)
@@ -550,12 +615,40 @@ int qpad_species_new(qpad_ctx *c, const qpad_species_cfg *cfg, int32_t *sp_id)
     return 0;
 }
 
+extern "C" {
+
 int qpad_species_renew(qpad_ctx *c, int32_t s)
 {
     if (!c || s < 0 || s >= c->nspecies) return fail(QPAD_E_INVALID, "bad species");
-    TRY(launch_inject(c, s));
-    TRY(species_charge_setup(c, s));
+    TRY(species_restart(c, s));
     return sync(c);
+}
+
+// A neutral gas: a species of push type cfg->push_type with no particles
+// until the field ionises it.  Checks first; on failure nothing changes.
+int qpad_neutral_new(qpad_ctx *c, const qpad_neutral_cfg *cfg, int32_t *sp_id)
+{
+    if (!c || !cfg) return fail(QPAD_E_INVALID, "null argument");
+    if (cfg->push_type != QPAD_PUSH2_STD && cfg->push_type != QPAD_PUSH2_ROBUST)
+        return fail(QPAD_E_INVALID, "neutrals push with QPAD_PUSH2_STD or QPAD_PUSH2_ROBUST");
+    if (cfg->ion_max < 1 || cfg->ion_max > 20 || !adk_row(cfg->element, cfg->ion_max))
+        return fail(QPAD_E_INVALID, "neutral: element must be 1, 2, 3, 6, 7, 8, 10 or 18 and ion_max within "
+                                    "its ionisation levels");
+    if (!(cfg->n0 > 0.0)) return fail(QPAD_E_INVALID, "neutral: n0 must be > 0");
+    if (1.0 * 1.0 * cfg->density < cfg->den_min)
+        return fail(QPAD_E_INVALID, "neutral: density below den_min");
+    qpad_species_cfg sc;
+    memset(&sc, 0, sizeof(sc));
+    sc.ppc1 = cfg->ppc1;
+    sc.ppc2 = cfg->ppc2;
+    sc.num_theta = cfg->num_theta;
+    sc.qm = cfg->qm;
+    sc.mass = cfg->mass;
+    sc.density = cfg->density;
+    sc.neutralized = 0;
+    sc.den_min = cfg->den_min;
+    sc.push_type = cfg->push_type;
+    return species_create(c, &sc, cfg, sp_id);
 }
 
 }  // extern "C"
@@ -832,6 +925,10 @@ static int slice_start(qpad_ctx *c, int j, bool overlap, SliceRun &st)
     TRY(launch_qdeposit_all(c));
     if (c->species_q)
         for (int s = 0; s < c->nspecies; s++) TRY(launch_species_q_start(c, s));
+    // neutral%ion_deposit (simulation_class.f03:351-354): rho_ion, which
+    // the sources add to q_spe after the electrons' charge
+    for (int s = 0; s < c->nspecies; s++)
+        if (c->sp[s].neutral) TRY(launch_ion_deposit(c, s));
     // laser_all of slice j (simulation_class.f03:362-367)
     if (c->laser) TRY(launch_laser_slice(c, j));
     TRY(launch_slice_sources(c));
@@ -868,6 +965,16 @@ static int slice_finish(qpad_ctx *c, SliceRun &st)
     // sim_lasers%deposit_chi (simulation_class.f03:400-401): the particles'
     // psi of the last current deposit, before the push
     if (c->laser) TRY(launch_deposit_chi(c, j));
+    // neutral%cbq's rho_ion copy, then neutral%update (simulation_class.f03:
+    // 406-407, 444-445): the levels from the final E and the electrons they
+    // release, which the push below moves with the rest
+    for (int s = 0; s < c->nspecies; s++) {
+        Species &sp = c->sp[s];
+        if (!sp.neutral) continue;
+        const size_t n1 = f1_size(c, 1);
+        TRY(launch_copy(c, sp.h_rho + (size_t)j * n1, sp.qn, n1));
+        TRY(launch_neutral_update(c, s));
+    }
     const bool net = seednet_active(c);
     if (net) TRY(seednet_mark(c));
     TRY(launch_push_all(c));
@@ -912,10 +1019,7 @@ int qpad_step3d_end(qpad_ctx *c)
 {
     if (!c) return fail(QPAD_E_INVALID, "null context");
     for (int b = 0; b < c->nbeams; b++) TRY(launch_beam_push(c, b));
-    for (int s = 0; s < c->nspecies; s++) {
-        TRY(launch_inject(c, s));
-        TRY(species_charge_setup(c, s));
-    }
+    for (int s = 0; s < c->nspecies; s++) TRY(species_restart(c, s));
     c->step_open = false;
     return sync(c);
 }
@@ -1036,6 +1140,10 @@ int qpad_species_cbq(qpad_ctx *c, int32_t s, int32_t pos)
     if (!c->species_q) return fail(QPAD_E_STATE, "per-species charge is off (qpad_species_charge_history)");
     if (pos < 1 || pos > c->g.nxi) return fail(QPAD_E_INVALID, "slice index out of range");
     TRY(launch_species_cbq(c, s, pos));
+    if (c->sp[s].neutral) {   // neutral%cbq also keeps rho_ion (neutral_class.f03:1158-1176)
+        const size_t n1 = f1_size(c, 1);
+        TRY(launch_copy(c, c->sp[s].h_rho + (size_t)pos * n1, c->sp[s].qn, n1));
+    }
     return sync(c);
 }
 
@@ -1106,6 +1214,44 @@ int qpad_deposit_chi(qpad_ctx *c, int32_t j)
     if (!c->laser) return fail(QPAD_E_STATE, "no laser: call qpad_set_laser or add a PGC species first");
     TRY(launch_deposit_chi(c, j));
     return sync(c);
+}
+
+int qpad_neutral_ion_deposit(qpad_ctx *c, int32_t s)
+{
+    if (!c || s < 0 || s >= c->nspecies || !c->sp[s].neutral) return fail(QPAD_E_INVALID, "bad neutral");
+    TRY(launch_ion_deposit(c, s));
+    TRY(launch_add_qn_into_qspe(c, s));
+    return sync(c);
+}
+
+int qpad_neutral_update(qpad_ctx *c, int32_t s)
+{
+    if (!c || s < 0 || s >= c->nspecies || !c->sp[s].neutral) return fail(QPAD_E_INVALID, "bad neutral");
+    return launch_neutral_update(c, s);
+}
+
+int qpad_get_neutral_levels(qpad_ctx *c, int32_t s, double *out)
+{
+    if (!c || s < 0 || s >= c->nspecies || !c->sp[s].neutral || !out) return fail(QPAD_E_INVALID, "bad neutral");
+    const Species &sp = c->sp[s];
+    const size_t n = (size_t)sp.cfg.num_theta * c->g.nr * (sp.multi_max + 2);
+    QPAD_HIP_CHECK(hipMemcpyAsync(out, sp.levels, n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    return sync(c);
+}
+
+int qpad_get_ion_density(qpad_ctx *c, int32_t s, int32_t slice, double *out)
+{
+    if (!c || s < 0 || s >= c->nspecies || !c->sp[s].neutral || !out) return fail(QPAD_E_INVALID, "bad neutral");
+    if (slice < 0 || slice > c->g.nxi + 1) return fail(QPAD_E_INVALID, "slice index out of range");
+    return copy_rows(c, c->sp[s].h_rho + (size_t)slice * f1_size(c, 1), 1, out, nullptr);
+}
+
+int qpad_neutral_adk(int32_t element, int32_t level, double *out)
+{
+    const double *r = adk_row(element, level);
+    if (!r || !out) return fail(QPAD_E_INVALID, "no ADK coefficients for this element and level");
+    for (int t = 0; t < 3; t++) out[t] = r[t];
+    return 0;
 }
 
 int qpad_species_interp_psi(qpad_ctx *c, int32_t s)
@@ -1550,9 +1696,14 @@ int qpad_batch_step3d(qpad_batch *b, int32_t *total_iters_out)
     // the eager modes queue like qpad_step3d does; a captured slice graph
     // holds no such second stream
     if (b->graph == 1)
-        for (qpad_ctx *c : b->ctx)
+        for (qpad_ctx *c : b->ctx) {
             if (seednet_active(c))
                 return fail(QPAD_E_STATE, "graph batches step without the seed network (use an eager mode)");
+            for (int s = 0; s < c->nspecies; s++)
+                if (c->sp[s].neutral)
+                    return fail(QPAD_E_STATE, "graph batches step without neutrals: each slice's release "
+                                              "count comes back to the host (use an eager mode)");
+        }
     for (qpad_ctx *c : b->ctx) {
         TRY(step_begin_enqueue(c));
         memset(c->it_host, 0, sizeof(IterState));

@@ -70,6 +70,23 @@ struct Species {
     // partial rows of the susceptibility deposit (laser contexts only):
     // [nr][nm] and [n_ov_max][nm]
     double *X0, *X1, *XO0, *XO1;
+    // a neutral gas (qpad_neutral_new): the particles are the electrons
+    // ionisation released, qn is the ion charge rho_ion they left
+    bool neutral;
+    int multi_max;           // ionisation levels followed
+    double wp;               // plasma frequency (SI) of the deck's n0
+    double *rate;            // [multi_max][3] ADK (C1, C2, C3)
+    double *levels;          // [num_theta * nr][multi_max + 2]: level fractions,
+                             // neutral residue, discrete total ion charge
+    int32_t *nadd, *noff;    // [num_theta * nr + 1]: electrons released by
+                             // cell in the last update, exclusive prefix
+    double *add_x1, *add_x2, *add_q;   // their positions and ion charges
+    int32_t *nstate;         // live before the last release, released then,
+                             // released since the renew
+    int32_t *nstate_host;    // pinned mirror
+    void *scan_tmp;
+    size_t scan_bytes;
+    double *h_rho;           // rho_ion history [slice 0..nxi+1][nm][ld]
 };
 
 struct Beam {

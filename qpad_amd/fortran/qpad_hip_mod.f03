@@ -46,6 +46,15 @@ module qpad_hip_mod
     integer(c_int32_t) :: push_type
   end type qpad_species_cfg
 
+  ! neutrals(i) of the input deck: element (atomic number) and ion_max set
+  ! the ADK levels followed, n0 the plasma frequency of the ADK rates
+  type, bind(C), public :: qpad_neutral_cfg
+    integer(c_int32_t) :: ppc1, ppc2, num_theta
+    real(c_double) :: qm, mass, density, den_min
+    integer(c_int32_t) :: element, ion_max, push_type
+    real(c_double) :: n0
+  end type qpad_neutral_cfg
+
   integer(c_int), parameter, public :: QPAD_PROF_UNIFORM = 0, QPAD_PROF_GAUSSIAN = 1, &
     QPAD_PROF_PIECEWISE = 2, QPAD_GEOM_CARTESIAN = 0, QPAD_GEOM_CYLINDRICAL = 1, &
     QPAD_PW_MAX = 32, QPAD_OFFSET_MAX = 8
@@ -438,6 +447,53 @@ module qpad_hip_mod
       integer(c_int) :: rc
     end function
 
+    ! neutral%new / %ion_deposit / %update, the level fractions, rho_ion
+    ! history, and the ADK coefficients of a level
+    function qpad_neutral_new(ctx, cfg, sp_id) bind(C, name="qpad_neutral_new") result(rc)
+      import :: c_int, c_int32_t, c_ptr, qpad_neutral_cfg
+      type(c_ptr), value :: ctx
+      type(qpad_neutral_cfg), intent(in) :: cfg
+      integer(c_int32_t), intent(out) :: sp_id
+      integer(c_int) :: rc
+    end function
+
+    function qpad_neutral_ion_deposit(ctx, sp) bind(C, name="qpad_neutral_ion_deposit") result(rc)
+      import :: c_int, c_int32_t, c_ptr
+      type(c_ptr), value :: ctx
+      integer(c_int32_t), value :: sp
+      integer(c_int) :: rc
+    end function
+
+    function qpad_neutral_update(ctx, sp) bind(C, name="qpad_neutral_update") result(rc)
+      import :: c_int, c_int32_t, c_ptr
+      type(c_ptr), value :: ctx
+      integer(c_int32_t), value :: sp
+      integer(c_int) :: rc
+    end function
+
+    function qpad_get_neutral_levels(ctx, sp, out) bind(C, name="qpad_get_neutral_levels") result(rc)
+      import :: c_int, c_int32_t, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      integer(c_int32_t), value :: sp
+      real(c_double), intent(out) :: out(*)
+      integer(c_int) :: rc
+    end function
+
+    function qpad_get_ion_density(ctx, sp, s, out) bind(C, name="qpad_get_ion_density") result(rc)
+      import :: c_int, c_int32_t, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      integer(c_int32_t), value :: sp, s
+      real(c_double), intent(out) :: out(*)
+      integer(c_int) :: rc
+    end function
+
+    function qpad_neutral_adk(element, level, out) bind(C, name="qpad_neutral_adk") result(rc)
+      import :: c_int, c_int32_t, c_double
+      integer(c_int32_t), value :: element, level
+      real(c_double), intent(out) :: out(3)
+      integer(c_int) :: rc
+    end function
+
     function qpad_get_species_qn(ctx, sp, out) bind(C, name="qpad_get_species_qn") result(rc)
       import :: c_int, c_int32_t, c_ptr, c_double
       type(c_ptr), value :: ctx
@@ -643,6 +699,8 @@ module qpad_hip_mod
     qpad_get_field, qpad_set_field, qpad_get_hist, qpad_set_hist, &
     qpad_species_count, qpad_get_species, qpad_set_species, &
     qpad_get_species_qn, qpad_set_species_qn, qpad_beam_count, qpad_get_beam, &
+    qpad_neutral_new, qpad_neutral_ion_deposit, qpad_neutral_update, qpad_get_neutral_levels, &
+    qpad_get_ion_density, qpad_neutral_adk, &
     qpad_set_beam, qpad_kernel_time, qpad_kernel_time_reset, qpad_iteration_policy, &
     qpad_seednet_create, qpad_seednet_get_info, qpad_seednet_mode, qpad_seednet_get_params, &
     qpad_seednet_set_params, qpad_seednet_losses, qpad_seednet_seed, qpad_seednet_learn, &

@@ -13,7 +13,7 @@ import numpy as np
 
 from ._lib import (BND_OPEN, BND_ZERO, GEOM_CARTESIAN, GEOM_CYLINDRICAL, OFFSET_MAX, PROF_GAUSSIAN,
                    PROF_PIECEWISE, PROF_UNIFORM, PUSH2_ROBUST, PUSH2_ROBUST_PGC, PUSH2_STD, PUSH2_STD_PGC,
-                   PW_MAX, BeamCfg, BeamStdCfg, Grid, QpadError, SpeciesCfg)
+                   PW_MAX, BeamCfg, BeamStdCfg, Grid, NeutralCfg, QpadError, SpeciesCfg)
 
 
 def _get(d, key, default=None, required=True):
@@ -65,8 +65,6 @@ def grid_from_deck(deck):
         raise QpadError('Invalid interpolation type! Only "linear" are supported currently.')
     if _get(sim, "algorithm", "standard", required=False) != "standard":
         raise QpadError("only algorithm 'standard' is on the GPU path")
-    if _get(sim, "nneutrals", 0, required=False) != 0:
-        raise QpadError("nneutrals > 0 is not on the GPU path")
     if _get(sim, "nlasers", 0, required=False) != 0:
         raise QpadError("nlasers > 0: the laser solver is not on the GPU path; give the envelope to "
                         "Simulation.set_laser (the PGC pushers and the chi deposit are)")
@@ -82,6 +80,37 @@ def grid_from_deck(deck):
 # species(i).push_type (sim_plasma_class.f03:123-137)
 _PUSHERS = {"standard": PUSH2_STD, "robust": PUSH2_ROBUST, "standard_pgc": PUSH2_STD_PGC,
             "robust_pgc": PUSH2_ROBUST_PGC}
+
+
+# neutrals(i).element: atomic numbers with ADK coefficients
+# (neutral_class.f03:23-299; param.f03:135-148)
+NEUTRAL_ELEMENTS = {1: 1, 2: 2, 3: 3, 6: 6, 7: 7, 8: 8, 10: 10, 18: 18}
+
+
+def neutral_from_deck(nt, n0):
+    """neutrals(i) section (sim_plasma_class.f03:148-176, fdist2d for the
+    'neutrals' section): q, m, element, ion_max, push_type ("standard" or
+    "robust"), ppc, num_theta, density; n0 of the simulation section."""
+    prof = _get(nt, "profile")
+    if list(prof) != ["uniform", "uniform"]:
+        raise QpadError("only uniform neutral profiles are on the GPU path")
+    push = _get(nt, "push_type")
+    if push not in ("standard", "robust"):
+        raise QpadError('Invalid pusher type! Only "std", "robust" are supported currently.')
+    elem = int(_get(nt, "element"))
+    if elem not in NEUTRAL_ELEMENTS:
+        raise QpadError(f"neutral element {elem}: the GPU path has the ADK coefficients of "
+                        f"{sorted(NEUTRAL_ELEMENTS)} (atomic numbers)")
+    ion_max = int(_get(nt, "ion_max"))
+    if not 1 <= ion_max <= NEUTRAL_ELEMENTS[elem]:
+        raise QpadError(f"neutral ion_max {ion_max}: element {elem} has {NEUTRAL_ELEMENTS[elem]} levels")
+    if _get(nt, "random_theta", False, required=False):
+        raise QpadError("random_theta is not on the GPU path")
+    ppc = _get(nt, "ppc")
+    return NeutralCfg(int(ppc[0]), int(ppc[1]), int(_get(nt, "num_theta")), float(_get(nt, "q")),
+                      float(_get(nt, "m")), float(_get(nt, "density")),
+                      float(_get(nt, "den_min", 1.0e-10, required=False)), elem, ion_max,
+                      PUSH2_STD if push == "standard" else PUSH2_ROBUST, float(n0))
 
 
 def species_from_deck(sp, seed=0):

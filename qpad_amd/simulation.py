@@ -11,7 +11,8 @@ import ctypes as C
 import numpy as np
 
 from . import _lib as L
-from .input import FileBeam, beam_from_deck, grid_from_deck, place_file_beam, species_from_deck
+from .input import (FileBeam, beam_from_deck, grid_from_deck, neutral_from_deck, place_file_beam,
+                    species_from_deck)
 
 
 class Field:
@@ -169,6 +170,42 @@ class Species2D:
         L.check(self.sim.lib.qpad_set_species_qn(self.sim.ctx, self.id, L.dptr(a)))
 
 
+class Neutral2D(Species2D):
+    """neutral (neutral_class.f03): a gas ionised by the slice's field; its
+    particles are the electrons ionisation released so far."""
+
+    def __init__(self, sim, cfg):
+        self.sim = sim
+        sid = C.c_int32(0)
+        L.check(sim.lib.qpad_neutral_new(sim.ctx, C.byref(cfg), C.byref(sid)))
+        self.id = sid.value
+        self.cfg = cfg
+        self.neutral = True
+
+    def ion_deposit(self, q):
+        """neutral%ion_deposit(q_spe) (neutral_class.f03:904-930)"""
+        if q.name != "q_spe":
+            raise L.QpadError("neutral%ion_deposit adds into q_spe")
+        L.check(self.sim.lib.qpad_neutral_ion_deposit(self.sim.ctx, self.id))
+
+    def update(self):
+        """neutral%update(e, psi, s) (neutral_class.f03:576-598)"""
+        L.check(self.sim.lib.qpad_neutral_update(self.sim.ctx, self.id))
+
+    def levels(self):
+        """[num_theta, nr, ion_max + 2]: level fractions, neutral residue,
+        discrete total ion charge"""
+        out = np.zeros((self.cfg.num_theta, self.sim.grid.nr, self.cfg.ion_max + 2))
+        L.check(self.sim.lib.qpad_get_neutral_levels(self.sim.ctx, self.id, L.dptr(out)))
+        return out
+
+    def ion_density(self, s):
+        """rho_ion of history slice s"""
+        out = np.zeros((1, self.sim.nm, self.sim.grid.nr + 2))
+        L.check(self.sim.lib.qpad_get_ion_density(self.sim.ctx, self.id, int(s), L.dptr(out)))
+        return out
+
+
 class Beam3D:
     """beam3d (beam3d_class.f03:51-52): a lattice ("standard"), random
     Gaussian or file beam, reduced pusher."""
@@ -294,6 +331,11 @@ class Simulation:
         nbm = deck["simulation"].get("nbeams", len(deck.get("beam", [])))
         self.spe = [Species2D(self, species_from_deck(sp, seed=seed + 1000 * k))
                     for k, sp in enumerate(deck.get("species", [])[:nsp])]
+        # neutrals after the species, as simulation%new creates them
+        nnt = deck["simulation"].get("nneutrals", len(deck.get("neutrals", [])))
+        n0 = float(deck["simulation"].get("n0", 1.0e16))
+        self.neut = [Neutral2D(self, neutral_from_deck(nt, n0)) for nt in deck.get("neutrals", [])[:nnt]]
+        self.spe += self.neut
         self.beam = [Beam3D(self, beam_from_deck(bm, z0, seed=seed + 7 * (k + 1)))
                      for k, bm in enumerate(deck.get("beam", [])[:nbm])]
         self.tstep = 0

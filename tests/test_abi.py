@@ -41,6 +41,7 @@ def test_struct_layout_matches_header(tmp_path):
         "qpad_species_cfg": [f for f, _ in L.SpeciesCfg._fields_],
         "qpad_beam_cfg": [f for f, _ in L.BeamCfg._fields_],
         "qpad_launch_shape": [f for f, _ in L.LaunchShape._fields_],
+        "qpad_neutral_cfg": [f for f, _ in L.NeutralCfg._fields_],
     }
     lines = ['#include <stdio.h>', '#include <stddef.h>', f'#include "{HEADER}"', "int main(void){"]
     for st, fs in fields.items():
@@ -54,7 +55,8 @@ def test_struct_layout_matches_header(tmp_path):
     out = dict(line.rsplit(" ", 1) for line in subprocess.run([str(exe)], check=True, capture_output=True,
                                                                text=True).stdout.split("\n") if line)
     for st, cls in (("qpad_grid", L.Grid), ("qpad_species_cfg", L.SpeciesCfg),
-                    ("qpad_beam_cfg", L.BeamCfg), ("qpad_launch_shape", L.LaunchShape)):
+                    ("qpad_beam_cfg", L.BeamCfg), ("qpad_launch_shape", L.LaunchShape),
+                    ("qpad_neutral_cfg", L.NeutralCfg)):
         assert int(out[st]) == C.sizeof(cls), st
         for f, _ in cls._fields_:
             assert int(out[f"{st}.{f}"]) == getattr(cls, f).offset, f"{st}.{f}"
@@ -101,3 +103,23 @@ def test_simulation_raises_without_device_or_library(monkeypatch):
     monkeypatch.setattr(lib_mod, "_lib", None)
     with pytest.raises(QpadError, match="missing"):
         lib_mod.load("/nonexistent/libqpad_hip.so")
+
+
+def test_adk_table_matches_oracle():
+    """qpad_neutral_adk (a host table read, no GPU) returns the oracle's ADK
+    coefficients for every element and level the path follows; levels
+    beyond an element's charge are refused"""
+    import pyoracle as po
+    from qpad_amd.input import NEUTRAL_ELEMENTS
+    lib = L.load()
+    ol = po.lib()
+    out = (C.c_double * 3)()
+    n = 0
+    for elem, nlev in NEUTRAL_ELEMENTS.items():
+        for lev in range(1, nlev + 1):
+            assert lib.qpad_neutral_adk(elem, lev, out) == 0, (elem, lev)
+            ref = ol.ora_adk(elem, lev)
+            assert [out[t] for t in range(3)] == [ref[t] for t in range(3)], (elem, lev)
+            n += 1
+        assert lib.qpad_neutral_adk(elem, nlev + 1, out) != 0
+    assert n == 55

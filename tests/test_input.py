@@ -7,7 +7,7 @@ import pytest
 
 from qpad_amd import QpadError, configs
 from qpad_amd._lib import BeamStdCfg
-from qpad_amd.input import (FileBeam, beam_from_deck, grid_from_deck, load_deck, place_file_beam,
+from qpad_amd.input import (FileBeam, beam_from_deck, grid_from_deck, load_deck, neutral_from_deck, place_file_beam,
                             species_from_deck, strip_comments, tri_gaussian_charge)
 
 # QPAD's own example decks, where the reference tree is present (this
@@ -100,14 +100,43 @@ def test_reference_decks_load_or_are_refused_clearly():
                 species_from_deck(sp)
             for bm in d.get("beam", [])[:d["simulation"].get("nbeams", 0)]:
                 beam_from_deck(bm, z0)
+            for nt in d.get("neutrals", [])[:d["simulation"].get("nneutrals", 0)]:
+                neutral_from_deck(nt, d["simulation"].get("n0", 1.0e16))
             ok.append(os.path.relpath(path, DECKS))
         except QpadError as e:
             refused[os.path.relpath(path, DECKS)] = str(e)
     assert {"blowout_regime/qpinput_tri-gaussian.json", "blowout_regime/qpinput_sawtooth.json",
-            "hosing/qpinput.json"} <= set(ok), (ok, refused)
+            "hosing/qpinput.json", "ionization/qpinput.json"} <= set(ok), (ok, refused)
     assert refused, "decks with lasers, neutrals or analytic profiles must be refused"
     for name, msg in refused.items():
         assert any(w in msg for w in ("nlasers", "nneutrals", "analytic", "pusher", "profile")), (name, msg)
+
+
+@pytest.mark.skipif(not os.path.isdir(DECKS), reason="QPAD's example decks are not in this tree")
+def test_ionization_config_restates_the_deck():
+    real = load_deck(os.path.join(DECKS, "ionization", "qpinput.json"))
+    mine = configs.ionization_example()
+    for sec in ("beam", "neutrals"):
+        assert len(real[sec]) == len(mine[sec])
+        for a, b in zip(real[sec], mine[sec]):
+            assert {k: v for k, v in a.items() if k != "diag"} == b
+    for k, v in mine["simulation"].items():
+        assert real["simulation"][k] == v, k
+
+
+def test_neutral_section():
+    """neutrals(i) (sim_plasma_class.f03:148-176): element, ion_max, pusher,
+    ppc and num_theta; n0 comes from the simulation section"""
+    d = configs.ionization_example()
+    c = neutral_from_deck(d["neutrals"][0], d["simulation"]["n0"])
+    assert (c.ppc1, c.ppc2, c.num_theta, c.element, c.ion_max) == (8, 8, 32, 3, 3)
+    assert c.push_type == 1 and c.qm == -1.0 and c.n0 == 1.0e17
+    for edit, match in [(dict(element=4), "element 4"), (dict(ion_max=4), "ion_max 4"),
+                        (dict(push_type="robust_pgc"), "pusher"), (dict(random_theta=True), "random_theta"),
+                        (dict(profile=["parabolic-channel", "uniform"]), "uniform")]:
+        nt = dict(d["neutrals"][0], **edit)
+        with pytest.raises(QpadError, match=match):
+            neutral_from_deck(nt, 1e17)
 
 
 @pytest.mark.skipif(not os.path.isdir(DECKS), reason="QPAD's example decks are not in this tree")

@@ -22,6 +22,9 @@ def make_twin(sim):
     o = po.Oracle(og)
     for sp in sim.spe:
         c = sp.cfg
+        if getattr(sp, "neutral", False):
+            o.add_neutral(_as_oracle_struct(po.NeutralCfg, c))
+            continue
         o.add_species(po.species_cfg(c.ppc1, c.ppc2, c.num_theta, c.qm, c.mass, c.density,
                                      tuple(c.uth), bool(c.neutralized), c.den_min, c.seed, c.push_type))
     for b in sim.beam:
@@ -51,7 +54,7 @@ def oracle_from_deck(deck):
     """The oracle set up from a qpinput deck on its own, as qpad_amd.Simulation
     sets up the library: same grid, same species and beam seeds; the oracle
     injects the plasma and the beam itself."""
-    from qpad_amd.input import beam_from_deck, grid_from_deck, species_from_deck
+    from qpad_amd.input import beam_from_deck, grid_from_deck, neutral_from_deck, species_from_deck
     g, _, z0 = grid_from_deck(deck)
     o = po.Oracle(_as_oracle_struct(po.Grid, g))
     sim = deck["simulation"]
@@ -60,6 +63,9 @@ def oracle_from_deck(deck):
     nbm = sim.get("nbeams", len(deck.get("beam", [])))
     for k, sp in enumerate(deck.get("species", [])[:nsp]):
         o.add_species(_as_oracle_struct(po.SpeciesCfg, species_from_deck(sp, seed=seed + 1000 * k)))
+    nnt = sim.get("nneutrals", len(deck.get("neutrals", [])))
+    for nt in deck.get("neutrals", [])[:nnt]:
+        o.add_neutral(_as_oracle_struct(po.NeutralCfg, neutral_from_deck(nt, sim.get("n0", 1.0e16))))
     for k, bm in enumerate(deck.get("beam", [])[:nbm]):
         cfg = beam_from_deck(bm, z0, seed=seed + 7 * (k + 1))
         if type(cfg).__name__ == "FileBeam":
