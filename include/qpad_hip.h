@@ -203,18 +203,46 @@ int qpad_species_cbq(qpad_ctx *ctx, int32_t sp, int32_t pos);
 int qpad_get_species_charge(qpad_ctx *ctx, int32_t sp, int32_t slice, double *out);
 /* species2d%amjdp: deposit cu, acu (dJ/dxi part) and amu of species sp into
    the totals using fields e and b (species2d_class.f03:233-280;
-   part2d_class.f03:746-1010 robust pusher, :478-744 standard pusher) */
+   part2d_class.f03:746-1010 robust pusher, :478-744 standard pusher), with
+   the 2D step dxi (the grid's dxi, or a sub-cycle's dxi / n) */
 int qpad_species_amjdp(qpad_ctx *ctx, int32_t sp, double dxi);
 /* species2d%push_u + %push_x (+ update_bound, move_part2d_comm):
    part2d_class.f03:1879-1965 (robust) or :1790-1877 (standard),
-   :2221-2262, :2307-2353 */
+   :2221-2262, :2307-2353, with the 2D step dxi */
 int qpad_species_push(qpad_ctx *ctx, int32_t sp, double dxi);
+/* the sub-cycled push of species2d_subcyc (proj_subcyc/
+   simulation_subcyc_class.f03:337-347): push_u, clamp_exp_fac
+   (part2d_subcyc_class.f03:46-66: a particle whose gamma / (gamma - p_z)
+   exceeds exp_fac_clamped (> 1) gets its momentum scaled so that the ratio
+   equals it), push_x; standard and robust pushers */
+int qpad_species_push_clamped(qpad_ctx *ctx, int32_t sp, double dxi, double exp_fac_clamped);
 /* species2d%interp_psi (species2d_class.f03:439-452, part2d_class.f03:
    2264-2305): psi of the current slice at the particles of a standard-pusher
    species, which its amjdp and push read; no effect on a robust species.
    QPAD calls it after field_psi%solve (simulation_class.f03:357-359); the
    fused forms do it at the same point. */
 int qpad_species_interp_psi(qpad_ctx *ctx, int32_t sp);
+
+/* ---- sub-cycling (simulation.algorithm "subcycling", proj_subcyc) ----
+   simulation_subcyc%new (simulation_subcyc_class.f03:76-88) with the deck's
+   expansion_fac_max (> 0), expansion_fac_clamped (> 1) and dt_2d_min (in
+   [dxi / 4096, dxi]); on = 0 returns to QPAD's standard slice.  Each fused
+   slice then takes the largest gamma / (gamma - p_z) of any particle
+   (get_exp_fac_max), runs ceiling(factor / expansion_fac_max) sub-cycles
+   of dxi / n (no more than floor(dxi / dt_2d_min); get_subcyc_step,
+   :431-449), each a charge deposit, the solves, the predictor-corrector
+   iteration with step dxi / n and the clamped push (neutrals ionised with
+   their own step dxi in each), and closes the slice once (:230-390).
+   qpad_slice's iteration count is the sum over the sub-cycles.  Not with a
+   laser, the PGC pushers, the seed network or a batch. */
+int qpad_set_subcycling(qpad_ctx *ctx, int32_t on, double exp_fac_max, double exp_fac_clamped, double dt_2d_min);
+/* sim_plasma_subcyc%get_exp_fac_max (sim_plasma_subcyc_class.f03:58-86,
+   part2d_subcyc_class.f03:27-44): the largest gamma / (gamma - p_z) over
+   every particle of every species and neutral, at least 1 */
+int qpad_exp_fac_max(qpad_ctx *ctx, double *out);
+/* the last fused slice's sub-cycles n, their 2D step and the factor they
+   came from (1, dxi, 0 before any sub-cycled slice) */
+int qpad_last_subcycles(qpad_ctx *ctx, int32_t *n, double *dxi_sub, double *exp_fac);
 
 /* ---- neutral gas and field ionisation ----
    neutrals(i) section (sim_plasma_class.f03:148-176, neutral_class.f03:

@@ -181,6 +181,13 @@ def load_library(path):
     L.ora_neutral_ion_deposit.argtypes = [C.c_void_p, C.POINTER(_Species)]
     L.ora_neutral_renew.argtypes = [C.c_void_p, C.POINTER(_Species)]
     L.ora_interp_point.argtypes = [C.c_void_p, _dp, C.c_int, C.c_double, C.c_double, _dp]
+    L.ora_set_subcycling.argtypes = [C.c_void_p, C.c_int, C.c_double, C.c_double, C.c_double]
+    L.ora_exp_fac_max.restype = C.c_double
+    L.ora_exp_fac_max.argtypes = [C.c_void_p]
+    L.ora_subcyc_step.restype = C.c_int
+    L.ora_subcyc_step.argtypes = [C.c_double, C.c_double, C.c_double, C.c_double, _dp]
+    L.ora_clamp_exp_fac.argtypes = [C.POINTER(_Species), C.c_double]
+    L.ora_last_subcycles.argtypes = [C.c_void_p, C.POINTER(C.c_int), _dp, _dp]
     L.ora_beam_qdeposit.argtypes = [C.c_void_p, C.c_int, _dp]
     L.ora_beam_push.argtypes = [C.c_void_p, C.c_int]
     L.ora_species_inject.argtypes = [C.c_void_p, C.c_int]
@@ -458,6 +465,19 @@ class Oracle:
     def last_iters(self):
         return self.L.ora_last_iters(self.h)
 
+    # -- sub-cycling (proj_subcyc)
+    def set_subcycling(self, exp_fac_max, exp_fac_clamped, dt_2d_min, on=True):
+        self.L.ora_set_subcycling(self.h, int(on), exp_fac_max, exp_fac_clamped, dt_2d_min)
+
+    def exp_fac_max(self):
+        return self.L.ora_exp_fac_max(self.h)
+
+    def last_subcycles(self):
+        """(sub-cycles, their 2D step, the expansion factor) of the last slice"""
+        n, d, f = C.c_int(0), np.zeros(1), np.zeros(1)
+        self.L.ora_last_subcycles(self.h, C.byref(n), ptr(d), ptr(f))
+        return n.value, float(d[0]), float(f[0])
+
     def last_residuals(self):
         """(relative, absolute) residual of the last slice's final 'compare'"""
         r, a = np.zeros(1), np.zeros(1)
@@ -471,3 +491,10 @@ class Oracle:
     def solve(self, name, *arrays):
         fn = getattr(self.L, "ora_solve_" + name)
         fn(self.h, *[ptr(a) for a in arrays])
+
+
+def subcyc_step(exp_fac, exp_fac_max, dt, dt_min, L=None):
+    """get_subcyc_step: (sub-cycles, their 2D step)"""
+    d = np.zeros(1)
+    n = (L or lib()).ora_subcyc_step(exp_fac, exp_fac_max, dt, dt_min, ptr(d))
+    return n, float(d[0])

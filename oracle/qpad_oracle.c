@@ -2207,21 +2207,19 @@ static void add_into(double *dst, const double *src, size_t n)
     for (size_t t = 0; t < n; t++) dst[t] = dst[t] + src[t];
 }
 
-/* One xi-slice: source/simulation_class.f03:342-469 */
-void ora_slice(ora_sim *s, int j, int *n_iter)
-{
-    int nm = s->nm, ld = s->ld;
-    size_t n1 = (size_t)nm * ld;
-    double *psi = s->f[ORA_PSI], *e = s->f[ORA_E], *e_spe = s->f[ORA_E_SPE];
-    double *b = s->f[ORA_B], *b_spe = s->f[ORA_B_SPE], *b_beam = s->f[ORA_B_BEAM];
-    double *cu = s->f[ORA_CU], *acu = s->f[ORA_ACU], *amu = s->f[ORA_AMU];
-    double *dcu = s->f[ORA_DCU], *q_spe = s->f[ORA_Q_SPE], *q_beam = s->f[ORA_Q_BEAM];
+/* The pieces of a slice (source/simulation_class.f03:342-469), shared by
+   the standard slice and the sub-cycled one
+   (proj_subcyc/simulation_subcyc_class.f03:230-390). */
 
+/* q_spe of every species and neutral (qdp, ion_deposit), psi, interp_psi
+   of the plasma species (QPAD's loop leaves out the neutrals, whose
+   electrons keep the psi = 0 of their release), the laser of slice j
+   (laser != 0), Bz from the J predictor: :346-367 */
+static void slice_charge(ora_sim *s, int j, int laser)
+{
+    const size_t n1 = (size_t)s->nm * s->ld;
+    double *psi = s->f[ORA_PSI], *cu = s->f[ORA_CU], *b_spe = s->f[ORA_B_SPE], *q_spe = s->f[ORA_Q_SPE];
     TSTART();
-    memcpy(q_beam, s->h_q_beam + (size_t)j * n1, n1 * sizeof(double));  /* copy_slice 2to1 */
-    TSTOP(ORA_T_COPY_SLICES);
-    ora_solve_bt_beam(s, q_beam, b_beam);
-    TSTOP(ORA_T_SOLVE_BEAM_BT);
     memset(q_spe, 0, n1 * sizeof(double));
     for (int k = 0; k < s->nspecies; k++) {                              /* species2d%qdp */
         ora_species *sp = &s->sp[k];
@@ -2240,12 +2238,23 @@ void ora_slice(ora_sim *s, int j, int *n_iter)
     TSTOP(ORA_T_SOLVE_PSI);
     for (int k = 0; k < s->nspecies; k++) {                              /* species2d%interp_psi */
         const int pt = s->sp[k].cfg.push_type;
+        if (s->sp[k].is_neutral) continue;
         if (pt == ORA_PUSH2_STD || pt == ORA_PUSH2_STD_PGC) ora_interp_psi(s, &s->sp[k], psi);
     }
-    if (s->laser_on) ora_laser_slice(s, j);                              /* laser_all of slice j */
+    if (laser && s->laser_on) ora_laser_slice(s, j);                     /* laser_all of slice j */
     ora_solve_bz(s, cu, b_spe);
     TSTOP(ORA_T_SOLVE_BZ);
+}
 
+/* the predictor-corrector iteration (:370-398) with the 2D step dt;
+   returns the iterations executed */
+static int slice_iterate(ora_sim *s, double dt)
+{
+    const size_t n1 = (size_t)s->nm * s->ld;
+    double *psi = s->f[ORA_PSI], *e = s->f[ORA_E];
+    double *b = s->f[ORA_B], *b_spe = s->f[ORA_B_SPE], *b_beam = s->f[ORA_B_BEAM];
+    double *cu = s->f[ORA_CU], *acu = s->f[ORA_ACU], *amu = s->f[ORA_AMU], *dcu = s->f[ORA_DCU];
+    TSTART();
     int l, nit = 0;
     double rel = 0.0, ab = 0.0;
     for (l = 1; l <= s->g.iter_max; l++) {
@@ -2267,12 +2276,12 @@ void ora_slice(ora_sim *s, int j, int *n_iter)
             memset(sp->amu_own, 0, 3 * n1 * sizeof(double));
             const int pt = sp->cfg.push_type;
             if (pt == ORA_PUSH2_STD)
-                ora_amjdeposit_std(s, sp, e, b, sp->cu_own, sp->amu_own, sp->acu_own, s->g.dxi);
+                ora_amjdeposit_std(s, sp, e, b, sp->cu_own, sp->amu_own, sp->acu_own, dt);
             else if (pt == ORA_PUSH2_STD_PGC || pt == ORA_PUSH2_ROBUST_PGC)
-                ora_amjdeposit_pgc(s, sp, e, b, sp->cu_own, sp->amu_own, sp->acu_own, s->g.dxi,
+                ora_amjdeposit_pgc(s, sp, e, b, sp->cu_own, sp->amu_own, sp->acu_own, dt,
                                    pt == ORA_PUSH2_STD_PGC);
             else
-                ora_amjdeposit_robust(s, sp, e, b, sp->cu_own, sp->amu_own, sp->acu_own, s->g.dxi);
+                ora_amjdeposit_robust(s, sp, e, b, sp->cu_own, sp->amu_own, sp->acu_own, dt);
             add_into(cu, sp->cu_own, 3 * n1);
             add_into(acu, sp->acu_own, 2 * n1);
             add_into(amu, sp->amu_own, 3 * n1);
@@ -2288,11 +2297,17 @@ void ora_slice(ora_sim *s, int j, int *n_iter)
         TSTOP(ORA_T_ARITHMETICS);
         if (rel < s->g.iter_reltol || ab < s->g.iter_abstol) break;
     }
-    s->last_iters = nit;
     s->last_rel = rel;
     s->last_abs = ab;
-    if (n_iter) *n_iter = nit;
+    return nit;
+}
 
+/* chi of the plasma species, cbq, the cu and q_spe histories (:400-412) */
+static void slice_chi_cbq(ora_sim *s, int j)
+{
+    const size_t n1 = (size_t)s->nm * s->ld;
+    double *cu = s->f[ORA_CU], *q_spe = s->f[ORA_Q_SPE];
+    TSTART();
     if (s->laser_on) {                                                   /* sim_lasers%deposit_chi */
         memset(s->chi, 0, n1 * sizeof(double));
         for (int k = 0; k < s->nspecies; k++)
@@ -2308,7 +2323,17 @@ void ora_slice(ora_sim *s, int j, int *n_iter)
     memcpy(s->h_cu + (size_t)j * 3 * n1, cu, 3 * n1 * sizeof(double));
     for (size_t t = 0; t < n1; t++) q_spe[t] = q_spe[t] + cu[2 * n1 + t];
     memcpy(s->h_q_spe + (size_t)j * n1, q_spe, n1 * sizeof(double));
+    TSTOP(ORA_T_COPY_SLICES);
+}
 
+/* b = b_spe + b_beam, E of the plasma alone, E of the converged currents
+   (:413-416) */
+static void slice_final_fields(ora_sim *s)
+{
+    const size_t n1 = (size_t)s->nm * s->ld;
+    double *psi = s->f[ORA_PSI], *e = s->f[ORA_E], *e_spe = s->f[ORA_E_SPE];
+    double *b = s->f[ORA_B], *b_spe = s->f[ORA_B_SPE], *b_beam = s->f[ORA_B_BEAM], *cu = s->f[ORA_CU];
+    TSTART();
     for (size_t t = 0; t < 3 * n1; t++) b[t] = b_spe[t] + b_beam[t];
     TSTOP(ORA_T_COPY_SLICES);
     ora_solve_et(s, b_spe, psi, e_spe);
@@ -2317,11 +2342,26 @@ void ora_slice(ora_sim *s, int j, int *n_iter)
     TSTOP(ORA_T_SOLVE_EZ);
     ora_solve_et(s, b, psi, e);
     TSTOP(ORA_T_SOLVE_PLASMA_ET);
+}
 
+/* the J_perp predictor of the next slice: dcu *= dxi, cu += dcu (:425-426) */
+static void slice_predictor(ora_sim *s)
+{
+    const size_t n1 = (size_t)s->nm * s->ld;
+    double *cu = s->f[ORA_CU], *dcu = s->f[ORA_DCU];
+    TSTART();
     for (size_t t = 0; t < 2 * n1; t++) dcu[t] = s->g.dxi * dcu[t];      /* dot_f1 */
     for (size_t t = 0; t < 2 * n1; t++) cu[t] = cu[t] + dcu[t];          /* J_perp predictor */
     TSTOP(ORA_T_ARITHMETICS);
+}
 
+/* the neutrals' update, then push_u, (clamp > 0) clamp_exp_fac, push_x and
+   update_bound of every species with the 2D step dt (:436-449; the
+   sub-cycled form :337-364) */
+static void slice_push(ora_sim *s, double dt, double clamp)
+{
+    double *e = s->f[ORA_E], *b = s->f[ORA_B];
+    TSTART();
     for (int k = 0; k < s->nspecies; k++) {
         ora_species *sp = &s->sp[k];
         if (sp->is_neutral) {                                            /* neutral%update */
@@ -2331,20 +2371,146 @@ void ora_slice(ora_sim *s, int j, int *n_iter)
             ora_neutral_add(s, sp);
         }
         const int pt = sp->cfg.push_type;
-        if (pt == ORA_PUSH2_STD) ora_push_u_std(s, sp, e, b, s->g.dxi);
-        else if (pt == ORA_PUSH2_STD_PGC || pt == ORA_PUSH2_ROBUST_PGC) ora_push_u_pgc(s, sp, e, b, s->g.dxi);
-        else ora_push_u_robust(s, sp, e, b, s->g.dxi);
-        ora_push_x(sp, s->g.dxi);
+        if (pt == ORA_PUSH2_STD) ora_push_u_std(s, sp, e, b, dt);
+        else if (pt == ORA_PUSH2_STD_PGC || pt == ORA_PUSH2_ROBUST_PGC) ora_push_u_pgc(s, sp, e, b, dt);
+        else ora_push_u_robust(s, sp, e, b, dt);
+        if (clamp > 0.0) ora_clamp_exp_fac(sp, clamp);
+        ora_push_x(sp, dt);
         TSTOP(ORA_T_PUSH_2D);
         ora_update_bound(s, sp);
         TSTOP(ORA_T_MOVE_2D);
     }
-    memcpy(s->h_e + (size_t)j * 3 * n1, e, 3 * n1 * sizeof(double));
-    memcpy(s->h_b + (size_t)j * 3 * n1, b, 3 * n1 * sizeof(double));
-    memcpy(s->h_psi + (size_t)j * n1, psi, n1 * sizeof(double));
-    memcpy(s->h_b_spe + (size_t)j * 3 * n1, b_spe, 3 * n1 * sizeof(double));
-    memcpy(s->h_e_spe + (size_t)j * 3 * n1, e_spe, 3 * n1 * sizeof(double));
+}
+
+/* the e, b, psi, b_spe, e_spe histories (:452-456) */
+static void slice_copies(ora_sim *s, int j)
+{
+    const size_t n1 = (size_t)s->nm * s->ld;
+    TSTART();
+    memcpy(s->h_e + (size_t)j * 3 * n1, s->f[ORA_E], 3 * n1 * sizeof(double));
+    memcpy(s->h_b + (size_t)j * 3 * n1, s->f[ORA_B], 3 * n1 * sizeof(double));
+    memcpy(s->h_psi + (size_t)j * n1, s->f[ORA_PSI], n1 * sizeof(double));
+    memcpy(s->h_b_spe + (size_t)j * 3 * n1, s->f[ORA_B_SPE], 3 * n1 * sizeof(double));
+    memcpy(s->h_e_spe + (size_t)j * 3 * n1, s->f[ORA_E_SPE], 3 * n1 * sizeof(double));
     TSTOP(ORA_T_COPY_SLICES);
+}
+
+/* get_exp_fac_max of every species and neutral
+   (proj_subcyc/sim_plasma_subcyc_class.f03:58-86, part2d_subcyc_class.f03:
+   27-44): the largest gamma / (gamma - p_z) of any particle, at least 1 */
+double ora_exp_fac_max(const ora_sim *s)
+{
+    double m = 1.0;
+    for (int k = 0; k < s->nspecies; k++) {
+        const ora_species *sp = &s->sp[k];
+        for (int64_t p = 0; p < sp->npp; p++) {
+            const double f = sp->gamma[p] / (sp->gamma[p] - sp->p3[p]);
+            if (f > m) m = f;
+        }
+    }
+    return m;
+}
+
+/* get_subcyc_step (proj_subcyc/simulation_subcyc_class.f03:431-449): the
+   sub-cycles of a slice and their 2D step */
+int ora_subcyc_step(double exp_fac, double exp_fac_max, double dt, double dt_min, double *dt_sub)
+{
+    int n = 1;
+    double d = dt;
+    if (exp_fac > exp_fac_max) {
+        n = (int)ceil(exp_fac / exp_fac_max);
+        d = dt / (double)n;
+        if (d < dt_min) {
+            n = (int)floor(dt / dt_min);
+            d = dt / (double)n;
+        }
+    }
+    *dt_sub = d;
+    return n;
+}
+
+/* clamp_exp_fac (proj_subcyc/part2d_subcyc_class.f03:46-66): a particle
+   whose gamma / (gamma - p_z) exceeds f has its momentum scaled so that the
+   ratio becomes f, and its gamma recomputed */
+void ora_clamp_exp_fac(ora_species *sp, double f)
+{
+    for (int64_t p = 0; p < sp->npp; p++) {
+        const double g = sp->gamma[p], pz = sp->p3[p];
+        const double ef = g / (g - pz);
+        if (ef > f) {
+            double sc = (f - 1.0) * (f - 1.0);
+            sc = sc / (f * f * pz * pz - sc * (g * g - 1.0));
+            sc = sqrt(sc);
+            sp->p1[p] = sp->p1[p] * sc;
+            sp->p2[p] = sp->p2[p] * sc;
+            sp->p3[p] = sp->p3[p] * sc;
+            sp->gamma[p] = sqrt(1.0 + sp->p1[p] * sp->p1[p] + sp->p2[p] * sp->p2[p] + sp->p3[p] * sp->p3[p]);
+        }
+    }
+}
+
+void ora_set_subcycling(ora_sim *s, int on, double exp_fac_max, double exp_fac_clamped, double dt_2d_min)
+{
+    s->subcyc = on;
+    s->sc_fac_max = exp_fac_max;
+    s->sc_fac_clamped = exp_fac_clamped;
+    s->sc_dt_min = dt_2d_min;
+}
+
+void ora_last_subcycles(const ora_sim *s, int *n, double *dt_sub, double *exp_fac)
+{
+    *n = s->last_nsub;
+    *dt_sub = s->last_dt_sub;
+    *exp_fac = s->last_exp_fac;
+}
+
+/* One xi-slice: source/simulation_class.f03:342-469, or with sub-cycling
+   on, proj_subcyc/simulation_subcyc_class.f03:230-390: the slice's plasma
+   response and push repeated n times with the 2D step dxi / n (n from the
+   largest gamma / (gamma - p_z) at the slice start), each push followed by
+   clamp_exp_fac, the neutrals ionised in every sub-cycle with their own
+   step dxi (neutral%dt); chi, cbq, the histories and the J_perp predictor
+   (with dxi) once after the last.  n_iter: the iterations summed over the
+   sub-cycles. */
+void ora_slice(ora_sim *s, int j, int *n_iter)
+{
+    const size_t n1 = (size_t)s->nm * s->ld;
+    TSTART();
+    memcpy(s->f[ORA_Q_BEAM], s->h_q_beam + (size_t)j * n1, n1 * sizeof(double));  /* copy_slice 2to1 */
+    TSTOP(ORA_T_COPY_SLICES);
+    ora_solve_bt_beam(s, s->f[ORA_Q_BEAM], s->f[ORA_B_BEAM]);
+    TSTOP(ORA_T_SOLVE_BEAM_BT);
+    int nit = 0;
+    if (!s->subcyc) {
+        slice_charge(s, j, 1);
+        nit = slice_iterate(s, s->g.dxi);
+        slice_chi_cbq(s, j);
+        slice_final_fields(s);
+        slice_predictor(s);
+        slice_push(s, s->g.dxi, 0.0);
+        s->last_nsub = 1;
+        s->last_dt_sub = s->g.dxi;
+        s->last_exp_fac = 0.0;
+    } else {
+        if (s->laser_on) ora_laser_slice(s, j);
+        const double ef = ora_exp_fac_max(s);
+        double dsub;
+        const int n = ora_subcyc_step(ef, s->sc_fac_max, s->g.dxi, s->sc_dt_min, &dsub);
+        for (int i = 1; i <= n; i++) {
+            slice_charge(s, j, 0);
+            nit += slice_iterate(s, dsub);
+            slice_final_fields(s);
+            slice_push(s, dsub, s->sc_fac_clamped);
+        }
+        slice_chi_cbq(s, j);
+        slice_predictor(s);
+        s->last_nsub = n;
+        s->last_dt_sub = dsub;
+        s->last_exp_fac = ef;
+    }
+    s->last_iters = nit;
+    if (n_iter) *n_iter = nit;
+    slice_copies(s, j);
 }
 
 /* 3D step epilogue: source/simulation_class.f03:485-501 (beam push, species

@@ -195,6 +195,13 @@ typedef struct {
     double *lz_ar, *lz_ai;
     double *ar, *ai, *ar_grad, *ai_grad;
     double *chi, *h_chi;
+    /* sub-cycling (proj_subcyc): on, expansion_fac_max,
+       expansion_fac_clamped, dt_2d_min; the last slice's sub-cycles, their
+       2D step and the expansion factor they came from */
+    int subcyc;
+    double sc_fac_max, sc_fac_clamped, sc_dt_min;
+    int last_nsub;
+    double last_dt_sub, last_exp_fac;
 } ora_sim;
 
 /* ---- low-level building blocks (exported for unit tests) ---- */
@@ -248,6 +255,12 @@ void ora_neutral_ionize(const ora_sim *s, ora_species *sp, const double *e);
 void ora_neutral_add(const ora_sim *s, ora_species *sp);
 void ora_neutral_ion_deposit(const ora_sim *s, ora_species *sp);
 void ora_neutral_renew(const ora_sim *s, ora_species *sp);
+/* sub-cycling */
+void ora_set_subcycling(ora_sim *s, int on, double exp_fac_max, double exp_fac_clamped, double dt_2d_min);
+double ora_exp_fac_max(const ora_sim *s);
+int ora_subcyc_step(double exp_fac, double exp_fac_max, double dt, double dt_min, double *dt_sub);
+void ora_clamp_exp_fac(ora_species *sp, double exp_fac_clamped);
+void ora_last_subcycles(const ora_sim *s, int *n, double *dt_sub, double *exp_fac);
 void ora_interp_point(const ora_sim *s, const double *f, int ncomp,
                       double x1, double x2, double *out);
 

@@ -114,6 +114,10 @@ SIGNATURES = {
     "qpad_get_species_charge": (C.c_int, [_CTX, C.c_int32, C.c_int32, _dp]),
     "qpad_species_amjdp": (C.c_int, [_CTX, C.c_int32, C.c_double]),
     "qpad_species_push": (C.c_int, [_CTX, C.c_int32, C.c_double]),
+    "qpad_species_push_clamped": (C.c_int, [_CTX, C.c_int32, C.c_double, C.c_double]),
+    "qpad_set_subcycling": (C.c_int, [_CTX, C.c_int32, C.c_double, C.c_double, C.c_double]),
+    "qpad_exp_fac_max": (C.c_int, [_CTX, _dp]),
+    "qpad_last_subcycles": (C.c_int, [_CTX, C.POINTER(C.c_int32), _dp, _dp]),
     "qpad_species_interp_psi": (C.c_int, [_CTX, C.c_int32]),
     "qpad_set_laser": (C.c_int, [_CTX, _dp, _dp]),
     "qpad_neutral_new": (C.c_int, [_CTX, C.POINTER(NeutralCfg), C.POINTER(C.c_int32)]),

@@ -132,6 +132,29 @@ def ionization(nr=48, nxi=40, max_mode=1, element=3, ion_max=3, ppc=(2, 2), num_
     return d
 
 
+def subcycling(nr=32, nxi=40, max_mode=1, n_peak=40.0, sigma_xi=0.2, exp_fac_max=1.1, exp_fac_clamped=1.5,
+               dt_2d_min=0.01, density=1.0, push_type="robust", neutral=None, offset_x=0.0, nstep=1):
+    """A reduced deck of QPAD's sub-cycling algorithm (simulation.algorithm
+    "subcycling", proj_subcyc): a strong narrow driver (n_b/n_p = n_peak,
+    sigma_r 0.25, sigma_xi) whose blowout drives gamma / (gamma - p_z) of the plasma
+    electrons above expansion_fac_max behind it, so that those slices are
+    pushed in several 2D steps; the clamp caps the factor at
+    expansion_fac_clamped.  density scales the plasma (and the gas); neutral:
+    (element, ion_max) of a uniform gas beside the plasma, n0 1e18."""
+    d = _deck(nr, 4.0, max_mode, nxi, 0.05, [_electrons(2, 4, push_type=push_type)],
+              _beam(nxi * 0.05, 0.25, sigma_xi, n_peak, 1 << 12, offset_x=offset_x), nstep=nstep)
+    d["species"][0]["density"] = density
+    sim = d["simulation"]
+    sim.update({"algorithm": "subcycling", "expansion_fac_max": exp_fac_max,
+                "expansion_fac_clamped": exp_fac_clamped, "dt_2d_min": dt_2d_min})
+    if neutral is not None:
+        sim["n0"] = 1.0e18
+        sim["nneutrals"] = 1
+        d["neutrals"] = [{"profile": ["uniform", "uniform"], "ppc": [2, 2], "num_theta": 8, "q": -1.0, "m": 1.0,
+                          "density": density, "element": neutral[0], "ion_max": neutral[1], "push_type": push_type}]
+    return d
+
+
 def ionization_example():
     """QPAD's own ionisation example, restated as data
     (input_file/ionization/qpinput.json, diagnostics left out): Nr 500,

@@ -989,8 +989,8 @@ static JDepArgs jdep_args(qpad_ctx *c, int s, int l, bool store_diag, bool repor
     a.nm = c->g.nm;
     a.idr = 1.0 / c->g.dr;
     a.qbm = sp.qbm;
-    a.qtmh = 0.5 * sp.qbm * c->g.dxi;
-    a.idt = 1.0 / c->g.dxi;
+    a.qtmh = 0.5 * sp.qbm * c->dt2d;
+    a.idt = 1.0 / c->dt2d;
     a.x1 = sp.x1; a.x2 = sp.x2; a.p1 = sp.p1; a.p2 = sp.p2; a.p3 = sp.p3; a.q = sp.q;
     a.gamma = sp.gamma;
     a.psi = sp.psi;
@@ -1125,6 +1125,7 @@ struct PushArgs {
     int nr, ld, nm;
     int64_t nslots;
     double idr, edge, qtmh, dt, qbm;
+    double clamp;        // expansion-factor clamp of a sub-cycled push (0: none)
     double *x1, *x2, *p1, *p2, *p3, *gamma;
     const double *psi;   // interpolated psi (standard pusher)
     const int32_t *cs;
@@ -1230,6 +1231,20 @@ __global__ __launch_bounds__(256) void k_push(const PushMulti m)
     } else {
         g2 = sqrt(1.0 + pp[0] * pp[0] + pp[1] * pp[1] + pp[2] * pp[2]);
     }
+    if (a.clamp > 0.0) {
+        // clamp_exp_fac (proj_subcyc/part2d_subcyc_class.f03:46-66): the
+        // momentum scaled so that gamma / (gamma - p_z) becomes the clamp
+        const double F = a.clamp;
+        if (g2 / (g2 - pp[2]) > F) {
+            double sc = (F - 1.0) * (F - 1.0);
+            sc = sc / (F * F * pp[2] * pp[2] - sc * (g2 * g2 - 1.0));
+            sc = sqrt(sc);
+            pp[0] = pp[0] * sc;
+            pp[1] = pp[1] * sc;
+            pp[2] = pp[2] * sc;
+            g2 = sqrt(1.0 + pp[0] * pp[0] + pp[1] * pp[1] + pp[2] * pp[2]);
+        }
+    }
     double dtc = a.dt / (g2 - pp[2]);
     x1 = x1 + pp[0] * dtc;
     x2 = x2 + pp[1] * dtc;
@@ -1252,8 +1267,9 @@ static PushArgs push_args(qpad_ctx *c, int s)
     a.nslots = sp.npp;
     a.idr = 1.0 / c->g.dr;
     a.edge = (double)c->g.nr * c->g.dr;
-    a.qtmh = sp.qbm * c->g.dxi * 0.5;
-    a.dt = c->g.dxi;
+    a.qtmh = sp.qbm * c->dt2d * 0.5;
+    a.dt = c->dt2d;
+    a.clamp = c->clamp2d;
     a.qbm = sp.qbm;
     a.x1 = sp.x1; a.x2 = sp.x2; a.p1 = sp.p1; a.p2 = sp.p2; a.p3 = sp.p3; a.gamma = sp.gamma;
     a.psi = sp.psi;

@@ -1117,6 +1117,7 @@ extern "C" {
 int qpad_seednet_create(qpad_ctx *c, const qpad_seednet_cfg *cfg)
 {
     if (!c || !cfg) return net_fail(QPAD_E_INVALID, "null argument");
+    if (c->subcyc) return net_fail(QPAD_E_STATE, "the seed network does not step sub-cycled slices");
     if (cfg->width != 32 && cfg->width != 64) return net_fail(QPAD_E_INVALID, "seed network width must be 32 or 64");
     if (cfg->layers < 2 || cfg->layers > kNetMaxLayers)
         return net_fail(QPAD_E_INVALID, "seed network layers must be 2..8");

@@ -133,6 +133,16 @@ int launch_neutral_update(qpad_ctx *c, int sp);
 // the ADK row (C1, C2, C3) of level `level` of element Z, or null
 const double *adk_row(int element, int level);
 
+// ---- sub-cycling (k_subcyc.hip) ----
+// the largest gamma / (gamma - p_z) of any particle, at least 1 (waits for
+// the stream)
+int launch_exp_fac_max(qpad_ctx *c, double *out);
+// fold the iterations of a sub-cycle into the step's counts (reset: at the
+// start of a slice)
+int launch_iter_fold(qpad_ctx *c, bool reset);
+// get_subcyc_step: the sub-cycles of a slice and their 2D step
+int subcyc_step(double exp_fac, double exp_fac_max, double dt, double dt_min, double *dt_sub);
+
 // ---- beam (k_beam.hip) ----
 int launch_beam_inject(qpad_ctx *c, int ib);
 // lattice beam (profile "standard"): lattice_count flags and numbers the

@@ -430,6 +430,7 @@ int qpad_create(const qpad_grid *gr, qpad_ctx **out)
     if (!rc) rc = dalloc(&c->conv_im, (size_t)g.nr + 2);
     if (!rc) rc = dalloc(&c->conv_blk, 2 * 2 * (size_t)conv_blocks(g.nr));
     if (!rc) rc = dalloc(&c->it, 1);
+    if (!rc) rc = dalloc(&c->sc_ef, 1);
     // factors stored transposed for the solve kernels: [kind][m][lo|inv|cp]
     // then row r = t * K + kk at kk * kSolveThreads + t; rows >= nr are
     // identity rows (lo = 0, inv = 1, cp = 0)
@@ -439,6 +440,10 @@ int qpad_create(const qpad_grid *gr, qpad_ctx **out)
     if (!rc) rc = dalloc(&c->src, (size_t)src_rows_host(g.M, g.nm) * g.nr);
     if (!rc) {
         he = hipHostMalloc((void **)&c->it_host, sizeof(IterState), hipHostMallocDefault);
+        if (he != hipSuccess) rc = fail(QPAD_E_NOMEM, "hipHostMalloc failed");
+    }
+    if (!rc) {
+        he = hipHostMalloc((void **)&c->sc_ef_host, sizeof(double), hipHostMallocDefault);
         if (he != hipSuccess) rc = fail(QPAD_E_NOMEM, "hipHostMalloc failed");
     }
     if (!rc) {
@@ -453,6 +458,10 @@ int qpad_create(const qpad_grid *gr, qpad_ctx **out)
     c->adaptive = 1;
     c->spec_margin = 0;
     c->spec_guess = 2;
+    c->dt2d = g.dxi;
+    c->clamp2d = 0.0;
+    c->last_nsub = 1;
+    c->last_dsub = g.dxi;
     c->dep_cells = kDepWaves;
     // the matrix form where it measured faster on MI355X: modes 0..3 and up
     // (C3, modes 0..4: 18.6 against 26.0 us per current deposit); the
@@ -513,6 +522,8 @@ int qpad_destroy(qpad_ctx *c)
     dfree(c->h_psi); dfree(c->h_e); dfree(c->h_b); dfree(c->h_b_spe); dfree(c->h_e_spe);
     dfree(c->h_cu); dfree(c->h_q_spe); dfree(c->h_q_beam); dfree(c->h_b_beam);
     dfree(c->conv_re); dfree(c->conv_im); dfree(c->conv_blk); dfree(c->it); dfree(c->fac); dfree(c->src);
+    dfree(c->sc_ef);
+    if (c->sc_ef_host) (void)hipHostFree(c->sc_ef_host);
     dfree(c->lz_ar); dfree(c->lz_ai); dfree(c->h_chi);
     if (c->it_host) (void)hipHostFree(c->it_host);
     if (c->it_report) (void)hipHostFree(c->it_report);
@@ -556,6 +567,8 @@ static int species_create(qpad_ctx *c, const qpad_species_cfg *cfg, const qpad_n
         return fail(QPAD_E_INVALID, "push_type must be QPAD_PUSH2_STD, QPAD_PUSH2_ROBUST, QPAD_PUSH2_STD_PGC or "
                                     "QPAD_PUSH2_ROBUST_PGC");
     if (cfg->mass == 0.0 || cfg->qm == 0.0) return fail(QPAD_E_INVALID, "q and m must be non-zero");
+    if (c->subcyc && is_pgc(cfg->push_type))
+        return fail(QPAD_E_STATE, "sub-cycling with the PGC pushers is not on the path");
     // npmax: fdist2d_class.f03:258-262 (xtra = 2)
     const int64_t nlat = (int64_t)c->g.nr * cfg->ppc1 * cfg->ppc2 * cfg->num_theta;
     if (nlat > 0x7fffffff / 2) return fail(QPAD_E_INVALID, "too many particles per species");
@@ -860,6 +873,7 @@ struct SliceRun {
     int known_iters = 0;  // iterations the slice needed, once known
     bool waiting = false; // a deciding deposit is in flight (dec_ev)
     bool overlap = false; // network overlap (qpad_step3d only)
+    int sub = 1, nsub = 1; // sub-cycle being queued, sub-cycles of the slice
 };
 
 static int slice_finish(qpad_ctx *c, SliceRun &st);
@@ -913,15 +927,13 @@ static int slice_decided(qpad_ctx *c, SliceRun &st)
 // queuing at the first converged report.  Fixed mode queues all iter_max
 // iterations.  Timing events (when enabled) bracket only the first
 // iteration, which always does work: an event per launch would cost a
-// quarter of the step.
-static int slice_start(qpad_ctx *c, int j, bool overlap, SliceRun &st)
+// quarter of the step.  With sub-cycling this is the start of every
+// sub-cycle (simulation_subcyc_class.f03:282-300).
+static int subcycle_start(qpad_ctx *c, SliceRun &st)
 {
-    // the learned seed (when on) is computed on the network's stream from
-    // the fields left by slice j - 1, beside the previous slice's push and
-    // sort and this slice's charge deposit, sources and solves
+    const int j = st.j;
     const bool net = seednet_active(c);
     const double *seed = nullptr;
-    if (net && c->net_pre != j) TRY(seednet_after_epilogue(c, 0, j));
     TRY(launch_qdeposit_all(c));
     if (c->species_q)
         for (int s = 0; s < c->nspecies; s++) TRY(launch_species_q_start(c, s));
@@ -929,20 +941,19 @@ static int slice_start(qpad_ctx *c, int j, bool overlap, SliceRun &st)
     // the sources add to q_spe after the electrons' charge
     for (int s = 0; s < c->nspecies; s++)
         if (c->sp[s].neutral) TRY(launch_ion_deposit(c, s));
-    // laser_all of slice j (simulation_class.f03:362-367)
-    if (c->laser) TRY(launch_laser_slice(c, j));
     TRY(launch_slice_sources(c));
     TRY(launch_start_solves(c));
     // species2d%interp_psi of the standard-pusher species
-    // (simulation_class.f03:357-359)
-    for (int s = 0; s < c->nspecies; s++) TRY(launch_interp_psi(c, s));
+    // (simulation_class.f03:357-359); QPAD's loop leaves out the neutrals,
+    // whose electrons keep the psi = 0 of their release
+    for (int s = 0; s < c->nspecies; s++)
+        if (!c->sp[s].neutral) TRY(launch_interp_psi(c, s));
     // the first kernel that reads B_perp waits for the seed
     if (net) TRY(seednet_before_sources(c, j, &seed));
     TRY(launch_start_eperp(c, j, seed));
     const int imax = c->g.iter_max;
-    st = SliceRun();
-    st.j = j;
-    st.overlap = overlap;
+    st.queued = 0;
+    st.waiting = false;
     st.known_iters = imax;
     // (at most imax - 1 up front, so that the deposit of iteration imax still
     // reports and the next slice's guess keeps following the iterations)
@@ -950,15 +961,63 @@ static int slice_start(qpad_ctx *c, int j, bool overlap, SliceRun &st)
     return slice_advance(c, st);
 }
 
+// the sub-cycles of slice j (get_exp_fac_max and get_subcyc_step,
+// simulation_subcyc_class.f03:270-276): the 2D step and the clamp of its
+// deposits and pushes
+static int subcycle_plan(qpad_ctx *c, SliceRun &st)
+{
+    double ef = 1.0, dsub = c->g.dxi;
+    TRY(launch_exp_fac_max(c, &ef));
+    if (!(ef < INFINITY)) return fail(QPAD_E_STATE, "sub-cycling: a particle has gamma - p_z = 0");
+    const int n = subcyc_step(ef, c->sc_fac_max, c->g.dxi, c->sc_dt_min, &dsub);
+    if (n < 1) return fail(QPAD_E_STATE, "sub-cycling: no sub-cycle fits dt_2d_min");
+    st.sub = 1;
+    st.nsub = n;
+    c->dt2d = dsub;
+    c->clamp2d = c->sc_fac_clamped;
+    c->last_nsub = n;
+    c->last_dsub = dsub;
+    c->last_ef = ef;
+    return launch_iter_fold(c, true);
+}
+
+static int slice_start(qpad_ctx *c, int j, bool overlap, SliceRun &st)
+{
+    // the learned seed (when on) is computed on the network's stream from
+    // the fields left by slice j - 1, beside the previous slice's push and
+    // sort and this slice's charge deposit, sources and solves
+    if (seednet_active(c) && c->net_pre != j) TRY(seednet_after_epilogue(c, 0, j));
+    st = SliceRun();
+    st.j = j;
+    st.overlap = overlap;
+    if (c->subcyc) TRY(subcycle_plan(c, st));
+    // laser_all of slice j (simulation_class.f03:362-367)
+    if (c->laser) TRY(launch_laser_slice(c, j));
+    return subcycle_start(c, st);
+}
+
 // slice epilogue: cbq, e_spe, history (simulation_class.f03:401-456), then
 // the push and the sort.  The network's work behind the epilogue (the next
 // seed, training) is queued after the push and the sort: the host reaches
 // this point only once the slice's convergence is known, and queuing the
 // network first left the main stream idle while the host was busy with it.
+// A sub-cycle before the slice's last only pushes (the neutrals' update,
+// push_u, clamp_exp_fac, push_x: simulation_subcyc_class.f03:337-364) and
+// starts the next; the last one's epilogue comes before its push, which
+// changes nothing it reads (QPAD closes the slice after the push).
 static int slice_finish(qpad_ctx *c, SliceRun &st)
 {
     const int j = st.j;
     if (c->adaptive) c->spec_guess = st.known_iters;
+    if (st.sub < st.nsub) {
+        TRY(launch_iter_fold(c, false));
+        for (int s = 0; s < c->nspecies; s++)
+            if (c->sp[s].neutral) TRY(launch_neutral_update(c, s));
+        TRY(launch_push_all(c));
+        TRY(launch_sort_all(c));
+        st.sub++;
+        return subcycle_start(c, st);
+    }
     TRY(launch_epilogue(c, j));
     if (c->species_q)
         for (int s = 0; s < c->nspecies; s++) TRY(launch_species_cbq(c, s, j));
@@ -980,6 +1039,8 @@ static int slice_finish(qpad_ctx *c, SliceRun &st)
     TRY(launch_push_all(c));
     TRY(launch_sort_all(c));
     if (net) TRY(seednet_after_epilogue(c, j, (st.overlap && j < c->g.nxi) ? j + 1 : 0, true));
+    c->dt2d = c->g.dxi;
+    c->clamp2d = 0.0;
     st.j = 0;
     return 0;
 }
@@ -1011,7 +1072,7 @@ int qpad_slice(qpad_ctx *c, int32_t j, int32_t *iters_out)
     if (j < 1 || j > c->g.nxi) return fail(QPAD_E_INVALID, "slice index out of range");
     TRY(slice_enqueue(c, j, false));
     TRY(read_iter_state(c));
-    if (iters_out) *iters_out = c->it_host->iters;
+    if (iters_out) *iters_out = c->it_host->iters + (c->subcyc ? c->it_host->sub_iters : 0);
     return 0;
 }
 
@@ -1178,8 +1239,11 @@ int qpad_get_species_charge(qpad_ctx *c, int32_t s, int32_t slice, double *out)
 int qpad_species_amjdp(qpad_ctx *c, int32_t s, double dxi)
 {
     if (!c || s < 0 || s >= c->nspecies) return fail(QPAD_E_INVALID, "bad species");
-    if (dxi != c->g.dxi) return fail(QPAD_E_INVALID, "dxi differs from the grid's dxi");
-    TRY(launch_amjdeposit(c, s, 0, true, false));
+    if (!(dxi > 0.0 && dxi < INFINITY)) return fail(QPAD_E_INVALID, "dxi must be > 0");
+    c->dt2d = dxi;
+    const int rc = launch_amjdeposit(c, s, 0, true, false);
+    c->dt2d = c->g.dxi;
+    TRY(rc);
     // add this species' cu/acu/amu into the totals
     TRY(launch_combine_j_species(c, s));
     return sync(c);
@@ -1188,6 +1252,7 @@ int qpad_species_amjdp(qpad_ctx *c, int32_t s, double dxi)
 int qpad_set_laser(qpad_ctx *c, const double *ar_hist, const double *ai_hist)
 {
     if (!c || !ar_hist || !ai_hist) return fail(QPAD_E_INVALID, "null argument");
+    if (c->subcyc) return fail(QPAD_E_STATE, "sub-cycling with a laser is not on the path");
     TRY(laser_ensure(c));
     // packed host [(nxi + 3)][nm][nr + 2] -> padded device rows of length ld
     const size_t rows = ((size_t)c->g.nxi + 3) * c->g.nm, w = (size_t)c->g.nr + 2;
@@ -1257,17 +1322,75 @@ int qpad_neutral_adk(int32_t element, int32_t level, double *out)
 int qpad_species_interp_psi(qpad_ctx *c, int32_t s)
 {
     if (!c || s < 0 || s >= c->nspecies) return fail(QPAD_E_INVALID, "bad species");
+    if (c->sp[s].neutral) return fail(QPAD_E_INVALID, "a neutral has no interp_psi (its electrons keep psi = 0)");
     TRY(launch_interp_psi(c, s));
     return sync(c);
 }
 
-int qpad_species_push(qpad_ctx *c, int32_t s, double dxi)
+// the push of species s with 2D step dxi and expansion-factor clamp (0: none)
+static int species_push(qpad_ctx *c, int32_t s, double dxi, double clamp)
 {
     if (!c || s < 0 || s >= c->nspecies) return fail(QPAD_E_INVALID, "bad species");
-    if (dxi != c->g.dxi) return fail(QPAD_E_INVALID, "dxi differs from the grid's dxi");
-    TRY(launch_push(c, s));
+    if (!(dxi > 0.0 && dxi < INFINITY)) return fail(QPAD_E_INVALID, "dxi must be > 0");
+    c->dt2d = dxi;
+    c->clamp2d = clamp;
+    int rc = launch_push(c, s);
+    c->dt2d = c->g.dxi;
+    c->clamp2d = 0.0;
+    TRY(rc);
     TRY(launch_sort(c, s));
     return sync(c);
+}
+
+int qpad_species_push(qpad_ctx *c, int32_t s, double dxi) { return species_push(c, s, dxi, 0.0); }
+
+int qpad_species_push_clamped(qpad_ctx *c, int32_t s, double dxi, double exp_fac_clamped)
+{
+    if (!(exp_fac_clamped > 1.0 && exp_fac_clamped < INFINITY))
+        return fail(QPAD_E_INVALID, "expansion_fac_clamped must be > 1");
+    if (c && s >= 0 && s < c->nspecies && is_pgc(c->sp[s].cfg.push_type))
+        return fail(QPAD_E_INVALID, "the clamp is for the standard and robust pushers");
+    return species_push(c, s, dxi, exp_fac_clamped);
+}
+
+int qpad_set_subcycling(qpad_ctx *c, int32_t on, double exp_fac_max, double exp_fac_clamped, double dt_2d_min)
+{
+    if (!c) return fail(QPAD_E_INVALID, "null context");
+    if (on != 0 && on != 1) return fail(QPAD_E_INVALID, "on must be 0 or 1");
+    if (!on) {
+        c->subcyc = false;
+        return 0;
+    }
+    if (!(exp_fac_max > 0.0 && exp_fac_max < INFINITY)) return fail(QPAD_E_INVALID, "expansion_fac_max must be > 0");
+    if (!(exp_fac_clamped > 1.0 && exp_fac_clamped < INFINITY))
+        return fail(QPAD_E_INVALID, "expansion_fac_clamped must be > 1");
+    if (!(dt_2d_min > 0.0 && dt_2d_min <= c->g.dxi && c->g.dxi / dt_2d_min <= 4096.0))
+        return fail(QPAD_E_INVALID, "dt_2d_min must be in [dxi / 4096, dxi]");
+    if (c->laser) return fail(QPAD_E_STATE, "sub-cycling with a laser is not on the path");
+    for (int s = 0; s < c->nspecies; s++)
+        if (is_pgc(c->sp[s].cfg.push_type))
+            return fail(QPAD_E_STATE, "sub-cycling with the PGC pushers is not on the path");
+    if (c->net) return fail(QPAD_E_STATE, "sub-cycling with the seed network is not on the path");
+    c->subcyc = true;
+    c->sc_fac_max = exp_fac_max;
+    c->sc_fac_clamped = exp_fac_clamped;
+    c->sc_dt_min = dt_2d_min;
+    return 0;
+}
+
+int qpad_exp_fac_max(qpad_ctx *c, double *out)
+{
+    if (!c || !out) return fail(QPAD_E_INVALID, "null argument");
+    return launch_exp_fac_max(c, out);
+}
+
+int qpad_last_subcycles(qpad_ctx *c, int32_t *n, double *dxi_sub, double *exp_fac)
+{
+    if (!c || !n || !dxi_sub || !exp_fac) return fail(QPAD_E_INVALID, "null argument");
+    *n = c->last_nsub;
+    *dxi_sub = c->last_dsub;
+    *exp_fac = c->last_ef;
+    return 0;
 }
 
 int qpad_field_solve_psi(qpad_ctx *c)
@@ -1695,6 +1818,8 @@ int qpad_batch_step3d(qpad_batch *b, int32_t *total_iters_out)
     // the seed network's stream joins a context's slice loop through events
     // the eager modes queue like qpad_step3d does; a captured slice graph
     // holds no such second stream
+    for (qpad_ctx *c : b->ctx)
+        if (c->subcyc) return fail(QPAD_E_STATE, "batches step without sub-cycling (its slices wait for the host)");
     if (b->graph == 1)
         for (qpad_ctx *c : b->ctx) {
             if (seednet_active(c))

@@ -115,7 +115,7 @@ struct IterState {        // device-resident iteration bookkeeping
     int iters;            // iterations executed in the current slice
     int total_iters;      // iterations summed over the slices of a 3D step
     int at_max;           // slices of the 3D step that ended at iter_max
-    int pad;
+    int sub_iters;        // iterations of the current slice's earlier sub-cycles
 };
 
 }  // namespace qpad
@@ -165,6 +165,18 @@ struct qpad_ctx {
     // deposits chi
     double *lz_ar, *lz_ai, *h_chi;
     bool laser;
+    // sub-cycling (simulation.algorithm "subcycling"): expansion_fac_max,
+    // expansion_fac_clamped, dt_2d_min; dt2d and clamp2d are the 2D step and
+    // the expansion-factor clamp of the current deposits and pushes (dxi and
+    // none outside a sub-cycled slice); the last slice's sub-cycles, their
+    // step and the expansion factor they came from
+    bool subcyc;
+    double sc_fac_max, sc_fac_clamped, sc_dt_min;
+    double dt2d, clamp2d;
+    double *sc_ef;            // device: the largest gamma / (gamma - p_z), as bits
+    double *sc_ef_host;       // pinned mirror
+    int last_nsub;
+    double last_dsub, last_ef;
 };
 
 namespace qpad {

@@ -212,6 +212,41 @@ module qpad_hip_mod
       integer(c_int) :: rc
     end function
 
+    ! species2d_subcyc's push: push_u, clamp_exp_fac, push_x
+    function qpad_species_push_clamped(ctx, sp, dxi, exp_fac_clamped) bind(C, name="qpad_species_push_clamped") &
+        result(rc)
+      import :: c_int, c_int32_t, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      integer(c_int32_t), value :: sp
+      real(c_double), value :: dxi, exp_fac_clamped
+      integer(c_int) :: rc
+    end function
+
+    ! simulation_subcyc%new, get_exp_fac_max, the last slice's sub-cycles
+    function qpad_set_subcycling(ctx, on, exp_fac_max, exp_fac_clamped, dt_2d_min) &
+        bind(C, name="qpad_set_subcycling") result(rc)
+      import :: c_int, c_int32_t, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      integer(c_int32_t), value :: on
+      real(c_double), value :: exp_fac_max, exp_fac_clamped, dt_2d_min
+      integer(c_int) :: rc
+    end function
+
+    function qpad_exp_fac_max(ctx, out) bind(C, name="qpad_exp_fac_max") result(rc)
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      real(c_double), intent(out) :: out
+      integer(c_int) :: rc
+    end function
+
+    function qpad_last_subcycles(ctx, n, dxi_sub, exp_fac) bind(C, name="qpad_last_subcycles") result(rc)
+      import :: c_int, c_int32_t, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      integer(c_int32_t), intent(out) :: n
+      real(c_double), intent(out) :: dxi_sub, exp_fac
+      integer(c_int) :: rc
+    end function
+
     ! iterations of the last 3D step: total, and slices that ended at iter_max
     function qpad_step_iterations(ctx, total_iters, slices_at_iter_max) &
         bind(C, name="qpad_step_iterations") result(rc)
@@ -690,7 +725,8 @@ module qpad_hip_mod
   public :: qpad_last_error, qpad_abi_version, qpad_create, qpad_destroy, qpad_get_grid, &
     qpad_species_new, qpad_species_renew, qpad_species_qdp, qpad_species_qdp_noalloc, &
     qpad_species_charge_history, qpad_species_cbq, qpad_get_species_charge, qpad_species_amjdp, &
-    qpad_species_push, qpad_species_interp_psi, qpad_step_iterations, qpad_field_solve_psi, qpad_field_solve_ez, &
+    qpad_species_push, qpad_species_push_clamped, qpad_set_subcycling, qpad_exp_fac_max, qpad_last_subcycles, &
+    qpad_species_interp_psi, qpad_step_iterations, qpad_field_solve_psi, qpad_field_solve_ez, &
     qpad_set_laser, qpad_laser_slice, qpad_deposit_chi, &
     qpad_field_solve_et, qpad_field_solve_bz, qpad_field_solve_bt_beam, &
     qpad_field_solve_djdxi, qpad_field_solve_bt_iter, qpad_field_add_b, &

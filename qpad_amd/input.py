@@ -63,8 +63,8 @@ def grid_from_deck(deck):
         raise QpadError('Invalid field boundary type! Only "open" and "zero" are supported currently.')
     if _get(sim, "interpolation", "linear", required=False) != "linear":
         raise QpadError('Invalid interpolation type! Only "linear" are supported currently.')
-    if _get(sim, "algorithm", "standard", required=False) != "standard":
-        raise QpadError("only algorithm 'standard' is on the GPU path")
+    if _get(sim, "algorithm", "standard", required=False) not in ("standard", "subcycling"):
+        raise QpadError("only algorithms 'standard' and 'subcycling' are on the GPU path")
     if _get(sim, "nlasers", 0, required=False) != 0:
         raise QpadError("nlasers > 0: the laser solver is not on the GPU path; give the envelope to "
                         "Simulation.set_laser (the PGC pushers and the chi deposit are)")
@@ -75,6 +75,17 @@ def grid_from_deck(deck):
              _get(sim, "dt"))
     nstep3d = int(_get(sim, "time") / _get(sim, "dt"))
     return g, nstep3d, z0
+
+
+def subcycling_from_deck(deck):
+    """simulation.algorithm "subcycling" (options_class.f03:55-69,
+    proj_subcyc/simulation_subcyc_class.f03:84-86): (expansion_fac_max,
+    expansion_fac_clamped, dt_2d_min); None for the standard algorithm."""
+    sim = deck["simulation"]
+    if _get(sim, "algorithm", "standard", required=False) != "subcycling":
+        return None
+    return (float(_get(sim, "expansion_fac_max")), float(_get(sim, "expansion_fac_clamped")),
+            float(_get(sim, "dt_2d_min")))
 
 
 # species(i).push_type (sim_plasma_class.f03:123-137)

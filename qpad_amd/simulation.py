@@ -11,7 +11,7 @@ import ctypes as C
 import numpy as np
 
 from . import _lib as L
-from .input import (FileBeam, beam_from_deck, grid_from_deck, neutral_from_deck, place_file_beam,
+from .input import (FileBeam, beam_from_deck, grid_from_deck, neutral_from_deck, place_file_beam, subcycling_from_deck,
                     species_from_deck)
 
 
@@ -127,9 +127,15 @@ class Species2D:
             raise L.QpadError("species2d%interp_psi takes psi")
         L.check(self.sim.lib.qpad_species_interp_psi(self.sim.ctx, self.id))
 
-    def push(self, dxi):
-        """push_u + push_x (+ update_bound)."""
-        L.check(self.sim.lib.qpad_species_push(self.sim.ctx, self.id, float(dxi)))
+    def push(self, dxi, exp_fac_clamped=None):
+        """push_u + push_x (+ update_bound); with exp_fac_clamped, the
+        sub-cycled push_u, clamp_exp_fac, push_x (part2d_subcyc_class.f03:
+        46-66)"""
+        if exp_fac_clamped is None:
+            L.check(self.sim.lib.qpad_species_push(self.sim.ctx, self.id, float(dxi)))
+        else:
+            L.check(self.sim.lib.qpad_species_push_clamped(self.sim.ctx, self.id, float(dxi),
+                                                           float(exp_fac_clamped)))
 
     def count(self):
         n = C.c_int64(0)
@@ -338,7 +344,28 @@ class Simulation:
         self.spe += self.neut
         self.beam = [Beam3D(self, beam_from_deck(bm, z0, seed=seed + 7 * (k + 1)))
                      for k, bm in enumerate(deck.get("beam", [])[:nbm])]
+        sc = subcycling_from_deck(deck)
+        if sc is not None:
+            self.set_subcycling(*sc)
         self.tstep = 0
+
+    # -- sub-cycling (simulation.algorithm "subcycling", proj_subcyc)
+    def set_subcycling(self, exp_fac_max, exp_fac_clamped, dt_2d_min, on=True):
+        """simulation_subcyc%new (simulation_subcyc_class.f03:76-88)"""
+        L.check(self.lib.qpad_set_subcycling(self.ctx, int(on), float(exp_fac_max), float(exp_fac_clamped),
+                                             float(dt_2d_min)))
+
+    def exp_fac_max(self):
+        """sim_plasma_subcyc%get_exp_fac_max: max(1, gamma / (gamma - p_z))"""
+        out = C.c_double(0.0)
+        L.check(self.lib.qpad_exp_fac_max(self.ctx, C.byref(out)))
+        return out.value
+
+    def last_subcycles(self):
+        """(sub-cycles, their 2D step, the expansion factor) of the last slice"""
+        n, d, f = C.c_int32(0), C.c_double(0.0), C.c_double(0.0)
+        L.check(self.lib.qpad_last_subcycles(self.ctx, C.byref(n), C.byref(d), C.byref(f)))
+        return n.value, d.value, f.value
 
     # -- laser (PGC pushers)
     def set_laser(self, ar_hist, ai_hist):

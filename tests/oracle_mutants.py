@@ -160,6 +160,22 @@ MUTANTS = {
     "update_bound_skips_one": ("while (i < sp->npp - 1) {\n        double pos = sqrt(sp->x1[i]",
                                "while (i < sp->npp - 2) {\n        double pos = sqrt(sp->x1[i]",
                                ["update_bound"]),
+    "subcyc_step_floor": ("n = (int)ceil(exp_fac / exp_fac_max);", "n = (int)floor(exp_fac / exp_fac_max);",
+                          ["subcyc_step"]),
+    "clamp_without_sqrt": ("            sc = sqrt(sc);\n            sp->p1[p] = sp->p1[p] * sc;",
+                           "            sp->p1[p] = sp->p1[p] * sc;", ["clamp_exp_fac"]),
+    "exp_fac_from_zero": ("    double m = 1.0;", "    double m = 0.0;", ["exp_fac_max"]),
+    "subcyc_push_full_step": ("slice_push(s, dsub, s->sc_fac_clamped);", "slice_push(s, s->g.dxi, s->sc_fac_clamped);",
+                              ["subcyc_slice"]),
+    "subcyc_without_clamp": ("slice_push(s, dsub, s->sc_fac_clamped);", "slice_push(s, dsub, 0.0);",
+                             ["subcyc_slice"]),
+    "subcyc_ionize_with_substep": (
+        "            ora_neutral_ionize(s, sp, e);\n            ora_neutral_add(s, sp);",
+        "            const double dt_keep = sp->dt;\n            sp->dt = dt;\n"
+        "            ora_neutral_ionize(s, sp, e);\n            sp->dt = dt_keep;\n            ora_neutral_add(s, sp);",
+        ["subcyc_slice"]),
+    "subcyc_without_predictor": ("        slice_chi_cbq(s, j);\n        slice_predictor(s);\n        s->last_nsub = n;",
+                                 "        slice_chi_cbq(s, j);\n        s->last_nsub = n;", ["subcyc_slice"]),
 }
 
 

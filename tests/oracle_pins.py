@@ -1360,6 +1360,149 @@ def pin_neutral_add(L):
     assert np.max(np.abs(qn)) > 0 and o.species_raw(isp).contents.n_add == 0
 
 
+# ---------------------------------------------------------------------------
+# sub-cycling (proj_subcyc)
+# ---------------------------------------------------------------------------
+def pin_subcyc_step(L):
+    """get_subcyc_step (simulation_subcyc_class.f03:431-449): one step of dxi
+    while the expansion factor is at most expansion_fac_max; above it,
+    ceiling(factor / max) sub-cycles of dxi / n, unless dxi / n falls below
+    dt_2d_min, then floor(dxi / dt_2d_min) of them."""
+    cases = [((1.2, 1.2, 0.05, 0.001), (1, 0.05)),
+             ((1.0, 1.5, 0.05, 0.001), (1, 0.05)),
+             ((3.0, 1.2, 0.05, 0.001), (3, 0.05 / 3)),       # 2.5 -> 3
+             ((3.6000001, 1.2, 0.05, 0.001), (4, 0.05 / 4)),
+             ((100.0, 1.5, 0.05, 0.011), (4, 0.05 / 4)),     # 67 sub-cycles would be below dt_2d_min
+             ((1.0, 0.3, 0.05, 0.001), (4, 0.05 / 4))]       # an expansion_fac_max below 1 sub-cycles every slice
+    for args, (n, d) in cases:
+        got = po.subcyc_step(*args, L=L)
+        assert got[0] == n and abs(got[1] - d) <= 1e-16, (args, got, (n, d))
+
+
+def _planted_species(L, n=400, seed=3):
+    o = po.Oracle(po.make_grid(16, 4, 1, 0.25, 0.05), L)
+    o.add_species(po.species_cfg(2, 2, 8))      # room for 2 x 16 x 32 particles
+    o.add_species(po.species_cfg(2, 2, 4))
+    rng = np.random.default_rng(seed)
+    out = []
+    for isp, m in ((0, n), (1, n // 2)):
+        p = rng.normal(size=(m, 3)) * np.array([0.7, 0.7, 1.5])
+        x = rng.uniform(-3.0, 3.0, size=(m, 2))
+        g = np.sqrt(1.0 + (p ** 2).sum(axis=1))
+        a = {"x1": x[:, 0], "x2": x[:, 1], "p1": p[:, 0], "p2": p[:, 1], "p3": p[:, 2], "gamma": g,
+             "q": np.full(m, -0.01), "psi": np.zeros(m), "id": np.arange(m)}
+        o.set_species(isp, a)
+        out.append(a)
+    return o, out
+
+
+def pin_clamp_exp_fac(L):
+    """clamp_exp_fac (part2d_subcyc_class.f03:46-66): a particle with
+    gamma / (gamma - p_z) above F keeps the direction of its momentum, which
+    is scaled down so that the factor becomes F exactly, and gets
+    gamma = sqrt(1 + |p|^2); every other particle is left as it was."""
+    o, planted = _planted_species(L)
+    F = 1.8
+    L.ora_clamp_exp_fac(o.species_raw(0), F)
+    s = o.species(0)
+    a = planted[0]
+    ef0 = a["gamma"] / (a["gamma"] - a["p3"])
+    hit = ef0 > F
+    assert 20 < hit.sum() < len(hit) - 20, hit.sum()
+    for k in ("p1", "p2", "p3", "gamma"):
+        assert np.array_equal(s[k][~hit], a[k][~hit]), k
+    p0 = np.stack([a["p1"], a["p2"], a["p3"]], axis=1)[hit]
+    p1 = np.stack([s["p1"], s["p2"], s["p3"]], axis=1)[hit]
+    sc = np.linalg.norm(p1, axis=1) / np.linalg.norm(p0, axis=1)
+    assert np.all(sc < 1.0) and np.allclose(p1, p0 * sc[:, None], rtol=1e-13, atol=0)
+    g = s["gamma"][hit]
+    assert np.allclose(g, np.sqrt(1.0 + (p1 ** 2).sum(axis=1)), rtol=1e-15, atol=0)
+    assert np.allclose(g / (g - p1[:, 2]), F, rtol=1e-11, atol=0)
+
+
+def pin_exp_fac_max(L):
+    """get_exp_fac_max (sim_plasma_subcyc_class.f03:58-86,
+    part2d_subcyc_class.f03:27-44): the largest gamma / (gamma - p_z) over
+    every particle of every species, and 1 where none exceeds 1 (the
+    species loop starts from 1; an empty species gives 1)."""
+    o, planted = _planted_species(L)
+    want = max(1.0, max(float(np.max(a["gamma"] / (a["gamma"] - a["p3"]))) for a in planted))
+    assert L.ora_exp_fac_max(o.h) == want
+    for isp, a in enumerate(planted):            # every particle moving backwards: factor < 1
+        b = dict(a, p3=-np.abs(a["p3"]) - 0.1)
+        o.set_species(isp, b)
+    assert L.ora_exp_fac_max(o.h) == 1.0
+    o.set_species(0, {k: v[:0] for k, v in planted[0].items()})
+    o.set_species(1, {k: v[:0] for k, v in planted[1].items()})
+    assert L.ora_exp_fac_max(o.h) == 1.0
+
+
+def pin_subcyc_slice(L):
+    """The sub-cycled slice (simulation_subcyc_class.f03:230-390).
+    (1) A sub-cycling run whose expansion_fac_max and clamp are never
+    reached is the standard run, bit for bit: fields, histories, particles.
+    (2) A plasma and a lithium gas of density 1e-9 in a strong driver (their
+    own fields 1e-9 of the driver's, so every sub-cycle pushes in the same
+    fields): one slice of n sub-cycles moves the plasma as n pushes of
+    dxi / n, each push_u followed by clamp_exp_fac, and advances the gas as
+    n ionisations with the neutral's own step dxi, on the slice's final E and
+    B."""
+    import twin
+    from qpad_amd import configs
+    # (1)
+    deck = configs.subcycling(nr=24, nxi=32)
+    a = twin.oracle_from_deck(configs.with_overrides(deck, algorithm="standard"), L)
+    b = twin.oracle_from_deck(configs.with_overrides(deck, expansion_fac_max=1e30, expansion_fac_clamped=1e30,
+                                                     dt_2d_min=1e-9), L)
+    for o in (a, b):
+        o.step_begin()
+        for j in range(1, 33):
+            o.slice(j)
+    assert b.last_subcycles()[0] == 1
+    for fid in (po.PSI, po.E, po.E_SPE, po.B, po.B_SPE, po.CU, po.Q_SPE, po.Q_BEAM):
+        assert np.array_equal(a.hist(fid), b.hist(fid)), po.FIELD_NAMES[fid]
+    for k in ("x1", "x2", "p1", "p2", "p3", "gamma", "id"):
+        assert np.array_equal(a.species(0)[k], b.species(0)[k]), k
+    # (2)
+    F = 1.8
+    deck = configs.subcycling(nr=24, nxi=32, density=1e-9, exp_fac_max=0.45, exp_fac_clamped=F, neutral=(3, 3))
+    o = twin.oracle_from_deck(deck, L)
+    o.step_begin()
+    j0 = 22
+    for j in range(1, j0):
+        o.slice(j)
+    before = {k: np.array(v) for k, v in o.species(0).items()}
+    lv0 = o.neutral_levels(1).copy()
+    o.slice(j0)
+    n, dsub, ef = o.last_subcycles()
+    assert n >= 3 and abs(dsub - o.grid.dxi / n) <= 1e-17, (n, dsub)
+    e, bf = o.field(po.E).copy(), o.field(po.B).copy()
+    r = twin.oracle_from_deck(deck, L)
+    r.set_species(0, before)
+    sp = r.species_raw(0)
+    for _ in range(n):
+        L.ora_push_u_robust(r.h, sp, po.ptr(e), po.ptr(bf), dsub)
+        L.ora_clamp_exp_fac(sp, F)
+        L.ora_push_x(sp, dsub)
+        L.ora_update_bound(r.h, sp)
+    got, want = twin.particles_by_id(o.species(0)), twin.particles_by_id(r.species(0))
+    assert np.array_equal(got["id"], want["id"])
+    bid = twin.particles_by_id(before)
+    moved = np.max(np.abs(want["p1"] - bid["p1"][np.isin(bid["id"], want["id"])]))
+    assert moved > 1e-3, moved
+    for k in ("x1", "x2", "p1", "p2", "p3"):
+        assert np.max(np.abs(got[k] - want[k])) <= 1e-6 * max(1.0, np.max(np.abs(want[k]))), k
+    rl = r.neutral_levels(1)
+    rl[...] = lv0
+    for _ in range(n):
+        L.ora_neutral_ionize(r.h, r.species_raw(1), po.ptr(e))
+    lv = o.neutral_levels(1)
+    assert np.max(lv[:, :, 0]) > 1e-3, "the driver must ionise the gas"
+    assert np.max(np.abs(lv[:, :, :4] - rl[:, :, :4])) <= 1e-6
+    for x in (a, b, o, r):
+        x.close()
+
+
 PINS = {
     "matrices": pin_matrices,
     "ez": pin_ez,
@@ -1384,5 +1527,9 @@ PINS = {
     "adk_table": pin_adk_table,
     "neutral_ionize": pin_neutral_ionize,
     "neutral_add": pin_neutral_add,
+    "subcyc_step": pin_subcyc_step,
+    "clamp_exp_fac": pin_clamp_exp_fac,
+    "exp_fac_max": pin_exp_fac_max,
+    "subcyc_slice": pin_subcyc_slice,
 }
 

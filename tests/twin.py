@@ -50,13 +50,18 @@ def _as_oracle_struct(cls, src):
     return out
 
 
-def oracle_from_deck(deck):
+def oracle_from_deck(deck, L=None):
     """The oracle set up from a qpinput deck on its own, as qpad_amd.Simulation
-    sets up the library: same grid, same species and beam seeds; the oracle
-    injects the plasma and the beam itself."""
-    from qpad_amd.input import beam_from_deck, grid_from_deck, neutral_from_deck, species_from_deck
+    sets up the library: same grid, same species and beam seeds, sub-cycling
+    when the deck asks for it; the oracle injects the plasma and the beam
+    itself.  L: the oracle library to run (default: the built one)."""
+    from qpad_amd.input import (beam_from_deck, grid_from_deck, neutral_from_deck, species_from_deck,
+                                subcycling_from_deck)
     g, _, z0 = grid_from_deck(deck)
-    o = po.Oracle(_as_oracle_struct(po.Grid, g))
+    o = po.Oracle(_as_oracle_struct(po.Grid, g), L)
+    sc = subcycling_from_deck(deck)
+    if sc is not None:
+        o.set_subcycling(*sc)
     sim = deck["simulation"]
     seed = int(sim.get("random_seed", 10))
     nsp = sim.get("nspecies", len(deck.get("species", [])))
