@@ -223,6 +223,21 @@ int qpad_species_push_clamped(qpad_ctx *ctx, int32_t sp, double dxi, double exp_
    fused forms do it at the same point. */
 int qpad_species_interp_psi(qpad_ctx *ctx, int32_t sp);
 
+/* A non-uniform plasma (fdist2d's profiles, fdist2d_lib.f03:28-415): the
+   density of the species' next injections (qpad_species_renew, the renew of
+   qpad_step3d_end).  den_perp holds the transverse density get_den_perp
+   gives at every lattice point, [num_theta][nr][ppc1][ppc2] in
+   inject_fdist2d's loop order (fdist2d_class.f03:317-333: theta bin j,
+   radial cell i, i1, i2; the point at rn = (i1 - 1/2)/ppc1 + i - 1,
+   theta = ((i2 - 1/2)/ppc2 + j - 1) 2 pi / num_theta), den_lon the
+   longitudinal factor get_den_lon gives at the injection's propagation
+   distance s; a point with den_lon den_perp density < den_min is skipped,
+   and a kept one carries the charge rn den_perp den_lon density coef.  The
+   caller evaluates its profiles (at s = 0 before a renew of the new
+   species, at s = i dt before the renew that ends 3D step i).  den_perp =
+   NULL returns to the uniform plasma.  Not for neutrals. */
+int qpad_species_set_profile(qpad_ctx *ctx, int32_t sp, const double *den_perp, double den_lon);
+
 /* ---- sub-cycling (simulation.algorithm "subcycling", proj_subcyc) ----
    simulation_subcyc%new (simulation_subcyc_class.f03:76-88) with the deck's
    expansion_fac_max (> 0), expansion_fac_clamped (> 1) and dt_2d_min (in

@@ -83,7 +83,7 @@ class _Species(C.Structure):
                 ("is_neutral", C.c_int), ("multi_max", C.c_int), ("wp", C.c_double), ("rate", _dp),
                 ("multi_ion", _dp), ("ion_old", _dp), ("ppc_add", C.POINTER(C.c_int)),
                 ("add_x1", _dp), ("add_x2", _dp), ("add_q", _dp), ("n_add", C.c_int64),
-                ("n_added", C.c_int64), ("rho_hist", _dp)]
+                ("n_added", C.c_int64), ("rho_hist", _dp), ("den_perp", _dp), ("den_lon", C.c_double)]
 
 
 class NeutralCfg(C.Structure):
@@ -181,6 +181,7 @@ def load_library(path):
     L.ora_neutral_ion_deposit.argtypes = [C.c_void_p, C.POINTER(_Species)]
     L.ora_neutral_renew.argtypes = [C.c_void_p, C.POINTER(_Species)]
     L.ora_interp_point.argtypes = [C.c_void_p, _dp, C.c_int, C.c_double, C.c_double, _dp]
+    L.ora_species_set_profile.argtypes = [C.c_void_p, C.c_int, _dp, C.c_double]
     L.ora_set_subcycling.argtypes = [C.c_void_p, C.c_int, C.c_double, C.c_double, C.c_double]
     L.ora_exp_fac_max.restype = C.c_double
     L.ora_exp_fac_max.argtypes = [C.c_void_p]
@@ -414,6 +415,16 @@ class Oracle:
 
     def laser_slice(self, j):
         self.L.ora_laser_slice(self.h, j)
+
+    def set_profile(self, isp, den_perp, den_lon):
+        """the profile of species isp's next injections (den_perp None: uniform)"""
+        d = None if den_perp is None else np.ascontiguousarray(den_perp, dtype=np.float64)
+        assert self.L.ora_species_set_profile(self.h, isp, None if d is None else ptr(d), float(den_lon)) == 0
+
+    def renew_species(self, isp):
+        """species2d%renew: inject again with the current profile, charge set-up"""
+        self.L.ora_species_inject(self.h, isp)
+        self.L.ora_species_init_q(self.h, isp)
 
     def species_raw(self, isp):
         return self.L.ora_get_species(self.h, isp)

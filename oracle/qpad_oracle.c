@@ -1381,16 +1381,17 @@ void ora_species_inject(ora_sim *s, int isp)
     double dr = s->g.dr;
     double dtheta = 2.0 * PI_DB / (double)c->num_theta;
     int ppc_tot = c->ppc1 * c->ppc2;
-    double den_lon = 1.0, den_perp = 1.0;
+    const double den_lon = sp->den_perp ? sp->den_lon : 1.0;
     double coef = copysign(1.0, c->qm) / ((double)ppc_tot * (double)c->num_theta);
-    int64_t ip = 0;
+    int64_t ip = 0, l = 0;
     for (int j = 1; j <= c->num_theta; j++) {
         for (int i = 1; i <= nr; i++) {
             for (int i1 = 1; i1 <= c->ppc1; i1++) {
                 double rn = ((double)i1 - 0.5) / (double)c->ppc1 + (double)(i - 1);
-                for (int i2 = 1; i2 <= c->ppc2; i2++) {
+                for (int i2 = 1; i2 <= c->ppc2; i2++, l++) {
                     double theta = (((double)i2 - 0.5) / (double)c->ppc2 + (double)j - 1.0) * dtheta;
                     double xt1 = rn * dr * cos(theta), xt2 = rn * dr * sin(theta);
+                    const double den_perp = sp->den_perp ? sp->den_perp[l] : 1.0;
                     if (den_lon * den_perp * c->density < c->den_min) continue;
                     sp->x1[ip] = xt1;
                     sp->x2[ip] = xt2;
@@ -1411,6 +1412,20 @@ void ora_species_inject(ora_sim *s, int isp)
         }
     }
     sp->npp = ip;
+}
+
+int ora_species_set_profile(ora_sim *s, int isp, const double *den_perp, double den_lon)
+{
+    ora_species *sp = &s->sp[isp];
+    const size_t n = (size_t)sp->cfg.num_theta * s->g.nr * sp->cfg.ppc1 * sp->cfg.ppc2;
+    free(sp->den_perp);
+    sp->den_perp = NULL;
+    sp->den_lon = den_lon;
+    if (!den_perp) return 0;
+    sp->den_perp = malloc(n * sizeof(double));
+    if (!sp->den_perp) return -1;
+    memcpy(sp->den_perp, den_perp, n * sizeof(double));
+    return 0;
 }
 
 /* species2d%new / renew charge set-up: source/species/species2d_class.f03:
@@ -1818,7 +1833,7 @@ void ora_destroy(ora_sim *s)
         free(sp->q); free(sp->gamma); free(sp->psi); free(sp->id);
         free(sp->q_own); free(sp->qn); free(sp->cu_own); free(sp->acu_own); free(sp->amu_own);
         free(sp->q_hist);
-        free(sp->multi_ion); free(sp->ion_old); free(sp->ppc_add);
+        free(sp->multi_ion); free(sp->ion_old); free(sp->ppc_add); free(sp->den_perp);
         free(sp->add_x1); free(sp->add_x2); free(sp->add_q); free(sp->rho_hist);
     }
     for (int i = 0; i < s->nbeams; i++) {

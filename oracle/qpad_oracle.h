@@ -144,6 +144,11 @@ typedef struct {
                                           positions and ion charges        */
     int64_t n_add, n_added;   /* in part_add; released since the renew     */
     double *rho_hist;         /* rho_ion history [(nxi + 2)][nm][ld]       */
+    /* a non-uniform plasma: the transverse density at every lattice point
+       [num_theta][nr][ppc1][ppc2] (NULL: uniform) and the longitudinal
+       factor of the next injection */
+    double *den_perp;
+    double den_lon;
 } ora_species;
 
 /* neutrals(i) section (sim_plasma_class.f03:148-176, fdist2d for the
@@ -274,6 +279,8 @@ int ora_beam_set(ora_sim *s, int ib, int64_t n, const double *x1,
                  const double *x2, const double *x3, const double *p1,
                  const double *p2, const double *p3, const double *q);
 void ora_species_inject(ora_sim *s, int isp);
+/* the profile of the species' next injections (den_perp NULL: uniform) */
+int ora_species_set_profile(ora_sim *s, int isp, const double *den_perp, double den_lon);
 void ora_species_init_q(ora_sim *s, int isp);
 void ora_beam_inject(ora_sim *s, int ib, const ora_beam_cfg *cfg);
 void ora_beam_qdeposit(ora_sim *s, int ib, double *q_hist);

@@ -117,6 +117,7 @@ SIGNATURES = {
     "qpad_species_push_clamped": (C.c_int, [_CTX, C.c_int32, C.c_double, C.c_double]),
     "qpad_set_subcycling": (C.c_int, [_CTX, C.c_int32, C.c_double, C.c_double, C.c_double]),
     "qpad_exp_fac_max": (C.c_int, [_CTX, _dp]),
+    "qpad_species_set_profile": (C.c_int, [_CTX, C.c_int32, _dp, C.c_double]),
     "qpad_last_subcycles": (C.c_int, [_CTX, C.POINTER(C.c_int32), _dp, _dp]),
     "qpad_species_interp_psi": (C.c_int, [_CTX, C.c_int32]),
     "qpad_set_laser": (C.c_int, [_CTX, _dp, _dp]),

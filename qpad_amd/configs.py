@@ -155,6 +155,34 @@ def subcycling(nr=32, nxi=40, max_mode=1, n_peak=40.0, sigma_xi=0.2, exp_fac_max
     return d
 
 
+def profiled(nr=48, nxi=32, max_mode=1, profile=None, den_min=1e-10, uth=(0.0, 0.0, 0.0), n_peak=6.0, nstep=2,
+             dt=10.0, push_type="robust"):
+    """A reduced deck with a non-uniform plasma (species(i).profile and its
+    parameters, fdist2d_lib.f03): by default a parabolic channel cut at
+    r = 3 whose density rises along the propagation distance (piecewise
+    linear from 0.5 at s = 0 to 1.5 at s = 20); `profile` replaces those
+    keys."""
+    sp = _electrons(2, 4, push_type=push_type)
+    sp.update(profile or {"profile": ["parabolic-channel", "piecewise-linear"], "channel_n0": 0.6,
+                          "channel_depth": 0.4, "channel_r0": 1.0, "channel_width": 1.5, "channel_rmax": 3.0,
+                          "piecewise_s": [-1.0, 20.0], "piecewise_fs": [0.5, 1.5]})
+    sp["den_min"] = den_min
+    sp["uth"] = list(uth)
+    d = _deck(nr, 4.0, max_mode, nxi, 0.05, [sp], _beam(nxi * 0.05, 0.3, 0.3, n_peak, 1 << 12), dt=dt,
+              nstep=nstep)
+    return d
+
+
+def analytic_profile(nr=48, nxi=40, max_mode=1):
+    """QPAD's analytic-profile example (input_file/analytic_density_profiles/
+    qpinput.json) in small: the plasma density
+    z^2/(z + 10)^2 exp(-(x^2 + y^2)/50) of the propagation distance z and the
+    transverse position, behind a Gaussian driver (the example's beam is an
+    analytic lattice beam, which stays off this path)."""
+    return profiled(nr=nr, nxi=nxi, max_mode=max_mode, dt=2.0, profile={
+        "profile": ["analytic", "analytic"], "math_func": "z^2/(z + 10.0)^2 * exp(-(x^2 + y^2)/50.0)"})
+
+
 def ionization_example():
     """QPAD's own ionisation example, restated as data
     (input_file/ionization/qpinput.json, diagnostics left out): Nr 500,

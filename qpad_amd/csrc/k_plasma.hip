@@ -41,24 +41,44 @@ __device__ inline uint32_t cell_key(double x1, double x2, double idr, double edg
 }
 
 // ---------------------------------------------------------------------------
-// injection: fdist2d_class.f03:289-357 (uniform profiles, lattice theta)
+// injection: fdist2d_class.f03:289-357 (lattice theta).  A uniform plasma
+// keeps every lattice point; a profiled one (den_perp given) keeps the points
+// whose den_lon * den_perp * density reaches den_min, at the exclusive prefix
+// `off` of the kept flags, so that the kept particles keep QPAD's order and
+// their index (which also draws their thermal momenta) counts kept points.
 // ---------------------------------------------------------------------------
 struct InjArgs {
     int nr, ppc1, ppc2, nth;
     int64_t n;
-    double dr, idr, edge, density, coef, qm, dtheta;
+    double dr, idr, edge, density, coef, qm, dtheta, den_lon, den_min;
     double uth0, uth1, uth2;
     uint64_t seed;
+    const double *den;       // transverse density per lattice point (null: 1)
+    const int32_t *off;      // kept points' exclusive prefix (null: all kept)
     double *x1, *x2, *p1, *p2, *p3, *q, *gamma, *psi;
     int32_t *id;
     uint32_t *key;
 };
 
+// kept flag of every lattice point, 0 in the pad entry n
+__global__ void k_inject_keep(int64_t n, const double *den, double den_lon, double density, double den_min,
+                              int32_t *keep)
+{
+    const int64_t l = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (l > n) return;
+    keep[l] = l < n ? (den_lon * den[l] * density < den_min ? 0 : 1) : 0;
+}
+
 __global__ void k_inject(InjArgs a)
 {
-    int64_t ip = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (ip >= a.n) return;
-    int64_t t = ip;
+    const int64_t l = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (l >= a.n) return;
+    int64_t ip = l;
+    if (a.off) {
+        if (a.off[l + 1] == a.off[l]) return;
+        ip = a.off[l];
+    }
+    int64_t t = l;
     const int i2 = (int)(t % a.ppc2); t /= a.ppc2;
     const int i1 = (int)(t % a.ppc1); t /= a.ppc1;
     const int i = (int)(t % a.nr);
@@ -71,9 +91,10 @@ __global__ void k_inject(InjArgs a)
     if (a.uth0 != 0.0 || a.uth1 != 0.0 || a.uth2 != 0.0) normal4(a.seed, (uint64_t)ip, 0u, th);
     double p1 = a.uth0 * th[0], p2 = a.uth1 * th[1], p3 = a.uth2 * th[2];
     double gam = sqrt(1.0 + p1 * p1 + p2 * p2 + p3 * p3);
+    const double den_perp = a.den ? a.den[l] : 1.0;
     a.x1[ip] = x1;
     a.x2[ip] = x2;
-    a.q[ip] = rn * 1.0 * 1.0 * a.density * a.coef;
+    a.q[ip] = rn * den_perp * a.den_lon * a.density * a.coef;
     a.p1[ip] = p1;
     a.p2[ip] = p2;
     a.p3[ip] = p3;
@@ -93,8 +114,6 @@ int launch_inject(qpad_ctx *c, int s)
     a.ppc2 = cf.ppc2;
     a.nth = cf.num_theta;
     a.n = (int64_t)cf.num_theta * c->g.nr * cf.ppc1 * cf.ppc2;
-    // den_lon * den_perp * density < den_min drops every particle
-    if (1.0 * 1.0 * cf.density < cf.den_min) a.n = 0;
     a.dr = c->g.dr;
     a.idr = 1.0 / c->g.dr;
     a.edge = (double)c->g.nr * c->g.dr;
@@ -102,18 +121,47 @@ int launch_inject(qpad_ctx *c, int s)
     a.coef = copysign(1.0, cf.qm) / ((double)(cf.ppc1 * cf.ppc2) * (double)cf.num_theta);
     a.qm = cf.qm;
     a.dtheta = 2.0 * (4.0 * atan(1.0)) / (double)cf.num_theta;
+    a.den_lon = sp.den_perp ? sp.den_lon : 1.0;
+    a.den_min = cf.den_min;
     a.uth0 = cf.uth[0];
     a.uth1 = cf.uth[1];
     a.uth2 = cf.uth[2];
     a.seed = cf.seed;
+    a.den = sp.den_perp;
+    a.off = nullptr;
     a.x1 = sp.x1; a.x2 = sp.x2; a.p1 = sp.p1; a.p2 = sp.p2; a.p3 = sp.p3;
     a.q = sp.q; a.gamma = sp.gamma; a.psi = sp.psi; a.id = sp.id;
     a.key = sp.key;
-    sp.npp = a.n;
-    if (a.n > 0) {
+    int64_t kept = a.n;
+    if (sp.den_perp) {
+        hipLaunchKernelGGL(k_inject_keep, dim3((unsigned)((a.n + 1 + 255) / 256)), dim3(256), 0, c->stream, a.n,
+                           sp.den_perp, a.den_lon, a.density, a.den_min, sp.inj_keep);
+        QPAD_HIP_CHECK(hipGetLastError());
+        size_t bytes = sp.inj_tmp_bytes;
+        QPAD_HIP_CHECK(rocprim::exclusive_scan(sp.inj_tmp, bytes, sp.inj_keep, sp.inj_off, 0, (size_t)a.n + 1,
+                                               rocprim::plus<int32_t>(), c->stream));
+        QPAD_HIP_CHECK(hipMemcpyAsync(sp.inj_host, sp.inj_off + a.n, sizeof(int32_t), hipMemcpyDeviceToHost,
+                                      c->stream));
+        QPAD_HIP_CHECK(hipStreamSynchronize(c->stream));
+        kept = *sp.inj_host;
+        a.off = sp.inj_off;
+    } else if (1.0 * 1.0 * cf.density < cf.den_min) {
+        // den_lon * den_perp * density < den_min drops every particle
+        kept = 0;
+    }
+    sp.npp = kept;
+    if (kept > 0) {
         hipLaunchKernelGGL(k_inject, dim3((unsigned)((a.n + 255) / 256)), dim3(256), 0, c->stream, a);
         QPAD_HIP_CHECK(hipGetLastError());
     }
+    return 0;
+}
+
+// scan scratch of the profiled injection of n lattice points
+int inject_scan_bytes(int64_t n, size_t *bytes)
+{
+    int32_t *z = nullptr;
+    QPAD_HIP_CHECK(rocprim::exclusive_scan(nullptr, *bytes, z, z, 0, (size_t)n + 1, rocprim::plus<int32_t>()));
     return 0;
 }
 

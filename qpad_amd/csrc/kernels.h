@@ -73,6 +73,8 @@ int src_rows_host(int M, int nm);
 
 // ---- plasma (k_plasma.hip) ----
 int launch_inject(qpad_ctx *c, int sp);
+// scan scratch of a profiled injection of n lattice points
+int inject_scan_bytes(int64_t n, size_t *bytes);
 // cell_start[c] = lower bound of c in the sorted keys, c = 0 .. ncs-1
 int launch_cell_start(qpad_ctx *c, const uint32_t *key, int64_t n, int ncs, int32_t *cs);
 int launch_sort(qpad_ctx *c, int sp);            // sort by key, permute, cell starts

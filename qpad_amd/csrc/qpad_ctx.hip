@@ -241,6 +241,11 @@ static void species_free(Species &sp)
     dfree(sp.add_x1); dfree(sp.add_x2); dfree(sp.add_q); dfree(sp.nstate); dfree(sp.h_rho);
     if (sp.scan_tmp) (void)hipFree(sp.scan_tmp);
     sp.scan_tmp = nullptr;
+    dfree(sp.den_perp); dfree(sp.inj_keep); dfree(sp.inj_off);
+    if (sp.inj_tmp) (void)hipFree(sp.inj_tmp);
+    sp.inj_tmp = nullptr;
+    if (sp.inj_host) (void)hipHostFree(sp.inj_host);
+    sp.inj_host = nullptr;
     if (sp.nstate_host) (void)hipHostFree(sp.nstate_host);
     sp.nstate_host = nullptr;
 }
@@ -1376,6 +1381,47 @@ int qpad_set_subcycling(qpad_ctx *c, int32_t on, double exp_fac_max, double exp_
     c->sc_fac_clamped = exp_fac_clamped;
     c->sc_dt_min = dt_2d_min;
     return 0;
+}
+
+int qpad_species_set_profile(qpad_ctx *c, int32_t s, const double *den_perp, double den_lon)
+{
+    if (!c || s < 0 || s >= c->nspecies) return fail(QPAD_E_INVALID, "bad species");
+    Species &sp = c->sp[s];
+    if (sp.neutral) return fail(QPAD_E_INVALID, "neutral profiles are uniform on this path");
+    if (!(den_lon >= 0.0 && den_lon < INFINITY)) return fail(QPAD_E_INVALID, "den_lon must be finite and >= 0");
+    const int64_t n = (int64_t)sp.cfg.num_theta * c->g.nr * sp.cfg.ppc1 * sp.cfg.ppc2;
+    if (!den_perp) {
+        // uniform again: every lattice point, no longitudinal factor
+        (void)hipStreamSynchronize(c->stream);
+        dfree(sp.den_perp);
+        sp.den_perp = nullptr;
+        sp.den_lon = 1.0;
+        return 0;
+    }
+    for (int64_t l = 0; l < n; l++)
+        if (!(den_perp[l] < INFINITY && den_perp[l] > -INFINITY))
+            return fail(QPAD_E_INVALID, "den_perp must be finite");
+    if (!sp.den_perp) {
+        int rc = dalloc(&sp.den_perp, (size_t)n);
+        if (!rc) rc = dalloc(&sp.inj_keep, (size_t)n + 1);
+        if (!rc) rc = dalloc(&sp.inj_off, (size_t)n + 1);
+        if (!rc) rc = inject_scan_bytes(n, &sp.inj_tmp_bytes);
+        if (!rc && hipMalloc(&sp.inj_tmp, std::max<size_t>(sp.inj_tmp_bytes, 1)) != hipSuccess)
+            rc = fail(QPAD_E_NOMEM, "hipMalloc of the injection scan scratch failed");
+        if (!rc && hipHostMalloc((void **)&sp.inj_host, sizeof(int32_t), hipHostMallocDefault) != hipSuccess)
+            rc = fail(QPAD_E_NOMEM, "hipHostMalloc failed");
+        if (rc) {
+            dfree(sp.den_perp); dfree(sp.inj_keep); dfree(sp.inj_off);
+            if (sp.inj_tmp) (void)hipFree(sp.inj_tmp);
+            sp.inj_tmp = nullptr;
+            sp.den_perp = nullptr;
+            return rc;
+        }
+    }
+    QPAD_HIP_CHECK(hipMemcpyAsync(sp.den_perp, den_perp, (size_t)n * sizeof(double), hipMemcpyHostToDevice,
+                                  c->stream));
+    sp.den_lon = den_lon;
+    return sync(c);
 }
 
 int qpad_exp_fac_max(qpad_ctx *c, double *out)

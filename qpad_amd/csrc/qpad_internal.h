@@ -87,6 +87,16 @@ struct Species {
     void *scan_tmp;
     size_t scan_bytes;
     double *h_rho;           // rho_ion history [slice 0..nxi+1][nm][ld]
+    // a non-uniform plasma (qpad_species_set_profile): the transverse
+    // density of every lattice point [num_theta][nr][ppc1][ppc2] and the
+    // longitudinal factor of the next injection; the kept points' exclusive
+    // prefix [lattice + 1] and its scan scratch
+    double *den_perp;
+    double den_lon;
+    int32_t *inj_keep, *inj_off;
+    void *inj_tmp;
+    size_t inj_tmp_bytes;
+    int32_t *inj_host;       // pinned: particles the last injection kept
 };
 
 struct Beam {

@@ -212,6 +212,18 @@ module qpad_hip_mod
       integer(c_int) :: rc
     end function
 
+    ! a non-uniform plasma: the transverse density of every lattice point and
+    ! the longitudinal factor of the next injections
+    function qpad_species_set_profile(ctx, sp, den_perp, den_lon) bind(C, name="qpad_species_set_profile") &
+        result(rc)
+      import :: c_int, c_int32_t, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      integer(c_int32_t), value :: sp
+      type(c_ptr), value :: den_perp
+      real(c_double), value :: den_lon
+      integer(c_int) :: rc
+    end function
+
     ! species2d_subcyc's push: push_u, clamp_exp_fac, push_x
     function qpad_species_push_clamped(ctx, sp, dxi, exp_fac_clamped) bind(C, name="qpad_species_push_clamped") &
         result(rc)
@@ -725,7 +737,7 @@ module qpad_hip_mod
   public :: qpad_last_error, qpad_abi_version, qpad_create, qpad_destroy, qpad_get_grid, &
     qpad_species_new, qpad_species_renew, qpad_species_qdp, qpad_species_qdp_noalloc, &
     qpad_species_charge_history, qpad_species_cbq, qpad_get_species_charge, qpad_species_amjdp, &
-    qpad_species_push, qpad_species_push_clamped, qpad_set_subcycling, qpad_exp_fac_max, qpad_last_subcycles, &
+    qpad_species_push, qpad_species_push_clamped, qpad_species_set_profile, qpad_set_subcycling, qpad_exp_fac_max, qpad_last_subcycles, &
     qpad_species_interp_psi, qpad_step_iterations, qpad_field_solve_psi, qpad_field_solve_ez, &
     qpad_set_laser, qpad_laser_slice, qpad_deposit_chi, &
     qpad_field_solve_et, qpad_field_solve_bz, qpad_field_solve_bt_beam, &

@@ -124,11 +124,18 @@ def neutral_from_deck(nt, n0):
                       PUSH2_STD if push == "standard" else PUSH2_ROBUST, float(n0))
 
 
+def profile_from_deck(sp):
+    """species(i).profile and its parameters (fdist2d_class.f03:140-215): a
+    PlasmaProfile, or None for a uniform plasma"""
+    from .profiles import PlasmaProfile
+    prof = PlasmaProfile(sp)
+    return None if prof.uniform else prof
+
+
 def species_from_deck(sp, seed=0):
-    """fdist2d_class.f03:117-272, sim_plasma_class.f03:116-157."""
-    prof = _get(sp, "profile")
-    if list(prof) != ["uniform", "uniform"]:
-        raise QpadError("only uniform plasma profiles are on the GPU path")
+    """fdist2d_class.f03:117-272, sim_plasma_class.f03:116-157 (the density
+    profile: profile_from_deck)."""
+    profile_from_deck(sp)   # checks the profile section
     push = _get(sp, "push_type")
     if push not in _PUSHERS:
         raise QpadError('Invalid pusher type! Only "standard", "robust", "standard_pgc" and "robust_pgc" '

@@ -11,7 +11,8 @@ import ctypes as C
 import numpy as np
 
 from . import _lib as L
-from .input import (FileBeam, beam_from_deck, grid_from_deck, neutral_from_deck, place_file_beam, subcycling_from_deck,
+from .input import (FileBeam, beam_from_deck, grid_from_deck, neutral_from_deck, place_file_beam, profile_from_deck,
+                    subcycling_from_deck,
                     species_from_deck)
 
 
@@ -100,12 +101,32 @@ class Species2D:
     """species2d (species2d_class.f03:24-54): standard, robust, standard_pgc
     or robust_pgc pusher."""
 
-    def __init__(self, sim, cfg):
+    def __init__(self, sim, cfg, profile=None):
         self.sim = sim
         sid = C.c_int32(0)
         L.check(sim.lib.qpad_species_new(sim.ctx, C.byref(cfg), C.byref(sid)))
         self.id = sid.value
         self.cfg = cfg
+        self.profile = None
+        if profile is not None:
+            self.set_profile(profile, 0.0)
+            self.renew()
+
+    def set_profile(self, profile, s):
+        """the density of the species' next injections: a PlasmaProfile
+        (qpad_amd.profiles) evaluated at the injection lattice and at
+        propagation distance s (inject_fdist2d, fdist2d_class.f03:312-336);
+        None: uniform"""
+        self.profile = profile
+        if profile is None:
+            L.check(self.sim.lib.qpad_species_set_profile(self.sim.ctx, self.id, None, 1.0))
+            return
+        from .profiles import plasma_lattice
+        g, c = self.sim.grid, self.cfg
+        x1, x2 = plasma_lattice(g.nr, g.dr, c.ppc1, c.ppc2, c.num_theta)
+        den = np.ascontiguousarray(profile.den_perp(x1, x2, s), dtype=np.float64)
+        L.check(self.sim.lib.qpad_species_set_profile(self.sim.ctx, self.id, L.dptr(den),
+                                                      float(profile.den_lon(s))))
 
     def renew(self):
         L.check(self.sim.lib.qpad_species_renew(self.sim.ctx, self.id))
@@ -335,7 +356,7 @@ class Simulation:
             ("ar", Field), ("ai", Field), ("ar_grad", Field), ("ai_grad", Field), ("chi", Field))}
         nsp = deck["simulation"].get("nspecies", len(deck.get("species", [])))
         nbm = deck["simulation"].get("nbeams", len(deck.get("beam", [])))
-        self.spe = [Species2D(self, species_from_deck(sp, seed=seed + 1000 * k))
+        self.spe = [Species2D(self, species_from_deck(sp, seed=seed + 1000 * k), profile_from_deck(sp))
                     for k, sp in enumerate(deck.get("species", [])[:nsp])]
         # neutrals after the species, as simulation%new creates them
         nnt = deck["simulation"].get("nneutrals", len(deck.get("neutrals", [])))
@@ -398,12 +419,21 @@ class Simulation:
         L.check(self.lib.qpad_slice(self.ctx, int(j), C.byref(it)))
         return it.value
 
+    def _profiles_at(self, s):
+        """the profiled species' densities for the renew at propagation
+        distance s (species2d%renew(s), simulation_class.f03:497-501)"""
+        for sp in self.spe:
+            if getattr(sp, "profile", None) is not None:
+                sp.set_profile(sp.profile, s)
+
     def step3d_end(self):
+        self._profiles_at((self.tstep + 1) * self.grid.dt)
         L.check(self.lib.qpad_step3d_end(self.ctx))
         self.tstep += 1
 
     def step3d(self):
         it = C.c_int32(0)
+        self._profiles_at((self.tstep + 1) * self.grid.dt)
         L.check(self.lib.qpad_step3d(self.ctx, C.byref(it)))
         self.tstep += 1
         return it.value
@@ -430,6 +460,7 @@ class Simulation:
             L.check(self.lib.qpad_beam_push(self.ctx))
             self.tstep += 1
             diag.run(self.tstep)
+            self._profiles_at(self.tstep * self.grid.dt)
             for sp in self.spe:
                 sp.renew()
         return total

@@ -46,7 +46,8 @@ def test_total_charge_definition():
 
 @pytest.mark.parametrize("edit,match", [
     (lambda d: d["species"][0].update(push_type="boris"), "pusher"),
-    (lambda d: d["species"][0].update(profile=["parabolic-channel", "uniform"]), "uniform"),
+    (lambda d: d["species"][0].update(profile=["gaussian-channel", "uniform"]), "Invalid density profile"),
+    (lambda d: d["species"][0].update(profile=["parabolic-channel", "uniform"]), "channel_n0"),
     (lambda d: d["simulation"].update(nlasers=1), "nlasers"),
     (lambda d: d["simulation"].update(field_boundary="periodic"), "boundary"),
     (lambda d: d["beam"][0].update(profile_type="gaussian"), "Invalid beam profile type"),

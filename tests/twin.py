@@ -25,8 +25,11 @@ def make_twin(sim):
         if getattr(sp, "neutral", False):
             o.add_neutral(_as_oracle_struct(po.NeutralCfg, c))
             continue
-        o.add_species(po.species_cfg(c.ppc1, c.ppc2, c.num_theta, c.qm, c.mass, c.density,
-                                     tuple(c.uth), bool(c.neutralized), c.den_min, c.seed, c.push_type))
+        k = o.add_species(po.species_cfg(c.ppc1, c.ppc2, c.num_theta, c.qm, c.mass, c.density,
+                                         tuple(c.uth), bool(c.neutralized), c.den_min, c.seed, c.push_type))
+        if getattr(sp, "profile", None) is not None:
+            oracle_profile(o, k, sp.profile, 0.0)
+            o.renew_species(k)
     for b in sim.beam:
         arr = b.get()
         c = b.cfg
@@ -55,8 +58,8 @@ def oracle_from_deck(deck, L=None):
     sets up the library: same grid, same species and beam seeds, sub-cycling
     when the deck asks for it; the oracle injects the plasma and the beam
     itself.  L: the oracle library to run (default: the built one)."""
-    from qpad_amd.input import (beam_from_deck, grid_from_deck, neutral_from_deck, species_from_deck,
-                                subcycling_from_deck)
+    from qpad_amd.input import (beam_from_deck, grid_from_deck, neutral_from_deck, profile_from_deck,
+                                species_from_deck, subcycling_from_deck)
     g, _, z0 = grid_from_deck(deck)
     o = po.Oracle(_as_oracle_struct(po.Grid, g), L)
     sc = subcycling_from_deck(deck)
@@ -68,6 +71,9 @@ def oracle_from_deck(deck, L=None):
     nbm = sim.get("nbeams", len(deck.get("beam", [])))
     for k, sp in enumerate(deck.get("species", [])[:nsp]):
         o.add_species(_as_oracle_struct(po.SpeciesCfg, species_from_deck(sp, seed=seed + 1000 * k)))
+        if profile_from_deck(sp) is not None:
+            oracle_profile(o, k, profile_from_deck(sp), 0.0)
+            o.renew_species(k)
     nnt = sim.get("nneutrals", len(deck.get("neutrals", [])))
     for nt in deck.get("neutrals", [])[:nnt]:
         o.add_neutral(_as_oracle_struct(po.NeutralCfg, neutral_from_deck(nt, sim.get("n0", 1.0e16))))
@@ -88,6 +94,26 @@ def oracle_from_deck(deck, L=None):
         else:
             o.add_beam(_as_oracle_struct(po.BeamCfg, cfg))
     return o
+
+
+def oracle_profile(o, isp, profile, s):
+    """species isp's profile at propagation distance s on the oracle, as
+    Species2D.set_profile gives it to the library"""
+    from qpad_amd.profiles import plasma_lattice
+    c = o.species_raw(isp).contents.cfg
+    g = o.grid
+    x1, x2 = plasma_lattice(g.nr, g.dr, c.ppc1, c.ppc2, c.num_theta)
+    o.set_profile(isp, profile.den_perp(x1, x2, s), profile.den_lon(s))
+
+
+def oracle_profiles_at(o, deck, s):
+    """every profiled species of the deck at s (before the oracle's renew)"""
+    from qpad_amd.input import profile_from_deck
+    nsp = deck["simulation"].get("nspecies", len(deck.get("species", [])))
+    for k, sp in enumerate(deck.get("species", [])[:nsp]):
+        prof = profile_from_deck(sp)
+        if prof is not None:
+            oracle_profile(o, k, prof, s)
 
 
 def laser_envelope(g, nm, a0=1.0, w0=1.0, xi_c=None, length=0.6, m1=0.15, curv=0.3):
