@@ -350,6 +350,22 @@ int qpad_beam_new(qpad_ctx *ctx, const qpad_beam_cfg *cfg, int32_t *beam_id);
    beam3d_class.f03:121-177, fdist3d_std_class.f03:439-613; the beam holds
    exactly the injected particles (QPAD's npmax is not needed) */
 int qpad_beam_new_std(qpad_ctx *ctx, const qpad_beam_std_cfg *cfg, int32_t *beam_id);
+/* The lattice beam with a density the caller evaluates: the profiles of
+   fdist3d_std_lib.f03 this library does not evaluate itself (super-gaussian
+   :129-172, parabolic :174-214, rational :216-286, analytic :46-84 with its
+   math_func of x, y and z) or any other.  den holds the density of every
+   lattice point of the box xi cells k0 .. k0 + nk - 1 (1-based), theta
+   sectors 1 .. num_theta, r cells 1 .. ni, in the order k > j > i > i3 > i2
+   > i1 of inject_fdist3d_std (fdist3d_std_class.f03:439-530; the point at
+   rn = (i1 - 1/2)/ppc1 + i - 1, theta = ((i2 - 1/2)/ppc2 + j - 1) 2 pi /
+   num_theta, zn = (i3 - 1/2)/ppc3 + k - 1, in cfg's geometry), without the
+   factor cfg->density; the box must hold the one qpad_beam_lattice_box
+   gives for cfg's ranges.  cfg's profile fields are not read; everything
+   else (ranges, den_min, quiet start, Twiss, offsets, momenta) is as in
+   qpad_beam_new_std. */
+int qpad_beam_lattice_box(qpad_ctx *ctx, const qpad_beam_std_cfg *cfg, int32_t *k0, int32_t *nk, int32_t *ni);
+int qpad_beam_new_std_density(qpad_ctx *ctx, const qpad_beam_std_cfg *cfg, const double *den, int32_t k0,
+                              int32_t nk, int32_t ni, int32_t *beam_id);
 /* beam3d%new with given particles: the "file" profile
    (fdist3d_file_class.f03:154-343), whose host side reads and places them
    (qpad_diag_read_beam_file); x3 relative to the box start */

@@ -133,6 +133,7 @@ def load_library(path):
     L.ora_add_species.argtypes = [C.c_void_p, C.POINTER(SpeciesCfg)]
     L.ora_add_beam.argtypes = [C.c_void_p, C.POINTER(BeamCfg), C.c_int64]
     L.ora_add_beam_std.argtypes = [C.c_void_p, C.POINTER(BeamStdCfg)]
+    L.ora_add_beam_std_table.argtypes = [C.c_void_p, C.POINTER(BeamStdCfg), _dp, C.c_int, C.c_int, C.c_int]
     L.ora_beam_set.argtypes = [C.c_void_p, C.c_int, C.c_int64] + [_dp] * 7
     L.ora_step3d_begin.argtypes = [C.c_void_p]
     L.ora_slice.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_int)]
@@ -356,6 +357,13 @@ class Oracle:
 
     def add_beam_std(self, cfg):
         r = self.L.ora_add_beam_std(self.h, C.byref(cfg))
+        self.nbeams += 1
+        return r
+
+    def add_beam_std_table(self, cfg, den, k0, nk, ni):
+        """the lattice beam with a tabulated density over the box k0, nk, ni"""
+        d = np.ascontiguousarray(den, dtype=np.float64)
+        r = self.L.ora_add_beam_std_table(self.h, C.byref(cfg), ptr(d), int(k0), int(nk), int(ni))
         self.nbeams += 1
         return r
 

@@ -1522,6 +1522,11 @@ static double std_profile(const ora_beam_std_cfg *c, int d, double x)
    and longitudinal partition).  Pass 0 counts the kept lattice points, pass
    1 stores them; momenta draw from Philox (counter = kept index) instead of
    gfortran's rand_norm. */
+/* a tabulated lattice density (ora_add_beam_std_table): the box xi cells
+   k0 .. k0 + nk - 1, r cells 1 .. ni, in the lattice order */
+static const double *g_tab;
+static int g_tab_k0, g_tab_nk, g_tab_ni;
+
 static int64_t std_lattice(ora_sim *s, const ora_beam_std_cfg *c, ora_beam *bm, int store)
 {
     int nrp = s->g.nr, nzp = s->g.nxi, nth = c->num_theta;
@@ -1552,9 +1557,18 @@ static int64_t std_lattice(ora_sim *s, const ora_beam_std_cfg *c, ora_beam *bm, 
                             for (int d = 0; d < 3; d++)
                                 if (xt[d] < c->range_lo[d] || xt[d] > c->range_hi[d]) out = 1;
                             if (out) continue;
-                            double dv[3];
-                            for (int d = 0; d < 3; d++) dv[d] = std_profile(c, d, xt[d]);
-                            double den_loc = dv[0] * dv[1] * dv[2] * c->density;
+                            double den_loc;
+                            if (g_tab) {
+                                if (k < g_tab_k0 || k >= g_tab_k0 + g_tab_nk || i > g_tab_ni) abort();
+                                const int64_t L = (((((int64_t)(k - g_tab_k0) * nth + (j - 1)) * g_tab_ni + (i - 1))
+                                                    * c->ppc[2] + (i3 - 1)) * c->ppc[1] + (i2 - 1)) * c->ppc[0]
+                                                  + (i1 - 1);
+                                den_loc = g_tab[L] * c->density;
+                            } else {
+                                double dv[3];
+                                for (int d = 0; d < 3; d++) dv[d] = std_profile(c, d, xt[d]);
+                                den_loc = dv[0] * dv[1] * dv[2] * c->density;
+                            }
                             if (den_loc < c->den_min) continue;
                             if (store) {
                                 if (c->geometry == 1) {
@@ -1580,6 +1594,17 @@ static int64_t std_lattice(ora_sim *s, const ora_beam_std_cfg *c, ora_beam *bm, 
                     }
                 }
     return ip;
+}
+
+int ora_add_beam_std_table(ora_sim *s, const ora_beam_std_cfg *c, const double *den, int k0, int nk, int ni)
+{
+    g_tab = den;
+    g_tab_k0 = k0;
+    g_tab_nk = nk;
+    g_tab_ni = ni;
+    const int r = ora_add_beam_std(s, c);
+    g_tab = NULL;
+    return r;
 }
 
 int ora_add_beam_std(ora_sim *s, const ora_beam_std_cfg *c)

@@ -275,6 +275,9 @@ void ora_destroy(ora_sim *s);
 int ora_add_species(ora_sim *s, const ora_species_cfg *cfg);
 int ora_add_beam(ora_sim *s, const ora_beam_cfg *cfg, int64_t npmax);
 int ora_add_beam_std(ora_sim *s, const ora_beam_std_cfg *cfg);
+/* the lattice beam with the density of every point of the box xi cells
+   k0 .. k0 + nk - 1, r cells 1 .. ni given (without cfg's density factor) */
+int ora_add_beam_std_table(ora_sim *s, const ora_beam_std_cfg *cfg, const double *den, int k0, int nk, int ni);
 int ora_beam_set(ora_sim *s, int ib, int64_t n, const double *x1,
                  const double *x2, const double *x3, const double *p1,
                  const double *p2, const double *p3, const double *q);

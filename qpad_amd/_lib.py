@@ -118,6 +118,10 @@ SIGNATURES = {
     "qpad_set_subcycling": (C.c_int, [_CTX, C.c_int32, C.c_double, C.c_double, C.c_double]),
     "qpad_exp_fac_max": (C.c_int, [_CTX, _dp]),
     "qpad_species_set_profile": (C.c_int, [_CTX, C.c_int32, _dp, C.c_double]),
+    "qpad_beam_lattice_box": (C.c_int, [_CTX, C.POINTER(BeamStdCfg), C.POINTER(C.c_int32), C.POINTER(C.c_int32),
+                                        C.POINTER(C.c_int32)]),
+    "qpad_beam_new_std_density": (C.c_int, [_CTX, C.POINTER(BeamStdCfg), _dp, C.c_int32, C.c_int32, C.c_int32,
+                                            C.POINTER(C.c_int32)]),
     "qpad_last_subcycles": (C.c_int, [_CTX, C.POINTER(C.c_int32), _dp, _dp]),
     "qpad_species_interp_psi": (C.c_int, [_CTX, C.c_int32]),
     "qpad_set_laser": (C.c_int, [_CTX, _dp, _dp]),

@@ -183,6 +183,36 @@ def analytic_profile(nr=48, nxi=40, max_mode=1):
         "profile": ["analytic", "analytic"], "math_func": "z^2/(z + 10.0)^2 * exp(-(x^2 + y^2)/50.0)"})
 
 
+def analytic_example():
+    """QPAD's analytic-profile example, restated as data
+    (input_file/analytic_density_profiles/qpinput.json, diagnostics and
+    restart settings left out): Nr 1024, r_max 60, Nxi 1024 over [-2, 12],
+    modes 0..1, a lattice beam of density exp(-z^2/0.5) exp(-(x^2+y^2)/8)
+    (ppc 2x2x2, 8 theta sectors, quiet start) and electrons of density
+    z^2/(z+10)^2 exp(-(x^2+y^2)/50) (ppc 2x2, 8 theta sectors), 100 3D
+    steps of dt 2 (run here for as many as asked)."""
+    return {
+        "simulation": {
+            "nodes": [1, 1], "grid": [1024, 1024], "box": {"r": [0.0, 60.0], "z": [-2.0, 12.0]},
+            "field_boundary": "open", "max_mode": 1, "interpolation": "linear", "n0": 1.0e16,
+            "time": 200.1, "dt": 2.0, "nbeams": 1, "nspecies": 1, "nneutrals": 0, "nlasers": 0,
+            "iter_max": 10, "iter_reltol": 1.0e-2, "iter_abstol": 1.0e-3, "random_seed": 10,
+            "algorithm": "standard",
+        },
+        "beam": [{
+            "npmax": 1000000, "ppc": [2, 2, 2], "num_theta": 8, "evolution": True, "quiet_start": True,
+            "geometry": "cartesian", "math_func": "exp(-z^2/0.5) * exp(-(x^2 + y^2)/8.0)",
+            "profile": ["analytic", "analytic", "analytic"], "q": -1.0, "m": 1.0, "density": 1.0,
+            "gamma": 20000.0, "uth": [10.0, 10.0, 0.0], "range1": [-10, 10], "range2": [-10, 10],
+            "range3": [-2, 2],
+        }],
+        "species": [{
+            "profile": ["analytic", "analytic"], "math_func": "z^2/(z + 10.0)^2 * exp(-(x^2 + y^2)/50.0)",
+            "ppc": [2, 2], "num_theta": 8, "q": -1.0, "m": 1.0, "density": 1.0, "push_type": "robust",
+        }],
+    }
+
+
 def ionization_example():
     """QPAD's own ionisation example, restated as data
     (input_file/ionization/qpinput.json, diagnostics left out): Nr 500,

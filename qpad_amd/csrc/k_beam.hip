@@ -175,6 +175,7 @@ struct LatticeArgs {
     qpad_beam_std_cfg cf;
     int64_t n;              // lattice points in the box
     int k0, ni;             // xi cells k0 .. (1-based), r cells 1 .. ni
+    const double *den_tab;  // the density of every box point, profiles aside (null: the profiles)
     double dr, dz, dtheta, coef;
     uint32_t *flag;
     const uint32_t *pos;
@@ -204,6 +205,7 @@ __device__ double lattice_profile(const qpad_beam_std_cfg &c, int d, double x)
 __device__ bool lattice_point(const LatticeArgs &a, int64_t L, double xt[3], double *rn, double *den)
 {
     const qpad_beam_std_cfg &c = a.cf;
+    const int64_t L0 = L;
     const int i1 = (int)(L % c.ppc[0]) + 1;
     L /= c.ppc[0];
     const int i2 = (int)(L % c.ppc[1]) + 1;
@@ -227,9 +229,13 @@ __device__ bool lattice_point(const LatticeArgs &a, int64_t L, double xt[3], dou
     xt[2] = zn * a.dz;
     for (int d = 0; d < 3; d++)
         if (xt[d] < c.range_lo[d] || xt[d] > c.range_hi[d]) return false;
-    double dv[3];
-    for (int d = 0; d < 3; d++) dv[d] = lattice_profile(c, d, xt[d]);
-    *den = dv[0] * dv[1] * dv[2] * c.density;
+    if (a.den_tab) {
+        *den = a.den_tab[L0] * c.density;
+    } else {
+        double dv[3];
+        for (int d = 0; d < 3; d++) dv[d] = lattice_profile(c, d, xt[d]);
+        *den = dv[0] * dv[1] * dv[2] * c.density;
+    }
     return !(*den < c.den_min);
 }
 
@@ -321,11 +327,34 @@ static int lattice_args(qpad_ctx *c, const qpad_beam_std_cfg &cf, LatticeArgs *a
     return 0;
 }
 
-int lattice_count(qpad_ctx *c, const qpad_beam_std_cfg &cf, LatticePlan *pl)
+// the lattice box of a tabulated density (k0, nk, ni) in place of the one
+// the ranges give
+static void lattice_box(LatticeArgs *a, const LatticeTable *t, const qpad_beam_std_cfg &cf)
+{
+    if (!t) return;
+    a->den_tab = t->den;
+    a->k0 = t->k0;
+    a->ni = t->ni;
+    a->n = (int64_t)t->nk * cf.num_theta * t->ni * cf.ppc[0] * cf.ppc[1] * cf.ppc[2];
+}
+
+int lattice_range_box(qpad_ctx *c, const qpad_beam_std_cfg &cf, int *k0, int *nk, int *ni)
+{
+    LatticeArgs a;
+    QPAD_TRY(lattice_args(c, cf, &a));
+    const int64_t per_k = (int64_t)cf.num_theta * a.ni * cf.ppc[0] * cf.ppc[1] * cf.ppc[2];
+    *k0 = a.k0;
+    *ni = a.ni;
+    *nk = per_k > 0 ? (int)(a.n / per_k) : 0;
+    return 0;
+}
+
+int lattice_count(qpad_ctx *c, const qpad_beam_std_cfg &cf, LatticePlan *pl, const LatticeTable *tab)
 {
     memset((void *)pl, 0, sizeof(*pl));
     LatticeArgs a;
     QPAD_TRY(lattice_args(c, cf, &a));
+    lattice_box(&a, tab, cf);
     pl->n = a.n;
     if (a.n == 0) return 0;
     if (a.n >= ((int64_t)1 << 31)) {
@@ -356,12 +385,13 @@ int lattice_count(qpad_ctx *c, const qpad_beam_std_cfg &cf, LatticePlan *pl)
     return 0;
 }
 
-int lattice_write(qpad_ctx *c, const qpad_beam_std_cfg &cf, const LatticePlan &pl, Beam &bm)
+int lattice_write(qpad_ctx *c, const qpad_beam_std_cfg &cf, const LatticePlan &pl, Beam &bm, const LatticeTable *tab)
 {
     int64_t nk = pl.kept;
     if (nk > 0) {
         LatticeArgs a;
         QPAD_TRY(lattice_args(c, cf, &a));
+        lattice_box(&a, tab, cf);
         a.flag = pl.flag;
         a.pos = pl.pos;
         a.x1 = bm.x1; a.x2 = bm.x2; a.x3 = bm.x3;

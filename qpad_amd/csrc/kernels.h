@@ -157,8 +157,16 @@ struct LatticePlan {
     void *tmp;
     size_t tmp_bytes;
 };
-int lattice_count(qpad_ctx *c, const qpad_beam_std_cfg &cf, LatticePlan *pl);
-int lattice_write(qpad_ctx *c, const qpad_beam_std_cfg &cf, const LatticePlan &pl, Beam &bm);
+// a tabulated lattice density: the box xi cells k0 .. k0 + nk - 1 and r
+// cells 1 .. ni, den in the lattice order (device memory)
+struct LatticeTable {
+    const double *den;
+    int k0, nk, ni;
+};
+int lattice_range_box(qpad_ctx *c, const qpad_beam_std_cfg &cf, int *k0, int *nk, int *ni);
+int lattice_count(qpad_ctx *c, const qpad_beam_std_cfg &cf, LatticePlan *pl, const LatticeTable *tab = nullptr);
+int lattice_write(qpad_ctx *c, const qpad_beam_std_cfg &cf, const LatticePlan &pl, Beam &bm,
+                  const LatticeTable *tab = nullptr);
 void lattice_free(LatticePlan *pl);
 int launch_beam_sort(qpad_ctx *c, int ib);
 int launch_beam_keys(qpad_ctx *c, int ib);

@@ -771,7 +771,37 @@ int qpad_beam_new_arrays(qpad_ctx *c, double qm, double mass, int32_t evol, int6
     return beam_finish(c, b, rc, beam_id);
 }
 
+// the lattice beam, its density given per box point (tab) or by its profiles
+static int beam_new_lattice(qpad_ctx *c, const qpad_beam_std_cfg *cfg, const double *den, int32_t k0, int32_t nk,
+                            int32_t ni, int32_t *beam_id);
+
 int qpad_beam_new_std(qpad_ctx *c, const qpad_beam_std_cfg *cfg, int32_t *beam_id)
+{
+    return beam_new_lattice(c, cfg, nullptr, 0, 0, 0, beam_id);
+}
+
+int qpad_beam_lattice_box(qpad_ctx *c, const qpad_beam_std_cfg *cfg, int32_t *k0, int32_t *nk, int32_t *ni)
+{
+    if (!c || !cfg || !k0 || !nk || !ni) return fail(QPAD_E_INVALID, "null argument");
+    if (cfg->ppc[0] < 1 || cfg->ppc[1] < 1 || cfg->ppc[2] < 1 || cfg->num_theta < 1)
+        return fail(QPAD_E_INVALID, "beam ppc and num_theta must be >= 1");
+    int a = 0, b = 0, d = 0;
+    TRY(lattice_range_box(c, *cfg, &a, &b, &d));
+    *k0 = a;
+    *nk = b;
+    *ni = d;
+    return 0;
+}
+
+int qpad_beam_new_std_density(qpad_ctx *c, const qpad_beam_std_cfg *cfg, const double *den, int32_t k0, int32_t nk,
+                              int32_t ni, int32_t *beam_id)
+{
+    if (!c || !cfg || !den) return fail(QPAD_E_INVALID, "null argument");
+    return beam_new_lattice(c, cfg, den, k0, nk, ni, beam_id);
+}
+
+static int beam_new_lattice(qpad_ctx *c, const qpad_beam_std_cfg *cfg, const double *den, int32_t k0, int32_t nk,
+                            int32_t ni, int32_t *beam_id)
 {
     if (!c || !cfg) return fail(QPAD_E_INVALID, "null argument");
     if (c->nbeams >= kMaxBeams) return fail(QPAD_E_INVALID, "too many beams");
@@ -780,7 +810,7 @@ int qpad_beam_new_std(qpad_ctx *c, const qpad_beam_std_cfg *cfg, int32_t *beam_i
         return fail(QPAD_E_INVALID, "beam ppc and num_theta must be >= 1");
     if (cfg->geometry != QPAD_GEOM_CARTESIAN && cfg->geometry != QPAD_GEOM_CYLINDRICAL)
         return fail(QPAD_E_INVALID, "beam geometry must be cartesian or cylindrical");
-    for (int d = 0; d < 3; d++) {
+    for (int d = 0; d < 3 && !den; d++) {
         const int pt = cfg->profile[d];
         if (pt != QPAD_PROF_UNIFORM && pt != QPAD_PROF_GAUSSIAN && pt != QPAD_PROF_PIECEWISE)
             return fail(QPAD_E_INVALID, "beam profile must be uniform, gaussian or piecewise-linear");
@@ -814,11 +844,34 @@ int qpad_beam_new_std(qpad_ctx *c, const qpad_beam_std_cfg *cfg, int32_t *beam_i
     bm.cfg.seed = cfg->seed;
     bm.qbm = cfg->qm / cfg->mass;
     LatticePlan pl;
-    int rc = lattice_count(c, *cfg, &pl);
+    memset((void *)&pl, 0, sizeof(pl));
+    LatticeTable tab{nullptr, k0, nk, ni};
+    double *dtab = nullptr;
+    int rc = 0;
+    if (den) {
+        // the table's box must hold every lattice point inside the ranges
+        int r0 = 0, rn = 0, ri = 0;
+        rc = lattice_range_box(c, *cfg, &r0, &rn, &ri);
+        if (!rc && (k0 < 1 || nk < 0 || ni < 0 || k0 + nk - 1 > c->g.nxi || ni > c->g.nr ||
+                    (rn > 0 && (k0 > r0 || k0 + nk < r0 + rn || ni < ri))))
+            rc = fail(QPAD_E_INVALID, "lattice density box: xi cells k0 .. k0+nk-1 within 1 .. nxi and r cells "
+                                      "1 .. ni <= nr, holding the box of the ranges (qpad_beam_lattice_box)");
+        const int64_t n = (int64_t)nk * cfg->num_theta * ni * cfg->ppc[0] * cfg->ppc[1] * cfg->ppc[2];
+        if (!rc && n > 0) {
+            rc = dalloc(&dtab, (size_t)n);
+            if (!rc && hipMemcpyAsync(dtab, den, (size_t)n * sizeof(double), hipMemcpyHostToDevice, c->stream) !=
+                           hipSuccess)
+                rc = fail(QPAD_E_HIP, "copy of the lattice density failed");
+        }
+        tab.den = dtab;
+    }
+    if (!rc) rc = lattice_count(c, *cfg, &pl, den ? &tab : nullptr);
     if (!rc && pl.kept > 0x3fffffff) rc = fail(QPAD_E_INVALID, "lattice beam: too many particles");
     if (!rc) rc = beam_alloc(c, bm, pl.kept * (cfg->quiet ? 2 : 1));
-    if (!rc) rc = lattice_write(c, *cfg, pl, bm);
+    if (!rc) rc = lattice_write(c, *cfg, pl, bm, den ? &tab : nullptr);
     if (!rc) rc = sync(c);
+    (void)hipStreamSynchronize(c->stream);
+    dfree(dtab);
     lattice_free(&pl);
     return beam_finish(c, b, rc, beam_id);
 }

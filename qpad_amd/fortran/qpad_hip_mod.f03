@@ -212,6 +212,26 @@ module qpad_hip_mod
       integer(c_int) :: rc
     end function
 
+    ! the lattice beam with a density the caller evaluates over the box of its ranges
+    function qpad_beam_lattice_box(ctx, cfg, k0, nk, ni) bind(C, name="qpad_beam_lattice_box") result(rc)
+      import :: c_int, c_int32_t, c_ptr, qpad_beam_std_cfg
+      type(c_ptr), value :: ctx
+      type(qpad_beam_std_cfg), intent(in) :: cfg
+      integer(c_int32_t), intent(out) :: k0, nk, ni
+      integer(c_int) :: rc
+    end function
+
+    function qpad_beam_new_std_density(ctx, cfg, den, k0, nk, ni, beam_id) &
+        bind(C, name="qpad_beam_new_std_density") result(rc)
+      import :: c_int, c_int32_t, c_ptr, c_double, qpad_beam_std_cfg
+      type(c_ptr), value :: ctx
+      type(qpad_beam_std_cfg), intent(in) :: cfg
+      real(c_double), intent(in) :: den(*)
+      integer(c_int32_t), value :: k0, nk, ni
+      integer(c_int32_t), intent(out) :: beam_id
+      integer(c_int) :: rc
+    end function
+
     ! a non-uniform plasma: the transverse density of every lattice point and
     ! the longitudinal factor of the next injections
     function qpad_species_set_profile(ctx, sp, den_perp, den_lon) bind(C, name="qpad_species_set_profile") &
@@ -737,7 +757,8 @@ module qpad_hip_mod
   public :: qpad_last_error, qpad_abi_version, qpad_create, qpad_destroy, qpad_get_grid, &
     qpad_species_new, qpad_species_renew, qpad_species_qdp, qpad_species_qdp_noalloc, &
     qpad_species_charge_history, qpad_species_cbq, qpad_get_species_charge, qpad_species_amjdp, &
-    qpad_species_push, qpad_species_push_clamped, qpad_species_set_profile, qpad_set_subcycling, qpad_exp_fac_max, qpad_last_subcycles, &
+    qpad_species_push, qpad_species_push_clamped, qpad_species_set_profile, qpad_beam_lattice_box, &
+    qpad_beam_new_std_density, qpad_set_subcycling, qpad_exp_fac_max, qpad_last_subcycles, &
     qpad_species_interp_psi, qpad_step_iterations, qpad_field_solve_psi, qpad_field_solve_ez, &
     qpad_set_laser, qpad_laser_slice, qpad_deposit_chi, &
     qpad_field_solve_et, qpad_field_solve_bz, qpad_field_solve_bt_beam, &

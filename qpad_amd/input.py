@@ -247,10 +247,11 @@ def _lattice_beam(bm, z0, seed):
         raise QpadError(f"Invalid beam geometry '{geom}'")
     c.geometry = GEOM_CARTESIAN if geom == "cartesian" else GEOM_CYLINDRICAL
     prof = list(_get(bm, "profile"))
+    from .profiles import BeamProfile
+    tab = BeamProfile(bm, z0)     # checks every direction's profile and its parameters
     for d, name in enumerate(prof):
         if name not in _PROFILES:
-            raise QpadError(f"beam density profile '{name}' is not on the GPU path "
-                            "(uniform, gaussian and piecewise-linear are)")
+            continue              # evaluated on the host (BeamProfile)
         c.profile[d] = _PROFILES[name]
         shift = z0 if d == 2 else 0.0
         if name == "gaussian":
@@ -303,7 +304,16 @@ def _lattice_beam(bm, z0, seed):
             for k, x in enumerate(v[1:]):
                 c.offset_c[d][k] = x
     c.seed = int(seed)
-    return c
+    return TabulatedBeam(c, tab) if tab.tabulated else c
+
+
+class TabulatedBeam:
+    """A lattice beam whose density the host evaluates (BeamProfile): cfg
+    holds the lattice settings, profile the density"""
+
+    def __init__(self, cfg, profile):
+        self.cfg = cfg
+        self.profile = profile
 
 
 def tri_gaussian_charge(n_peak, sigma):

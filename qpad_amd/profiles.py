@@ -332,3 +332,130 @@ def plasma_lattice(nr, dr, ppc1, ppc2, num_theta):
     rn = (i1 - 0.5) / ppc1 + (i - 1.0)
     theta = ((i2 - 0.5) / ppc2 + j - 1.0) * (2.0 * math.pi / num_theta)
     return (rn * dr * np.cos(theta)).ravel(), (rn * dr * np.sin(theta)).ravel()
+
+
+# ---------------------------------------------------------------------------
+# lattice beam profiles (fdist3d_std_lib.f03) the library does not evaluate
+# itself: per direction super-gaussian, parabolic, rational (besides uniform,
+# gaussian, piecewise-linear), or one analytic math_func of (x, y, z) with z
+# absolute (the box start added back)
+# ---------------------------------------------------------------------------
+class BeamProfile:
+    """beam(i).profile of a lattice ("standard") beam, evaluated on the host
+    (fdist3d_std_class.f03:160-300, fdist3d_std_lib.f03:20-363); z0 is the
+    box start, which the deck's longitudinal positions are relative to."""
+
+    TABULATED = ("super-gaussian", "parabolic", "rational", "analytic")
+
+    def __init__(self, bm, z0):
+        prof = list(_deck_get(bm, "profile"))
+        self.kinds = prof
+        self.z0 = float(z0)
+        self.func = None
+        self.pars = []
+        if "analytic" in prof:
+            self.func = FParser(_deck_get(bm, "math_func"))
+        for d, name in enumerate(prof):
+            shift = self.z0 if d == 2 else 0.0
+            dim = str(d + 1)
+            if name == "gaussian":
+                self.pars.append((float(bm["gauss_center"][d]) - shift, float(bm["gauss_sigma"][d])))
+            elif name == "super-gaussian":
+                order = float(bm["super_gauss_order"][d])
+                if order < 1.0:
+                    raise QpadError("The order of super-Gaussian profile must be >= 1.")
+                self.pars.append((float(bm["super_gauss_center"][d]) - shift, float(bm["super_gauss_sigma"][d]),
+                                  order))
+            elif name == "parabolic":
+                self.pars.append((float(bm["parabolic_center"][d]) - shift, float(bm["parabolic_radius"][d])))
+            elif name == "rational":
+                self.pars.append((float(bm["poly_x0"][d]) - shift,
+                                  np.asarray(_deck_get(bm, "poly_numerator" + dim), dtype=float),
+                                  np.asarray(_deck_get(bm, "poly_denominator" + dim), dtype=float)))
+            elif name == "piecewise-linear":
+                xs = np.asarray(_deck_get(bm, "piecewise_x" + dim), dtype=float) - shift
+                if np.any(np.diff(xs) <= 0):
+                    raise QpadError("Position array must be monotonically increasing!")
+                self.pars.append((xs, np.asarray(_deck_get(bm, "piecewise_fx" + dim), dtype=float)))
+            elif name in ("uniform", "analytic"):
+                self.pars.append(())
+            else:
+                raise QpadError(f"Invalid beam density profile '{name}'")
+
+    @property
+    def tabulated(self):
+        return any(k in self.TABULATED for k in self.kinds)
+
+    def _dim(self, d, x):
+        kind, p = self.kinds[d], self.pars[d]
+        if kind == "uniform":
+            return np.ones_like(x)
+        if kind == "gaussian":
+            return np.exp(-0.5 * ((x - p[0]) / p[1]) ** 2)
+        if kind == "super-gaussian":
+            return np.exp(-(0.5 * ((x - p[0]) / p[1]) ** 2) ** p[2])
+        if kind == "parabolic":
+            t = (x - p[0]) ** 2 / (p[1] * p[1])
+            return np.where(t < 1.0, 1.0 - t, 0.0)
+        if kind == "rational":
+            x0, pn, pd = p
+            num = np.zeros_like(x)
+            den = np.zeros_like(x)
+            for i, c in enumerate(pn):
+                num = num + c * (x - x0) ** i
+            for i, c in enumerate(pd):
+                den = den + c * (x - x0) ** i
+            return num / den
+        xs, fx = p
+        out = np.zeros_like(x)
+        inside = (x >= xs[0]) & (x <= xs[-1])
+        i = np.clip(np.searchsorted(xs, x, side="left"), 1, len(xs) - 1)
+        lin = fx[i - 1] + (fx[i] - fx[i - 1]) / (xs[i] - xs[i - 1]) * (x - xs[i - 1])
+        return np.where(inside, lin, out)
+
+    def density(self, x1, x2, x3):
+        """the density at lattice positions in the beam's geometry
+        coordinates (x3 box-relative), without the beam's density factor"""
+        with np.errstate(all="ignore"):
+            if self.func is not None:
+                return np.asarray(self.func(x1, x2, x3 + self.z0), dtype=float) * np.ones_like(x1)
+            return self._dim(0, x1) * self._dim(1, x2) * self._dim(2, x3)
+
+
+def beam_lattice_box(nr, nxi, dr, dxi, cfg):
+    """the lattice box that holds every point inside the beam's ranges (the
+    library's own choice, qpad_beam_lattice_box): first xi cell k0 (1-based),
+    xi cells nk, r cells ni"""
+    if cfg.geometry == 1:
+        rmax = cfg.range_hi[0]
+    else:
+        mx = max(abs(cfg.range_lo[0]), abs(cfg.range_hi[0]))
+        my = max(abs(cfg.range_lo[1]), abs(cfg.range_hi[1]))
+        rmax = math.sqrt(mx * mx + my * my)
+    ni = int(min(float(nr), max(math.floor(rmax / dr) + 2.0, 0.0)))
+    k0 = math.floor(cfg.range_lo[2] / dxi) - 1.0
+    k1 = math.floor(cfg.range_hi[2] / dxi) + 2.0
+    kk0 = int(max(1.0, min(k0, nxi + 1.0)))
+    kk1 = int(max(0.0, min(k1, float(nxi))))
+    nk = kk1 - kk0 + 1 if kk1 >= kk0 else 0
+    if ni <= 0:
+        nk = 0
+    return kk0, nk, ni
+
+
+def beam_lattice_density(profile, cfg, nr, dr, dxi, box):
+    """the density of every point of the lattice box in inject_fdist3d_std's
+    order (k > j > i > i3 > i2 > i1, fdist3d_std_class.f03:439-490)"""
+    k0, nk, ni = box
+    p1, p2, p3, nth = cfg.ppc[0], cfg.ppc[1], cfg.ppc[2], cfg.num_theta
+    k, j, i, i3, i2, i1 = np.meshgrid(np.arange(k0, k0 + nk), np.arange(1, nth + 1), np.arange(1, ni + 1),
+                                      np.arange(1, p3 + 1), np.arange(1, p2 + 1), np.arange(1, p1 + 1),
+                                      indexing="ij")
+    zn = (i3 - 0.5) / p3 + (k - 1.0)
+    theta = ((i2 - 0.5) / p2 + j - 1.0) * (2.0 * math.pi / nth)
+    rn = (i1 - 0.5) / p1 + (i - 1.0)
+    if cfg.geometry == 1:
+        xa, xb = rn * dr, theta
+    else:
+        xa, xb = rn * dr * np.cos(theta), rn * dr * np.sin(theta)
+    return np.ascontiguousarray(profile.density(xa, xb, zn * dxi).ravel(), dtype=np.float64)

@@ -11,9 +11,8 @@ import ctypes as C
 import numpy as np
 
 from . import _lib as L
-from .input import (FileBeam, beam_from_deck, grid_from_deck, neutral_from_deck, place_file_beam, profile_from_deck,
-                    subcycling_from_deck,
-                    species_from_deck)
+from .input import (FileBeam, TabulatedBeam, beam_from_deck, grid_from_deck, neutral_from_deck, place_file_beam,
+                    profile_from_deck, species_from_deck, subcycling_from_deck)
 
 
 class Field:
@@ -247,6 +246,16 @@ class Beam3D:
             L.check(sim.lib.qpad_beam_new_arrays(sim.ctx, cfg.qm, cfg.mass, int(cfg.evol), len(a["x1"]),
                                                  *[L.dptr(a[k]) for k in ("x1", "x2", "x3", "p1", "p2", "p3",
                                                                           "q")], C.byref(bid)))
+        elif isinstance(cfg, TabulatedBeam):
+            # a lattice beam whose profile the host evaluates over the box of its ranges
+            from .profiles import beam_lattice_density
+            k0, nk, ni = C.c_int32(0), C.c_int32(0), C.c_int32(0)
+            L.check(sim.lib.qpad_beam_lattice_box(sim.ctx, C.byref(cfg.cfg), C.byref(k0), C.byref(nk), C.byref(ni)))
+            box = (k0.value, nk.value, ni.value)
+            g = sim.grid
+            den = beam_lattice_density(cfg.profile, cfg.cfg, g.nr, g.dr, g.dxi, box)
+            L.check(sim.lib.qpad_beam_new_std_density(sim.ctx, C.byref(cfg.cfg), L.dptr(den), *box, C.byref(bid)))
+            cfg = cfg.cfg
         else:
             new = sim.lib.qpad_beam_new_std if isinstance(cfg, L.BeamStdCfg) else sim.lib.qpad_beam_new
             L.check(new(sim.ctx, C.byref(cfg), C.byref(bid)))

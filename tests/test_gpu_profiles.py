@@ -86,3 +86,74 @@ def test_analytic_profile_deck():
     assert np.allclose(q, want, rtol=1e-12, atol=0)
     sim.close()
     o.close()
+
+
+def _shaped_beam_deck():
+    d = configs.tri_gaussian()
+    d["simulation"]["grid"] = [128, 256]
+    d["beam"][0].update({"profile": ["super-gaussian", "parabolic", "rational"],
+                         "super_gauss_center": [0.0, 0.0, 0.0], "super_gauss_sigma": [0.3, 1.0, 1.0],
+                         "super_gauss_order": [2.0, 1.0, 1.0], "parabolic_center": [0.0, 0.05, 0.0],
+                         "parabolic_radius": [1.0, 0.6, 1.0], "poly_x0": [0.0, 0.0, -2.5],
+                         "poly_numerator3": [1.0], "poly_denominator3": [1.0, 0.0, 4.0]})
+    return d
+
+
+@pytest.mark.parametrize("which", ["shaped", "analytic"])
+def test_tabulated_beam_matches_oracle(which):
+    """lattice beams whose profiles the host evaluates (super-gaussian x
+    parabolic x rational; an analytic math_func): the library's lattice box
+    is the host's, the injected beam is the oracle's, and one 3D step
+    matches slice by slice"""
+    from qpad_amd.input import beam_from_deck, grid_from_deck
+    from qpad_amd.profiles import beam_lattice_box
+    import ctypes as C
+    if which == "shaped":
+        deck = _shaped_beam_deck()
+    else:
+        deck = _shaped_beam_deck()
+        deck["beam"][0].update({"profile": ["analytic"] * 3,
+                                "math_func": "exp(-z^2/0.5)*exp(-(x^2+y^2)/0.18)*(1+0.5*step(x))"})
+    print("tabulated beam:", which, flush=True)
+    sim = Simulation(deck)
+    o = twin.oracle_from_deck(deck)
+    g, _, z0 = grid_from_deck(deck)
+    tb = beam_from_deck(deck["beam"][0], z0, seed=10 + 7)
+    k0, nk, ni = C.c_int32(0), C.c_int32(0), C.c_int32(0)
+    assert sim.lib.qpad_beam_lattice_box(sim.ctx, C.byref(tb.cfg), C.byref(k0), C.byref(nk), C.byref(ni)) == 0
+    assert (k0.value, nk.value, ni.value) == beam_lattice_box(g.nr, g.nxi, g.dr, g.dxi, tb.cfg)
+    gb, ob = sim.beam[0].get(), o.beam(0)
+    assert len(gb["x1"]) == len(ob["x1"]) > 1000
+    for k in ("x1", "x2", "x3", "p1", "p2", "p3", "q"):
+        assert rel_err(np.sort(gb[k]), np.sort(ob[k])) <= 1e-14, k
+    nxi = sim.grid.nxi
+    sim.step3d_begin()
+    o.step_begin()
+    assert [sim.slice(j) for j in range(1, nxi + 1)] == [o.slice(j) for j in range(1, nxi + 1)]
+    for name in ("psi", "e", "b", "q_spe"):
+        gh = np.stack([sim.fields[name].hist(j) for j in range(1, nxi + 1)])
+        assert rel_err(gh, o.hist(twin.ORA_ID[name])[1:nxi + 1]) <= TOL, name
+    sim.close()
+    o.close()
+
+
+def test_analytic_example_deck_runs():
+    """QPAD's analytic-profile example deck at its size (Nr 1024, Nxi 1024,
+    an analytic lattice beam of about 4 million points with quiet start, an
+    analytic plasma): one 3D step; the beam keeps its charge, the plasma of
+    the second step follows the expression"""
+    deck = configs.analytic_example()
+    sim = Simulation(deck)
+    nb = sim.beam[0].count()
+    assert nb > 4_000_000 and sim.spe[0].count() == 0
+    its = sim.step3d()
+    n = sim.spe[0].count()
+    b = sim.fields["b"].hist(146)              # the beam's centre (absolute z = 0)
+    print("beam particles", nb, "iterations", its, "plasma particles after the renew", n)
+    assert n > 0 and np.all(np.isfinite(b)) and np.max(np.abs(b)) > 0
+    q = sim.spe[0].get()
+    z = sim.grid.dt
+    r2 = q["x1"] ** 2 + q["x2"] ** 2
+    want = -np.sqrt(r2) / sim.grid.dr * z ** 2 / (z + 10.0) ** 2 * np.exp(-r2 / 50.0) / 32.0
+    assert np.allclose(q["q"], want, rtol=1e-12, atol=0)
+    sim.close()

@@ -51,7 +51,9 @@ def test_total_charge_definition():
     (lambda d: d["simulation"].update(nlasers=1), "nlasers"),
     (lambda d: d["simulation"].update(field_boundary="periodic"), "boundary"),
     (lambda d: d["beam"][0].update(profile_type="gaussian"), "Invalid beam profile type"),
-    (lambda d: d["beam"][0].update(profile_type="standard", profile=["analytic"] * 3), "analytic"),
+    (lambda d: d["beam"][0].update(profile_type="standard", profile=["analytic"] * 3), "math_func"),
+    (lambda d: d["beam"][0].update(profile_type="standard", profile=["gaussian", "cosine", "uniform"]),
+     "Invalid beam density profile"),
     (lambda d: d["beam"][0].update(push_type="boris"), "reduced"),
 ])
 def test_unsupported_features_raise(edit, match):
@@ -107,10 +109,22 @@ def test_reference_decks_load_or_are_refused_clearly():
         except QpadError as e:
             refused[os.path.relpath(path, DECKS)] = str(e)
     assert {"blowout_regime/qpinput_tri-gaussian.json", "blowout_regime/qpinput_sawtooth.json",
-            "hosing/qpinput.json", "ionization/qpinput.json"} <= set(ok), (ok, refused)
+            "hosing/qpinput.json", "ionization/qpinput.json",
+            "analytic_density_profiles/qpinput.json"} <= set(ok), (ok, refused)
     assert refused, "decks with lasers, neutrals or analytic profiles must be refused"
     for name, msg in refused.items():
         assert any(w in msg for w in ("nlasers", "nneutrals", "analytic", "pusher", "profile")), (name, msg)
+
+
+@pytest.mark.skipif(not os.path.isdir(DECKS), reason="QPAD's example decks are not in this tree")
+def test_analytic_config_restates_the_deck():
+    real = load_deck(os.path.join(DECKS, "analytic_density_profiles", "qpinput.json"))
+    mine = configs.analytic_example()
+    for sec in ("beam", "species"):
+        for a, b in zip(real[sec], mine[sec]):
+            assert {k: v for k, v in a.items() if k != "diag"} == b
+    for k, v in mine["simulation"].items():
+        assert real["simulation"][k] == v, k
 
 
 @pytest.mark.skipif(not os.path.isdir(DECKS), reason="QPAD's example decks are not in this tree")

@@ -144,3 +144,54 @@ def test_oracle_profiled_injection():
     inner = (r > 0.9) & (r < 3.0)        # every particle near these nodes kept
     assert np.max(np.abs(-q[0, 0, inner] - n[inner])) <= 2.0 * dr ** 2 / 9.0 + 1e-12
     o.close()
+
+
+def test_beam_profiles():
+    """get_den_super_gauss / parabolic / rational / analytic
+    (fdist3d_std_lib.f03:46-286): closed forms, z of the analytic function
+    absolute (the box start added back)"""
+    from qpad_amd.profiles import BeamProfile
+    bm = {"profile": ["super-gaussian", "parabolic", "rational"], "super_gauss_center": [0.5, 0, 0],
+          "super_gauss_sigma": [2.0, 1, 1], "super_gauss_order": [2.0, 1, 1], "parabolic_center": [0, 1.0, 0],
+          "parabolic_radius": [1, 2.0, 1], "poly_x0": [0, 0, 3.0], "poly_numerator3": [1.0, 2.0],
+          "poly_denominator3": [2.0, 0.0, 1.0]}
+    p = BeamProfile(bm, z0=-2.0)
+    x = np.array([0.5, 2.5, -1.5])
+    y = np.array([1.0, 2.0, 3.5])
+    z = np.array([5.0, 6.0, 4.0])            # box-relative: absolute z = z - 2
+    zr = z - 5.0                              # x - x0 with x0 = 3 - z0
+    want = (np.exp(-(0.5 * ((x - 0.5) / 2.0) ** 2) ** 2) * np.where((y - 1.0) ** 2 / 4.0 < 1, 1 - (y - 1.0) ** 2 / 4.0, 0)
+            * (1.0 + 2.0 * zr) / (2.0 + zr ** 2))
+    assert np.allclose(p.density(x, y, z), want, rtol=1e-14, atol=0) and p.tabulated
+    a = BeamProfile({"profile": ["analytic"] * 3, "math_func": "x + 10*y + 100*z"}, z0=-2.0)
+    assert a.density(np.array([1.0]), np.array([2.0]), np.array([3.0]))[0] == pytest.approx(1 + 20 + 100 * 1.0)
+    with pytest.raises(QpadError, match="super-Gaussian"):
+        BeamProfile(dict(bm, super_gauss_order=[0.5, 1, 1]), z0=0.0)
+
+
+def test_oracle_tabulated_beam_equals_profile_beam():
+    """a lattice beam given by its density table (the host's Gaussian) is the
+    beam the oracle's own Gaussian profile injects: same particles, charges
+    to rounding"""
+    from qpad_amd import configs
+    from qpad_amd.input import _lattice_beam, grid_from_deck
+    from qpad_amd.profiles import BeamProfile, beam_lattice_box, beam_lattice_density
+    import twin
+    deck = configs.tri_gaussian()
+    g, _, z0 = grid_from_deck(deck)
+    bm = deck["beam"][0]
+    cfg = _lattice_beam(bm, z0, seed=17)
+    prof = BeamProfile(bm, z0)
+    box = beam_lattice_box(g.nr, g.nxi, g.dr, g.dxi, cfg)
+    den = beam_lattice_density(prof, cfg, g.nr, g.dr, g.dxi, box)
+    og = twin._as_oracle_struct(po.Grid, g)
+    a, b = po.Oracle(og), po.Oracle(og)
+    a.add_beam_std(twin._as_oracle_struct(po.BeamStdCfg, cfg))
+    b.add_beam_std_table(twin._as_oracle_struct(po.BeamStdCfg, cfg), den, *box)
+    ba, bb = a.beam(0), b.beam(0)
+    assert len(ba["x1"]) == len(bb["x1"]) > 1000
+    for k in ("x1", "x2", "x3", "p1", "p2", "p3"):
+        assert np.array_equal(ba[k], bb[k]), k
+    assert np.allclose(ba["q"], bb["q"], rtol=1e-14, atol=0)
+    a.close()
+    b.close()

@@ -91,6 +91,11 @@ def oracle_from_deck(deck, L=None):
             o.set_beam(ib, a["x1"], a["x2"], a["x3"], a["p1"], a["p2"], a["p3"], a["q"])
         elif type(cfg).__name__ == "BeamStdCfg":
             o.add_beam_std(_as_oracle_struct(po.BeamStdCfg, cfg))
+        elif type(cfg).__name__ == "TabulatedBeam":
+            from qpad_amd.profiles import beam_lattice_box, beam_lattice_density
+            box = beam_lattice_box(g.nr, g.nxi, g.dr, g.dxi, cfg.cfg)
+            den = beam_lattice_density(cfg.profile, cfg.cfg, g.nr, g.dr, g.dxi, box)
+            o.add_beam_std_table(_as_oracle_struct(po.BeamStdCfg, cfg.cfg), den, *box)
         else:
             o.add_beam(_as_oracle_struct(po.BeamCfg, cfg))
     return o
