@@ -105,6 +105,7 @@ def neutral_from_deck(nt, n0):
     prof = _get(nt, "profile")
     if list(prof) != ["uniform", "uniform"]:
         raise QpadError("only uniform neutral profiles are on the GPU path")
+    _no_smoothing(nt, "neutrals")
     push = _get(nt, "push_type")
     if push not in ("standard", "robust"):
         raise QpadError('Invalid pusher type! Only "std", "robust" are supported currently.')
@@ -132,10 +133,19 @@ def profile_from_deck(sp):
     return None if prof.uniform else prof
 
 
+def _no_smoothing(sec, what):
+    # species(i) / neutrals(i).smooth_order (sim_plasma_class.f03:140-141,
+    # 169-170): the binomial smoothing of the species' deposits
+    # (field_src_class.f03:102-265) is not on this path
+    if int(_get(sec, "smooth_order", 0, required=False)) > 0:
+        raise QpadError(f"{what}.smooth_order > 0: the smoothing of the deposits is not on the GPU path")
+
+
 def species_from_deck(sp, seed=0):
     """fdist2d_class.f03:117-272, sim_plasma_class.f03:116-157 (the density
     profile: profile_from_deck)."""
     profile_from_deck(sp)   # checks the profile section
+    _no_smoothing(sp, "species")
     push = _get(sp, "push_type")
     if push not in _PUSHERS:
         raise QpadError('Invalid pusher type! Only "standard", "robust", "standard_pgc" and "robust_pgc" '

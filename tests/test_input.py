@@ -49,6 +49,7 @@ def test_total_charge_definition():
     (lambda d: d["species"][0].update(profile=["gaussian-channel", "uniform"]), "Invalid density profile"),
     (lambda d: d["species"][0].update(profile=["parabolic-channel", "uniform"]), "channel_n0"),
     (lambda d: d["simulation"].update(nlasers=1), "nlasers"),
+    (lambda d: d["species"][0].update(smooth_order=2), "smooth_order"),
     (lambda d: d["simulation"].update(field_boundary="periodic"), "boundary"),
     (lambda d: d["beam"][0].update(profile_type="gaussian"), "Invalid beam profile type"),
     (lambda d: d["beam"][0].update(profile_type="standard", profile=["analytic"] * 3), "math_func"),
