@@ -1446,8 +1446,13 @@ int qpad_species_set_profile(qpad_ctx *c, int32_t s, const double *den_perp, dou
     if (!den_perp) {
         // uniform again: every lattice point, no longitudinal factor
         (void)hipStreamSynchronize(c->stream);
-        dfree(sp.den_perp);
+        dfree(sp.den_perp); dfree(sp.inj_keep); dfree(sp.inj_off);
+        if (sp.inj_tmp) (void)hipFree(sp.inj_tmp);
+        if (sp.inj_host) (void)hipHostFree(sp.inj_host);
         sp.den_perp = nullptr;
+        sp.inj_keep = sp.inj_off = nullptr;
+        sp.inj_tmp = nullptr;
+        sp.inj_host = nullptr;
         sp.den_lon = 1.0;
         return 0;
     }
@@ -1466,8 +1471,9 @@ int qpad_species_set_profile(qpad_ctx *c, int32_t s, const double *den_perp, dou
         if (rc) {
             dfree(sp.den_perp); dfree(sp.inj_keep); dfree(sp.inj_off);
             if (sp.inj_tmp) (void)hipFree(sp.inj_tmp);
-            sp.inj_tmp = nullptr;
             sp.den_perp = nullptr;
+            sp.inj_keep = sp.inj_off = nullptr;
+            sp.inj_tmp = nullptr;
             return rc;
         }
     }

@@ -157,3 +157,27 @@ def test_analytic_example_deck_runs():
     want = -np.sqrt(r2) / sim.grid.dr * z ** 2 / (z + 10.0) ** 2 * np.exp(-r2 / 50.0) / 32.0
     assert np.allclose(q["q"], want, rtol=1e-12, atol=0)
     sim.close()
+
+
+def test_profile_on_off_on():
+    """a species returned to the uniform plasma and profiled again: every
+    injection has the lattice the current setting gives"""
+    from qpad_amd.profiles import PlasmaProfile
+    deck = configs.profiled()
+    sim = Simulation(deck)
+    sp = sim.spe[0]
+    prof = sp.profile
+    n_prof = sp.count()
+    sp.set_profile(None, 0.0)
+    sp.renew()
+    assert sp.count() == sim.grid.nr * 2 * 4 and n_prof < sp.count()
+    hollow = PlasmaProfile({"profile": ["hollow-channel", "uniform"], "channel_rmin": 1.0, "channel_rmax": 2.0,
+                            "channel_depth": 1.0})
+    sp.set_profile(hollow, 0.0)
+    sp.renew()
+    r = np.hypot(sp.get()["x1"], sp.get()["x2"])
+    assert sp.count() > 0 and np.all((r > 1.0) & (r < 2.0))
+    sp.set_profile(prof, 0.0)
+    sp.renew()
+    assert sp.count() == n_prof
+    sim.close()
