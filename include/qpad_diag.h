@@ -70,6 +70,18 @@ int qpad_diag_species_raw(qpad_ctx *ctx, int32_t sp, int32_t psample, const char
 int qpad_diag_beam_raw(qpad_ctx *ctx, int32_t ib, int32_t psample, const char *dir, int32_t iter,
                        const qpad_diag_box *box);
 
+/* neutrals(nt+1) diagnostics (diagnostics_class.f03:324-390, 909-928) of
+   the neutral whose species id is sp: charge_cyl_m (its electrons' charge
+   history, qpad_species_charge_history on) into <dir>/Neutral<nt+1>/Charge/,
+   ion_cyl_m (rho_ion, neutral%wr_ion) into <dir>/Neutral<nt+1>/Ion_charge/
+   (dataname ion_charge), raw (its electrons) into <dir>/Neutral<nt+1>/Raw/ */
+int qpad_diag_neutral_charge(qpad_ctx *ctx, int32_t sp, int32_t nt, const char *dir, int32_t iter,
+                             const qpad_diag_box *box);
+int qpad_diag_neutral_ion(qpad_ctx *ctx, int32_t sp, int32_t nt, const char *dir, int32_t iter,
+                          const qpad_diag_box *box);
+int qpad_diag_neutral_raw(qpad_ctx *ctx, int32_t sp, int32_t nt, int32_t psample, const char *dir, int32_t iter,
+                          const qpad_diag_box *box);
+
 /* the particle file of a "file" beam (fdist3d_file_class.f03:154-343):
    datasets x1, x2, x3, p1, p2, p3, q of the root group, each holding n
    values (rank 1, or rank 2 with one extent 1).  With x1 null only n is

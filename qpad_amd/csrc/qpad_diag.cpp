@@ -307,14 +307,39 @@ int qpad_diag_field(qpad_ctx *c, const char *name, const char *dir, int32_t iter
                             fd->units, fd->label, iter, box, sh.nr, sh.nxi, sh.nm);
 }
 
-int qpad_diag_species_charge(qpad_ctx *c, int32_t sp, const char *dir, int32_t iter, const qpad_diag_box *box)
+// the charge history of species sp under <dir>/<folder>/Charge/
+static int species_charge_files(qpad_ctx *c, int32_t sp, const std::string &folder, const char *dir, int32_t iter,
+                                const qpad_diag_box *box)
 {
-    if (!c || sp < 0 || !valid_box(box)) return fail(QPAD_E_INVALID, "bad species-charge arguments");
     Shape sh;
     if (grid_of(c, &sh)) return QPAD_E_INVALID;
     auto get = [&](int s, double *buf) { return qpad_get_species_charge(c, sp, s, buf); };
-    return write_mode_files(c, get, 1, 0, slash(dir) + "Species" + std::to_string(sp + 1) + "/Charge/", "charge",
-                            "n_0", "\\rho", iter, box, sh.nr, sh.nxi, sh.nm);
+    return write_mode_files(c, get, 1, 0, slash(dir) + folder + "/Charge/", "charge", "n_0", "\\rho", iter, box,
+                            sh.nr, sh.nxi, sh.nm);
+}
+
+int qpad_diag_species_charge(qpad_ctx *c, int32_t sp, const char *dir, int32_t iter, const qpad_diag_box *box)
+{
+    if (!c || sp < 0 || !valid_box(box)) return fail(QPAD_E_INVALID, "bad species-charge arguments");
+    return species_charge_files(c, sp, "Species" + std::to_string(sp + 1), dir, iter, box);
+}
+
+int qpad_diag_neutral_charge(qpad_ctx *c, int32_t sp, int32_t nt, const char *dir, int32_t iter,
+                             const qpad_diag_box *box)
+{
+    if (!c || sp < 0 || nt < 0 || !valid_box(box)) return fail(QPAD_E_INVALID, "bad neutral-charge arguments");
+    return species_charge_files(c, sp, "Neutral" + std::to_string(nt + 1), dir, iter, box);
+}
+
+int qpad_diag_neutral_ion(qpad_ctx *c, int32_t sp, int32_t nt, const char *dir, int32_t iter,
+                          const qpad_diag_box *box)
+{
+    if (!c || sp < 0 || nt < 0 || !valid_box(box)) return fail(QPAD_E_INVALID, "bad neutral-ion arguments");
+    Shape sh;
+    if (grid_of(c, &sh)) return QPAD_E_INVALID;
+    auto get = [&](int s, double *buf) { return qpad_get_ion_density(c, sp, s, buf); };
+    return write_mode_files(c, get, 1, 0, slash(dir) + "Neutral" + std::to_string(nt + 1) + "/Ion_charge/",
+                            "ion_charge", "n_0", "\\rho", iter, box, sh.nr, sh.nxi, sh.nm);
 }
 
 int qpad_diag_beam_charge(qpad_ctx *c, int32_t ib, const char *dir, int32_t iter, const qpad_diag_box *box)
@@ -333,10 +358,28 @@ int qpad_diag_beam_charge(qpad_ctx *c, int32_t ib, const char *dir, int32_t iter
                             "n_0", "\\rho", iter, box, sh.nr, sh.nxi, sh.nm);
 }
 
+// every psample-th particle of species sp into <dir>/<folder>/Raw/
+static int species_raw_file(qpad_ctx *c, int32_t sp, int32_t psample, const std::string &folder, const char *label,
+                            const char *dir, int32_t iter, const qpad_diag_box *box);
+
 int qpad_diag_species_raw(qpad_ctx *c, int32_t sp, int32_t psample, const char *dir, int32_t iter,
                           const qpad_diag_box *box)
 {
     if (!c || sp < 0 || psample < 1 || !valid_box(box)) return fail(QPAD_E_INVALID, "bad species-raw arguments");
+    return species_raw_file(c, sp, psample, "Species" + std::to_string(sp + 1), "Species Raw", dir, iter, box);
+}
+
+int qpad_diag_neutral_raw(qpad_ctx *c, int32_t sp, int32_t nt, int32_t psample, const char *dir, int32_t iter,
+                          const qpad_diag_box *box)
+{
+    if (!c || sp < 0 || nt < 0 || psample < 1 || !valid_box(box))
+        return fail(QPAD_E_INVALID, "bad neutral-raw arguments");
+    return species_raw_file(c, sp, psample, "Neutral" + std::to_string(nt + 1), "Neutral Raw", dir, iter, box);
+}
+
+static int species_raw_file(qpad_ctx *c, int32_t sp, int32_t psample, const std::string &folder, const char *label,
+                            const char *dir, int32_t iter, const qpad_diag_box *box)
+{
     int64_t n = 0;
     if (qpad_species_count(c, sp, &n)) return fail(QPAD_E_INVALID, std::string("species: ") + qpad_last_error());
     std::vector<double> a[8];
@@ -348,7 +391,7 @@ int qpad_diag_species_raw(qpad_ctx *c, int32_t sp, int32_t psample, const char *
     // wpart (hdf5io_class.f03:1584-1700): tp = npp / dspl, rows of the
     // Fortran (2, tp) / (3, tp) arrays are particles
     const int64_t tp = n / psample;
-    const std::string path = iter_file(slash(dir) + "Species" + std::to_string(sp + 1) + "/Raw/", "raw", iter);
+    const std::string path = iter_file(slash(dir) + folder + "/Raw/", "raw", iter);
     if (make_dirs(path)) return QPAD_E_INVALID;
     const hid_t file = H5Fcreate(path.c_str(), H5F_ACC_TRUNC, H5P_DEFAULT, H5P_DEFAULT);
     if (file < 0) return fail(QPAD_E_INVALID, "cannot create " + path);
@@ -373,9 +416,9 @@ int qpad_diag_species_raw(qpad_ctx *c, int32_t sp, int32_t psample, const char *
             q[t] = a[5][i];
         }
         const hsize_t dx[2] = {(hsize_t)tp, 2}, dp[2] = {(hsize_t)tp, 3}, dq[1] = {(hsize_t)tp};
-        ok = ok && write_dataset(root, "x", x.data(), 2, dx, "", kFortranLen, "Species Raw", kFortranLen);
-        ok = ok && write_dataset(root, "p", p.data(), 2, dp, "", kFortranLen, "Species Raw", kFortranLen);
-        ok = ok && write_dataset(root, "q", q.data(), 1, dq, "", kFortranLen, "Species Raw", kFortranLen);
+        ok = ok && write_dataset(root, "x", x.data(), 2, dx, "", kFortranLen, label, kFortranLen);
+        ok = ok && write_dataset(root, "p", p.data(), 2, dp, "", kFortranLen, label, kFortranLen);
+        ok = ok && write_dataset(root, "q", q.data(), 1, dq, "", kFortranLen, label, kFortranLen);
     }
     if (root >= 0) H5Gclose(root);
     ok = H5Fclose(file) >= 0 && ok;

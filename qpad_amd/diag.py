@@ -45,6 +45,10 @@ def load():
             getattr(d, name).argtypes = [C.c_void_p, C.c_int32, C.c_char_p, C.c_int32, C.POINTER(Box)]
         for name in ("qpad_diag_species_raw", "qpad_diag_beam_raw"):
             getattr(d, name).argtypes = [C.c_void_p, C.c_int32, C.c_int32, C.c_char_p, C.c_int32, C.POINTER(Box)]
+        for name in ("qpad_diag_neutral_charge", "qpad_diag_neutral_ion"):
+            getattr(d, name).argtypes = [C.c_void_p, C.c_int32, C.c_int32, C.c_char_p, C.c_int32, C.POINTER(Box)]
+        d.qpad_diag_neutral_raw.argtypes = [C.c_void_p, C.c_int32, C.c_int32, C.c_int32, C.c_char_p, C.c_int32,
+                                            C.POINTER(Box)]
         d.qpad_diag_read_beam_file.argtypes = [C.c_char_p, C.POINTER(C.c_int64)] + [L._dp] * 7
         _dlib = d
     return _dlib
@@ -110,10 +114,14 @@ class Diagnostics:
                 raise L.QpadError(f"field diagnostic '{name}' is not on the GPU path")
         self.species = [(k, e) for k, sp in enumerate(deck.get("species", [])[:len(sim.spe)]) for e in _entries(sp)]
         self.beams = [(k, e) for k, bm in enumerate(deck.get("beam", [])[:len(sim.beam)]) for e in _entries(bm)]
+        self.neutrals = [(k, e) for k, nt in enumerate(deck.get("neutrals", [])[:len(sim.neut)]) for e in _entries(nt)]
         for _, (name, _, _) in self.species + self.beams:
             if name not in ("charge_cyl_m", "raw"):
                 raise L.QpadError(f"particle diagnostic '{name}' is not on the GPU path")
-        if any(name == "charge_cyl_m" for _, (name, _, _) in self.species):
+        for _, (name, _, _) in self.neutrals:
+            if name not in ("charge_cyl_m", "ion_cyl_m", "raw"):
+                raise L.QpadError(f"neutral diagnostic '{name}' is not on the GPU path")
+        if any(name == "charge_cyl_m" for _, (name, _, _) in self.species + self.neutrals):
             L.check(sim.lib.qpad_species_charge_history(sim.ctx, 1))
         load()
 
@@ -131,6 +139,16 @@ class Diagnostics:
                 check(d.qpad_diag_species_charge(ctx, k, dirb, tstep, b))
             else:
                 check(d.qpad_diag_species_raw(ctx, k, ps, dirb, tstep, b))
+        for k, (name, nd, ps) in self.neutrals:
+            if tstep % nd:
+                continue
+            sp = self.sim.neut[k].id
+            if name == "charge_cyl_m":
+                check(d.qpad_diag_neutral_charge(ctx, sp, k, dirb, tstep, b))
+            elif name == "ion_cyl_m":
+                check(d.qpad_diag_neutral_ion(ctx, sp, k, dirb, tstep, b))
+            else:
+                check(d.qpad_diag_neutral_raw(ctx, sp, k, ps, dirb, tstep, b))
         for k, (name, nd, ps) in self.beams:
             if tstep % nd:
                 continue

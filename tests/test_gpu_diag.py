@@ -104,3 +104,35 @@ def test_run_diagnostics_match_oracle(tmp_path):
     # one folder per mode array of every grid diagnostic
     assert sorted(os.path.basename(p) for p in glob.glob(os.path.join(tmp_path, "Fields", "Psi", "*"))) == \
         ["Im1", "Im2", "Re0", "Re1", "Re2"]
+
+
+def test_neutral_diagnostics_match_oracle(tmp_path):
+    """neutrals(1) charge_cyl_m, ion_cyl_m and raw (diagnostics_class.f03:
+    324-390, 909-928) of an ionising gas beside a plasma, after one step"""
+    d = configs.ionization(element=7, ion_max=5, plasma=True, n0=1e18, n_peak=8.0, max_mode=2, nr=32, nxi=48)
+    d["neutrals"][0]["diag"] = [{"name": ["charge_cyl_m", "ion_cyl_m"], "ndump": 1},
+                                {"name": ["raw"], "ndump": 1, "psample": 2}]
+    sim = Simulation(d)
+    dg = diag.Diagnostics(sim, str(tmp_path))
+    sim.run(1, diag=dg)
+    nxi, nm = sim.grid.nxi, sim.nm
+    isp = sim.neut[0].id
+    sim.close()
+    o = twin.oracle_from_deck(d)
+    o.step_begin()
+    for j in range(1, nxi + 1):
+        o.slice(j)
+    h = o.neutral_rho_hist(isp)[1:nxi + 1, 0]
+    assert np.max(np.abs(h)) > 0, "the gas must ionise"
+    check_mode_files(os.path.join(tmp_path, "Neutral1", "Ion_charge"), "ion_charge", modes(h, nm), 1)
+    hq = o.species_q_hist(isp)[1:nxi + 1, 0]
+    check_mode_files(os.path.join(tmp_path, "Neutral1", "Charge"), "charge", modes(hq, nm), 1)
+    with h5read.File(os.path.join(tmp_path, "Neutral1", "Raw", "raw_00000001.h5")) as f:
+        x, q = f.dataset("x"), f.dataset("q")
+        assert f.attr("/", "tp") == len(q)
+    s = o.species(isp)
+    assert len(q) == len(s["q"]) // 2 > 0
+    # step 0: nothing ionised yet
+    with h5read.File(os.path.join(tmp_path, "Neutral1", "Ion_charge", "Re0", "ion_charge_00000000.h5")) as f:
+        assert np.all(f.dataset("ion_charge") == 0.0)
+    o.close()
