@@ -468,7 +468,12 @@ __global__ void k_beam_permute(BPermArgs a)
     int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= a.n) return;
     uint32_t src = a.idx[p];
-    for (int f = 0; f < 7; f++) a.d[f][p] = a.s[f][src];
+    // the seven loads in flight together, then the stores
+    double v[7];
+#pragma unroll
+    for (int f = 0; f < 7; f++) v[f] = a.s[f][src];
+#pragma unroll
+    for (int f = 0; f < 7; f++) a.d[f][p] = v[f];
 }
 
 int launch_beam_sort(qpad_ctx *c, int ib)

@@ -618,41 +618,60 @@ __global__ __launch_bounds__(kNodeThreads) void k_epilogue(EpiArgs a)
         }
     }
     if (i > nr + 1) return;
+    // every load first, then every store: the arrays are distinct, and
+    // interleaved copies would wait for each load in turn (the compiler must
+    // assume a store may alias the next load)
+    double ve[3], vb[3], vcu[3], vbs[3], vbb[3];
+#pragma unroll
     for (int c = 0; c < 3; c++) {
-        FX(a.h_e, c, k, i) = FX(a.e, c, k, i);
-        FX(a.h_b, c, k, i) = FX(a.b, c, k, i);
-        FX(a.h_cu, c, k, i) = FX(a.cu, c, k, i);
-        FX(a.h_b_spe, c, k, i) = FX(a.b_spe, c, k, i);
-        FX(a.b_beam_f1, c, k, i) = FX(a.b_beam_j, c, k, i);
+        ve[c] = FX(a.e, c, k, i);
+        vb[c] = FX(a.b, c, k, i);
+        vcu[c] = FX(a.cu, c, k, i);
+        vbs[c] = FX(a.b_spe, c, k, i);
+        vbb[c] = FX(a.b_beam_j, c, k, i);
     }
     // E_perp of the plasma fields alone (e_spe, for the next slice's pipeline
     // and diagnostics)
+    const bool inner = i >= 1 && i <= nr;
     double ev[2];
-    if (i >= 1 && i <= nr) {
-        et_node(g, FX(a.b_spe, 0, k, i), FX(a.b_spe, 1, k, i), FieldArr{a.psi, nm, ld}, k, i, ev);
-        FX(a.e_spe, 0, k, i) = ev[0];
-        FX(a.e_spe, 1, k, i) = ev[1];
+    if (inner) {
+        et_node(g, vbs[0], vbs[1], FieldArr{a.psi, nm, ld}, k, i, ev);
     } else {
         ev[0] = FX(a.e_spe, 0, k, i);
         ev[1] = FX(a.e_spe, 1, k, i);
     }
+    const double es2 = FX(a.e_spe, 2, k, i);
+    const size_t o = (size_t)k * ld + i;
+    const double qs = a.q_spe[o], ps = a.psi[o], qb = a.q_beam_j[o];
+    const double dc0 = FX(a.dcu, 0, k, i), dc1 = FX(a.dcu, 1, k, i);
+#pragma unroll
+    for (int c = 0; c < 3; c++) {
+        FX(a.h_e, c, k, i) = ve[c];
+        FX(a.h_b, c, k, i) = vb[c];
+        FX(a.h_cu, c, k, i) = vcu[c];
+        FX(a.h_b_spe, c, k, i) = vbs[c];
+        FX(a.b_beam_f1, c, k, i) = vbb[c];
+    }
+    if (inner) {
+        FX(a.e_spe, 0, k, i) = ev[0];
+        FX(a.e_spe, 1, k, i) = ev[1];
+    }
     FX(a.h_e_spe, 0, k, i) = ev[0];
     FX(a.h_e_spe, 1, k, i) = ev[1];
-    FX(a.h_e_spe, 2, k, i) = FX(a.e_spe, 2, k, i);
+    FX(a.h_e_spe, 2, k, i) = es2;
     // q_spe = (rho - Jz) + Jz (species2d%cbq)
-    const size_t o = (size_t)k * ld + i;
-    const double q = a.q_spe[o] + FX(a.cu, 2, k, i);
+    const double q = qs + vcu[2];
     a.q_spe[o] = q;
     a.h_q_spe[o] = q;
-    a.h_psi[o] = a.psi[o];
-    a.q_beam_f1[o] = a.q_beam_j[o];
+    a.h_psi[o] = ps;
+    a.q_beam_f1[o] = qb;
     // J_perp predictor for the next slice, after the history copy of cu
     // (simulation_class.f03:425-426)
-    for (int c = 0; c < 2; c++) {
-        const double d = a.dxi * FX(a.dcu, c, k, i);
-        FX(a.dcu, c, k, i) = d;
-        FX(a.cu, c, k, i) = FX(a.cu, c, k, i) + d;
-    }
+    const double d0 = a.dxi * dc0, d1 = a.dxi * dc1;
+    FX(a.dcu, 0, k, i) = d0;
+    FX(a.dcu, 1, k, i) = d1;
+    FX(a.cu, 0, k, i) = vcu[0] + d0;
+    FX(a.cu, 1, k, i) = vcu[1] + d1;
 }
 
 int launch_epilogue(qpad_ctx *c, int j)
